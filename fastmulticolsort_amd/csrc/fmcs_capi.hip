@@ -1,0 +1,825 @@
+// fmcs_capi.hip -- context, workspace and the C-ABI entry points (fmcs.h).
+//
+// Host orchestration of the kernels in fmcs_kernels.hip.  All launches go to
+// the context's stream; temporary device memory comes from a per-context
+// workspace that grows on demand and is reused across calls.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fmcs.h"
+#include "fmcs_kernels.h"
+
+using namespace fmcs;
+
+namespace {
+
+enum Slot {
+  WS_KA, WS_KB, WS_VB, WS_T0K, WS_T0V, WS_T1K, WS_T1V, WS_STATUS, WS_HIST, WS_DBASE, WS_SEGOFF, WS_SEGOFF2,
+  WS_ST, WS_SZ, WS_BST, WS_BSZ, WS_FLAG, WS_POS, WS_LLO, WS_LHI, WS_LTILES, WS_TBASE, WS_TLO, WS_THI, WS_TS0,
+  WS_TS1, WS_COMP, WS_COMP2, WS_GROUPTMP, WS_MISC, WS_HASH, WS_HASH2, WS_HASH3, WS_COUNT
+};
+
+struct Buf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+constexpr u32 PIN_WORDS = 4096;
+
+struct fmcs_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  Buf ws[WS_COUNT];
+  u32* small_d = nullptr;   // 64 words of device scratch (counters, totals, consts)
+  u32* host_pin = nullptr;  // PIN_WORDS words pinned for readbacks
+  fmcs_tuning tune = {1024, 1024, 8};
+  hipEvent_t ev[8] = {};
+  double breakdown[4] = {0, 0, 0, 0};
+};
+
+#define HIPCHK(x)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) return fail(ctx, FMCS_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define TRY(x)              \
+  do {                      \
+    int rc_ = (x);          \
+    if (rc_ != FMCS_OK) return rc_; \
+  } while (0)
+
+namespace {
+
+int fail(fmcs_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int getbuf(fmcs_ctx* ctx, Slot s, size_t bytes, void** out) {
+  bytes = std::max<size_t>(bytes, 256);
+  Buf& b = ctx->ws[s];
+  if (b.bytes < bytes) {
+    if (b.p) {
+      hipError_t e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess) return fail(ctx, FMCS_E_HIP, hipGetErrorString(e));
+      (void)hipFree(b.p);
+      b.p = nullptr;
+      b.bytes = 0;
+    }
+    size_t want = bytes + bytes / 8;
+    if (hipMalloc(&b.p, want) != hipSuccess) {
+      (void)hipGetLastError();
+      b.p = nullptr;
+      return fail(ctx, FMCS_E_NOMEM, "workspace allocation of " + std::to_string(want) + " bytes failed");
+    }
+    b.bytes = want;
+  }
+  *out = b.p;
+  return FMCS_OK;
+}
+
+template <class T>
+int getbuf_t(fmcs_ctx* ctx, Slot s, size_t count, T** out) {
+  void* p;
+  TRY(getbuf(ctx, s, count * sizeof(T), &p));
+  *out = (T*)p;
+  return FMCS_OK;
+}
+
+int launch_check(fmcs_ctx* ctx, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(ctx, FMCS_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return FMCS_OK;
+}
+
+int fill(fmcs_ctx* ctx, u32* p, u64 n, u32 v) {
+  if (n == 0) return FMCS_OK;
+  u32 blocks = (u32)std::min<u64>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, ctx->stream, p, n, v);
+  return launch_check(ctx, "k_fill_u32");
+}
+
+int readback(fmcs_ctx* ctx, const u32* d, u32 count, u32* h) {
+  if (count > PIN_WORDS) return fail(ctx, FMCS_E_ARG, "readback too large");
+  hipError_t e = hipMemcpyAsync(ctx->host_pin, d, count * sizeof(u32), hipMemcpyDeviceToHost, ctx->stream);
+  if (e != hipSuccess) return fail(ctx, FMCS_E_HIP, hipGetErrorString(e));
+  e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return fail(ctx, FMCS_E_HIP, hipGetErrorString(e));
+  std::memcpy(h, ctx->host_pin, count * sizeof(u32));
+  return FMCS_OK;
+}
+
+int upload(fmcs_ctx* ctx, u32* d, const u32* h, u32 count) {
+  // small host->device constant upload through the pinned buffer
+  if (count > 64) return fail(ctx, FMCS_E_ARG, "upload too large");
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return fail(ctx, FMCS_E_HIP, hipGetErrorString(e));
+  std::memcpy(ctx->host_pin, h, count * sizeof(u32));
+  e = hipMemcpyAsync(d, ctx->host_pin, count * sizeof(u32), hipMemcpyHostToDevice, ctx->stream);
+  if (e != hipSuccess) return fail(ctx, FMCS_E_HIP, hipGetErrorString(e));
+  e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return fail(ctx, FMCS_E_HIP, hipGetErrorString(e));
+  return FMCS_OK;
+}
+
+bool bank_ok(u32 b) { return b == 2 || b == 4 || b == 8; }
+
+// exclusive scan of n u32 values -> out; *total_h (host) if non-null
+int scan_excl(fmcs_ctx* ctx, const u32* in, u32* out, u64 n, u32* total_d) {
+  if (n == 0) return FMCS_OK;
+  u64 ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  u32* status;
+  TRY(getbuf_t(ctx, WS_STATUS, ntiles, &status));
+  TRY(fill(ctx, status, ntiles, 0));
+  TRY(fill(ctx, ctx->small_d + 8, 1, 0));
+  hipLaunchKernelGGL((k_scan_excl<SCAN_ITEMS>), dim3((u32)ntiles), dim3(256), 0, ctx->stream, in, out, n, status,
+                     ctx->small_d + 8, total_d);
+  return launch_check(ctx, "k_scan_excl");
+}
+
+// ------------------------------------------------------------------------
+// LSD radix sort of (kin, vin) -> (kout, vout) over the low keybits bits.
+// Ping-pongs through two workspace pairs; never writes a buffer a pass reads.
+// ------------------------------------------------------------------------
+template <class K>
+int radix_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, u64 n, u32 keybits) {
+  if (n == 0) return FMCS_OK;
+  if (n >= (1ull << 30)) return fail(ctx, FMCS_E_UNSUPPORTED, "radix sort supports n < 2^30");
+  keybits = std::min<u32>(keybits, 8 * sizeof(K));
+  u32 npasses = (keybits + RADIX_BITS - 1) / RADIX_BITS;
+  if (n == 1 || npasses == 0) {
+    if ((const void*)kout != (const void*)kin) HIPCHK(hipMemcpyAsync(kout, kin, n * sizeof(K), hipMemcpyDeviceToDevice, ctx->stream));
+    if (vout != vin) HIPCHK(hipMemcpyAsync(vout, vin, n * sizeof(u32), hipMemcpyDeviceToDevice, ctx->stream));
+    return FMCS_OK;
+  }
+  u32 last_bits = keybits - (npasses - 1) * RADIX_BITS;
+  u32 *hist, *dbase, *status;
+  TRY(getbuf_t(ctx, WS_HIST, (u64)npasses * RADIX, &hist));
+  TRY(getbuf_t(ctx, WS_DBASE, (u64)npasses * RADIX, &dbase));
+  u64 ntiles = (n + ONESWEEP_TILE - 1) / ONESWEEP_TILE;
+  TRY(getbuf_t(ctx, WS_STATUS, (u64)npasses * ntiles * RADIX, &status));
+  TRY(fill(ctx, hist, (u64)npasses * RADIX, 0));
+  {
+    u32 blocks = (u32)std::min<u64>((n + 255) / 256, 2048);
+    // the last pass's digit is masked to last_bits inside k_hist_all via a
+    // key mask applied here: keys are assumed < 2^keybits, and the same mask
+    // is used by the passes, so histogram and scatter always agree.
+    hipLaunchKernelGGL((k_hist_all<K>), dim3(blocks), dim3(256), 0, ctx->stream, kin, n, npasses, (u32)RADIX_BITS, hist);
+    TRY(launch_check(ctx, "k_hist_all"));
+  }
+  // per-pass trivial detection (one digit holds every row) and digit bases
+  std::vector<u32> h((size_t)npasses * RADIX);
+  TRY(readback(ctx, hist, npasses * RADIX, h.data()));
+  hipLaunchKernelGGL(k_digit_base, dim3(npasses), dim3(256), 0, ctx->stream, hist, (const u32*)nullptr, dbase,
+                     (u32*)nullptr, (u32*)nullptr);
+  TRY(launch_check(ctx, "k_digit_base"));
+  std::vector<u32> passes;
+  for (u32 p = 0; p < npasses; ++p) {
+    u32 bits = (p == npasses - 1) ? last_bits : RADIX_BITS;
+    u64 tot = 0, mx = 0;
+    for (u32 d = 0; d < RADIX; ++d) {
+      tot += h[p * RADIX + d];
+      mx = std::max<u64>(mx, h[p * RADIX + d]);
+    }
+    // digits above `bits` in the last pass would break the scatter: the
+    // histogram used 8-bit digits, so the pass must too unless all keys fit.
+    bool fits = true;
+    if (bits < RADIX_BITS)
+      for (u32 d = (1u << bits); d < RADIX; ++d) fits = fits && (h[p * RADIX + d] == 0);
+    if (!fits) return fail(ctx, FMCS_E_ARG, "keys exceed keybits");
+    if (mx != tot) passes.push_back(p);
+  }
+  K *t0k, *t1k;
+  u32 *t0v, *t1v;
+  TRY(getbuf_t(ctx, WS_T0K, n, &t0k));
+  TRY(getbuf_t(ctx, WS_T0V, n, &t0v));
+  TRY(getbuf_t(ctx, WS_T1K, n, &t1k));
+  TRY(getbuf_t(ctx, WS_T1V, n, &t1v));
+  // constants: seg_lo=0, seg_hi=n, tile_base={0, ntiles}
+  u32 consts[4] = {0u, (u32)n, 0u, (u32)ntiles};
+  TRY(upload(ctx, ctx->small_d + 16, consts, 4));
+  TRY(fill(ctx, status, (u64)npasses * ntiles * RADIX, 0));
+  TRY(fill(ctx, ctx->small_d + 24, npasses, 0));
+  const K* sk = kin;
+  const u32* sv = vin;
+  for (size_t i = 0; i < passes.size(); ++i) {
+    u32 p = passes[i];
+    bool last = (i + 1 == passes.size());
+    K* dk;
+    u32* dv;
+    if (last && (const void*)kout != (const void*)sk && vout != sv) {
+      dk = kout;
+      dv = vout;
+    } else {
+      dk = (i & 1) ? t1k : t0k;
+      dv = (i & 1) ? t1v : t0v;
+    }
+    hipLaunchKernelGGL((k_onesweep<K, ONESWEEP_ITEMS>), dim3((u32)ntiles), dim3(256), 0, ctx->stream, sk, sv, dk, dv,
+                       ctx->small_d + 16, ctx->small_d + 17, ctx->small_d + 18, 1u, dbase + (u64)p * RADIX,
+                       status + (u64)p * ntiles * RADIX, ctx->small_d + 24 + p, p * RADIX_BITS, (u32)RADIX_BITS);
+    TRY(launch_check(ctx, "k_onesweep"));
+    sk = dk;
+    sv = dv;
+  }
+  if ((const void*)sk != (const void*)kout) HIPCHK(hipMemcpyAsync(kout, sk, n * sizeof(K), hipMemcpyDeviceToDevice, ctx->stream));
+  if (sv != vout) HIPCHK(hipMemcpyAsync(vout, sv, n * sizeof(u32), hipMemcpyDeviceToDevice, ctx->stream));
+  return FMCS_OK;
+}
+
+// boundary: dense ids + optional segment offsets; returns count on host
+template <class K>
+int boundary(fmcs_ctx* ctx, const K* v, const u32* in_group, u64 n, u32* out_group, u32* seg_off, u32* ng) {
+  if (n == 0) {
+    *ng = 0;
+    if (seg_off) TRY(fill(ctx, seg_off, 1, 0));
+    return FMCS_OK;
+  }
+  if (n >= (1ull << 30)) return fail(ctx, FMCS_E_UNSUPPORTED, "boundary supports n < 2^30");
+  u64 ntiles = (n + BOUNDARY_TILE - 1) / BOUNDARY_TILE;
+  u32* status;
+  TRY(getbuf_t(ctx, WS_STATUS, ntiles, &status));
+  TRY(fill(ctx, status, ntiles, 0));
+  TRY(fill(ctx, ctx->small_d + 40, 2, 0));
+  hipLaunchKernelGGL((k_boundary<K, BOUNDARY_ITEMS>), dim3((u32)ntiles), dim3(256), 0, ctx->stream, v, in_group, n,
+                     out_group, seg_off, status, ctx->small_d + 40, ctx->small_d + 41);
+  TRY(launch_check(ctx, "k_boundary"));
+  return readback(ctx, ctx->small_d + 41, 1, ng);
+}
+
+template <class K>
+int tile_sort_launch(fmcs_ctx* ctx, TileSortArgs<K> a, u32 grid) {
+  if (grid == 0) return FMCS_OK;
+  hipLaunchKernelGGL((k_tile_sort<K, TILESORT_E>), dim3(grid), dim3(256), 0, ctx->stream, a);
+  return launch_check(ctx, "k_tile_sort");
+}
+
+constexpr u32 TS_SEGBITS = 12;  // TILESORT_T = 2048 < 2^11, pad index 2048 needs 12 bits
+
+// ------------------------------------------------------------------------
+// Batched, size-binned segmented sort of (keys, vals) within segments.
+// Level 0 data in (keys, vals); results in place.
+// ------------------------------------------------------------------------
+template <class K>
+int seg_sort_batched(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* seg_off, u32 nseg) {
+  if (n <= 1 || nseg == 0) return FMCS_OK;
+  const u32 T = TILESORT_T;
+  const u32 small_max = std::min<u32>(ctx->tune.small_max, T / 2);
+  u32 W = std::max<u32>(1, std::min<u32>(ctx->tune.window, small_max + 1));
+  if (W + small_max > T) W = T - small_max;
+  const u32 msd_bits = std::max<u32>(1, std::min<u32>(ctx->tune.msd_bits, RADIX_BITS));
+  keybits = std::min<u32>(keybits, 8 * sizeof(K));
+  K* kb_;
+  u32* vb_;
+  TRY(getbuf_t(ctx, WS_KB, n, &kb_));
+  TRY(getbuf_t(ctx, WS_VB, n, &vb_));
+  // two segment-list slots: the current level's list and the next one
+  const Slot lst[2] = {WS_ST, WS_BST}, lsz[2] = {WS_SZ, WS_BSZ};
+  int cur = 0;
+  u32 *cst, *csz;
+  TRY(getbuf_t(ctx, lst[cur], (u64)nseg + 1, &cst));
+  TRY(getbuf_t(ctx, lsz[cur], (u64)nseg + 1, &csz));
+  hipLaunchKernelGGL(k_offsets_to_list, dim3((nseg + 255) / 256), dim3(256), 0, ctx->stream, seg_off, nseg, cst, csz);
+  TRY(launch_check(ctx, "k_offsets_to_list"));
+  u32 cur_n = nseg;
+  u32 kb = keybits;
+  u32 level = 0;
+  for (;;) {
+    u32 sm = small_max;
+    u32 tkb = kb;
+    if (kb + TS_SEGBITS > 64) {
+      // too many key bits to pack a segment index: only single rows are
+      // packed (their key bits need no comparing), the rest go to MSD.
+      sm = 1;
+      tkb = 0;
+    }
+    const u32 Wl = std::min<u32>(W, sm + 1);
+    const u64 nw = (n + Wl - 1) / Wl;
+    u32 *tlo, *thi, *ts0, *ts1;
+    TRY(getbuf_t(ctx, WS_TLO, nw, &tlo));
+    TRY(getbuf_t(ctx, WS_THI, nw, &thi));
+    TRY(getbuf_t(ctx, WS_TS0, nw, &ts0));
+    TRY(getbuf_t(ctx, WS_TS1, nw, &ts1));
+    TRY(fill(ctx, tlo, nw, 0));
+    TRY(fill(ctx, thi, nw, 0));
+    hipLaunchKernelGGL(k_windows, dim3((cur_n + 255) / 256), dim3(256), 0, ctx->stream, cst, csz, cur_n, Wl, sm, tlo,
+                       thi, ts0, ts1);
+    TRY(launch_check(ctx, "k_windows"));
+    TileSortArgs<K> a;
+    std::memset(&a, 0, sizeof(a));
+    a.kin[0] = keys; a.vin[0] = vals; a.kin[1] = kb_; a.vin[1] = vb_;
+    a.kout = keys; a.vout = vals;
+    a.tile_lo = tlo; a.tile_hi = thi; a.tile_s0 = ts0; a.tile_s1 = ts1; a.seg_start = cst;
+    a.src_sel = level & 1; a.kb = tkb; a.pad_seg = T; a.direct = 0;
+    TRY(tile_sort_launch<K>(ctx, a, (u32)nw));
+    // large segments of this level
+    u32 *flag, *pos;
+    TRY(getbuf_t(ctx, WS_FLAG, (u64)cur_n + 1, &flag));
+    TRY(getbuf_t(ctx, WS_POS, (u64)cur_n + 1, &pos));
+    hipLaunchKernelGGL(k_large_flags, dim3((cur_n + 255) / 256), dim3(256), 0, ctx->stream, csz, cur_n, sm, flag);
+    TRY(launch_check(ctx, "k_large_flags"));
+    TRY(scan_excl(ctx, flag, pos, cur_n, ctx->small_d + 48));
+    u32 nlarge;
+    TRY(readback(ctx, ctx->small_d + 48, 1, &nlarge));
+    if (nlarge == 0) break;
+    u32 *llo, *lhi, *ltiles, *tbase;
+    TRY(getbuf_t(ctx, WS_LLO, nlarge, &llo));
+    TRY(getbuf_t(ctx, WS_LHI, nlarge, &lhi));
+    TRY(getbuf_t(ctx, WS_LTILES, (u64)nlarge + 1, &ltiles));
+    TRY(getbuf_t(ctx, WS_TBASE, (u64)nlarge + 1, &tbase));
+    hipLaunchKernelGGL(k_large_compact, dim3((cur_n + 255) / 256), dim3(256), 0, ctx->stream, cst, csz, cur_n, sm, pos,
+                       (u32)ONESWEEP_TILE, llo, lhi, ltiles);
+    TRY(launch_check(ctx, "k_large_compact"));
+    TRY(scan_excl(ctx, ltiles, tbase, nlarge, ctx->small_d + 49));
+    u32 ntiles;
+    TRY(readback(ctx, ctx->small_d + 49, 1, &ntiles));
+    const K* skey = (level & 1) ? kb_ : keys;
+    const u32* sval = (level & 1) ? vb_ : vals;
+    K* dkey = (level & 1) ? keys : kb_;
+    u32* dval = (level & 1) ? vals : vb_;
+    if (kb == 0) {
+      // every key bit consumed: rows are already in final order
+      if (level & 1) {
+        hipLaunchKernelGGL((k_copy_segs<K>), dim3(ntiles), dim3(256), 0, ctx->stream, skey, sval, keys, vals, llo, lhi,
+                           tbase, nlarge, (u32)ONESWEEP_TILE);
+        TRY(launch_check(ctx, "k_copy_segs"));
+      }
+      break;
+    }
+    const u32 D = std::min<u32>(msd_bits, kb);
+    const u32 shift = kb - D;
+    u32 *hist, *dbase, *status;
+    TRY(getbuf_t(ctx, WS_HIST, (u64)nlarge * RADIX, &hist));
+    TRY(getbuf_t(ctx, WS_DBASE, (u64)nlarge * RADIX, &dbase));
+    TRY(getbuf_t(ctx, WS_STATUS, (u64)ntiles * RADIX, &status));
+    TRY(fill(ctx, hist, (u64)nlarge * RADIX, 0));
+    hipLaunchKernelGGL((k_seg_hist<K, ONESWEEP_ITEMS>), dim3(ntiles), dim3(256), 0, ctx->stream, skey, llo, lhi, tbase,
+                       nlarge, shift, D, hist);
+    TRY(launch_check(ctx, "k_seg_hist"));
+    const int nxt = cur ^ 1;
+    u32 *nst, *nsz;
+    TRY(getbuf_t(ctx, lst[nxt], (u64)nlarge * RADIX + 1, &nst));
+    TRY(getbuf_t(ctx, lsz[nxt], (u64)nlarge * RADIX + 1, &nsz));
+    hipLaunchKernelGGL(k_digit_base, dim3(nlarge), dim3(256), 0, ctx->stream, hist, llo, dbase, nst, nsz);
+    TRY(launch_check(ctx, "k_digit_base"));
+    TRY(fill(ctx, status, (u64)ntiles * RADIX, 0));
+    TRY(fill(ctx, ctx->small_d + 50, 1, 0));
+    hipLaunchKernelGGL((k_onesweep<K, ONESWEEP_ITEMS>), dim3(ntiles), dim3(256), 0, ctx->stream, skey, sval, dkey, dval,
+                       llo, lhi, tbase, nlarge, dbase, status, ctx->small_d + 50, shift, D);
+    TRY(launch_check(ctx, "k_onesweep(msd)"));
+    cur = nxt;
+    cst = nst;
+    csz = nsz;
+    cur_n = nlarge * RADIX;
+    kb -= D;
+    ++level;
+  }
+  return FMCS_OK;
+}
+
+// plain per-round GPU form: one sort dispatch per group, in group order
+template <class K>
+int seg_sort_plain(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* seg_off, u32 nseg) {
+  std::vector<u32> off((size_t)nseg + 1);
+  HIPCHK(hipMemcpyAsync(off.data(), seg_off, ((size_t)nseg + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (u32 s = 0; s < nseg; ++s) {
+    u32 lo = off[s], hi = off[s + 1];
+    if (hi < lo || hi > n) return fail(ctx, FMCS_E_ARG, "segment offsets out of range");
+    u32 sz = hi - lo;
+    if (sz <= 1) continue;
+    if (sz <= TILESORT_T && keybits + TS_SEGBITS <= 64) {
+      TileSortArgs<K> a;
+      std::memset(&a, 0, sizeof(a));
+      a.kin[0] = keys; a.vin[0] = vals; a.kout = keys; a.vout = vals;
+      a.kb = keybits; a.pad_seg = TILESORT_T; a.direct = 1; a.d_lo = lo; a.d_hi = hi;
+      TRY(tile_sort_launch<K>(ctx, a, 1));
+    } else {
+      TRY(radix_sort<K>(ctx, keys + lo, vals + lo, keys + lo, vals + lo, sz, keybits));
+    }
+  }
+  return FMCS_OK;
+}
+
+// single unbinned segmented sort: LSD radix over (group id, key) composites
+template <class K>
+int seg_sort_unbinned(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* group, u32 ngroups) {
+  u32 gbits = 0;
+  while (gbits < 32 && (1ull << gbits) < ngroups) ++gbits;
+  if (gbits + keybits > 64) return fail(ctx, FMCS_E_UNSUPPORTED, "composite key exceeds 64 bits");
+  u64 *c, *c2;
+  TRY(getbuf_t(ctx, WS_COMP, n, &c));
+  TRY(getbuf_t(ctx, WS_COMP2, n, &c2));
+  u32 blocks = (u32)std::min<u64>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL((k_composite<K>), dim3(blocks), dim3(256), 0, ctx->stream, keys, group, n, keybits, c);
+  TRY(launch_check(ctx, "k_composite"));
+  TRY(radix_sort<u64>(ctx, c, vals, c2, vals, n, gbits + keybits));
+  u64 km = keybits >= 64 ? ~0ull : ((1ull << keybits) - 1);
+  hipLaunchKernelGGL((k_strip_composite<K>), dim3(blocks), dim3(256), 0, ctx->stream, c2, n, km, keys);
+  return launch_check(ctx, "k_strip_composite");
+}
+
+template <class K>
+int seg_sort(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* seg_off, u32 nseg, int mode,
+             const u32* group) {
+  switch (mode) {
+    case FMCS_MODE_BATCHED: return seg_sort_batched<K>(ctx, keys, vals, n, keybits, seg_off, nseg);
+    case FMCS_MODE_PLAIN: return seg_sort_plain<K>(ctx, keys, vals, n, keybits, seg_off, nseg);
+    case FMCS_MODE_UNBINNED:
+      if (!group) return fail(ctx, FMCS_E_ARG, "unbinned mode needs group ids");
+      return seg_sort_unbinned<K>(ctx, keys, vals, n, keybits, group, nseg);
+    default: return fail(ctx, FMCS_E_ARG, "unknown mode");
+  }
+}
+
+int group_info_from_offsets(fmcs_ctx* ctx, u32 bank, const u32* seg_off, u32 nseg, fmcs_group_info* info) {
+  std::vector<u32> off((size_t)nseg + 1);
+  HIPCHK(hipMemcpyAsync(off.data(), seg_off, ((size_t)nseg + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  u32 bs = (256u * 1024u) / (4u * std::max<u32>(bank, 4));  // BLOCKSIZE<T>() (src/params.h:141-143)
+  fmcs_group_info g = {0, 0, 0, 0, 0};
+  for (u32 s = 0; s < nseg; ++s) {
+    u32 sz = off[s + 1] - off[s];
+    g.ngroups++;
+    g.sum_group_size += sz;
+    if (sz < bs) g.ngroups_incache++; else g.ngroups_outofcache++;
+    g.ngroups_gtOne += (sz > 1);
+  }
+  *info = g;
+  return FMCS_OK;
+}
+
+template <class K>
+int sort_by_group_t(fmcs_ctx* ctx, u32 keybits, u64 n, const K* column, u32* order, const u32* in_group,
+                    u32* out_group, int last_round, int mode, K* sorted, fmcs_group_info* info, u32* ngroups,
+                    const u32* in_seg_off, u32 in_nseg, u32* out_seg_off, double* t_sort, double* t_gather,
+                    double* t_extract) {
+  K* ka = sorted;
+  if (!ka) TRY(getbuf_t(ctx, WS_KA, n, &ka));
+  hipEvent_t* ev = ctx->ev;
+  HIPCHK(hipEventRecord(ev[0], ctx->stream));
+  if (!in_group) {
+    HIPCHK(hipEventRecord(ev[1], ctx->stream));
+    TRY(radix_sort<K>(ctx, column, order, ka, order, n, keybits));
+  } else {
+    u32 blocks = (u32)((n + 1023) / 1024);
+    if (blocks) {
+      if (((uintptr_t)order & 15) != 0) return fail(ctx, FMCS_E_ARG, "order must be 16-byte aligned");
+      hipLaunchKernelGGL((k_gather<K>), dim3(blocks), dim3(256), 0, ctx->stream, column, order, ka, n);
+      TRY(launch_check(ctx, "k_gather"));
+    }
+    HIPCHK(hipEventRecord(ev[1], ctx->stream));
+    const u32* so = in_seg_off;
+    u32 nseg = in_nseg;
+    if (!so) {
+      u32* tmp;
+      TRY(getbuf_t(ctx, WS_SEGOFF, n + 1, &tmp));
+      TRY(boundary<u32>(ctx, in_group, nullptr, n, nullptr, tmp, &nseg));
+      so = tmp;
+    }
+    if (info) TRY(group_info_from_offsets(ctx, sizeof(K), so, nseg, info));
+    TRY(seg_sort<K>(ctx, ka, order, n, keybits, so, nseg, mode, in_group));
+  }
+  HIPCHK(hipEventRecord(ev[2], ctx->stream));
+  u32 ng = 0;
+  if (!last_round) TRY(boundary<K>(ctx, ka, in_group, n, out_group, out_seg_off, &ng));
+  HIPCHK(hipEventRecord(ev[3], ctx->stream));
+  HIPCHK(hipEventSynchronize(ev[3]));
+  float a = 0, b = 0, c = 0;
+  HIPCHK(hipEventElapsedTime(&a, ev[0], ev[1]));
+  HIPCHK(hipEventElapsedTime(&b, ev[1], ev[2]));
+  HIPCHK(hipEventElapsedTime(&c, ev[2], ev[3]));
+  if (t_gather) *t_gather += a;
+  if (t_sort) *t_sort += b;
+  if (t_extract) *t_extract += c;
+  if (ngroups) *ngroups = ng;
+  return FMCS_OK;
+}
+
+}  // namespace
+
+// =========================================================================
+// C ABI
+// =========================================================================
+extern "C" {
+
+int fmcs_ctx_create(int device, fmcs_ctx** out) {
+  if (!out) return FMCS_E_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return FMCS_E_HIP;
+  if (device < 0 || device >= ndev) return FMCS_E_ARG;
+  if (hipSetDevice(device) != hipSuccess) return FMCS_E_HIP;
+  fmcs_ctx* c = new fmcs_ctx();
+  c->device = device;
+  if (hipMalloc(&c->small_d, 64 * sizeof(u32)) != hipSuccess) { delete c; return FMCS_E_NOMEM; }
+  if (hipHostMalloc((void**)&c->host_pin, PIN_WORDS * sizeof(u32), hipHostMallocDefault) != hipSuccess) {
+    (void)hipFree(c->small_d);
+    delete c;
+    return FMCS_E_NOMEM;
+  }
+  for (auto& e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) return FMCS_E_HIP;
+  *out = c;
+  return FMCS_OK;
+}
+
+int fmcs_ctx_destroy(fmcs_ctx* ctx) {
+  if (!ctx) return FMCS_E_ARG;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& b : ctx->ws)
+    if (b.p) (void)hipFree(b.p);
+  (void)hipFree(ctx->small_d);
+  (void)hipHostFree(ctx->host_pin);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  delete ctx;
+  return FMCS_OK;
+}
+
+int fmcs_ctx_set_stream(fmcs_ctx* ctx, void* stream) {
+  if (!ctx) return FMCS_E_ARG;
+  ctx->stream = (hipStream_t)stream;
+  return FMCS_OK;
+}
+
+int fmcs_ctx_sync(fmcs_ctx* ctx) {
+  if (!ctx) return FMCS_E_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return FMCS_OK;
+}
+
+int fmcs_ctx_set_tuning(fmcs_ctx* ctx, const fmcs_tuning* t) {
+  if (!ctx || !t) return FMCS_E_ARG;
+  if (t->small_max < 1 || t->small_max > (u32)TILESORT_T / 2 || t->window < 1 || t->window > t->small_max + 1 ||
+      t->msd_bits < 1 || t->msd_bits > (u32)RADIX_BITS)
+    return fail(ctx, FMCS_E_ARG, "tuning out of range");
+  ctx->tune = *t;
+  return FMCS_OK;
+}
+
+int fmcs_ctx_get_tuning(fmcs_ctx* ctx, fmcs_tuning* t) {
+  if (!ctx || !t) return FMCS_E_ARG;
+  *t = ctx->tune;
+  return FMCS_OK;
+}
+
+int fmcs_ctx_trim(fmcs_ctx* ctx) {
+  if (!ctx) return FMCS_E_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (auto& b : ctx->ws) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  return FMCS_OK;
+}
+
+const char* fmcs_last_error(fmcs_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+uint32_t fmcs_bank_bytes(uint32_t width, int allow_u8) {
+  if (allow_u8 && width <= 8) return 1;
+  if (width <= 16) return 2;
+  if (width <= 32) return 4;
+  return 8;
+}
+
+int fmcs_construct_entries(uint32_t ncols, const uint32_t* bw, const uint32_t* col_order, uint32_t nsplit,
+                           const uint32_t* split, fmcs_entry* entries, uint32_t max_entries, uint32_t* n_out) {
+  if (!bw || !col_order || !split || !entries || !n_out || ncols == 0 || nsplit == 0) return FMCS_E_ARG;
+  uint64_t so = 0, sn = 0;
+  std::vector<char> seen(ncols, 0);
+  for (u32 i = 0; i < ncols; ++i) {
+    if (col_order[i] >= ncols || seen[col_order[i]]) return FMCS_E_ARG;
+    seen[col_order[i]] = 1;
+    if (bw[col_order[i]] == 0 || bw[col_order[i]] > 64) return FMCS_E_ARG;
+    so += bw[col_order[i]];
+  }
+  for (u32 i = 0; i < nsplit; ++i) {
+    if (split[i] == 0 || split[i] > 64) return FMCS_E_ARG;
+    sn += split[i];
+  }
+  if (so != sn) return FMCS_E_ARG;
+  u32 oi = 0, ni = 0, start = 1, n = 0;
+  u32 ow = bw[col_order[0]], nw = split[0];
+  while (ni < nsplit) {
+    u32 m = std::min(ow, nw);
+    ow -= m;
+    nw -= m;
+    if (n >= max_entries) return FMCS_E_ARG;
+    entries[n++] = {ni, col_order[oi], start, start + m - 1};
+    start += m;
+    if (ow == 0) {
+      ++oi;
+      if (oi < ncols) ow = bw[col_order[oi]];
+      start = 1;
+    }
+    if (nw == 0) {
+      ++ni;
+      if (ni < nsplit) nw = split[ni];
+    }
+  }
+  *n_out = n;
+  return FMCS_OK;
+}
+
+int fmcs_stitch(fmcs_ctx* ctx, uint64_t n, uint32_t nsrc, const void* const* src_d, const uint32_t* src_bank,
+                const uint32_t* src_bw, uint32_t nent, const fmcs_entry* entries, uint32_t ndst, void* const* dst_d,
+                const uint32_t* dst_bank) {
+  if (!ctx) return FMCS_E_ARG;
+  if (nsrc == 0 || nsrc > STITCH_MAX_SRC || ndst == 0 || ndst > STITCH_MAX_DST || nent == 0 || nent > STITCH_MAX_ENT)
+    return fail(ctx, FMCS_E_ARG, "stitch: too many columns or entries");
+  StitchArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.n = n;
+  a.ndst = ndst;
+  for (u32 s = 0; s < nsrc; ++s) {
+    if (!(src_bank[s] == 1 || bank_ok(src_bank[s])) || src_bw[s] == 0 || src_bw[s] > 8 * src_bank[s])
+      return fail(ctx, FMCS_E_ARG, "stitch: bad source bank/width");
+    if (((uintptr_t)src_d[s] & 15) != 0) return fail(ctx, FMCS_E_ARG, "stitch: source not 16-byte aligned");
+    a.src[s] = src_d[s];
+    a.src_bank[s] = src_bank[s];
+  }
+  std::vector<u32> width(ndst, 0);
+  u32 prev = 0;
+  for (u32 e = 0; e < nent; ++e) {
+    const fmcs_entry& x = entries[e];
+    if (x.newcol >= ndst || x.src >= nsrc || x.start < 1 || x.end < x.start || x.end > src_bw[x.src] ||
+        (e && x.newcol < prev))
+      return fail(ctx, FMCS_E_ARG, "stitch: bad entry");
+    prev = x.newcol;
+    a.e_src[e] = (uint8_t)x.src;
+    a.e_width[e] = (uint8_t)(x.end - x.start + 1);
+    a.e_shift[e] = (uint8_t)(src_bw[x.src] - x.end);
+    width[x.newcol] += x.end - x.start + 1;
+  }
+  u32 e = 0;
+  for (u32 c = 0; c < ndst; ++c) {
+    if (!(dst_bank[c] == 1 || bank_ok(dst_bank[c])) || width[c] == 0 || width[c] > 8 * dst_bank[c])
+      return fail(ctx, FMCS_E_ARG, "stitch: destination too narrow");
+    if (((uintptr_t)dst_d[c] & 15) != 0) return fail(ctx, FMCS_E_ARG, "stitch: destination not 16-byte aligned");
+    a.dst[c] = dst_d[c];
+    a.dst_bank[c] = dst_bank[c];
+    a.ent_begin[c] = e;
+    while (e < nent && entries[e].newcol == c) ++e;
+  }
+  a.ent_begin[ndst] = e;
+  if (n == 0) return FMCS_OK;
+  u64 threads = (n + 3) / 4;
+  u32 blocks = (u32)((threads + 255) / 256);
+  hipLaunchKernelGGL(k_stitch, dim3(blocks), dim3(256), 0, ctx->stream, a);
+  return launch_check(ctx, "k_stitch");
+}
+
+int fmcs_gather(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* values_d, const uint32_t* order_d,
+                void* out_d) {
+  if (!ctx) return FMCS_E_ARG;
+  if (!bank_ok(bank)) return fail(ctx, FMCS_E_ARG, "gather: bank must be 2, 4 or 8");
+  if (n == 0) return FMCS_OK;
+  if (((uintptr_t)order_d & 15) != 0) return fail(ctx, FMCS_E_ARG, "gather: order must be 16-byte aligned");
+  u32 blocks = (u32)((n + 1023) / 1024);
+  switch (bank) {
+    case 2: hipLaunchKernelGGL((k_gather<u16>), dim3(blocks), dim3(256), 0, ctx->stream, (const u16*)values_d, order_d, (u16*)out_d, n); break;
+    case 4: hipLaunchKernelGGL((k_gather<u32>), dim3(blocks), dim3(256), 0, ctx->stream, (const u32*)values_d, order_d, (u32*)out_d, n); break;
+    default: hipLaunchKernelGGL((k_gather<u64>), dim3(blocks), dim3(256), 0, ctx->stream, (const u64*)values_d, order_d, (u64*)out_d, n); break;
+  }
+  return launch_check(ctx, "k_gather");
+}
+
+int fmcs_extract_group_info(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* sorted_d, const uint32_t* in_group_d,
+                            uint32_t* out_group_d, uint32_t* seg_offsets_d, uint32_t* ngroups) {
+  if (!ctx || !ngroups) return FMCS_E_ARG;
+  switch (bank) {
+    case 2: return boundary<u16>(ctx, (const u16*)sorted_d, in_group_d, n, out_group_d, seg_offsets_d, ngroups);
+    case 4: return boundary<u32>(ctx, (const u32*)sorted_d, in_group_d, n, out_group_d, seg_offsets_d, ngroups);
+    case 8: return boundary<u64>(ctx, (const u64*)sorted_d, in_group_d, n, out_group_d, seg_offsets_d, ngroups);
+    default: return fail(ctx, FMCS_E_ARG, "bank must be 2, 4 or 8");
+  }
+}
+
+int fmcs_sort_pairs(fmcs_ctx* ctx, uint32_t bank, uint32_t keybits, uint64_t n, void* keys_d, uint32_t* oids_d) {
+  if (!ctx) return FMCS_E_ARG;
+  if (keybits == 0 || keybits > 8 * bank) return fail(ctx, FMCS_E_ARG, "bad keybits");
+  switch (bank) {
+    case 2: return radix_sort<u16>(ctx, (u16*)keys_d, oids_d, (u16*)keys_d, oids_d, n, keybits);
+    case 4: return radix_sort<u32>(ctx, (u32*)keys_d, oids_d, (u32*)keys_d, oids_d, n, keybits);
+    case 8: return radix_sort<u64>(ctx, (u64*)keys_d, oids_d, (u64*)keys_d, oids_d, n, keybits);
+    default: return fail(ctx, FMCS_E_ARG, "bank must be 2, 4 or 8");
+  }
+}
+
+int fmcs_segmented_sort(fmcs_ctx* ctx, uint32_t bank, uint32_t keybits, uint64_t n, void* keys_d, uint32_t* oids_d,
+                        const uint32_t* seg_offsets_d, uint32_t nseg, int mode, fmcs_group_info* info) {
+  if (!ctx) return FMCS_E_ARG;
+  if (keybits == 0 || keybits > 8 * bank) return fail(ctx, FMCS_E_ARG, "bad keybits");
+  if (n >= (1ull << 30)) return fail(ctx, FMCS_E_UNSUPPORTED, "n < 2^30 required");
+  if (info) TRY(group_info_from_offsets(ctx, bank, seg_offsets_d, nseg, info));
+  const u32* grp = nullptr;
+  if (mode == FMCS_MODE_UNBINNED) {
+    // group ids from the offsets: ids of a run of equal ids are its heads
+    u32* g;
+    TRY(getbuf_t(ctx, WS_GROUPTMP, n, &g));
+    std::vector<u32> off((size_t)nseg + 1);
+    HIPCHK(hipMemcpyAsync(off.data(), seg_offsets_d, ((size_t)nseg + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::vector<u32> hg(n);
+    for (u32 s = 0; s < nseg; ++s)
+      for (u32 i = off[s]; i < off[s + 1] && i < n; ++i) hg[i] = s;
+    HIPCHK(hipMemcpyAsync(g, hg.data(), n * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    grp = g;
+  }
+  switch (bank) {
+    case 2: return seg_sort<u16>(ctx, (u16*)keys_d, oids_d, n, keybits, seg_offsets_d, nseg, mode, grp);
+    case 4: return seg_sort<u32>(ctx, (u32*)keys_d, oids_d, n, keybits, seg_offsets_d, nseg, mode, grp);
+    case 8: return seg_sort<u64>(ctx, (u64*)keys_d, oids_d, n, keybits, seg_offsets_d, nseg, mode, grp);
+    default: return fail(ctx, FMCS_E_ARG, "bank must be 2, 4 or 8");
+  }
+}
+
+int fmcs_sort_by_group(fmcs_ctx* ctx, uint32_t bank, uint32_t keybits, uint64_t n, const void* column_d,
+                       uint32_t* order_d, const uint32_t* in_group_d, uint32_t* out_group_d, int last_round, int mode,
+                       void* sorted_d, fmcs_group_info* info, uint32_t* ngroups) {
+  if (!ctx) return FMCS_E_ARG;
+  if (keybits == 0 || keybits > 8 * bank) return fail(ctx, FMCS_E_ARG, "bad keybits");
+  if (!last_round && !out_group_d) return fail(ctx, FMCS_E_ARG, "out_group required unless last round");
+  if (n >= (1ull << 30)) return fail(ctx, FMCS_E_UNSUPPORTED, "n < 2^30 required");
+  switch (bank) {
+    case 2: return sort_by_group_t<u16>(ctx, keybits, n, (const u16*)column_d, order_d, in_group_d, out_group_d, last_round, mode, (u16*)sorted_d, info, ngroups, nullptr, 0, nullptr, nullptr, nullptr, nullptr);
+    case 4: return sort_by_group_t<u32>(ctx, keybits, n, (const u32*)column_d, order_d, in_group_d, out_group_d, last_round, mode, (u32*)sorted_d, info, ngroups, nullptr, 0, nullptr, nullptr, nullptr, nullptr);
+    case 8: return sort_by_group_t<u64>(ctx, keybits, n, (const u64*)column_d, order_d, in_group_d, out_group_d, last_round, mode, (u64*)sorted_d, info, ngroups, nullptr, 0, nullptr, nullptr, nullptr, nullptr);
+    default: return fail(ctx, FMCS_E_ARG, "bank must be 2, 4 or 8");
+  }
+}
+
+int fmcs_multi_round_sort(fmcs_ctx* ctx, uint32_t nrounds, const void* const* cols_d, const uint32_t* banks,
+                          const uint32_t* bitwidths, uint64_t n, uint32_t* order_d, uint32_t* groups_d, int mode,
+                          uint32_t* ngroups) {
+  if (!ctx || !cols_d || !banks || !bitwidths || !order_d || nrounds == 0) return FMCS_E_ARG;
+  if (n >= (1ull << 30)) return fail(ctx, FMCS_E_UNSUPPORTED, "n < 2^30 required");
+  for (u32 r = 0; r < nrounds; ++r)
+    if (!bank_ok(banks[r]) || bitwidths[r] == 0 || bitwidths[r] > 8 * banks[r])
+      return fail(ctx, FMCS_E_ARG, "bad bank/bitwidth");
+  if (n == 0) return FMCS_OK;
+  // order = identity (src/composer.cpp:60-63)
+  hipLaunchKernelGGL(k_iota_u32, dim3((u32)std::min<u64>((n + 255) / 256, 8192)), dim3(256), 0, ctx->stream, order_d, n);
+  TRY(launch_check(ctx, "k_iota_u32"));
+  u32 *ga, *gb, *offa, *offb;
+  TRY(getbuf_t(ctx, WS_HASH, n, &ga));
+  TRY(getbuf_t(ctx, WS_HASH2, n, &gb));
+  TRY(getbuf_t(ctx, WS_SEGOFF, n + 1, &offa));
+  TRY(getbuf_t(ctx, WS_SEGOFF2, n + 1, &offb));
+  const u32* gin = nullptr;
+  const u32* soff = nullptr;
+  u32 nseg = 0, ng = 0;
+  double tg = 0, ts1 = 0, ts = 0, tx = 0;
+  for (u32 r = 0; r < nrounds; ++r) {
+    bool last = (r + 1 == nrounds);
+    bool want_groups = !last || groups_d;
+    u32* gout = last && groups_d ? groups_d : ((gin == ga) ? gb : ga);
+    u32* oout = (soff == offa) ? offb : offa;
+    double* tsort = (r == 0) ? &ts1 : &ts;
+    int rc;
+    switch (banks[r]) {
+      case 2: rc = sort_by_group_t<u16>(ctx, bitwidths[r], n, (const u16*)cols_d[r], order_d, gin, gout, !want_groups, mode, nullptr, nullptr, &ng, soff, nseg, oout, tsort, &tg, &tx); break;
+      case 4: rc = sort_by_group_t<u32>(ctx, bitwidths[r], n, (const u32*)cols_d[r], order_d, gin, gout, !want_groups, mode, nullptr, nullptr, &ng, soff, nseg, oout, tsort, &tg, &tx); break;
+      default: rc = sort_by_group_t<u64>(ctx, bitwidths[r], n, (const u64*)cols_d[r], order_d, gin, gout, !want_groups, mode, nullptr, nullptr, &ng, soff, nseg, oout, tsort, &tg, &tx); break;
+    }
+    if (rc != FMCS_OK) return rc;
+    if (!last) {
+      gin = gout;
+      soff = oout;
+      nseg = ng;
+    }
+  }
+  ctx->breakdown[0] = tg;
+  ctx->breakdown[1] = ts1;
+  ctx->breakdown[2] = ts;
+  ctx->breakdown[3] = tx;
+  if (ngroups) *ngroups = groups_d ? ng : nseg;
+  return FMCS_OK;
+}
+
+int fmcs_last_breakdown(fmcs_ctx* ctx, double* ms4) {
+  if (!ctx || !ms4) return FMCS_E_ARG;
+  for (int i = 0; i < 4; ++i) ms4[i] = ctx->breakdown[i];
+  return FMCS_OK;
+}
+
+int fmcs_hash_round(fmcs_ctx* ctx, uint32_t, uint64_t, const void*, const uint32_t*, const uint32_t*, uint32_t*,
+                    uint32_t*, uint32_t*, int, uint32_t*) {
+  return fail(ctx, FMCS_E_UNSUPPORTED, "hash grouping not built yet");
+}
+
+int fmcs_multi_round_hash(fmcs_ctx* ctx, uint32_t, const void* const*, const uint32_t*, uint64_t, uint32_t*,
+                          uint32_t*, uint32_t*) {
+  return fail(ctx, FMCS_E_UNSUPPORTED, "hash grouping not built yet");
+}
+
+}  // extern "C"

@@ -1,0 +1,105 @@
+// fmcs_common.h -- shared device helpers for the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fmcs {
+
+typedef uint16_t u16;
+typedef uint32_t u32;
+typedef uint64_t u64;
+
+constexpr int WAVE = 64;
+constexpr int RADIX_BITS = 8;
+constexpr int RADIX = 1 << RADIX_BITS;
+
+// Decoupled look-back status word: 2 flag bits + 30 value bits.
+constexpr u32 ST_AGG = 1u << 30;
+constexpr u32 ST_INC = 2u << 30;
+constexpr u32 ST_VAL = (1u << 30) - 1;
+
+__device__ __forceinline__ u32 status_load(const u32* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void status_store(u32* p, u32 v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Walks back from tile `t` (exclusive) over status words status[p*stride]
+// for p = t-1, t-2, ... >= stop, summing aggregates until an inclusive
+// prefix is found.  Every predecessor took its tile id from an atomic
+// counter before this tile did, so it is resident and publishes its
+// aggregate without waiting on anything: the spin always ends.
+__device__ __forceinline__ u32 lookback(const u32* status, u32 t, u32 stop, u32 stride) {
+  u32 excl = 0;
+  u32 p = t;
+  while (p > stop) {
+    --p;
+    u32 s;
+    do {
+      s = status_load(status + (u64)p * stride);
+    } while ((s & ~ST_VAL) == 0);
+    excl += s & ST_VAL;
+    if (s & ST_INC) break;
+  }
+  return excl;
+}
+
+__device__ __forceinline__ u64 lanemask_lt() {
+  u32 lane = __lane_id();
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Inclusive wave scan of u32.
+__device__ __forceinline__ u32 wave_incl_scan(u32 v) {
+  u32 lane = __lane_id();
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    u32 t = __shfl_up(v, o);
+    if (lane >= (u32)o) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan over a block of NT threads (NT/64 waves), one value each.
+// s_tmp needs NT/64 + 1 words.  Returns the exclusive prefix; *total gets
+// the block sum.  Contains __syncthreads.
+template <int NT>
+__device__ __forceinline__ u32 block_excl_scan(u32 v, u32* s_tmp, u32* total) {
+  constexpr int NW = NT / WAVE;
+  u32 lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+  u32 inc = wave_incl_scan(v);
+  if (lane == WAVE - 1) s_tmp[wv] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u32 run = 0;
+    for (int w = 0; w < NW; ++w) {
+      u32 c = s_tmp[w];
+      s_tmp[w] = run;
+      run += c;
+    }
+    s_tmp[NW] = run;
+  }
+  __syncthreads();
+  u32 r = s_tmp[wv] + inc - v;
+  *total = s_tmp[NW];
+  __syncthreads();
+  return r;
+}
+
+template <class K>
+__device__ __forceinline__ u32 digit_of(K k, u32 shift, u32 mask) {
+  return (u32)((u64)k >> shift) & mask;
+}
+
+// Largest i in [0, n) with a[i] <= x (a sorted ascending, a[0] <= x).
+__device__ __forceinline__ u32 upper_bound_m1(const u32* a, u32 n, u32 x) {
+  u32 lo = 0, hi = n;
+  while (hi - lo > 1) {
+    u32 mid = (lo + hi) >> 1;
+    if (a[mid] <= x) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+}  // namespace fmcs

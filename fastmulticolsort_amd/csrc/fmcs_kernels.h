@@ -1,0 +1,83 @@
+// fmcs_kernels.h -- kernel declarations and argument blocks.
+#pragma once
+#include "fmcs_common.h"
+
+namespace fmcs {
+
+constexpr int STITCH_MAX_SRC = 16;
+constexpr int STITCH_MAX_DST = 16;
+constexpr int STITCH_MAX_ENT = 64;
+
+// Kernel-argument block of the fused stitch kernel (passed by value; lives
+// in the kernarg segment, read with scalar loads).
+struct StitchArgs {
+  const void* src[STITCH_MAX_SRC];
+  void* dst[STITCH_MAX_DST];
+  u32 src_bank[STITCH_MAX_SRC];
+  u32 dst_bank[STITCH_MAX_DST];
+  u32 ent_begin[STITCH_MAX_DST + 1];
+  uint8_t e_src[STITCH_MAX_ENT];
+  uint8_t e_shift[STITCH_MAX_ENT];  // bw_src - end
+  uint8_t e_width[STITCH_MAX_ENT];  // end - start + 1
+  u32 ndst;
+  u64 n;
+};
+
+template <class K>
+struct TileSortArgs {
+  const K* kin[2];   // source banks by level parity
+  const u32* vin[2];
+  K* kout;           // may be null (keys not needed)
+  u32* vout;
+  const u32* tile_lo;
+  const u32* tile_hi;
+  const u32* tile_s0;
+  const u32* tile_s1;
+  const u32* seg_start;
+  u32 src_sel;
+  u32 kb;       // key bits still to sort (low bits of the key)
+  u32 pad_seg;  // sentinel segment index, above any real one
+  u32 direct;   // 1: one tile given by d_lo/d_hi (plain per-segment form)
+  u32 d_lo, d_hi;
+};
+
+constexpr int ONESWEEP_ITEMS = 16;
+constexpr int ONESWEEP_TILE = 256 * ONESWEEP_ITEMS;
+constexpr int BOUNDARY_ITEMS = 16;
+constexpr int BOUNDARY_TILE = 256 * BOUNDARY_ITEMS;
+constexpr int TILESORT_E = 8;
+constexpr int TILESORT_T = 256 * TILESORT_E;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = 256 * SCAN_ITEMS;
+
+__global__ void k_stitch(StitchArgs a);
+template <class T>
+__global__ void k_gather(const T*, const u32*, T*, u64);
+template <class K, int ITEMS>
+__global__ void k_boundary(const K*, const u32*, u64, u32*, u32*, u32*, u32*, u32*);
+template <class K>
+__global__ void k_hist_all(const K*, u64, u32, u32, u32*);
+template <class K, int ITEMS>
+__global__ void k_seg_hist(const K*, const u32*, const u32*, const u32*, u32, u32, u32, u32*);
+__global__ void k_digit_base(const u32*, const u32*, u32*, u32*, u32*);
+template <class K, int ITEMS>
+__global__ void k_onesweep(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, u32, const u32*,
+                           u32*, u32*, u32, u32);
+template <class K, int E>
+__global__ void k_tile_sort(TileSortArgs<K>);
+__global__ void k_windows(const u32*, const u32*, u32, u32, u32, u32*, u32*, u32*, u32*);
+__global__ void k_offsets_to_list(const u32*, u32, u32*, u32*);
+__global__ void k_large_flags(const u32*, u32, u32, u32*);
+__global__ void k_large_compact(const u32*, const u32*, u32, u32, const u32*, u32, u32*, u32*, u32*);
+template <class K>
+__global__ void k_copy_segs(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, u32, u32);
+template <int ITEMS>
+__global__ void k_scan_excl(const u32*, u32*, u64, u32*, u32*, u32*);
+__global__ void k_fill_u32(u32*, u64, u32);
+__global__ void k_iota_u32(u32*, u64);
+template <class K>
+__global__ void k_composite(const K*, const u32*, u64, u32, u64*);
+template <class K>
+__global__ void k_strip_composite(const u64*, u64, u64, K*);
+
+}  // namespace fmcs

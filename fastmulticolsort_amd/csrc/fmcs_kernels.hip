@@ -1,0 +1,711 @@
+// fmcs_kernels.hip -- gfx950 kernels of the multi-column sorting path.
+//
+// Every kernel here is integer, bandwidth-oriented work: no MFMA.  Keys live
+// in 16/32/64-bit banks (structure-of-arrays), oids are u32.  The kernels are
+// wave64-native: ranking uses 64-bit ballots, scans use wave shuffles.
+#include "fmcs_kernels.h"
+
+namespace fmcs {
+
+// ---------------------------------------------------------------------------
+// a3: fused stitch / bit-borrow.  One read of each source bank and one write
+// of each destination bank per row (src/column.h:105-1059 does one
+// read-modify-write pass per entry plus a shift pass per entry).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void load4(const void* p, u32 bank, u64 i, bool full, u64 n, u64 v[4]) {
+  if (full) {
+    switch (bank) {
+      case 1: {
+        u32 x = *(const u32*)((const uint8_t*)p + i);
+        for (int j = 0; j < 4; ++j) v[j] = (x >> (8 * j)) & 0xff;
+        break;
+      }
+      case 2: {
+        uint2 x = *(const uint2*)((const u16*)p + i);
+        v[0] = x.x & 0xffff; v[1] = x.x >> 16; v[2] = x.y & 0xffff; v[3] = x.y >> 16;
+        break;
+      }
+      case 4: {
+        uint4 x = *(const uint4*)((const u32*)p + i);
+        v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+        break;
+      }
+      default: {
+        const ulonglong2* q = (const ulonglong2*)((const u64*)p + i);
+        ulonglong2 a = q[0], b = q[1];
+        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+        break;
+      }
+    }
+  } else {
+    for (int j = 0; j < 4; ++j) {
+      u64 r = i + j;
+      if (r >= n) { v[j] = 0; continue; }
+      switch (bank) {
+        case 1: v[j] = ((const uint8_t*)p)[r]; break;
+        case 2: v[j] = ((const u16*)p)[r]; break;
+        case 4: v[j] = ((const u32*)p)[r]; break;
+        default: v[j] = ((const u64*)p)[r]; break;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void store4(void* p, u32 bank, u64 i, bool full, u64 n, const u64 v[4]) {
+  if (full) {
+    switch (bank) {
+      case 1: {
+        u32 x = (u32)(v[0] & 0xff) | ((u32)(v[1] & 0xff) << 8) | ((u32)(v[2] & 0xff) << 16) | ((u32)(v[3] & 0xff) << 24);
+        *(u32*)((uint8_t*)p + i) = x;
+        break;
+      }
+      case 2: {
+        uint2 x;
+        x.x = (u32)(v[0] & 0xffff) | ((u32)(v[1] & 0xffff) << 16);
+        x.y = (u32)(v[2] & 0xffff) | ((u32)(v[3] & 0xffff) << 16);
+        *(uint2*)((u16*)p + i) = x;
+        break;
+      }
+      case 4: {
+        uint4 x = make_uint4((u32)v[0], (u32)v[1], (u32)v[2], (u32)v[3]);
+        *(uint4*)((u32*)p + i) = x;
+        break;
+      }
+      default: {
+        ulonglong2* q = (ulonglong2*)((u64*)p + i);
+        q[0] = make_ulonglong2(v[0], v[1]);
+        q[1] = make_ulonglong2(v[2], v[3]);
+        break;
+      }
+    }
+  } else {
+    for (int j = 0; j < 4; ++j) {
+      u64 r = i + j;
+      if (r >= n) continue;
+      switch (bank) {
+        case 1: ((uint8_t*)p)[r] = (uint8_t)v[j]; break;
+        case 2: ((u16*)p)[r] = (u16)v[j]; break;
+        case 4: ((u32*)p)[r] = (u32)v[j]; break;
+        default: ((u64*)p)[r] = v[j]; break;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_stitch(StitchArgs a) {
+  u64 base = ((u64)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (base >= a.n) return;
+  bool full = base + 4 <= a.n;
+  for (u32 c = 0; c < a.ndst; ++c) {
+    u64 acc[4] = {0, 0, 0, 0};
+    for (u32 e = a.ent_begin[c]; e < a.ent_begin[c + 1]; ++e) {
+      u32 s = a.e_src[e], w = a.e_width[e], sh = a.e_shift[e];
+      u64 v[4];
+      load4(a.src[s], a.src_bank[s], base, full, a.n, v);
+      u64 m = w >= 64 ? ~0ull : ((1ull << w) - 1);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[j] = (w >= 64 ? 0 : (acc[j] << w)) | ((v[j] >> sh) & m);
+    }
+    store4(a.dst[c], a.dst_bank[c], base, full, a.n, acc);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// a6: oid-permutation gather, out[i] = vals[order[i]] (src/composer.cpp:608-628).
+// Four rows per thread: one 16-byte order load, four random loads, one
+// vector store.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_gather(const T* __restrict__ vals, const u32* __restrict__ order,
+                                                T* __restrict__ out, u64 n) {
+  u64 i = ((u64)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (i + 4 <= n) {
+    uint4 o = *(const uint4*)(order + i);
+    T r0 = vals[o.x], r1 = vals[o.y], r2 = vals[o.z], r3 = vals[o.w];
+    out[i] = r0; out[i + 1] = r1; out[i + 2] = r2; out[i + 3] = r3;
+  } else {
+    for (; i < n; ++i) out[i] = vals[order[i]];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// a9: group boundaries (src/composer.cpp:630-678) as head flags plus a
+// single-pass decoupled look-back scan.  Writes dense monotone group ids and,
+// optionally, the start row of every group (segment offsets, with
+// offsets[ngroups] = n).
+// Tile layout: row = tile_base + j*256 + tid (coalesced per item index j).
+// ---------------------------------------------------------------------------
+template <class K, int ITEMS>
+__global__ __launch_bounds__(256) void k_boundary(const K* __restrict__ v, const u32* __restrict__ in_group, u64 n,
+                                                  u32* __restrict__ out_group, u32* __restrict__ seg_off,
+                                                  u32* __restrict__ status, u32* __restrict__ counter,
+                                                  u32* __restrict__ ngroups_out) {
+  constexpr int NT = 256, NW = NT / WAVE, TILE = NT * ITEMS;
+  __shared__ u32 s_tile;
+  __shared__ u32 s_cnt[ITEMS][NW];
+  __shared__ u32 s_excl;
+  u32 tid = threadIdx.x, wv = tid / WAVE, lane = tid & (WAVE - 1);
+  if (tid == 0) s_tile = atomicAdd(counter, 1u);
+  __syncthreads();
+  u32 tile = s_tile;
+  u64 base = (u64)tile * TILE;
+  u64 masks[ITEMS];
+  u64 lt = lanemask_lt();
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    u64 r = base + (u64)j * NT + tid;
+    bool head = false;
+    if (r < n) {
+      head = (r == 0);
+      if (!head) {
+        head = v[r] != v[r - 1];
+        if (in_group) head = head || (in_group[r] != in_group[r - 1]);
+      }
+    }
+    u64 m = __ballot(head);
+    masks[j] = m;
+    if (lane == 0) s_cnt[j][wv] = (u32)__popcll(m);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    u32 tot = 0;
+    for (int j = 0; j < ITEMS; ++j)
+      for (int w = 0; w < NW; ++w) tot += s_cnt[j][w];
+    u32 excl = 0;
+    if (tile == 0) {
+      status_store(status, ST_INC | tot);
+    } else {
+      status_store(status + tile, ST_AGG | tot);
+      excl = lookback(status, tile, 0, 1);
+      status_store(status + tile, ST_INC | (excl + tot));
+    }
+    s_excl = excl;
+    if (base + TILE >= n) {
+      *ngroups_out = excl + tot;
+      if (seg_off) seg_off[excl + tot] = (u32)n;
+    }
+  }
+  __syncthreads();
+  u32 run = s_excl;
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    u32 before_w = 0, tot_j = 0;
+    for (int w = 0; w < NW; ++w) {
+      u32 c = s_cnt[j][w];
+      before_w += (w < (int)wv) ? c : 0;
+      tot_j += c;
+    }
+    u64 r = base + (u64)j * NT + tid;
+    if (r < n) {
+      bool head = (masks[j] >> lane) & 1;
+      u32 prefix = run + before_w + (u32)__popcll(masks[j] & lt);  // heads strictly before r
+      if (out_group) out_group[r] = head ? prefix : prefix - 1u;
+      if (head && seg_off) seg_off[prefix] = (u32)r;
+    }
+    run += tot_j;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Radix histograms.
+// k_hist_all: one pass over all keys computing the digit histogram of every
+// LSD pass at once (npasses <= 8).
+// k_seg_hist: per-segment histogram of one digit, over tiles that never
+// cross a segment (used by the segmented MSD partition).
+// ---------------------------------------------------------------------------
+template <class K>
+__global__ __launch_bounds__(256) void k_hist_all(const K* __restrict__ keys, u64 n, u32 npasses, u32 dbits,
+                                                  u32* __restrict__ hist) {
+  __shared__ u32 sh[8][RADIX];
+  for (u32 i = threadIdx.x; i < 8 * RADIX; i += blockDim.x) (&sh[0][0])[i] = 0;
+  __syncthreads();
+  u32 mask = (1u << dbits) - 1;
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) {
+    u64 k = (u64)keys[i];
+    for (u32 p = 0; p < npasses; ++p) atomicAdd(&sh[p][(u32)(k >> (p * dbits)) & mask], 1u);
+  }
+  __syncthreads();
+  for (u32 p = 0; p < npasses; ++p) {
+    u32 c = sh[p][threadIdx.x];
+    if (c) atomicAdd(&hist[p * RADIX + threadIdx.x], c);
+  }
+}
+
+template <class K, int ITEMS>
+__global__ __launch_bounds__(256) void k_seg_hist(const K* __restrict__ keys, const u32* __restrict__ seg_lo,
+                                                  const u32* __restrict__ seg_hi, const u32* __restrict__ tile_base,
+                                                  u32 nsegs, u32 shift, u32 dbits, u32* __restrict__ hist) {
+  constexpr int TILE = 256 * ITEMS;
+  __shared__ u32 sh[RADIX];
+  __shared__ u32 s_seg;
+  u32 tile = blockIdx.x;
+  sh[threadIdx.x] = 0;
+  if (threadIdx.x == 0) s_seg = upper_bound_m1(tile_base, nsegs, tile);
+  __syncthreads();
+  u32 seg = s_seg;
+  u64 row0 = (u64)seg_lo[seg] + (u64)(tile - tile_base[seg]) * TILE;
+  u64 rowend = min((u64)seg_hi[seg], row0 + TILE);
+  u32 mask = (1u << dbits) - 1;
+  for (u64 r = row0 + threadIdx.x; r < rowend; r += 256) atomicAdd(&sh[digit_of(keys[r], shift, mask)], 1u);
+  __syncthreads();
+  u32 c = sh[threadIdx.x];
+  if (c) atomicAdd(&hist[(u64)seg * RADIX + threadIdx.x], c);
+}
+
+// digit_base[s][d] = lo[s] + sum_{d' < d} hist[s][d'] ; one block per set.
+// Optionally emits the bucket list (start, size) of set s in (s, d) order.
+__global__ __launch_bounds__(256) void k_digit_base(const u32* __restrict__ hist, const u32* __restrict__ lo,
+                                                    u32* __restrict__ digit_base, u32* __restrict__ bstart,
+                                                    u32* __restrict__ bsize) {
+  __shared__ u32 tmp[8];
+  u32 s = blockIdx.x, d = threadIdx.x;
+  u32 c = hist[(u64)s * RADIX + d];
+  u32 tot;
+  u32 ex = block_excl_scan<256>(c, tmp, &tot);
+  u32 b = (lo ? lo[s] : 0u) + ex;
+  digit_base[(u64)s * RADIX + d] = b;
+  if (bstart) {
+    bstart[(u64)s * RADIX + d] = b;
+    bsize[(u64)s * RADIX + d] = c;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One radix pass (LSD over the whole array, or one MSD level over a list of
+// segments) in a single sweep with decoupled look-back, after the
+// "onesweep" scheme: tiles take ids from an atomic counter, rank their keys
+// stably with 64-lane ballot matching into per-wave LDS histograms, look back
+// over earlier tiles of the same segment for the per-digit global offset,
+// then scatter through LDS so the global writes are contiguous runs.
+// Tile layout: wave w owns rows [row0 + w*64*ITEMS, +64*ITEMS); item j of
+// lane l is row wave_base + j*64 + l, so the ranking order (j, lane) is the
+// row order and the pass is stable.
+// ---------------------------------------------------------------------------
+template <class K, int ITEMS>
+__global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, const u32* __restrict__ vin,
+                                                  K* __restrict__ kout, u32* __restrict__ vout,
+                                                  const u32* __restrict__ seg_lo, const u32* __restrict__ seg_hi,
+                                                  const u32* __restrict__ tile_base, u32 nsegs,
+                                                  const u32* __restrict__ digit_base, u32* __restrict__ status,
+                                                  u32* __restrict__ counter, u32 shift, u32 dbits) {
+  constexpr int NT = 256, NW = NT / WAVE, TILE = NT * ITEMS, WITEMS = WAVE * ITEMS;
+  __shared__ u32 s_tile, s_seg;
+  __shared__ u32 s_whist[NW][RADIX];
+  __shared__ u32 s_tdig[RADIX];
+  __shared__ u32 s_goff[RADIX];
+  __shared__ u32 s_tmp[NW + 1];
+  __shared__ K s_keys[TILE];
+  __shared__ u32 s_vals[TILE];
+
+  u32 tid = threadIdx.x, wv = tid / WAVE, lane = tid & (WAVE - 1);
+  for (u32 i = tid; i < NW * RADIX; i += NT) (&s_whist[0][0])[i] = 0;
+  if (tid == 0) {
+    u32 t = atomicAdd(counter, 1u);
+    s_tile = t;
+    s_seg = upper_bound_m1(tile_base, nsegs, t);
+  }
+  __syncthreads();
+  const u32 tile = s_tile, seg = s_seg;
+  const u32 first_tile = tile_base[seg];
+  const u64 row0 = (u64)seg_lo[seg] + (u64)(tile - first_tile) * TILE;
+  const u64 rowend = min((u64)seg_hi[seg], row0 + TILE);
+  const u32 mask = (1u << dbits) - 1;
+  const u64 wbase = row0 + (u64)wv * WITEMS;
+  const u64 lt = lanemask_lt();
+
+  K keys[ITEMS];
+  u32 vals[ITEMS];
+  u32 rank[ITEMS];
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    u64 r = wbase + (u64)j * WAVE + lane;
+    bool ok = r < rowend;
+    keys[j] = ok ? kin[r] : (K)0;
+    vals[j] = ok ? vin[r] : 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    u64 r = wbase + (u64)j * WAVE + lane;
+    bool ok = r < rowend;
+    u32 d = digit_of(keys[j], shift, mask);
+    u64 peers = __ballot(ok);
+    for (u32 b = 0; b < dbits; ++b) {
+      u64 bb = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    if (!ok) peers = 0;
+    u32 leader = ok ? (u32)(__ffsll((long long)peers) - 1) : lane;
+    u32 old = 0;
+    if (ok && lane == leader) {
+      old = s_whist[wv][d];
+      s_whist[wv][d] = old + (u32)__popcll(peers);
+    }
+    old = __shfl(old, (int)leader);
+    rank[j] = old + (u32)__popcll(peers & lt);
+  }
+  __syncthreads();
+  // per-digit: wave offsets within the tile and the tile's count
+  u32 d = tid;
+  u32 run = 0;
+  for (int w = 0; w < NW; ++w) {
+    u32 c = s_whist[w][d];
+    s_whist[w][d] = run;
+    run += c;
+  }
+  const u32 tcount = run;
+  u32* st = status + d;
+  u32 excl = 0;
+  if (tile == first_tile) {
+    status_store(st + (u64)tile * RADIX, ST_INC | tcount);
+  } else {
+    status_store(st + (u64)tile * RADIX, ST_AGG | tcount);
+    excl = lookback(st, tile, first_tile, RADIX);
+    status_store(st + (u64)tile * RADIX, ST_INC | (excl + tcount));
+  }
+  u32 ttot;
+  u32 tdig = block_excl_scan<NT>(tcount, s_tmp, &ttot);
+  s_tdig[d] = tdig;
+  s_goff[d] = digit_base[(u64)seg * RADIX + d] + excl - tdig;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    u64 r = wbase + (u64)j * WAVE + lane;
+    if (r < rowend) {
+      u32 dd = digit_of(keys[j], shift, mask);
+      u32 pos = s_tdig[dd] + s_whist[wv][dd] + rank[j];
+      s_keys[pos] = keys[j];
+      s_vals[pos] = vals[j];
+    }
+  }
+  __syncthreads();
+  const u32 cnt = (u32)(rowend - row0);
+  for (u32 i = tid; i < cnt; i += NT) {
+    K k = s_keys[i];
+    u32 g = s_goff[digit_of(k, shift, mask)] + i;
+    kout[g] = k;
+    vout[g] = s_vals[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LDS tile sort for packed small segments: a block merge sort over the
+// composite (local segment, masked key, oid).  Each of the 256 threads sorts
+// E elements in registers, then log2(256) merge-path levels run through LDS.
+// A tile is a window of rows holding whole segments; the segment index is
+// recomputed from the segment starts, so no per-row segment ids are read.
+// The top (already partitioned) key bits of every segment are kept in LDS
+// and restored on the way out.
+// ---------------------------------------------------------------------------
+struct Elt {
+  u64 h;
+  u32 o;
+};
+__device__ __forceinline__ bool elt_lt(const Elt& a, const Elt& b) { return a.h < b.h || (a.h == b.h && a.o < b.o); }
+__device__ __forceinline__ void cas(Elt& a, Elt& b) {
+  if (elt_lt(b, a)) {
+    Elt t = a;
+    a = b;
+    b = t;
+  }
+}
+
+template <int E>
+__device__ __forceinline__ void reg_sort(Elt (&x)[E]) {
+  // bitonic network over E (power of two) registers
+#pragma unroll
+  for (int k = 2; k <= E; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        int l = i ^ j;
+        if (l > i) {
+          if ((i & k) == 0) cas(x[i], x[l]);
+          else cas(x[l], x[i]);
+        }
+      }
+    }
+  }
+}
+
+template <class K, int E>
+__global__ __launch_bounds__(256) void k_tile_sort(TileSortArgs<K> a) {
+  constexpr int NT = 256, T = NT * E;
+  __shared__ u64 s_h[T];
+  __shared__ u32 s_o[T];
+  __shared__ u64 s_top[T];
+  __shared__ u32 s_tmp[NT / WAVE + 1];
+  u32 tid = threadIdx.x;
+  u32 lo, hi, s0, s1;
+  if (a.direct) {
+    lo = a.d_lo; hi = a.d_hi; s0 = 0; s1 = 0;
+  } else {
+    u32 w = blockIdx.x;
+    lo = a.tile_lo[w]; hi = a.tile_hi[w]; s0 = a.tile_s0[w]; s1 = a.tile_s1[w];
+  }
+  if (lo >= hi) return;
+  const u32 cnt = hi - lo;
+  const u32 kb = a.kb;
+  const u64 kmask = kb >= 64 ? ~0ull : ((1ull << kb) - 1);
+  const K* kin = a.kin[a.direct ? 0 : a.src_sel];
+  const u32* vin = a.vin[a.direct ? 0 : a.src_sel];
+  // segment-start flags -> s_o temporarily
+  for (u32 i = tid; i < T; i += NT) s_o[i] = 0;
+  __syncthreads();
+  if (!a.direct) {
+    for (u32 s = s0 + tid; s <= s1; s += NT) {
+      u32 st = a.seg_start[s];
+      if (st >= lo && st < hi) s_o[st - lo] = 1;
+    }
+  }
+  if (tid == 0) s_o[0] = 1;
+  __syncthreads();
+  // blocked inclusive scan of flags -> local segment index
+  u32 f[E];
+  u32 sum = 0;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    f[e] = s_o[tid * E + e];
+    sum += f[e];
+  }
+  u32 tot;
+  u32 ex = block_excl_scan<NT>(sum, s_tmp, &tot);
+  u32 segi[E];
+  u32 runs = ex;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    runs += f[e];
+    segi[e] = runs - 1;
+  }
+  __syncthreads();
+  // stage keys: coalesced global loads into LDS
+  for (u32 i = tid; i < T; i += NT) {
+    if (i < cnt) {
+      s_h[i] = (u64)kin[lo + i];
+      s_o[i] = vin[lo + i];
+    }
+  }
+  __syncthreads();
+  Elt x[E];
+  const u64 pad = ((u64)a.pad_seg << kb);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    u32 i = tid * E + e;
+    if (i < cnt) {
+      u64 k = s_h[i];
+      if (f[e]) s_top[segi[e]] = k & ~kmask;
+      x[e].h = ((u64)segi[e] << kb) | (k & kmask);
+      x[e].o = s_o[i];
+    } else {
+      x[e].h = pad;
+      x[e].o = 0xffffffffu;
+    }
+  }
+  reg_sort<E>(x);
+  for (u32 w = E; w < (u32)T; w <<= 1) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      s_h[tid * E + e] = x[e].h;
+      s_o[tid * E + e] = x[e].o;
+    }
+    __syncthreads();
+    u32 out0 = tid * E;
+    u32 base = out0 & ~(2 * w - 1);
+    u32 dgn = out0 - base;
+    u32 alo = dgn > w ? dgn - w : 0, ahi = dgn < w ? dgn : w;
+    while (alo < ahi) {
+      u32 mid = (alo + ahi) >> 1;
+      Elt A = {s_h[base + mid], s_o[base + mid]};
+      Elt B = {s_h[base + w + dgn - 1 - mid], s_o[base + w + dgn - 1 - mid]};
+      if (elt_lt(B, A)) ahi = mid; else alo = mid + 1;
+    }
+    u32 ia = base + alo, ib = base + w + (dgn - alo);
+    const u32 ea = base + w, eb = base + 2 * w;
+    Elt A, B;
+    if (ia < ea) { A.h = s_h[ia]; A.o = s_o[ia]; }
+    if (ib < eb) { B.h = s_h[ib]; B.o = s_o[ib]; }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      bool takeB = (ia >= ea) || (ib < eb && elt_lt(B, A));
+      if (takeB) {
+        x[e] = B;
+        ++ib;
+        if (ib < eb) { B.h = s_h[ib]; B.o = s_o[ib]; }
+      } else {
+        x[e] = A;
+        ++ia;
+        if (ia < ea) { A.h = s_h[ia]; A.o = s_o[ia]; }
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    s_h[tid * E + e] = x[e].h;
+    s_o[tid * E + e] = x[e].o;
+  }
+  __syncthreads();
+  for (u32 i = tid; i < cnt; i += NT) {
+    u64 h = s_h[i];
+    u32 sg = (u32)(h >> kb);
+    u64 k = s_top[sg] | (h & kmask);
+    if (a.kout) a.kout[lo + i] = (K)k;
+    a.vout[lo + i] = s_o[i];
+  }
+}
+
+// Window packing of small segments (size <= small_max) of a sorted,
+// disjoint segment list: segment i belongs to window start[i] / W.  A large
+// segment (> small_max >= W-1 rows) that starts in a window runs past its
+// end, so the small segments of a window are contiguous and precede it.
+__global__ void k_windows(const u32* __restrict__ st, const u32* __restrict__ sz, u32 nseg, u32 W, u32 small_max,
+                          u32* __restrict__ tlo, u32* __restrict__ thi, u32* __restrict__ ts0,
+                          u32* __restrict__ ts1) {
+  u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nseg) return;
+  u32 size = sz[i];
+  if (size > small_max) return;
+  u32 w = st[i] / W;
+  bool first = (i == 0) || (st[i - 1] / W != w);
+  bool last = (i + 1 == nseg) || (st[i + 1] / W != w) || (sz[i + 1] > small_max);
+  if (first) { tlo[w] = st[i]; ts0[w] = i; }
+  if (last) { thi[w] = st[i] + size; ts1[w] = i; }
+}
+
+// Segment list from offsets: start[i] = off[i], size[i] = off[i+1]-off[i].
+__global__ void k_offsets_to_list(const u32* __restrict__ off, u32 nseg, u32* __restrict__ st, u32* __restrict__ sz) {
+  u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nseg) return;
+  u32 a = off[i], b = off[i + 1];
+  st[i] = a;
+  sz[i] = b - a;
+}
+
+// Large segments: flag, then (after a scan) compact to (lo, hi) and count
+// their tiles.
+__global__ void k_large_flags(const u32* __restrict__ sz, u32 nseg, u32 small_max, u32* __restrict__ flag) {
+  u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nseg) return;
+  flag[i] = sz[i] > small_max ? 1u : 0u;
+}
+
+__global__ void k_large_compact(const u32* __restrict__ st, const u32* __restrict__ sz, u32 nseg, u32 small_max,
+                                const u32* __restrict__ pos, u32 tile_rows, u32* __restrict__ llo,
+                                u32* __restrict__ lhi, u32* __restrict__ ltiles) {
+  u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nseg) return;
+  u32 s = sz[i];
+  if (s <= small_max) return;
+  u32 p = pos[i];
+  llo[p] = st[i];
+  lhi[p] = st[i] + s;
+  ltiles[p] = (s + tile_rows - 1) / tile_rows;
+}
+
+// Copy rows of segments that need no sorting (all key bits consumed).
+template <class K>
+__global__ void k_copy_segs(const K* __restrict__ kin, const u32* __restrict__ vin, K* __restrict__ kout,
+                            u32* __restrict__ vout, const u32* __restrict__ llo, const u32* __restrict__ lhi,
+                            const u32* __restrict__ tile_base, u32 nsegs, u32 tile_rows) {
+  __shared__ u32 s_seg;
+  if (threadIdx.x == 0) s_seg = upper_bound_m1(tile_base, nsegs, blockIdx.x);
+  __syncthreads();
+  u32 seg = s_seg;
+  u64 r0 = (u64)llo[seg] + (u64)(blockIdx.x - tile_base[seg]) * tile_rows;
+  u64 r1 = min((u64)lhi[seg], r0 + tile_rows);
+  for (u64 r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+    if (kout) kout[r] = kin[r];
+    vout[r] = vin[r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Device scan (exclusive, u32) in one sweep with decoupled look-back; total
+// written to *total.  Values must sum below 2^30.
+// ---------------------------------------------------------------------------
+template <int ITEMS>
+__global__ __launch_bounds__(256) void k_scan_excl(const u32* __restrict__ in, u32* __restrict__ out, u64 n,
+                                                   u32* __restrict__ status, u32* __restrict__ counter,
+                                                   u32* __restrict__ total) {
+  constexpr int NT = 256, TILE = NT * ITEMS;
+  __shared__ u32 s_tile, s_excl;
+  __shared__ u32 s_tmp[NT / WAVE + 1];
+  u32 tid = threadIdx.x;
+  if (tid == 0) s_tile = atomicAdd(counter, 1u);
+  __syncthreads();
+  u32 tile = s_tile;
+  u64 base = (u64)tile * TILE + (u64)tid * ITEMS;
+  u32 v[ITEMS];
+  u32 sum = 0;
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    v[j] = (base + j < n) ? in[base + j] : 0u;
+    sum += v[j];
+  }
+  u32 tot;
+  u32 ex = block_excl_scan<NT>(sum, s_tmp, &tot);
+  if (tid == 0) {
+    u32 e = 0;
+    if (tile == 0) {
+      status_store(status, ST_INC | tot);
+    } else {
+      status_store(status + tile, ST_AGG | tot);
+      e = lookback(status, tile, 0, 1);
+      status_store(status + tile, ST_INC | (e + tot));
+    }
+    s_excl = e;
+    if ((u64)(tile + 1) * TILE >= n && total) *total = e + tot;
+  }
+  __syncthreads();
+  u32 run = s_excl + ex;
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    if (base + j < n) out[base + j] = run;
+    run += v[j];
+  }
+}
+
+__global__ void k_fill_u32(u32* __restrict__ p, u64 n, u32 v) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_iota_u32(u32* __restrict__ p, u64 n) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) p[i] = (u32)i;
+}
+
+// Composite key for the single unbinned segmented sort: (group << kb) | key.
+template <class K>
+__global__ void k_composite(const K* __restrict__ k, const u32* __restrict__ g, u64 n, u32 kb, u64* __restrict__ out) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x)
+    out[i] = ((u64)g[i] << kb) | (u64)k[i];
+}
+
+template <class K>
+__global__ void k_strip_composite(const u64* __restrict__ c, u64 n, u64 kmask, K* __restrict__ k) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x)
+    k[i] = (K)(c[i] & kmask);
+}
+
+// ---------------------------------------------------------------------------
+// Explicit instantiations used by the launchers.
+// ---------------------------------------------------------------------------
+#define FMCS_INST_K(K)                                                                                              \
+  template __global__ void k_gather<K>(const K*, const u32*, K*, u64);                                              \
+  template __global__ void k_boundary<K, 16>(const K*, const u32*, u64, u32*, u32*, u32*, u32*, u32*);              \
+  template __global__ void k_hist_all<K>(const K*, u64, u32, u32, u32*);                                            \
+  template __global__ void k_seg_hist<K, 16>(const K*, const u32*, const u32*, const u32*, u32, u32, u32, u32*);     \
+  template __global__ void k_onesweep<K, 16>(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*,    \
+                                             u32, const u32*, u32*, u32*, u32, u32);                                \
+  template __global__ void k_tile_sort<K, 8>(TileSortArgs<K>);                                                      \
+  template __global__ void k_copy_segs<K>(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, u32,  \
+                                          u32);                                                                     \
+  template __global__ void k_composite<K>(const K*, const u32*, u64, u32, u64*);                                    \
+  template __global__ void k_strip_composite<K>(const u64*, u64, u64, K*);
+
+FMCS_INST_K(u16)
+FMCS_INST_K(u32)
+FMCS_INST_K(u64)
+template __global__ void k_scan_excl<8>(const u32*, u32*, u64, u32*, u32*, u32*);
+
+}  // namespace fmcs

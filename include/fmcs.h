@@ -1,0 +1,180 @@
+/*
+ * fmcs.h -- C ABI of the MI355X-native multi-column sorting path.
+ *
+ * The reference (FastMultiColSort) is a C++ class library with no C ABI;
+ * each entry point below replaces one of its hot-path seams, cited as
+ * file:line in the reference tree.  Conventions:
+ *   - every array argument named *_d is DEVICE memory owned by the caller;
+ *     host arrays (lists of pointers, widths, entries) are plain host memory;
+ *   - results go to caller-named outputs; nothing is swapped or freed;
+ *   - calls are ordered on the context's stream; functions that return a
+ *     count to the host synchronize that stream;
+ *   - every function returns FMCS_OK or an error code and never exits;
+ *     fmcs_last_error() gives the message of the last failure;
+ *   - a context is not shared between threads.
+ * A bank is the 1/2/4/8-byte container of a column; its value uses the low
+ * `bitwidth` bits.  Oids are uint32 (src/params.h:9-13).
+ */
+#ifndef FMCS_H
+#define FMCS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FMCS_OK 0
+#define FMCS_E_ARG 1
+#define FMCS_E_NOMEM 2
+#define FMCS_E_HIP 3
+#define FMCS_E_UNSUPPORTED 4
+
+/* Segmented-sort forms for rounds > 1. */
+#define FMCS_MODE_BATCHED 0  /* size-binned: packed LDS tiles + segmented MSD radix */
+#define FMCS_MODE_PLAIN 1    /* one sort dispatch per group (src/composer.cpp:1285-1323) */
+#define FMCS_MODE_UNBINNED 2 /* one LSD radix sort of (group id, key) composites */
+
+typedef struct fmcs_ctx fmcs_ctx;
+
+/* One unit of bit movement into a new column: construct_entry_t
+ * (src/types.h:42-46) plus the index of the new column it builds.
+ * [start, end] is 1-based from the source column's MSB, inclusive. */
+typedef struct {
+  uint32_t newcol;
+  uint32_t src;
+  uint32_t start;
+  uint32_t end;
+} fmcs_entry;
+
+/* group_info_t (src/types.h:188-195). */
+typedef struct {
+  uint32_t ngroups;
+  uint32_t sum_group_size;
+  uint32_t ngroups_incache;
+  uint32_t ngroups_outofcache;
+  uint32_t ngroups_gtOne;
+} fmcs_group_info;
+
+/* Tunables of the batched segmented sort. */
+typedef struct {
+  uint32_t small_max; /* segments up to this many rows go to packed LDS tiles */
+  uint32_t window;    /* rows per packing window (<= small_max + 1) */
+  uint32_t msd_bits;  /* digit width of one MSD level (1..8) */
+} fmcs_tuning;
+
+/* ---- context -------------------------------------------------------- */
+int fmcs_ctx_create(int device, fmcs_ctx** out);
+int fmcs_ctx_destroy(fmcs_ctx* ctx);
+/* hipStream_t as void*; NULL selects the default stream. */
+int fmcs_ctx_set_stream(fmcs_ctx* ctx, void* stream);
+int fmcs_ctx_sync(fmcs_ctx* ctx);
+int fmcs_ctx_set_tuning(fmcs_ctx* ctx, const fmcs_tuning* t);
+int fmcs_ctx_get_tuning(fmcs_ctx* ctx, fmcs_tuning* t);
+/* Frees the context's cached workspace. */
+int fmcs_ctx_trim(fmcs_ctx* ctx);
+const char* fmcs_last_error(fmcs_ctx* ctx);
+
+/* ---- host-side plan helpers ----------------------------------------- */
+
+/* Bank bytes for a width: <=16 -> 2, <=32 -> 4, else 8; allow_u8 adds 1 for
+ * <=8 (src/stitch_composer.cpp:622-650, hash path :883-890). */
+uint32_t fmcs_bank_bytes(uint32_t width, int allow_u8);
+
+/* Step a of StitchComposer::run_single_plan (src/stitch_composer.cpp:
+ * 571-612): maps a column order and a split (new round widths) to the
+ * construct entries, MSB first.  Returns the entry count in *n_out. */
+int fmcs_construct_entries(uint32_t ncols, const uint32_t* bw, const uint32_t* col_order, uint32_t nsplit,
+                           const uint32_t* split, fmcs_entry* entries, uint32_t max_entries, uint32_t* n_out);
+
+/* ---- kernels -------------------------------------------------------- */
+
+/* Step b2 of run_single_plan (src/stitch_composer.cpp:652-703):
+ * SpecialiedColumn<T>::ShiftLeftBatch (src/column.h:105-180) and
+ * AssembleFromColumn (src/column.h:208-1059), fused: every destination bank
+ * is written once from zero.  src_d[i]: bank src_bank[i] holding
+ * src_bw[i]-bit values; dst_d[c]: bank dst_bank[c].  Entries must be grouped
+ * by newcol in ascending order, as fmcs_construct_entries emits them. */
+int fmcs_stitch(fmcs_ctx* ctx, uint64_t n, uint32_t nsrc, const void* const* src_d, const uint32_t* src_bank,
+                const uint32_t* src_bw, uint32_t nent, const fmcs_entry* entries, uint32_t ndst,
+                void* const* dst_d, const uint32_t* dst_bank);
+
+/* Composer::BATproject<T> (src/composer.cpp:608-628): out[i] = values[order[i]]. */
+int fmcs_gather(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* values_d, const uint32_t* order_d,
+                void* out_d);
+
+/* Composer::extractGroupInfo<T> (src/composer.cpp:630-678): dense monotone
+ * group ids of sorted values (and the previous groups, if in_group_d is not
+ * NULL).  seg_offsets_d (optional, ngroups+1 entries, up to n+1) receives the
+ * first row of every group and n.  *ngroups (host) receives the count. */
+int fmcs_extract_group_info(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* sorted_d,
+                            const uint32_t* in_group_d, uint32_t* out_group_d, uint32_t* seg_offsets_d,
+                            uint32_t* ngroups);
+
+/* Mergesort::avxmergesort_anytype<T> (src/mergesort.h:62-64, :2878-3096)
+ * as used by Composer::subsort (src/composer.cpp:490-589): sorts the pairs
+ * (keys_d[i], oids_d[i]) by the low keybits bits of the key, in place.  Ties
+ * keep their incoming order (LSD radix sort, stable). */
+int fmcs_sort_pairs(fmcs_ctx* ctx, uint32_t bank, uint32_t keybits, uint64_t n, void* keys_d, uint32_t* oids_d);
+
+/* The per-group loop of Composer::sortByGroup (src/composer.cpp:1280-1335):
+ * sorts (keys, oids) within each segment [off[s], off[s+1]), in place, by
+ * (key, oid).  mode: FMCS_MODE_*.  info (host, optional) receives the group
+ * statistics the reference accumulates. */
+int fmcs_segmented_sort(fmcs_ctx* ctx, uint32_t bank, uint32_t keybits, uint64_t n, void* keys_d,
+                        uint32_t* oids_d, const uint32_t* seg_offsets_d, uint32_t nseg, int mode,
+                        fmcs_group_info* info);
+
+/* Composer::sortByGroup<T> (src/composer.cpp:1222-1408; declared
+ * src/composer.h:85-88).  column_d: the round's bank indexed by oid.
+ * order_d: in = order after the previous round (round 1: identity),
+ * out = order after this round.  in_group_d: NULL in round 1.
+ * out_group_d: written unless last_round.  sorted_d (optional): the round's
+ * sorted values.  *ngroups (host, optional) receives the new group count
+ * (0 on the last round). */
+int fmcs_sort_by_group(fmcs_ctx* ctx, uint32_t bank, uint32_t keybits, uint64_t n, const void* column_d,
+                       uint32_t* order_d, const uint32_t* in_group_d, uint32_t* out_group_d, int last_round,
+                       int mode, void* sorted_d, fmcs_group_info* info, uint32_t* ngroups);
+
+/* Composer::multiRoundSorting (src/composer.cpp:48-135) with its results
+ * kept: order_d receives the final oid permutation; groups_d (optional)
+ * receives the group ids after the last round (this also computes the last
+ * round's groups, which the reference skips); *ngroups (optional) the final
+ * group count (groups of the full key when groups_d is given, else of the
+ * second-to-last round). */
+int fmcs_multi_round_sort(fmcs_ctx* ctx, uint32_t nrounds, const void* const* cols_d, const uint32_t* banks,
+                          const uint32_t* bitwidths, uint64_t n, uint32_t* order_d, uint32_t* groups_d, int mode,
+                          uint32_t* ngroups);
+
+/* Per-phase times of the last fmcs_multi_round_sort / fmcs_multi_round_hash
+ * call in milliseconds, measured with HIP events on the context's stream:
+ * [0] gather (tuple reconstruct), [1] first-round sort, [2] later-round
+ * sorts, [3] group extraction.  Mirrors setting_t's timing fields
+ * (src/types.h:206-221). */
+int fmcs_last_breakdown(fmcs_ctx* ctx, double* ms4);
+
+/* ---- hash grouping (src/composer.cpp:137-227, :814-976;
+ *      src/hashing.h:289-385; src/partition.h:44-161) ------------------ */
+
+/* One round of hashByGroup_reordered: within each segment of the previous
+ * round's groups (in_group_d NULL in round 1) give dense ids in order of
+ * first appearance (Hashing::subgroup_range), numbered across segments in
+ * segment order.  gid_d receives the ids per row; unless last_round the
+ * rows are clustered stably by id (Partition::histogram_based_reordering):
+ * out_order_d and out_group_d receive the clustered oids and ids.
+ * values_d is the round's bank indexed by oid; order_d the current order. */
+int fmcs_hash_round(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* values_d, const uint32_t* order_d,
+                    const uint32_t* in_group_d, uint32_t* gid_d, uint32_t* out_order_d, uint32_t* out_group_d,
+                    int last_round, uint32_t* ngroups);
+
+/* Composer::multiRoundHashing_reordered (src/composer.cpp:137-227) with its
+ * results kept: order_d = order the last round hashed in, gid_d = the last
+ * round's ids per row, *ngroups = number of groups. */
+int fmcs_multi_round_hash(fmcs_ctx* ctx, uint32_t nrounds, const void* const* cols_d, const uint32_t* banks,
+                          uint64_t n, uint32_t* order_d, uint32_t* gid_d, uint32_t* ngroups);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FMCS_H */

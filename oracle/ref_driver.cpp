@@ -1,0 +1,151 @@
+// ref_driver.cpp -- a thin C entry layer over the reference's own sources,
+// compiled (by oracle/Makefile, target `ref`) from /root/reference/src where
+// they lie.  It is TEST INFRASTRUCTURE: it exists to cross-check the oracle
+// and to time the reference's AVX2 path as the CPU baseline.  Nothing in the
+// product links it.
+//
+// The reference's round loop (src/composer.cpp:48-135) frees its results, so
+// these wrappers call its per-round members directly and keep the outputs.
+#include "composer.cpp"        // Composer::sortByGroup / subsort_mergesort / hashByGroup_* templates
+#include "chain_composer.cpp"  // ChainComposer (concrete Composer)
+
+#include <cstring>
+#include <cstdint>
+
+using namespace multiAttrSort;
+
+namespace {
+
+template <class T>
+T *aligned_copy(const void *src, uint64_t n) {
+  T *p = (T *)malloc_aligned((n + 64) * sizeof(T));
+  std::memcpy(p, src, n * sizeof(T));
+  return p;
+}
+
+Column *make_column(uint32_t bank, void *data) {
+  switch (bank) {
+    case 2: { auto *c = new SpecialiedColumn<uint16_t>(); c->SetColumn((uint16_t *)data); c->SetWidth(2); return c; }
+    case 4: { auto *c = new SpecialiedColumn<uint32_t>(); c->SetColumn((uint32_t *)data); c->SetWidth(4); return c; }
+    default: { auto *c = new SpecialiedColumn<uint64_t>(); c->SetColumn((uint64_t *)data); c->SetWidth(8); return c; }
+  }
+}
+
+void *copy_bank(uint32_t bank, const void *src, uint64_t n) {
+  switch (bank) {
+    case 1: return aligned_copy<uint8_t>(src, n);
+    case 2: return aligned_copy<uint16_t>(src, n);
+    case 4: return aligned_copy<uint32_t>(src, n);
+    default: return aligned_copy<uint64_t>(src, n);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Composer::subsort_mergesort<T> on one whole array (src/composer.cpp:490-556).
+int ref_mergesort(uint32_t bank, void *keys, uint32_t *oids, uint64_t n) {
+  setting_t s;
+  compose_params_t p;
+  ChainComposer comp(s, 1, n, p, nullptr);
+  void *k = copy_bank(bank, keys, n);
+  uint32_t *o = aligned_copy<uint32_t>(oids, n);
+  switch (bank) {
+    case 2: { BAT_t<uint16_t> b{o, (uint16_t *)k, n}; comp.subsort<uint16_t>(&b, 0, n, true, 16); break; }
+    case 4: { BAT_t<uint32_t> b{o, (uint32_t *)k, n}; comp.subsort<uint32_t>(&b, 0, n, true, 32); break; }
+    case 8: { BAT_t<uint64_t> b{o, (uint64_t *)k, n}; comp.subsort<uint64_t>(&b, 0, n, true, 64); break; }
+    default: free(k); free(o); return -1;
+  }
+  std::memcpy(keys, k, n * bank);
+  std::memcpy(oids, o, n * 4);
+  free(k);
+  free(o);
+  return 0;
+}
+
+// The round loop of Composer::multiRoundSorting (src/composer.cpp:48-135)
+// with sortByGroup<T> (:1222-1408), keeping order and per-round groups.
+int ref_sort_rounds(uint32_t nrounds, const void *const *cols, const uint32_t *banks,
+                    const uint32_t *bitwidths, uint64_t n, uint32_t *order_out,
+                    uint32_t *groups_out) {
+  setting_t s;
+  compose_params_t p;
+  bool asc[64];
+  uint32_t bws[64];
+  Column *colv[64];
+  void *data[64];
+  if (nrounds == 0 || nrounds > 64) return -1;
+  for (uint32_t r = 0; r < nrounds; ++r) {
+    asc[r] = true;
+    bws[r] = bitwidths[r];
+    data[r] = copy_bank(banks[r], cols[r], n);
+    colv[r] = make_column(banks[r], data[r]);
+  }
+  p.column_asc_desc = asc;
+  p.bitwidth = bws;
+  ChainComposer comp(s, nrounds, n, p, colv);
+  surrogate_t *order = (surrogate_t *)malloc_aligned(n * sizeof(surrogate_t));
+  for (uint64_t i = 0; i < n; ++i) order[i] = (surrogate_t)i;
+  surrogate_t *outGroup = (surrogate_t *)malloc_aligned(n * sizeof(surrogate_t));
+  surrogate_t *inGroup = nullptr;
+  for (uint32_t r = 0; r < nrounds; ++r) {
+    bool last = (r == nrounds - 1);
+    switch (banks[r]) {
+      case 2: comp.sortByGroup<uint16_t>(outGroup, (uint16_t *)data[r], order, inGroup, true, bws[r], last); break;
+      case 4: comp.sortByGroup<uint32_t>(outGroup, (uint32_t *)data[r], order, inGroup, true, bws[r], last); break;
+      default: comp.sortByGroup<uint64_t>(outGroup, (uint64_t *)data[r], order, inGroup, true, bws[r], last); break;
+    }
+    if (!last && groups_out) std::memcpy(groups_out + (uint64_t)r * n, outGroup, n * sizeof(surrogate_t));
+    if (!inGroup) inGroup = (surrogate_t *)malloc_aligned(n * sizeof(surrogate_t));
+    swap<surrogate_t>(&inGroup, &outGroup);
+  }
+  std::memcpy(order_out, order, n * sizeof(surrogate_t));
+  free(order);
+  free(inGroup);
+  free(outGroup);
+  for (uint32_t r = 0; r < nrounds; ++r) { free(data[r]); delete colv[r]; }
+  return 0;
+}
+
+// ShiftLeftBatch + AssembleFromColumn over a construct-entry list
+// (src/stitch_composer.cpp:652-703, src/column.h:105-1059).
+int ref_stitch(uint64_t n, const void *const *src, const uint32_t *src_bank, const uint32_t *src_bw,
+               uint32_t nent, const uint32_t *ent_newcol, const uint32_t *ent_src,
+               const uint32_t *ent_start, const uint32_t *ent_end, uint32_t ndst, void *const *dst,
+               const uint32_t *dst_bank) {
+  Column *srcc[64];
+  void *srcd[64];
+  uint32_t nsrc = 0;
+  for (uint32_t e = 0; e < nent; ++e) nsrc = ent_src[e] + 1 > nsrc ? ent_src[e] + 1 : nsrc;
+  if (nsrc > 64 || ndst > 64) return -1;
+  for (uint32_t s = 0; s < nsrc; ++s) {
+    srcd[s] = copy_bank(src_bank[s], src[s], n);
+    srcc[s] = make_column(src_bank[s], srcd[s]);
+  }
+  for (uint32_t c = 0; c < ndst; ++c) {
+    void *d = copy_bank(dst_bank[c], dst[c], n);
+    std::memset(d, 0, n * dst_bank[c]);
+    Column *dc = make_column(dst_bank[c], d);
+    bool first = true;
+    for (uint32_t e = 0; e < nent; ++e) {
+      if (ent_newcol[e] != c) continue;
+      uint32_t w = ent_end[e] - ent_start[e] + 1;
+      if (!first) dc->ShiftLeftBatch(n, w);
+      first = false;
+      uint32_t s = ent_src[e];
+      switch (src_bank[s]) {
+        case 2: dc->AssembleFromColumn(reinterpret_cast<SpecialiedColumn<uint16_t> *>(srcc[s]), ent_start[e], ent_end[e], n, src_bw[s]); break;
+        case 4: dc->AssembleFromColumn(reinterpret_cast<SpecialiedColumn<uint32_t> *>(srcc[s]), ent_start[e], ent_end[e], n, src_bw[s]); break;
+        default: dc->AssembleFromColumn(reinterpret_cast<SpecialiedColumn<uint64_t> *>(srcc[s]), ent_start[e], ent_end[e], n, src_bw[s]); break;
+      }
+    }
+    std::memcpy(dst[c], d, n * dst_bank[c]);
+    free(d);
+    delete dc;
+  }
+  for (uint32_t s = 0; s < nsrc; ++s) { free(srcd[s]); delete srcc[s]; }
+  return 0;
+}
+
+}  // extern "C"

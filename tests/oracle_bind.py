@@ -1,0 +1,221 @@
+"""ctypes bindings to the CPU oracle (oracle/libfmcs_oracle.so) and, when it
+was built in this container, to the reference's own code (oracle/_ref/libref.so).
+
+Test infrastructure only: these are the checkers, never the thing measured.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "libfmcs_oracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+
+
+class GroupInfo(C.Structure):
+    _fields_ = [("ngroups", C.c_uint32), ("sum_group_size", C.c_uint32),
+                ("ngroups_incache", C.c_uint32), ("ngroups_outofcache", C.c_uint32),
+                ("ngroups_gtOne", C.c_uint32)]
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            import subprocess
+            subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+        _lib = C.CDLL(ORACLE_SO)
+        _lib.orc_bank_bytes.restype = C.c_uint32
+        _lib.orc_construct_entries.restype = C.c_int
+        _lib.orc_extract_group_info.restype = C.c_uint32
+        _lib.orc_sort_by_group.restype = C.c_uint32
+        _lib.orc_multi_round_sort.restype = C.c_uint32
+        _lib.orc_subgroup_range.restype = C.c_uint32
+        _lib.orc_multi_round_hash.restype = C.c_uint32
+        _lib.orc_hashmask.restype = C.c_uint64
+        _lib.orc_hashmask.argtypes = [C.c_uint64]
+        _lib.orc_mix_int.restype = C.c_uint32
+        _lib.orc_mix_int.argtypes = [C.c_uint32]
+        _lib.orc_mix_lng.restype = C.c_uint32
+        _lib.orc_mix_lng.argtypes = [C.c_uint64]
+        _lib.orc_split_exhaustive.restype = C.c_uint64
+    return _lib
+
+
+def ref():
+    """The reference library, or None where it was not built (e.g. GPU box)."""
+    global _ref
+    if _ref is None and os.path.exists(REF_SO):
+        _ref = C.CDLL(REF_SO)
+    return _ref
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _ptrs(arrs):
+    return (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+
+def _u32(xs):
+    a = np.ascontiguousarray(xs, dtype=np.uint32)
+    return a
+
+
+def bank_bytes(width, allow_u8=False):
+    return lib().orc_bank_bytes(C.c_uint32(width), C.c_int(int(allow_u8)))
+
+
+BANK_DTYPE = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}
+
+
+def construct_entries(bw, col_order, split):
+    bw = _u32(bw); col_order = _u32(col_order); split = _u32(split)
+    cap = 256
+    out = [np.zeros(cap, np.uint32) for _ in range(4)]
+    n = lib().orc_construct_entries(C.c_uint32(len(bw)), _p(bw), _p(col_order),
+                                    C.c_uint32(len(split)), _p(split),
+                                    *[_p(o) for o in out], C.c_uint32(cap))
+    if n < 0:
+        raise ValueError("invalid split")
+    return [tuple(int(o[i]) for o in out) for i in range(n)]  # (newcol, src, start, end)
+
+
+def stitch(cols, bws, entries, split, use_ref=False):
+    n = len(cols[0])
+    src = [np.ascontiguousarray(c) for c in cols]
+    src_bank = _u32([c.dtype.itemsize for c in src])
+    src_bw = _u32(bws)
+    dst_bank = _u32([bank_bytes(w) for w in split])
+    dst = [np.zeros(n, BANK_DTYPE[int(b)]) for b in dst_bank]
+    e = np.array(entries, dtype=np.uint32).reshape(-1, 4)
+    cols4 = [np.ascontiguousarray(e[:, i]) for i in range(4)]
+    fn = ref().ref_stitch if use_ref else lib().orc_stitch
+    fn(C.c_uint64(n), _ptrs(src), _p(src_bank), _p(src_bw), C.c_uint32(len(e)),
+       *[_p(c) for c in cols4], C.c_uint32(len(dst)), _ptrs(dst), _p(dst_bank))
+    return dst
+
+
+def gather(values, order):
+    out = np.empty_like(values)
+    lib().orc_gather(C.c_uint32(values.dtype.itemsize), C.c_uint64(len(values)), _p(values),
+                     _p(order), _p(out))
+    return out
+
+
+def extract_group_info(sorted_values, in_group=None):
+    n = len(sorted_values)
+    out = np.zeros(n, np.uint32)
+    ng = lib().orc_extract_group_info(C.c_uint32(sorted_values.dtype.itemsize), C.c_uint64(n),
+                                      _p(sorted_values),
+                                      None if in_group is None else _p(in_group), _p(out))
+    return out, ng
+
+
+def sort_by_group(column, order, in_group=None, last_round=False):
+    n = len(column)
+    order = order.copy()
+    out_group = np.zeros(n, np.uint32)
+    sorted_out = np.empty_like(column)
+    info = GroupInfo()
+    ng = lib().orc_sort_by_group(C.c_uint32(column.dtype.itemsize), C.c_uint64(n), _p(column),
+                                 _p(order), None if in_group is None else _p(in_group),
+                                 _p(out_group), C.c_int(int(last_round)), _p(sorted_out),
+                                 C.byref(info))
+    return order, (None if last_round else out_group), ng, sorted_out, info
+
+
+def multi_round_sort(cols):
+    n = len(cols[0])
+    cols = [np.ascontiguousarray(c) for c in cols]
+    banks = _u32([c.dtype.itemsize for c in cols])
+    order = np.zeros(n, np.uint32)
+    r = len(cols)
+    groups = np.zeros(max(r - 1, 1) * n, np.uint32)
+    info = GroupInfo()
+    ng = lib().orc_multi_round_sort(C.c_uint32(r), _ptrs(cols), _p(banks), C.c_uint64(n),
+                                    _p(order), _p(groups), C.byref(info))
+    return order, groups[: (r - 1) * n].reshape(r - 1, n), ng, info
+
+
+def multi_round_hash(cols):
+    n = len(cols[0])
+    cols = [np.ascontiguousarray(c) for c in cols]
+    banks = _u32([c.dtype.itemsize for c in cols])
+    order = np.zeros(n, np.uint32)
+    gid = np.zeros(n, np.uint32)
+    r = len(cols)
+    rg = np.zeros(max(r - 1, 1) * n, np.uint32)
+    total = lib().orc_multi_round_hash(C.c_uint32(r), _ptrs(cols), _p(banks), C.c_uint64(n),
+                                       _p(order), _p(gid), _p(rg))
+    return order, gid, rg[: (r - 1) * n].reshape(r - 1, n), total
+
+
+def hashmask(cnt):
+    return int(lib().orc_hashmask(C.c_uint64(cnt)))
+
+
+def mix_int(x):
+    return int(lib().orc_mix_int(C.c_uint32(x)))
+
+
+def mix_lng(x):
+    return int(lib().orc_mix_lng(C.c_uint64(x)))
+
+
+def split_exhaustive(S, rounds, maxv=64, cap=200000):
+    out = np.zeros(cap * rounds, np.uint32)
+    cnt = lib().orc_split_exhaustive(C.c_uint32(S), C.c_uint32(rounds), C.c_uint32(maxv), _p(out),
+                                     C.c_uint64(cap))
+    k = min(cnt, cap)
+    return int(cnt), out[: k * rounds].reshape(k, rounds)
+
+
+# ---- the reference itself (this container only) ----
+
+def ref_sort_rounds(cols, bitwidths):
+    n = len(cols[0])
+    cols = [np.ascontiguousarray(c) for c in cols]
+    banks = _u32([c.dtype.itemsize for c in cols])
+    bws = _u32(bitwidths)
+    order = np.zeros(n, np.uint32)
+    r = len(cols)
+    groups = np.zeros(max(r - 1, 1) * n, np.uint32)
+    rc = ref().ref_sort_rounds(C.c_uint32(r), _ptrs(cols), _p(banks), _p(bws), C.c_uint64(n),
+                               _p(order), _p(groups))
+    assert rc == 0
+    return order, groups[: (r - 1) * n].reshape(r - 1, n)
+
+
+def ref_mergesort(keys, oids):
+    keys = keys.copy(); oids = oids.copy()
+    rc = ref().ref_mergesort(C.c_uint32(keys.dtype.itemsize), _p(keys), _p(oids),
+                             C.c_uint64(len(keys)))
+    assert rc == 0
+    return keys, oids
+
+
+def canonical_order(full_key_sorted_runs, order):
+    """Sort oids ascending within runs of equal full keys: the reference's
+    tie order is arbitrary, the contract is the oid tie-break."""
+    out = order.copy()
+    keys = full_key_sorted_runs
+    n = len(order)
+    if n == 0:
+        return out
+    brk = np.flatnonzero(np.any(keys[:, 1:] != keys[:, :-1], axis=0)) + 1 if keys.ndim == 2 \
+        else np.flatnonzero(keys[1:] != keys[:-1]) + 1
+    starts = np.concatenate([[0], brk])
+    ends = np.concatenate([brk, [n]])
+    for s, e in zip(starts, ends):
+        if e - s > 1:
+            out[s:e] = np.sort(out[s:e])
+    return out

@@ -1,0 +1,211 @@
+"""GPU parity of the HIP kernels against the CPU oracle, through the C ABI.
+
+Bar: bit-exact.  Sorted orders are compared under the oid tie-break.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle_bind as ob
+from fastmulticolsort_amd import datagen
+from fastmulticolsort_amd.engine import (MODE_BATCHED, MODE_PLAIN, MODE_UNBINNED, construct_entries)
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+# ---- stitch ---------------------------------------------------------------
+
+STITCH_CASES = [
+    ([20, 36], [24, 32], 1000),             # bit_borrow shape (tests/misc_test.cpp:1263-1295)
+    ([12, 20, 30], [62], 4099),             # 62-bit stitch
+    ([17, 33], [10, 40], 777),              # borrow 7 bits forward
+    ([17, 33], [50], 4096),                 # single round
+    ([2, 1, 12, 27], [3, 12, 27], 5000),
+    ([2, 1, 12, 27], [16, 16, 10], 3333),
+    ([63, 20], [31, 21, 21, 10], 2048),     # stitchTest layout (tests/misc_test.cpp:741-773)
+    ([5, 7], [1, 11], 1),
+    ([64], [64], 100),
+]
+
+
+@pytest.mark.parametrize("bws,split,n", STITCH_CASES)
+def test_stitch_matches_oracle(ctx, bws, split, n):
+    log("stitch", bws, split, n)
+    cols = [datagen.uniform_bits(1000 + i, n, w).astype(datagen.bank_dtype(w)) for i, w in enumerate(bws)]
+    ents = construct_entries(bws, list(range(len(bws))), split)
+    assert ents == ob.construct_entries(bws, list(range(len(bws))), split)
+    exp = ob.stitch(cols, bws, ents, split)
+    dsts = [torch.zeros(n, dtype=torch.from_numpy(e[:0]).dtype, device="cuda") for e in exp]
+    ctx.stitch([dev(c) for c in cols], bws, ents, dsts)
+    for d, e in zip(dsts, exp):
+        assert np.array_equal(host(d), e)
+
+
+def test_stitch_bit_borrow_known_answer(ctx):
+    # tests/misc_test.cpp:1263-1295: 0x12345 (20b) | 0x678912345 (36b) -> 24+32
+    c1 = np.full(8, 0x12345, np.uint32)
+    c2 = np.full(8, 0x678912345, np.uint64)
+    ents = construct_entries([20, 36], [0, 1], [24, 32])
+    d1 = torch.zeros(8, dtype=torch.uint32, device="cuda")
+    d2 = torch.zeros(8, dtype=torch.uint32, device="cuda")
+    ctx.stitch([dev(c1), dev(c2)], [20, 36], ents, [d1, d2])
+    assert host(d1)[0] == 0x123456 and host(d2)[0] == 0x78912345
+
+
+# ---- gather / boundaries ---------------------------------------------------
+
+@pytest.mark.parametrize("dt", [np.uint16, np.uint32, np.uint64])
+@pytest.mark.parametrize("n", [1, 7, 4096, 100003])
+def test_gather(ctx, dt, n):
+    v = datagen.splitmix64(7, n).astype(dt)
+    order = datagen.permutation(3, n).astype(np.uint32)
+    out = ctx.gather(dev(v), dev(order))
+    assert np.array_equal(host(out), v[order])
+
+
+@pytest.mark.parametrize("dt", [np.uint16, np.uint32, np.uint64])
+@pytest.mark.parametrize("n", [1, 2, 4095, 4096, 4097, 300001])
+def test_extract_group_info(ctx, dt, n):
+    log("boundary", dt, n)
+    v = np.sort(datagen.uniform_below(11, n, max(2, n // 5)).astype(dt))
+    ing = np.sort(datagen.uniform_below(12, n, max(2, n // 50))).astype(np.uint32)
+    exp0, ng0 = ob.extract_group_info(v)
+    exp1, ng1 = ob.extract_group_info(v, ing)
+    off = torch.zeros(n + 1, dtype=torch.uint32, device="cuda")
+    g0, m0 = ctx.extract_group_info(dev(v), seg_offsets=off)
+    assert m0 == ng0 and np.array_equal(host(g0), exp0)
+    heads = np.flatnonzero(np.r_[1, np.diff(exp0.astype(np.int64)) != 0])
+    assert np.array_equal(host(off)[: ng0 + 1], np.r_[heads, n].astype(np.uint32))
+    g1, m1 = ctx.extract_group_info(dev(v), in_group=dev(ing))
+    assert m1 == ng1 and np.array_equal(host(g1), exp1)
+
+
+# ---- sort_pairs (LSD onesweep) ---------------------------------------------
+
+SORT_CASES = [
+    (np.uint16, 16, 1), (np.uint16, 16, 4096), (np.uint16, 12, 70001),
+    (np.uint32, 17, 4097), (np.uint32, 32, 262145), (np.uint32, 20, 1 << 20),
+    (np.uint64, 33, 100000), (np.uint64, 62, 300007), (np.uint64, 64, 65536), (np.uint64, 40, 12345),
+]
+
+
+@pytest.mark.parametrize("dt,bits,n", SORT_CASES)
+def test_sort_pairs(ctx, dt, bits, n):
+    log("sort_pairs", dt, bits, n)
+    k = datagen.uniform_bits(21, n, bits).astype(dt)
+    if n > 10:
+        k[: n // 3] = k[0]  # heavy ties: stability is visible
+    oids = np.arange(n, dtype=np.uint32)
+    kd, od = dev(k), dev(oids)
+    ctx.sort_pairs(kd, od, bits)
+    exp = np.argsort(k, kind="stable").astype(np.uint32)
+    assert np.array_equal(host(od), exp)
+    assert np.array_equal(host(kd), k[exp])
+
+
+def test_sort_pairs_matches_reference_mergesort(ctx):
+    if ob.ref() is None:
+        pytest.skip("reference library not built here")
+    n = 50000
+    k = datagen.uniform_bits(5, n, 30).astype(np.uint32)
+    o = np.arange(n, dtype=np.uint32)
+    rk, ro = ob.ref_mergesort(k, o)
+    kd, od = dev(k), dev(o)
+    ctx.sort_pairs(kd, od, 30)
+    assert np.array_equal(host(kd), rk)
+    assert np.array_equal(host(od), ob.canonical_order(rk, ro))
+
+
+# ---- segmented sort ----------------------------------------------------------
+
+def _segments(n, seed, kind):
+    if kind == "zipf":
+        sizes = datagen.zipf_counts(n, max(1, n // 40), 1.1)
+    elif kind == "tiny":
+        sizes = datagen.zipf_counts(n, max(1, n // 3), 0.3)
+    elif kind == "one":
+        sizes = np.array([n])
+    else:  # mixed: huge + many small + singles
+        rest = n - n // 2
+        small = datagen.zipf_counts(rest, max(1, rest // 20), 0.8)
+        sizes = np.r_[small[: len(small) // 2], n // 2, small[len(small) // 2:]]
+    off = np.r_[0, np.cumsum(sizes)].astype(np.uint32)
+    return off
+
+
+@pytest.mark.parametrize("mode", [MODE_BATCHED, MODE_UNBINNED, MODE_PLAIN])
+@pytest.mark.parametrize("dt,bits", [(np.uint64, 40), (np.uint32, 20), (np.uint16, 9), (np.uint64, 60)])
+@pytest.mark.parametrize("kind", ["zipf", "tiny", "one", "mixed"])
+def test_segmented_sort(ctx, mode, dt, bits, kind):
+    n = 200003 if mode != MODE_PLAIN else 40009
+    off = _segments(n, 3, kind)
+    nseg = len(off) - 1
+    if mode == MODE_UNBINNED and bits + int(np.ceil(np.log2(max(nseg, 2)))) > 64:
+        pytest.skip("composite exceeds 64 bits")
+    log("segsort", mode, dt, bits, kind, n, nseg)
+    k = datagen.uniform_bits(31, n, bits).astype(dt)
+    k[::7] = k[0]
+    oids = datagen.permutation(9, n).astype(np.uint32)
+    # within a segment incoming rows are in ascending oid order
+    for s in range(nseg):
+        oids[off[s]:off[s + 1]] = np.sort(oids[off[s]:off[s + 1]])
+    kd, od = dev(k), dev(oids)
+    info = ctx.segmented_sort(kd, od, dev(off), nseg, bits, mode=mode, want_info=True)
+    seg = np.repeat(np.arange(nseg), np.diff(off.astype(np.int64)))
+    idx = np.lexsort((oids, k, seg))
+    assert np.array_equal(host(od), oids[idx])
+    assert np.array_equal(host(kd), k[idx])
+    sz = np.diff(off.astype(np.int64))
+    assert info.ngroups == nseg and info.ngroups_gtOne == int((sz > 1).sum())
+
+
+# ---- sort_by_group / multi-round ----------------------------------------------
+
+@pytest.mark.parametrize("mode", [MODE_BATCHED, MODE_UNBINNED, MODE_PLAIN])
+def test_multi_round_sort_config1_shape(ctx, mode):
+    n = 1 << 16 if mode != MODE_PLAIN else 1 << 14
+    (c1, c2), bws = datagen.config1(n)
+    log("multi_round config1-shape", mode, n)
+    order, ng = ctx.multi_round_sort([dev(c1), dev(c2)], bws, mode=mode)
+    eo, eg, eng, _ = ob.multi_round_sort([c1, c2])
+    assert np.array_equal(host(order), eo)
+    assert ng == eng
+    assert np.array_equal(host(order), np.lexsort((c2, c1)).astype(np.uint32))
+
+
+def test_sort_by_group_rounds_match_oracle(ctx):
+    n = 1 << 17
+    (c1, c2), bws = datagen.config3(n, ngroups=1 << 10)
+    order = dev(np.arange(n, dtype=np.uint32))
+    o0 = np.arange(n, dtype=np.uint32)
+    g1, ng1, _ = ctx.sort_by_group(dev(c1), order, 20)
+    eo1, eg1, eng1, _, _ = ob.sort_by_group(c1, o0)
+    assert np.array_equal(host(order), eo1) and np.array_equal(host(g1), eg1) and ng1 == eng1
+    g2, ng2, info = ctx.sort_by_group(dev(c2), order, 40, in_group=g1, want_info=True)
+    eo2, eg2, eng2, _, einfo = ob.sort_by_group(c2, eo1, eg1)
+    assert np.array_equal(host(order), eo2) and np.array_equal(host(g2), eg2) and ng2 == eng2
+    assert (info.ngroups, info.ngroups_gtOne, info.sum_group_size, info.ngroups_incache) == \
+        (einfo.ngroups, einfo.ngroups_gtOne, einfo.sum_group_size, einfo.ngroups_incache)
+
+
+def test_multi_round_matches_reference(ctx):
+    if ob.ref() is None:
+        pytest.skip("reference library not built here")
+    n = 1 << 15
+    (c1, c2), bws = datagen.config1(n)
+    ro, rg = ob.ref_sort_rounds([c1, c2], bws)
+    order, _ = ctx.multi_round_sort([dev(c1), dev(c2)], bws)
+    keys = np.stack([c1[ro], c2[ro]])
+    assert np.array_equal(host(order), ob.canonical_order(keys, ro))
