@@ -1,0 +1,202 @@
+"""Benchmark: ORDER BY c1, c2 over 2^27 rows with 2^20 Zipf(1.1)-sized c1
+groups (c1: 20-bit code in a u32 bank, c2: 40-bit uniform in a u64 bank).
+
+One step = one full two-round sort on the GPU, inputs resident in HBM:
+identity order, round-1 LSD radix sort of c1 with the oid payload, group
+boundaries, oid gather of c2, round-2 batched segmented sort of c2 within
+the c1 groups (the reference's Composer::multiRoundSorting,
+src/composer.cpp:48-135).  The plain per-round form (one sort dispatch per
+group, src/composer.cpp:1285-1323) and the single unbinned segmented sort
+are timed beside it.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def check_sorted(c1, c2, order):
+    """Size-independent properties: order is a permutation and the rows are
+    in (c1, c2, oid) order."""
+    n = len(order)
+    seen = np.zeros(n, dtype=bool)
+    seen[order] = True
+    if not seen.all():
+        return False
+    a, b = c1[order].astype(np.int64), c2[order]
+    o = order.astype(np.int64)
+    da = np.diff(a)
+    if (da < 0).any():
+        return False
+    eq = da == 0
+    bb = b[1:] < b[:-1]
+    if (eq & bb).any():
+        return False
+    tie = eq & (b[1:] == b[:-1])
+    return not (tie & (o[1:] < o[:-1])).any()
+
+
+def cpu_baseline(c1, c2, sample_rows):
+    """The reference's own AVX2 round loop (oracle/_ref, built from the
+    reference sources) on the first `sample_rows` rows, one pinned thread.
+    Falls back to the C restatement (oracle) where the reference build is
+    absent."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+    s1 = np.ascontiguousarray(c1[:sample_rows])
+    s2 = np.ascontiguousarray(c2[:sample_rows])
+    try:
+        old = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {sorted(old)[0]})
+    except Exception:
+        old = None
+    kind = "reference" if ob.ref() is not None else "port"
+    times = []
+    try:
+        for it in range(3):
+            t0 = time.perf_counter()
+            if kind == "reference":
+                ob.ref_sort_rounds([s1, s2], [20, 40])
+            else:
+                ob.multi_round_sort([s1, s2])
+            times.append(time.perf_counter() - t0)
+    finally:
+        if old is not None:
+            os.sched_setaffinity(0, old)
+    t = float(np.median(times[1:]))
+    return {"value": sample_rows / t / 1e6, "unit": "Mrows/s", "cores": 1, "kind": kind,
+            "sample": f"first {sample_rows} rows of the same c1/c2 columns, 2 rounds (c1 u32, c2 u64), "
+                      f"1 warm-up + median of 2, one pinned thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rows-log2", type=int, default=27)
+    ap.add_argument("--groups-log2", type=int, default=20)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample-log2", type=int, default=23)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU path is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from fastmulticolsort_amd import datagen
+    from fastmulticolsort_amd.engine import Context, MODE_BATCHED, MODE_PLAIN, MODE_UNBINNED
+
+    n = 1 << args.rows_log2
+    ng = 1 << args.groups_log2
+    t0 = time.time()
+    (c1, c2), bws = datagen.config3(n, ngroups=ng)
+    log(f"generated {n} rows in {time.time() - t0:.1f}s")
+    d1 = torch.from_numpy(c1).cuda()
+    d2 = torch.from_numpy(c2).cuda()
+    order = torch.empty(n, dtype=torch.uint32, device="cuda")
+    ctx = Context(0)
+
+    def step(mode=MODE_BATCHED):
+        ctx.multi_round_sort([d1, d2], bws, order=order, mode=mode)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ok = check_sorted(c1, c2, order.cpu().numpy())
+    log(f"full-size property check (permutation, (c1,c2,oid) order): {ok}")
+    if not ok:
+        raise SystemExit("batched result failed the sortedness check")
+
+    ctx.set_profiling(True)
+    ctx.kernel_stats(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    stats = ctx.kernel_stats(reset=True)
+    ctx.set_profiling(False)
+    bd = ctx.last_breakdown()
+    ms = (t1 - t0) * 1e3 / args.steps
+    value = n / (ms / 1e3) / 1e6
+    log(f"batched: {ms:.3f} ms/step, {value:.1f} Mrows/s; breakdown {bd}")
+
+    # dominant kernel and its roofline (algorithmic bytes / avg duration)
+    dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
+    name, st = dom
+    avg_ms = st["ms"] / st["launches"]
+    bytes_per_launch = st["bytes"] / st["launches"]
+    achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
+    kernels = {k: {"launches": v["launches"], "ms_per_step": v["ms"] / args.steps,
+                   "GBs": (v["bytes"] / (v["ms"] / 1e3) / 1e9) if v["ms"] > 0 else None}
+               for k, v in stats.items()}
+    log("kernels: " + json.dumps(kernels))
+
+    # comparators
+    ctx.multi_round_sort([d1, d2], bws, order=order, mode=MODE_UNBINNED)
+    torch.cuda.synchronize()
+    tu0 = time.perf_counter()
+    ctx.multi_round_sort([d1, d2], bws, order=order, mode=MODE_UNBINNED)
+    torch.cuda.synchronize()
+    unb_ms = (time.perf_counter() - tu0) * 1e3
+    log(f"unbinned: {unb_ms:.3f} ms")
+    plain_ms = None
+    if not args.no_plain:
+        tp0 = time.perf_counter()
+        ctx.multi_round_sort([d1, d2], bws, order=order, mode=MODE_PLAIN)
+        torch.cuda.synchronize()
+        plain_ms = (time.perf_counter() - tp0) * 1e3
+        log(f"plain per-round: {plain_ms:.1f} ms")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(c1, c2, min(n, 1 << args.cpu_sample_log2))
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": "Mrows/s, batched segmented sort vs plain per-round launches; % of HBM peak",
+        "value": round(value, 2),
+        "unit": "Mrows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded SplitMix64 / fill_zipf counts)",
+        "config": {"workload": f"ORDER BY c1,c2: 2^{args.rows_log2} rows, 2^{args.groups_log2} Zipf(1.1) c1 groups, "
+                               f"c1 20-bit u32, c2 40-bit u64, batched segmented round 2",
+                   "rows": n, "groups": ng},
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "avg_launch_ms": round(avg_ms, 5), "bytes_per_launch": bytes_per_launch},
+        "cpu_baseline": cpu,
+        "comparators": {"plain_per_round_ms": plain_ms, "unbinned_ms": round(unb_ms, 3),
+                        "batched_vs_plain": (round(plain_ms / ms, 2) if plain_ms else None),
+                        "batched_vs_unbinned": round(unb_ms / ms, 3)},
+        "breakdown_ms": bd,
+        "kernels": kernels,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

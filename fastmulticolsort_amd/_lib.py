@@ -26,6 +26,7 @@ EXPORTS = [
     "fmcs_bank_bytes", "fmcs_construct_entries", "fmcs_stitch", "fmcs_gather",
     "fmcs_extract_group_info", "fmcs_sort_pairs", "fmcs_segmented_sort", "fmcs_sort_by_group",
     "fmcs_multi_round_sort", "fmcs_last_breakdown", "fmcs_hash_round", "fmcs_multi_round_hash",
+    "fmcs_ctx_set_profiling", "fmcs_kernel_stats", "fmcs_kernel_stats_reset",
 ]
 
 
@@ -84,6 +85,9 @@ def load():
         "fmcs_last_breakdown": (i32, [vp, P(C.c_double)]),
         "fmcs_hash_round": (i32, [vp, u32, u64, vp, vp, vp, vp, vp, vp, i32, P(u32)]),
         "fmcs_multi_round_hash": (i32, [vp, u32, P(vp), P(u32), u64, vp, vp, P(u32)]),
+        "fmcs_ctx_set_profiling": (i32, [vp, i32]),
+        "fmcs_kernel_stats": (i32, [vp, C.c_char_p, u64]),
+        "fmcs_kernel_stats_reset": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

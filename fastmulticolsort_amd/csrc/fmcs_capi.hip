@@ -4,6 +4,7 @@
 // the context's stream; temporary device memory comes from a per-context
 // workspace that grows on demand and is reused across calls.
 #include <algorithm>
+#include <map>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -41,6 +42,20 @@ struct fmcs_ctx {
   fmcs_tuning tune = {1024, 1024, 8};
   hipEvent_t ev[8] = {};
   double breakdown[4] = {0, 0, 0, 0};
+  // per-kernel event timing (off unless enabled)
+  struct Rec {
+    const char* name;
+    hipEvent_t e0, e1;
+    double bytes;
+  };
+  struct Agg {
+    u64 launches = 0;
+    double ms = 0, bytes = 0;
+  };
+  bool prof = false;
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, Agg> agg;
 };
 
 #define HIPCHK(x)                                                                          \
@@ -131,6 +146,55 @@ int upload(fmcs_ctx* ctx, u32* d, const u32* h, u32 count) {
 
 bool bank_ok(u32 b) { return b == 2 || b == 4 || b == 8; }
 
+hipEvent_t pool_get(fmcs_ctx* ctx) {
+  if (!ctx->pool.empty()) {
+    hipEvent_t e = ctx->pool.back();
+    ctx->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+int prof_flush(fmcs_ctx* ctx) {
+  if (ctx->recs.empty()) return FMCS_OK;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (auto& r : ctx->recs) {
+    float ms = 0;
+    if (r.e0 && r.e1 && hipEventElapsedTime(&ms, r.e0, r.e1) == hipSuccess) {
+      auto& a = ctx->agg[r.name];
+      a.launches++;
+      a.ms += ms;
+      a.bytes += r.bytes;
+    }
+    if (r.e0) ctx->pool.push_back(r.e0);
+    if (r.e1) ctx->pool.push_back(r.e1);
+  }
+  ctx->recs.clear();
+  return FMCS_OK;
+}
+
+// Brackets one launch with events on the context's stream when profiling.
+struct Prof {
+  fmcs_ctx* ctx;
+  int idx = -1;
+  Prof(fmcs_ctx* c, const char* name, double bytes) : ctx(c) {
+    if (!ctx->prof) return;
+    if (ctx->recs.size() > 8192) prof_flush(ctx);
+    fmcs_ctx::Rec r{name, pool_get(ctx), pool_get(ctx), bytes};
+    if (r.e0) (void)hipEventRecord(r.e0, ctx->stream);
+    ctx->recs.push_back(r);
+    idx = (int)ctx->recs.size() - 1;
+  }
+  void set_bytes(double b) {
+    if (idx >= 0) ctx->recs[idx].bytes = b;
+  }
+  ~Prof() {
+    if (idx >= 0 && ctx->recs[idx].e1) (void)hipEventRecord(ctx->recs[idx].e1, ctx->stream);
+  }
+};
+
 // exclusive scan of n u32 values -> out; *total_h (host) if non-null
 int scan_excl(fmcs_ctx* ctx, const u32* in, u32* out, u64 n, u32* total_d) {
   if (n == 0) return FMCS_OK;
@@ -171,6 +235,7 @@ int radix_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, 
     // the last pass's digit is masked to last_bits inside k_hist_all via a
     // key mask applied here: keys are assumed < 2^keybits, and the same mask
     // is used by the passes, so histogram and scatter always agree.
+    Prof pf(ctx, "k_hist_all", (double)n * sizeof(K));
     hipLaunchKernelGGL((k_hist_all<K>), dim3(blocks), dim3(256), 0, ctx->stream, kin, n, npasses, (u32)RADIX_BITS, hist);
     TRY(launch_check(ctx, "k_hist_all"));
   }
@@ -221,6 +286,7 @@ int radix_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, 
       dk = (i & 1) ? t1k : t0k;
       dv = (i & 1) ? t1v : t0v;
     }
+    Prof pf(ctx, "k_onesweep", 2.0 * n * (sizeof(K) + 4));
     hipLaunchKernelGGL((k_onesweep<K, ONESWEEP_ITEMS>), dim3((u32)ntiles), dim3(256), 0, ctx->stream, sk, sv, dk, dv,
                        ctx->small_d + 16, ctx->small_d + 17, ctx->small_d + 18, 1u, dbase + (u64)p * RADIX,
                        status + (u64)p * ntiles * RADIX, ctx->small_d + 24 + p, p * RADIX_BITS, (u32)RADIX_BITS);
@@ -247,15 +313,19 @@ int boundary(fmcs_ctx* ctx, const K* v, const u32* in_group, u64 n, u32* out_gro
   TRY(getbuf_t(ctx, WS_STATUS, ntiles, &status));
   TRY(fill(ctx, status, ntiles, 0));
   TRY(fill(ctx, ctx->small_d + 40, 2, 0));
-  hipLaunchKernelGGL((k_boundary<K, BOUNDARY_ITEMS>), dim3((u32)ntiles), dim3(256), 0, ctx->stream, v, in_group, n,
-                     out_group, seg_off, status, ctx->small_d + 40, ctx->small_d + 41);
-  TRY(launch_check(ctx, "k_boundary"));
+  {
+    Prof pf(ctx, "k_boundary", (double)n * (sizeof(K) + (in_group ? 4 : 0) + (out_group ? 4 : 0)));
+    hipLaunchKernelGGL((k_boundary<K, BOUNDARY_ITEMS>), dim3((u32)ntiles), dim3(256), 0, ctx->stream, v, in_group, n,
+                       out_group, seg_off, status, ctx->small_d + 40, ctx->small_d + 41);
+    TRY(launch_check(ctx, "k_boundary"));
+  }
   return readback(ctx, ctx->small_d + 41, 1, ng);
 }
 
 template <class K>
-int tile_sort_launch(fmcs_ctx* ctx, TileSortArgs<K> a, u32 grid) {
+int tile_sort_launch(fmcs_ctx* ctx, TileSortArgs<K> a, u32 grid, double rows) {
   if (grid == 0) return FMCS_OK;
+  Prof pf(ctx, "k_tile_sort", 2.0 * rows * (sizeof(K) + 4));
   hipLaunchKernelGGL((k_tile_sort<K, TILESORT_E>), dim3(grid), dim3(256), 0, ctx->stream, a);
   return launch_check(ctx, "k_tile_sort");
 }
@@ -290,6 +360,11 @@ int seg_sort_batched(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, cons
   u32 cur_n = nseg;
   u32 kb = keybits;
   u32 level = 0;
+  // parents of the current bucket list (level >= 1): the previous level's
+  // large segments and their tile bases
+  const u32* plo = nullptr;
+  const u32* ptb = nullptr;
+  u32 pntiles = 0;
   for (;;) {
     u32 sm = small_max;
     u32 tkb = kb;
@@ -300,7 +375,8 @@ int seg_sort_batched(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, cons
       tkb = 0;
     }
     const u32 Wl = std::min<u32>(W, sm + 1);
-    const u64 nw = (n + Wl - 1) / Wl;
+    const u32 wpt = (ONESWEEP_TILE + Wl - 1) / Wl;
+    const u64 nw = plo ? (u64)pntiles * wpt : (n + Wl - 1) / Wl;
     u32 *tlo, *thi, *ts0, *ts1;
     TRY(getbuf_t(ctx, WS_TLO, nw, &tlo));
     TRY(getbuf_t(ctx, WS_THI, nw, &thi));
@@ -308,25 +384,30 @@ int seg_sort_batched(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, cons
     TRY(getbuf_t(ctx, WS_TS1, nw, &ts1));
     TRY(fill(ctx, tlo, nw, 0));
     TRY(fill(ctx, thi, nw, 0));
-    hipLaunchKernelGGL(k_windows, dim3((cur_n + 255) / 256), dim3(256), 0, ctx->stream, cst, csz, cur_n, Wl, sm, tlo,
-                       thi, ts0, ts1);
+    hipLaunchKernelGGL(k_windows, dim3((cur_n + 255) / 256), dim3(256), 0, ctx->stream, cst, csz, cur_n, Wl, sm, plo,
+                       ptb, wpt, tlo, thi, ts0, ts1);
     TRY(launch_check(ctx, "k_windows"));
+    // large segments of this level, and the row counts of both bins
+    u32 *flag, *pos;
+    TRY(getbuf_t(ctx, WS_FLAG, (u64)cur_n + 1, &flag));
+    TRY(getbuf_t(ctx, WS_POS, (u64)cur_n + 1, &pos));
+    TRY(fill(ctx, ctx->small_d + 52, 2, 0));
+    hipLaunchKernelGGL(k_large_flags, dim3((cur_n + 255) / 256), dim3(256), 0, ctx->stream, csz, cur_n, sm, flag,
+                       ctx->small_d + 52);
+    TRY(launch_check(ctx, "k_large_flags"));
+    TRY(scan_excl(ctx, flag, pos, cur_n, ctx->small_d + 48));
+    u32 cnts[6];
+    TRY(readback(ctx, ctx->small_d + 48, 6, cnts));
+    const u32 nlarge = cnts[0];
+    const u64 tile_rows_est = cnts[4];
+    const u64 large_rows = cnts[5];
     TileSortArgs<K> a;
     std::memset(&a, 0, sizeof(a));
     a.kin[0] = keys; a.vin[0] = vals; a.kin[1] = kb_; a.vin[1] = vb_;
     a.kout = keys; a.vout = vals;
     a.tile_lo = tlo; a.tile_hi = thi; a.tile_s0 = ts0; a.tile_s1 = ts1; a.seg_start = cst;
     a.src_sel = level & 1; a.kb = tkb; a.pad_seg = T; a.direct = 0;
-    TRY(tile_sort_launch<K>(ctx, a, (u32)nw));
-    // large segments of this level
-    u32 *flag, *pos;
-    TRY(getbuf_t(ctx, WS_FLAG, (u64)cur_n + 1, &flag));
-    TRY(getbuf_t(ctx, WS_POS, (u64)cur_n + 1, &pos));
-    hipLaunchKernelGGL(k_large_flags, dim3((cur_n + 255) / 256), dim3(256), 0, ctx->stream, csz, cur_n, sm, flag);
-    TRY(launch_check(ctx, "k_large_flags"));
-    TRY(scan_excl(ctx, flag, pos, cur_n, ctx->small_d + 48));
-    u32 nlarge;
-    TRY(readback(ctx, ctx->small_d + 48, 1, &nlarge));
+    TRY(tile_sort_launch<K>(ctx, a, (u32)nw, (double)tile_rows_est));
     if (nlarge == 0) break;
     u32 *llo, *lhi, *ltiles, *tbase;
     TRY(getbuf_t(ctx, WS_LLO, nlarge, &llo));
@@ -359,9 +440,12 @@ int seg_sort_batched(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, cons
     TRY(getbuf_t(ctx, WS_DBASE, (u64)nlarge * RADIX, &dbase));
     TRY(getbuf_t(ctx, WS_STATUS, (u64)ntiles * RADIX, &status));
     TRY(fill(ctx, hist, (u64)nlarge * RADIX, 0));
-    hipLaunchKernelGGL((k_seg_hist<K, ONESWEEP_ITEMS>), dim3(ntiles), dim3(256), 0, ctx->stream, skey, llo, lhi, tbase,
-                       nlarge, shift, D, hist);
-    TRY(launch_check(ctx, "k_seg_hist"));
+    {
+      Prof pf(ctx, "k_seg_hist", (double)large_rows * sizeof(K));
+      hipLaunchKernelGGL((k_seg_hist<K, ONESWEEP_ITEMS>), dim3(ntiles), dim3(256), 0, ctx->stream, skey, llo, lhi,
+                         tbase, nlarge, shift, D, hist);
+      TRY(launch_check(ctx, "k_seg_hist"));
+    }
     const int nxt = cur ^ 1;
     u32 *nst, *nsz;
     TRY(getbuf_t(ctx, lst[nxt], (u64)nlarge * RADIX + 1, &nst));
@@ -370,13 +454,19 @@ int seg_sort_batched(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, cons
     TRY(launch_check(ctx, "k_digit_base"));
     TRY(fill(ctx, status, (u64)ntiles * RADIX, 0));
     TRY(fill(ctx, ctx->small_d + 50, 1, 0));
-    hipLaunchKernelGGL((k_onesweep<K, ONESWEEP_ITEMS>), dim3(ntiles), dim3(256), 0, ctx->stream, skey, sval, dkey, dval,
-                       llo, lhi, tbase, nlarge, dbase, status, ctx->small_d + 50, shift, D);
-    TRY(launch_check(ctx, "k_onesweep(msd)"));
+    {
+      Prof pf(ctx, "k_onesweep_msd", 2.0 * large_rows * (sizeof(K) + 4));
+      hipLaunchKernelGGL((k_onesweep<K, ONESWEEP_ITEMS>), dim3(ntiles), dim3(256), 0, ctx->stream, skey, sval, dkey,
+                         dval, llo, lhi, tbase, nlarge, dbase, status, ctx->small_d + 50, shift, D);
+      TRY(launch_check(ctx, "k_onesweep(msd)"));
+    }
     cur = nxt;
     cst = nst;
     csz = nsz;
     cur_n = nlarge * RADIX;
+    plo = llo;
+    ptb = tbase;
+    pntiles = ntiles;
     kb -= D;
     ++level;
   }
@@ -399,7 +489,7 @@ int seg_sort_plain(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const 
       std::memset(&a, 0, sizeof(a));
       a.kin[0] = keys; a.vin[0] = vals; a.kout = keys; a.vout = vals;
       a.kb = keybits; a.pad_seg = TILESORT_T; a.direct = 1; a.d_lo = lo; a.d_hi = hi;
-      TRY(tile_sort_launch<K>(ctx, a, 1));
+      TRY(tile_sort_launch<K>(ctx, a, 1, (double)sz));
     } else {
       TRY(radix_sort<K>(ctx, keys + lo, vals + lo, keys + lo, vals + lo, sz, keybits));
     }
@@ -471,6 +561,7 @@ int sort_by_group_t(fmcs_ctx* ctx, u32 keybits, u64 n, const K* column, u32* ord
     u32 blocks = (u32)((n + 1023) / 1024);
     if (blocks) {
       if (((uintptr_t)order & 15) != 0) return fail(ctx, FMCS_E_ARG, "order must be 16-byte aligned");
+      Prof pf(ctx, "k_gather", (double)n * (4 + 2 * sizeof(K)));
       hipLaunchKernelGGL((k_gather<K>), dim3(blocks), dim3(256), 0, ctx->stream, column, order, ka, n);
       TRY(launch_check(ctx, "k_gather"));
     }
@@ -539,6 +630,8 @@ int fmcs_ctx_destroy(fmcs_ctx* ctx) {
   (void)hipHostFree(ctx->host_pin);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
+  prof_flush(ctx);
+  for (auto& e : ctx->pool) (void)hipEventDestroy(e);
   delete ctx;
   return FMCS_OK;
 }
@@ -803,6 +896,39 @@ int fmcs_multi_round_sort(fmcs_ctx* ctx, uint32_t nrounds, const void* const* co
   ctx->breakdown[2] = ts;
   ctx->breakdown[3] = tx;
   if (ngroups) *ngroups = groups_d ? ng : nseg;
+  return FMCS_OK;
+}
+
+int fmcs_ctx_set_profiling(fmcs_ctx* ctx, int on) {
+  if (!ctx) return FMCS_E_ARG;
+  TRY(prof_flush(ctx));
+  ctx->prof = on != 0;
+  return FMCS_OK;
+}
+
+int fmcs_kernel_stats(fmcs_ctx* ctx, char* buf, uint64_t cap) {
+  if (!ctx || !buf || cap == 0) return FMCS_E_ARG;
+  TRY(prof_flush(ctx));
+  std::string js = "{";
+  bool first = true;
+  for (auto& kv : ctx->agg) {
+    char line[256];
+    std::snprintf(line, sizeof(line), "%s\"%s\": {\"launches\": %llu, \"ms\": %.6f, \"bytes\": %.0f}",
+                  first ? "" : ", ", kv.first.c_str(), (unsigned long long)kv.second.launches, kv.second.ms,
+                  kv.second.bytes);
+    js += line;
+    first = false;
+  }
+  js += "}";
+  if (js.size() + 1 > cap) return fail(ctx, FMCS_E_ARG, "stats buffer too small");
+  std::memcpy(buf, js.c_str(), js.size() + 1);
+  return FMCS_OK;
+}
+
+int fmcs_kernel_stats_reset(fmcs_ctx* ctx) {
+  if (!ctx) return FMCS_E_ARG;
+  TRY(prof_flush(ctx));
+  ctx->agg.clear();
   return FMCS_OK;
 }
 

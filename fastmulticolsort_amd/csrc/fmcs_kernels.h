@@ -65,9 +65,9 @@ __global__ void k_onesweep(const K*, const u32*, K*, u32*, const u32*, const u32
                            u32*, u32*, u32, u32);
 template <class K, int E>
 __global__ void k_tile_sort(TileSortArgs<K>);
-__global__ void k_windows(const u32*, const u32*, u32, u32, u32, u32*, u32*, u32*, u32*);
+__global__ void k_windows(const u32*, const u32*, u32, u32, u32, const u32*, const u32*, u32, u32*, u32*, u32*, u32*);
 __global__ void k_offsets_to_list(const u32*, u32, u32*, u32*);
-__global__ void k_large_flags(const u32*, u32, u32, u32*);
+__global__ void k_large_flags(const u32*, u32, u32, u32*, u32*);
 __global__ void k_large_compact(const u32*, const u32*, u32, u32, const u32*, u32, u32*, u32*, u32*);
 template <class K>
 __global__ void k_copy_segs(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, u32, u32);

@@ -557,18 +557,28 @@ __global__ __launch_bounds__(256) void k_tile_sort(TileSortArgs<K> a) {
 
 // Window packing of small segments (size <= small_max) of a sorted,
 // disjoint segment list: segment i belongs to window start[i] / W.  A large
-// segment (> small_max >= W-1 rows) that starts in a window runs past its
-// end, so the small segments of a window are contiguous and precede it.
+// segment (>= W rows) that starts in a window runs past its end, so the
+// small segments of a window are contiguous and precede it.  Lists of MSD
+// buckets (RADIX per parent segment, parents not adjacent in general) use
+// windows relative to their parent: parent_wbase[p] * wpt + (start - lo[p]) / W,
+// so a window never spans the gap between two parents.
+__device__ __forceinline__ u32 window_of(const u32* st, u32 i, u32 W, const u32* plo, const u32* pwb, u32 wpt) {
+  if (!plo) return st[i] / W;
+  u32 p = i >> RADIX_BITS;
+  return pwb[p] * wpt + (st[i] - plo[p]) / W;
+}
+
 __global__ void k_windows(const u32* __restrict__ st, const u32* __restrict__ sz, u32 nseg, u32 W, u32 small_max,
+                          const u32* __restrict__ plo, const u32* __restrict__ pwb, u32 wpt,
                           u32* __restrict__ tlo, u32* __restrict__ thi, u32* __restrict__ ts0,
                           u32* __restrict__ ts1) {
   u32 i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nseg) return;
   u32 size = sz[i];
   if (size > small_max) return;
-  u32 w = st[i] / W;
-  bool first = (i == 0) || (st[i - 1] / W != w);
-  bool last = (i + 1 == nseg) || (st[i + 1] / W != w) || (sz[i + 1] > small_max);
+  u32 w = window_of(st, i, W, plo, pwb, wpt);
+  bool first = (i == 0) || (window_of(st, i - 1, W, plo, pwb, wpt) != w);
+  bool last = (i + 1 == nseg) || (window_of(st, i + 1, W, plo, pwb, wpt) != w) || (sz[i + 1] > small_max);
   if (first) { tlo[w] = st[i]; ts0[w] = i; }
   if (last) { thi[w] = st[i] + size; ts1[w] = i; }
 }
@@ -584,10 +594,22 @@ __global__ void k_offsets_to_list(const u32* __restrict__ off, u32 nseg, u32* __
 
 // Large segments: flag, then (after a scan) compact to (lo, hi) and count
 // their tiles.
-__global__ void k_large_flags(const u32* __restrict__ sz, u32 nseg, u32 small_max, u32* __restrict__ flag) {
+__global__ void k_large_flags(const u32* __restrict__ sz, u32 nseg, u32 small_max, u32* __restrict__ flag,
+                              u32* __restrict__ rows) {
   u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nseg) return;
-  flag[i] = sz[i] > small_max ? 1u : 0u;
+  u32 s = i < nseg ? sz[i] : 0u;
+  bool large = s > small_max;
+  if (i < nseg) flag[i] = large ? 1u : 0u;
+  u32 a = large ? 0u : s, b = large ? s : 0u;
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) {
+    a += __shfl_down(a, o);
+    b += __shfl_down(b, o);
+  }
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    if (a) atomicAdd(rows, a);
+    if (b) atomicAdd(rows + 1, b);
+  }
 }
 
 __global__ void k_large_compact(const u32* __restrict__ st, const u32* __restrict__ sz, u32 nseg, u32 small_max,

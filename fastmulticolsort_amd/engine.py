@@ -160,6 +160,17 @@ class Context:
                                                  C.byref(ng)))
         return order, ng.value
 
+    def set_profiling(self, on):
+        self._chk(self.lib.fmcs_ctx_set_profiling(self.h, int(bool(on))))
+
+    def kernel_stats(self, reset=False):
+        import json
+        buf = C.create_string_buffer(1 << 16)
+        self._chk(self.lib.fmcs_kernel_stats(self.h, buf, len(buf)))
+        if reset:
+            self._chk(self.lib.fmcs_kernel_stats_reset(self.h))
+        return json.loads(buf.value.decode())
+
     def last_breakdown(self):
         a = (C.c_double * 4)()
         self._chk(self.lib.fmcs_last_breakdown(self.h, a))

@@ -153,6 +153,15 @@ int fmcs_multi_round_sort(fmcs_ctx* ctx, uint32_t nrounds, const void* const* co
  * (src/types.h:206-221). */
 int fmcs_last_breakdown(fmcs_ctx* ctx, double* ms4);
 
+/* Per-kernel timing with HIP events on the context's stream (no
+ * counterpart in the reference, whose HybridTimer brackets whole phases,
+ * src/hybrid_timer.h:16-57).  When on, every major kernel launch is
+ * bracketed by two events; fmcs_kernel_stats writes a JSON object
+ * {kernel: {"launches": L, "ms": total, "bytes": algorithmic bytes}}. */
+int fmcs_ctx_set_profiling(fmcs_ctx* ctx, int on);
+int fmcs_kernel_stats(fmcs_ctx* ctx, char* buf, uint64_t cap);
+int fmcs_kernel_stats_reset(fmcs_ctx* ctx);
+
 /* ---- hash grouping (src/composer.cpp:137-227, :814-976;
  *      src/hashing.h:289-385; src/partition.h:44-161) ------------------ */
 

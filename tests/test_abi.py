@@ -1,0 +1,48 @@
+"""The C-ABI library loads and exports every entry point of include/fmcs.h;
+host-only entry points agree with the oracle.  CPU only (no kernel runs)."""
+import os
+import re
+
+import pytest
+
+import oracle_bind as ob
+from fastmulticolsort_amd import _lib
+from fastmulticolsort_amd.engine import bank_bytes, construct_entries
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "fmcs.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(fmcs_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = header_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert sorted(_lib.EXPORTS) == names
+
+
+def test_bank_bytes_matches_oracle():
+    for w in range(1, 65):
+        assert bank_bytes(w) == ob.bank_bytes(w)
+        assert bank_bytes(w, True) == ob.bank_bytes(w, True)
+
+
+@pytest.mark.parametrize("bws,order,split", [
+    ([17, 33], [0, 1], [20, 30]), ([12, 20, 30], [0, 1, 2], [62]), ([2, 1, 12, 27], [3, 1, 0, 2], [10, 10, 22]),
+    ([63, 20], [0, 1], [31, 21, 21, 10]), ([40, 40, 40], [2, 0, 1], [64, 56]),
+])
+def test_construct_entries_matches_oracle(bws, order, split):
+    assert construct_entries(bws, order, split) == ob.construct_entries(bws, order, split)
+
+
+def test_construct_entries_rejects_bad_split():
+    with pytest.raises(Exception):
+        construct_entries([17, 33], [0, 1], [20, 31])
+    with pytest.raises(Exception):
+        construct_entries([17, 33], [0, 0], [20, 30])

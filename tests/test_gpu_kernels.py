@@ -137,6 +137,16 @@ def _segments(n, seed, kind):
         sizes = datagen.zipf_counts(n, max(1, n // 3), 0.3)
     elif kind == "one":
         sizes = np.array([n])
+    elif kind == "alt":
+        # large segments separated by runs of small ones: MSD bucket lists
+        # have gaps between parents that windows must not span
+        pat = np.array([3000, 5, 9, 1, 0, 2500, 40, 1300, 7, 1100, 1025, 2, 60000])
+        reps = n // int(pat.sum()) + 1
+        sizes = np.tile(pat, reps)
+        cs = np.cumsum(sizes)
+        k = int(np.searchsorted(cs, n))
+        sizes = sizes[: k + 1].copy()
+        sizes[-1] -= int(cs[k] - n)
     else:  # mixed: huge + many small + singles
         rest = n - n // 2
         small = datagen.zipf_counts(rest, max(1, rest // 20), 0.8)
@@ -147,7 +157,7 @@ def _segments(n, seed, kind):
 
 @pytest.mark.parametrize("mode", [MODE_BATCHED, MODE_UNBINNED, MODE_PLAIN])
 @pytest.mark.parametrize("dt,bits", [(np.uint64, 40), (np.uint32, 20), (np.uint16, 9), (np.uint64, 60)])
-@pytest.mark.parametrize("kind", ["zipf", "tiny", "one", "mixed"])
+@pytest.mark.parametrize("kind", ["zipf", "tiny", "one", "mixed", "alt"])
 def test_segmented_sort(ctx, mode, dt, bits, kind):
     n = 200003 if mode != MODE_PLAIN else 40009
     off = _segments(n, 3, kind)
