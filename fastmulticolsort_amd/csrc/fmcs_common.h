@@ -30,17 +30,28 @@ __device__ __forceinline__ void status_store(u32* p, u32 v) {
 // prefix is found.  Every predecessor took its tile id from an atomic
 // counter before this tile did, so it is resident and publishes its
 // aggregate without waiting on anything: the spin always ends.
+// Status words are fetched LB_BATCH predecessors at a time so their
+// cross-XCD load latencies overlap.
+constexpr u32 LB_BATCH = 8;
 __device__ __forceinline__ u32 lookback(const u32* status, u32 t, u32 stop, u32 stride) {
   u32 excl = 0;
   u32 p = t;
   while (p > stop) {
-    --p;
-    u32 s;
-    do {
-      s = status_load(status + (u64)p * stride);
-    } while ((s & ~ST_VAL) == 0);
-    excl += s & ST_VAL;
-    if (s & ST_INC) break;
+    const u32 m = min(LB_BATCH, p - stop);
+    u32 s[LB_BATCH];
+#pragma unroll
+    for (u32 j = 0; j < LB_BATCH; ++j) s[j] = (j < m) ? status_load(status + (u64)(p - 1 - j) * stride) : 0u;
+    bool done = false;
+#pragma unroll
+    for (u32 j = 0; j < LB_BATCH; ++j) {
+      if (done || j >= m) continue;
+      u32 v = s[j];
+      while ((v & ~ST_VAL) == 0) v = status_load(status + (u64)(p - 1 - j) * stride);
+      excl += v & ST_VAL;
+      if (v & ST_INC) done = true;
+    }
+    if (done) break;
+    p -= m;
   }
   return excl;
 }

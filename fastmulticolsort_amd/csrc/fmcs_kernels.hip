@@ -388,57 +388,84 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
 }
 
 // ---------------------------------------------------------------------------
-// LDS tile sort for packed small segments: a block merge sort over the
-// composite (local segment, masked key, oid).  Each of the 256 threads sorts
-// E elements in registers, then log2(256) merge-path levels run through LDS.
-// A tile is a window of rows holding whole segments; the segment index is
-// recomputed from the segment starts, so no per-row segment ids are read.
-// The top (already partitioned) key bits of every segment are kept in LDS
-// and restored on the way out.
+// LDS tile sort for packed small segments.  A tile is a window of rows that
+// holds whole segments; the segment index is recomputed from the segment
+// starts (no per-row segment ids are read).  The tile is sorted by the
+// composite (local segment << kb) | (key & mask(kb)) with an LSD radix sort
+// in LDS: each pass ranks keys stably with 64-lane ballot matching into
+// per-wave histograms and scatters them through LDS.  Stability keeps equal
+// keys in their incoming (ascending-oid) order, so no oid compare is needed.
+// Passes cover only the bits that vary inside the tile.  The top (already
+// partitioned) key bits of every segment are kept in LDS and restored on the
+// way out.
 // ---------------------------------------------------------------------------
-struct Elt {
-  u64 h;
-  u32 o;
-};
-__device__ __forceinline__ bool elt_lt(const Elt& a, const Elt& b) { return a.h < b.h || (a.h == b.h && a.o < b.o); }
-__device__ __forceinline__ void cas(Elt& a, Elt& b) {
-  if (elt_lt(b, a)) {
-    Elt t = a;
-    a = b;
-    b = t;
+
+// Stable ranking of E items per lane (item j of lane l is the j*64+l-th
+// item of the wave) into the wave's LDS digit counters.
+template <int E>
+__device__ __forceinline__ void wave_rank(const u32 (&dig)[E], const bool (&ok)[E], u32 dbits, u32* whist,
+                                          u32 (&rank)[E]) {
+  const u32 lane = __lane_id();
+  const u64 lt = lanemask_lt();
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    u32 d = dig[j];
+    u64 peers = __ballot(ok[j]);
+    for (u32 b = 0; b < dbits; ++b) {
+      u64 bb = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    if (!ok[j]) peers = 0;
+    u32 leader = ok[j] ? (u32)(__ffsll((long long)peers) - 1) : lane;
+    u32 old = 0;
+    if (ok[j] && lane == leader) {
+      old = whist[d];
+      whist[d] = old + (u32)__popcll(peers);
+    }
+    old = __shfl(old, (int)leader);
+    rank[j] = old + (u32)__popcll(peers & lt);
   }
 }
 
-template <int E>
-__device__ __forceinline__ void reg_sort(Elt (&x)[E]) {
-  // bitonic network over E (power of two) registers
+template <int NT>
+__device__ __forceinline__ void block_or_and(u64 v_or, u64 v_and, u64* s_red, u64* out_or, u64* out_and) {
+  constexpr int NW = NT / WAVE;
 #pragma unroll
-  for (int k = 2; k <= E; k <<= 1) {
-#pragma unroll
-    for (int j = k >> 1; j > 0; j >>= 1) {
-#pragma unroll
-      for (int i = 0; i < E; ++i) {
-        int l = i ^ j;
-        if (l > i) {
-          if ((i & k) == 0) cas(x[i], x[l]);
-          else cas(x[l], x[i]);
-        }
-      }
-    }
+  for (int o = WAVE / 2; o > 0; o >>= 1) {
+    v_or |= __shfl_xor(v_or, o);
+    v_and &= __shfl_xor(v_and, o);
   }
+  u32 wv = threadIdx.x / WAVE;
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    s_red[wv] = v_or;
+    s_red[NW + wv] = v_and;
+  }
+  __syncthreads();
+  u64 ro = 0, ra = ~0ull;
+  for (int w = 0; w < NW; ++w) {
+    ro |= s_red[w];
+    ra &= s_red[NW + w];
+  }
+  *out_or = ro;
+  *out_and = ra;
+  __syncthreads();
 }
 
 template <class K, int E>
 __global__ __launch_bounds__(256) void k_tile_sort(TileSortArgs<K> a) {
-  constexpr int NT = 256, T = NT * E;
-  __shared__ u64 s_h[T];
+  constexpr int NT = 256, NW = NT / WAVE, T = NT * E, WE = WAVE * E;
+  __shared__ u64 s_c[T];
   __shared__ u32 s_o[T];
   __shared__ u64 s_top[T];
-  __shared__ u32 s_tmp[NT / WAVE + 1];
-  u32 tid = threadIdx.x;
+  __shared__ uint8_t s_f[T];
+  __shared__ u32 s_whist[NW][RADIX];
+  __shared__ u32 s_dig[RADIX];
+  __shared__ u32 s_tmp[NW + 1];
+  __shared__ u64 s_red[2 * NW];
+  const u32 tid = threadIdx.x, wv = tid / WAVE, lane = tid & (WAVE - 1);
   u32 lo, hi, s0, s1;
   if (a.direct) {
-    lo = a.d_lo; hi = a.d_hi; s0 = 0; s1 = 0;
+    lo = a.d_lo; hi = a.d_hi; s0 = 1; s1 = 0;
   } else {
     u32 w = blockIdx.x;
     lo = a.tile_lo[w]; hi = a.tile_hi[w]; s0 = a.tile_s0[w]; s1 = a.tile_s1[w];
@@ -449,107 +476,95 @@ __global__ __launch_bounds__(256) void k_tile_sort(TileSortArgs<K> a) {
   const u64 kmask = kb >= 64 ? ~0ull : ((1ull << kb) - 1);
   const K* kin = a.kin[a.direct ? 0 : a.src_sel];
   const u32* vin = a.vin[a.direct ? 0 : a.src_sel];
-  // segment-start flags -> s_o temporarily
-  for (u32 i = tid; i < T; i += NT) s_o[i] = 0;
-  __syncthreads();
-  if (!a.direct) {
-    for (u32 s = s0 + tid; s <= s1; s += NT) {
-      u32 st = a.seg_start[s];
-      if (st >= lo && st < hi) s_o[st - lo] = 1;
-    }
+  // stage rows (coalesced) and mark segment starts
+  for (u32 i = tid; i < (u32)T; i += NT) s_f[i] = 0;
+  for (u32 i = tid; i < cnt; i += NT) {
+    s_c[i] = (u64)kin[lo + i];
+    s_o[i] = vin[lo + i];
   }
-  if (tid == 0) s_o[0] = 1;
   __syncthreads();
-  // blocked inclusive scan of flags -> local segment index
+  for (u32 s = s0 + tid; s <= s1; s += NT) {
+    u32 st = a.seg_start[s];
+    if (st >= lo && st < hi) s_f[st - lo] = 1;
+  }
+  if (tid == 0) s_f[0] = 1;
+  __syncthreads();
+  // blocked scan of the flags -> local segment index; build composites
   u32 f[E];
   u32 sum = 0;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    f[e] = s_o[tid * E + e];
+    f[e] = s_f[tid * E + e];
     sum += f[e];
   }
-  u32 tot;
-  u32 ex = block_excl_scan<NT>(sum, s_tmp, &tot);
-  u32 segi[E];
-  u32 runs = ex;
+  u32 nsegs;
+  u32 run = block_excl_scan<NT>(sum, s_tmp, &nsegs);
+  u64 vor = 0, vand = ~0ull;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    runs += f[e];
-    segi[e] = runs - 1;
-  }
-  __syncthreads();
-  // stage keys: coalesced global loads into LDS
-  for (u32 i = tid; i < T; i += NT) {
-    if (i < cnt) {
-      s_h[i] = (u64)kin[lo + i];
-      s_o[i] = vin[lo + i];
+    u32 p = tid * E + e;
+    run += f[e];
+    if (p < cnt) {
+      u64 k = s_c[p];
+      u32 sg = run - 1;
+      if (f[e]) s_top[sg] = k & ~kmask;
+      u64 c = ((u64)sg << kb) | (k & kmask);
+      s_c[p] = c;
+      vor |= c;
+      vand &= c;
     }
   }
-  __syncthreads();
-  Elt x[E];
-  const u64 pad = ((u64)a.pad_seg << kb);
+  u64 cor, cand;
+  block_or_and<NT>(vor, vand, s_red, &cor, &cand);  // contains __syncthreads
+  const u64 vary = cor ^ cand;
+  u32 segbits = 0;
+  while (segbits < 32 && (1u << segbits) < nsegs) ++segbits;
+  const u32 B = kb + segbits;
+  for (u32 shift = 0; shift < B; shift += RADIX_BITS) {
+    const u32 dbits = min((u32)RADIX_BITS, B - shift);
+    const u32 dmask = (1u << dbits) - 1;
+    if (((vary >> shift) & dmask) == 0) continue;  // every row has the same digit
+    u64 c[E];
+    u32 o[E], dig[E], rank[E];
+    bool ok[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    u32 i = tid * E + e;
-    if (i < cnt) {
-      u64 k = s_h[i];
-      if (f[e]) s_top[segi[e]] = k & ~kmask;
-      x[e].h = ((u64)segi[e] << kb) | (k & kmask);
-      x[e].o = s_o[i];
-    } else {
-      x[e].h = pad;
-      x[e].o = 0xffffffffu;
+    for (int j = 0; j < E; ++j) {
+      u32 idx = wv * WE + j * WAVE + lane;
+      ok[j] = idx < cnt;
+      c[j] = ok[j] ? s_c[idx] : 0ull;
+      o[j] = ok[j] ? s_o[idx] : 0u;
+      dig[j] = (u32)(c[j] >> shift) & dmask;
     }
-  }
-  reg_sort<E>(x);
-  for (u32 w = E; w < (u32)T; w <<= 1) {
+    for (u32 i = tid; i < (u32)(NW * RADIX); i += NT) (&s_whist[0][0])[i] = 0;
+    __syncthreads();
+    wave_rank<E>(dig, ok, dbits, s_whist[wv], rank);
+    __syncthreads();
+    {
+      u32 d = tid, r = 0;
+      for (int w = 0; w < NW; ++w) {
+        u32 x = s_whist[w][d];
+        s_whist[w][d] = r;
+        r += x;
+      }
+      u32 tot;
+      s_dig[d] = block_excl_scan<NT>(r, s_tmp, &tot);
+    }
     __syncthreads();
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      s_h[tid * E + e] = x[e].h;
-      s_o[tid * E + e] = x[e].o;
-    }
-    __syncthreads();
-    u32 out0 = tid * E;
-    u32 base = out0 & ~(2 * w - 1);
-    u32 dgn = out0 - base;
-    u32 alo = dgn > w ? dgn - w : 0, ahi = dgn < w ? dgn : w;
-    while (alo < ahi) {
-      u32 mid = (alo + ahi) >> 1;
-      Elt A = {s_h[base + mid], s_o[base + mid]};
-      Elt B = {s_h[base + w + dgn - 1 - mid], s_o[base + w + dgn - 1 - mid]};
-      if (elt_lt(B, A)) ahi = mid; else alo = mid + 1;
-    }
-    u32 ia = base + alo, ib = base + w + (dgn - alo);
-    const u32 ea = base + w, eb = base + 2 * w;
-    Elt A, B;
-    if (ia < ea) { A.h = s_h[ia]; A.o = s_o[ia]; }
-    if (ib < eb) { B.h = s_h[ib]; B.o = s_o[ib]; }
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      bool takeB = (ia >= ea) || (ib < eb && elt_lt(B, A));
-      if (takeB) {
-        x[e] = B;
-        ++ib;
-        if (ib < eb) { B.h = s_h[ib]; B.o = s_o[ib]; }
-      } else {
-        x[e] = A;
-        ++ia;
-        if (ia < ea) { A.h = s_h[ia]; A.o = s_o[ia]; }
+    for (int j = 0; j < E; ++j) {
+      if (ok[j]) {
+        u32 pos = s_dig[dig[j]] + s_whist[wv][dig[j]] + rank[j];
+        s_c[pos] = c[j];
+        s_o[pos] = o[j];
       }
     }
+    __syncthreads();
   }
-  __syncthreads();
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    s_h[tid * E + e] = x[e].h;
-    s_o[tid * E + e] = x[e].o;
-  }
-  __syncthreads();
   for (u32 i = tid; i < cnt; i += NT) {
-    u64 h = s_h[i];
-    u32 sg = (u32)(h >> kb);
-    u64 k = s_top[sg] | (h & kmask);
+    u64 c = s_c[i];
+    u32 sg = (u32)(c >> kb);
+    if (kb >= 64) sg = 0;
+    u64 k = s_top[sg] | (c & kmask);
     if (a.kout) a.kout[lo + i] = (K)k;
     a.vout[lo + i] = s_o[i];
   }
