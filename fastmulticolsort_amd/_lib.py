@@ -84,7 +84,7 @@ def load():
         "fmcs_multi_round_sort": (i32, [vp, u32, P(vp), P(u32), P(u32), u64, vp, vp, i32, P(u32)]),
         "fmcs_last_breakdown": (i32, [vp, P(C.c_double)]),
         "fmcs_hash_round": (i32, [vp, u32, u64, vp, vp, vp, vp, vp, vp, i32, P(u32)]),
-        "fmcs_multi_round_hash": (i32, [vp, u32, P(vp), P(u32), u64, vp, vp, P(u32)]),
+        "fmcs_multi_round_hash": (i32, [vp, u32, P(vp), P(u32), u64, i32, vp, vp, P(u32)]),
         "fmcs_ctx_set_profiling": (i32, [vp, i32]),
         "fmcs_kernel_stats": (i32, [vp, C.c_char_p, u64]),
         "fmcs_kernel_stats_reset": (i32, [vp]),

@@ -20,7 +20,8 @@ namespace {
 enum Slot {
   WS_KA, WS_KB, WS_VB, WS_T0K, WS_T0V, WS_T1K, WS_T1V, WS_STATUS, WS_HIST, WS_DBASE, WS_SEGOFF, WS_SEGOFF2,
   WS_ST, WS_SZ, WS_BST, WS_BSZ, WS_FLAG, WS_POS, WS_LLO, WS_LHI, WS_LTILES, WS_TBASE, WS_TLO, WS_THI, WS_TS0,
-  WS_TS1, WS_COMP, WS_COMP2, WS_GROUPTMP, WS_MISC, WS_HASH, WS_HASH2, WS_HASH3, WS_COUNT
+  WS_TS1, WS_COMP, WS_COMP2, WS_GROUPTMP, WS_MISC, WS_HASH, WS_HASH2, WS_HASH3, WS_HT_TABLE, WS_HT_FIRST,
+  WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_COUNT
 };
 
 struct Buf {
@@ -593,6 +594,44 @@ int sort_by_group_t(fmcs_ctx* ctx, u32 keybits, u64 n, const K* column, u32* ord
   return FMCS_OK;
 }
 
+// First-appearance group ids of (seg[i], keys[i]) over rows in order,
+// numbered 0.. across the whole array (segments contiguous or not).
+int hash_ids(fmcs_ctx* ctx, u32 bank, const void* keys, const u32* seg, u64 n, u32* gid, u32* ng) {
+  *ng = 0;
+  if (n == 0) return FMCS_OK;
+  u64 cap = 1;
+  while (cap < 2 * n) cap <<= 1;
+  if (cap > (1ull << 31)) return fail(ctx, FMCS_E_UNSUPPORTED, "hash table too large");
+  u32 *table, *first, *rank, *slot, *head, *prefix;
+  TRY(getbuf_t(ctx, WS_HT_TABLE, cap, &table));
+  TRY(getbuf_t(ctx, WS_HT_FIRST, cap, &first));
+  TRY(getbuf_t(ctx, WS_HT_RANK, cap, &rank));
+  TRY(getbuf_t(ctx, WS_HT_SLOT, n, &slot));
+  TRY(getbuf_t(ctx, WS_HT_HEAD, n, &head));
+  TRY(getbuf_t(ctx, WS_HT_PREFIX, n, &prefix));
+  TRY(fill(ctx, table, cap, 0xffffffffu));
+  TRY(fill(ctx, first, cap, 0xffffffffu));
+  u32 blocks = (u32)std::min<u64>((n + 255) / 256, 16384);
+  {
+    Prof pf(ctx, "k_hash_insert", (double)n * (bank + (seg ? 4 : 0) + 4));
+    switch (bank) {
+      case 1: hipLaunchKernelGGL((k_hash_insert<uint8_t>), dim3(blocks), dim3(256), 0, ctx->stream, (const uint8_t*)keys, seg, n, table, (u32)(cap - 1), slot, first); break;
+      case 2: hipLaunchKernelGGL((k_hash_insert<u16>), dim3(blocks), dim3(256), 0, ctx->stream, (const u16*)keys, seg, n, table, (u32)(cap - 1), slot, first); break;
+      case 4: hipLaunchKernelGGL((k_hash_insert<u32>), dim3(blocks), dim3(256), 0, ctx->stream, (const u32*)keys, seg, n, table, (u32)(cap - 1), slot, first); break;
+      default: hipLaunchKernelGGL((k_hash_insert<u64>), dim3(blocks), dim3(256), 0, ctx->stream, (const u64*)keys, seg, n, table, (u32)(cap - 1), slot, first); break;
+    }
+    TRY(launch_check(ctx, "k_hash_insert"));
+  }
+  hipLaunchKernelGGL(k_hash_heads, dim3(blocks), dim3(256), 0, ctx->stream, slot, first, n, head);
+  TRY(launch_check(ctx, "k_hash_heads"));
+  TRY(scan_excl(ctx, head, prefix, n, ctx->small_d + 56));
+  hipLaunchKernelGGL(k_hash_rank_slots, dim3(blocks), dim3(256), 0, ctx->stream, slot, head, prefix, n, rank);
+  TRY(launch_check(ctx, "k_hash_rank_slots"));
+  hipLaunchKernelGGL(k_hash_assign, dim3(blocks), dim3(256), 0, ctx->stream, slot, rank, n, gid);
+  TRY(launch_check(ctx, "k_hash_assign"));
+  return readback(ctx, ctx->small_d + 56, 1, ng);
+}
+
 }  // namespace
 
 // =========================================================================
@@ -938,14 +977,95 @@ int fmcs_last_breakdown(fmcs_ctx* ctx, double* ms4) {
   return FMCS_OK;
 }
 
-int fmcs_hash_round(fmcs_ctx* ctx, uint32_t, uint64_t, const void*, const uint32_t*, const uint32_t*, uint32_t*,
-                    uint32_t*, uint32_t*, int, uint32_t*) {
-  return fail(ctx, FMCS_E_UNSUPPORTED, "hash grouping not built yet");
+int fmcs_hash_round(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* values_d, const uint32_t* order_d,
+                    const uint32_t* in_group_d, uint32_t* gid_d, uint32_t* out_order_d, uint32_t* out_group_d,
+                    int last_round, uint32_t* ngroups) {
+  if (!ctx || !gid_d) return FMCS_E_ARG;
+  if (!(bank == 1 || bank_ok(bank))) return fail(ctx, FMCS_E_ARG, "bank must be 1, 2, 4 or 8");
+  if (!last_round && (!out_order_d || !out_group_d || !order_d))
+    return fail(ctx, FMCS_E_ARG, "order, out_order and out_group required unless last round");
+  const void* keys = values_d;
+  if (in_group_d && n) {
+    if (!order_d) return fail(ctx, FMCS_E_ARG, "order required after round 1");
+    if (((uintptr_t)order_d & 15) != 0) return fail(ctx, FMCS_E_ARG, "order must be 16-byte aligned");
+    void* ka;
+    TRY(getbuf(ctx, WS_KA, n * bank, &ka));
+    u32 blocks = (u32)((n + 1023) / 1024);
+    Prof pf(ctx, "k_gather", (double)n * (4 + 2 * bank));
+    switch (bank) {
+      case 1: hipLaunchKernelGGL((k_gather<uint8_t>), dim3(blocks), dim3(256), 0, ctx->stream, (const uint8_t*)values_d, order_d, (uint8_t*)ka, n); break;
+      case 2: hipLaunchKernelGGL((k_gather<u16>), dim3(blocks), dim3(256), 0, ctx->stream, (const u16*)values_d, order_d, (u16*)ka, n); break;
+      case 4: hipLaunchKernelGGL((k_gather<u32>), dim3(blocks), dim3(256), 0, ctx->stream, (const u32*)values_d, order_d, (u32*)ka, n); break;
+      default: hipLaunchKernelGGL((k_gather<u64>), dim3(blocks), dim3(256), 0, ctx->stream, (const u64*)values_d, order_d, (u64*)ka, n); break;
+    }
+    TRY(launch_check(ctx, "k_gather"));
+    keys = ka;
+  }
+  u32 ng = 0;
+  TRY(hash_ids(ctx, bank, keys, in_group_d, n, gid_d, &ng));
+  if (!last_round && n) {
+    u32 bits = 1;
+    while (bits < 32 && (1ull << bits) < ng) ++bits;
+    TRY(radix_sort<u32>(ctx, gid_d, order_d, out_group_d, out_order_d, n, bits));
+  }
+  if (ngroups) *ngroups = ng;
+  return FMCS_OK;
 }
 
-int fmcs_multi_round_hash(fmcs_ctx* ctx, uint32_t, const void* const*, const uint32_t*, uint64_t, uint32_t*,
-                          uint32_t*, uint32_t*) {
-  return fail(ctx, FMCS_E_UNSUPPORTED, "hash grouping not built yet");
+int fmcs_multi_round_hash(fmcs_ctx* ctx, uint32_t nrounds, const void* const* cols_d, const uint32_t* banks,
+                          uint64_t n, int hash_type, uint32_t* order_d, uint32_t* gid_d, uint32_t* ngroups) {
+  if (!ctx || !cols_d || !banks || !order_d || !gid_d || nrounds == 0) return FMCS_E_ARG;
+  if (n >= (1ull << 30)) return fail(ctx, FMCS_E_UNSUPPORTED, "n < 2^30 required");
+  for (u32 r = 0; r < nrounds; ++r)
+    if (!(banks[r] == 1 || bank_ok(banks[r]))) return fail(ctx, FMCS_E_ARG, "bad bank");
+  if (n == 0) {
+    if (ngroups) *ngroups = 0;
+    return FMCS_OK;
+  }
+  hipLaunchKernelGGL(k_iota_u32, dim3((u32)std::min<u64>((n + 255) / 256, 8192)), dim3(256), 0, ctx->stream, order_d, n);
+  TRY(launch_check(ctx, "k_iota_u32"));
+  HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+  u32 ng = 0;
+  if (hash_type == 0) {
+    // hashByGroup_non_reordered (src/composer.cpp:1153-1219): every round
+    // hashes (previous id, value) over the rows in their original order.
+    u32* prev;
+    TRY(getbuf_t(ctx, WS_GROUPTMP, n, &prev));
+    for (u32 r = 0; r < nrounds; ++r) {
+      if (r) HIPCHK(hipMemcpyAsync(prev, gid_d, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+      TRY(hash_ids(ctx, banks[r], cols_d[r], r ? prev : nullptr, n, gid_d, &ng));
+    }
+  } else {
+    // hashByGroup_reordered (src/composer.cpp:814-976)
+    u32 *ob, *ga, *gb;
+    TRY(getbuf_t(ctx, WS_HASH, n, &ga));
+    TRY(getbuf_t(ctx, WS_HASH2, n, &gb));
+    TRY(getbuf_t(ctx, WS_HASH3, n, &ob));
+    u32* cur_order = order_d;
+    u32* other_order = ob;
+    const u32* gin = nullptr;
+    for (u32 r = 0; r < nrounds; ++r) {
+      bool last = (r + 1 == nrounds);
+      u32* gout = (gin == ga) ? gb : ga;
+      TRY(fmcs_hash_round(ctx, banks[r], n, cols_d[r], cur_order, gin, gid_d, last ? nullptr : other_order,
+                          last ? nullptr : gout, last ? 1 : 0, &ng));
+      if (!last) {
+        std::swap(cur_order, other_order);
+        gin = gout;
+      }
+    }
+    if (cur_order != order_d) HIPCHK(hipMemcpyAsync(order_d, cur_order, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+  HIPCHK(hipEventSynchronize(ctx->ev[1]));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+  ctx->breakdown[0] = 0;
+  ctx->breakdown[1] = ms;
+  ctx->breakdown[2] = 0;
+  ctx->breakdown[3] = 0;
+  if (ngroups) *ngroups = ng;
+  return FMCS_OK;
 }
 
 }  // extern "C"

@@ -79,5 +79,10 @@ template <class K>
 __global__ void k_composite(const K*, const u32*, u64, u32, u64*);
 template <class K>
 __global__ void k_strip_composite(const u64*, u64, u64, K*);
+template <class K>
+__global__ void k_hash_insert(const K*, const u32*, u64, u32*, u32, u32*, u32*);
+__global__ void k_hash_heads(const u32*, const u32*, u64, u32*);
+__global__ void k_hash_rank_slots(const u32*, const u32*, const u32*, u64, u32*);
+__global__ void k_hash_assign(const u32*, const u32*, u64, u32*);
 
 }  // namespace fmcs

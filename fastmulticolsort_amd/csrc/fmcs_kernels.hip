@@ -725,6 +725,68 @@ __global__ void k_strip_composite(const u64* __restrict__ c, u64 n, u64 kmask, K
 }
 
 // ---------------------------------------------------------------------------
+// a10-a12: hash grouping.  One open-addressing table over (segment, key)
+// for all segments at once; a slot holds the row index of its first
+// inserter (one 32-bit CAS; keys are compared through the row index, so the
+// table stores no keys).  The group's id is fixed by the smallest row
+// index that maps to it (atomicMin), so ids follow first appearance
+// (src/hashing.h:289-385), whatever order the inserts raced in.
+// ---------------------------------------------------------------------------
+constexpr u32 HASH_EMPTY = 0xffffffffu;
+
+__device__ __forceinline__ u64 mix64(u64 x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
+}
+
+// table has cmask+1 >= 2n slots, so a free or matching slot always exists
+// and every probe sequence ends.
+template <class K>
+__global__ __launch_bounds__(256) void k_hash_insert(const K* __restrict__ keys, const u32* __restrict__ seg, u64 n,
+                                                     u32* __restrict__ table, u32 cmask, u32* __restrict__ slot_of,
+                                                     u32* __restrict__ first) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) {
+    const K k = keys[i];
+    const u32 g = seg ? seg[i] : 0u;
+    u32 h = (u32)mix64((u64)k ^ ((u64)g * 0x9E3779B97F4A7C15ull)) & cmask;
+    for (;;) {
+      u32 cur = __hip_atomic_load(table + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (cur == HASH_EMPTY) {
+        u32 prev = atomicCAS(table + h, HASH_EMPTY, (u32)i);
+        if (prev == HASH_EMPTY) break;
+        cur = prev;
+      }
+      if (keys[cur] == k && (!seg || seg[cur] == g)) break;
+      h = (h + 1) & cmask;
+    }
+    slot_of[i] = h;
+    atomicMin(first + h, (u32)i);
+  }
+}
+
+__global__ void k_hash_heads(const u32* __restrict__ slot_of, const u32* __restrict__ first, u64 n,
+                             u32* __restrict__ head) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x)
+    head[i] = (first[slot_of[i]] == (u32)i) ? 1u : 0u;
+}
+
+__global__ void k_hash_rank_slots(const u32* __restrict__ slot_of, const u32* __restrict__ head,
+                                  const u32* __restrict__ prefix, u64 n, u32* __restrict__ rank_of_slot) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x)
+    if (head[i]) rank_of_slot[slot_of[i]] = prefix[i];
+}
+
+__global__ void k_hash_assign(const u32* __restrict__ slot_of, const u32* __restrict__ rank_of_slot, u64 n,
+                              u32* __restrict__ gid) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x)
+    gid[i] = rank_of_slot[slot_of[i]];
+}
+
+// ---------------------------------------------------------------------------
 // Explicit instantiations used by the launchers.
 // ---------------------------------------------------------------------------
 #define FMCS_INST_K(K)                                                                                              \
@@ -738,10 +800,13 @@ __global__ void k_strip_composite(const u64* __restrict__ c, u64 n, u64 kmask, K
   template __global__ void k_copy_segs<K>(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, u32,  \
                                           u32);                                                                     \
   template __global__ void k_composite<K>(const K*, const u32*, u64, u32, u64*);                                    \
-  template __global__ void k_strip_composite<K>(const u64*, u64, u64, K*);
+  template __global__ void k_strip_composite<K>(const u64*, u64, u64, K*);                                          \
+  template __global__ void k_hash_insert<K>(const K*, const u32*, u64, u32*, u32, u32*, u32*);
 
 FMCS_INST_K(u16)
 FMCS_INST_K(u32)
+template __global__ void k_hash_insert<uint8_t>(const uint8_t*, const u32*, u64, u32*, u32, u32*, u32*);
+template __global__ void k_gather<uint8_t>(const uint8_t*, const u32*, uint8_t*, u64);
 FMCS_INST_K(u64)
 template __global__ void k_scan_excl<8>(const u32*, u32*, u64, u32*, u32*, u32*);
 

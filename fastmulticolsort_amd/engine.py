@@ -189,7 +189,7 @@ class Context:
                                            _p(gid), _p(oo), _p(og), int(last_round), C.byref(ng)))
         return gid, oo, og, ng.value
 
-    def multi_round_hash(self, cols):
+    def multi_round_hash(self, cols, hash_type=1):
         n = cols[0].numel()
         dev = cols[0].device
         r = len(cols)
@@ -198,7 +198,8 @@ class Context:
         order = torch.empty(n, dtype=torch.uint32, device=dev)
         gid = torch.empty(n, dtype=torch.uint32, device=dev)
         ng = C.c_uint32(0)
-        self._chk(self.lib.fmcs_multi_round_hash(self.h, r, cp, bk, n, _p(order), _p(gid), C.byref(ng)))
+        self._chk(self.lib.fmcs_multi_round_hash(self.h, r, cp, bk, n, hash_type, _p(order), _p(gid),
+                                                 C.byref(ng)))
         return order, gid, ng.value
 
 

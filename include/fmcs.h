@@ -171,16 +171,22 @@ int fmcs_kernel_stats_reset(fmcs_ctx* ctx);
  * segment order.  gid_d receives the ids per row; unless last_round the
  * rows are clustered stably by id (Partition::histogram_based_reordering):
  * out_order_d and out_group_d receive the clustered oids and ids.
- * values_d is the round's bank indexed by oid; order_d the current order. */
+ * values_d is the round's bank indexed by oid; order_d the current order
+ * (in round 1, in_group_d NULL, values are read in row order: the order is
+ * the identity there, as in the reference). */
 int fmcs_hash_round(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* values_d, const uint32_t* order_d,
                     const uint32_t* in_group_d, uint32_t* gid_d, uint32_t* out_order_d, uint32_t* out_group_d,
                     int last_round, uint32_t* ngroups);
 
-/* Composer::multiRoundHashing_reordered (src/composer.cpp:137-227) with its
- * results kept: order_d = order the last round hashed in, gid_d = the last
- * round's ids per row, *ngroups = number of groups. */
+/* Composer::multiRoundHashing_reordered / _non_reordered
+ * (src/composer.cpp:137-227, :321-381; chosen by compose_params_t.hash_type,
+ * src/chain_composer.cpp:16-25) with the results kept.  hash_type 1 or 2:
+ * order_d = order the last round hashed in, gid_d = the last round's ids
+ * per row of that order.  hash_type 0: no reordering; gid_d = first-
+ * appearance ids of the full tuple per original row, order_d = identity.
+ * *ngroups = number of groups of the full tuple. */
 int fmcs_multi_round_hash(fmcs_ctx* ctx, uint32_t nrounds, const void* const* cols_d, const uint32_t* banks,
-                          uint64_t n, uint32_t* order_d, uint32_t* gid_d, uint32_t* ngroups);
+                          uint64_t n, int hash_type, uint32_t* order_d, uint32_t* gid_d, uint32_t* ngroups);
 
 #ifdef __cplusplus
 }
