@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/fmcs.h"
+#include "fmcs_internal.h"
 #include "fmcs_kernels.h"
 
 using namespace fmcs;
@@ -21,7 +22,7 @@ enum Slot {
   WS_KA, WS_KB, WS_VB, WS_T0K, WS_T0V, WS_T1K, WS_T1V, WS_STATUS, WS_HIST, WS_DBASE, WS_SEGOFF, WS_SEGOFF2,
   WS_ST, WS_SZ, WS_BST, WS_BSZ, WS_FLAG, WS_POS, WS_LLO, WS_LHI, WS_LTILES, WS_TBASE, WS_TLO, WS_THI, WS_TS0,
   WS_TS1, WS_COMP, WS_COMP2, WS_GROUPTMP, WS_MISC, WS_HASH, WS_HASH2, WS_HASH3, WS_HT_TABLE, WS_HT_FIRST,
-  WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_COUNT
+  WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_USER0, WS_COUNT = WS_USER0 + 24
 };
 
 struct Buf {
@@ -633,6 +634,44 @@ int hash_ids(fmcs_ctx* ctx, u32 bank, const void* keys, const u32* seg, u64 n, u
 }
 
 }  // namespace
+
+// =========================================================================
+// Services for the host composer layer
+// =========================================================================
+namespace fmcs {
+namespace internal {
+
+hipStream_t stream_of(fmcs_ctx* ctx) { return ctx->stream; }
+
+int set_error(fmcs_ctx* ctx, int code, const char* msg) { return fail(ctx, code, msg); }
+
+int scratch(fmcs_ctx* ctx, int slot, size_t bytes, void** out) {
+  if (slot < 0 || slot >= USER_SLOTS) return fail(ctx, FMCS_E_ARG, "bad scratch slot");
+  return getbuf(ctx, (Slot)(WS_USER0 + slot), bytes, out);
+}
+
+int fill_random(fmcs_ctx* ctx, void* p, uint32_t bank, uint64_t n, uint64_t seed, uint32_t bits) {
+  if (n == 0) return FMCS_OK;
+  u32 blocks = (u32)std::min<u64>((n + 255) / 256, 8192);
+  switch (bank) {
+    case 1: hipLaunchKernelGGL((k_fill_random<uint8_t>), dim3(blocks), dim3(256), 0, ctx->stream, (uint8_t*)p, n, seed, bits); break;
+    case 2: hipLaunchKernelGGL((k_fill_random<u16>), dim3(blocks), dim3(256), 0, ctx->stream, (u16*)p, n, seed, bits); break;
+    case 4: hipLaunchKernelGGL((k_fill_random<u32>), dim3(blocks), dim3(256), 0, ctx->stream, (u32*)p, n, seed, bits); break;
+    case 8: hipLaunchKernelGGL((k_fill_random<u64>), dim3(blocks), dim3(256), 0, ctx->stream, (u64*)p, n, seed, bits); break;
+    default: return fail(ctx, FMCS_E_ARG, "bad bank");
+  }
+  return launch_check(ctx, "k_fill_random");
+}
+
+int uniform_offsets(fmcs_ctx* ctx, uint32_t* off, uint32_t nseg, uint64_t n, uint32_t size) {
+  u64 cnt = (u64)nseg + 1;
+  u32 blocks = (u32)std::min<u64>((cnt + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_uniform_offsets, dim3(blocks), dim3(256), 0, ctx->stream, off, nseg, size, n);
+  return launch_check(ctx, "k_uniform_offsets");
+}
+
+}  // namespace internal
+}  // namespace fmcs
 
 // =========================================================================
 // C ABI

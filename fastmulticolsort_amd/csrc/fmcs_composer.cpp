@@ -1,0 +1,390 @@
+// fmcs_composer.cpp -- the reference's composer API on the host, driving
+// the GPU through the C ABI.
+//
+// Column / SpecialiedColumn<T> (src/column.h:12-63) hold device banks;
+// Composer (src/composer.h:20-176) runs the round loop; ChainComposer
+// (src/chain_composer.h) is the concrete composer; StitchComposer
+// (src/stitch_composer.h:13-252) massages columns into new banks and runs
+// one plan.  Times are HIP-event milliseconds on the context's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "../../include/fmcs.h"
+#include "../../include/fmcs_plan.h"
+#include "fmcs_internal.h"
+
+namespace fmcs {
+namespace host {
+
+using internal::scratch;
+using internal::set_error;
+using internal::stream_of;
+
+// construct_entry_t (src/types.h:42-46)
+struct construct_entry_t {
+  uint32_t finalIdx;
+  uint32_t start;
+  uint32_t end;
+};
+
+// setting_t's timing fields (src/types.h:198-236), in milliseconds.
+struct setting_t {
+  double time_tupleReconstruct = 0;
+  double time_orderGroupInfo = 0;
+  double time_multipass_sort = 0;
+  double time_firstpass_sort = 0;
+  double time_extractGrop = 0;
+  double time_stitching = 0;
+};
+
+// compose_params_t (src/types.h:169-186), the fields this path uses.
+struct compose_params_t {
+  std::vector<uint32_t> bitwidth;
+  uint32_t ordered = 1;
+  uint32_t hash_type = 1;
+  int mode = FMCS_MODE_BATCHED;
+};
+
+class Column {
+ public:
+  virtual ~Column() {}
+  virtual void* GetColumn() const = 0;
+  virtual uint32_t GetWidth() const = 0;
+};
+
+template <class T>
+class SpecialiedColumn : public Column {
+ public:
+  explicit SpecialiedColumn(T* values) : values_(values) {}
+  void* GetColumn() const override { return values_; }
+  uint32_t GetWidth() const override { return sizeof(T); }
+
+ private:
+  T* values_;
+};
+
+std::unique_ptr<Column> make_column(uint32_t bank, void* p) {
+  switch (bank) {
+    case 1: return std::unique_ptr<Column>(new SpecialiedColumn<uint8_t>((uint8_t*)p));
+    case 2: return std::unique_ptr<Column>(new SpecialiedColumn<uint16_t>((uint16_t*)p));
+    case 4: return std::unique_ptr<Column>(new SpecialiedColumn<uint32_t>((uint32_t*)p));
+    default: return std::unique_ptr<Column>(new SpecialiedColumn<uint64_t>((uint64_t*)p));
+  }
+}
+
+class Composer {
+ public:
+  Composer(fmcs_ctx* ctx, uint32_t ncolumns, uint64_t nrows, compose_params_t params, std::vector<Column*> cols)
+      : ctx_(ctx), num_columns_(ncolumns), num_rows_(nrows), params_(std::move(params)), cols_(std::move(cols)) {}
+  virtual ~Composer() {}
+
+  // Composer::SortAllColumns -> multiRoundSorting (src/composer.cpp:18-27, :48-135)
+  int SortAllColumns(uint32_t* order_d, uint32_t* groups_d, uint32_t* ngroups) {
+    std::vector<const void*> p(num_columns_);
+    std::vector<uint32_t> banks(num_columns_);
+    for (uint32_t i = 0; i < num_columns_; ++i) {
+      p[i] = cols_[i]->GetColumn();
+      banks[i] = cols_[i]->GetWidth();
+    }
+    int rc = fmcs_multi_round_sort(ctx_, num_columns_, p.data(), banks.data(), params_.bitwidth.data(), num_rows_,
+                                   order_d, groups_d, params_.mode, ngroups);
+    if (rc != FMCS_OK) return rc;
+    double b[4];
+    fmcs_last_breakdown(ctx_, b);
+    setting_.time_tupleReconstruct = b[0];
+    setting_.time_firstpass_sort = b[1];
+    setting_.time_multipass_sort = b[1] + b[2];
+    setting_.time_extractGrop = b[3];
+    setting_.time_orderGroupInfo = b[3];
+    return FMCS_OK;
+  }
+
+  // ChainComposer::HashAllColumns (src/chain_composer.cpp:16-25)
+  int HashAllColumns(uint32_t* order_d, uint32_t* gid_d, uint32_t* ngroups) {
+    std::vector<const void*> p(num_columns_);
+    std::vector<uint32_t> banks(num_columns_);
+    for (uint32_t i = 0; i < num_columns_; ++i) {
+      p[i] = cols_[i]->GetColumn();
+      banks[i] = cols_[i]->GetWidth();
+    }
+    int rc = fmcs_multi_round_hash(ctx_, num_columns_, p.data(), banks.data(), num_rows_, (int)params_.hash_type,
+                                   order_d, gid_d, ngroups);
+    if (rc != FMCS_OK) return rc;
+    double b[4];
+    fmcs_last_breakdown(ctx_, b);
+    setting_.time_multipass_sort = b[1];
+    return FMCS_OK;
+  }
+
+  setting_t getSetting() const { return setting_; }
+
+ protected:
+  fmcs_ctx* ctx_;
+  uint32_t num_columns_;
+  uint64_t num_rows_;
+  compose_params_t params_;
+  std::vector<Column*> cols_;
+  setting_t setting_;
+};
+
+class ChainComposer : public Composer {
+ public:
+  using Composer::Composer;
+};
+
+class StitchComposer : public Composer {
+ public:
+  using Composer::Composer;
+
+  // Step a of run_single_plan (src/stitch_composer.cpp:571-612).
+  int construct(const uint32_t* columnOrder, const std::vector<uint32_t>& split,
+                std::vector<fmcs_entry>& entries) {
+    entries.resize(64 * 64);
+    uint32_t ne = 0;
+    int rc = fmcs_construct_entries(num_columns_, params_.bitwidth.data(), columnOrder, (uint32_t)split.size(),
+                                    split.data(), entries.data(), (uint32_t)entries.size(), &ne);
+    if (rc != FMCS_OK) return set_error(ctx_, rc, "invalid split for these columns");
+    entries.resize(ne);
+    return FMCS_OK;
+  }
+
+  // StitchComposer::run_single_plan / run_single_plan_hash
+  // (src/stitch_composer.cpp:555-806, :808-1079): b1 allocate the new banks,
+  // b2 stitch them (timed as breakdown[0]), c run a ChainComposer on them.
+  int run_single_plan(const uint32_t* columnOrder, const std::vector<uint32_t>& split, bool hash,
+                      uint32_t* order_d, uint32_t* groups_d, double* breakdown, double* total_ms,
+                      uint32_t* ngroups) {
+    const uint32_t nn = (uint32_t)split.size();
+    if (nn == 0 || nn > 16) return set_error(ctx_, FMCS_E_ARG, "1..16 rounds supported");
+    std::vector<fmcs_entry> entries;
+    int rc = construct(columnOrder, split, entries);
+    if (rc != FMCS_OK) return rc;
+    std::vector<void*> dst(nn);
+    std::vector<uint32_t> dbank(nn);
+    for (uint32_t c = 0; c < nn; ++c) {
+      dbank[c] = fmcs_bank_bytes(split[c], hash ? 1 : 0);
+      rc = scratch(ctx_, (int)c, (num_rows_ + 16) * dbank[c], &dst[c]);
+      if (rc != FMCS_OK) return rc;
+    }
+    std::vector<const void*> src(num_columns_);
+    std::vector<uint32_t> sbank(num_columns_);
+    for (uint32_t i = 0; i < num_columns_; ++i) {
+      src[i] = cols_[i]->GetColumn();
+      sbank[i] = cols_[i]->GetWidth();
+    }
+    hipStream_t st = stream_of(ctx_);
+    hipEvent_t e0, e1, e2;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess || hipEventCreate(&e2) != hipSuccess)
+      return set_error(ctx_, FMCS_E_HIP, "event creation failed");
+    (void)hipEventRecord(e0, st);
+    rc = fmcs_stitch(ctx_, num_rows_, num_columns_, src.data(), sbank.data(), params_.bitwidth.data(),
+                     (uint32_t)entries.size(), entries.data(), nn, dst.data(), dbank.data());
+    (void)hipEventRecord(e1, st);
+    std::vector<std::unique_ptr<Column>> owned;
+    std::vector<Column*> newcols;
+    for (uint32_t c = 0; c < nn; ++c) {
+      owned.push_back(make_column(dbank[c], dst[c]));
+      newcols.push_back(owned.back().get());
+    }
+    compose_params_t p2;
+    p2.bitwidth = split;
+    p2.ordered = params_.ordered;
+    p2.hash_type = params_.hash_type;
+    p2.mode = params_.mode;
+    ChainComposer chain(ctx_, nn, num_rows_, p2, newcols);
+    if (rc == FMCS_OK)
+      rc = hash ? chain.HashAllColumns(order_d, groups_d, ngroups) : chain.SortAllColumns(order_d, groups_d, ngroups);
+    (void)hipEventRecord(e2, st);
+    (void)hipEventSynchronize(e2);
+    float ms_stitch = 0, ms_all = 0;
+    (void)hipEventElapsedTime(&ms_stitch, e0, e1);
+    (void)hipEventElapsedTime(&ms_all, e0, e2);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    if (rc != FMCS_OK) return rc;
+    setting_t u = chain.getSetting();
+    if (breakdown) {
+      breakdown[0] = ms_stitch;
+      breakdown[1] = u.time_tupleReconstruct;
+      breakdown[2] = u.time_orderGroupInfo;
+      breakdown[3] = u.time_multipass_sort;
+      breakdown[4] = u.time_firstpass_sort;
+      breakdown[5] = u.time_extractGrop;
+    }
+    if (total_ms) *total_ms = ms_all;
+    return FMCS_OK;
+  }
+};
+
+// ---- calibration -----------------------------------------------------------
+
+double time_ms(fmcs_ctx* ctx, int reps, int (*fn)(fmcs_ctx*, void*), void* arg) {
+  hipStream_t st = stream_of(ctx);
+  hipEvent_t a, b;
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
+  fn(ctx, arg);  // warm-up
+  (void)hipEventRecord(a, st);
+  for (int i = 0; i < reps; ++i)
+    if (fn(ctx, arg) != FMCS_OK) break;
+  (void)hipEventRecord(b, st);
+  (void)hipEventSynchronize(b);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms / reps;
+}
+
+struct CalArgs {
+  uint32_t bank, bits;
+  uint64_t n;
+  void *keys, *keys0, *vals2;
+  uint32_t *oids, *oids0, *perm, *off, *grp;
+  uint32_t nseg;
+};
+
+int reset_pairs(fmcs_ctx* ctx, CalArgs* a) {
+  hipStream_t st = stream_of(ctx);
+  if (hipMemcpyAsync(a->keys, a->keys0, a->n * a->bank, hipMemcpyDeviceToDevice, st) != hipSuccess) return FMCS_E_HIP;
+  if (hipMemcpyAsync(a->oids, a->oids0, a->n * 4, hipMemcpyDeviceToDevice, st) != hipSuccess) return FMCS_E_HIP;
+  return FMCS_OK;
+}
+
+}  // namespace host
+}  // namespace fmcs
+
+using namespace fmcs::host;
+
+extern "C" {
+
+int fmcs_run_plan(fmcs_ctx* ctx, uint32_t ncols, const void* const* cols_d, const uint32_t* banks, const uint32_t* bw,
+                  const uint32_t* col_order, uint32_t nsplit, const uint32_t* split, int mode, uint64_t nrows,
+                  uint32_t* order_d, uint32_t* groups_d, double* ms6, double* total_ms, uint32_t* ngroups) {
+  if (!ctx || !cols_d || !banks || !bw || !col_order || !split || !order_d || ncols == 0 || ncols > 64)
+    return FMCS_E_ARG;
+  std::vector<std::unique_ptr<Column>> owned;
+  std::vector<Column*> cols;
+  for (uint32_t i = 0; i < ncols; ++i) {
+    owned.push_back(make_column(banks[i], const_cast<void*>(cols_d[i])));
+    cols.push_back(owned.back().get());
+  }
+  compose_params_t p;
+  p.bitwidth.assign(bw, bw + ncols);
+  p.mode = mode;
+  StitchComposer sc(ctx, ncols, nrows, p, cols);
+  std::vector<uint32_t> sp(split, split + nsplit);
+  return sc.run_single_plan(col_order, sp, false, order_d, groups_d, ms6, total_ms, ngroups);
+}
+
+int fmcs_run_plan_hash(fmcs_ctx* ctx, uint32_t ncols, const void* const* cols_d, const uint32_t* banks,
+                       const uint32_t* bw, const uint32_t* col_order, uint32_t nsplit, const uint32_t* split,
+                       int hash_type, uint64_t nrows, uint32_t* order_d, uint32_t* gid_d, double* total_ms,
+                       uint32_t* ngroups) {
+  if (!ctx || !cols_d || !banks || !bw || !col_order || !split || !order_d || !gid_d || ncols == 0 || ncols > 64)
+    return FMCS_E_ARG;
+  std::vector<std::unique_ptr<Column>> owned;
+  std::vector<Column*> cols;
+  for (uint32_t i = 0; i < ncols; ++i) {
+    owned.push_back(make_column(banks[i], const_cast<void*>(cols_d[i])));
+    cols.push_back(owned.back().get());
+  }
+  compose_params_t p;
+  p.bitwidth.assign(bw, bw + ncols);
+  p.hash_type = (uint32_t)hash_type;
+  StitchComposer sc(ctx, ncols, nrows, p, cols);
+  std::vector<uint32_t> sp(split, split + nsplit);
+  return sc.run_single_plan(col_order, sp, true, order_d, gid_d, nullptr, total_ms, ngroups);
+}
+
+int fmcs_calibrate(fmcs_ctx* ctx, uint64_t nrows, fmcs_cost_model* out) {
+  if (!ctx || !out || nrows < 4096 || nrows >= (1ull << 30)) return FMCS_E_ARG;
+  fmcs_cost_model_default(out);
+  const uint64_t n = nrows;
+  CalArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.n = n;
+  int rc;
+  void *k, *k0, *v2;
+  uint32_t *o, *o0, *perm, *off;
+  if ((rc = scratch(ctx, 16, n * 8 + 64, &k)) || (rc = scratch(ctx, 17, n * 8 + 64, &k0)) ||
+      (rc = scratch(ctx, 18, n * 8 + 64, &v2)) || (rc = scratch(ctx, 19, n * 4 + 64, (void**)&o)) ||
+      (rc = scratch(ctx, 20, n * 4 + 64, (void**)&o0)) || (rc = scratch(ctx, 21, n * 4 + 64, (void**)&perm)) ||
+      (rc = scratch(ctx, 22, n * 4 + 64, (void**)&off)))
+    return rc;
+  a.keys = k; a.keys0 = k0; a.vals2 = v2; a.oids = o; a.oids0 = o0; a.perm = perm; a.off = off;
+  // a random permutation: sort random keys with an iota payload
+  if ((rc = fmcs::internal::fill_random(ctx, k0, 8, n, 12345, 64))) return rc;
+  if ((rc = fmcs::internal::uniform_offsets(ctx, o0, (uint32_t)n - 1, n, 1))) return rc;  // o0 = 0..n-1
+  if (hipMemcpyAsync(perm, o0, n * 4, hipMemcpyDeviceToDevice, stream_of(ctx)) != hipSuccess) return FMCS_E_HIP;
+  if ((rc = fmcs_sort_pairs(ctx, 8, 64, n, k0, perm))) return rc;
+  const uint32_t banks[3] = {2, 4, 8}, bits[3] = {16, 32, 64};
+  for (int b = 0; b < 3; ++b) {
+    a.bank = banks[b];
+    a.bits = bits[b];
+    if ((rc = fmcs::internal::fill_random(ctx, k0, a.bank, n, 777 + b, 8))) return rc;
+    // one LSD pass (8-bit keys) including the reset copies, minus the copies
+    double t_pass = time_ms(ctx, 3, [](fmcs_ctx* c, void* x) {
+      CalArgs* q = (CalArgs*)x;
+      int r = reset_pairs(c, q);
+      return r ? r : fmcs_sort_pairs(c, q->bank, 8, q->n, q->keys, q->oids);
+    }, &a);
+    double t_copy = time_ms(ctx, 3, [](fmcs_ctx* c, void* x) { return reset_pairs(c, (CalArgs*)x); }, &a);
+    out->radix_pass_ns[b] = std::max(1e-6, (t_pass - t_copy)) * 1e6 / n;
+    if ((rc = fmcs::internal::fill_random(ctx, k0, a.bank, n, 999 + b, a.bits))) return rc;
+    double t_g = time_ms(ctx, 3, [](fmcs_ctx* c, void* x) {
+      CalArgs* q = (CalArgs*)x;
+      return fmcs_gather(c, q->bank, q->n, q->keys0, q->perm, q->keys);
+    }, &a);
+    out->gather_ns[b] = t_g * 1e6 / n;
+    // boundary over sorted keys
+    if ((rc = reset_pairs(ctx, &a)) || (rc = fmcs_sort_pairs(ctx, a.bank, a.bits, n, a.keys, a.oids))) return rc;
+    double t_b = time_ms(ctx, 3, [](fmcs_ctx* c, void* x) {
+      CalArgs* q = (CalArgs*)x;
+      uint32_t ng;
+      return fmcs_extract_group_info(c, q->bank, q->n, q->keys, q->oids, (uint32_t*)q->vals2, nullptr, &ng);
+    }, &a);
+    out->boundary_ns[b] = t_b * 1e6 / n;
+    // segmented sorts over uniform groups of 2^i rows
+    for (int i = 0; i < FMCS_COST_GBUCKETS; ++i) {
+      uint64_t g = 1ull << i;
+      if (g > n) {
+        out->segsort_ns[b][i] = out->segsort_ns[b][i - 1];
+        continue;
+      }
+      a.nseg = (uint32_t)((n + g - 1) / g);
+      if ((rc = fmcs::internal::uniform_offsets(ctx, off, a.nseg, n, (uint32_t)g))) return rc;
+      double t = time_ms(ctx, 2, [](fmcs_ctx* c, void* x) {
+        CalArgs* q = (CalArgs*)x;
+        int r = reset_pairs(c, q);
+        return r ? r : fmcs_segmented_sort(c, q->bank, q->bits, q->n, q->keys, q->oids, q->off, q->nseg,
+                                           FMCS_MODE_BATCHED, nullptr);
+      }, &a);
+      out->segsort_ns[b][i] = std::max(1e-6, t - t_copy) * 1e6 / n;
+    }
+  }
+  // stitch: two u32 sources into one u64 bank
+  {
+    a.bank = 4;
+    if ((rc = fmcs::internal::fill_random(ctx, k0, 4, n, 5, 20)) || (rc = fmcs::internal::fill_random(ctx, k, 4, n, 6, 20)))
+      return rc;
+    double t_s = time_ms(ctx, 3, [](fmcs_ctx* c, void* x) {
+      CalArgs* q = (CalArgs*)x;
+      const void* src[2] = {q->keys0, q->keys};
+      uint32_t sb[2] = {4, 4}, sw[2] = {20, 20}, db[1] = {8};
+      fmcs_entry e[2] = {{0, 0, 1, 20}, {0, 1, 1, 20}};
+      void* dst[1] = {q->vals2};
+      return fmcs_stitch(c, q->n, 2, src, sb, sw, 2, e, 1, dst, db);
+    }, &a);
+    out->stitch_ns_per_byte = t_s * 1e6 / (n * 16.0);
+  }
+  return FMCS_OK;
+}
+
+}  // extern "C"

@@ -84,5 +84,8 @@ __global__ void k_hash_insert(const K*, const u32*, u64, u32*, u32, u32*, u32*);
 __global__ void k_hash_heads(const u32*, const u32*, u64, u32*);
 __global__ void k_hash_rank_slots(const u32*, const u32*, const u32*, u64, u32*);
 __global__ void k_hash_assign(const u32*, const u32*, u64, u32*);
+template <class K>
+__global__ void k_fill_random(K*, u64, u64, u32);
+__global__ void k_uniform_offsets(u32*, u32, u32, u64);
 
 }  // namespace fmcs

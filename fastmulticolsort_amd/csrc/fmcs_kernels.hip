@@ -786,6 +786,23 @@ __global__ void k_hash_assign(const u32* __restrict__ slot_of, const u32* __rest
     gid[i] = rank_of_slot[slot_of[i]];
 }
 
+// Synthetic banks and offsets for on-device calibration.
+template <class K>
+__global__ void k_fill_random(K* __restrict__ p, u64 n, u64 seed, u32 bits) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) {
+    u64 z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    p[i] = (K)(bits >= 64 ? z : (z >> (64 - bits)));
+  }
+}
+
+__global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64 n) {
+  for (u64 s = (u64)blockIdx.x * blockDim.x + threadIdx.x; s <= nseg; s += (u64)gridDim.x * blockDim.x)
+    off[s] = (u32)min((u64)s * size, n);
+}
+
 // ---------------------------------------------------------------------------
 // Explicit instantiations used by the launchers.
 // ---------------------------------------------------------------------------
@@ -801,12 +818,14 @@ __global__ void k_hash_assign(const u32* __restrict__ slot_of, const u32* __rest
                                           u32);                                                                     \
   template __global__ void k_composite<K>(const K*, const u32*, u64, u32, u64*);                                    \
   template __global__ void k_strip_composite<K>(const u64*, u64, u64, K*);                                          \
-  template __global__ void k_hash_insert<K>(const K*, const u32*, u64, u32*, u32, u32*, u32*);
+  template __global__ void k_hash_insert<K>(const K*, const u32*, u64, u32*, u32, u32*, u32*);                    \
+  template __global__ void k_fill_random<K>(K*, u64, u64, u32);
 
 FMCS_INST_K(u16)
 FMCS_INST_K(u32)
 template __global__ void k_hash_insert<uint8_t>(const uint8_t*, const u32*, u64, u32*, u32, u32*, u32*);
 template __global__ void k_gather<uint8_t>(const uint8_t*, const u32*, uint8_t*, u64);
+template __global__ void k_fill_random<uint8_t>(uint8_t*, u64, u64, u32);
 FMCS_INST_K(u64)
 template __global__ void k_scan_excl<8>(const u32*, u32*, u64, u32*, u32*, u32*);
 

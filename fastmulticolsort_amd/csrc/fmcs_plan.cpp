@@ -1,0 +1,581 @@
+// fmcs_plan.cpp -- host-side massaging planner: plan enumeration, the
+// analytic cost model (structure of the reference's, constants measured on
+// the GPU), greedy plans, Recursive Random Search and the optimizer.
+// Pure host C++; no GPU is touched here.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <random>
+#include <set>
+#include <vector>
+
+#include "../../include/fmcs.h"
+#include "../../include/fmcs_plan.h"
+
+namespace {
+
+typedef std::vector<uint32_t> Split;
+
+int bank_type(uint32_t width) { return width <= 16 ? 0 : (width <= 32 ? 1 : 2); }
+
+// ---- cost model ---------------------------------------------------------
+
+// Per-row segmented-sort time for an average group size g (log2 buckets,
+// linear interpolation in log2 g).
+double segsort_ns(const fmcs_cost_model* m, int bt, double g) {
+  double lg = std::log2(std::max(1.0, g));
+  int i = (int)std::floor(lg);
+  if (i >= FMCS_COST_GBUCKETS - 1) return m->segsort_ns[bt][FMCS_COST_GBUCKETS - 1];
+  double f = lg - i;
+  return m->segsort_ns[bt][i] * (1 - f) + m->segsort_ns[bt][i + 1] * f;
+}
+
+// The round-1 radix sort cost: one LSD pass per 8 key bits.
+double radix_ms(const fmcs_cost_model* m, int bt, uint32_t bits, double n) {
+  uint32_t passes = (bits + 7) / 8;
+  return n * passes * m->radix_pass_ns[bt] * 1e-6;
+}
+
+// StitchComposer::calculate_single_column_cost (src/stitch_composer.cpp:
+// 2186-2226): (n/g) groups times the sort cost of one group, scaled by
+// f(g) = 1/(1 + 1/(g-1)) (groups of one row need no sort).  With per-row
+// GPU costs this is n * f(g) * per_row(g).
+double single_column_cost(const fmcs_cost_model* m, double n, double g, int bt) {
+  double f = (g - 1.0) < 1e-6 ? 0.0 : 1.0 / (1.0 + 1.0 / (g - 1.0));
+  return n * f * segsort_ns(m, bt, g) * 1e-6;
+}
+
+struct Stats {
+  uint32_t assemble[3][3];  // [from bank][to bank] entry counts
+  uint32_t shift[3];        // entries per destination bank
+};
+
+// getCardinalityFromStatistics (src/stitch_composer.cpp:288-553) and
+// PlanEnumRRS::GetStatistics (src/plan_enum_rrs.cpp:269-396): walk the
+// construct entries of the split, count assemble/shift operations by bank,
+// and multiply card_per_bit^width into each new column's cardinality.
+// truncate=true reproduces the uint64 cardinalities of the RRS variant.
+void plan_stats(uint32_t ncols, const uint32_t* bw, const uint64_t* card, const uint32_t* order, const Split& split,
+                std::vector<double>& cardinality, Stats& st, bool truncate) {
+  std::memset(&st, 0, sizeof(st));
+  const size_t nn = split.size();
+  cardinality.assign(nn, 1.0);
+  std::vector<double> cpb(ncols);
+  for (uint32_t c = 0; c < ncols; ++c) cpb[c] = std::pow((double)card[c], 1.0 / (double)bw[c]);
+  uint32_t oi = 0, ni = 0;
+  uint32_t ow = bw[order[0]], nw = split[0];
+  while (ni < nn) {
+    uint32_t m = std::min(ow, nw);
+    ow -= m;
+    nw -= m;
+    uint32_t src = order[oi];
+    int to = bank_type(split[ni]);
+    int from = bank_type(bw[src]);
+    st.shift[to]++;
+    st.assemble[from][to]++;
+    if (ni + 1 < nn) {
+      double v = cardinality[ni] * std::pow(cpb[src], (double)m);
+      cardinality[ni] = truncate ? (double)(uint64_t)v : v;
+    }
+    if (ow == 0) {
+      ++oi;
+      if (oi < ncols) ow = bw[order[oi]];
+    }
+    if (nw == 0) {
+      ++ni;
+      if (ni < nn) nw = split[ni];
+    }
+  }
+}
+
+// estimate_plan_collection_greedy's per-plan costing (src/stitch_composer.cpp:
+// 2228-2378) with GPU terms: stitch, tuple reconstruction (gather), multi-pass
+// sort and group extraction.
+double estimate(const fmcs_cost_model* m, double n, uint32_t ncols, const uint32_t* bw, const uint64_t* card,
+                const uint32_t* order, const Split& split, bool truncate, double* parts) {
+  std::vector<double> cardv;
+  Stats st;
+  plan_stats(ncols, bw, card, order, split, cardv, st, truncate);
+  const size_t R = split.size();
+  // groups before each round: G_0 = 1, G_r = prod_{i<r} card_i
+  std::vector<double> G(R, 1.0);
+  double acc = 1.0;
+  for (size_t r = 0; r < R; ++r) {
+    G[r] = acc;
+    acc *= cardv[r];
+  }
+  double sort_ms = 0, gather_ms = 0, extract_ms = 0;
+  for (size_t r = 0; r < R; ++r) {
+    int bt = bank_type(split[r]);
+    if (r == 0) {
+      sort_ms += radix_ms(m, bt, split[0], n);
+    } else {
+      double g = std::max(1.0, n / std::min(G[r], n));
+      sort_ms += single_column_cost(m, n, g, bt);
+      gather_ms += n * m->gather_ns[bt] * 1e-6;
+    }
+    if (r + 1 < R) extract_ms += n * m->boundary_ns[bt] * 1e-6;
+  }
+  // fused stitch: every source bank read once, every new bank written once
+  double src_bytes = 0, dst_bytes = 0;
+  for (uint32_t c = 0; c < ncols; ++c) src_bytes += fmcs_bank_bytes(bw[c], 0);
+  for (size_t r = 0; r < R; ++r) dst_bytes += fmcs_bank_bytes(split[r], 0);
+  double stitch_ms = n * (src_bytes + dst_bytes) * m->stitch_ns_per_byte * 1e-6;
+  double total = stitch_ms + gather_ms + sort_ms + extract_ms + R * m->round_overhead_ms;
+  if (parts) {
+    parts[0] = stitch_ms;
+    parts[1] = gather_ms;
+    parts[2] = sort_ms;
+    parts[3] = extract_ms;
+  }
+  return total;
+}
+
+// ---- enumerators -----------------------------------------------------------
+
+// split_exhaustive_certain_round (src/stitch_composer.cpp:205-230)
+void split_rec(int n, int cur, int rounds, int maxv, uint32_t* part, std::vector<Split>& out) {
+  if (cur == rounds - 1) {
+    if (n > maxv) return;
+    part[cur] = n;
+    out.emplace_back(part, part + rounds);
+    return;
+  }
+  for (int i = n > maxv ? maxv : n; i > 0; --i) {
+    part[cur] = i;
+    if (n - i > 0) split_rec(n - i, cur + 1, rounds, maxv, part, out);
+  }
+}
+
+std::vector<Split> split_exhaustive(uint32_t S, uint32_t rounds, uint32_t maxv) {
+  std::vector<Split> out;
+  if (rounds == 0 || rounds > 64 || S == 0) return out;
+  uint32_t part[64];
+  split_rec((int)S, 0, (int)rounds, (int)maxv, part, out);
+  return out;
+}
+
+// enum_banksize_category (src/stitch_composer.cpp:1717-1734): 3^r patterns
+// in lexicographic order.
+std::vector<Split> bank_categories(uint32_t rounds) {
+  std::vector<Split> out;
+  Split cur(rounds, 0);
+  uint64_t total = 1;
+  for (uint32_t i = 0; i < rounds; ++i) total *= 3;
+  for (uint64_t k = 0; k < total; ++k) {
+    uint64_t x = k;
+    for (int i = (int)rounds - 1; i >= 0; --i) {
+      cur[i] = (uint32_t)(x % 3);
+      x /= 3;
+    }
+    out.push_back(cur);
+  }
+  return out;
+}
+
+// estimateGroupNum (src/stitch_composer.cpp:1845-1867)
+double estimate_group_num(uint32_t bit_position, uint32_t ncols, const uint32_t* bw, const uint64_t* card,
+                          const uint32_t* order) {
+  uint32_t avail = bit_position;
+  double g = 1.0;
+  for (uint32_t c = 0; c < ncols; ++c) {
+    uint32_t src = order[c];
+    uint32_t take = std::min(avail, bw[src]);
+    g *= std::pow(std::pow((double)card[src], 1.0 / (double)bw[src]), (double)take);
+    avail -= take;
+    if (avail == 0) break;
+  }
+  return g;
+}
+
+// find_greedy_plans (src/stitch_composer.cpp:1992-2125): per bank pattern,
+// every cut is placed where the NEXT round's sort is cheapest
+// (determineBitRange :1736-1763, determineOptBitPosition_v2 :2155-2184).
+std::vector<Split> greedy_plans(const fmcs_cost_model* m, double n, uint32_t ncols, const uint32_t* bw,
+                                const uint64_t* card, const uint32_t* order, uint32_t rounds) {
+  std::vector<Split> out;
+  uint32_t S = 0;
+  for (uint32_t c = 0; c < ncols; ++c) S += bw[c];
+  for (const Split& cate : bank_categories(rounds)) {
+    Split plan;
+    uint32_t off = 0;
+    bool valid = true;
+    for (uint32_t r = 0; r + 1 < rounds; ++r) {
+      int lo = 0, hi = 0;
+      switch (cate[r]) {
+        case 0: lo = off + 1; hi = off + 16; break;
+        case 1: lo = off + 17; hi = off + 32; break;
+        default: lo = off + 33; hi = off + 64; break;
+      }
+      if (lo > (int)S) { valid = false; break; }
+      hi = std::min(hi, (int)S);
+      double best = std::numeric_limits<double>::max();
+      int opt = -1;
+      for (int pos = lo; pos <= hi; ++pos) {
+        double gsz = n / estimate_group_num((uint32_t)pos, ncols, bw, card, order);
+        double c = single_column_cost(m, n, std::max(1.0, gsz), (int)cate[r + 1]);
+        if (c <= best) {
+          best = c;
+          opt = pos;
+        }
+      }
+      if (opt <= (int)off) { valid = false; break; }
+      plan.push_back((uint32_t)opt - off);
+      off = (uint32_t)opt;
+    }
+    if (!valid) continue;
+    uint32_t last = S - off;
+    switch (cate[rounds - 1]) {
+      case 0: valid = last > 0 && last <= 16; break;
+      case 1: valid = last > 16 && last <= 32; break;
+      default: valid = last > 32 && last <= 64; break;
+    }
+    if (valid) {
+      plan.push_back(last);
+      out.push_back(plan);
+    }
+  }
+  return out;
+}
+
+// ---- Recursive Random Search (Ye et al. 2003; src/rrs.h:80-170) ------------
+
+class PlanEnumRRS {
+ public:
+  PlanEnumRRS(const fmcs_cost_model* m, double n, uint32_t S, uint32_t max_round, uint32_t ncols,
+              const uint32_t* bw, const uint64_t* card, const uint32_t* order, uint64_t seed)
+      : m_(m), n_(n), S_(S), max_round_(max_round), ncols_(ncols), bw_(bw), card_(card), order_(order),
+        gen_(seed), dist_(0.0, 1.0) {}
+
+  // RRS<P,C>::Optimize with (p,r,q,v,c,st) = (0.99,0.1,0.99,0.8,0.5,0.001)
+  // (src/plan_enum_rrs.h:29)
+  void Optimize() {
+    const double p = 0.99, r = 0.1, q = 0.99, v = 0.8, c = 0.5, st = 0.001;
+    const size_t nsamp = (size_t)std::ceil(std::log(1 - p) / std::log(1 - r));
+    const size_t l = (size_t)std::ceil(std::log(1 - q) / std::log(1 - v));
+    opt_point_ = Sample();
+    opt_cost_ = Cost(opt_point_);
+    for (size_t i = 1; i < nsamp; ++i) {
+      Split x = Sample();
+      double cx = Cost(x);
+      if (cx < opt_cost_) {
+        opt_cost_ = cx;
+        opt_point_ = x;
+      }
+    }
+    Split x0 = opt_point_;
+    double cost0 = opt_cost_, yr = opt_cost_;
+    std::vector<double> F{opt_cost_};
+    bool exploit = true;
+    size_t i = 0;
+    while (!Stop()) {
+      if (exploit) {
+        size_t j = 0;
+        double fc = Cost(x0);
+        Split xl = x0;
+        double ro = r;
+        while (ro > st) {
+          Split x1 = Neighbor(xl, ro);
+          double c1 = Cost(x1);
+          if (c1 < fc) {
+            xl = x1;
+            fc = c1;
+            j = 0;
+          } else {
+            ++j;
+          }
+          if (j == l) {
+            ro = c * ro;
+            j = 0;
+          }
+        }
+        exploit = false;
+        if (fc < opt_cost_) {
+          opt_cost_ = fc;
+          opt_point_ = xl;
+        }
+      }
+      Split x1 = Sample();
+      double c1 = Cost(x1);
+      if (c1 < cost0) {
+        x0 = x1;
+        cost0 = c1;
+      }
+      if (c1 < yr) exploit = true;
+      if (i == nsamp) {
+        F.push_back(cost0);
+        yr = std::accumulate(F.begin(), F.end(), 0.0) / F.size();
+        i = 0;
+      }
+      ++i;
+    }
+  }
+
+  const std::set<Split>& Plans() const { return plans_; }
+  const Split& Best() const { return opt_point_; }
+  double BestCost() const { return opt_cost_; }
+
+ private:
+  double Dice() { return dist_(gen_); }
+
+  // PlanEnumRRS::GetRandomSample (src/plan_enum_rrs.cpp:37-92)
+  Split Sample() {
+    Split s;
+    for (;;) {
+      s.clear();
+      uint32_t rounds = std::min<uint32_t>((uint32_t)std::ceil(Dice() * max_round_), max_round_);
+      if (rounds == 0) rounds = 1;
+      uint32_t avail = S_, cur = 0;
+      while (cur + 1 < rounds) {
+        uint32_t w = 1 + (uint32_t)std::floor(Dice() * (double)(avail - 1));
+        w = std::min<uint32_t>(w, 64);
+        s.push_back(w);
+        avail -= w;
+        ++cur;
+        if (avail == 0) break;
+      }
+      if (avail > 0 && avail <= 64) {
+        s.push_back(avail);
+        if (s.size() == rounds) {
+          plans_.insert(s);
+          return s;
+        }
+      }
+    }
+  }
+
+  // PlanEnumRRS::GetCost (src/plan_enum_rrs.cpp:94-197): the uint64
+  // truncating cardinality variant.
+  double Cost(const Split& s) { return estimate(m_, n_, ncols_, bw_, card_, order_, s, true, nullptr); }
+
+  // PlanEnumRRS::StopCritera (src/plan_enum_rrs.cpp:199-204)
+  bool Stop() { return ++counter_ > 10; }
+
+  // PlanEnumRRS::GetRandomNeighbor (src/plan_enum_rrs.cpp:206-267): move
+  // single bits between adjacent rounds; a move off either end appends a new
+  // last round, as the reference does.
+  Split Neighbor(const Split& pt, double ro) {
+    Split t(pt.begin(), pt.end());
+    uint32_t steps = (uint32_t)std::ceil(S_ * ro);
+    while (steps > 0) {
+      int dec = (int)std::floor(Dice() * t.size());
+      if (dec >= (int)t.size()) dec = (int)t.size() - 1;
+      uint32_t coin = ((uint32_t)(100.0 * Dice())) % 2;
+      int inc = coin ? dec - 1 : dec + 1;
+      if (inc == -1 || inc == (int)t.size()) {
+        t.push_back(0);
+        inc = (int)t.size() - 1;
+      }
+      if (t[dec] > 0 && t[inc] < 64) {
+        t[dec]--;
+        t[inc]++;
+      }
+      --steps;
+    }
+    Split out;
+    for (uint32_t w : t)
+      if (w) out.push_back(w);
+    plans_.insert(out);
+    return out;
+  }
+
+  const fmcs_cost_model* m_;
+  double n_;
+  uint32_t S_, max_round_, ncols_;
+  const uint32_t* bw_;
+  const uint64_t* card_;
+  const uint32_t* order_;
+  std::mt19937_64 gen_;
+  std::uniform_real_distribution<double> dist_;
+  size_t counter_ = 0;
+  std::set<Split> plans_;
+  Split opt_point_;
+  double opt_cost_ = 0;
+};
+
+int write_plans(const std::vector<Split>& plans, uint32_t* out, uint64_t cap_words, uint64_t* nplans,
+                uint64_t* words) {
+  uint64_t w = 0;
+  for (const Split& s : plans) {
+    if (out) {
+      if (w + 1 + s.size() > cap_words) return FMCS_E_ARG;
+      out[w] = (uint32_t)s.size();
+      for (size_t i = 0; i < s.size(); ++i) out[w + 1 + i] = s[i];
+    }
+    w += 1 + s.size();
+  }
+  if (nplans) *nplans = plans.size();
+  if (words) *words = w;
+  return FMCS_OK;
+}
+
+bool inputs_ok(uint32_t ncols, const uint32_t* bw, const uint64_t* card) {
+  if (ncols == 0 || ncols > 64 || !bw || !card) return false;
+  for (uint32_t c = 0; c < ncols; ++c)
+    if (bw[c] == 0 || bw[c] > 64 || card[c] == 0) return false;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fmcs_cost_model_default(fmcs_cost_model* m) {
+  if (!m) return FMCS_E_ARG;
+  std::memset(m, 0, sizeof(*m));
+  // Per-row times measured on MI355X (2^27 rows); replaced by
+  // fmcs_calibrate() on the running device.
+  const double radix[3] = {0.0060, 0.0067, 0.0105};
+  const double gather[3] = {0.0150, 0.0170, 0.0213};
+  const double bnd[3] = {0.0045, 0.0055, 0.0075};
+  for (int b = 0; b < 3; ++b) {
+    m->radix_pass_ns[b] = radix[b];
+    m->gather_ns[b] = gather[b];
+    m->boundary_ns[b] = bnd[b];
+    for (int i = 0; i < FMCS_COST_GBUCKETS; ++i) {
+      // small groups: one LDS tile pass; large: MSD passes plus the tile
+      double base = 0.020 * (1 + 0.5 * b);
+      double msd_levels = i <= 10 ? 0 : std::ceil((i - 10) / 8.0);
+      m->segsort_ns[b][i] = base + msd_levels * radix[b] * 1.4;
+    }
+  }
+  m->stitch_ns_per_byte = 1.0 / 5000.0;  // 5 TB/s
+  m->round_overhead_ms = 0.05;
+  return FMCS_OK;
+}
+
+int fmcs_estimate_plan(const fmcs_cost_model* m, uint64_t nrows, uint32_t ncols, const uint32_t* bw,
+                       const uint64_t* card, const uint32_t* col_order, uint32_t nsplit, const uint32_t* split,
+                       int truncate_cardinality, double* est_ms, double* parts4) {
+  if (!m || !est_ms || !split || nsplit == 0 || !col_order || !inputs_ok(ncols, bw, card)) return FMCS_E_ARG;
+  uint64_t S = 0, T = 0;
+  for (uint32_t c = 0; c < ncols; ++c) S += bw[c];
+  for (uint32_t r = 0; r < nsplit; ++r) {
+    if (split[r] == 0 || split[r] > 64) return FMCS_E_ARG;
+    T += split[r];
+  }
+  if (S != T) return FMCS_E_ARG;
+  Split s(split, split + nsplit);
+  *est_ms = estimate(m, (double)nrows, ncols, bw, card, col_order, s, truncate_cardinality != 0, parts4);
+  return FMCS_OK;
+}
+
+int fmcs_split_exhaustive(uint32_t S, uint32_t rounds, uint32_t maxv, uint32_t* out, uint64_t cap_plans,
+                          uint64_t* count) {
+  if (!count) return FMCS_E_ARG;
+  std::vector<Split> v = split_exhaustive(S, rounds, maxv);
+  *count = v.size();
+  if (out)
+    for (uint64_t i = 0; i < v.size() && i < cap_plans; ++i)
+      for (uint32_t r = 0; r < rounds; ++r) out[i * rounds + r] = v[i][r];
+  return FMCS_OK;
+}
+
+int fmcs_enum_banksize_category(uint32_t rounds, uint32_t* out, uint64_t cap_plans, uint64_t* count) {
+  if (!count || rounds == 0 || rounds > 20) return FMCS_E_ARG;
+  std::vector<Split> v = bank_categories(rounds);
+  *count = v.size();
+  if (out)
+    for (uint64_t i = 0; i < v.size() && i < cap_plans; ++i)
+      for (uint32_t r = 0; r < rounds; ++r) out[i * rounds + r] = v[i][r];
+  return FMCS_OK;
+}
+
+int fmcs_greedy_plans(const fmcs_cost_model* m, uint64_t nrows, uint32_t ncols, const uint32_t* bw,
+                      const uint64_t* card, const uint32_t* col_order, uint32_t rounds, uint32_t* out,
+                      uint64_t cap_words, uint64_t* nplans, uint64_t* words) {
+  if (!m || !col_order || rounds < 1 || rounds > 20 || !inputs_ok(ncols, bw, card)) return FMCS_E_ARG;
+  return write_plans(greedy_plans(m, (double)nrows, ncols, bw, card, col_order, rounds), out, cap_words, nplans,
+                     words);
+}
+
+int fmcs_plan_enum_rrs(const fmcs_cost_model* m, uint64_t nrows, uint32_t ncols, const uint32_t* bw,
+                       const uint64_t* card, const uint32_t* col_order, uint32_t max_round, uint64_t seed,
+                       uint32_t* out, uint64_t cap_words, uint64_t* nplans, uint64_t* words) {
+  if (!m || !col_order || max_round == 0 || !inputs_ok(ncols, bw, card)) return FMCS_E_ARG;
+  uint32_t S = 0;
+  for (uint32_t c = 0; c < ncols; ++c) S += bw[c];
+  if (max_round * 64u < S) return FMCS_E_ARG;
+  PlanEnumRRS e(m, (double)nrows, S, max_round, ncols, bw, card, col_order, seed);
+  e.Optimize();
+  std::vector<Split> v(e.Plans().begin(), e.Plans().end());
+  return write_plans(v, out, cap_words, nplans, words);
+}
+
+int fmcs_optimize(const fmcs_cost_model* m, uint64_t nrows, uint32_t ncols, const uint32_t* bw,
+                  const uint64_t* card, int ordered, uint32_t* best_order, uint32_t* best_split,
+                  uint32_t* best_nsplit, double* best_cost, uint64_t* nplans_costed) {
+  if (!m || !best_order || !best_split || !best_nsplit || !best_cost || !inputs_ok(ncols, bw, card))
+    return FMCS_E_ARG;
+  uint32_t S = 0;
+  for (uint32_t c = 0; c < ncols; ++c) S += bw[c];
+  // column orders: n! permutations for GROUP BY (ordered == 0), in the
+  // swap order of StitchComposer::permutations (src/stitch_composer.cpp:15-35)
+  std::vector<Split> orders;
+  Split base(ncols);
+  for (uint32_t c = 0; c < ncols; ++c) base[c] = c;
+  if (ordered || ncols > 8) {
+    orders.push_back(base);
+  } else {
+    std::vector<uint32_t> arr = base;
+    struct Rec {
+      static void go(std::vector<uint32_t>& a, uint32_t k, uint32_t mm, std::vector<Split>& out) {
+        if (k == mm) {
+          out.push_back(a);
+          return;
+        }
+        for (uint32_t i = k; i <= mm; ++i) {
+          std::swap(a[k], a[i]);
+          go(a, k + 1, mm, out);
+          std::swap(a[k], a[i]);
+        }
+      }
+    };
+    Rec::go(arr, 0, ncols - 1, orders);
+  }
+  // max_round_num = ceil((2S - 2) / MIN_BANK_SIZE) (src/stitch_composer.cpp:3889-3890)
+  uint32_t max_round = (uint32_t)std::ceil((2.0 * S - 2.0) / 16.0);
+  max_round = std::max<uint32_t>(max_round, 1);
+  double best = std::numeric_limits<double>::max();
+  uint64_t costed = 0;
+  auto t0 = std::chrono::steady_clock::now();
+  double checkpoint = 0;
+  for (const Split& ord : orders) {
+    for (uint32_t r = 1; r <= max_round; ++r) {
+      std::vector<Split> plans = r <= 2 ? split_exhaustive(S, r, 64)
+                                        : greedy_plans(m, (double)nrows, ncols, bw, card, ord.data(), r);
+      if (plans.empty()) continue;
+      double loc = std::numeric_limits<double>::max();
+      size_t li = 0;
+      for (size_t i = 0; i < plans.size(); ++i) {
+        double c = estimate(m, (double)nrows, ncols, bw, card, ord.data(), plans[i], false, nullptr);
+        ++costed;
+        if (c < loc) {
+          loc = c;
+          li = i;
+        }
+      }
+      if (loc < best) {
+        best = loc;
+        std::copy(ord.begin(), ord.end(), best_order);
+        std::copy(plans[li].begin(), plans[li].end(), best_split);
+        *best_nsplit = (uint32_t)plans[li].size();
+      }
+      // stop once optimisation time exceeds THRESHOLD_PERC of the best
+      // plan's cost (src/stitch_composer.cpp:3946-3953)
+      double elapsed = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      if (elapsed - checkpoint > 0.01 * best) {
+        checkpoint = elapsed;
+        break;
+      }
+    }
+  }
+  *best_cost = best;
+  if (nplans_costed) *nplans_costed = costed;
+  return best < std::numeric_limits<double>::max() ? FMCS_OK : FMCS_E_ARG;
+}
+
+}  // extern "C"

@@ -1,0 +1,106 @@
+/*
+ * fmcs_plan.h -- C ABI of the host-side massaging planner and of the
+ * composer-level executor (run one plan end to end on the GPU).
+ *
+ * The planner keeps the reference's structure (src/stitch_composer.cpp,
+ * src/plan_enum_rrs.cpp, src/rrs.h); its cost constants are per-row GPU
+ * times (fmcs_cost_model) instead of the reference's CPU cycle tables
+ * (src/params.h:200-2283).  Plans are splits: new round widths, MSB first,
+ * summing to the total key width.  Plan lists are returned flattened as
+ * [len, w0, w1, ...] records.
+ */
+#ifndef FMCS_PLAN_H
+#define FMCS_PLAN_H
+
+#include <stdint.h>
+
+#include "fmcs.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FMCS_COST_GBUCKETS 32
+
+/* Per-row GPU costs in nanoseconds, by bank type (0: u16, 1: u32, 2: u64). */
+typedef struct {
+  double radix_pass_ns[3];                     /* one LSD pass of the round-1 sort */
+  double segsort_ns[3][FMCS_COST_GBUCKETS];    /* later rounds, by log2(avg group size) */
+  double gather_ns[3];                         /* oid gather (tuple reconstruction) */
+  double boundary_ns[3];                       /* group boundary extraction */
+  double stitch_ns_per_byte;                   /* fused stitch, per byte moved */
+  double round_overhead_ms;                    /* fixed cost per round */
+} fmcs_cost_model;
+
+int fmcs_cost_model_default(fmcs_cost_model* m);
+
+/* Estimated cost of one plan (estimate_plan_collection_greedy's per-plan
+ * costing, src/stitch_composer.cpp:2228-2378; truncate_cardinality selects
+ * PlanEnumRRS::GetCost's uint64 cardinalities, src/plan_enum_rrs.cpp:129-153).
+ * parts4 (optional): stitch, tuple reconstruction, sort, group extraction. */
+int fmcs_estimate_plan(const fmcs_cost_model* m, uint64_t nrows, uint32_t ncols, const uint32_t* bw,
+                       const uint64_t* card, const uint32_t* col_order, uint32_t nsplit, const uint32_t* split,
+                       int truncate_cardinality, double* est_ms, double* parts4);
+
+/* split_exhaustive_certain_round (src/stitch_composer.cpp:205-230): all
+ * compositions of S into `rounds` parts <= maxv, descending-first; writes up
+ * to cap_plans plans of `rounds` values, *count = total. */
+int fmcs_split_exhaustive(uint32_t S, uint32_t rounds, uint32_t maxv, uint32_t* out, uint64_t cap_plans,
+                          uint64_t* count);
+
+/* enum_banksize_category (src/stitch_composer.cpp:1717-1734): the 3^rounds
+ * bank patterns (0: 16, 1: 32, 2: 64 bit), lexicographic. */
+int fmcs_enum_banksize_category(uint32_t rounds, uint32_t* out, uint64_t cap_plans, uint64_t* count);
+
+/* find_greedy_plans (src/stitch_composer.cpp:1992-2125): one plan per valid
+ * bank pattern. */
+int fmcs_greedy_plans(const fmcs_cost_model* m, uint64_t nrows, uint32_t ncols, const uint32_t* bw,
+                      const uint64_t* card, const uint32_t* col_order, uint32_t rounds, uint32_t* out,
+                      uint64_t cap_words, uint64_t* nplans, uint64_t* words);
+
+/* PlanEnumRRS + RRS::Optimize (src/plan_enum_rrs.cpp:37-267, src/rrs.h:80-170)
+ * with an explicit seed (the reference seeds with time(0), src/rrs.h:68):
+ * the set of visited plans, ascending. */
+int fmcs_plan_enum_rrs(const fmcs_cost_model* m, uint64_t nrows, uint32_t ncols, const uint32_t* bw,
+                       const uint64_t* card, const uint32_t* col_order, uint32_t max_round, uint64_t seed,
+                       uint32_t* out, uint64_t cap_words, uint64_t* nplans, uint64_t* words);
+
+/* StitchComposer::optimizeAndRun's search (src/stitch_composer.cpp:
+ * 3839-3985): column orders (all permutations when ordered == 0), rounds
+ * 1..ceil((2S-2)/16), exhaustive for <= 2 rounds, greedy beyond, early stop
+ * at 1% of the best cost.  Returns the best order and split. */
+int fmcs_optimize(const fmcs_cost_model* m, uint64_t nrows, uint32_t ncols, const uint32_t* bw,
+                  const uint64_t* card, int ordered, uint32_t* best_order, uint32_t* best_split,
+                  uint32_t* best_nsplit, double* best_cost, uint64_t* nplans_costed);
+
+/* ---- composer-level execution (needs a context) -------------------- */
+
+/* StitchComposer::run_single_plan (src/stitch_composer.cpp:555-806):
+ * construct entries, stitch the columns into new banks, then sort all
+ * rounds (ChainComposer::SortAllColumns -> multiRoundSorting).  order_d
+ * receives the final oid permutation; groups_d (optional) the final group
+ * ids; ms6 (optional) the reference's breakdown: [0] stitching,
+ * [1] tuple reconstruct, [2] order/group info, [3] all sorting rounds,
+ * [4] first-round sort, [5] group extraction; *total_ms the whole plan. */
+int fmcs_run_plan(fmcs_ctx* ctx, uint32_t ncols, const void* const* cols_d, const uint32_t* banks,
+                  const uint32_t* bw, const uint32_t* col_order, uint32_t nsplit, const uint32_t* split, int mode,
+                  uint64_t nrows, uint32_t* order_d, uint32_t* groups_d, double* ms6, double* total_ms,
+                  uint32_t* ngroups);
+
+/* StitchComposer::run_single_plan_hash (src/stitch_composer.cpp:808-1079):
+ * the same with u8 banks allowed and hash grouping per round. */
+int fmcs_run_plan_hash(fmcs_ctx* ctx, uint32_t ncols, const void* const* cols_d, const uint32_t* banks,
+                       const uint32_t* bw, const uint32_t* col_order, uint32_t nsplit, const uint32_t* split,
+                       int hash_type, uint64_t nrows, uint32_t* order_d, uint32_t* gid_d, double* total_ms,
+                       uint32_t* ngroups);
+
+/* Measures the cost model on this device with synthetic banks of nrows rows
+ * (radix passes, gather, boundary, stitch, and segmented sorts over
+ * groups of 2^k rows). */
+int fmcs_calibrate(fmcs_ctx* ctx, uint64_t nrows, fmcs_cost_model* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FMCS_PLAN_H */
