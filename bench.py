@@ -91,6 +91,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample-log2", type=int, default=23)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one plain 256 MiB device copy (the traffic reference for PMC counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -110,6 +112,12 @@ def main():
     d2 = torch.from_numpy(c2).cuda()
     order = torch.empty(n, dtype=torch.uint32, device="cuda")
     ctx = Context(0)
+    if args.copy_probe:
+        a = torch.empty(1 << 28, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy probe: 268435456 bytes read + 268435456 bytes written")
 
     def step(mode=MODE_BATCHED):
         ctx.multi_round_sort([d1, d2], bws, order=order, mode=mode)

@@ -621,9 +621,18 @@ __global__ void k_large_flags(const u32* __restrict__ sz, u32 nseg, u32 small_ma
     a += __shfl_down(a, o);
     b += __shfl_down(b, o);
   }
+  // one pair of global atomics per block
+  __shared__ u32 s_ab[2];
+  if (threadIdx.x == 0) s_ab[0] = s_ab[1] = 0;
+  __syncthreads();
   if ((threadIdx.x & (WAVE - 1)) == 0) {
-    if (a) atomicAdd(rows, a);
-    if (b) atomicAdd(rows + 1, b);
+    if (a) atomicAdd(&s_ab[0], a);
+    if (b) atomicAdd(&s_ab[1], b);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s_ab[0]) atomicAdd(rows, s_ab[0]);
+    if (s_ab[1]) atomicAdd(rows + 1, s_ab[1]);
   }
 }
 

@@ -1,0 +1,80 @@
+"""Composer-level execution on the GPU: every massaging plan gives the
+same result (the plans re-cut one bit string), equal to the oracle."""
+import numpy as np
+import pytest
+import torch
+
+import oracle_bind as ob
+from fastmulticolsort_amd import datagen, planner
+from fastmulticolsort_amd.engine import MODE_BATCHED, MODE_UNBINNED
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def test_two_round_sweep_all_plans_identical(ctx):
+    """TwoRoundsExhaustive (src/stitch_composer.cpp:4197-4258): shift bits
+    s in [-w2, w1], split (w1-s, w2+s) without zero widths."""
+    n = 1 << 16
+    (c1, c2), bws = datagen.config1(n)
+    exp = np.lexsort((c2, c1)).astype(np.uint32)
+    cols = [dev(c1), dev(c2)]
+    for s in range(-bws[1], bws[0] + 1):
+        split = [w for w in (bws[0] - s, bws[1] + s) if w > 0]
+        order, bd, tot, ng = planner.run_plan(ctx, cols, bws, [0, 1], split)
+        assert np.array_equal(order.cpu().numpy(), exp), split
+        assert tot > 0 and bd[0] >= 0
+
+
+@pytest.mark.parametrize("mode", [MODE_BATCHED, MODE_UNBINNED])
+def test_config4_plans_match_oracle(ctx, mode):
+    n = 1 << 18
+    cols, bws = datagen.config4(n)
+    stacked = [c.astype(np.uint64) for c in cols]
+    exp = np.lexsort(tuple(reversed(stacked))).astype(np.uint32)
+    dcols = [dev(c) for c in cols]
+    m = planner.default_cost_model()
+    card = [3, 2, 2526, 1 << 27]
+    plans = [[42], [21, 21], [3, 12, 27], [16, 16, 10]] + \
+        planner.plan_enum_rrs(m, n, bws, card, [0, 1, 2, 3], 6, 1)[:12]
+    for split in plans:
+        if mode == MODE_UNBINNED and len(split) > 1 and max(split[1:]) > 40:
+            continue
+        order, _, _, _ = planner.run_plan(ctx, dcols, bws, [0, 1, 2, 3], split, mode=mode)
+        assert np.array_equal(order.cpu().numpy(), exp), split
+
+
+def test_plan_groups_are_final_groups(ctx):
+    n = 1 << 15
+    (c1, c2), bws = datagen.config1(n)
+    groups = torch.empty(n, dtype=torch.uint32, device="cuda")
+    order, _, _, ng = planner.run_plan(ctx, [dev(c1), dev(c2)], bws, [0, 1], [25, 25], groups=groups)
+    o = order.cpu().numpy()
+    key = (c1.astype(np.uint64) << np.uint64(33)) | c2
+    ks = key[o]
+    exp = np.r_[0, np.cumsum(ks[1:] != ks[:-1])].astype(np.uint32)
+    assert np.array_equal(groups.cpu().numpy(), exp) and ng == len(np.unique(key))
+
+
+def test_hash_plans_count_groups_exactly(ctx):
+    n = 1 << 18
+    cols, bws = datagen.config4(n)
+    dcols = [dev(c) for c in cols]
+    stacked = np.stack([c.astype(np.uint64) for c in cols], axis=1)
+    nd = len(np.unique(stacked, axis=0))
+    for split in ([42], [3, 12, 27], [2, 1, 12, 27], [8, 8, 26]):
+        for ht in (0, 1):
+            order, gid, tot, ng = planner.run_plan_hash(ctx, dcols, bws, [0, 1, 2, 3], split, hash_type=ht)
+            assert ng == nd, (split, ht)
+
+
+def test_calibrate_produces_positive_costs(ctx):
+    m = planner.calibrate(ctx, 1 << 20)
+    d = m.to_dict()
+    for b in range(3):
+        assert d["radix_pass_ns"][b] > 0 and d["gather_ns"][b] > 0 and d["boundary_ns"][b] > 0
+        assert all(x > 0 for x in d["segsort_ns"][b][:21])
+    assert d["stitch_ns_per_byte"] > 0
