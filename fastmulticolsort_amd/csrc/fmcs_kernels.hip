@@ -404,11 +404,12 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
 // item of the wave) into the wave's LDS digit counters.
 template <int E>
 __device__ __forceinline__ void wave_rank(const u32 (&dig)[E], const bool (&ok)[E], u32 dbits, u32* whist,
-                                          u32 (&rank)[E]) {
+                                          u32 (&rank)[E], u32 jn = E) {
   const u32 lane = __lane_id();
   const u64 lt = lanemask_lt();
 #pragma unroll
   for (int j = 0; j < E; ++j) {
+    if ((u32)j >= jn) break;  // wave-uniform: no item row beyond jn holds a row
     u32 d = dig[j];
     u64 peers = __ballot(ok[j]);
     for (u32 b = 0; b < dbits; ++b) {
@@ -453,7 +454,7 @@ __device__ __forceinline__ void block_or_and(u64 v_or, u64 v_and, u64* s_red, u6
 
 template <class K, int E>
 __global__ __launch_bounds__(256) void k_tile_sort(TileSortArgs<K> a) {
-  constexpr int NT = 256, NW = NT / WAVE, T = NT * E, WE = WAVE * E;
+  constexpr int NT = 256, NW = NT / WAVE, T = NT * E;
   __shared__ u64 s_c[T];
   __shared__ u32 s_o[T];
   __shared__ u64 s_top[T];
@@ -520,6 +521,11 @@ __global__ __launch_bounds__(256) void k_tile_sort(TileSortArgs<K> a) {
   u32 segbits = 0;
   while (segbits < 32 && (1u << segbits) < nsegs) ++segbits;
   const u32 B = kb + segbits;
+  // Rows are spread over all waves: Ee items per lane, so the ranking work
+  // follows the rows actually in the tile, not its capacity.
+  const u32 Ee = (cnt + NT - 1) / NT;
+  const u32 WEe = WAVE * Ee;
+  const u32 jn = (cnt > wv * WEe) ? min(Ee, (cnt - wv * WEe + WAVE - 1) / WAVE) : 0u;
   for (u32 shift = 0; shift < B; shift += RADIX_BITS) {
     const u32 dbits = min((u32)RADIX_BITS, B - shift);
     const u32 dmask = (1u << dbits) - 1;
@@ -529,15 +535,15 @@ __global__ __launch_bounds__(256) void k_tile_sort(TileSortArgs<K> a) {
     bool ok[E];
 #pragma unroll
     for (int j = 0; j < E; ++j) {
-      u32 idx = wv * WE + j * WAVE + lane;
-      ok[j] = idx < cnt;
+      u32 idx = wv * WEe + j * WAVE + lane;
+      ok[j] = (u32)j < Ee && idx < cnt;
       c[j] = ok[j] ? s_c[idx] : 0ull;
       o[j] = ok[j] ? s_o[idx] : 0u;
       dig[j] = (u32)(c[j] >> shift) & dmask;
     }
     for (u32 i = tid; i < (u32)(NW * RADIX); i += NT) (&s_whist[0][0])[i] = 0;
     __syncthreads();
-    wave_rank<E>(dig, ok, dbits, s_whist[wv], rank);
+    wave_rank<E>(dig, ok, dbits, s_whist[wv], rank, jn);
     __syncthreads();
     {
       u32 d = tid, r = 0;
