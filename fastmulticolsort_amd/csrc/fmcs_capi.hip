@@ -22,7 +22,7 @@ enum Slot {
   WS_KA, WS_KB, WS_VB, WS_T0K, WS_T0V, WS_T1K, WS_T1V, WS_STATUS, WS_HIST, WS_DBASE, WS_SEGOFF, WS_SEGOFF2,
   WS_ST, WS_SZ, WS_BST, WS_BSZ, WS_FLAG, WS_POS, WS_LLO, WS_LHI, WS_LTILES, WS_TBASE, WS_TLO, WS_THI, WS_TS0,
   WS_TS1, WS_COMP, WS_COMP2, WS_GROUPTMP, WS_MISC, WS_HASH, WS_HASH2, WS_HASH3, WS_HT_TABLE, WS_HT_FIRST,
-  WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_USER0, WS_COUNT = WS_USER0 + 24
+  WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_USER0, WS_COUNT = WS_USER0 + 24
 };
 
 struct Buf {
@@ -409,7 +409,19 @@ int seg_sort_batched(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, cons
     a.kout = keys; a.vout = vals;
     a.tile_lo = tlo; a.tile_hi = thi; a.tile_s0 = ts0; a.tile_s1 = ts1; a.seg_start = cst;
     a.src_sel = level & 1; a.kb = tkb; a.pad_seg = T; a.direct = 0;
-    TRY(tile_sort_launch<K>(ctx, a, (u32)nw, (double)tile_rows_est));
+    {
+      // bucket fast path; tiles it refuses go to the radix tile sort
+      u32* fb;
+      TRY(getbuf_t(ctx, WS_FALLBACK, nw, &fb));
+      TRY(fill(ctx, fb, nw, 0));
+      if (nw) {
+        Prof pf(ctx, "k_tile_bucket", 2.0 * (double)tile_rows_est * (sizeof(K) + 4));
+        hipLaunchKernelGGL((k_tile_bucket<K, TILESORT_E>), dim3((u32)nw), dim3(256), 0, ctx->stream, a, fb);
+        TRY(launch_check(ctx, "k_tile_bucket"));
+      }
+      a.only = fb;
+      TRY(tile_sort_launch<K>(ctx, a, (u32)nw, 0.0));
+    }
     if (nlarge == 0) break;
     u32 *llo, *lhi, *ltiles, *tbase;
     TRY(getbuf_t(ctx, WS_LLO, nlarge, &llo));

@@ -469,6 +469,7 @@ __global__ __launch_bounds__(256) void k_tile_sort(TileSortArgs<K> a) {
     lo = a.d_lo; hi = a.d_hi; s0 = 1; s1 = 0;
   } else {
     u32 w = blockIdx.x;
+    if (a.only && a.only[w] == 0) return;
     lo = a.tile_lo[w]; hi = a.tile_hi[w]; s0 = a.tile_s0[w]; s1 = a.tile_s1[w];
   }
   if (lo >= hi) return;
@@ -572,6 +573,178 @@ __global__ __launch_bounds__(256) void k_tile_sort(TileSortArgs<K> a) {
     if (kb >= 64) sg = 0;
     u64 k = s_top[sg] | (c & kmask);
     if (a.kout) a.kout[lo + i] = (K)k;
+    a.vout[lo + i] = s_o[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fast path of the tile sort: a per-segment bucket sort.  Segment s of m_s
+// rows gets 2^ceil(log2 m_s) bins (about one row per bin) cut from the top of
+// its unsorted key bits; bins are laid out in segment order, so a row's final
+// place is the offset of its bin plus its rank inside the bin by (key, oid).
+// Rows are linked into their bin's list with one LDS exchange; each thread
+// then owns TB consecutive bins and ranks their (few) rows by comparison.
+// The cost no longer grows with the key width.  A tile whose largest bin
+// exceeds BUCKET_LIMIT rows (heavy duplicates or skew) is flagged in
+// `fallback` and left untouched for the radix tile sort.
+// ---------------------------------------------------------------------------
+constexpr u32 BUCKET_LIMIT = 24;
+constexpr u16 LIST_END = 0xffff;
+
+template <class K, int E>
+__global__ __launch_bounds__(256) void k_tile_bucket(TileSortArgs<K> a, u32* __restrict__ fallback) {
+  constexpr int NT = 256, T = NT * E, NB = 2 * T, TB = NB / NT;
+  __shared__ u64 s_k[T];
+  __shared__ u32 s_o[T];
+  __shared__ u32 s_head[NB];
+  __shared__ u16 s_next[T];
+  __shared__ u16 s_pos[T];
+  __shared__ u16 s_segstart[T + 1];
+  __shared__ u16 s_segbase[T];
+  __shared__ uint8_t s_lb[T];
+  __shared__ uint8_t s_f[T];
+  __shared__ u32 s_tmp[NT / WAVE + 1];
+  __shared__ u32 s_over;
+  const u32 tid = threadIdx.x;
+  const u32 w = blockIdx.x;
+  const u32 lo = a.tile_lo[w], hi = a.tile_hi[w];
+  if (lo >= hi) return;
+  const u32 s0 = a.tile_s0[w], s1 = a.tile_s1[w];
+  const u32 cnt = hi - lo;
+  const u32 kb = a.kb;
+  const u64 kmask = kb >= 64 ? ~0ull : ((1ull << kb) - 1);
+  const K* kin = a.kin[a.src_sel];
+  const u32* vin = a.vin[a.src_sel];
+  for (u32 i = tid; i < (u32)T; i += NT) s_f[i] = 0;
+  for (u32 i = tid; i < (u32)NB; i += NT) s_head[i] = LIST_END;
+  for (u32 i = tid; i < cnt; i += NT) {
+    s_k[i] = (u64)kin[lo + i];
+    s_o[i] = vin[lo + i];
+  }
+  if (tid == 0) s_over = 0;
+  __syncthreads();
+  for (u32 s = s0 + tid; s <= s1; s += NT) {
+    u32 st = a.seg_start[s];
+    if (st >= lo && st < hi) s_f[st - lo] = 1;
+  }
+  if (tid == 0) s_f[0] = 1;
+  __syncthreads();
+  // local segment of every row (blocked), segment starts
+  u32 f[E], sg[E];
+  u32 sum = 0;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    f[e] = s_f[tid * E + e];
+    sum += f[e];
+  }
+  u32 nsegs;
+  u32 run = block_excl_scan<NT>(sum, s_tmp, &nsegs);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    run += f[e];
+    sg[e] = run - 1;
+    u32 p = tid * E + e;
+    if (f[e] && p < cnt) s_segstart[sg[e]] = (u16)p;
+  }
+  if (tid == 0) s_segstart[nsegs] = (u16)cnt;
+  __syncthreads();
+  // bins per segment: 2^lb >= m, laid out in segment order
+  constexpr int SE = T / NT;  // segments per thread (blocked)
+  u32 bsum = 0;
+  u32 lbs[SE];
+#pragma unroll
+  for (int e = 0; e < SE; ++e) {
+    u32 s = tid * SE + e;
+    u32 lb = 0;
+    if (s < nsegs) {
+      u32 m = (u32)s_segstart[s + 1] - (u32)s_segstart[s];
+      lb = m <= 1 ? 0u : 32u - __clz(m - 1);
+      bsum += 1u << lb;
+    }
+    lbs[e] = lb;
+  }
+  u32 btot;
+  u32 brun = block_excl_scan<NT>(bsum, s_tmp, &btot);
+#pragma unroll
+  for (int e = 0; e < SE; ++e) {
+    u32 s = tid * SE + e;
+    if (s < nsegs) {
+      s_lb[s] = (uint8_t)lbs[e];
+      s_segbase[s] = (u16)brun;
+      brun += 1u << lbs[e];
+    }
+  }
+  __syncthreads();
+  // link every row into its bin
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    u32 p = tid * E + e;
+    if (p < cnt) {
+      u64 km = s_k[p] & kmask;
+      u32 lb = s_lb[sg[e]];
+      u32 off = 0;
+      if (lb) off = kb >= lb ? (u32)(km >> (kb - lb)) : (u32)(km << (lb - kb));
+      u32 bin = (u32)s_segbase[sg[e]] + off;
+      u32 old = atomicExch(&s_head[bin], p);
+      s_next[p] = (u16)old;
+    }
+  }
+  __syncthreads();
+  // count the rows of this thread's bins; refuse oversized bins
+  u32 mine = 0;
+  for (int b = 0; b < TB; ++b) {
+    u32 m = 0;
+    for (u32 q = s_head[tid * TB + b]; q != LIST_END; q = s_next[q]) ++m;
+    if (m > BUCKET_LIMIT) s_over = 1;
+    mine += m;
+  }
+  u32 tot;
+  u32 pos0 = block_excl_scan<NT>(mine, s_tmp, &tot);
+  if (s_over) {
+    if (tid == 0) fallback[w] = 1;
+    return;
+  }
+  // rank inside each bin by (key, oid)
+  for (int b = 0; b < TB; ++b) {
+    u32 m = 0;
+    for (u32 q = s_head[tid * TB + b]; q != LIST_END; q = s_next[q]) {
+      u64 kq = s_k[q];
+      u32 oq = s_o[q];
+      u32 r = 0;
+      for (u32 t = s_head[tid * TB + b]; t != LIST_END; t = s_next[t]) {
+        u64 kt = s_k[t];
+        r += (kt < kq) || (kt == kq && s_o[t] < oq);
+      }
+      s_pos[q] = (u16)(pos0 + r);
+      ++m;
+    }
+    pos0 += m;
+  }
+  __syncthreads();
+  // move rows to their places, then write the tile out coalesced
+  u64 kk[E];
+  u32 oo[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    u32 p = tid * E + e;
+    if (p < cnt) {
+      kk[e] = s_k[p];
+      oo[e] = s_o[p];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    u32 p = tid * E + e;
+    if (p < cnt) {
+      u32 d = s_pos[p];
+      s_k[d] = kk[e];
+      s_o[d] = oo[e];
+    }
+  }
+  __syncthreads();
+  for (u32 i = tid; i < cnt; i += NT) {
+    if (a.kout) a.kout[lo + i] = (K)s_k[i];
     a.vout[lo + i] = s_o[i];
   }
 }
@@ -829,6 +1002,7 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
   template __global__ void k_onesweep<K, 16>(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*,    \
                                              u32, const u32*, u32*, u32*, u32, u32);                                \
   template __global__ void k_tile_sort<K, 8>(TileSortArgs<K>);                                                      \
+  template __global__ void k_tile_bucket<K, 8>(TileSortArgs<K>, u32*);                                                      \
   template __global__ void k_copy_segs<K>(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, u32,  \
                                           u32);                                                                     \
   template __global__ void k_composite<K>(const K*, const u32*, u64, u32, u64*);                                    \
