@@ -41,7 +41,7 @@ struct fmcs_ctx {
   Buf ws[WS_COUNT];
   u32* small_d = nullptr;   // 64 words of device scratch (counters, totals, consts)
   u32* host_pin = nullptr;  // PIN_WORDS words pinned for readbacks
-  fmcs_tuning tune = {1024, 1024, 8};
+  fmcs_tuning tune = {512, 512, 8};
   hipEvent_t ev[8] = {};
   double breakdown[4] = {0, 0, 0, 0};
   // per-kernel event timing (off unless enabled)
@@ -332,7 +332,7 @@ int tile_sort_launch(fmcs_ctx* ctx, TileSortArgs<K> a, u32 grid, double rows) {
   return launch_check(ctx, "k_tile_sort");
 }
 
-constexpr u32 TS_SEGBITS = 12;  // TILESORT_T = 2048 < 2^11, pad index 2048 needs 12 bits
+constexpr u32 TS_SEGBITS = 11;  // local segment index of a TILESORT_T = 1024-row tile, with headroom
 
 // ------------------------------------------------------------------------
 // Batched, size-binned segmented sort of (keys, vals) within segments.

@@ -46,7 +46,7 @@ constexpr int ONESWEEP_ITEMS = 16;
 constexpr int ONESWEEP_TILE = 256 * ONESWEEP_ITEMS;
 constexpr int BOUNDARY_ITEMS = 16;
 constexpr int BOUNDARY_TILE = 256 * BOUNDARY_ITEMS;
-constexpr int TILESORT_E = 8;
+constexpr int TILESORT_E = 4;
 constexpr int TILESORT_T = 256 * TILESORT_E;
 constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = 256 * SCAN_ITEMS;

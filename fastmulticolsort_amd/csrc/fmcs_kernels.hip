@@ -580,25 +580,25 @@ __global__ __launch_bounds__(256) void k_tile_sort(TileSortArgs<K> a) {
 // ---------------------------------------------------------------------------
 // Fast path of the tile sort: a per-segment bucket sort.  Segment s of m_s
 // rows gets 2^ceil(log2 m_s) bins (about one row per bin) cut from the top of
-// its unsorted key bits; bins are laid out in segment order, so a row's final
-// place is the offset of its bin plus its rank inside the bin by (key, oid).
-// Rows are linked into their bin's list with one LDS exchange; each thread
-// then owns TB consecutive bins and ranks their (few) rows by comparison.
-// The cost no longer grows with the key width.  A tile whose largest bin
-// exceeds BUCKET_LIMIT rows (heavy duplicates or skew) is flagged in
-// `fallback` and left untouched for the radix tile sort.
+// its unsorted key bits; bins are laid out in segment order.  Rows are
+// counted into bins (LDS atomics), the counts scanned, rows scattered into
+// their bin's range, and every row then ranks itself inside its (short,
+// contiguous) bin by (key, oid): its final place is bin start + rank.  The
+// work no longer grows with the key width and has no dependent LDS chains.
+// A tile whose largest bin exceeds BUCKET_LIMIT rows (heavy duplicates or
+// skew) is flagged in `fallback` and left to the radix tile sort.
 // ---------------------------------------------------------------------------
 constexpr u32 BUCKET_LIMIT = 24;
-constexpr u16 LIST_END = 0xffff;
 
 template <class K, int E>
 __global__ __launch_bounds__(256) void k_tile_bucket(TileSortArgs<K> a, u32* __restrict__ fallback) {
-  constexpr int NT = 256, T = NT * E, NB = 2 * T, TB = NB / NT;
+  constexpr int NT = 256, T = NT * E, NB = 2 * T, BPT = NB / NT, SE = T / NT;
   __shared__ u64 s_k[T];
   __shared__ u32 s_o[T];
-  __shared__ u32 s_head[NB];
-  __shared__ u16 s_next[T];
-  __shared__ u16 s_pos[T];
+  __shared__ u64 s_k2[T];
+  __shared__ u32 s_o2[T];
+  __shared__ u32 s_cnt[NB];
+  __shared__ u32 s_off[NB];
   __shared__ u16 s_segstart[T + 1];
   __shared__ u16 s_segbase[T];
   __shared__ uint8_t s_lb[T];
@@ -616,7 +616,7 @@ __global__ __launch_bounds__(256) void k_tile_bucket(TileSortArgs<K> a, u32* __r
   const K* kin = a.kin[a.src_sel];
   const u32* vin = a.vin[a.src_sel];
   for (u32 i = tid; i < (u32)T; i += NT) s_f[i] = 0;
-  for (u32 i = tid; i < (u32)NB; i += NT) s_head[i] = LIST_END;
+  for (u32 i = tid; i < (u32)NB; i += NT) s_cnt[i] = 0;
   for (u32 i = tid; i < cnt; i += NT) {
     s_k[i] = (u64)kin[lo + i];
     s_o[i] = vin[lo + i];
@@ -649,7 +649,6 @@ __global__ __launch_bounds__(256) void k_tile_bucket(TileSortArgs<K> a, u32* __r
   if (tid == 0) s_segstart[nsegs] = (u16)cnt;
   __syncthreads();
   // bins per segment: 2^lb >= m, laid out in segment order
-  constexpr int SE = T / NT;  // segments per thread (blocked)
   u32 bsum = 0;
   u32 lbs[SE];
 #pragma unroll
@@ -675,53 +674,45 @@ __global__ __launch_bounds__(256) void k_tile_bucket(TileSortArgs<K> a, u32* __r
     }
   }
   __syncthreads();
-  // link every row into its bin
+  // bin of every row, and the bin counts
+  u32 bin[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     u32 p = tid * E + e;
+    bin[e] = 0;
     if (p < cnt) {
       u64 km = s_k[p] & kmask;
       u32 lb = s_lb[sg[e]];
       u32 off = 0;
       if (lb) off = kb >= lb ? (u32)(km >> (kb - lb)) : (u32)(km << (lb - kb));
-      u32 bin = (u32)s_segbase[sg[e]] + off;
-      u32 old = atomicExch(&s_head[bin], p);
-      s_next[p] = (u16)old;
+      bin[e] = (u32)s_segbase[sg[e]] + off;
+      atomicAdd(&s_cnt[bin[e]], 1u);
     }
   }
   __syncthreads();
-  // count the rows of this thread's bins; refuse oversized bins
-  u32 mine = 0;
-  for (int b = 0; b < TB; ++b) {
-    u32 m = 0;
-    for (u32 q = s_head[tid * TB + b]; q != LIST_END; q = s_next[q]) ++m;
-    if (m > BUCKET_LIMIT) s_over = 1;
-    mine += m;
+  // exclusive scan of the counts (BPT consecutive bins per thread)
+  u32 c[BPT];
+  u32 csum = 0, cmax = 0;
+#pragma unroll
+  for (int j = 0; j < BPT; ++j) {
+    c[j] = s_cnt[tid * BPT + j];
+    csum += c[j];
+    cmax = max(cmax, c[j]);
   }
+  if (cmax > BUCKET_LIMIT) s_over = 1;
   u32 tot;
-  u32 pos0 = block_excl_scan<NT>(mine, s_tmp, &tot);
+  u32 crun = block_excl_scan<NT>(csum, s_tmp, &tot);
   if (s_over) {
     if (tid == 0) fallback[w] = 1;
     return;
   }
-  // rank inside each bin by (key, oid)
-  for (int b = 0; b < TB; ++b) {
-    u32 m = 0;
-    for (u32 q = s_head[tid * TB + b]; q != LIST_END; q = s_next[q]) {
-      u64 kq = s_k[q];
-      u32 oq = s_o[q];
-      u32 r = 0;
-      for (u32 t = s_head[tid * TB + b]; t != LIST_END; t = s_next[t]) {
-        u64 kt = s_k[t];
-        r += (kt < kq) || (kt == kq && s_o[t] < oq);
-      }
-      s_pos[q] = (u16)(pos0 + r);
-      ++m;
-    }
-    pos0 += m;
+#pragma unroll
+  for (int j = 0; j < BPT; ++j) {
+    s_off[tid * BPT + j] = crun;
+    crun += c[j];
   }
   __syncthreads();
-  // move rows to their places, then write the tile out coalesced
+  // scatter rows into their bins (order inside a bin is arbitrary)
   u64 kk[E];
   u32 oo[E];
 #pragma unroll
@@ -730,16 +721,25 @@ __global__ __launch_bounds__(256) void k_tile_bucket(TileSortArgs<K> a, u32* __r
     if (p < cnt) {
       kk[e] = s_k[p];
       oo[e] = s_o[p];
+      u32 slot = atomicAdd(&s_off[bin[e]], 1u);
+      s_k2[slot] = kk[e];
+      s_o2[slot] = oo[e];
     }
   }
   __syncthreads();
+  // rank inside the bin: [end - count, end) after the scatter
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     u32 p = tid * E + e;
     if (p < cnt) {
-      u32 d = s_pos[p];
-      s_k[d] = kk[e];
-      s_o[d] = oo[e];
+      u32 end = s_off[bin[e]], start = end - s_cnt[bin[e]];
+      u32 r = start;
+      for (u32 q = start; q < end; ++q) {
+        u64 kq = s_k2[q];
+        r += (kq < kk[e]) || (kq == kk[e] && s_o2[q] < oo[e]);
+      }
+      s_k[r] = kk[e];
+      s_o[r] = oo[e];
     }
   }
   __syncthreads();
@@ -1001,8 +1001,8 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
   template __global__ void k_seg_hist<K, 16>(const K*, const u32*, const u32*, const u32*, u32, u32, u32, u32*);     \
   template __global__ void k_onesweep<K, 16>(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*,    \
                                              u32, const u32*, u32*, u32*, u32, u32);                                \
-  template __global__ void k_tile_sort<K, 8>(TileSortArgs<K>);                                                      \
-  template __global__ void k_tile_bucket<K, 8>(TileSortArgs<K>, u32*);                                                      \
+  template __global__ void k_tile_sort<K, 4>(TileSortArgs<K>);                                                      \
+  template __global__ void k_tile_bucket<K, 4>(TileSortArgs<K>, u32*);                                                      \
   template __global__ void k_copy_segs<K>(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, u32,  \
                                           u32);                                                                     \
   template __global__ void k_composite<K>(const K*, const u32*, u64, u32, u64*);                                    \
