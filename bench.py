@@ -91,6 +91,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample-log2", type=int, default=23)
+    ap.add_argument("--autotune", action="store_true",
+                    help="sweep the batched sort's bin thresholds and tile shapes first and save the best")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one plain 256 MiB device copy (the traffic reference for PMC counters)")
     args = ap.parse_args()
@@ -121,6 +123,16 @@ def main():
 
     def step(mode=MODE_BATCHED):
         ctx.multi_round_sort([d1, d2], bws, order=order, mode=mode)
+
+    from fastmulticolsort_amd import autotune
+    tuning = None
+    if args.autotune:
+        best, table = autotune.tune(ctx, step, log=log)
+        autotune.save(best, table, f"config3 2^{args.rows_log2} rows")
+        tuning = best
+    else:
+        tuning = autotune.load(ctx)
+    log(f"tuning: {tuning or ctx.tuning()}")
 
     for _ in range(args.warmup):
         step()
@@ -201,6 +213,7 @@ def main():
                         "batched_vs_plain": (round(plain_ms / ms, 2) if plain_ms else None),
                         "batched_vs_unbinned": round(unb_ms / ms, 3)},
         "breakdown_ms": bd,
+        "tuning": tuning or ctx.tuning(),
         "kernels": kernels,
     }
     print(json.dumps(out), flush=True)

@@ -1,0 +1,67 @@
+"""Autotuner for the batched segmented sort.
+
+Sweeps the bin thresholds and tile shapes of fmcs_tuning (small_max: the
+largest segment packed into an LDS tile, which also picks the 512- or
+1024-row tile; window: rows per packing window; msd_bits: digit width of an
+MSD level) on a workload, timing whole multi-round sorts with HIP events,
+and writes the fastest setting as JSON.  Results never depend on the
+tuning, only the time does.
+"""
+import itertools
+import json
+import os
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_PATH = os.path.join(HERE, "tuning_mi355x.json")
+
+GRID = {
+    "small_max": [128, 256, 512],
+    "window_frac": [1.0, 0.5],
+    "msd_bits": [6, 7, 8],
+}
+
+
+def candidates(grid=GRID):
+    for sm, wf, mb in itertools.product(grid["small_max"], grid["window_frac"], grid["msd_bits"]):
+        yield {"small_max": sm, "window": max(1, int(sm * wf)), "msd_bits": mb}
+
+
+def time_setting(ctx, run, setting, reps=3):
+    import torch
+    ctx.set_tuning(setting["small_max"], setting["window"], setting["msd_bits"])
+    run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / reps
+
+
+def tune(ctx, run, grid=GRID, reps=3, log=None):
+    """run(): one multi-round sort on resident data.  Returns (best, table)."""
+    table = []
+    for st in candidates(grid):
+        ms = time_setting(ctx, run, st, reps)
+        table.append(dict(st, ms=round(ms, 4)))
+        if log:
+            log(f"tune {st} -> {ms:.3f} ms")
+    best = min(table, key=lambda r: r["ms"])
+    ctx.set_tuning(best["small_max"], best["window"], best["msd_bits"])
+    return best, table
+
+
+def save(best, table, workload, path=DEFAULT_PATH):
+    with open(path, "w") as f:
+        json.dump({"workload": workload, "best": best, "table": table}, f, indent=1)
+
+
+def load(ctx, path=DEFAULT_PATH):
+    """Apply a saved tuning; returns it, or None when there is none."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        best = json.load(f)["best"]
+    ctx.set_tuning(best["small_max"], best["window"], best["msd_bits"])
+    return best

@@ -41,7 +41,7 @@ struct fmcs_ctx {
   Buf ws[WS_COUNT];
   u32* small_d = nullptr;   // 64 words of device scratch (counters, totals, consts)
   u32* host_pin = nullptr;  // PIN_WORDS words pinned for readbacks
-  fmcs_tuning tune = {512, 512, 8};
+  fmcs_tuning tune = {256, 256, 8};
   hipEvent_t ev[8] = {};
   double breakdown[4] = {0, 0, 0, 0};
   // per-kernel event timing (off unless enabled)
@@ -324,12 +324,29 @@ int boundary(fmcs_ctx* ctx, const K* v, const u32* in_group, u64 n, u32* out_gro
   return readback(ctx, ctx->small_d + 41, 1, ng);
 }
 
+// Tiles hold 256 * E rows: E = 2 (512 rows) when small segments are at most
+// 256 rows, else E = 4 (1024 rows).  Smaller tiles need half the LDS, so
+// twice as many are resident per CU to hide their load/scan latencies.
 template <class K>
-int tile_sort_launch(fmcs_ctx* ctx, TileSortArgs<K> a, u32 grid, double rows) {
+int tile_sort_launch(fmcs_ctx* ctx, TileSortArgs<K> a, u32 grid, double rows, u32 T = TILESORT_T) {
   if (grid == 0) return FMCS_OK;
   Prof pf(ctx, "k_tile_sort", 2.0 * rows * (sizeof(K) + 4));
-  hipLaunchKernelGGL((k_tile_sort<K, TILESORT_E>), dim3(grid), dim3(256), 0, ctx->stream, a);
+  if (T <= 512)
+    hipLaunchKernelGGL((k_tile_sort<K, 2>), dim3(grid), dim3(256), 0, ctx->stream, a);
+  else
+    hipLaunchKernelGGL((k_tile_sort<K, 4>), dim3(grid), dim3(256), 0, ctx->stream, a);
   return launch_check(ctx, "k_tile_sort");
+}
+
+template <class K>
+int tile_bucket_launch(fmcs_ctx* ctx, TileSortArgs<K> a, u32* fb, u32 grid, double rows, u32 T) {
+  if (grid == 0) return FMCS_OK;
+  Prof pf(ctx, "k_tile_bucket", 2.0 * rows * (sizeof(K) + 4));
+  if (T <= 512)
+    hipLaunchKernelGGL((k_tile_bucket<K, 2>), dim3(grid), dim3(256), 0, ctx->stream, a, fb);
+  else
+    hipLaunchKernelGGL((k_tile_bucket<K, 4>), dim3(grid), dim3(256), 0, ctx->stream, a, fb);
+  return launch_check(ctx, "k_tile_bucket");
 }
 
 constexpr u32 TS_SEGBITS = 11;  // local segment index of a TILESORT_T = 1024-row tile, with headroom
@@ -341,8 +358,8 @@ constexpr u32 TS_SEGBITS = 11;  // local segment index of a TILESORT_T = 1024-ro
 template <class K>
 int seg_sort_batched(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* seg_off, u32 nseg) {
   if (n <= 1 || nseg == 0) return FMCS_OK;
-  const u32 T = TILESORT_T;
-  const u32 small_max = std::min<u32>(ctx->tune.small_max, T / 2);
+  const u32 small_max = std::min<u32>(ctx->tune.small_max, (u32)TILESORT_T / 2);
+  const u32 T = small_max <= 256 ? 512u : (u32)TILESORT_T;
   u32 W = std::max<u32>(1, std::min<u32>(ctx->tune.window, small_max + 1));
   if (W + small_max > T) W = T - small_max;
   const u32 msd_bits = std::max<u32>(1, std::min<u32>(ctx->tune.msd_bits, RADIX_BITS));
@@ -414,13 +431,9 @@ int seg_sort_batched(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, cons
       u32* fb;
       TRY(getbuf_t(ctx, WS_FALLBACK, nw, &fb));
       TRY(fill(ctx, fb, nw, 0));
-      if (nw) {
-        Prof pf(ctx, "k_tile_bucket", 2.0 * (double)tile_rows_est * (sizeof(K) + 4));
-        hipLaunchKernelGGL((k_tile_bucket<K, TILESORT_E>), dim3((u32)nw), dim3(256), 0, ctx->stream, a, fb);
-        TRY(launch_check(ctx, "k_tile_bucket"));
-      }
+      TRY(tile_bucket_launch<K>(ctx, a, fb, (u32)nw, (double)tile_rows_est, T));
       a.only = fb;
-      TRY(tile_sort_launch<K>(ctx, a, (u32)nw, 0.0));
+      TRY(tile_sort_launch<K>(ctx, a, (u32)nw, 0.0, T));
     }
     if (nlarge == 0) break;
     u32 *llo, *lhi, *ltiles, *tbase;

@@ -44,7 +44,7 @@ struct TileSortArgs {
 
 constexpr int ONESWEEP_ITEMS = 16;
 constexpr int ONESWEEP_TILE = 256 * ONESWEEP_ITEMS;
-constexpr int BOUNDARY_ITEMS = 16;
+constexpr int BOUNDARY_ITEMS = 32;
 constexpr int BOUNDARY_TILE = 256 * BOUNDARY_ITEMS;
 constexpr int TILESORT_E = 4;
 constexpr int TILESORT_T = 256 * TILESORT_E;

@@ -480,11 +480,12 @@ __global__ __launch_bounds__(256) void k_tile_sort(TileSortArgs<K> a) {
   const u32* vin = a.vin[a.direct ? 0 : a.src_sel];
   // stage rows (coalesced) and mark segment starts
   for (u32 i = tid; i < (u32)T; i += NT) s_f[i] = 0;
+  __syncthreads();
+  // both global read phases (rows, segment starts) in flight together
   for (u32 i = tid; i < cnt; i += NT) {
     s_c[i] = (u64)kin[lo + i];
     s_o[i] = vin[lo + i];
   }
-  __syncthreads();
   for (u32 s = s0 + tid; s <= s1; s += NT) {
     u32 st = a.seg_start[s];
     if (st >= lo && st < hi) s_f[st - lo] = 1;
@@ -617,12 +618,13 @@ __global__ __launch_bounds__(256) void k_tile_bucket(TileSortArgs<K> a, u32* __r
   const u32* vin = a.vin[a.src_sel];
   for (u32 i = tid; i < (u32)T; i += NT) s_f[i] = 0;
   for (u32 i = tid; i < (u32)NB; i += NT) s_cnt[i] = 0;
+  if (tid == 0) s_over = 0;
+  __syncthreads();
+  // both global read phases (rows, segment starts) in flight together
   for (u32 i = tid; i < cnt; i += NT) {
     s_k[i] = (u64)kin[lo + i];
     s_o[i] = vin[lo + i];
   }
-  if (tid == 0) s_over = 0;
-  __syncthreads();
   for (u32 s = s0 + tid; s <= s1; s += NT) {
     u32 st = a.seg_start[s];
     if (st >= lo && st < hi) s_f[st - lo] = 1;
@@ -996,13 +998,15 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
 // ---------------------------------------------------------------------------
 #define FMCS_INST_K(K)                                                                                              \
   template __global__ void k_gather<K>(const K*, const u32*, K*, u64);                                              \
-  template __global__ void k_boundary<K, 16>(const K*, const u32*, u64, u32*, u32*, u32*, u32*, u32*);              \
+  template __global__ void k_boundary<K, 32>(const K*, const u32*, u64, u32*, u32*, u32*, u32*, u32*);              \
   template __global__ void k_hist_all<K>(const K*, u64, u32, u32, u32*);                                            \
   template __global__ void k_seg_hist<K, 16>(const K*, const u32*, const u32*, const u32*, u32, u32, u32, u32*);     \
   template __global__ void k_onesweep<K, 16>(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*,    \
                                              u32, const u32*, u32*, u32*, u32, u32);                                \
   template __global__ void k_tile_sort<K, 4>(TileSortArgs<K>);                                                      \
-  template __global__ void k_tile_bucket<K, 4>(TileSortArgs<K>, u32*);                                                      \
+  template __global__ void k_tile_sort<K, 2>(TileSortArgs<K>);                                                      \
+  template __global__ void k_tile_bucket<K, 4>(TileSortArgs<K>, u32*);                                             \
+  template __global__ void k_tile_bucket<K, 2>(TileSortArgs<K>, u32*);                                                      \
   template __global__ void k_copy_segs<K>(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, u32,  \
                                           u32);                                                                     \
   template __global__ void k_composite<K>(const K*, const u32*, u64, u32, u64*);                                    \
