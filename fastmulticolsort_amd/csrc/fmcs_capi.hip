@@ -22,7 +22,7 @@ enum Slot {
   WS_KA, WS_KB, WS_VB, WS_T0K, WS_T0V, WS_T1K, WS_T1V, WS_STATUS, WS_HIST, WS_DBASE, WS_SEGOFF, WS_SEGOFF2,
   WS_ST, WS_SZ, WS_BST, WS_BSZ, WS_FLAG, WS_POS, WS_LLO, WS_LHI, WS_LTILES, WS_TBASE, WS_TLO, WS_THI, WS_TS0,
   WS_TS1, WS_COMP, WS_COMP2, WS_GROUPTMP, WS_MISC, WS_HASH, WS_HASH2, WS_HASH3, WS_HT_TABLE, WS_HT_FIRST,
-  WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_USER0, WS_COUNT = WS_USER0 + 24
+  WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_USER0, WS_COUNT = WS_USER0 + 24
 };
 
 struct Buf {
@@ -401,10 +401,13 @@ int seg_sort_batched(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, cons
     TRY(getbuf_t(ctx, WS_THI, nw, &thi));
     TRY(getbuf_t(ctx, WS_TS0, nw, &ts0));
     TRY(getbuf_t(ctx, WS_TS1, nw, &ts1));
+    u32* wlist;
+    TRY(getbuf_t(ctx, WS_WLIST, nw, &wlist));
     TRY(fill(ctx, tlo, nw, 0));
     TRY(fill(ctx, thi, nw, 0));
+    TRY(fill(ctx, ctx->small_d + 51, 1, 0));
     hipLaunchKernelGGL(k_windows, dim3((cur_n + 255) / 256), dim3(256), 0, ctx->stream, cst, csz, cur_n, Wl, sm, plo,
-                       ptb, wpt, tlo, thi, ts0, ts1);
+                       ptb, wpt, tlo, thi, ts0, ts1, wlist, ctx->small_d + 51);
     TRY(launch_check(ctx, "k_windows"));
     // large segments of this level, and the row counts of both bins
     u32 *flag, *pos;
@@ -418,6 +421,7 @@ int seg_sort_batched(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, cons
     u32 cnts[6];
     TRY(readback(ctx, ctx->small_d + 48, 6, cnts));
     const u32 nlarge = cnts[0];
+    const u32 nwin = cnts[3];  // windows holding rows (compact list)
     const u64 tile_rows_est = cnts[4];
     const u64 large_rows = cnts[5];
     TileSortArgs<K> a;
@@ -425,15 +429,15 @@ int seg_sort_batched(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, cons
     a.kin[0] = keys; a.vin[0] = vals; a.kin[1] = kb_; a.vin[1] = vb_;
     a.kout = keys; a.vout = vals;
     a.tile_lo = tlo; a.tile_hi = thi; a.tile_s0 = ts0; a.tile_s1 = ts1; a.seg_start = cst;
-    a.src_sel = level & 1; a.kb = tkb; a.pad_seg = T; a.direct = 0;
+    a.src_sel = level & 1; a.kb = tkb; a.pad_seg = T; a.direct = 0; a.wlist = wlist;
     {
       // bucket fast path; tiles it refuses go to the radix tile sort
       u32* fb;
       TRY(getbuf_t(ctx, WS_FALLBACK, nw, &fb));
       TRY(fill(ctx, fb, nw, 0));
-      TRY(tile_bucket_launch<K>(ctx, a, fb, (u32)nw, (double)tile_rows_est, T));
+      TRY(tile_bucket_launch<K>(ctx, a, fb, nwin, (double)tile_rows_est, T));
       a.only = fb;
-      TRY(tile_sort_launch<K>(ctx, a, (u32)nw, 0.0, T));
+      TRY(tile_sort_launch<K>(ctx, a, nwin, 0.0, T));
     }
     if (nlarge == 0) break;
     u32 *llo, *lhi, *ltiles, *tbase;

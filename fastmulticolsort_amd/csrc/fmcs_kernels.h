@@ -39,7 +39,8 @@ struct TileSortArgs {
   u32 pad_seg;  // sentinel segment index, above any real one
   u32 direct;   // 1: one tile given by d_lo/d_hi (plain per-segment form)
   u32 d_lo, d_hi;
-  const u32* only;  // if set, only tiles w with only[w] != 0 are sorted
+  const u32* only;   // if set, only tiles w with only[w] != 0 are sorted
+  const u32* wlist;  // if set, block b sorts window wlist[b]
 };
 
 constexpr int ONESWEEP_ITEMS = 16;
@@ -68,7 +69,8 @@ template <class K, int E>
 __global__ void k_tile_sort(TileSortArgs<K>);
 template <class K, int E>
 __global__ void k_tile_bucket(TileSortArgs<K>, u32*);
-__global__ void k_windows(const u32*, const u32*, u32, u32, u32, const u32*, const u32*, u32, u32*, u32*, u32*, u32*);
+__global__ void k_windows(const u32*, const u32*, u32, u32, u32, const u32*, const u32*, u32, u32*, u32*, u32*, u32*,
+                          u32*, u32*);
 __global__ void k_offsets_to_list(const u32*, u32, u32*, u32*);
 __global__ void k_large_flags(const u32*, u32, u32, u32*, u32*);
 __global__ void k_large_compact(const u32*, const u32*, u32, u32, const u32*, u32, u32*, u32*, u32*);

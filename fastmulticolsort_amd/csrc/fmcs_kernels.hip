@@ -468,7 +468,7 @@ __global__ __launch_bounds__(256) void k_tile_sort(TileSortArgs<K> a) {
   if (a.direct) {
     lo = a.d_lo; hi = a.d_hi; s0 = 1; s1 = 0;
   } else {
-    u32 w = blockIdx.x;
+    u32 w = a.wlist ? a.wlist[blockIdx.x] : blockIdx.x;
     if (a.only && a.only[w] == 0) return;
     lo = a.tile_lo[w]; hi = a.tile_hi[w]; s0 = a.tile_s0[w]; s1 = a.tile_s1[w];
   }
@@ -607,7 +607,7 @@ __global__ __launch_bounds__(256) void k_tile_bucket(TileSortArgs<K> a, u32* __r
   __shared__ u32 s_tmp[NT / WAVE + 1];
   __shared__ u32 s_over;
   const u32 tid = threadIdx.x;
-  const u32 w = blockIdx.x;
+  const u32 w = a.wlist ? a.wlist[blockIdx.x] : blockIdx.x;
   const u32 lo = a.tile_lo[w], hi = a.tile_hi[w];
   if (lo >= hi) return;
   const u32 s0 = a.tile_s0[w], s1 = a.tile_s1[w];
@@ -767,16 +767,30 @@ __device__ __forceinline__ u32 window_of(const u32* st, u32 i, u32 W, const u32*
 __global__ void k_windows(const u32* __restrict__ st, const u32* __restrict__ sz, u32 nseg, u32 W, u32 small_max,
                           const u32* __restrict__ plo, const u32* __restrict__ pwb, u32 wpt,
                           u32* __restrict__ tlo, u32* __restrict__ thi, u32* __restrict__ ts0,
-                          u32* __restrict__ ts1) {
+                          u32* __restrict__ ts1, u32* __restrict__ wlist, u32* __restrict__ wcount) {
   u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nseg) return;
-  u32 size = sz[i];
-  if (size > small_max) return;
-  u32 w = window_of(st, i, W, plo, pwb, wpt);
-  bool first = (i == 0) || (window_of(st, i - 1, W, plo, pwb, wpt) != w);
-  bool last = (i + 1 == nseg) || (window_of(st, i + 1, W, plo, pwb, wpt) != w) || (sz[i + 1] > small_max);
-  if (first) { tlo[w] = st[i]; ts0[w] = i; }
-  if (last) { thi[w] = st[i] + size; ts1[w] = i; }
+  bool first = false;
+  u32 w = 0;
+  if (i < nseg) {
+    u32 size = sz[i];
+    if (size <= small_max) {
+      w = window_of(st, i, W, plo, pwb, wpt);
+      first = (i == 0) || (window_of(st, i - 1, W, plo, pwb, wpt) != w);
+      bool last = (i + 1 == nseg) || (window_of(st, i + 1, W, plo, pwb, wpt) != w) || (sz[i + 1] > small_max);
+      if (first) { tlo[w] = st[i]; ts0[w] = i; }
+      if (last) { thi[w] = st[i] + size; ts1[w] = i; }
+    }
+  }
+  // append the windows that hold rows to a compact list (one atomic per wave)
+  u64 m = __ballot(first);
+  if (m) {
+    u32 lane = __lane_id();
+    u32 leader = (u32)(__ffsll((long long)m) - 1);
+    u32 base = 0;
+    if (lane == leader) base = atomicAdd(wcount, (u32)__popcll(m));
+    base = __shfl(base, (int)leader);
+    if (first) wlist[base + (u32)__popcll(m & lanemask_lt())] = w;
+  }
 }
 
 // Segment list from offsets: start[i] = off[i], size[i] = off[i+1]-off[i].
