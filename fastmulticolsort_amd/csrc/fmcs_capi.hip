@@ -41,7 +41,7 @@ struct fmcs_ctx {
   Buf ws[WS_COUNT];
   u32* small_d = nullptr;   // 64 words of device scratch (counters, totals, consts)
   u32* host_pin = nullptr;  // PIN_WORDS words pinned for readbacks
-  fmcs_tuning tune = {256, 256, 8};
+  fmcs_tuning tune = {512, 512, 8};
   hipEvent_t ev[8] = {};
   double breakdown[4] = {0, 0, 0, 0};
   // per-kernel event timing (off unless enabled)

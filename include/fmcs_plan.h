@@ -99,6 +99,33 @@ int fmcs_run_plan_hash(fmcs_ctx* ctx, uint32_t ncols, const void* const* cols_d,
  * groups of 2^k rows). */
 int fmcs_calibrate(fmcs_ctx* ctx, uint64_t nrows, fmcs_cost_model* out);
 
+/* ---- learned plan-cost model (no counterpart in the reference; its plan
+ *      costs are analytic, src/stitch_composer.cpp:2186-2378) ---------- */
+
+/* fp32 MLP nin -> h1 -> h2 -> 1 with ReLU; each of nin, h1, h2 <= 64.
+ * Parameters (device, floats): W1[h1][nin], b1[h1], W2[h2][h1], b2[h2],
+ * W3[h2], b3. */
+typedef struct {
+  uint32_t nin, h1, h2;
+} fmcs_mlp_shape;
+
+int fmcs_mlp_param_count(const fmcs_mlp_shape* s, uint32_t* count);
+
+/* y_d[s] = MLP(x_d[s, 0..nin)); acts_d (optional, batch x (h1+h2)) keeps the
+ * post-ReLU activations for the backward pass. */
+int fmcs_mlp_forward(fmcs_ctx* ctx, const fmcs_mlp_shape* s, const float* params_d, const float* x_d,
+                     uint32_t batch, float* y_d, float* acts_d);
+
+/* Gradient of L = sum_s 0.5 (y_s - target_s)^2 / batch into grad_d (one
+ * float per parameter), L into loss_d (optional).  Deterministic. */
+int fmcs_mlp_backward(fmcs_ctx* ctx, const fmcs_mlp_shape* s, const float* params_d, const float* x_d,
+                      const float* acts_d, const float* y_d, const float* target_d, uint32_t batch, float* grad_d,
+                      float* loss_d);
+
+/* One Adam step (bias-corrected, step >= 1) over nparams parameters. */
+int fmcs_mlp_adam(fmcs_ctx* ctx, uint32_t nparams, float* params_d, const float* grad_d, float* m_d, float* v_d,
+                  float lr, float beta1, float beta2, float eps, uint32_t step);
+
 #ifdef __cplusplus
 }
 #endif

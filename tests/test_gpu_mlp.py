@@ -1,0 +1,142 @@
+"""The learned cost model's kernels: forward, backward and Adam against an
+fp64 restatement (pinned itself by central finite differences), and the
+plan picker's gain over the analytic model on GPU-measured plans."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from fastmulticolsort_amd import learned, planner
+
+pytestmark = pytest.mark.gpu
+
+
+def unpack(p, nin, h1, h2):
+    o = 0
+    W1 = p[o:o + h1 * nin].reshape(h1, nin); o += h1 * nin
+    b1 = p[o:o + h1]; o += h1
+    W2 = p[o:o + h2 * h1].reshape(h2, h1); o += h2 * h1
+    b2 = p[o:o + h2]; o += h2
+    W3 = p[o:o + h2]; o += h2
+    b3 = p[o]
+    return W1, b1, W2, b2, W3, b3
+
+
+def fwd64(p, X, nin, h1, h2):
+    W1, b1, W2, b2, W3, b3 = unpack(p, nin, h1, h2)
+    z1 = X @ W1.T + b1
+    a1 = np.maximum(z1, 0)
+    z2 = a1 @ W2.T + b2
+    a2 = np.maximum(z2, 0)
+    return a2 @ W3 + b3, (z1, a1, z2, a2)
+
+
+def loss64(p, X, t, nin, h1, h2):
+    y, _ = fwd64(p, X, nin, h1, h2)
+    return 0.5 * np.sum((y - t) ** 2) / len(t)
+
+
+def grad64(p, X, t, nin, h1, h2):
+    W1, b1, W2, b2, W3, b3 = unpack(p, nin, h1, h2)
+    y, (z1, a1, z2, a2) = fwd64(p, X, nin, h1, h2)
+    dy = (y - t) / len(t)
+    dW3 = a2.T @ dy
+    db3 = dy.sum()
+    dz2 = np.outer(dy, W3) * (z2 > 0)
+    dW2 = dz2.T @ a1
+    db2 = dz2.sum(0)
+    dz1 = (dz2 @ W2) * (z1 > 0)
+    dW1 = dz1.T @ X
+    db1 = dz1.sum(0)
+    return np.concatenate([dW1.ravel(), db1, dW2.ravel(), db2, dW3, [db3]])
+
+
+def test_fp64_gradient_matches_finite_differences():
+    rng = np.random.default_rng(0)
+    nin, h1, h2, B = 5, 7, 6, 9
+    P = h1 * nin + h1 + h2 * h1 + h2 + h2 + 1
+    p = rng.normal(0, 0.5, P)
+    X = rng.normal(0, 1, (B, nin))
+    t = rng.normal(0, 1, B)
+    g = grad64(p, X, t, nin, h1, h2)
+    eps = 1e-6
+    for i in range(P):
+        e = np.zeros(P); e[i] = eps
+        fd = (loss64(p + e, X, t, nin, h1, h2) - loss64(p - e, X, t, nin, h1, h2)) / (2 * eps)
+        assert abs(fd - g[i]) < 1e-6 * max(1.0, abs(fd)), i
+
+
+@pytest.mark.parametrize("B", [1, 17, 300])
+def test_forward_and_backward_match_fp64(ctx, B):
+    m = learned.Mlp(ctx, nin=learned.NFEAT, h1=64, h2=64, seed=3)
+    rng = np.random.default_rng(B)
+    X = rng.uniform(0, 1, (B, learned.NFEAT)).astype(np.float32)
+    t = rng.normal(0, 1, B).astype(np.float32)
+    Xd, td = torch.from_numpy(X).cuda(), torch.from_numpy(t).cuda()
+    y, loss = m.backward(Xd, td)
+    p = m.params.cpu().numpy().astype(np.float64)
+    ey, _ = fwd64(p, X.astype(np.float64), learned.NFEAT, 64, 64)
+    np.testing.assert_allclose(y.cpu().numpy(), ey, rtol=1e-4, atol=1e-4)
+    eg = grad64(p, X.astype(np.float64), t.astype(np.float64), learned.NFEAT, 64, 64)
+    np.testing.assert_allclose(m.grad.cpu().numpy(), eg, rtol=2e-3, atol=2e-5)
+    assert abs(loss.item() - loss64(p, X.astype(np.float64), t, learned.NFEAT, 64, 64)) < 1e-4 * max(1, loss.item())
+
+
+def test_backward_is_deterministic(ctx):
+    m = learned.Mlp(ctx, seed=1)
+    X = torch.rand(777, learned.NFEAT, device="cuda")
+    t = torch.rand(777, device="cuda")
+    m.backward(X, t)
+    g1 = m.grad.clone()
+    m.backward(X, t)
+    assert torch.equal(g1, m.grad)
+
+
+def test_adam_step_matches_fp64(ctx):
+    m = learned.Mlp(ctx, seed=2)
+    X = torch.rand(64, learned.NFEAT, device="cuda")
+    t = torch.rand(64, device="cuda")
+    p0 = m.params.cpu().numpy().astype(np.float64)
+    m.backward(X, t)
+    g = m.grad.cpu().numpy().astype(np.float64)
+    m.adam(lr=1e-2)
+    mm = 0.1 * g
+    vv = 0.001 * g * g
+    exp = p0 - 1e-2 * (mm / 0.1) / (np.sqrt(vv / 0.001) + 1e-8)
+    np.testing.assert_allclose(m.params.cpu().numpy(), exp, rtol=1e-5, atol=1e-6)
+
+
+def test_training_fits_a_smooth_function(ctx):
+    rng = np.random.default_rng(0)
+    X = rng.uniform(0, 1, (2048, learned.NFEAT)).astype(np.float32)
+    y = (np.sin(3 * X[:, 0]) + X[:, 3] * X[:, 5] + 0.5 * X[:, 7]).astype(np.float32)
+    m = learned.Mlp(ctx, seed=4)
+    hist = m.train(torch.from_numpy(X).cuda(), torch.from_numpy(y).cuda(), epochs=60, batch=128, lr=3e-3)
+    assert hist[-1] < hist[0] / 10
+
+
+def test_learned_picker_beats_analytic_on_measured_plans(ctx):
+    """Train on GPU-measured plan times of seeded requests, then compare plan
+    regret (measured time of the pick / best measured - 1) of the MLP pick
+    and of the analytic cost model's pick on held-out requests."""
+    data = learned.collect(ctx, nreq=24, seed=11, max_log2=20, max_plans=24)
+    X, y, req = data["X"], data["y"], data["req"]
+    train = req < 16
+    m = learned.Mlp(ctx, seed=0)
+    m.train(torch.from_numpy(X[train]).cuda(), torch.from_numpy(y[train]).cuda(), epochs=200, batch=64, lr=3e-3)
+    test = ~train
+    pred = m.forward(torch.from_numpy(X[test]).cuda()).cpu().numpy()
+    reg_mlp = learned.regret_table(pred, y[test], req[test])
+    # analytic picks
+    import json
+    reqs = json.loads(data["requests"])
+    model = planner.default_cost_model()
+    est = []
+    for x_plan, r in zip(data["plan"][test], req[test]):
+        q = reqs[r]
+        p = [int(w) for w in x_plan.split(",")]
+        est.append(planner.estimate_plan(model, q["n"], q["bws"], q["cards"], list(range(len(q["bws"]))), p)[0])
+    reg_an = learned.regret_table(np.log(np.array(est)), y[test], req[test])
+    print("regret mlp mean %.3f analytic mean %.3f" % (reg_mlp.mean(), reg_an.mean()), flush=True)
+    assert reg_mlp.mean() <= reg_an.mean() + 0.02
