@@ -8,63 +8,9 @@ import pytest
 import torch
 
 from fastmulticolsort_amd import learned, planner
+from mlp_ref import fwd64, grad64, loss64
 
 pytestmark = pytest.mark.gpu
-
-
-def unpack(p, nin, h1, h2):
-    o = 0
-    W1 = p[o:o + h1 * nin].reshape(h1, nin); o += h1 * nin
-    b1 = p[o:o + h1]; o += h1
-    W2 = p[o:o + h2 * h1].reshape(h2, h1); o += h2 * h1
-    b2 = p[o:o + h2]; o += h2
-    W3 = p[o:o + h2]; o += h2
-    b3 = p[o]
-    return W1, b1, W2, b2, W3, b3
-
-
-def fwd64(p, X, nin, h1, h2):
-    W1, b1, W2, b2, W3, b3 = unpack(p, nin, h1, h2)
-    z1 = X @ W1.T + b1
-    a1 = np.maximum(z1, 0)
-    z2 = a1 @ W2.T + b2
-    a2 = np.maximum(z2, 0)
-    return a2 @ W3 + b3, (z1, a1, z2, a2)
-
-
-def loss64(p, X, t, nin, h1, h2):
-    y, _ = fwd64(p, X, nin, h1, h2)
-    return 0.5 * np.sum((y - t) ** 2) / len(t)
-
-
-def grad64(p, X, t, nin, h1, h2):
-    W1, b1, W2, b2, W3, b3 = unpack(p, nin, h1, h2)
-    y, (z1, a1, z2, a2) = fwd64(p, X, nin, h1, h2)
-    dy = (y - t) / len(t)
-    dW3 = a2.T @ dy
-    db3 = dy.sum()
-    dz2 = np.outer(dy, W3) * (z2 > 0)
-    dW2 = dz2.T @ a1
-    db2 = dz2.sum(0)
-    dz1 = (dz2 @ W2) * (z1 > 0)
-    dW1 = dz1.T @ X
-    db1 = dz1.sum(0)
-    return np.concatenate([dW1.ravel(), db1, dW2.ravel(), db2, dW3, [db3]])
-
-
-def test_fp64_gradient_matches_finite_differences():
-    rng = np.random.default_rng(0)
-    nin, h1, h2, B = 5, 7, 6, 9
-    P = h1 * nin + h1 + h2 * h1 + h2 + h2 + 1
-    p = rng.normal(0, 0.5, P)
-    X = rng.normal(0, 1, (B, nin))
-    t = rng.normal(0, 1, B)
-    g = grad64(p, X, t, nin, h1, h2)
-    eps = 1e-6
-    for i in range(P):
-        e = np.zeros(P); e[i] = eps
-        fd = (loss64(p + e, X, t, nin, h1, h2) - loss64(p - e, X, t, nin, h1, h2)) / (2 * eps)
-        assert abs(fd - g[i]) < 1e-6 * max(1.0, abs(fd)), i
 
 
 @pytest.mark.parametrize("B", [1, 17, 300])
