@@ -5,9 +5,12 @@ One step = one full two-round sort on the GPU, inputs resident in HBM:
 identity order, round-1 LSD radix sort of c1 with the oid payload, group
 boundaries, oid gather of c2, round-2 batched segmented sort of c2 within
 the c1 groups (the reference's Composer::multiRoundSorting,
-src/composer.cpp:48-135).  The plain per-round form (one sort dispatch per
-group, src/composer.cpp:1285-1323) and the single unbinned segmented sort
-are timed beside it.  Prints one JSON line.
+src/composer.cpp:48-135).  The timed steps replay the device-driven round
+as one hipGraph (--mode graph, the default); per-kernel HIP-event times come
+from a second pass over the same kernels launched eagerly.  The
+host-synchronised batched form, the plain per-round form (one sort dispatch
+per group, src/composer.cpp:1285-1323) and the single unbinned segmented
+sort are timed beside it.  Prints one JSON line.
 """
 import argparse
 import json
@@ -93,6 +96,7 @@ def main():
     ap.add_argument("--cpu-sample-log2", type=int, default=23)
     ap.add_argument("--autotune", action="store_true",
                     help="sweep the batched sort's bin thresholds and tile shapes first and save the best")
+    ap.add_argument("--mode", choices=["graph", "device", "batched"], default="graph")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one plain 256 MiB device copy (the traffic reference for PMC counters)")
     args = ap.parse_args()
@@ -103,7 +107,8 @@ def main():
 
     import torch
     from fastmulticolsort_amd import datagen
-    from fastmulticolsort_amd.engine import Context, MODE_BATCHED, MODE_PLAIN, MODE_UNBINNED
+    from fastmulticolsort_amd.engine import (Context, MODE_BATCHED, MODE_DEVICE, MODE_GRAPH, MODE_PLAIN,
+                                             MODE_UNBINNED)
 
     n = 1 << args.rows_log2
     ng = 1 << args.groups_log2
@@ -121,8 +126,11 @@ def main():
         torch.cuda.synchronize()
         log("copy probe: 268435456 bytes read + 268435456 bytes written")
 
-    def step(mode=MODE_BATCHED):
-        ctx.multi_round_sort([d1, d2], bws, order=order, mode=mode)
+    main_mode = {"graph": MODE_GRAPH, "device": MODE_DEVICE, "batched": MODE_BATCHED}[args.mode]
+    prof_mode = MODE_DEVICE if main_mode == MODE_GRAPH else main_mode
+
+    def step(mode=main_mode):
+        ctx.multi_round_sort([d1, d2], bws, order=order, mode=mode, want_count=False)
 
     from fastmulticolsort_amd import autotune
     tuning = None
@@ -136,26 +144,33 @@ def main():
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+    ctx.sync()
     ok = check_sorted(c1, c2, order.cpu().numpy())
     log(f"full-size property check (permutation, (c1,c2,oid) order): {ok}")
     if not ok:
-        raise SystemExit("batched result failed the sortedness check")
+        raise SystemExit("result failed the sortedness check")
 
-    ctx.set_profiling(True)
-    ctx.kernel_stats(reset=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    stats = ctx.kernel_stats(reset=True)
-    ctx.set_profiling(False)
-    bd = ctx.last_breakdown()
+    ctx.sync()  # reports a device-detected input error, if any
     ms = (t1 - t0) * 1e3 / args.steps
     value = n / (ms / 1e3) / 1e6
-    log(f"batched: {ms:.3f} ms/step, {value:.1f} Mrows/s; breakdown {bd}")
+    log(f"{args.mode}: {ms:.3f} ms/step, {value:.1f} Mrows/s")
+
+    # per-kernel HIP-event times: the same kernels launched eagerly
+    ctx.set_profiling(True)
+    ctx.kernel_stats(reset=True)
+    for _ in range(args.steps):
+        step(prof_mode)
+    stats = ctx.kernel_stats(reset=True)
+    ctx.set_profiling(False)
+    ok = check_sorted(c1, c2, order.cpu().numpy())
+    if not ok:
+        raise SystemExit("profiled pass failed the sortedness check")
 
     # dominant kernel and its roofline (algorithmic bytes / avg duration)
     dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
@@ -167,6 +182,19 @@ def main():
                    "GBs": (v["bytes"] / (v["ms"] / 1e3) / 1e9) if v["ms"] > 0 else None}
                for k, v in stats.items()}
     log("kernels: " + json.dumps(kernels))
+    kernel_ms = sum(v["ms_per_step"] for v in kernels.values())
+
+    # host-synchronised batched form (one readback per level)
+    batched_ms = None
+    if main_mode != MODE_BATCHED:
+        step(MODE_BATCHED)
+        torch.cuda.synchronize()
+        tb0 = time.perf_counter()
+        for _ in range(3):
+            step(MODE_BATCHED)
+        torch.cuda.synchronize()
+        batched_ms = (time.perf_counter() - tb0) * 1e3 / 3
+        log(f"batched (host-synchronised): {batched_ms:.3f} ms")
 
     # comparators
     ctx.multi_round_sort([d1, d2], bws, order=order, mode=MODE_UNBINNED)
@@ -210,9 +238,11 @@ def main():
                      "avg_launch_ms": round(avg_ms, 5), "bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
         "comparators": {"plain_per_round_ms": plain_ms, "unbinned_ms": round(unb_ms, 3),
+                        "host_synchronised_batched_ms": (round(batched_ms, 3) if batched_ms else None),
                         "batched_vs_plain": (round(plain_ms / ms, 2) if plain_ms else None),
                         "batched_vs_unbinned": round(unb_ms / ms, 3)},
-        "breakdown_ms": bd,
+        "mode": args.mode,
+        "kernel_ms_per_step": round(kernel_ms, 4),
         "tuning": tuning or ctx.tuning(),
         "kernels": kernels,
     }

@@ -18,6 +18,8 @@ FMCS_E_UNSUPPORTED = 4
 MODE_BATCHED = 0
 MODE_PLAIN = 1
 MODE_UNBINNED = 2
+MODE_DEVICE = 3
+MODE_GRAPH = 4
 
 # Every exported symbol of include/fmcs.h.
 EXPORTS = [

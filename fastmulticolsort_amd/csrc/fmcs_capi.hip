@@ -5,6 +5,8 @@
 // workspace that grows on demand and is reused across calls.
 #include <algorithm>
 #include <map>
+#include <tuple>
+#include <utility>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -22,7 +24,8 @@ enum Slot {
   WS_KA, WS_KB, WS_VB, WS_T0K, WS_T0V, WS_T1K, WS_T1V, WS_STATUS, WS_HIST, WS_DBASE, WS_SEGOFF, WS_SEGOFF2,
   WS_ST, WS_SZ, WS_BST, WS_BSZ, WS_FLAG, WS_POS, WS_LLO, WS_LHI, WS_LTILES, WS_TBASE, WS_TLO, WS_THI, WS_TS0,
   WS_TS1, WS_COMP, WS_COMP2, WS_GROUPTMP, WS_MISC, WS_HASH, WS_HASH2, WS_HASH3, WS_HT_TABLE, WS_HT_FIRST,
-  WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_USER0, WS_COUNT = WS_USER0 + 24
+  WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_DD, WS_SCANST, WS_FBLIST, WS_USER0,
+  WS_COUNT = WS_USER0 + 24
 };
 
 struct Buf {
@@ -49,6 +52,8 @@ struct fmcs_ctx {
     const char* name;
     hipEvent_t e0, e1;
     double bytes;
+    const u32* dcount;  // if set: bytes += per_unit * (*dcount), read when the records are flushed
+    double per_unit;
   };
   struct Agg {
     u64 launches = 0;
@@ -58,6 +63,17 @@ struct fmcs_ctx {
   std::vector<Rec> recs;
   std::vector<hipEvent_t> pool;
   std::map<std::string, Agg> agg;
+  // device-driven path: workspace reallocation epoch (a captured graph is
+  // valid only while no buffer it names moved), capture state, cached graph
+  u64 ws_epoch = 0;
+  bool capturing = false;
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<u64> gkey;
+  u32 ncu = 0;
+  std::map<const void*, u32> resident;
+  bool dd_pending = false;  // a device-driven call's key-error word is not yet checked
 };
 
 #define HIPCHK(x)                                                                          \
@@ -83,6 +99,8 @@ int getbuf(fmcs_ctx* ctx, Slot s, size_t bytes, void** out) {
   bytes = std::max<size_t>(bytes, 256);
   Buf& b = ctx->ws[s];
   if (b.bytes < bytes) {
+    if (ctx->capturing) return fail(ctx, FMCS_E_HIP, "workspace growth during graph capture");
+    ctx->ws_epoch++;
     if (b.p) {
       hipError_t e = hipStreamSynchronize(ctx->stream);
       if (e != hipSuccess) return fail(ctx, FMCS_E_HIP, hipGetErrorString(e));
@@ -165,10 +183,16 @@ int prof_flush(fmcs_ctx* ctx) {
   for (auto& r : ctx->recs) {
     float ms = 0;
     if (r.e0 && r.e1 && hipEventElapsedTime(&ms, r.e0, r.e1) == hipSuccess) {
+      double bytes = r.bytes;
+      if (r.dcount) {
+        u32 c = 0;
+        HIPCHK(hipMemcpy(&c, r.dcount, sizeof(u32), hipMemcpyDeviceToHost));
+        bytes += r.per_unit * c;
+      }
       auto& a = ctx->agg[r.name];
       a.launches++;
       a.ms += ms;
-      a.bytes += r.bytes;
+      a.bytes += bytes;
     }
     if (r.e0) ctx->pool.push_back(r.e0);
     if (r.e1) ctx->pool.push_back(r.e1);
@@ -181,10 +205,10 @@ int prof_flush(fmcs_ctx* ctx) {
 struct Prof {
   fmcs_ctx* ctx;
   int idx = -1;
-  Prof(fmcs_ctx* c, const char* name, double bytes) : ctx(c) {
-    if (!ctx->prof) return;
+  Prof(fmcs_ctx* c, const char* name, double bytes, const u32* dcount = nullptr, double per_unit = 0) : ctx(c) {
+    if (!ctx->prof || ctx->capturing) return;
     if (ctx->recs.size() > 8192) prof_flush(ctx);
-    fmcs_ctx::Rec r{name, pool_get(ctx), pool_get(ctx), bytes};
+    fmcs_ctx::Rec r{name, pool_get(ctx), pool_get(ctx), bytes, dcount, per_unit};
     if (r.e0) (void)hipEventRecord(r.e0, ctx->stream);
     ctx->recs.push_back(r);
     idx = (int)ctx->recs.size() - 1;
@@ -291,7 +315,8 @@ int radix_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, 
     Prof pf(ctx, "k_onesweep", 2.0 * n * (sizeof(K) + 4));
     hipLaunchKernelGGL((k_onesweep<K, ONESWEEP_ITEMS>), dim3((u32)ntiles), dim3(256), 0, ctx->stream, sk, sv, dk, dv,
                        ctx->small_d + 16, ctx->small_d + 17, ctx->small_d + 18, 1u, dbase + (u64)p * RADIX,
-                       status + (u64)p * ntiles * RADIX, ctx->small_d + 24 + p, p * RADIX_BITS, (u32)RADIX_BITS);
+                       status + (u64)p * ntiles * RADIX, ctx->small_d + 24 + p, p * RADIX_BITS, (u32)RADIX_BITS,
+                       (const u32*)nullptr, (const u32*)nullptr);
     TRY(launch_check(ctx, "k_onesweep"));
     sk = dk;
     sv = dv;
@@ -488,7 +513,8 @@ int seg_sort_batched(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, cons
     {
       Prof pf(ctx, "k_onesweep_msd", 2.0 * large_rows * (sizeof(K) + 4));
       hipLaunchKernelGGL((k_onesweep<K, ONESWEEP_ITEMS>), dim3(ntiles), dim3(256), 0, ctx->stream, skey, sval, dkey,
-                         dval, llo, lhi, tbase, nlarge, dbase, status, ctx->small_d + 50, shift, D);
+                         dval, llo, lhi, tbase, nlarge, dbase, status, ctx->small_d + 50, shift, D,
+                         (const u32*)nullptr, (const u32*)(ctx->small_d + 49));
       TRY(launch_check(ctx, "k_onesweep(msd)"));
     }
     cur = nxt;
@@ -547,6 +573,9 @@ int seg_sort_unbinned(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, con
 }
 
 template <class K>
+int seg_sort_dd_host(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* seg_off, u32 nseg);
+
+template <class K>
 int seg_sort(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* seg_off, u32 nseg, int mode,
              const u32* group) {
   switch (mode) {
@@ -555,6 +584,8 @@ int seg_sort(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* s
     case FMCS_MODE_UNBINNED:
       if (!group) return fail(ctx, FMCS_E_ARG, "unbinned mode needs group ids");
       return seg_sort_unbinned<K>(ctx, keys, vals, n, keybits, group, nseg);
+    case FMCS_MODE_DEVICE:
+    case FMCS_MODE_GRAPH: return seg_sort_dd_host<K>(ctx, keys, vals, n, keybits, seg_off, nseg);
     default: return fail(ctx, FMCS_E_ARG, "unknown mode");
   }
 }
@@ -621,6 +652,464 @@ int sort_by_group_t(fmcs_ctx* ctx, u32 keybits, u64 n, const K* column, u32* ord
   if (t_sort) *t_sort += b;
   if (t_extract) *t_extract += c;
   if (ngroups) *ngroups = ng;
+  return FMCS_OK;
+}
+
+// ========================================================================
+// Device-driven batched path.  Counts stay in device memory; every kernel
+// reads its sizes from there and runs grid-stride or persistent over them,
+// so a multi-round sort issues no host synchronisation and can be captured
+// in a hipGraph.  Buffers are sized from host-side bounds: a level's large
+// segments each hold more than small_max rows, so at most n / (small_max+1)
+// of them exist; their tiles number at most n / TILE + that.
+// ========================================================================
+constexpr u32 DD_LVL_WORDS = 16;
+constexpr u32 DD_LVL_MAX = 72;
+constexpr u32 DD_ROUNDS_MAX = 64;
+constexpr u32 DD_HDR = 256;  // [0, 64) group count per round, [64, 72) LSD pass counters, [72] key error, [76, 80) consts
+constexpr u32 DD_ERR = 72;
+constexpr u32 DD_CONST = 76;
+constexpr u64 DD_WORDS = DD_HDR + (u64)DD_ROUNDS_MAX * DD_LVL_MAX * DD_LVL_WORDS;
+enum { LV_NSEG, LV_WCOUNT, LV_ROWS_SMALL, LV_ROWS_LARGE, LV_NLARGE, LV_NTILES, LV_SCAN_A, LV_SCAN_B, LV_OS, LV_FB };
+
+// Blocks of `fn` resident on the whole device at 256 threads (cached).
+u32 resident(fmcs_ctx* ctx, const void* fn) {
+  auto it = ctx->resident.find(fn);
+  if (it != ctx->resident.end()) return it->second;
+  if (ctx->ncu == 0) {
+    hipDeviceProp_t prop;
+    ctx->ncu = (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
+                   ? (u32)prop.multiProcessorCount
+                   : 256u;
+  }
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, 0) != hipSuccess || nb <= 0) {
+    (void)hipGetLastError();
+    nb = 1;
+  }
+  u32 r = (u32)nb * ctx->ncu;
+  ctx->resident[fn] = r;
+  return r;
+}
+
+template <typename... KA, size_t... I>
+hipError_t launch_tuple(const void* fn, u32 grid, hipStream_t st, std::tuple<KA...>& t, std::index_sequence<I...>) {
+  void* argv[sizeof...(KA) + 1] = {(void*)&std::get<I>(t)...};
+  return hipLaunchKernel(fn, dim3(grid), dim3(256), argv, 0, st);
+}
+
+// Launches kern<<<grid, 256>>> on the context's stream with per-kernel
+// timing when profiling; bytes = static bytes + per_unit * (*dcount).
+template <typename... KA, typename... A>
+int klaunch(fmcs_ctx* ctx, const char* name, void (*kern)(KA...), u64 grid, double bytes, const u32* dcount,
+            double per_unit, A... args) {
+  if (grid == 0) return FMCS_OK;
+  std::tuple<KA...> t(args...);
+  Prof pf(ctx, name, bytes, dcount, per_unit);
+  hipError_t e = launch_tuple((const void*)kern, (u32)grid, ctx->stream, t, std::index_sequence_for<KA...>{});
+  if (e != hipSuccess) return fail(ctx, FMCS_E_HIP, std::string(name) + ": " + hipGetErrorString(e));
+  return launch_check(ctx, name);
+}
+
+u64 cap_grid(u64 want, u64 cap) { return std::max<u64>(1, std::min<u64>(want, cap)); }
+
+// p[0 .. ceil(*cnt * mult / div) + add) = v, bounded by max_words
+int dd_fill(fmcs_ctx* ctx, u32* p, const u32* cnt, u32 mult, u32 div, u32 add, u32 v, u64 max_words) {
+  return klaunch(ctx, "k_fill", kd_fill, cap_grid((max_words + 255) / 256, 1024), 0.0, nullptr, 0.0, p, cnt, mult,
+                 div, add, v);
+}
+
+int dd_scan(fmcs_ctx* ctx, const u32* in, u32* out, const u32* n_d, u64 n_max, u32* counter, u32* total) {
+  u32* st;
+  const u64 tiles_max = std::max<u64>(1, (n_max + SCAN_TILE - 1) / SCAN_TILE);
+  TRY(getbuf_t(ctx, WS_SCANST, tiles_max, &st));
+  TRY(dd_fill(ctx, st, n_d, 1, SCAN_TILE, 0, 0, tiles_max));
+  return klaunch(ctx, "k_scan", kd_scan_excl<SCAN_ITEMS>,
+                 cap_grid(tiles_max, resident(ctx, (const void*)kd_scan_excl<SCAN_ITEMS>)), 0.0, nullptr, 0.0, in, out,
+                 n_d, st, counter, total);
+}
+
+// LSD radix sort over every pass (no host-side trivial-pass skipping).
+// Keys above keybits set the error word dd[DD_ERR].
+template <class K>
+int radix_sort_dd(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, u64 n, u32 keybits, u32* dd) {
+  if (n == 0) return FMCS_OK;
+  keybits = std::min<u32>(keybits, 8 * sizeof(K));
+  const u32 npasses = (keybits + RADIX_BITS - 1) / RADIX_BITS;
+  if (n == 1 || npasses == 0) {
+    if ((const void*)kout != (const void*)kin || vout != vin)
+      return klaunch(ctx, "k_copy", kd_copy_pairs<K>, 1, 0.0, nullptr, 0.0, kin, vin, kout, vout, n);
+    return FMCS_OK;
+  }
+  const u32 last_bits = keybits - (npasses - 1) * RADIX_BITS;
+  const u64 ntiles = (n + ONESWEEP_TILE - 1) / ONESWEEP_TILE;
+  u32 *hist, *dbase, *status;
+  TRY(getbuf_t(ctx, WS_HIST, (u64)npasses * RADIX, &hist));
+  TRY(getbuf_t(ctx, WS_DBASE, (u64)npasses * RADIX, &dbase));
+  TRY(getbuf_t(ctx, WS_STATUS, (u64)npasses * ntiles * RADIX, &status));
+  K *t0k, *t1k;
+  u32 *t0v, *t1v;
+  TRY(getbuf_t(ctx, WS_T0K, n, &t0k));
+  TRY(getbuf_t(ctx, WS_T0V, n, &t0v));
+  TRY(getbuf_t(ctx, WS_T1K, n, &t1k));
+  TRY(getbuf_t(ctx, WS_T1V, n, &t1v));
+  TRY(fill(ctx, hist, (u64)npasses * RADIX, 0));
+  TRY(fill(ctx, dd + 64, 8, 0));
+  TRY(fill(ctx, status, (u64)npasses * ntiles * RADIX, 0));
+  TRY(klaunch(ctx, "k_hist_all", k_hist_all<K>, cap_grid((n + 255) / 256, 2048), (double)n * sizeof(K), nullptr, 0.0,
+              kin, n, npasses, (u32)RADIX_BITS, hist));
+  TRY(klaunch(ctx, "k_digit_base", k_digit_base, npasses, 0.0, nullptr, 0.0, (const u32*)hist, (const u32*)nullptr,
+              dbase, (u32*)nullptr, (u32*)nullptr));
+  if (last_bits < (u32)RADIX_BITS)
+    TRY(klaunch(ctx, "k_check_digits", kd_check_digits, 1, 0.0, nullptr, 0.0, (const u32*)hist + (u64)(npasses - 1) * RADIX,
+                last_bits, dd + DD_ERR));
+  // one segment [0, n) of ntiles tiles: seg_lo, seg_hi, tile_base[0], tile_base[1]
+  u32* cst = dd + DD_CONST;
+  TRY(klaunch(ctx, "k_set", kd_set4, 1, 0.0, nullptr, 0.0, cst, 0u, (u32)n, 0u, (u32)ntiles));
+  const u64 grid = cap_grid(ntiles, resident(ctx, (const void*)k_onesweep<K, ONESWEEP_ITEMS>));
+  const K* sk = kin;
+  const u32* sv = vin;
+  for (u32 p = 0; p < npasses; ++p) {
+    const bool last = (p + 1 == npasses);
+    K* dk;
+    u32* dv;
+    if (last && (const void*)kout != (const void*)sk && vout != sv) {
+      dk = kout;
+      dv = vout;
+    } else {
+      dk = (p & 1) ? t1k : t0k;
+      dv = (p & 1) ? t1v : t0v;
+    }
+    TRY(klaunch(ctx, "k_onesweep", k_onesweep<K, ONESWEEP_ITEMS>, grid, 2.0 * n * (sizeof(K) + 4), nullptr, 0.0, sk,
+                sv, dk, dv, (const u32*)cst, (const u32*)cst + 1, (const u32*)cst + 2, 1u,
+                (const u32*)dbase + (u64)p * RADIX, status + (u64)p * ntiles * RADIX, dd + 64 + p, p * RADIX_BITS,
+                (u32)RADIX_BITS, (const u32*)nullptr, (const u32*)nullptr));
+    sk = dk;
+    sv = dv;
+  }
+  if ((const void*)sk != (const void*)kout || sv != vout)
+    TRY(klaunch(ctx, "k_copy", kd_copy_pairs<K>, cap_grid((n + 255) / 256, 4096), 2.0 * n * (sizeof(K) + 4), nullptr,
+                0.0, sk, sv, kout, vout, n));
+  return FMCS_OK;
+}
+
+// Group boundaries; the group count goes to *count_d (device).
+template <class K>
+int boundary_dd(fmcs_ctx* ctx, const K* v, const u32* in_group, u64 n, u32* out_group, u32* seg_off, u32* count_d) {
+  if (n == 0) return FMCS_OK;
+  const u64 ntiles = (n + BOUNDARY_TILE - 1) / BOUNDARY_TILE;
+  u32* status;
+  TRY(getbuf_t(ctx, WS_STATUS, ntiles, &status));
+  TRY(fill(ctx, status, ntiles, 0));
+  TRY(fill(ctx, ctx->small_d + 40, 1, 0));
+  return klaunch(ctx, "k_boundary", k_boundary<K, BOUNDARY_ITEMS>, ntiles,
+                 (double)n * (sizeof(K) + (in_group ? 4 : 0) + (out_group ? 4 : 0)), nullptr, 0.0, v, in_group, n,
+                 out_group, seg_off, status, ctx->small_d + 40, count_d);
+}
+
+struct DdLevel {
+  u32 kb, sm, tkb, Wl, wpt, D, shift;
+  bool last;
+  u64 nseg_max, nw_max, nlarge_max, ntiles_max;
+};
+
+// Batched segmented sort of (keys, vals) within the *nseg_d segments given
+// by seg_off; results in place.  lv: DD_LVL_MAX level-state records, zero.
+template <class K>
+int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* seg_off, const u32* nseg_d,
+                u64 nseg_max, u32* lv) {
+  if (n <= 1 || nseg_max == 0) return FMCS_OK;
+  const u32 small_max = std::min<u32>(ctx->tune.small_max, (u32)TILESORT_T / 2);
+  const u32 T = small_max <= 256 ? 512u : (u32)TILESORT_T;
+  u32 W = std::max<u32>(1, std::min<u32>(ctx->tune.window, small_max + 1));
+  if (W + small_max > T) W = T - small_max;
+  const u32 msd_bits = std::max<u32>(1, std::min<u32>(ctx->tune.msd_bits, RADIX_BITS));
+  // static level schedule with buffer bounds
+  std::vector<DdLevel> L;
+  {
+    u32 kb = std::min<u32>(keybits, 8 * sizeof(K));
+    u64 cur_max = nseg_max, prev_tiles = 0;
+    for (;;) {
+      DdLevel d{};
+      d.kb = kb;
+      d.sm = small_max;
+      d.tkb = kb;
+      if (kb + TS_SEGBITS > 64) {
+        d.sm = 1;
+        d.tkb = 0;
+      }
+      d.Wl = std::min<u32>(W, d.sm + 1);
+      d.wpt = (ONESWEEP_TILE + d.Wl - 1) / d.Wl;
+      d.nseg_max = cur_max;
+      d.nw_max = L.empty() ? (n + d.Wl - 1) / d.Wl : prev_tiles * d.wpt;
+      d.nlarge_max = std::min<u64>(cur_max, n / ((u64)d.sm + 1));
+      d.ntiles_max = std::min<u64>(n / ONESWEEP_TILE + d.nlarge_max, n);
+      d.last = (kb == 0);
+      if (!d.last) {
+        d.D = std::min<u32>(msd_bits, kb);
+        d.shift = kb - d.D;
+      }
+      L.push_back(d);
+      if (d.last || d.nlarge_max == 0) break;
+      if (L.size() >= DD_LVL_MAX - 1) return fail(ctx, FMCS_E_UNSUPPORTED, "too many segmented-sort levels");
+      cur_max = d.nlarge_max * RADIX;
+      prev_tiles = d.ntiles_max;
+      kb -= d.D;
+    }
+  }
+  u64 m_list[2] = {0, 0}, m_nw = 1, m_flag = 1, m_large = 1, m_tiles = 1;
+  for (size_t l = 0; l < L.size(); ++l) {
+    m_list[l & 1] = std::max<u64>(m_list[l & 1], L[l].nseg_max + 1);
+    m_nw = std::max<u64>(m_nw, L[l].nw_max);
+    m_flag = std::max<u64>(m_flag, L[l].nseg_max + 1);
+    m_large = std::max<u64>(m_large, L[l].nlarge_max + 1);
+    m_tiles = std::max<u64>(m_tiles, L[l].ntiles_max + 1);
+  }
+  K* kb_;
+  u32* vb_;
+  TRY(getbuf_t(ctx, WS_KB, n, &kb_));
+  TRY(getbuf_t(ctx, WS_VB, n, &vb_));
+  u32 *lst[2], *lsz[2];
+  TRY(getbuf_t(ctx, WS_ST, std::max<u64>(m_list[0], 1), &lst[0]));
+  TRY(getbuf_t(ctx, WS_SZ, std::max<u64>(m_list[0], 1), &lsz[0]));
+  TRY(getbuf_t(ctx, WS_BST, std::max<u64>(m_list[1], 1), &lst[1]));
+  TRY(getbuf_t(ctx, WS_BSZ, std::max<u64>(m_list[1], 1), &lsz[1]));
+  u32 *tlo, *thi, *ts0, *ts1, *wlist, *fb, *fblist, *flag, *pos, *llo, *lhi, *ltiles, *tbase, *hist, *dbase, *status;
+  TRY(getbuf_t(ctx, WS_TLO, m_nw, &tlo));
+  TRY(getbuf_t(ctx, WS_THI, m_nw, &thi));
+  TRY(getbuf_t(ctx, WS_TS0, m_nw, &ts0));
+  TRY(getbuf_t(ctx, WS_TS1, m_nw, &ts1));
+  TRY(getbuf_t(ctx, WS_WLIST, m_nw, &wlist));
+  TRY(getbuf_t(ctx, WS_FALLBACK, m_nw, &fb));
+  TRY(getbuf_t(ctx, WS_FBLIST, m_nw, &fblist));
+  TRY(getbuf_t(ctx, WS_FLAG, m_flag, &flag));
+  TRY(getbuf_t(ctx, WS_POS, m_flag, &pos));
+  TRY(getbuf_t(ctx, WS_LLO, m_large, &llo));
+  TRY(getbuf_t(ctx, WS_LHI, m_large, &lhi));
+  TRY(getbuf_t(ctx, WS_LTILES, m_large, &ltiles));
+  TRY(getbuf_t(ctx, WS_TBASE, m_large, &tbase));
+  TRY(getbuf_t(ctx, WS_HIST, m_large * RADIX, &hist));
+  TRY(getbuf_t(ctx, WS_DBASE, m_large * RADIX, &dbase));
+  TRY(getbuf_t(ctx, WS_STATUS, m_tiles * RADIX, &status));
+  // the tile kernels' grid: resident blocks of the chosen tile shape
+  const void* fn_bucket = T <= 512 ? (const void*)k_tile_bucket<K, 2> : (const void*)k_tile_bucket<K, 4>;
+  const void* fn_tsort = T <= 512 ? (const void*)k_tile_sort<K, 2> : (const void*)k_tile_sort<K, 4>;
+  TRY(klaunch(ctx, "k_offsets_to_list", kd_offsets_to_list, cap_grid((nseg_max + 255) / 256, 2048), 0.0, nullptr, 0.0,
+              seg_off, nseg_d, lst[0], lsz[0]));
+  const u32* plo = nullptr;
+  const u32* ptb = nullptr;
+  for (size_t l = 0; l < L.size(); ++l) {
+    const DdLevel& d = L[l];
+    u32* S = lv + l * DD_LVL_WORDS;
+    const u32* nseg_p = (l == 0) ? nseg_d : S + LV_NSEG;
+    const int cur = (int)(l & 1);
+    u32* cst = lst[cur];
+    u32* csz = lsz[cur];
+    TRY(klaunch(ctx, "k_windows", kd_windows, cap_grid((d.nseg_max + 2047) / 2048, 2048), 0.0, nullptr, 0.0,
+                (const u32*)cst, (const u32*)csz, nseg_p, d.Wl, d.sm, plo, ptb, d.wpt, tlo, thi, ts0, ts1, wlist,
+                S + LV_WCOUNT));
+    TRY(klaunch(ctx, "k_large_flags", kd_large_flags, cap_grid((d.nseg_max + 255) / 256, 1024), 0.0, nullptr, 0.0,
+                (const u32*)csz, nseg_p, d.sm, flag, S + LV_ROWS_SMALL));
+    TRY(dd_scan(ctx, flag, pos, nseg_p, d.nseg_max, S + LV_SCAN_A, S + LV_NLARGE));
+    TileSortArgs<K> a;
+    std::memset(&a, 0, sizeof(a));
+    a.kin[0] = keys; a.vin[0] = vals; a.kin[1] = kb_; a.vin[1] = vb_;
+    a.kout = keys; a.vout = vals;
+    a.tile_lo = tlo; a.tile_hi = thi; a.tile_s0 = ts0; a.tile_s1 = ts1; a.seg_start = cst;
+    a.src_sel = (u32)(l & 1); a.kb = d.tkb; a.pad_seg = T; a.direct = 0; a.wlist = wlist;
+    a.wcount = S + LV_WCOUNT; a.fblist = fblist; a.fbcount = S + LV_FB;
+    {
+      const u64 g = cap_grid(d.nw_max, resident(ctx, fn_bucket));
+      const double per = 2.0 * (sizeof(K) + 4);
+      if (T <= 512)
+        TRY(klaunch(ctx, "k_tile_bucket", k_tile_bucket<K, 2>, g, 0.0, S + LV_ROWS_SMALL, per, a, fb));
+      else
+        TRY(klaunch(ctx, "k_tile_bucket", k_tile_bucket<K, 4>, g, 0.0, S + LV_ROWS_SMALL, per, a, fb));
+    }
+    {
+      TileSortArgs<K> b = a;
+      b.wcount = nullptr;
+      const u64 g = cap_grid(d.nw_max, resident(ctx, fn_tsort));
+      if (T <= 512)
+        TRY(klaunch(ctx, "k_tile_sort", k_tile_sort<K, 2>, g, 0.0, nullptr, 0.0, b));
+      else
+        TRY(klaunch(ctx, "k_tile_sort", k_tile_sort<K, 4>, g, 0.0, nullptr, 0.0, b));
+    }
+    if (d.nlarge_max == 0) break;
+    const K* skey = (l & 1) ? kb_ : keys;
+    const u32* sval = (l & 1) ? vb_ : vals;
+    K* dkey = (l & 1) ? keys : kb_;
+    u32* dval = (l & 1) ? vals : vb_;
+    if (d.last && !(l & 1)) break;  // every key bit consumed; large segments already in place
+    TRY(klaunch(ctx, "k_large_compact", kd_large_compact, cap_grid((d.nseg_max + 255) / 256, 2048), 0.0, nullptr, 0.0,
+                (const u32*)cst, (const u32*)csz, nseg_p, d.sm, (const u32*)pos, (u32)ONESWEEP_TILE, llo, lhi, ltiles));
+    TRY(dd_scan(ctx, ltiles, tbase, S + LV_NLARGE, d.nlarge_max, S + LV_SCAN_B, S + LV_NTILES));
+    if (d.last) {
+      // odd level, every key bit consumed: move the large segments back
+      TRY(klaunch(ctx, "k_copy_segs", kd_copy_segs<K>, cap_grid(d.ntiles_max, 2048), 0.0, S + LV_ROWS_LARGE,
+                  2.0 * (sizeof(K) + 4), skey, sval, keys, vals, (const u32*)llo, (const u32*)lhi, (const u32*)tbase,
+                  (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES), (u32)ONESWEEP_TILE));
+      break;
+    }
+    TRY(dd_fill(ctx, hist, S + LV_NLARGE, RADIX, 1, 0, 0, d.nlarge_max * RADIX));
+    TRY(klaunch(ctx, "k_seg_hist", kd_seg_hist<K, ONESWEEP_ITEMS>, cap_grid(d.ntiles_max, 2048), 0.0,
+                S + LV_ROWS_LARGE, (double)sizeof(K), (const K*)skey, (const u32*)llo, (const u32*)lhi,
+                (const u32*)tbase, (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES), d.shift, d.D, hist));
+    const int nxt = cur ^ 1;
+    TRY(klaunch(ctx, "k_digit_base", kd_digit_base, cap_grid(d.nlarge_max, 2048), 0.0, nullptr, 0.0,
+                (const u32*)hist, (const u32*)llo, dbase, lst[nxt], lsz[nxt], (const u32*)(S + LV_NLARGE)));
+    TRY(dd_fill(ctx, status, S + LV_NTILES, RADIX, 1, 0, 0, d.ntiles_max * RADIX));
+    TRY(klaunch(ctx, "k_onesweep_msd", k_onesweep<K, ONESWEEP_ITEMS>,
+                cap_grid(d.ntiles_max, resident(ctx, (const void*)k_onesweep<K, ONESWEEP_ITEMS>)), 0.0,
+                S + LV_ROWS_LARGE, 2.0 * (sizeof(K) + 4), skey, sval, dkey, dval, (const u32*)llo, (const u32*)lhi,
+                (const u32*)tbase, 0u, (const u32*)dbase, status, S + LV_OS, d.shift, d.D,
+                (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES)));
+    if (l + 1 < L.size())
+      TRY(klaunch(ctx, "k_set", kd_mul, 1, 0.0, nullptr, 0.0, (const u32*)(S + LV_NLARGE), (u32)RADIX,
+                  S + DD_LVL_WORDS + LV_NSEG));
+    plo = llo;
+    ptb = tbase;
+  }
+  return FMCS_OK;
+}
+
+// Device-driven segmented sort with a host-known segment count (one level
+// area; the count is written on the device, no synchronisation).
+template <class K>
+int seg_sort_dd_host(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* seg_off, u32 nseg) {
+  if (n <= 1 || nseg == 0) return FMCS_OK;
+  u32* dd;
+  TRY(getbuf_t(ctx, WS_DD, DD_WORDS, &dd));
+  TRY(fill(ctx, dd, DD_HDR + (u64)DD_LVL_MAX * DD_LVL_WORDS, 0));
+  TRY(klaunch(ctx, "k_set", kd_set4, 1, 0.0, nullptr, 0.0, dd + 80, nseg, 0u, 0u, 0u));
+  TRY(seg_sort_dd<K>(ctx, keys, vals, n, keybits, seg_off, dd + 80, nseg, dd + DD_HDR));
+  if (ctx->prof) TRY(prof_flush(ctx));
+  return FMCS_OK;
+}
+
+template <class K>
+int gather_launch(fmcs_ctx* ctx, const K* column, const u32* order, K* out, u64 n) {
+  if (((uintptr_t)order & 15) != 0) return fail(ctx, FMCS_E_ARG, "order must be 16-byte aligned");
+  return klaunch(ctx, "k_gather", k_gather<K>, (n + 1023) / 1024, (double)n * (4 + 2 * sizeof(K)), nullptr, 0.0,
+                 column, order, out, n);
+}
+
+template <class K>
+int dd_round(fmcs_ctx* ctx, u32 r, u32 keybits, u64 n, const K* column, u32* order, const u32* gin,
+             const u32* soff, u32* gout, u32* oout, bool want_groups, u32* dd) {
+  K* ka;
+  TRY(getbuf_t(ctx, WS_KA, n, &ka));
+  if (r == 0) {
+    TRY(radix_sort_dd<K>(ctx, column, order, ka, order, n, keybits, dd));
+  } else {
+    TRY(gather_launch<K>(ctx, column, order, ka, n));
+    u32* lv = dd + DD_HDR + (u64)r * DD_LVL_MAX * DD_LVL_WORDS;
+    TRY(seg_sort_dd<K>(ctx, ka, order, n, keybits, soff, dd + (r - 1), n, lv));
+  }
+  if (want_groups) TRY(boundary_dd<K>(ctx, ka, gin, n, gout, oout, dd + r));
+  return FMCS_OK;
+}
+
+// The whole multi-round sort, enqueued without a host synchronisation.
+int mrs_dd(fmcs_ctx* ctx, u32 nrounds, const void* const* cols, const u32* banks, const u32* bw, u64 n, u32* order,
+           u32* groups_d) {
+  u32* dd;
+  TRY(getbuf_t(ctx, WS_DD, DD_WORDS, &dd));
+  TRY(fill(ctx, dd, DD_WORDS, 0));
+  TRY(klaunch(ctx, "k_iota", k_iota_u32, cap_grid((n + 255) / 256, 8192), 0.0, nullptr, 0.0, order, n));
+  u32 *ga, *gb, *offa, *offb;
+  TRY(getbuf_t(ctx, WS_HASH, n, &ga));
+  TRY(getbuf_t(ctx, WS_HASH2, n, &gb));
+  TRY(getbuf_t(ctx, WS_SEGOFF, n + 1, &offa));
+  TRY(getbuf_t(ctx, WS_SEGOFF2, n + 1, &offb));
+  const u32* gin = nullptr;
+  const u32* soff = nullptr;
+  for (u32 r = 0; r < nrounds; ++r) {
+    const bool last = (r + 1 == nrounds);
+    const bool want_groups = !last || groups_d;
+    u32* gout = last && groups_d ? groups_d : ((gin == ga) ? gb : ga);
+    u32* oout = (soff == offa) ? offb : offa;
+    int rc;
+    switch (banks[r]) {
+      case 2: rc = dd_round<u16>(ctx, r, bw[r], n, (const u16*)cols[r], order, gin, soff, gout, oout, want_groups, dd); break;
+      case 4: rc = dd_round<u32>(ctx, r, bw[r], n, (const u32*)cols[r], order, gin, soff, gout, oout, want_groups, dd); break;
+      default: rc = dd_round<u64>(ctx, r, bw[r], n, (const u64*)cols[r], order, gin, soff, gout, oout, want_groups, dd); break;
+    }
+    if (rc != FMCS_OK) return rc;
+    gin = gout;
+    soff = oout;
+  }
+  return FMCS_OK;
+}
+
+// Host view of the device-driven run: the group count and the key error.
+int dd_finish(fmcs_ctx* ctx, u32 nrounds, bool have_groups, u32* ngroups) {
+  u32* dd;
+  TRY(getbuf_t(ctx, WS_DD, DD_WORDS, &dd));
+  u32 h[DD_ERR + 1];
+  TRY(readback(ctx, dd, DD_ERR + 1, h));
+  ctx->dd_pending = false;
+  if (h[DD_ERR]) return fail(ctx, FMCS_E_ARG, "keys exceed keybits");
+  if (ngroups) *ngroups = have_groups ? h[nrounds - 1] : (nrounds >= 2 ? h[nrounds - 2] : 0u);
+  return FMCS_OK;
+}
+
+void graph_drop(fmcs_ctx* ctx) {
+  if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+  ctx->gexec = nullptr;
+  ctx->graph = nullptr;
+  ctx->gkey.clear();
+}
+
+// Replays the cached graph of this exact call, or runs the call eagerly and
+// captures it for the next one.  The graph names workspace buffers, so the
+// key holds the workspace epoch: any reallocation forces a new capture.
+int mrs_graph(fmcs_ctx* ctx, u32 nrounds, const void* const* cols, const u32* banks, const u32* bw, u64 n, u32* order,
+              u32* groups_d) {
+  std::vector<u64> key = {(u64)nrounds, n, (u64)(uintptr_t)order, (u64)(uintptr_t)groups_d, ctx->tune.small_max,
+                          ctx->tune.window, ctx->tune.msd_bits, (u64)(uintptr_t)ctx->stream};
+  for (u32 r = 0; r < nrounds; ++r) {
+    key.push_back((u64)(uintptr_t)cols[r]);
+    key.push_back(((u64)banks[r] << 32) | bw[r]);
+  }
+  if (ctx->gexec && ctx->gkey.size() == key.size() + 1 && std::equal(key.begin(), key.end(), ctx->gkey.begin()) &&
+      ctx->gkey.back() == ctx->ws_epoch) {
+    HIPCHK(hipGraphLaunch(ctx->gexec, ctx->stream));
+    return FMCS_OK;
+  }
+  graph_drop(ctx);
+  TRY(mrs_dd(ctx, nrounds, cols, banks, bw, n, order, groups_d));  // allocates, computes this call's result
+  if (!ctx->cap_stream) HIPCHK(hipStreamCreateWithFlags(&ctx->cap_stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  hipStream_t user = ctx->stream;
+  ctx->stream = ctx->cap_stream;
+  ctx->capturing = true;
+  hipGraph_t g = nullptr;
+  int rc = FMCS_OK;
+  if (hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    rc = fail(ctx, FMCS_E_HIP, "hipStreamBeginCapture failed");
+  } else {
+    rc = mrs_dd(ctx, nrounds, cols, banks, bw, n, order, groups_d);
+    hipError_t e = hipStreamEndCapture(ctx->cap_stream, &g);
+    if (rc == FMCS_OK && e != hipSuccess) rc = fail(ctx, FMCS_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  }
+  ctx->capturing = false;
+  ctx->stream = user;
+  if (rc != FMCS_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return rc;
+  }
+  hipGraphExec_t ex = nullptr;
+  if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return fail(ctx, FMCS_E_HIP, "hipGraphInstantiate failed");
+  }
+  ctx->graph = g;
+  ctx->gexec = ex;
+  ctx->gkey = key;
+  ctx->gkey.push_back(ctx->ws_epoch);
   return FMCS_OK;
 }
 
@@ -731,6 +1220,8 @@ int fmcs_ctx_create(int device, fmcs_ctx** out) {
 int fmcs_ctx_destroy(fmcs_ctx* ctx) {
   if (!ctx) return FMCS_E_ARG;
   (void)hipStreamSynchronize(ctx->stream);
+  graph_drop(ctx);
+  if (ctx->cap_stream) (void)hipStreamDestroy(ctx->cap_stream);
   for (auto& b : ctx->ws)
     if (b.p) (void)hipFree(b.p);
   (void)hipFree(ctx->small_d);
@@ -752,6 +1243,7 @@ int fmcs_ctx_set_stream(fmcs_ctx* ctx, void* stream) {
 int fmcs_ctx_sync(fmcs_ctx* ctx) {
   if (!ctx) return FMCS_E_ARG;
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (ctx->dd_pending) TRY(dd_finish(ctx, 1, true, nullptr));
   return FMCS_OK;
 }
 
@@ -773,6 +1265,8 @@ int fmcs_ctx_get_tuning(fmcs_ctx* ctx, fmcs_tuning* t) {
 int fmcs_ctx_trim(fmcs_ctx* ctx) {
   if (!ctx) return FMCS_E_ARG;
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  graph_drop(ctx);
+  ctx->ws_epoch++;
   for (auto& b : ctx->ws) {
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
@@ -967,6 +1461,19 @@ int fmcs_multi_round_sort(fmcs_ctx* ctx, uint32_t nrounds, const void* const* co
     if (!bank_ok(banks[r]) || bitwidths[r] == 0 || bitwidths[r] > 8 * banks[r])
       return fail(ctx, FMCS_E_ARG, "bad bank/bitwidth");
   if (n == 0) return FMCS_OK;
+  if (mode == FMCS_MODE_DEVICE || mode == FMCS_MODE_GRAPH) {
+    if (nrounds > DD_ROUNDS_MAX) return fail(ctx, FMCS_E_UNSUPPORTED, "device-driven path supports at most 64 rounds");
+    if (((uintptr_t)order_d & 15) != 0) return fail(ctx, FMCS_E_ARG, "order must be 16-byte aligned");
+    if (mode == FMCS_MODE_DEVICE) {
+      TRY(mrs_dd(ctx, nrounds, cols_d, banks, bitwidths, n, order_d, groups_d));
+      if (ctx->prof) TRY(prof_flush(ctx));
+    } else {
+      TRY(mrs_graph(ctx, nrounds, cols_d, banks, bitwidths, n, order_d, groups_d));
+    }
+    ctx->dd_pending = true;
+    if (ngroups) TRY(dd_finish(ctx, nrounds, groups_d != nullptr, ngroups));
+    return FMCS_OK;
+  }
   // order = identity (src/composer.cpp:60-63)
   hipLaunchKernelGGL(k_iota_u32, dim3((u32)std::min<u64>((n + 255) / 256, 8192)), dim3(256), 0, ctx->stream, order_d, n);
   TRY(launch_check(ctx, "k_iota_u32"));

@@ -1,0 +1,295 @@
+// fmcs_dd.hip -- device-driven list kernels of the batched segmented sort.
+//
+// Every size these kernels work on (segment-list lengths, large-segment and
+// tile counts, window counts) is read from device memory, and the launches
+// are grid-stride or persistent, so a whole round runs without one host
+// synchronisation and can be captured in a hipGraph.
+#include "fmcs_kernels.h"
+
+namespace fmcs {
+
+__global__ void kd_set4(u32* __restrict__ p, u32 a, u32 b, u32 c, u32 d) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    p[0] = a;
+    p[1] = b;
+    p[2] = c;
+    p[3] = d;
+  }
+}
+
+// *out = *a * m
+__global__ void kd_mul(const u32* __restrict__ a, u32 m, u32* __restrict__ out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *out = *a * m;
+}
+
+// p[0 .. ceil(*cnt * mult / div) + add) = v
+__global__ void kd_fill(u32* __restrict__ p, const u32* __restrict__ cnt, u32 mult, u32 div, u32 add, u32 v) {
+  const u64 words = ((u64)*cnt * mult + div - 1) / div + add;
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (u64)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void kd_offsets_to_list(const u32* __restrict__ off, const u32* __restrict__ nseg_d, u32* __restrict__ st,
+                                   u32* __restrict__ sz) {
+  const u32 nseg = *nseg_d;
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < nseg; i += gridDim.x * blockDim.x) {
+    u32 a = off[i], b = off[i + 1];
+    st[i] = a;
+    sz[i] = b - a;
+  }
+}
+
+__device__ __forceinline__ u32 window_of_d(const u32* st, u32 i, u32 W, const u32* plo, const u32* pwb, u32 wpt) {
+  if (!plo) return st[i] / W;
+  u32 p = i >> RADIX_BITS;
+  return pwb[p] * wpt + (st[i] - plo[p]) / W;
+}
+
+// Window packing (see k_windows).  A block takes WIN_CHUNK consecutive
+// segments at a time, collects the windows that start a run in LDS (one
+// LDS atomic per wave) and appends them to the compact list with one global
+// atomic per chunk, so the list counter sees few contended updates.
+constexpr u32 WIN_CHUNK = 2048;
+__global__ __launch_bounds__(256) void kd_windows(const u32* __restrict__ st, const u32* __restrict__ sz,
+                                                  const u32* __restrict__ nseg_d, u32 W, u32 small_max,
+                                                  const u32* __restrict__ plo, const u32* __restrict__ pwb, u32 wpt,
+                                                  u32* __restrict__ tlo, u32* __restrict__ thi, u32* __restrict__ ts0,
+                                                  u32* __restrict__ ts1, u32* __restrict__ wlist,
+                                                  u32* __restrict__ wcount) {
+  __shared__ u32 s_w[WIN_CHUNK];
+  __shared__ u32 s_n, s_base;
+  const u32 nseg = *nseg_d;
+  const u32 lane = __lane_id();
+  for (u64 c0 = (u64)blockIdx.x * WIN_CHUNK; c0 < nseg; c0 += (u64)gridDim.x * WIN_CHUNK) {
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    for (u32 j = 0; j < WIN_CHUNK; j += 256) {
+      const u64 i64 = c0 + j + threadIdx.x;
+      bool first = false;
+      u32 w = 0;
+      if (i64 < nseg) {
+        const u32 i = (u32)i64;
+        const u32 size = sz[i];
+        if (size <= small_max) {
+          w = window_of_d(st, i, W, plo, pwb, wpt);
+          first = (i == 0) || (window_of_d(st, i - 1, W, plo, pwb, wpt) != w);
+          bool last = (i + 1 == nseg) || (window_of_d(st, i + 1, W, plo, pwb, wpt) != w) || (sz[i + 1] > small_max);
+          if (first) { tlo[w] = st[i]; ts0[w] = i; }
+          if (last) { thi[w] = st[i] + size; ts1[w] = i; }
+        }
+      }
+      const u64 m = __ballot(first);
+      if (m) {
+        const u32 leader = (u32)(__ffsll((long long)m) - 1);
+        u32 b = 0;
+        if (lane == leader) b = atomicAdd(&s_n, (u32)__popcll(m));
+        b = __shfl(b, (int)leader);
+        if (first) s_w[b + (u32)__popcll(m & lanemask_lt())] = w;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_base = s_n ? atomicAdd(wcount, s_n) : 0u;
+    __syncthreads();
+    for (u32 k = threadIdx.x; k < s_n; k += 256) wlist[s_base + k] = s_w[k];
+    __syncthreads();
+  }
+}
+
+// flag[i] = segment i is large; rows[0] += small rows, rows[1] += large rows
+__global__ void kd_large_flags(const u32* __restrict__ sz, const u32* __restrict__ n_d, u32 small_max,
+                               u32* __restrict__ flag, u32* __restrict__ rows) {
+  const u32 nseg = *n_d;
+  u32 a = 0, b = 0;
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < nseg; i += gridDim.x * blockDim.x) {
+    u32 s = sz[i];
+    bool large = s > small_max;
+    flag[i] = large ? 1u : 0u;
+    a += large ? 0u : s;
+    b += large ? s : 0u;
+  }
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) {
+    a += __shfl_down(a, o);
+    b += __shfl_down(b, o);
+  }
+  __shared__ u32 s_ab[2];
+  if (threadIdx.x == 0) s_ab[0] = s_ab[1] = 0;
+  __syncthreads();
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    if (a) atomicAdd(&s_ab[0], a);
+    if (b) atomicAdd(&s_ab[1], b);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s_ab[0]) atomicAdd(rows, s_ab[0]);
+    if (s_ab[1]) atomicAdd(rows + 1, s_ab[1]);
+  }
+}
+
+// Exclusive scan with decoupled look-back, persistent: blocks take tiles from
+// the counter until ceil(n / TILE) are done.  *total = the sum (0 when n = 0).
+template <int ITEMS>
+__global__ __launch_bounds__(256) void kd_scan_excl(const u32* __restrict__ in, u32* __restrict__ out,
+                                                    const u32* __restrict__ n_d, u32* __restrict__ status,
+                                                    u32* __restrict__ counter, u32* __restrict__ total) {
+  constexpr int NT = 256, TILE = NT * ITEMS;
+  __shared__ u32 s_tile, s_excl;
+  __shared__ u32 s_tmp[NT / WAVE + 1];
+  const u32 n = *n_d;
+  const u32 ntiles = (n + TILE - 1) / TILE;
+  const u32 tid = threadIdx.x;
+  if (n == 0) {
+    if (blockIdx.x == 0 && tid == 0) *total = 0;
+    return;
+  }
+  for (;;) {
+    if (tid == 0) s_tile = atomicAdd(counter, 1u);
+    __syncthreads();
+    const u32 tile = s_tile;
+    if (tile >= ntiles) break;
+    const u64 base = (u64)tile * TILE + (u64)tid * ITEMS;
+    u32 v[ITEMS];
+    u32 sum = 0;
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      v[j] = (base + j < n) ? in[base + j] : 0u;
+      sum += v[j];
+    }
+    u32 tot;
+    u32 ex = block_excl_scan<NT>(sum, s_tmp, &tot);
+    if (tid == 0) {
+      u32 e = 0;
+      if (tile == 0) {
+        status_store(status, ST_INC | tot);
+      } else {
+        status_store(status + tile, ST_AGG | tot);
+        e = lookback(status, tile, 0, 1);
+        status_store(status + tile, ST_INC | (e + tot));
+      }
+      s_excl = e;
+      if (tile + 1 == ntiles) *total = e + tot;
+    }
+    __syncthreads();
+    u32 run = s_excl + ex;
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      if (base + j < n) out[base + j] = run;
+      run += v[j];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void kd_large_compact(const u32* __restrict__ st, const u32* __restrict__ sz, const u32* __restrict__ n_d,
+                                 u32 small_max, const u32* __restrict__ pos, u32 tile_rows, u32* __restrict__ llo,
+                                 u32* __restrict__ lhi, u32* __restrict__ ltiles) {
+  const u32 nseg = *n_d;
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < nseg; i += gridDim.x * blockDim.x) {
+    u32 s = sz[i];
+    if (s <= small_max) continue;
+    u32 p = pos[i];
+    llo[p] = st[i];
+    lhi[p] = st[i] + s;
+    ltiles[p] = (s + tile_rows - 1) / tile_rows;
+  }
+}
+
+// Per-segment digit histogram, tiles never crossing a segment; grid-stride
+// over *ntiles_d tiles of the *nsegs_d large segments.
+template <class K, int ITEMS>
+__global__ __launch_bounds__(256) void kd_seg_hist(const K* __restrict__ keys, const u32* __restrict__ seg_lo,
+                                                   const u32* __restrict__ seg_hi, const u32* __restrict__ tile_base,
+                                                   const u32* __restrict__ nsegs_d, const u32* __restrict__ ntiles_d,
+                                                   u32 shift, u32 dbits, u32* __restrict__ hist) {
+  constexpr int TILE = 256 * ITEMS;
+  __shared__ u32 sh[RADIX];
+  __shared__ u32 s_seg;
+  const u32 nsegs = *nsegs_d, ntiles = *ntiles_d;
+  const u32 mask = (1u << dbits) - 1;
+  for (u32 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    sh[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s_seg = upper_bound_m1(tile_base, nsegs, tile);
+    __syncthreads();
+    const u32 seg = s_seg;
+    const u64 row0 = (u64)seg_lo[seg] + (u64)(tile - tile_base[seg]) * TILE;
+    const u64 rowend = min((u64)seg_hi[seg], row0 + TILE);
+    for (u64 r = row0 + threadIdx.x; r < rowend; r += 256) atomicAdd(&sh[digit_of(keys[r], shift, mask)], 1u);
+    __syncthreads();
+    u32 c = sh[threadIdx.x];
+    if (c) atomicAdd(&hist[(u64)seg * RADIX + threadIdx.x], c);
+    __syncthreads();
+  }
+}
+
+// digit_base[s][d] = lo[s] + prefix; bucket list (start, size) in (s, d)
+// order; loops over *nsets_d sets.
+__global__ __launch_bounds__(256) void kd_digit_base(const u32* __restrict__ hist, const u32* __restrict__ lo,
+                                                     u32* __restrict__ digit_base, u32* __restrict__ bstart,
+                                                     u32* __restrict__ bsize, const u32* __restrict__ nsets_d) {
+  __shared__ u32 tmp[8];
+  const u32 nsets = *nsets_d;
+  const u32 d = threadIdx.x;
+  for (u32 s = blockIdx.x; s < nsets; s += gridDim.x) {
+    u32 c = hist[(u64)s * RADIX + d];
+    u32 tot;
+    u32 ex = block_excl_scan<256>(c, tmp, &tot);
+    u32 b = lo[s] + ex;
+    digit_base[(u64)s * RADIX + d] = b;
+    bstart[(u64)s * RADIX + d] = b;
+    bsize[(u64)s * RADIX + d] = c;
+  }
+}
+
+template <class K>
+__global__ void kd_copy_segs(const K* __restrict__ kin, const u32* __restrict__ vin, K* __restrict__ kout,
+                             u32* __restrict__ vout, const u32* __restrict__ llo, const u32* __restrict__ lhi,
+                             const u32* __restrict__ tile_base, const u32* __restrict__ nsegs_d,
+                             const u32* __restrict__ ntiles_d, u32 tile_rows) {
+  __shared__ u32 s_seg;
+  const u32 nsegs = *nsegs_d, ntiles = *ntiles_d;
+  for (u32 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    if (threadIdx.x == 0) s_seg = upper_bound_m1(tile_base, nsegs, tile);
+    __syncthreads();
+    const u32 seg = s_seg;
+    const u64 r0 = (u64)llo[seg] + (u64)(tile - tile_base[seg]) * tile_rows;
+    const u64 r1 = min((u64)lhi[seg], r0 + tile_rows);
+    for (u64 r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+      kout[r] = kin[r];
+      vout[r] = vin[r];
+    }
+    __syncthreads();
+  }
+}
+
+// Keys of the last LSD pass must fit its `bits`: a count in a digit at or
+// above 2^bits means a key exceeds the declared bit width; flag it.
+__global__ void kd_check_digits(const u32* __restrict__ hist, u32 bits, u32* __restrict__ err) {
+  const u32 d = threadIdx.x;
+  if (d >= (1u << bits) && d < RADIX && hist[d] != 0) atomicOr(err, 1u);
+}
+
+template <class K>
+__global__ void kd_copy_pairs(const K* __restrict__ kin, const u32* __restrict__ vin, K* __restrict__ kout,
+                              u32* __restrict__ vout, u64 n) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) {
+    kout[i] = kin[i];
+    vout[i] = vin[i];
+  }
+}
+
+// ---- profiling helper: snapshot a device counter into a record slot
+__global__ void kd_copy1(const u32* __restrict__ src, u32* __restrict__ dst) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *dst = *src;
+}
+
+#define FMCS_DD_INST(K)                                                                                            \
+  template __global__ void kd_seg_hist<K, 16>(const K*, const u32*, const u32*, const u32*, const u32*,          \
+                                              const u32*, u32, u32, u32*);                                       \
+  template __global__ void kd_copy_segs<K>(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*,   \
+                                           const u32*, const u32*, u32);                            \
+  template __global__ void kd_copy_pairs<K>(const K*, const u32*, K*, u32*, u64);
+FMCS_DD_INST(u16)
+FMCS_DD_INST(u32)
+FMCS_DD_INST(u64)
+template __global__ void kd_scan_excl<8>(const u32*, u32*, const u32*, u32*, u32*, u32*);
+
+}  // namespace fmcs

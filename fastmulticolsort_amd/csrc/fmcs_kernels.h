@@ -41,6 +41,9 @@ struct TileSortArgs {
   u32 d_lo, d_hi;
   const u32* only;   // if set, only tiles w with only[w] != 0 are sorted
   const u32* wlist;  // if set, block b sorts window wlist[b]
+  const u32* wcount;  // if set: persistent loop over wlist[0 .. *wcount)
+  u32* fblist;        // bucket tiles refused -> appended here (with fbcount)
+  u32* fbcount;       // radix tile sort: persistent loop over fblist[0 .. *fbcount)
 };
 
 constexpr int ONESWEEP_ITEMS = 16;
@@ -64,7 +67,7 @@ __global__ void k_seg_hist(const K*, const u32*, const u32*, const u32*, u32, u3
 __global__ void k_digit_base(const u32*, const u32*, u32*, u32*, u32*);
 template <class K, int ITEMS>
 __global__ void k_onesweep(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, u32, const u32*,
-                           u32*, u32*, u32, u32);
+                           u32*, u32*, u32, u32, const u32*, const u32*);
 template <class K, int E>
 __global__ void k_tile_sort(TileSortArgs<K>);
 template <class K, int E>
@@ -92,5 +95,28 @@ __global__ void k_hash_assign(const u32*, const u32*, u64, u32*);
 template <class K>
 __global__ void k_fill_random(K*, u64, u64, u32);
 __global__ void k_uniform_offsets(u32*, u32, u32, u64);
+
+
+// Device-driven list kernels (fmcs_dd.hip): sizes come from device words.
+__global__ void kd_set4(u32*, u32, u32, u32, u32);
+__global__ void kd_mul(const u32*, u32, u32*);
+__global__ void kd_fill(u32*, const u32*, u32, u32, u32, u32);
+__global__ void kd_offsets_to_list(const u32*, const u32*, u32*, u32*);
+__global__ void kd_windows(const u32*, const u32*, const u32*, u32, u32, const u32*, const u32*, u32, u32*, u32*,
+                           u32*, u32*, u32*, u32*);
+__global__ void kd_large_flags(const u32*, const u32*, u32, u32*, u32*);
+template <int ITEMS>
+__global__ void kd_scan_excl(const u32*, u32*, const u32*, u32*, u32*, u32*);
+__global__ void kd_large_compact(const u32*, const u32*, const u32*, u32, const u32*, u32, u32*, u32*, u32*);
+template <class K, int ITEMS>
+__global__ void kd_seg_hist(const K*, const u32*, const u32*, const u32*, const u32*, const u32*, u32, u32, u32*);
+__global__ void kd_digit_base(const u32*, const u32*, u32*, u32*, u32*, const u32*);
+template <class K>
+__global__ void kd_copy_segs(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, const u32*,
+                             const u32*, u32);
+__global__ void kd_copy1(const u32*, u32*);
+__global__ void kd_check_digits(const u32*, u32, u32*);
+template <class K>
+__global__ void kd_copy_pairs(const K*, const u32*, K*, u32*, u64);
 
 }  // namespace fmcs

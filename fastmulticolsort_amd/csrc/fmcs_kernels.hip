@@ -287,7 +287,8 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
                                                   const u32* __restrict__ seg_lo, const u32* __restrict__ seg_hi,
                                                   const u32* __restrict__ tile_base, u32 nsegs,
                                                   const u32* __restrict__ digit_base, u32* __restrict__ status,
-                                                  u32* __restrict__ counter, u32 shift, u32 dbits) {
+                                                  u32* __restrict__ counter, u32 shift, u32 dbits,
+                                                  const u32* __restrict__ nsegs_d, const u32* __restrict__ ntiles_d) {
   constexpr int NT = 256, NW = NT / WAVE, TILE = NT * ITEMS, WITEMS = WAVE * ITEMS;
   __shared__ u32 s_tile, s_seg;
   __shared__ u32 s_whist[NW][RADIX];
@@ -298,13 +299,20 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
   __shared__ u32 s_vals[TILE];
 
   u32 tid = threadIdx.x, wv = tid / WAVE, lane = tid & (WAVE - 1);
+  if (nsegs_d) nsegs = *nsegs_d;
+  const u32 ntiles = ntiles_d ? *ntiles_d : tile_base[nsegs];
+  // Persistent: a block takes tiles from the counter until none is left.
+  // Tiles are taken in id order, so every predecessor a look-back waits on
+  // is held by a running block that publishes without waiting.
+  for (;;) {
   for (u32 i = tid; i < NW * RADIX; i += NT) (&s_whist[0][0])[i] = 0;
   if (tid == 0) {
     u32 t = atomicAdd(counter, 1u);
     s_tile = t;
-    s_seg = upper_bound_m1(tile_base, nsegs, t);
+    s_seg = t < ntiles ? upper_bound_m1(tile_base, nsegs, t) : 0u;
   }
   __syncthreads();
+  if (s_tile >= ntiles) break;
   const u32 tile = s_tile, seg = s_seg;
   const u32 first_tile = tile_base[seg];
   const u64 row0 = (u64)seg_lo[seg] + (u64)(tile - first_tile) * TILE;
@@ -385,6 +393,8 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
     kout[g] = k;
     vout[g] = s_vals[i];
   }
+  __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -453,7 +463,7 @@ __device__ __forceinline__ void block_or_and(u64 v_or, u64 v_and, u64* s_red, u6
 }
 
 template <class K, int E>
-__global__ __launch_bounds__(256) void k_tile_sort(TileSortArgs<K> a) {
+__device__ __forceinline__ void tile_sort_one(const TileSortArgs<K>& a, u32 w) {
   constexpr int NT = 256, NW = NT / WAVE, T = NT * E;
   __shared__ u64 s_c[T];
   __shared__ u32 s_o[T];
@@ -468,7 +478,6 @@ __global__ __launch_bounds__(256) void k_tile_sort(TileSortArgs<K> a) {
   if (a.direct) {
     lo = a.d_lo; hi = a.d_hi; s0 = 1; s1 = 0;
   } else {
-    u32 w = a.wlist ? a.wlist[blockIdx.x] : blockIdx.x;
     if (a.only && a.only[w] == 0) return;
     lo = a.tile_lo[w]; hi = a.tile_hi[w]; s0 = a.tile_s0[w]; s1 = a.tile_s1[w];
   }
@@ -578,6 +587,21 @@ __global__ __launch_bounds__(256) void k_tile_sort(TileSortArgs<K> a) {
   }
 }
 
+// One tile per block, or (a.fbcount set) a persistent loop over the tiles
+// the bucket sort refused.
+template <class K, int E>
+__global__ __launch_bounds__(256) void k_tile_sort(TileSortArgs<K> a) {
+  if (a.fbcount) {
+    const u32 cnt = *a.fbcount;
+    for (u32 b = blockIdx.x; b < cnt; b += gridDim.x) {
+      tile_sort_one<K, E>(a, a.fblist[b]);
+      __syncthreads();
+    }
+  } else {
+    tile_sort_one<K, E>(a, a.wlist ? a.wlist[blockIdx.x] : blockIdx.x);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Fast path of the tile sort: a per-segment bucket sort.  Segment s of m_s
 // rows gets 2^ceil(log2 m_s) bins (about one row per bin) cut from the top of
@@ -592,7 +616,7 @@ __global__ __launch_bounds__(256) void k_tile_sort(TileSortArgs<K> a) {
 constexpr u32 BUCKET_LIMIT = 24;
 
 template <class K, int E>
-__global__ __launch_bounds__(256) void k_tile_bucket(TileSortArgs<K> a, u32* __restrict__ fallback) {
+__device__ __forceinline__ void tile_bucket_one(const TileSortArgs<K>& a, u32 w, u32* __restrict__ fallback) {
   constexpr int NT = 256, T = NT * E, NB = 2 * T, BPT = NB / NT, SE = T / NT;
   __shared__ u64 s_k[T];
   __shared__ u32 s_o[T];
@@ -607,7 +631,6 @@ __global__ __launch_bounds__(256) void k_tile_bucket(TileSortArgs<K> a, u32* __r
   __shared__ u32 s_tmp[NT / WAVE + 1];
   __shared__ u32 s_over;
   const u32 tid = threadIdx.x;
-  const u32 w = a.wlist ? a.wlist[blockIdx.x] : blockIdx.x;
   const u32 lo = a.tile_lo[w], hi = a.tile_hi[w];
   if (lo >= hi) return;
   const u32 s0 = a.tile_s0[w], s1 = a.tile_s1[w];
@@ -705,7 +728,10 @@ __global__ __launch_bounds__(256) void k_tile_bucket(TileSortArgs<K> a, u32* __r
   u32 tot;
   u32 crun = block_excl_scan<NT>(csum, s_tmp, &tot);
   if (s_over) {
-    if (tid == 0) fallback[w] = 1;
+    if (tid == 0) {
+      fallback[w] = 1;
+      if (a.fblist) a.fblist[atomicAdd(a.fbcount, 1u)] = w;
+    }
     return;
   }
 #pragma unroll
@@ -748,6 +774,21 @@ __global__ __launch_bounds__(256) void k_tile_bucket(TileSortArgs<K> a, u32* __r
   for (u32 i = tid; i < cnt; i += NT) {
     if (a.kout) a.kout[lo + i] = (K)s_k[i];
     a.vout[lo + i] = s_o[i];
+  }
+}
+
+// One window per block, or (a.wcount set) a persistent loop over the
+// compact window list.
+template <class K, int E>
+__global__ __launch_bounds__(256) void k_tile_bucket(TileSortArgs<K> a, u32* __restrict__ fallback) {
+  if (a.wcount) {
+    const u32 cnt = *a.wcount;
+    for (u32 b = blockIdx.x; b < cnt; b += gridDim.x) {
+      tile_bucket_one<K, E>(a, a.wlist[b], fallback);
+      __syncthreads();
+    }
+  } else {
+    tile_bucket_one<K, E>(a, a.wlist ? a.wlist[blockIdx.x] : blockIdx.x, fallback);
   }
 }
 
@@ -1016,7 +1057,7 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
   template __global__ void k_hist_all<K>(const K*, u64, u32, u32, u32*);                                            \
   template __global__ void k_seg_hist<K, 16>(const K*, const u32*, const u32*, const u32*, u32, u32, u32, u32*);     \
   template __global__ void k_onesweep<K, 16>(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*,    \
-                                             u32, const u32*, u32*, u32*, u32, u32);                                \
+                                             u32, const u32*, u32*, u32*, u32, u32, const u32*, const u32*);                                \
   template __global__ void k_tile_sort<K, 4>(TileSortArgs<K>);                                                      \
   template __global__ void k_tile_sort<K, 2>(TileSortArgs<K>);                                                      \
   template __global__ void k_tile_bucket<K, 4>(TileSortArgs<K>, u32*);                                             \

@@ -9,7 +9,7 @@ import ctypes as C
 import torch
 
 from . import _lib
-from ._lib import Entry, FmcsError, GroupInfo, Tuning, MODE_BATCHED, MODE_PLAIN, MODE_UNBINNED
+from ._lib import Entry, FmcsError, GroupInfo, Tuning, MODE_BATCHED, MODE_PLAIN, MODE_UNBINNED, MODE_DEVICE, MODE_GRAPH
 
 _BANK = {torch.uint8: 1, torch.uint16: 2, torch.uint32: 4, torch.uint64: 8}
 _DTYPE = {1: torch.uint8, 2: torch.uint16, 4: torch.uint32, 8: torch.uint64}
@@ -147,7 +147,10 @@ class Context:
                                               C.byref(ng)))
         return out_group, ng.value, info
 
-    def multi_round_sort(self, cols, bitwidths, order=None, groups=None, mode=MODE_BATCHED):
+    def multi_round_sort(self, cols, bitwidths, order=None, groups=None, mode=MODE_BATCHED, want_count=True):
+        """ORDER BY cols (one round per column).  With want_count False the
+        group count is not returned (None), which lets the device-driven
+        modes run without a host synchronisation."""
         n = cols[0].numel()
         if order is None:
             order = torch.empty(n, dtype=torch.uint32, device=cols[0].device)
@@ -157,8 +160,8 @@ class Context:
         bw = (C.c_uint32 * r)(*bitwidths)
         ng = C.c_uint32(0)
         self._chk(self.lib.fmcs_multi_round_sort(self.h, r, cp, bk, bw, n, _p(order), _p(groups), mode,
-                                                 C.byref(ng)))
-        return order, ng.value
+                                                 C.byref(ng) if want_count else None))
+        return order, (ng.value if want_count else None)
 
     def set_profiling(self, on):
         self._chk(self.lib.fmcs_ctx_set_profiling(self.h, int(bool(on))))
@@ -204,4 +207,4 @@ class Context:
 
 
 __all__ = ["Context", "bank_bytes", "construct_entries", "bank_of", "dtype_of_bank",
-           "MODE_BATCHED", "MODE_PLAIN", "MODE_UNBINNED"]
+           "MODE_BATCHED", "MODE_PLAIN", "MODE_UNBINNED", "MODE_DEVICE", "MODE_GRAPH"]

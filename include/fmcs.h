@@ -34,6 +34,14 @@ extern "C" {
 #define FMCS_MODE_BATCHED 0  /* size-binned: packed LDS tiles + segmented MSD radix */
 #define FMCS_MODE_PLAIN 1    /* one sort dispatch per group (src/composer.cpp:1285-1323) */
 #define FMCS_MODE_UNBINNED 2 /* one LSD radix sort of (group id, key) composites */
+/* The batched form, device-driven: counts stay on the GPU and every kernel
+ * reads its sizes there, so a call issues no host synchronisation (unless it
+ * returns a count).  Same results as FMCS_MODE_BATCHED. */
+#define FMCS_MODE_DEVICE 3
+/* FMCS_MODE_DEVICE captured in a hipGraph: fmcs_multi_round_sort replays the
+ * context's cached graph when called again with the same columns, widths,
+ * n, output pointers and tuning, else runs the call and captures it. */
+#define FMCS_MODE_GRAPH 4
 
 typedef struct fmcs_ctx fmcs_ctx;
 

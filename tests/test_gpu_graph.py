@@ -1,0 +1,131 @@
+"""Device-driven and graph-captured multi-round sort: parity with the
+oracle and with the eager batched path, replay after replay.
+
+The graph is replayed several times with new column contents written into
+the same device buffers, with eager work and host synchronisations between
+the replays; every replay is checked.  Bar: bit-exact.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle_bind as ob
+from fastmulticolsort_amd import datagen
+from fastmulticolsort_amd._lib import FmcsError
+from fastmulticolsort_amd.engine import MODE_BATCHED, MODE_DEVICE, MODE_GRAPH
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def lex(cols):
+    return np.lexsort(tuple(reversed(cols))).astype(np.uint32)
+
+
+def test_graph_replays_match_eager_and_oracle(ctx):
+    n = 1 << 18
+    (c1, c2), bws = datagen.config3(n, ngroups=1 << 10)
+    d1, d2 = dev(c1), dev(c2)
+    order = torch.empty(n, dtype=torch.uint32, device="cuda")
+    groups = torch.empty(n, dtype=torch.uint32, device="cuda")
+    eager = torch.empty(n, dtype=torch.uint32, device="cuda")
+    for rep in range(5):
+        # new contents in the same buffers: a replay must sort what is there now
+        k1 = datagen.zipf_column(100 + rep, n, 1 << 10, 1.1,
+                                 datagen.permutation(7 + rep, 1 << 20)[: 1 << 10].astype(np.uint32)).astype(np.uint32)
+        k2 = datagen.uniform_bits(200 + rep, n, 40).astype(np.uint64)
+        if rep == 3:
+            k2[::3] = k2[0]  # many equal keys: ties broken by oid
+        d1.copy_(dev(k1))
+        d2.copy_(dev(k2))
+        log("graph replay", rep, n)
+        _, ng = ctx.multi_round_sort([d1, d2], bws, order=order, groups=groups, mode=MODE_GRAPH,
+                                     want_count=(rep % 2 == 0))
+        torch.cuda.synchronize()
+        ctx.multi_round_sort([d1, d2], bws, order=eager, mode=MODE_BATCHED)
+        torch.cuda.synchronize()
+        exp = lex([k1, k2])
+        got = host(order)
+        assert np.array_equal(got, exp), f"replay {rep}"
+        assert np.array_equal(got, host(eager))
+        eo, eg, eng, _ = ob.multi_round_sort([k1, k2])
+        assert np.array_equal(got, eo)
+        # dense group ids of the last round over (c1, c2)
+        s1, s2 = k1[got], k2[got]
+        head = np.r_[True, (s1[1:] != s1[:-1]) | (s2[1:] != s2[:-1])]
+        assert np.array_equal(host(groups), (np.cumsum(head) - 1).astype(np.uint32))
+        if rep % 2 == 0:
+            assert ng == int(head.sum())
+
+
+def test_graph_recaptures_on_tuning_and_shape_change(ctx):
+    old = ctx.tuning()
+    try:
+        for (n, sm, w, mb) in [(1 << 16, 512, 512, 8), (1 << 16, 256, 128, 6), (50001, 256, 128, 6),
+                               (50001, 512, 512, 8)]:
+            (c1, c2), bws = datagen.config3(n, ngroups=1 << 8)
+            ctx.set_tuning(sm, w, mb)
+            d1, d2 = dev(c1), dev(c2)
+            order = torch.empty(n, dtype=torch.uint32, device="cuda")
+            log("graph recapture", n, sm, w, mb)
+            for rep in range(3):
+                ctx.multi_round_sort([d1, d2], bws, order=order, mode=MODE_GRAPH, want_count=False)
+                torch.cuda.synchronize()
+                assert np.array_equal(host(order), lex([c1, c2])), (n, sm, rep)
+    finally:
+        ctx.set_tuning(old["small_max"], old["window"], old["msd_bits"])
+
+
+@pytest.mark.parametrize("mode", [MODE_DEVICE, MODE_GRAPH])
+def test_device_modes_four_rounds_config4_shape(ctx, mode):
+    n = 1 << 17
+    cols, bws = datagen.config4(n)
+    log("device modes config4-shape", mode, n)
+    dc = [dev(c) for c in cols]
+    order = torch.empty(n, dtype=torch.uint32, device="cuda")
+    for rep in range(2):
+        _, ng = ctx.multi_round_sort(dc, bws, order=order, mode=mode)
+        eo, _, eng, _ = ob.multi_round_sort(cols)
+        assert np.array_equal(host(order), eo)
+        assert np.array_equal(host(order), lex(cols))
+        assert ng == eng
+
+
+@pytest.mark.parametrize("mode", [MODE_DEVICE, MODE_GRAPH])
+def test_device_modes_wide_keys_and_single_rows(ctx, mode):
+    # 60-bit second column: levels whose key bits leave no room for a packed
+    # segment index (single rows only), many MSD levels; plus tiny inputs
+    for n in (1, 2, 3, 4097, 70001):
+        c1 = datagen.uniform_below(5, n, 3).astype(np.uint32)
+        c2 = datagen.uniform_bits(6, n, 60).astype(np.uint64)
+        c2[::5] = c2[0]
+        order = torch.empty(n, dtype=torch.uint32, device="cuda")
+        log("device modes wide", mode, n)
+        _, ng = ctx.multi_round_sort([dev(c1), dev(c2)], [2, 60], order=order, mode=mode)
+        assert np.array_equal(host(order), lex([c1, c2])), n
+        eo, _, eng, _ = ob.multi_round_sort([c1, c2])
+        assert ng == eng
+
+
+def test_device_mode_reports_keys_wider_than_declared(ctx):
+    n = 10000
+    c1 = datagen.uniform_bits(8, n, 24).astype(np.uint32)  # declared as 20 bits below
+    c2 = datagen.uniform_bits(9, n, 40).astype(np.uint64)
+    order = torch.empty(n, dtype=torch.uint32, device="cuda")
+    with pytest.raises(FmcsError):
+        ctx.multi_round_sort([dev(c1), dev(c2)], [20, 40], order=order, mode=MODE_DEVICE)
+    # a later good call is unaffected
+    c1 &= (1 << 20) - 1
+    ctx.multi_round_sort([dev(c1), dev(c2)], [20, 40], order=order, mode=MODE_DEVICE)
+    assert np.array_equal(host(order), lex([c1, c2]))

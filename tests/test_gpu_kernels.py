@@ -8,7 +8,7 @@ import torch
 
 import oracle_bind as ob
 from fastmulticolsort_amd import datagen
-from fastmulticolsort_amd.engine import (MODE_BATCHED, MODE_PLAIN, MODE_UNBINNED, construct_entries)
+from fastmulticolsort_amd.engine import (MODE_BATCHED, MODE_DEVICE, MODE_PLAIN, MODE_UNBINNED, construct_entries)
 
 pytestmark = pytest.mark.gpu
 
@@ -155,7 +155,7 @@ def _segments(n, seed, kind):
     return off
 
 
-@pytest.mark.parametrize("mode", [MODE_BATCHED, MODE_UNBINNED, MODE_PLAIN])
+@pytest.mark.parametrize("mode", [MODE_BATCHED, MODE_DEVICE, MODE_UNBINNED, MODE_PLAIN])
 @pytest.mark.parametrize("dt,bits", [(np.uint64, 40), (np.uint32, 20), (np.uint16, 9), (np.uint64, 60)])
 @pytest.mark.parametrize("kind", ["zipf", "tiny", "one", "mixed", "alt"])
 def test_segmented_sort(ctx, mode, dt, bits, kind):
@@ -183,7 +183,7 @@ def test_segmented_sort(ctx, mode, dt, bits, kind):
 
 # ---- sort_by_group / multi-round ----------------------------------------------
 
-@pytest.mark.parametrize("mode", [MODE_BATCHED, MODE_UNBINNED, MODE_PLAIN])
+@pytest.mark.parametrize("mode", [MODE_BATCHED, MODE_DEVICE, MODE_UNBINNED, MODE_PLAIN])
 def test_multi_round_sort_config1_shape(ctx, mode):
     n = 1 << 16 if mode != MODE_PLAIN else 1 << 14
     (c1, c2), bws = datagen.config1(n)
