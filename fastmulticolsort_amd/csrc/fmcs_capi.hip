@@ -134,6 +134,33 @@ int launch_check(fmcs_ctx* ctx, const char* what) {
   return FMCS_OK;
 }
 
+// Blocks of `fn` resident on the whole device at 256 threads (cached).
+u32 resident(fmcs_ctx* ctx, const void* fn) {
+  auto it = ctx->resident.find(fn);
+  if (it != ctx->resident.end()) return it->second;
+  if (ctx->ncu == 0) {
+    hipDeviceProp_t prop;
+    ctx->ncu = (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
+                   ? (u32)prop.multiProcessorCount
+                   : 256u;
+  }
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, 0) != hipSuccess || nb <= 0) {
+    (void)hipGetLastError();
+    nb = 1;
+  }
+  u32 r = (u32)nb * ctx->ncu;
+  ctx->resident[fn] = r;
+  return r;
+}
+
+u64 cap_grid(u64 want, u64 cap) { return std::max<u64>(1, std::min<u64>(want, cap)); }
+
+// Grid of a persistent kernel (blocks loop over work items taken from an
+// atomic counter): twice the resident blocks, so a block that finishes early
+// is replaced at once and the CUs stay full; extra blocks find no work.
+u64 persist_grid(fmcs_ctx* ctx, const void* fn, u64 want) { return cap_grid(want, 2ull * resident(ctx, fn)); }
+
 int fill(fmcs_ctx* ctx, u32* p, u64 n, u32 v) {
   if (n == 0) return FMCS_OK;
   u32 blocks = (u32)std::min<u64>((n + 255) / 256, 4096);
@@ -313,7 +340,9 @@ int radix_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, 
       dv = (i & 1) ? t1v : t0v;
     }
     Prof pf(ctx, "k_onesweep", 2.0 * n * (sizeof(K) + 4));
-    hipLaunchKernelGGL((k_onesweep<K, ONESWEEP_ITEMS>), dim3((u32)ntiles), dim3(256), 0, ctx->stream, sk, sv, dk, dv,
+    hipLaunchKernelGGL((k_onesweep<K, ONESWEEP_ITEMS>),
+                       dim3((u32)persist_grid(ctx, (const void*)k_onesweep<K, ONESWEEP_ITEMS>, ntiles)), dim3(256), 0,
+                       ctx->stream, sk, sv, dk, dv,
                        ctx->small_d + 16, ctx->small_d + 17, ctx->small_d + 18, 1u, dbase + (u64)p * RADIX,
                        status + (u64)p * ntiles * RADIX, ctx->small_d + 24 + p, p * RADIX_BITS, (u32)RADIX_BITS,
                        (const u32*)nullptr, (const u32*)nullptr);
@@ -512,7 +541,9 @@ int seg_sort_batched(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, cons
     TRY(fill(ctx, ctx->small_d + 50, 1, 0));
     {
       Prof pf(ctx, "k_onesweep_msd", 2.0 * large_rows * (sizeof(K) + 4));
-      hipLaunchKernelGGL((k_onesweep<K, ONESWEEP_ITEMS>), dim3(ntiles), dim3(256), 0, ctx->stream, skey, sval, dkey,
+      hipLaunchKernelGGL((k_onesweep<K, ONESWEEP_ITEMS>),
+                         dim3((u32)persist_grid(ctx, (const void*)k_onesweep<K, ONESWEEP_ITEMS>, ntiles)), dim3(256), 0,
+                         ctx->stream, skey, sval, dkey,
                          dval, llo, lhi, tbase, nlarge, dbase, status, ctx->small_d + 50, shift, D,
                          (const u32*)nullptr, (const u32*)(ctx->small_d + 49));
       TRY(launch_check(ctx, "k_onesweep(msd)"));
@@ -672,26 +703,6 @@ constexpr u32 DD_CONST = 76;
 constexpr u64 DD_WORDS = DD_HDR + (u64)DD_ROUNDS_MAX * DD_LVL_MAX * DD_LVL_WORDS;
 enum { LV_NSEG, LV_WCOUNT, LV_ROWS_SMALL, LV_ROWS_LARGE, LV_NLARGE, LV_NTILES, LV_SCAN_A, LV_SCAN_B, LV_OS, LV_FB };
 
-// Blocks of `fn` resident on the whole device at 256 threads (cached).
-u32 resident(fmcs_ctx* ctx, const void* fn) {
-  auto it = ctx->resident.find(fn);
-  if (it != ctx->resident.end()) return it->second;
-  if (ctx->ncu == 0) {
-    hipDeviceProp_t prop;
-    ctx->ncu = (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
-                   ? (u32)prop.multiProcessorCount
-                   : 256u;
-  }
-  int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, 0) != hipSuccess || nb <= 0) {
-    (void)hipGetLastError();
-    nb = 1;
-  }
-  u32 r = (u32)nb * ctx->ncu;
-  ctx->resident[fn] = r;
-  return r;
-}
-
 template <typename... KA, size_t... I>
 hipError_t launch_tuple(const void* fn, u32 grid, hipStream_t st, std::tuple<KA...>& t, std::index_sequence<I...>) {
   void* argv[sizeof...(KA) + 1] = {(void*)&std::get<I>(t)...};
@@ -710,8 +721,6 @@ int klaunch(fmcs_ctx* ctx, const char* name, void (*kern)(KA...), u64 grid, doub
   if (e != hipSuccess) return fail(ctx, FMCS_E_HIP, std::string(name) + ": " + hipGetErrorString(e));
   return launch_check(ctx, name);
 }
-
-u64 cap_grid(u64 want, u64 cap) { return std::max<u64>(1, std::min<u64>(want, cap)); }
 
 // p[0 .. ceil(*cnt * mult / div) + add) = v, bounded by max_words
 int dd_fill(fmcs_ctx* ctx, u32* p, const u32* cnt, u32 mult, u32 div, u32 add, u32 v, u64 max_words) {
@@ -766,7 +775,7 @@ int radix_sort_dd(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vou
   // one segment [0, n) of ntiles tiles: seg_lo, seg_hi, tile_base[0], tile_base[1]
   u32* cst = dd + DD_CONST;
   TRY(klaunch(ctx, "k_set", kd_set4, 1, 0.0, nullptr, 0.0, cst, 0u, (u32)n, 0u, (u32)ntiles));
-  const u64 grid = cap_grid(ntiles, resident(ctx, (const void*)k_onesweep<K, ONESWEEP_ITEMS>));
+  const u64 grid = persist_grid(ctx, (const void*)k_onesweep<K, ONESWEEP_ITEMS>, ntiles);
   const K* sk = kin;
   const u32* sv = vin;
   for (u32 p = 0; p < npasses; ++p) {
@@ -919,7 +928,7 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
     a.src_sel = (u32)(l & 1); a.kb = d.tkb; a.pad_seg = T; a.direct = 0; a.wlist = wlist;
     a.wcount = S + LV_WCOUNT; a.fblist = fblist; a.fbcount = S + LV_FB;
     {
-      const u64 g = cap_grid(d.nw_max, resident(ctx, fn_bucket));
+      const u64 g = persist_grid(ctx, fn_bucket, d.nw_max);
       const double per = 2.0 * (sizeof(K) + 4);
       if (T <= 512)
         TRY(klaunch(ctx, "k_tile_bucket", k_tile_bucket<K, 2>, g, 0.0, S + LV_ROWS_SMALL, per, a, fb));
@@ -929,7 +938,7 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
     {
       TileSortArgs<K> b = a;
       b.wcount = nullptr;
-      const u64 g = cap_grid(d.nw_max, resident(ctx, fn_tsort));
+      const u64 g = persist_grid(ctx, fn_tsort, d.nw_max);
       if (T <= 512)
         TRY(klaunch(ctx, "k_tile_sort", k_tile_sort<K, 2>, g, 0.0, nullptr, 0.0, b));
       else
@@ -960,7 +969,7 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
                 (const u32*)hist, (const u32*)llo, dbase, lst[nxt], lsz[nxt], (const u32*)(S + LV_NLARGE)));
     TRY(dd_fill(ctx, status, S + LV_NTILES, RADIX, 1, 0, 0, d.ntiles_max * RADIX));
     TRY(klaunch(ctx, "k_onesweep_msd", k_onesweep<K, ONESWEEP_ITEMS>,
-                cap_grid(d.ntiles_max, resident(ctx, (const void*)k_onesweep<K, ONESWEEP_ITEMS>)), 0.0,
+                persist_grid(ctx, (const void*)k_onesweep<K, ONESWEEP_ITEMS>, d.ntiles_max), 0.0,
                 S + LV_ROWS_LARGE, 2.0 * (sizeof(K) + 4), skey, sval, dkey, dval, (const u32*)llo, (const u32*)lhi,
                 (const u32*)tbase, 0u, (const u32*)dbase, status, S + LV_OS, d.shift, d.D,
                 (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES)));
@@ -1028,6 +1037,8 @@ int mrs_dd(fmcs_ctx* ctx, u32 nrounds, const void* const* cols, const u32* banks
     const bool last = (r + 1 == nrounds);
     const bool want_groups = !last || groups_d;
     u32* gout = last && groups_d ? groups_d : ((gin == ga) ? gb : ga);
+    // dense ids of this round feed only the next round's boundary
+    if (!last && !(r + 2 < nrounds || groups_d)) gout = nullptr;
     u32* oout = (soff == offa) ? offb : offa;
     int rc;
     switch (banks[r]) {

@@ -151,17 +151,27 @@ __global__ __launch_bounds__(256) void k_boundary(const K* __restrict__ v, const
   u64 base = (u64)tile * TILE;
   u64 masks[ITEMS];
   u64 lt = lanemask_lt();
+  // one load per row: the predecessor comes from the next lane down; lane 0
+  // loads its own predecessor
+  K cur[ITEMS];
+  u32 gcur[ITEMS];
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
-    u64 r = base + (u64)j * NT + tid;
-    bool head = false;
-    if (r < n) {
-      head = (r == 0);
-      if (!head) {
-        head = v[r] != v[r - 1];
-        if (in_group) head = head || (in_group[r] != in_group[r - 1]);
-      }
+    const u64 r = base + (u64)j * NT + tid;
+    cur[j] = r < n ? v[r] : (K)0;
+    gcur[j] = (in_group && r < n) ? in_group[r] : 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    const u64 r = base + (u64)j * NT + tid;
+    u64 pv = (u64)__shfl_up((unsigned long long)cur[j], 1);
+    u32 pg = __shfl_up(gcur[j], 1);
+    if (lane == 0 && r > 0 && r < n) {
+      pv = (u64)v[r - 1];
+      if (in_group) pg = in_group[r - 1];
     }
+    bool head = false;
+    if (r < n) head = (r == 0) || ((u64)cur[j] != pv) || (in_group && gcur[j] != pg);
     u64 m = __ballot(head);
     masks[j] = m;
     if (lane == 0) s_cnt[j][wv] = (u32)__popcll(m);
@@ -618,16 +628,14 @@ constexpr u32 BUCKET_LIMIT = 24;
 template <class K, int E>
 __device__ __forceinline__ void tile_bucket_one(const TileSortArgs<K>& a, u32 w, u32* __restrict__ fallback) {
   constexpr int NT = 256, T = NT * E, NB = 2 * T, BPT = NB / NT, SE = T / NT;
-  __shared__ u64 s_k[T];
+  __shared__ K s_k[T];          // rows in bin order, then in final order
   __shared__ u32 s_o[T];
-  __shared__ u64 s_k2[T];
-  __shared__ u32 s_o2[T];
-  __shared__ u32 s_cnt[NB];
-  __shared__ u32 s_off[NB];
+  __shared__ u32 s_cnt[NB];     // bin counts -> bin cursors -> bin ends
+  __shared__ u16 s_sid[T];      // local segment of every row position
   __shared__ u16 s_segstart[T + 1];
-  __shared__ u16 s_segbase[T];
-  __shared__ uint8_t s_lb[T];
-  __shared__ uint8_t s_f[T];
+  __shared__ u16 s_segbase[T];  // first bin of every segment
+  __shared__ uint8_t s_lb[T];   // log2 bins of every segment
+  __shared__ uint8_t s_f[T];    // segment-start flags
   __shared__ u32 s_tmp[NT / WAVE + 1];
   __shared__ u32 s_over;
   const u32 tid = threadIdx.x;
@@ -639,63 +647,71 @@ __device__ __forceinline__ void tile_bucket_one(const TileSortArgs<K>& a, u32 w,
   const u64 kmask = kb >= 64 ? ~0ull : ((1ull << kb) - 1);
   const K* kin = a.kin[a.src_sel];
   const u32* vin = a.vin[a.src_sel];
+  // rows straight into registers, row p = e * NT + tid (coalesced)
+  K kk[E];
+  u32 oo[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const u32 p = e * NT + tid;
+    if (p < cnt) {
+      kk[e] = kin[lo + p];
+      oo[e] = vin[lo + p];
+    }
+  }
   for (u32 i = tid; i < (u32)T; i += NT) s_f[i] = 0;
   for (u32 i = tid; i < (u32)NB; i += NT) s_cnt[i] = 0;
   if (tid == 0) s_over = 0;
   __syncthreads();
-  // both global read phases (rows, segment starts) in flight together
-  for (u32 i = tid; i < cnt; i += NT) {
-    s_k[i] = (u64)kin[lo + i];
-    s_o[i] = vin[lo + i];
-  }
   for (u32 s = s0 + tid; s <= s1; s += NT) {
-    u32 st = a.seg_start[s];
+    const u32 st = a.seg_start[s];
     if (st >= lo && st < hi) s_f[st - lo] = 1;
   }
   if (tid == 0) s_f[0] = 1;
   __syncthreads();
-  // local segment of every row (blocked), segment starts
-  u32 f[E], sg[E];
-  u32 sum = 0;
+  // local segment of every position (blocked scan of the flags)
+  {
+    u32 f[E];
+    u32 sum = 0;
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    f[e] = s_f[tid * E + e];
-    sum += f[e];
-  }
-  u32 nsegs;
-  u32 run = block_excl_scan<NT>(sum, s_tmp, &nsegs);
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    run += f[e];
-    sg[e] = run - 1;
-    u32 p = tid * E + e;
-    if (f[e] && p < cnt) s_segstart[sg[e]] = (u16)p;
-  }
-  if (tid == 0) s_segstart[nsegs] = (u16)cnt;
-  __syncthreads();
-  // bins per segment: 2^lb >= m, laid out in segment order
-  u32 bsum = 0;
-  u32 lbs[SE];
-#pragma unroll
-  for (int e = 0; e < SE; ++e) {
-    u32 s = tid * SE + e;
-    u32 lb = 0;
-    if (s < nsegs) {
-      u32 m = (u32)s_segstart[s + 1] - (u32)s_segstart[s];
-      lb = m <= 1 ? 0u : 32u - __clz(m - 1);
-      bsum += 1u << lb;
+    for (int e = 0; e < E; ++e) {
+      f[e] = s_f[tid * E + e];
+      sum += f[e];
     }
-    lbs[e] = lb;
-  }
-  u32 btot;
-  u32 brun = block_excl_scan<NT>(bsum, s_tmp, &btot);
+    u32 nsegs;
+    u32 run = block_excl_scan<NT>(sum, s_tmp, &nsegs);
 #pragma unroll
-  for (int e = 0; e < SE; ++e) {
-    u32 s = tid * SE + e;
-    if (s < nsegs) {
-      s_lb[s] = (uint8_t)lbs[e];
-      s_segbase[s] = (u16)brun;
-      brun += 1u << lbs[e];
+    for (int e = 0; e < E; ++e) {
+      run += f[e];
+      const u32 p = tid * E + e;
+      s_sid[p] = (u16)(run - 1);
+      if (f[e] && p < cnt) s_segstart[run - 1] = (u16)p;
+    }
+    if (tid == 0) s_segstart[nsegs] = (u16)cnt;
+    __syncthreads();
+    // bins per segment: 2^lb >= m, laid out in segment order
+    u32 bsum = 0;
+    u32 lbs[SE];
+#pragma unroll
+    for (int e = 0; e < SE; ++e) {
+      const u32 sg = tid * SE + e;
+      u32 lb = 0;
+      if (sg < nsegs) {
+        const u32 m = (u32)s_segstart[sg + 1] - (u32)s_segstart[sg];
+        lb = m <= 1 ? 0u : 32u - __clz(m - 1);
+        bsum += 1u << lb;
+      }
+      lbs[e] = lb;
+    }
+    u32 btot;
+    u32 brun = block_excl_scan<NT>(bsum, s_tmp, &btot);
+#pragma unroll
+    for (int e = 0; e < SE; ++e) {
+      const u32 sg = tid * SE + e;
+      if (sg < nsegs) {
+        s_lb[sg] = (uint8_t)lbs[e];
+        s_segbase[sg] = (u16)brun;
+        brun += 1u << lbs[e];
+      }
     }
   }
   __syncthreads();
@@ -703,76 +719,85 @@ __device__ __forceinline__ void tile_bucket_one(const TileSortArgs<K>& a, u32 w,
   u32 bin[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    u32 p = tid * E + e;
+    const u32 p = e * NT + tid;
     bin[e] = 0;
     if (p < cnt) {
-      u64 km = s_k[p] & kmask;
-      u32 lb = s_lb[sg[e]];
+      const u32 sg = s_sid[p];
+      const u64 km = (u64)kk[e] & kmask;
+      const u32 lb = s_lb[sg];
       u32 off = 0;
       if (lb) off = kb >= lb ? (u32)(km >> (kb - lb)) : (u32)(km << (lb - kb));
-      bin[e] = (u32)s_segbase[sg[e]] + off;
+      bin[e] = (u32)s_segbase[sg] + off;
       atomicAdd(&s_cnt[bin[e]], 1u);
     }
   }
   __syncthreads();
-  // exclusive scan of the counts (BPT consecutive bins per thread)
-  u32 c[BPT];
-  u32 csum = 0, cmax = 0;
+  // exclusive scan of the counts (BPT consecutive bins per thread) -> cursors
+  {
+    u32 c[BPT];
+    u32 csum = 0, cmax = 0;
 #pragma unroll
-  for (int j = 0; j < BPT; ++j) {
-    c[j] = s_cnt[tid * BPT + j];
-    csum += c[j];
-    cmax = max(cmax, c[j]);
-  }
-  if (cmax > BUCKET_LIMIT) s_over = 1;
-  u32 tot;
-  u32 crun = block_excl_scan<NT>(csum, s_tmp, &tot);
-  if (s_over) {
-    if (tid == 0) {
-      fallback[w] = 1;
-      if (a.fblist) a.fblist[atomicAdd(a.fbcount, 1u)] = w;
+    for (int j = 0; j < BPT; ++j) {
+      c[j] = s_cnt[tid * BPT + j];
+      csum += c[j];
+      cmax = max(cmax, c[j]);
     }
-    return;
-  }
+    if (cmax > BUCKET_LIMIT) s_over = 1;
+    u32 tot;
+    u32 crun = block_excl_scan<NT>(csum, s_tmp, &tot);
+    if (s_over) {
+      if (tid == 0) {
+        fallback[w] = 1;
+        if (a.fblist) a.fblist[atomicAdd(a.fbcount, 1u)] = w;
+      }
+      return;
+    }
 #pragma unroll
-  for (int j = 0; j < BPT; ++j) {
-    s_off[tid * BPT + j] = crun;
-    crun += c[j];
-  }
-  __syncthreads();
-  // scatter rows into their bins (order inside a bin is arbitrary)
-  u64 kk[E];
-  u32 oo[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    u32 p = tid * E + e;
-    if (p < cnt) {
-      kk[e] = s_k[p];
-      oo[e] = s_o[p];
-      u32 slot = atomicAdd(&s_off[bin[e]], 1u);
-      s_k2[slot] = kk[e];
-      s_o2[slot] = oo[e];
+    for (int j = 0; j < BPT; ++j) {
+      s_cnt[tid * BPT + j] = crun;
+      crun += c[j];
     }
   }
   __syncthreads();
-  // rank inside the bin: [end - count, end) after the scatter
+  // scatter rows into their bins (order inside a bin is arbitrary); the
+  // cursors end as the bin ends, and bin b starts where bin b - 1 ends
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    u32 p = tid * E + e;
+    const u32 p = e * NT + tid;
     if (p < cnt) {
-      u32 end = s_off[bin[e]], start = end - s_cnt[bin[e]];
+      const u32 slot = atomicAdd(&s_cnt[bin[e]], 1u);
+      s_k[slot] = kk[e];
+      s_o[slot] = oo[e];
+    }
+  }
+  __syncthreads();
+  // rank inside the bin by (key, oid)
+  u32 rk[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const u32 p = e * NT + tid;
+    rk[e] = 0;
+    if (p < cnt) {
+      const u32 end = s_cnt[bin[e]], start = bin[e] ? s_cnt[bin[e] - 1] : 0u;
       u32 r = start;
       for (u32 q = start; q < end; ++q) {
-        u64 kq = s_k2[q];
-        r += (kq < kk[e]) || (kq == kk[e] && s_o2[q] < oo[e]);
+        const K kq = s_k[q];
+        r += (kq < kk[e]) || (kq == kk[e] && s_o[q] < oo[e]);
       }
-      s_k[r] = kk[e];
-      s_o[r] = oo[e];
+      rk[e] = r;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    if (e * NT + tid < cnt) {
+      s_k[rk[e]] = kk[e];
+      s_o[rk[e]] = oo[e];
     }
   }
   __syncthreads();
   for (u32 i = tid; i < cnt; i += NT) {
-    if (a.kout) a.kout[lo + i] = (K)s_k[i];
+    if (a.kout) a.kout[lo + i] = s_k[i];
     a.vout[lo + i] = s_o[i];
   }
 }
