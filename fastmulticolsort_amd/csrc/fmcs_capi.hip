@@ -24,7 +24,7 @@ enum Slot {
   WS_KA, WS_KB, WS_VB, WS_T0K, WS_T0V, WS_T1K, WS_T1V, WS_STATUS, WS_HIST, WS_DBASE, WS_SEGOFF, WS_SEGOFF2,
   WS_ST, WS_SZ, WS_BST, WS_BSZ, WS_FLAG, WS_POS, WS_LLO, WS_LHI, WS_LTILES, WS_TBASE, WS_TLO, WS_THI, WS_TS0,
   WS_TS1, WS_COMP, WS_COMP2, WS_GROUPTMP, WS_MISC, WS_HASH, WS_HASH2, WS_HASH3, WS_HT_TABLE, WS_HT_FIRST,
-  WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_DD, WS_SCANST, WS_FBLIST, WS_USER0,
+  WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_DD, WS_SCANST, WS_FBLIST, WS_TCOUNT, WS_CARRY, WS_BCNT, WS_USER0,
   WS_COUNT = WS_USER0 + 24
 };
 
@@ -248,6 +248,44 @@ struct Prof {
   }
 };
 
+template <typename... KA, size_t... I>
+hipError_t launch_tuple(const void* fn, u32 grid, hipStream_t st, std::tuple<KA...>& t, std::index_sequence<I...>) {
+  void* argv[sizeof...(KA) + 1] = {(void*)&std::get<I>(t)...};
+  return hipLaunchKernel(fn, dim3(grid), dim3(256), argv, 0, st);
+}
+
+// Launches kern<<<grid, 256>>> on the context's stream with per-kernel
+// timing when profiling; bytes = static bytes + per_unit * (*dcount).
+template <typename... KA, typename... A>
+int klaunch(fmcs_ctx* ctx, const char* name, void (*kern)(KA...), u64 grid, double bytes, const u32* dcount,
+            double per_unit, A... args) {
+  if (grid == 0) return FMCS_OK;
+  std::tuple<KA...> t(args...);
+  Prof pf(ctx, name, bytes, dcount, per_unit);
+  hipError_t e = launch_tuple((const void*)kern, (u32)grid, ctx->stream, t, std::index_sequence_for<KA...>{});
+  if (e != hipSuccess) return fail(ctx, FMCS_E_HIP, std::string(name) + ": " + hipGetErrorString(e));
+  return launch_check(ctx, name);
+}
+
+// Segmented tile-offset scan of the per-tile digit counts (written by
+// kd_seg_hist into tcount over the same tiling) and the scatter pass.
+template <class K>
+int scatter_pass(fmcs_ctx* ctx, const char* name, const K* kin, const u32* vin, K* kout, u32* vout, const u32* lo,
+                 const u32* hi, const u32* tbase, const u32* nsegs_d, const u32* ntiles_d, u64 ntiles_max,
+                 const u32* dbase, u32* tcount, u32 shift, u32 dbits, double bytes, const u32* dcount, double per) {
+  const u64 nch = std::max<u64>(1, (ntiles_max + TSCAN_CT - 1) / TSCAN_CT);
+  u32* carry;
+  TRY(getbuf_t(ctx, WS_CARRY, nch * (RADIX + 1), &carry));
+  u32* cflag = carry + nch * RADIX;
+  TRY(klaunch(ctx, "k_tile_scan", kd_tscan_local, cap_grid(nch, 2048), 0.0, nullptr, 0.0, tcount, tbase, nsegs_d,
+              ntiles_d, carry, cflag));
+  TRY(klaunch(ctx, "k_tile_scan", kd_tscan_carry, RADIX, 0.0, nullptr, 0.0, carry, (const u32*)cflag, ntiles_d));
+  return klaunch(ctx, name, k_onesweep<K, ONESWEEP_ITEMS>,
+                 persist_grid(ctx, (const void*)k_onesweep<K, ONESWEEP_ITEMS>, ntiles_max), bytes, dcount, per, kin, vin,
+                 kout, vout, lo, hi, tbase, 0u, dbase, (const u32*)tcount, (const u32*)carry, shift, dbits, nsegs_d,
+                 ntiles_d);
+}
+
 // exclusive scan of n u32 values -> out; *total_h (host) if non-null
 int scan_excl(fmcs_ctx* ctx, const u32* in, u32* out, u64 n, u32* total_d) {
   if (n == 0) return FMCS_OK;
@@ -277,11 +315,11 @@ int radix_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, 
     return FMCS_OK;
   }
   u32 last_bits = keybits - (npasses - 1) * RADIX_BITS;
-  u32 *hist, *dbase, *status;
+  u32 *hist, *dbase, *tcount;
   TRY(getbuf_t(ctx, WS_HIST, (u64)npasses * RADIX, &hist));
   TRY(getbuf_t(ctx, WS_DBASE, (u64)npasses * RADIX, &dbase));
   u64 ntiles = (n + ONESWEEP_TILE - 1) / ONESWEEP_TILE;
-  TRY(getbuf_t(ctx, WS_STATUS, (u64)npasses * ntiles * RADIX, &status));
+  TRY(getbuf_t(ctx, WS_TCOUNT, ntiles * RADIX, &tcount));
   TRY(fill(ctx, hist, (u64)npasses * RADIX, 0));
   {
     u32 blocks = (u32)std::min<u64>((n + 255) / 256, 2048);
@@ -320,11 +358,10 @@ int radix_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, 
   TRY(getbuf_t(ctx, WS_T0V, n, &t0v));
   TRY(getbuf_t(ctx, WS_T1K, n, &t1k));
   TRY(getbuf_t(ctx, WS_T1V, n, &t1v));
-  // constants: seg_lo=0, seg_hi=n, tile_base={0, ntiles}
-  u32 consts[4] = {0u, (u32)n, 0u, (u32)ntiles};
-  TRY(upload(ctx, ctx->small_d + 16, consts, 4));
-  TRY(fill(ctx, status, (u64)npasses * ntiles * RADIX, 0));
-  TRY(fill(ctx, ctx->small_d + 24, npasses, 0));
+  // constants: seg_lo=0, seg_hi=n, tile_base={0, ntiles}, nsegs = 1
+  u32 consts[5] = {0u, (u32)n, 0u, (u32)ntiles, 1u};
+  TRY(upload(ctx, ctx->small_d + 16, consts, 5));
+  const u32* cst = ctx->small_d + 16;
   const K* sk = kin;
   const u32* sv = vin;
   for (size_t i = 0; i < passes.size(); ++i) {
@@ -339,20 +376,33 @@ int radix_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, 
       dk = (i & 1) ? t1k : t0k;
       dv = (i & 1) ? t1v : t0v;
     }
-    Prof pf(ctx, "k_onesweep", 2.0 * n * (sizeof(K) + 4));
-    hipLaunchKernelGGL((k_onesweep<K, ONESWEEP_ITEMS>),
-                       dim3((u32)persist_grid(ctx, (const void*)k_onesweep<K, ONESWEEP_ITEMS>, ntiles)), dim3(256), 0,
-                       ctx->stream, sk, sv, dk, dv,
-                       ctx->small_d + 16, ctx->small_d + 17, ctx->small_d + 18, 1u, dbase + (u64)p * RADIX,
-                       status + (u64)p * ntiles * RADIX, ctx->small_d + 24 + p, p * RADIX_BITS, (u32)RADIX_BITS,
-                       (const u32*)nullptr, (const u32*)nullptr);
-    TRY(launch_check(ctx, "k_onesweep"));
+    TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<K, ONESWEEP_ITEMS>, cap_grid(ntiles, 2048), (double)n * sizeof(K),
+                nullptr, 0.0, sk, cst, cst + 1, cst + 2, cst + 4, cst + 3, p * RADIX_BITS, (u32)RADIX_BITS,
+                (u32*)nullptr, tcount));
+    TRY(scatter_pass<K>(ctx, "k_onesweep", sk, sv, dk, dv, cst, cst + 1, cst + 2, cst + 4, cst + 3, ntiles,
+                        dbase + (u64)p * RADIX, tcount, p * RADIX_BITS, (u32)RADIX_BITS, 2.0 * n * (sizeof(K) + 4),
+                        nullptr, 0.0));
     sk = dk;
     sv = dv;
   }
   if ((const void*)sk != (const void*)kout) HIPCHK(hipMemcpyAsync(kout, sk, n * sizeof(K), hipMemcpyDeviceToDevice, ctx->stream));
   if (sv != vout) HIPCHK(hipMemcpyAsync(vout, sv, n * sizeof(u32), hipMemcpyDeviceToDevice, ctx->stream));
   return FMCS_OK;
+}
+
+// Group boundaries: head counts per tile, their exclusive scan, then ids
+// and offsets; the group count goes to *count_d (device).
+template <class K>
+int boundary_launch(fmcs_ctx* ctx, const K* v, const u32* in_group, u64 n, u32* out_group, u32* seg_off,
+                    u32* count_d) {
+  const u64 ntiles = (n + BOUNDARY_TILE - 1) / BOUNDARY_TILE;
+  u32* tc;
+  TRY(getbuf_t(ctx, WS_BCNT, 2 * ntiles, &tc));
+  const double rd = (double)n * (sizeof(K) + (in_group ? 4 : 0));
+  TRY(klaunch(ctx, "k_bcount", k_bcount<K, BOUNDARY_ITEMS>, ntiles, rd, nullptr, 0.0, v, in_group, n, tc));
+  TRY(scan_excl(ctx, tc, tc + ntiles, ntiles, nullptr));
+  return klaunch(ctx, "k_bwrite", k_bwrite<K, BOUNDARY_ITEMS>, ntiles, rd + (out_group ? 4.0 * n : 0.0), nullptr,
+                 0.0, v, in_group, n, (const u32*)(tc + ntiles), out_group, seg_off, count_d);
 }
 
 // boundary: dense ids + optional segment offsets; returns count on host
@@ -364,17 +414,7 @@ int boundary(fmcs_ctx* ctx, const K* v, const u32* in_group, u64 n, u32* out_gro
     return FMCS_OK;
   }
   if (n >= (1ull << 30)) return fail(ctx, FMCS_E_UNSUPPORTED, "boundary supports n < 2^30");
-  u64 ntiles = (n + BOUNDARY_TILE - 1) / BOUNDARY_TILE;
-  u32* status;
-  TRY(getbuf_t(ctx, WS_STATUS, ntiles, &status));
-  TRY(fill(ctx, status, ntiles, 0));
-  TRY(fill(ctx, ctx->small_d + 40, 2, 0));
-  {
-    Prof pf(ctx, "k_boundary", (double)n * (sizeof(K) + (in_group ? 4 : 0) + (out_group ? 4 : 0)));
-    hipLaunchKernelGGL((k_boundary<K, BOUNDARY_ITEMS>), dim3((u32)ntiles), dim3(256), 0, ctx->stream, v, in_group, n,
-                       out_group, seg_off, status, ctx->small_d + 40, ctx->small_d + 41);
-    TRY(launch_check(ctx, "k_boundary"));
-  }
+  TRY(boundary_launch<K>(ctx, v, in_group, n, out_group, seg_off, ctx->small_d + 41));
   return readback(ctx, ctx->small_d + 41, 1, ng);
 }
 
@@ -520,34 +560,24 @@ int seg_sort_batched(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, cons
     }
     const u32 D = std::min<u32>(msd_bits, kb);
     const u32 shift = kb - D;
-    u32 *hist, *dbase, *status;
+    u32 *hist, *dbase, *tcount;
     TRY(getbuf_t(ctx, WS_HIST, (u64)nlarge * RADIX, &hist));
     TRY(getbuf_t(ctx, WS_DBASE, (u64)nlarge * RADIX, &dbase));
-    TRY(getbuf_t(ctx, WS_STATUS, (u64)ntiles * RADIX, &status));
+    TRY(getbuf_t(ctx, WS_TCOUNT, (u64)ntiles * RADIX, &tcount));
     TRY(fill(ctx, hist, (u64)nlarge * RADIX, 0));
-    {
-      Prof pf(ctx, "k_seg_hist", (double)large_rows * sizeof(K));
-      hipLaunchKernelGGL((k_seg_hist<K, ONESWEEP_ITEMS>), dim3(ntiles), dim3(256), 0, ctx->stream, skey, llo, lhi,
-                         tbase, nlarge, shift, D, hist);
-      TRY(launch_check(ctx, "k_seg_hist"));
-    }
+    const u32* nlarge_d = ctx->small_d + 48;
+    const u32* ntiles_d = ctx->small_d + 49;
+    TRY(klaunch(ctx, "k_seg_hist", kd_seg_hist<K, ONESWEEP_ITEMS>, cap_grid(ntiles, 2048),
+                (double)large_rows * sizeof(K), nullptr, 0.0, skey, (const u32*)llo, (const u32*)lhi,
+                (const u32*)tbase, nlarge_d, ntiles_d, shift, D, hist, tcount));
     const int nxt = cur ^ 1;
     u32 *nst, *nsz;
     TRY(getbuf_t(ctx, lst[nxt], (u64)nlarge * RADIX + 1, &nst));
     TRY(getbuf_t(ctx, lsz[nxt], (u64)nlarge * RADIX + 1, &nsz));
     hipLaunchKernelGGL(k_digit_base, dim3(nlarge), dim3(256), 0, ctx->stream, hist, llo, dbase, nst, nsz);
     TRY(launch_check(ctx, "k_digit_base"));
-    TRY(fill(ctx, status, (u64)ntiles * RADIX, 0));
-    TRY(fill(ctx, ctx->small_d + 50, 1, 0));
-    {
-      Prof pf(ctx, "k_onesweep_msd", 2.0 * large_rows * (sizeof(K) + 4));
-      hipLaunchKernelGGL((k_onesweep<K, ONESWEEP_ITEMS>),
-                         dim3((u32)persist_grid(ctx, (const void*)k_onesweep<K, ONESWEEP_ITEMS>, ntiles)), dim3(256), 0,
-                         ctx->stream, skey, sval, dkey,
-                         dval, llo, lhi, tbase, nlarge, dbase, status, ctx->small_d + 50, shift, D,
-                         (const u32*)nullptr, (const u32*)(ctx->small_d + 49));
-      TRY(launch_check(ctx, "k_onesweep(msd)"));
-    }
+    TRY(scatter_pass<K>(ctx, "k_onesweep_msd", skey, sval, dkey, dval, llo, lhi, tbase, nlarge_d, ntiles_d, ntiles,
+                        dbase, tcount, shift, D, 2.0 * large_rows * (sizeof(K) + 4), nullptr, 0.0));
     cur = nxt;
     cst = nst;
     csz = nsz;
@@ -697,30 +727,11 @@ int sort_by_group_t(fmcs_ctx* ctx, u32 keybits, u64 n, const K* column, u32* ord
 constexpr u32 DD_LVL_WORDS = 16;
 constexpr u32 DD_LVL_MAX = 72;
 constexpr u32 DD_ROUNDS_MAX = 64;
-constexpr u32 DD_HDR = 256;  // [0, 64) group count per round, [64, 72) LSD pass counters, [72] key error, [76, 80) consts
+constexpr u32 DD_HDR = 256;  // [0, 64) group count per round, [72] key error, [76, 81) LSD consts, [88] segment count
 constexpr u32 DD_ERR = 72;
 constexpr u32 DD_CONST = 76;
 constexpr u64 DD_WORDS = DD_HDR + (u64)DD_ROUNDS_MAX * DD_LVL_MAX * DD_LVL_WORDS;
 enum { LV_NSEG, LV_WCOUNT, LV_ROWS_SMALL, LV_ROWS_LARGE, LV_NLARGE, LV_NTILES, LV_SCAN_A, LV_SCAN_B, LV_OS, LV_FB };
-
-template <typename... KA, size_t... I>
-hipError_t launch_tuple(const void* fn, u32 grid, hipStream_t st, std::tuple<KA...>& t, std::index_sequence<I...>) {
-  void* argv[sizeof...(KA) + 1] = {(void*)&std::get<I>(t)...};
-  return hipLaunchKernel(fn, dim3(grid), dim3(256), argv, 0, st);
-}
-
-// Launches kern<<<grid, 256>>> on the context's stream with per-kernel
-// timing when profiling; bytes = static bytes + per_unit * (*dcount).
-template <typename... KA, typename... A>
-int klaunch(fmcs_ctx* ctx, const char* name, void (*kern)(KA...), u64 grid, double bytes, const u32* dcount,
-            double per_unit, A... args) {
-  if (grid == 0) return FMCS_OK;
-  std::tuple<KA...> t(args...);
-  Prof pf(ctx, name, bytes, dcount, per_unit);
-  hipError_t e = launch_tuple((const void*)kern, (u32)grid, ctx->stream, t, std::index_sequence_for<KA...>{});
-  if (e != hipSuccess) return fail(ctx, FMCS_E_HIP, std::string(name) + ": " + hipGetErrorString(e));
-  return launch_check(ctx, name);
-}
 
 // p[0 .. ceil(*cnt * mult / div) + add) = v, bounded by max_words
 int dd_fill(fmcs_ctx* ctx, u32* p, const u32* cnt, u32 mult, u32 div, u32 add, u32 v, u64 max_words) {
@@ -752,10 +763,10 @@ int radix_sort_dd(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vou
   }
   const u32 last_bits = keybits - (npasses - 1) * RADIX_BITS;
   const u64 ntiles = (n + ONESWEEP_TILE - 1) / ONESWEEP_TILE;
-  u32 *hist, *dbase, *status;
+  u32 *hist, *dbase, *tcount;
   TRY(getbuf_t(ctx, WS_HIST, (u64)npasses * RADIX, &hist));
   TRY(getbuf_t(ctx, WS_DBASE, (u64)npasses * RADIX, &dbase));
-  TRY(getbuf_t(ctx, WS_STATUS, (u64)npasses * ntiles * RADIX, &status));
+  TRY(getbuf_t(ctx, WS_TCOUNT, ntiles * RADIX, &tcount));
   K *t0k, *t1k;
   u32 *t0v, *t1v;
   TRY(getbuf_t(ctx, WS_T0K, n, &t0k));
@@ -763,19 +774,11 @@ int radix_sort_dd(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vou
   TRY(getbuf_t(ctx, WS_T1K, n, &t1k));
   TRY(getbuf_t(ctx, WS_T1V, n, &t1v));
   TRY(fill(ctx, hist, (u64)npasses * RADIX, 0));
-  TRY(fill(ctx, dd + 64, 8, 0));
-  TRY(fill(ctx, status, (u64)npasses * ntiles * RADIX, 0));
-  TRY(klaunch(ctx, "k_hist_all", k_hist_all<K>, cap_grid((n + 255) / 256, 2048), (double)n * sizeof(K), nullptr, 0.0,
-              kin, n, npasses, (u32)RADIX_BITS, hist));
-  TRY(klaunch(ctx, "k_digit_base", k_digit_base, npasses, 0.0, nullptr, 0.0, (const u32*)hist, (const u32*)nullptr,
-              dbase, (u32*)nullptr, (u32*)nullptr));
-  if (last_bits < (u32)RADIX_BITS)
-    TRY(klaunch(ctx, "k_check_digits", kd_check_digits, 1, 0.0, nullptr, 0.0, (const u32*)hist + (u64)(npasses - 1) * RADIX,
-                last_bits, dd + DD_ERR));
-  // one segment [0, n) of ntiles tiles: seg_lo, seg_hi, tile_base[0], tile_base[1]
+  // one segment [0, n) of ntiles tiles: seg_lo, seg_hi, tile_base[0], tile_base[1] (= ntiles); nsegs = 1
   u32* cst = dd + DD_CONST;
   TRY(klaunch(ctx, "k_set", kd_set4, 1, 0.0, nullptr, 0.0, cst, 0u, (u32)n, 0u, (u32)ntiles));
-  const u64 grid = persist_grid(ctx, (const void*)k_onesweep<K, ONESWEEP_ITEMS>, ntiles);
+  TRY(klaunch(ctx, "k_set", kd_set4, 1, 0.0, nullptr, 0.0, cst + 4, 1u, 0u, 0u, 0u));
+  const u32* one = cst + 4;
   const K* sk = kin;
   const u32* sv = vin;
   for (u32 p = 0; p < npasses; ++p) {
@@ -789,10 +792,18 @@ int radix_sort_dd(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vou
       dk = (p & 1) ? t1k : t0k;
       dv = (p & 1) ? t1v : t0v;
     }
-    TRY(klaunch(ctx, "k_onesweep", k_onesweep<K, ONESWEEP_ITEMS>, grid, 2.0 * n * (sizeof(K) + 4), nullptr, 0.0, sk,
-                sv, dk, dv, (const u32*)cst, (const u32*)cst + 1, (const u32*)cst + 2, 1u,
-                (const u32*)dbase + (u64)p * RADIX, status + (u64)p * ntiles * RADIX, dd + 64 + p, p * RADIX_BITS,
-                (u32)RADIX_BITS, (const u32*)nullptr, (const u32*)nullptr));
+    u32* hp = hist + (u64)p * RADIX;
+    TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<K, ONESWEEP_ITEMS>, cap_grid(ntiles, 2048), (double)n * sizeof(K),
+                nullptr, 0.0, sk, (const u32*)cst, (const u32*)cst + 1, (const u32*)cst + 2, one, (const u32*)cst + 3,
+                p * RADIX_BITS, (u32)RADIX_BITS, hp, tcount));
+    TRY(klaunch(ctx, "k_digit_base", kd_digit_base, 1, 0.0, nullptr, 0.0, (const u32*)hp, (const u32*)cst,
+                dbase + (u64)p * RADIX, (u32*)nullptr, (u32*)nullptr, one));
+    if (last && last_bits < (u32)RADIX_BITS)
+      TRY(klaunch(ctx, "k_check_digits", kd_check_digits, 1, 0.0, nullptr, 0.0, (const u32*)hp, last_bits,
+                  dd + DD_ERR));
+    TRY(scatter_pass<K>(ctx, "k_onesweep", sk, sv, dk, dv, cst, cst + 1, cst + 2, one, cst + 3, ntiles,
+                        dbase + (u64)p * RADIX, tcount, p * RADIX_BITS, (u32)RADIX_BITS, 2.0 * n * (sizeof(K) + 4),
+                        nullptr, 0.0));
     sk = dk;
     sv = dv;
   }
@@ -802,18 +813,10 @@ int radix_sort_dd(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vou
   return FMCS_OK;
 }
 
-// Group boundaries; the group count goes to *count_d (device).
 template <class K>
 int boundary_dd(fmcs_ctx* ctx, const K* v, const u32* in_group, u64 n, u32* out_group, u32* seg_off, u32* count_d) {
   if (n == 0) return FMCS_OK;
-  const u64 ntiles = (n + BOUNDARY_TILE - 1) / BOUNDARY_TILE;
-  u32* status;
-  TRY(getbuf_t(ctx, WS_STATUS, ntiles, &status));
-  TRY(fill(ctx, status, ntiles, 0));
-  TRY(fill(ctx, ctx->small_d + 40, 1, 0));
-  return klaunch(ctx, "k_boundary", k_boundary<K, BOUNDARY_ITEMS>, ntiles,
-                 (double)n * (sizeof(K) + (in_group ? 4 : 0) + (out_group ? 4 : 0)), nullptr, 0.0, v, in_group, n,
-                 out_group, seg_off, status, ctx->small_d + 40, count_d);
+  return boundary_launch<K>(ctx, v, in_group, n, out_group, seg_off, count_d);
 }
 
 struct DdLevel {
@@ -883,7 +886,7 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
   TRY(getbuf_t(ctx, WS_SZ, std::max<u64>(m_list[0], 1), &lsz[0]));
   TRY(getbuf_t(ctx, WS_BST, std::max<u64>(m_list[1], 1), &lst[1]));
   TRY(getbuf_t(ctx, WS_BSZ, std::max<u64>(m_list[1], 1), &lsz[1]));
-  u32 *tlo, *thi, *ts0, *ts1, *wlist, *fb, *fblist, *flag, *pos, *llo, *lhi, *ltiles, *tbase, *hist, *dbase, *status;
+  u32 *tlo, *thi, *ts0, *ts1, *wlist, *fb, *fblist, *flag, *pos, *llo, *lhi, *ltiles, *tbase, *hist, *dbase, *tcount;
   TRY(getbuf_t(ctx, WS_TLO, m_nw, &tlo));
   TRY(getbuf_t(ctx, WS_THI, m_nw, &thi));
   TRY(getbuf_t(ctx, WS_TS0, m_nw, &ts0));
@@ -899,7 +902,7 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
   TRY(getbuf_t(ctx, WS_TBASE, m_large, &tbase));
   TRY(getbuf_t(ctx, WS_HIST, m_large * RADIX, &hist));
   TRY(getbuf_t(ctx, WS_DBASE, m_large * RADIX, &dbase));
-  TRY(getbuf_t(ctx, WS_STATUS, m_tiles * RADIX, &status));
+  TRY(getbuf_t(ctx, WS_TCOUNT, m_tiles * RADIX, &tcount));
   // the tile kernels' grid: resident blocks of the chosen tile shape
   const void* fn_bucket = T <= 512 ? (const void*)k_tile_bucket<K, 2> : (const void*)k_tile_bucket<K, 4>;
   const void* fn_tsort = T <= 512 ? (const void*)k_tile_sort<K, 2> : (const void*)k_tile_sort<K, 4>;
@@ -963,16 +966,14 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
     TRY(dd_fill(ctx, hist, S + LV_NLARGE, RADIX, 1, 0, 0, d.nlarge_max * RADIX));
     TRY(klaunch(ctx, "k_seg_hist", kd_seg_hist<K, ONESWEEP_ITEMS>, cap_grid(d.ntiles_max, 2048), 0.0,
                 S + LV_ROWS_LARGE, (double)sizeof(K), (const K*)skey, (const u32*)llo, (const u32*)lhi,
-                (const u32*)tbase, (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES), d.shift, d.D, hist));
+                (const u32*)tbase, (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES), d.shift, d.D, hist,
+                tcount));
     const int nxt = cur ^ 1;
     TRY(klaunch(ctx, "k_digit_base", kd_digit_base, cap_grid(d.nlarge_max, 2048), 0.0, nullptr, 0.0,
                 (const u32*)hist, (const u32*)llo, dbase, lst[nxt], lsz[nxt], (const u32*)(S + LV_NLARGE)));
-    TRY(dd_fill(ctx, status, S + LV_NTILES, RADIX, 1, 0, 0, d.ntiles_max * RADIX));
-    TRY(klaunch(ctx, "k_onesweep_msd", k_onesweep<K, ONESWEEP_ITEMS>,
-                persist_grid(ctx, (const void*)k_onesweep<K, ONESWEEP_ITEMS>, d.ntiles_max), 0.0,
-                S + LV_ROWS_LARGE, 2.0 * (sizeof(K) + 4), skey, sval, dkey, dval, (const u32*)llo, (const u32*)lhi,
-                (const u32*)tbase, 0u, (const u32*)dbase, status, S + LV_OS, d.shift, d.D,
-                (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES)));
+    TRY(scatter_pass<K>(ctx, "k_onesweep_msd", skey, sval, dkey, dval, llo, lhi, tbase, S + LV_NLARGE,
+                        S + LV_NTILES, d.ntiles_max, dbase, tcount, d.shift, d.D, 0.0, S + LV_ROWS_LARGE,
+                        2.0 * (sizeof(K) + 4)));
     if (l + 1 < L.size())
       TRY(klaunch(ctx, "k_set", kd_mul, 1, 0.0, nullptr, 0.0, (const u32*)(S + LV_NLARGE), (u32)RADIX,
                   S + DD_LVL_WORDS + LV_NSEG));
@@ -990,8 +991,8 @@ int seg_sort_dd_host(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, cons
   u32* dd;
   TRY(getbuf_t(ctx, WS_DD, DD_WORDS, &dd));
   TRY(fill(ctx, dd, DD_HDR + (u64)DD_LVL_MAX * DD_LVL_WORDS, 0));
-  TRY(klaunch(ctx, "k_set", kd_set4, 1, 0.0, nullptr, 0.0, dd + 80, nseg, 0u, 0u, 0u));
-  TRY(seg_sort_dd<K>(ctx, keys, vals, n, keybits, seg_off, dd + 80, nseg, dd + DD_HDR));
+  TRY(klaunch(ctx, "k_set", kd_set4, 1, 0.0, nullptr, 0.0, dd + 88, nseg, 0u, 0u, 0u));
+  TRY(seg_sort_dd<K>(ctx, keys, vals, n, keybits, seg_off, dd + 88, nseg, dd + DD_HDR));
   if (ctx->prof) TRY(prof_flush(ctx));
   return FMCS_OK;
 }

@@ -103,6 +103,25 @@ __device__ __forceinline__ u32 digit_of(K k, u32 shift, u32 mask) {
   return (u32)((u64)k >> shift) & mask;
 }
 
+// Keys of one 16-byte vector.
+template <class K>
+__device__ __forceinline__ void unpack16(const uint4& x, K (&k)[16 / sizeof(K)]) {
+  const u32 w[4] = {x.x, x.y, x.z, x.w};
+  if constexpr (sizeof(K) == 8) {
+    k[0] = (K)w[0] | ((K)w[1] << 32);
+    k[1] = (K)w[2] | ((K)w[3] << 32);
+  } else if constexpr (sizeof(K) == 4) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) k[i] = (K)w[i];
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      k[2 * i] = (K)(w[i] & 0xffffu);
+      k[2 * i + 1] = (K)(w[i] >> 16);
+    }
+  }
+}
+
 // Largest i in [0, n) with a[i] <= x (a sorted ascending, a[0] <= x).
 __device__ __forceinline__ u32 upper_bound_m1(const u32* a, u32 n, u32 x) {
   u32 lo = 0, hi = n;

@@ -193,30 +193,164 @@ __global__ void kd_large_compact(const u32* __restrict__ st, const u32* __restri
   }
 }
 
-// Per-segment digit histogram, tiles never crossing a segment; grid-stride
-// over *ntiles_d tiles of the *nsegs_d large segments.
+// Per-segment digit histogram, tiles never crossing a segment, grid-stride
+// over the *ntiles_d tiles of the *nsegs_d segments.  tile_counts (if set)
+// gets every tile's 256 counts, the input of the tile-offset scan of the
+// scatter pass (same tiling); hist (if set) gets the per-segment totals.
+// A block keeps its digit's sum in a register while its tiles stay in one
+// segment and adds it to hist when the segment changes, so a huge segment
+// (the whole array in an LSD pass) sees one global atomic per block and
+// digit.  Full tiles that start on a 16-byte boundary are read with 16-byte
+// loads (the rate of 4-byte loads is the limit for 16- and 32-bit keys).
 template <class K, int ITEMS>
 __global__ __launch_bounds__(256) void kd_seg_hist(const K* __restrict__ keys, const u32* __restrict__ seg_lo,
                                                    const u32* __restrict__ seg_hi, const u32* __restrict__ tile_base,
                                                    const u32* __restrict__ nsegs_d, const u32* __restrict__ ntiles_d,
-                                                   u32 shift, u32 dbits, u32* __restrict__ hist) {
+                                                   u32 shift, u32 dbits, u32* __restrict__ hist,
+                                                   u32* __restrict__ tile_counts) {
   constexpr int TILE = 256 * ITEMS;
-  __shared__ u32 sh[RADIX];
+  constexpr int PV = 16 / sizeof(K);           // keys per 16-byte vector
+  constexpr int NV = TILE / PV / 256;          // vectors per thread in a full tile
+  // HCOPY sub-histograms, lane l counting into copy l % HCOPY: lanes of one
+  // instruction that share a digit mostly hit different copies, so a skewed
+  // key does not serialize the LDS atomics.  Copies are HSTRIDE words apart
+  // so the copies of one digit sit in different banks.
+  constexpr u32 HCOPY = 8, HSTRIDE = RADIX + 1;
+  __shared__ u32 sh[HCOPY * HSTRIDE];
   __shared__ u32 s_seg;
   const u32 nsegs = *nsegs_d, ntiles = *ntiles_d;
   const u32 mask = (1u << dbits) - 1;
+  u32* myh = sh + (threadIdx.x % HCOPY) * HSTRIDE;
+  u32 acc = 0;
+  u32 cur = 0xffffffffu;
   for (u32 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    sh[threadIdx.x] = 0;
+    for (u32 i = threadIdx.x; i < HCOPY * HSTRIDE; i += 256) sh[i] = 0;
     if (threadIdx.x == 0) s_seg = upper_bound_m1(tile_base, nsegs, tile);
     __syncthreads();
     const u32 seg = s_seg;
+    if (seg != cur) {
+      if (hist && acc) atomicAdd(&hist[(u64)cur * RADIX + threadIdx.x], acc);
+      acc = 0;
+      cur = seg;
+    }
     const u64 row0 = (u64)seg_lo[seg] + (u64)(tile - tile_base[seg]) * TILE;
     const u64 rowend = min((u64)seg_hi[seg], row0 + TILE);
-    for (u64 r = row0 + threadIdx.x; r < rowend; r += 256) atomicAdd(&sh[digit_of(keys[r], shift, mask)], 1u);
+    if (rowend - row0 == (u64)TILE && ((uintptr_t)(keys + row0) & 15) == 0) {
+      const uint4* vp = (const uint4*)(keys + row0);
+      uint4 x[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) x[v] = vp[v * 256 + threadIdx.x];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        K k[PV];
+        unpack16<K>(x[v], k);
+#pragma unroll
+        for (int i = 0; i < PV; ++i) atomicAdd(&myh[digit_of(k[i], shift, mask)], 1u);
+      }
+    } else {
+      K k[ITEMS];
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j) {
+        const u64 r = row0 + (u64)j * 256 + threadIdx.x;
+        k[j] = r < rowend ? keys[r] : (K)0;
+      }
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j)
+        if (row0 + (u64)j * 256 + threadIdx.x < rowend) atomicAdd(&myh[digit_of(k[j], shift, mask)], 1u);
+    }
     __syncthreads();
-    u32 c = sh[threadIdx.x];
-    if (c) atomicAdd(&hist[(u64)seg * RADIX + threadIdx.x], c);
+    u32 c = 0;
+#pragma unroll
+    for (u32 h = 0; h < HCOPY; ++h) c += sh[h * HSTRIDE + threadIdx.x];
+    acc += c;
+    if (tile_counts) tile_counts[(u64)tile * RADIX + threadIdx.x] = c;
     __syncthreads();
+  }
+  if (hist && acc) atomicAdd(&hist[(u64)cur * RADIX + threadIdx.x], acc);
+}
+
+// Per-segment exclusive scan of the tile counts along the tiles, in two
+// steps.  kd_tscan_local: for every chunk of TSCAN_CT tiles, thread d
+// replaces counts[t][d] by the sum of counts[t'][d] over earlier tiles t' of
+// the chunk in the same segment; carry[c][d] = that running sum at the chunk
+// end, cflag[c] = 1 when a segment starts inside chunk c.
+__global__ __launch_bounds__(256) void kd_tscan_local(u32* __restrict__ counts, const u32* __restrict__ tile_base,
+                                                      const u32* __restrict__ nsegs_d,
+                                                      const u32* __restrict__ ntiles_d, u32* __restrict__ carry,
+                                                      u32* __restrict__ cflag) {
+  __shared__ u32 s_first[TSCAN_CT];
+  const u32 nsegs = *nsegs_d, ntiles = *ntiles_d;
+  const u32 nch = (ntiles + TSCAN_CT - 1) / TSCAN_CT;
+  const u32 d = threadIdx.x;
+  for (u32 c = blockIdx.x; c < nch; c += gridDim.x) {
+    const u32 t0 = c * TSCAN_CT;
+    const u32 m = min(ntiles - t0, TSCAN_CT);
+    if (d < m) s_first[d] = tile_base[upper_bound_m1(tile_base, nsegs, t0 + d)] == t0 + d;
+    __syncthreads();
+    u32 v[TSCAN_CT];
+#pragma unroll
+    for (u32 j = 0; j < TSCAN_CT; ++j) v[j] = j < m ? counts[(u64)(t0 + j) * RADIX + d] : 0u;
+    u32 run = 0, any = 0;
+#pragma unroll
+    for (u32 j = 0; j < TSCAN_CT; ++j) {
+      if (j < m) {
+        if (s_first[j]) {
+          run = 0;
+          any = 1;
+        }
+        counts[(u64)(t0 + j) * RADIX + d] = run;
+        run += v[j];
+      }
+    }
+    carry[(u64)c * RADIX + d] = run;
+    if (d == 0) cflag[c] = any;
+    __syncthreads();
+  }
+}
+
+// kd_tscan_carry: block d (grid = RADIX) turns carry[.][d] into carry_in:
+// the sum over earlier chunks back to (and including the tail of) the chunk
+// where the segment of the chunk's first tile starts; a segmented exclusive
+// scan over the chunks with resets at chunks that hold a segment start.
+__global__ __launch_bounds__(256) void kd_tscan_carry(u32* __restrict__ carry, const u32* __restrict__ cflag,
+                                                      const u32* __restrict__ ntiles_d) {
+  __shared__ u32 s_f[256], s_v[256];
+  const u32 d = blockIdx.x, tid = threadIdx.x;
+  const u32 nch = (*ntiles_d + TSCAN_CT - 1) / TSCAN_CT;
+  const u32 per = (nch + 255) / 256;
+  const u32 c0 = min(nch, tid * per), c1 = min(nch, c0 + per);
+  // this thread's chunks reduced to (has reset, sum since the last reset)
+  u32 f = 0, v = 0;
+  for (u32 c = c0; c < c1; ++c) {
+    const u32 x = carry[(u64)c * RADIX + d];
+    if (cflag[c]) {
+      f = 1;
+      v = x;
+    } else {
+      v += x;
+    }
+  }
+  // inclusive segmented scan over the threads: (a) then (b) is
+  // (a.f | b.f, b.f ? b.v : a.v + b.v)
+  s_f[tid] = f;
+  s_v[tid] = v;
+  __syncthreads();
+  for (u32 o = 1; o < 256; o <<= 1) {
+    const u32 pf = tid >= o ? s_f[tid - o] : 0u, pv = tid >= o ? s_v[tid - o] : 0u;
+    __syncthreads();
+    if (tid >= o && !f) {
+      v += pv;
+      f |= pf;
+    }
+    s_f[tid] = f;
+    s_v[tid] = v;
+    __syncthreads();
+  }
+  u32 acc = tid ? s_v[tid - 1] : 0u;
+  for (u32 c = c0; c < c1; ++c) {
+    const u32 x = carry[(u64)c * RADIX + d];
+    carry[(u64)c * RADIX + d] = acc;
+    acc = cflag[c] ? x : acc + x;
   }
 }
 
@@ -234,8 +368,10 @@ __global__ __launch_bounds__(256) void kd_digit_base(const u32* __restrict__ his
     u32 ex = block_excl_scan<256>(c, tmp, &tot);
     u32 b = lo[s] + ex;
     digit_base[(u64)s * RADIX + d] = b;
-    bstart[(u64)s * RADIX + d] = b;
-    bsize[(u64)s * RADIX + d] = c;
+    if (bstart) {
+      bstart[(u64)s * RADIX + d] = b;
+      bsize[(u64)s * RADIX + d] = c;
+    }
   }
 }
 
@@ -282,8 +418,8 @@ __global__ void kd_copy1(const u32* __restrict__ src, u32* __restrict__ dst) {
 }
 
 #define FMCS_DD_INST(K)                                                                                            \
-  template __global__ void kd_seg_hist<K, 16>(const K*, const u32*, const u32*, const u32*, const u32*,          \
-                                              const u32*, u32, u32, u32*);                                       \
+  template __global__ void kd_seg_hist<K, ONESWEEP_ITEMS>(const K*, const u32*, const u32*, const u32*, const u32*,          \
+                                              const u32*, u32, u32, u32*, u32*);                                       \
   template __global__ void kd_copy_segs<K>(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*,   \
                                            const u32*, const u32*, u32);                            \
   template __global__ void kd_copy_pairs<K>(const K*, const u32*, K*, u32*, u64);

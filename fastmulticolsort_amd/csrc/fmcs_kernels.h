@@ -53,21 +53,22 @@ constexpr int BOUNDARY_TILE = 256 * BOUNDARY_ITEMS;
 constexpr int TILESORT_E = 4;
 constexpr int TILESORT_T = 256 * TILESORT_E;
 constexpr int SCAN_ITEMS = 8;
+constexpr u32 TSCAN_CT = 32;  // tiles per chunk of the per-segment tile-count scan
 constexpr int SCAN_TILE = 256 * SCAN_ITEMS;
 
 __global__ void k_stitch(StitchArgs a);
 template <class T>
 __global__ void k_gather(const T*, const u32*, T*, u64);
 template <class K, int ITEMS>
-__global__ void k_boundary(const K*, const u32*, u64, u32*, u32*, u32*, u32*, u32*);
+__global__ void k_bcount(const K*, const u32*, u64, u32*);
+template <class K, int ITEMS>
+__global__ void k_bwrite(const K*, const u32*, u64, const u32*, u32*, u32*, u32*);
 template <class K>
 __global__ void k_hist_all(const K*, u64, u32, u32, u32*);
-template <class K, int ITEMS>
-__global__ void k_seg_hist(const K*, const u32*, const u32*, const u32*, u32, u32, u32, u32*);
 __global__ void k_digit_base(const u32*, const u32*, u32*, u32*, u32*);
 template <class K, int ITEMS>
 __global__ void k_onesweep(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, u32, const u32*,
-                           u32*, u32*, u32, u32, const u32*, const u32*);
+                           const u32*, const u32*, u32, u32, const u32*, const u32*);
 template <class K, int E>
 __global__ void k_tile_sort(TileSortArgs<K>);
 template <class K, int E>
@@ -109,7 +110,10 @@ template <int ITEMS>
 __global__ void kd_scan_excl(const u32*, u32*, const u32*, u32*, u32*, u32*);
 __global__ void kd_large_compact(const u32*, const u32*, const u32*, u32, const u32*, u32, u32*, u32*, u32*);
 template <class K, int ITEMS>
-__global__ void kd_seg_hist(const K*, const u32*, const u32*, const u32*, const u32*, const u32*, u32, u32, u32*);
+__global__ void kd_seg_hist(const K*, const u32*, const u32*, const u32*, const u32*, const u32*, u32, u32, u32*,
+                            u32*);
+__global__ void kd_tscan_local(u32*, const u32*, const u32*, const u32*, u32*, u32*);
+__global__ void kd_tscan_carry(u32*, const u32*, const u32*);
 __global__ void kd_digit_base(const u32*, const u32*, u32*, u32*, u32*, const u32*);
 template <class K>
 __global__ void kd_copy_segs(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, const u32*,

@@ -129,90 +129,160 @@ __global__ __launch_bounds__(256) void k_gather(const T* __restrict__ vals, cons
 }
 
 // ---------------------------------------------------------------------------
-// a9: group boundaries (src/composer.cpp:630-678) as head flags plus a
-// single-pass decoupled look-back scan.  Writes dense monotone group ids and,
-// optionally, the start row of every group (segment offsets, with
-// offsets[ngroups] = n).
-// Tile layout: row = tile_base + j*256 + tid (coalesced per item index j).
+// extractGroupInfo (src/composer.cpp:630-678): dense group ids and group
+// start offsets (offsets[ngroups] = n) of a sorted column, in two passes
+// with no inter-tile waiting.  Row r is a head when
+// r == 0 or its value (or incoming group id) differs from row r - 1.
+// k_bcount counts the heads of every tile; after an exclusive scan of the
+// counts, k_bwrite recomputes the heads and writes ids / offsets.
 // ---------------------------------------------------------------------------
+// Row r of a tile is read as element i of lane l's 16-byte vector v:
+// r = base + (v * 256 + tid) * PV + i, PV = 16 / sizeof(K) keys per vector.
+// hm[v] gets the head bits of the lane's PV rows.  The predecessor of a
+// lane's first row is the next lane down's last row; lane 0 loads it.
 template <class K, int ITEMS>
-__global__ __launch_bounds__(256) void k_boundary(const K* __restrict__ v, const u32* __restrict__ in_group, u64 n,
-                                                  u32* __restrict__ out_group, u32* __restrict__ seg_off,
-                                                  u32* __restrict__ status, u32* __restrict__ counter,
-                                                  u32* __restrict__ ngroups_out) {
-  constexpr int NT = 256, NW = NT / WAVE, TILE = NT * ITEMS;
-  __shared__ u32 s_tile;
-  __shared__ u32 s_cnt[ITEMS][NW];
-  __shared__ u32 s_excl;
-  u32 tid = threadIdx.x, wv = tid / WAVE, lane = tid & (WAVE - 1);
-  if (tid == 0) s_tile = atomicAdd(counter, 1u);
-  __syncthreads();
-  u32 tile = s_tile;
-  u64 base = (u64)tile * TILE;
-  u64 masks[ITEMS];
-  u64 lt = lanemask_lt();
-  // one load per row: the predecessor comes from the next lane down; lane 0
-  // loads its own predecessor
-  K cur[ITEMS];
-  u32 gcur[ITEMS];
+__device__ __forceinline__ void tile_heads(const K* __restrict__ v, const u32* __restrict__ in_group, u64 n,
+                                           u64 base, bool vec_ok, u32 (&hm)[ITEMS * sizeof(K) / 16]) {
+  constexpr int PV = 16 / sizeof(K), NV = ITEMS / PV;
+  const u32 tid = threadIdx.x, lane = tid & (WAVE - 1);
+  K k[NV][PV];
+  u32 g[NV][PV];
 #pragma unroll
-  for (int j = 0; j < ITEMS; ++j) {
-    const u64 r = base + (u64)j * NT + tid;
-    cur[j] = r < n ? v[r] : (K)0;
-    gcur[j] = (in_group && r < n) ? in_group[r] : 0u;
-  }
+  for (int vv = 0; vv < NV; ++vv) {
+    const u64 e0 = base + ((u64)vv * 256 + tid) * PV;
+    if (vec_ok && e0 + PV <= n) {
+      unpack16<K>(*(const uint4*)(v + e0), k[vv]);
+      if (in_group) {
+        if constexpr (PV >= 4) {
 #pragma unroll
-  for (int j = 0; j < ITEMS; ++j) {
-    const u64 r = base + (u64)j * NT + tid;
-    u64 pv = (u64)__shfl_up((unsigned long long)cur[j], 1);
-    u32 pg = __shfl_up(gcur[j], 1);
-    if (lane == 0 && r > 0 && r < n) {
-      pv = (u64)v[r - 1];
-      if (in_group) pg = in_group[r - 1];
+          for (int i = 0; i < PV; i += 4) {
+            const uint4 x = *(const uint4*)(in_group + e0 + i);
+            g[vv][i] = x.x;
+            g[vv][i + 1] = x.y;
+            g[vv][i + 2] = x.z;
+            g[vv][i + 3] = x.w;
+          }
+        } else {
+          const uint2 x = *(const uint2*)(in_group + e0);
+          g[vv][0] = x.x;
+          g[vv][1] = x.y;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < PV; ++i) {
+        k[vv][i] = e0 + i < n ? v[e0 + i] : (K)0;
+        if (in_group) g[vv][i] = e0 + i < n ? in_group[e0 + i] : 0u;
+      }
     }
-    bool head = false;
-    if (r < n) head = (r == 0) || ((u64)cur[j] != pv) || (in_group && gcur[j] != pg);
-    u64 m = __ballot(head);
-    masks[j] = m;
-    if (lane == 0) s_cnt[j][wv] = (u32)__popcll(m);
+    if (!in_group) {
+#pragma unroll
+      for (int i = 0; i < PV; ++i) g[vv][i] = 0;
+    }
   }
+#pragma unroll
+  for (int vv = 0; vv < NV; ++vv) {
+    const u64 e0 = base + ((u64)vv * 256 + tid) * PV;
+    u64 pv = (u64)__shfl_up((unsigned long long)k[vv][PV - 1], 1);
+    u32 pg = __shfl_up(g[vv][PV - 1], 1);
+    if (lane == 0 && e0 > 0 && e0 < n) {
+      pv = (u64)v[e0 - 1];
+      if (in_group) pg = in_group[e0 - 1];
+    }
+    u32 m = 0;
+#pragma unroll
+    for (int i = 0; i < PV; ++i) {
+      const u64 pk = i ? (u64)k[vv][i - 1] : pv;
+      const u32 pgi = i ? g[vv][i - 1] : pg;
+      const bool head = (e0 + i < n) && (e0 + i == 0 || (u64)k[vv][i] != pk || g[vv][i] != pgi);
+      m |= (head ? 1u : 0u) << i;
+    }
+    hm[vv] = m;
+  }
+}
+
+template <class K, int ITEMS>
+__global__ __launch_bounds__(256) void k_bcount(const K* __restrict__ v, const u32* __restrict__ in_group, u64 n,
+                                                u32* __restrict__ tcnt) {
+  constexpr int NT = 256, NW = NT / WAVE, TILE = NT * ITEMS, NV = ITEMS * sizeof(K) / 16;
+  __shared__ u32 s_w[NW];
+  const u32 tid = threadIdx.x;
+  const bool vec_ok = ((uintptr_t)v & 15) == 0 && ((uintptr_t)in_group & 15) == 0;
+  u32 hm[NV];
+  tile_heads<K, ITEMS>(v, in_group, n, (u64)blockIdx.x * TILE, vec_ok, hm);
+  u32 c = 0;
+#pragma unroll
+  for (int vv = 0; vv < NV; ++vv) c += (u32)__popc(hm[vv]);
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) c += __shfl_down(c, o);
+  if ((tid & (WAVE - 1)) == 0) s_w[tid / WAVE] = c;
   __syncthreads();
   if (tid == 0) {
-    u32 tot = 0;
-    for (int j = 0; j < ITEMS; ++j)
-      for (int w = 0; w < NW; ++w) tot += s_cnt[j][w];
-    u32 excl = 0;
-    if (tile == 0) {
-      status_store(status, ST_INC | tot);
-    } else {
-      status_store(status + tile, ST_AGG | tot);
-      excl = lookback(status, tile, 0, 1);
-      status_store(status + tile, ST_INC | (excl + tot));
-    }
-    s_excl = excl;
-    if (base + TILE >= n) {
-      *ngroups_out = excl + tot;
-      if (seg_off) seg_off[excl + tot] = (u32)n;
-    }
+    u32 t = 0;
+    for (int w = 0; w < NW; ++w) t += s_w[w];
+    tcnt[blockIdx.x] = t;
+  }
+}
+
+template <class K, int ITEMS>
+__global__ __launch_bounds__(256) void k_bwrite(const K* __restrict__ v, const u32* __restrict__ in_group, u64 n,
+                                                const u32* __restrict__ tpre, u32* __restrict__ out_group,
+                                                u32* __restrict__ seg_off, u32* __restrict__ ngroups_out) {
+  constexpr int NT = 256, NW = NT / WAVE, TILE = NT * ITEMS, PV = 16 / sizeof(K), NV = ITEMS / PV;
+  __shared__ u32 s_cnt[NV][NW];
+  const u32 tid = threadIdx.x, wv = tid / WAVE, lane = tid & (WAVE - 1);
+  const u32 tile = blockIdx.x;
+  const u64 base = (u64)tile * TILE;
+  const bool vec_ok = ((uintptr_t)v & 15) == 0 && ((uintptr_t)in_group & 15) == 0;
+  u32 hm[NV], lex[NV];
+  tile_heads<K, ITEMS>(v, in_group, n, base, vec_ok, hm);
+#pragma unroll
+  for (int vv = 0; vv < NV; ++vv) {
+    const u32 c = (u32)__popc(hm[vv]);
+    const u32 inc = wave_incl_scan(c);
+    lex[vv] = inc - c;
+    if (lane == WAVE - 1) s_cnt[vv][wv] = inc;
   }
   __syncthreads();
-  u32 run = s_excl;
+  u32 run = tpre[tile];
+  const bool ovec = out_group && ((uintptr_t)out_group & 15) == 0;
 #pragma unroll
-  for (int j = 0; j < ITEMS; ++j) {
-    u32 before_w = 0, tot_j = 0;
+  for (int vv = 0; vv < NV; ++vv) {
+    u32 before_w = 0, tot_v = 0;
     for (int w = 0; w < NW; ++w) {
-      u32 c = s_cnt[j][w];
+      const u32 c = s_cnt[vv][w];
       before_w += (w < (int)wv) ? c : 0;
-      tot_j += c;
+      tot_v += c;
     }
-    u64 r = base + (u64)j * NT + tid;
-    if (r < n) {
-      bool head = (masks[j] >> lane) & 1;
-      u32 prefix = run + before_w + (u32)__popcll(masks[j] & lt);  // heads strictly before r
-      if (out_group) out_group[r] = head ? prefix : prefix - 1u;
-      if (head && seg_off) seg_off[prefix] = (u32)r;
+    const u64 e0 = base + ((u64)vv * 256 + tid) * PV;
+    u32 p = run + before_w + lex[vv];  // heads strictly before row e0
+    u32 ids[PV];
+#pragma unroll
+    for (int i = 0; i < PV; ++i) {
+      const bool head = (hm[vv] >> i) & 1u;
+      if (head && seg_off) seg_off[p] = (u32)(e0 + i);
+      ids[i] = head ? p : p - 1u;
+      p += head ? 1u : 0u;
     }
-    run += tot_j;
+    if (out_group) {
+      if (ovec && e0 + PV <= n) {
+        if constexpr (PV >= 4) {
+#pragma unroll
+          for (int i = 0; i < PV; i += 4) *(uint4*)(out_group + e0 + i) = make_uint4(ids[i], ids[i + 1], ids[i + 2], ids[i + 3]);
+        } else {
+          *(uint2*)(out_group + e0) = make_uint2(ids[0], ids[1]);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < PV; ++i)
+          if (e0 + i < n) out_group[e0 + i] = ids[i];
+      }
+    }
+    run += tot_v;
+  }
+  if (base + TILE >= n && tid == 0) {
+    *ngroups_out = run;
+    if (seg_off) seg_off[run] = (u32)n;
   }
 }
 
@@ -220,8 +290,8 @@ __global__ __launch_bounds__(256) void k_boundary(const K* __restrict__ v, const
 // Radix histograms.
 // k_hist_all: one pass over all keys computing the digit histogram of every
 // LSD pass at once (npasses <= 8).
-// k_seg_hist: per-segment histogram of one digit, over tiles that never
-// cross a segment (used by the segmented MSD partition).
+// (kd_seg_hist, fmcs_dd.hip: per-segment and per-tile histograms of one
+// digit over tiles that never cross a segment.)
 // ---------------------------------------------------------------------------
 template <class K>
 __global__ __launch_bounds__(256) void k_hist_all(const K* __restrict__ keys, u64 n, u32 npasses, u32 dbits,
@@ -241,29 +311,6 @@ __global__ __launch_bounds__(256) void k_hist_all(const K* __restrict__ keys, u6
   }
 }
 
-template <class K, int ITEMS>
-__global__ __launch_bounds__(256) void k_seg_hist(const K* __restrict__ keys, const u32* __restrict__ seg_lo,
-                                                  const u32* __restrict__ seg_hi, const u32* __restrict__ tile_base,
-                                                  u32 nsegs, u32 shift, u32 dbits, u32* __restrict__ hist) {
-  constexpr int TILE = 256 * ITEMS;
-  __shared__ u32 sh[RADIX];
-  __shared__ u32 s_seg;
-  u32 tile = blockIdx.x;
-  sh[threadIdx.x] = 0;
-  if (threadIdx.x == 0) s_seg = upper_bound_m1(tile_base, nsegs, tile);
-  __syncthreads();
-  u32 seg = s_seg;
-  u64 row0 = (u64)seg_lo[seg] + (u64)(tile - tile_base[seg]) * TILE;
-  u64 rowend = min((u64)seg_hi[seg], row0 + TILE);
-  u32 mask = (1u << dbits) - 1;
-  for (u64 r = row0 + threadIdx.x; r < rowend; r += 256) atomicAdd(&sh[digit_of(keys[r], shift, mask)], 1u);
-  __syncthreads();
-  u32 c = sh[threadIdx.x];
-  if (c) atomicAdd(&hist[(u64)seg * RADIX + threadIdx.x], c);
-}
-
-// digit_base[s][d] = lo[s] + sum_{d' < d} hist[s][d'] ; one block per set.
-// Optionally emits the bucket list (start, size) of set s in (s, d) order.
 __global__ __launch_bounds__(256) void k_digit_base(const u32* __restrict__ hist, const u32* __restrict__ lo,
                                                     u32* __restrict__ digit_base, u32* __restrict__ bstart,
                                                     u32* __restrict__ bsize) {
@@ -281,12 +328,15 @@ __global__ __launch_bounds__(256) void k_digit_base(const u32* __restrict__ hist
 }
 
 // ---------------------------------------------------------------------------
-// One radix pass (LSD over the whole array, or one MSD level over a list of
-// segments) in a single sweep with decoupled look-back, after the
-// "onesweep" scheme: tiles take ids from an atomic counter, rank their keys
-// stably with 64-lane ballot matching into per-wave LDS histograms, look back
-// over earlier tiles of the same segment for the per-digit global offset,
-// then scatter through LDS so the global writes are contiguous runs.
+// Scatter of one radix pass (LSD over the whole array, or one MSD level over
+// a list of segments).  The per-tile digit counts were written by the
+// histogram kernel and scanned per segment (kd_tscan_local/_carry), so a
+// tile's global offset of digit d is
+//   digit_base[seg][d] + tile_pre[tile][d] + (carry_in[chunk][d] if the
+//   segment started before the tile's scan chunk),
+// known before the tile starts: no tile waits on another.  A tile ranks its
+// keys stably with 64-lane ballot matching into per-wave LDS histograms and
+// scatters through LDS so the global writes are contiguous runs.
 // Tile layout: wave w owns rows [row0 + w*64*ITEMS, +64*ITEMS); item j of
 // lane l is row wave_base + j*64 + l, so the ranking order (j, lane) is the
 // row order and the pass is stable.
@@ -296,11 +346,11 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
                                                   K* __restrict__ kout, u32* __restrict__ vout,
                                                   const u32* __restrict__ seg_lo, const u32* __restrict__ seg_hi,
                                                   const u32* __restrict__ tile_base, u32 nsegs,
-                                                  const u32* __restrict__ digit_base, u32* __restrict__ status,
-                                                  u32* __restrict__ counter, u32 shift, u32 dbits,
+                                                  const u32* __restrict__ digit_base, const u32* __restrict__ tile_pre,
+                                                  const u32* __restrict__ carry_in, u32 shift, u32 dbits,
                                                   const u32* __restrict__ nsegs_d, const u32* __restrict__ ntiles_d) {
   constexpr int NT = 256, NW = NT / WAVE, TILE = NT * ITEMS, WITEMS = WAVE * ITEMS;
-  __shared__ u32 s_tile, s_seg;
+  __shared__ u32 s_seg;
   __shared__ u32 s_whist[NW][RADIX];
   __shared__ u32 s_tdig[RADIX];
   __shared__ u32 s_goff[RADIX];
@@ -311,19 +361,12 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
   u32 tid = threadIdx.x, wv = tid / WAVE, lane = tid & (WAVE - 1);
   if (nsegs_d) nsegs = *nsegs_d;
   const u32 ntiles = ntiles_d ? *ntiles_d : tile_base[nsegs];
-  // Persistent: a block takes tiles from the counter until none is left.
-  // Tiles are taken in id order, so every predecessor a look-back waits on
-  // is held by a running block that publishes without waiting.
-  for (;;) {
+  // persistent: block b takes tiles b, b + grid, ...
+  for (u32 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
   for (u32 i = tid; i < NW * RADIX; i += NT) (&s_whist[0][0])[i] = 0;
-  if (tid == 0) {
-    u32 t = atomicAdd(counter, 1u);
-    s_tile = t;
-    s_seg = t < ntiles ? upper_bound_m1(tile_base, nsegs, t) : 0u;
-  }
+  if (tid == 0) s_seg = upper_bound_m1(tile_base, nsegs, tile);
   __syncthreads();
-  if (s_tile >= ntiles) break;
-  const u32 tile = s_tile, seg = s_seg;
+  const u32 seg = s_seg;
   const u32 first_tile = tile_base[seg];
   const u64 row0 = (u64)seg_lo[seg] + (u64)(tile - first_tile) * TILE;
   const u64 rowend = min((u64)seg_hi[seg], row0 + TILE);
@@ -371,15 +414,9 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
     run += c;
   }
   const u32 tcount = run;
-  u32* st = status + d;
-  u32 excl = 0;
-  if (tile == first_tile) {
-    status_store(st + (u64)tile * RADIX, ST_INC | tcount);
-  } else {
-    status_store(st + (u64)tile * RADIX, ST_AGG | tcount);
-    excl = lookback(st, tile, first_tile, RADIX);
-    status_store(st + (u64)tile * RADIX, ST_INC | (excl + tcount));
-  }
+  const u32 chunk = tile / TSCAN_CT;
+  u32 excl = tile_pre[(u64)tile * RADIX + d];
+  if (first_tile < chunk * TSCAN_CT) excl += carry_in[(u64)chunk * RADIX + d];
   u32 ttot;
   u32 tdig = block_excl_scan<NT>(tcount, s_tmp, &ttot);
   s_tdig[d] = tdig;
@@ -1078,11 +1115,11 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
 // ---------------------------------------------------------------------------
 #define FMCS_INST_K(K)                                                                                              \
   template __global__ void k_gather<K>(const K*, const u32*, K*, u64);                                              \
-  template __global__ void k_boundary<K, 32>(const K*, const u32*, u64, u32*, u32*, u32*, u32*, u32*);              \
+  template __global__ void k_bcount<K, BOUNDARY_ITEMS>(const K*, const u32*, u64, u32*);                            \
+  template __global__ void k_bwrite<K, BOUNDARY_ITEMS>(const K*, const u32*, u64, const u32*, u32*, u32*, u32*);      \
   template __global__ void k_hist_all<K>(const K*, u64, u32, u32, u32*);                                            \
-  template __global__ void k_seg_hist<K, 16>(const K*, const u32*, const u32*, const u32*, u32, u32, u32, u32*);     \
-  template __global__ void k_onesweep<K, 16>(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*,    \
-                                             u32, const u32*, u32*, u32*, u32, u32, const u32*, const u32*);                                \
+  template __global__ void k_onesweep<K, ONESWEEP_ITEMS>(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*,    \
+                                             u32, const u32*, const u32*, const u32*, u32, u32, const u32*, const u32*);                                \
   template __global__ void k_tile_sort<K, 4>(TileSortArgs<K>);                                                      \
   template __global__ void k_tile_sort<K, 2>(TileSortArgs<K>);                                                      \
   template __global__ void k_tile_bucket<K, 4>(TileSortArgs<K>, u32*);                                             \
