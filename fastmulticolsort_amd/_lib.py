@@ -43,7 +43,8 @@ class GroupInfo(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("small_max", C.c_uint32), ("window", C.c_uint32), ("msd_bits", C.c_uint32)]
+    _fields_ = [("small_max", C.c_uint32), ("window", C.c_uint32), ("msd_bits", C.c_uint32),
+                ("gather_window_kb", C.c_uint32)]
 
 
 class FmcsError(RuntimeError):

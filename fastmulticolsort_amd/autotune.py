@@ -3,9 +3,11 @@
 Sweeps the bin thresholds and tile shapes of fmcs_tuning (small_max: the
 largest segment packed into an LDS tile, which also picks the 512- or
 1024-row tile; window: rows per packing window; msd_bits: digit width of an
-MSD level) on a workload, timing whole multi-round sorts with HIP events,
-and writes the fastest setting as JSON.  Results never depend on the
-tuning, only the time does.
+MSD level) and the source-window size of the oid gather (gather_window_kb,
+KiB, 0 = one pass) on a workload, timing whole multi-round sorts, and writes the
+fastest setting as JSON.  The gather window is swept first with the sort
+settings fixed, then the sort grid with the best window.  Results never
+depend on the tuning, only the time does.
 """
 import itertools
 import json
@@ -20,6 +22,7 @@ GRID = {
     "window_frac": [1.0, 0.5],
     "msd_bits": [6, 7, 8],
 }
+GATHER_KB = [0, 262144]
 
 
 def candidates(grid=GRID):
@@ -27,9 +30,13 @@ def candidates(grid=GRID):
         yield {"small_max": sm, "window": max(1, int(sm * wf)), "msd_bits": mb}
 
 
+def apply(ctx, setting):
+    ctx.set_tuning(setting["small_max"], setting["window"], setting["msd_bits"], setting.get("gather_window_kb", 0))
+
+
 def time_setting(ctx, run, setting, reps=3):
     import torch
-    ctx.set_tuning(setting["small_max"], setting["window"], setting["msd_bits"])
+    apply(ctx, setting)
     run()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -39,16 +46,23 @@ def time_setting(ctx, run, setting, reps=3):
     return (time.perf_counter() - t0) * 1e3 / reps
 
 
-def tune(ctx, run, grid=GRID, reps=3, log=None):
+def tune(ctx, run, grid=GRID, gather_kb=GATHER_KB, reps=3, log=None):
     """run(): one multi-round sort on resident data.  Returns (best, table)."""
     table = []
-    for st in candidates(grid):
+
+    def one(st):
         ms = time_setting(ctx, run, st, reps)
         table.append(dict(st, ms=round(ms, 4)))
         if log:
             log(f"tune {st} -> {ms:.3f} ms")
+        return ms
+
+    base = {k: v for k, v in ctx.tuning().items()}
+    gmb = min(gather_kb, key=lambda g: one(dict(base, gather_window_kb=g)))
+    for st in candidates(grid):
+        one(dict(st, gather_window_kb=gmb))
     best = min(table, key=lambda r: r["ms"])
-    ctx.set_tuning(best["small_max"], best["window"], best["msd_bits"])
+    apply(ctx, best)
     return best, table
 
 
@@ -63,5 +77,5 @@ def load(ctx, path=DEFAULT_PATH):
         return None
     with open(path) as f:
         best = json.load(f)["best"]
-    ctx.set_tuning(best["small_max"], best["window"], best["msd_bits"])
+    apply(ctx, best)
     return best

@@ -44,7 +44,7 @@ struct fmcs_ctx {
   Buf ws[WS_COUNT];
   u32* small_d = nullptr;   // 64 words of device scratch (counters, totals, consts)
   u32* host_pin = nullptr;  // PIN_WORDS words pinned for readbacks
-  fmcs_tuning tune = {512, 512, 8};
+  fmcs_tuning tune = {512, 512, 8, 0};
   hipEvent_t ev[8] = {};
   double breakdown[4] = {0, 0, 0, 0};
   // per-kernel event timing (off unless enabled)
@@ -160,6 +160,12 @@ u64 cap_grid(u64 want, u64 cap) { return std::max<u64>(1, std::min<u64>(want, ca
 // atomic counter): twice the resident blocks, so a block that finishes early
 // is replaced at once and the CUs stay full; extra blocks find no work.
 u64 persist_grid(fmcs_ctx* ctx, const void* fn, u64 want) { return cap_grid(want, 2ull * resident(ctx, fn)); }
+
+// Grid of k_onesweep: persistent, a multiple of XCDS (its XCD-aware tile walk).
+u64 onesweep_grid(fmcs_ctx* ctx, const void* fn, u64 ntiles) {
+  const u64 g = std::min<u64>(std::max<u64>(ntiles, 1), resident(ctx, fn));
+  return (g + XCDS - 1) / XCDS * XCDS;
+}
 
 int fill(fmcs_ctx* ctx, u32* p, u64 n, u32 v) {
   if (n == 0) return FMCS_OK;
@@ -281,7 +287,7 @@ int scatter_pass(fmcs_ctx* ctx, const char* name, const K* kin, const u32* vin, 
               ntiles_d, carry, cflag));
   TRY(klaunch(ctx, "k_tile_scan", kd_tscan_carry, RADIX, 0.0, nullptr, 0.0, carry, (const u32*)cflag, ntiles_d));
   return klaunch(ctx, name, k_onesweep<K, ONESWEEP_ITEMS>,
-                 persist_grid(ctx, (const void*)k_onesweep<K, ONESWEEP_ITEMS>, ntiles_max), bytes, dcount, per, kin, vin,
+                 onesweep_grid(ctx, (const void*)k_onesweep<K, ONESWEEP_ITEMS>, ntiles_max), bytes, dcount, per, kin, vin,
                  kout, vout, lo, hi, tbase, 0u, dbase, (const u32*)tcount, (const u32*)carry, shift, dbits, nsegs_d,
                  ntiles_d);
 }
@@ -669,6 +675,9 @@ int group_info_from_offsets(fmcs_ctx* ctx, u32 bank, const u32* seg_off, u32 nse
 }
 
 template <class K>
+int gather_launch(fmcs_ctx* ctx, const K* column, const u32* order, K* out, u64 n);
+
+template <class K>
 int sort_by_group_t(fmcs_ctx* ctx, u32 keybits, u64 n, const K* column, u32* order, const u32* in_group,
                     u32* out_group, int last_round, int mode, K* sorted, fmcs_group_info* info, u32* ngroups,
                     const u32* in_seg_off, u32 in_nseg, u32* out_seg_off, double* t_sort, double* t_gather,
@@ -681,13 +690,7 @@ int sort_by_group_t(fmcs_ctx* ctx, u32 keybits, u64 n, const K* column, u32* ord
     HIPCHK(hipEventRecord(ev[1], ctx->stream));
     TRY(radix_sort<K>(ctx, column, order, ka, order, n, keybits));
   } else {
-    u32 blocks = (u32)((n + 1023) / 1024);
-    if (blocks) {
-      if (((uintptr_t)order & 15) != 0) return fail(ctx, FMCS_E_ARG, "order must be 16-byte aligned");
-      Prof pf(ctx, "k_gather", (double)n * (4 + 2 * sizeof(K)));
-      hipLaunchKernelGGL((k_gather<K>), dim3(blocks), dim3(256), 0, ctx->stream, column, order, ka, n);
-      TRY(launch_check(ctx, "k_gather"));
-    }
+    if (n) TRY(gather_launch<K>(ctx, column, order, ka, n));
     HIPCHK(hipEventRecord(ev[1], ctx->stream));
     const u32* so = in_seg_off;
     u32 nseg = in_nseg;
@@ -997,11 +1000,23 @@ int seg_sort_dd_host(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, cons
   return FMCS_OK;
 }
 
+// out[i] = column[order[i]] (BATproject, src/composer.cpp:94-99): one pass,
+// or passes over source windows of tune.gather_window_kb KiB.
 template <class K>
 int gather_launch(fmcs_ctx* ctx, const K* column, const u32* order, K* out, u64 n) {
   if (((uintptr_t)order & 15) != 0) return fail(ctx, FMCS_E_ARG, "order must be 16-byte aligned");
-  return klaunch(ctx, "k_gather", k_gather<K>, (n + 1023) / 1024, (double)n * (4 + 2 * sizeof(K)), nullptr, 0.0,
-                 column, order, out, n);
+  const u64 grid = (n + 1023) / 1024;
+  const u64 wrows = (u64)ctx->tune.gather_window_kb * 1024 / sizeof(K);
+  if (wrows == 0 || wrows >= n)
+    return klaunch(ctx, "k_gather", k_gather<K>, grid, (double)n * (4 + 2 * sizeof(K)), nullptr, 0.0, column, order,
+                   out, n);
+  for (u64 lo = 0; lo < n; lo += wrows) {
+    const u64 len = std::min<u64>(wrows, n - lo);
+    // each pass reads the whole order; the value reads and writes add up to one
+    TRY(klaunch(ctx, "k_gather", k_gather_win<K>, grid, (double)n * 4 + (double)len * 2 * sizeof(K), nullptr, 0.0,
+                column, order, out, n, (u32)lo, (u32)len));
+  }
+  return FMCS_OK;
 }
 
 template <class K>
@@ -1080,7 +1095,8 @@ void graph_drop(fmcs_ctx* ctx) {
 int mrs_graph(fmcs_ctx* ctx, u32 nrounds, const void* const* cols, const u32* banks, const u32* bw, u64 n, u32* order,
               u32* groups_d) {
   std::vector<u64> key = {(u64)nrounds, n, (u64)(uintptr_t)order, (u64)(uintptr_t)groups_d, ctx->tune.small_max,
-                          ctx->tune.window, ctx->tune.msd_bits, (u64)(uintptr_t)ctx->stream};
+                          ctx->tune.window, ctx->tune.msd_bits, ctx->tune.gather_window_kb,
+                          (u64)(uintptr_t)ctx->stream};
   for (u32 r = 0; r < nrounds; ++r) {
     key.push_back((u64)(uintptr_t)cols[r]);
     key.push_back(((u64)banks[r] << 32) | bw[r]);
@@ -1262,7 +1278,7 @@ int fmcs_ctx_sync(fmcs_ctx* ctx) {
 int fmcs_ctx_set_tuning(fmcs_ctx* ctx, const fmcs_tuning* t) {
   if (!ctx || !t) return FMCS_E_ARG;
   if (t->small_max < 1 || t->small_max > (u32)TILESORT_T / 2 || t->window < 1 || t->window > t->small_max + 1 ||
-      t->msd_bits < 1 || t->msd_bits > (u32)RADIX_BITS)
+      t->msd_bits < 1 || t->msd_bits > (u32)RADIX_BITS || t->gather_window_kb > (1u << 24))
     return fail(ctx, FMCS_E_ARG, "tuning out of range");
   ctx->tune = *t;
   return FMCS_OK;
@@ -1389,13 +1405,11 @@ int fmcs_gather(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* values_d, 
   if (!bank_ok(bank)) return fail(ctx, FMCS_E_ARG, "gather: bank must be 2, 4 or 8");
   if (n == 0) return FMCS_OK;
   if (((uintptr_t)order_d & 15) != 0) return fail(ctx, FMCS_E_ARG, "gather: order must be 16-byte aligned");
-  u32 blocks = (u32)((n + 1023) / 1024);
   switch (bank) {
-    case 2: hipLaunchKernelGGL((k_gather<u16>), dim3(blocks), dim3(256), 0, ctx->stream, (const u16*)values_d, order_d, (u16*)out_d, n); break;
-    case 4: hipLaunchKernelGGL((k_gather<u32>), dim3(blocks), dim3(256), 0, ctx->stream, (const u32*)values_d, order_d, (u32*)out_d, n); break;
-    default: hipLaunchKernelGGL((k_gather<u64>), dim3(blocks), dim3(256), 0, ctx->stream, (const u64*)values_d, order_d, (u64*)out_d, n); break;
+    case 2: return gather_launch<u16>(ctx, (const u16*)values_d, order_d, (u16*)out_d, n);
+    case 4: return gather_launch<u32>(ctx, (const u32*)values_d, order_d, (u32*)out_d, n);
+    default: return gather_launch<u64>(ctx, (const u64*)values_d, order_d, (u64*)out_d, n);
   }
-  return launch_check(ctx, "k_gather");
 }
 
 int fmcs_extract_group_info(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* sorted_d, const uint32_t* in_group_d,

@@ -46,6 +46,7 @@ struct TileSortArgs {
   u32* fbcount;       // radix tile sort: persistent loop over fblist[0 .. *fbcount)
 };
 
+constexpr u32 XCDS = 8;  // accelerator dies of an MI355X; workgroup b runs on die b % XCDS
 constexpr int ONESWEEP_ITEMS = 16;
 constexpr int ONESWEEP_TILE = 256 * ONESWEEP_ITEMS;
 constexpr int BOUNDARY_ITEMS = 32;
@@ -59,6 +60,8 @@ constexpr int SCAN_TILE = 256 * SCAN_ITEMS;
 __global__ void k_stitch(StitchArgs a);
 template <class T>
 __global__ void k_gather(const T*, const u32*, T*, u64);
+template <class T>
+__global__ void k_gather_win(const T*, const u32*, T*, u64, u32, u32);
 template <class K, int ITEMS>
 __global__ void k_bcount(const K*, const u32*, u64, u32*);
 template <class K, int ITEMS>

@@ -286,6 +286,29 @@ __global__ __launch_bounds__(256) void k_bwrite(const K* __restrict__ v, const u
   }
 }
 
+// Windowed gather: out[i] = vals[order[i]] only for the rows whose oid lies
+// in [wlo, whi).  Passes over consecutive windows of the source read each
+// window while it stays in the memory-side cache: a window's random reads
+// hit there instead of fetching a full DRAM burst per 8-byte value, and the
+// rows of one large group (ascending oids) write contiguous runs per pass.
+template <class T>
+__global__ __launch_bounds__(256) void k_gather_win(const T* __restrict__ vals, const u32* __restrict__ order,
+                                                    T* __restrict__ out, u64 n, u32 wlo, u32 wlen) {
+  u64 i = ((u64)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (i + 4 <= n) {
+    const uint4 o = *(const uint4*)(order + i);
+    if (o.x - wlo < wlen) out[i] = vals[o.x];
+    if (o.y - wlo < wlen) out[i + 1] = vals[o.y];
+    if (o.z - wlo < wlen) out[i + 2] = vals[o.z];
+    if (o.w - wlo < wlen) out[i + 3] = vals[o.w];
+  } else {
+    for (; i < n; ++i) {
+      const u32 o = order[i];
+      if (o - wlo < wlen) out[i] = vals[o];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Radix histograms.
 // k_hist_all: one pass over all keys computing the digit histogram of every
@@ -361,8 +384,15 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
   u32 tid = threadIdx.x, wv = tid / WAVE, lane = tid & (WAVE - 1);
   if (nsegs_d) nsegs = *nsegs_d;
   const u32 ntiles = ntiles_d ? *ntiles_d : tile_base[nsegs];
-  // persistent: block b takes tiles b, b + grid, ...
-  for (u32 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  // Persistent and XCD-aware: workgroups are dealt to the XCDs round-robin
+  // (block b runs on XCD b % XCDS), so XCD x takes the x-th contiguous
+  // eighth of the tiles and its blocks walk it together.  Neighbouring tiles
+  // write neighbouring parts of every digit's output run, so the partial
+  // lines at run ends meet in one XCD's L2 instead of going out to memory
+  // from several.  The grid is a multiple of XCDS.
+  const u32 xcd = blockIdx.x % XCDS, per_x = gridDim.x / XCDS;
+  const u32 t_hi = (u32)((u64)ntiles * (xcd + 1) / XCDS);
+  for (u32 tile = (u32)((u64)ntiles * xcd / XCDS) + blockIdx.x / XCDS; tile < t_hi; tile += per_x) {
   for (u32 i = tid; i < NW * RADIX; i += NT) (&s_whist[0][0])[i] = 0;
   if (tid == 0) s_seg = upper_bound_m1(tile_base, nsegs, tile);
   __syncthreads();
@@ -384,29 +414,39 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
     keys[j] = ok ? kin[r] : (K)0;
     vals[j] = ok ? vin[r] : 0u;
   }
+  // this tile's global offset of digit d = tid, loaded now so the loads
+  // overlap the ranking
+  const u32 d = tid;
+  const u32 chunk = tile / TSCAN_CT;
+  u32 gbase = digit_base[(u64)seg * RADIX + d] + tile_pre[(u64)tile * RADIX + d];
+  if (first_tile < chunk * TSCAN_CT) gbase += carry_in[(u64)chunk * RADIX + d];
+  // Stable ranking: lanes with equal digits are matched by ballots; the
+  // lowest such lane adds the match count to the wave's LDS counter with one
+  // returning atomic.  The atomics of all items are issued back to back (LDS
+  // executes a wave's operations in order, so item j sees the counts of
+  // items < j) and their results are collected afterwards.
+  // rank[j] first holds (leader lane << 8) | matches in lower lanes
+  u32 old[ITEMS];
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
     u64 r = wbase + (u64)j * WAVE + lane;
     bool ok = r < rowend;
-    u32 d = digit_of(keys[j], shift, mask);
+    u32 dg = digit_of(keys[j], shift, mask);
     u64 peers = __ballot(ok);
     for (u32 b = 0; b < dbits; ++b) {
-      u64 bb = __ballot((d >> b) & 1u);
-      peers &= ((d >> b) & 1u) ? bb : ~bb;
+      u64 bb = __ballot((dg >> b) & 1u);
+      peers &= ((dg >> b) & 1u) ? bb : ~bb;
     }
     if (!ok) peers = 0;
-    u32 leader = ok ? (u32)(__ffsll((long long)peers) - 1) : lane;
-    u32 old = 0;
-    if (ok && lane == leader) {
-      old = s_whist[wv][d];
-      s_whist[wv][d] = old + (u32)__popcll(peers);
-    }
-    old = __shfl(old, (int)leader);
-    rank[j] = old + (u32)__popcll(peers & lt);
+    const u32 lead = ok ? (u32)(__ffsll((long long)peers) - 1) : lane;
+    rank[j] = (lead << 8) | (u32)__popcll(peers & lt);
+    old[j] = 0;
+    if (ok && lane == lead) old[j] = atomicAdd(&s_whist[wv][dg], (u32)__popcll(peers));
   }
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) rank[j] = __shfl(old[j], (int)(rank[j] >> 8)) + (rank[j] & 0xffu);
   __syncthreads();
   // per-digit: wave offsets within the tile and the tile's count
-  u32 d = tid;
   u32 run = 0;
   for (int w = 0; w < NW; ++w) {
     u32 c = s_whist[w][d];
@@ -414,13 +454,10 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
     run += c;
   }
   const u32 tcount = run;
-  const u32 chunk = tile / TSCAN_CT;
-  u32 excl = tile_pre[(u64)tile * RADIX + d];
-  if (first_tile < chunk * TSCAN_CT) excl += carry_in[(u64)chunk * RADIX + d];
   u32 ttot;
   u32 tdig = block_excl_scan<NT>(tcount, s_tmp, &ttot);
   s_tdig[d] = tdig;
-  s_goff[d] = digit_base[(u64)seg * RADIX + d] + excl - tdig;
+  s_goff[d] = gbase - tdig;
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
@@ -434,11 +471,24 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
   }
   __syncthreads();
   const u32 cnt = (u32)(rowend - row0);
-  for (u32 i = tid; i < cnt; i += NT) {
-    K k = s_keys[i];
-    u32 g = s_goff[digit_of(k, shift, mask)] + i;
-    kout[g] = k;
-    vout[g] = s_vals[i];
+  if (cnt == (u32)TILE) {
+    // full tile: independent iterations, so the LDS reads of all rows
+    // overlap
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+      const u32 i = it * NT + tid;
+      const K k = s_keys[i];
+      const u32 g = s_goff[digit_of(k, shift, mask)] + i;
+      kout[g] = k;
+      vout[g] = s_vals[i];
+    }
+  } else {
+    for (u32 i = tid; i < cnt; i += NT) {
+      K k = s_keys[i];
+      u32 g = s_goff[digit_of(k, shift, mask)] + i;
+      kout[g] = k;
+      vout[g] = s_vals[i];
+    }
   }
   __syncthreads();
   }
@@ -1115,6 +1165,7 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
 // ---------------------------------------------------------------------------
 #define FMCS_INST_K(K)                                                                                              \
   template __global__ void k_gather<K>(const K*, const u32*, K*, u64);                                              \
+  template __global__ void k_gather_win<K>(const K*, const u32*, K*, u64, u32, u32);                                \
   template __global__ void k_bcount<K, BOUNDARY_ITEMS>(const K*, const u32*, u64, u32*);                            \
   template __global__ void k_bwrite<K, BOUNDARY_ITEMS>(const K*, const u32*, u64, const u32*, u32*, u32*, u32*);      \
   template __global__ void k_hist_all<K>(const K*, u64, u32, u32, u32*);                                            \

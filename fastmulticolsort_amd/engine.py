@@ -90,10 +90,13 @@ class Context:
     def tuning(self):
         t = Tuning()
         self._chk(self.lib.fmcs_ctx_get_tuning(self.h, C.byref(t)))
-        return {"small_max": t.small_max, "window": t.window, "msd_bits": t.msd_bits}
+        return {"small_max": t.small_max, "window": t.window, "msd_bits": t.msd_bits,
+                "gather_window_kb": t.gather_window_kb}
 
-    def set_tuning(self, small_max, window, msd_bits):
-        t = Tuning(small_max, window, msd_bits)
+    def set_tuning(self, small_max, window, msd_bits, gather_window_kb=None):
+        if gather_window_kb is None:
+            gather_window_kb = self.tuning()["gather_window_kb"]
+        t = Tuning(small_max, window, msd_bits, gather_window_kb)
         self._chk(self.lib.fmcs_ctx_set_tuning(self.h, C.byref(t)))
 
     # ---- kernels -------------------------------------------------------

@@ -72,19 +72,19 @@ def test_graph_replays_match_eager_and_oracle(ctx):
 def test_graph_recaptures_on_tuning_and_shape_change(ctx):
     old = ctx.tuning()
     try:
-        for (n, sm, w, mb) in [(1 << 16, 512, 512, 8), (1 << 16, 256, 128, 6), (50001, 256, 128, 6),
-                               (50001, 512, 512, 8)]:
+        for (n, sm, w, mb, gkb) in [(1 << 16, 512, 512, 8, 0), (1 << 16, 256, 128, 6, 64), (50001, 256, 128, 6, 0),
+                                    (50001, 512, 512, 8, 96)]:
             (c1, c2), bws = datagen.config3(n, ngroups=1 << 8)
-            ctx.set_tuning(sm, w, mb)
+            ctx.set_tuning(sm, w, mb, gkb)
             d1, d2 = dev(c1), dev(c2)
             order = torch.empty(n, dtype=torch.uint32, device="cuda")
-            log("graph recapture", n, sm, w, mb)
+            log("graph recapture", n, sm, w, mb, gkb)
             for rep in range(3):
                 ctx.multi_round_sort([d1, d2], bws, order=order, mode=MODE_GRAPH, want_count=False)
                 torch.cuda.synchronize()
                 assert np.array_equal(host(order), lex([c1, c2])), (n, sm, rep)
     finally:
-        ctx.set_tuning(old["small_max"], old["window"], old["msd_bits"])
+        ctx.set_tuning(old["small_max"], old["window"], old["msd_bits"], old["gather_window_kb"])
 
 
 @pytest.mark.parametrize("mode", [MODE_DEVICE, MODE_GRAPH])

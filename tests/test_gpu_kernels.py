@@ -76,6 +76,24 @@ def test_gather(ctx, dt, n):
 
 
 @pytest.mark.parametrize("dt", [np.uint16, np.uint32, np.uint64])
+@pytest.mark.parametrize("kb", [1, 3, 64])
+def test_gather_windowed(ctx, dt, kb):
+    # passes over source windows of kb KiB (a window edge inside a 4-row
+    # vector, a last partial window) give the one-pass result
+    n = 100003
+    old = ctx.tuning()
+    try:
+        ctx.set_tuning(old["small_max"], old["window"], old["msd_bits"], kb)
+        v = datagen.splitmix64(8, n).astype(dt)
+        order = datagen.permutation(4, n).astype(np.uint32)
+        order[: n // 2] = np.sort(order[: n // 2])  # ascending runs, as in a sorted group
+        out = ctx.gather(dev(v), dev(order))
+        assert np.array_equal(host(out), v[order])
+    finally:
+        ctx.set_tuning(old["small_max"], old["window"], old["msd_bits"], old["gather_window_kb"])
+
+
+@pytest.mark.parametrize("dt", [np.uint16, np.uint32, np.uint64])
 @pytest.mark.parametrize("n", [1, 2, 4095, 4096, 4097, 300001])
 def test_extract_group_info(ctx, dt, n):
     log("boundary", dt, n)
