@@ -753,7 +753,8 @@ int dd_scan(fmcs_ctx* ctx, const u32* in, u32* out, const u32* n_d, u64 n_max, u
 }
 
 // LSD radix sort over every pass (no host-side trivial-pass skipping).
-// Keys above keybits set the error word dd[DD_ERR].
+// Keys above keybits set the error word dd[DD_ERR].  vin == nullptr stands
+// for the identity order (values = row indices), generated in the first pass.
 template <class K>
 int radix_sort_dd(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, u64 n, u32 keybits, u32* dd) {
   if (n == 0) return FMCS_OK;
@@ -934,7 +935,7 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
     a.src_sel = (u32)(l & 1); a.kb = d.tkb; a.pad_seg = T; a.direct = 0; a.wlist = wlist;
     a.wcount = S + LV_WCOUNT; a.fblist = fblist; a.fbcount = S + LV_FB;
     {
-      const u64 g = persist_grid(ctx, fn_bucket, d.nw_max);
+      const u64 g = (persist_grid(ctx, fn_bucket, d.nw_max) + XCDS - 1) / XCDS * XCDS;
       const double per = 2.0 * (sizeof(K) + 4);
       if (T <= 512)
         TRY(klaunch(ctx, "k_tile_bucket", k_tile_bucket<K, 2>, g, 0.0, S + LV_ROWS_SMALL, per, a, fb));
@@ -1025,7 +1026,8 @@ int dd_round(fmcs_ctx* ctx, u32 r, u32 keybits, u64 n, const K* column, u32* ord
   K* ka;
   TRY(getbuf_t(ctx, WS_KA, n, &ka));
   if (r == 0) {
-    TRY(radix_sort_dd<K>(ctx, column, order, ka, order, n, keybits, dd));
+    // the identity order of round 1 (src/composer.cpp:60-63) comes from the first pass
+    TRY(radix_sort_dd<K>(ctx, column, nullptr, ka, order, n, keybits, dd));
   } else {
     TRY(gather_launch<K>(ctx, column, order, ka, n));
     u32* lv = dd + DD_HDR + (u64)r * DD_LVL_MAX * DD_LVL_WORDS;
@@ -1041,7 +1043,6 @@ int mrs_dd(fmcs_ctx* ctx, u32 nrounds, const void* const* cols, const u32* banks
   u32* dd;
   TRY(getbuf_t(ctx, WS_DD, DD_WORDS, &dd));
   TRY(fill(ctx, dd, DD_WORDS, 0));
-  TRY(klaunch(ctx, "k_iota", k_iota_u32, cap_grid((n + 255) / 256, 8192), 0.0, nullptr, 0.0, order, n));
   u32 *ga, *gb, *offa, *offb;
   TRY(getbuf_t(ctx, WS_HASH, n, &ga));
   TRY(getbuf_t(ctx, WS_HASH2, n, &gb));

@@ -408,7 +408,7 @@ __global__ void kd_copy_pairs(const K* __restrict__ kin, const u32* __restrict__
                               u32* __restrict__ vout, u64 n) {
   for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) {
     kout[i] = kin[i];
-    vout[i] = vin[i];
+    vout[i] = vin ? vin[i] : (u32)i;  // no vin: the identity order
   }
 }
 

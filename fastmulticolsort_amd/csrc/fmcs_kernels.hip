@@ -412,7 +412,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
     u64 r = wbase + (u64)j * WAVE + lane;
     bool ok = r < rowend;
     keys[j] = ok ? kin[r] : (K)0;
-    vals[j] = ok ? vin[r] : 0u;
+    vals[j] = ok ? (vin ? vin[r] : (u32)r) : 0u;  // no vin: the row index (identity order)
   }
   // this tile's global offset of digit d = tid, loaded now so the loads
   // overlap the ranking
@@ -890,12 +890,17 @@ __device__ __forceinline__ void tile_bucket_one(const TileSortArgs<K>& a, u32 w,
 }
 
 // One window per block, or (a.wcount set) a persistent loop over the
-// compact window list.
+// compact window list, which lists windows roughly in row order: die x
+// (blocks b with b % XCDS == x) takes the x-th contiguous part, so the
+// shared lines at window edges are written from one L2.  The persistent
+// grid is a multiple of XCDS.
 template <class K, int E>
 __global__ __launch_bounds__(256) void k_tile_bucket(TileSortArgs<K> a, u32* __restrict__ fallback) {
   if (a.wcount) {
     const u32 cnt = *a.wcount;
-    for (u32 b = blockIdx.x; b < cnt; b += gridDim.x) {
+    const u32 xcd = blockIdx.x % XCDS, per_x = gridDim.x / XCDS;
+    const u32 hi = (u32)((u64)cnt * (xcd + 1) / XCDS);
+    for (u32 b = (u32)((u64)cnt * xcd / XCDS) + blockIdx.x / XCDS; b < hi; b += per_x) {
       tile_bucket_one<K, E>(a, a.wlist[b], fallback);
       __syncthreads();
     }
