@@ -8,7 +8,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libfmcs.so")
 
-HIP_SOURCES = ["fmcs_kernels.hip", "fmcs_dd.hip", "fmcs_capi.hip", "fmcs_mlp.hip"]
+HIP_SOURCES = ["fmcs_kernels.hip", "fmcs_dd.hip", "fmcs_merge.hip", "fmcs_capi.hip", "fmcs_mlp.hip"]
 HOST_SOURCES = ["fmcs_plan.cpp", "fmcs_composer.cpp"]
 
 HIPCC_FLAGS = [

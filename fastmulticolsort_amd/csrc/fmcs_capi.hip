@@ -396,6 +396,36 @@ int radix_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, 
   return FMCS_OK;
 }
 
+// Comparison sort of (kin, vin) -> (kout, vout): bitonic LDS tiles, then
+// merge-path passes doubling the sorted run length, ping-ponging through
+// the workspace.
+template <class K>
+int merge_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, u64 n) {
+  if (n == 0) return FMCS_OK;
+  if (n >= (1ull << 30)) return fail(ctx, FMCS_E_UNSUPPORTED, "merge sort supports n < 2^30");
+  K *t0k, *t1k;
+  u32 *t0v, *t1v;
+  TRY(getbuf_t(ctx, WS_T0K, n, &t0k));
+  TRY(getbuf_t(ctx, WS_T0V, n, &t0v));
+  TRY(getbuf_t(ctx, WS_T1K, n, &t1k));
+  TRY(getbuf_t(ctx, WS_T1V, n, &t1v));
+  const u64 grid = (n + MERGE_TILE - 1) / MERGE_TILE;
+  const double pass_bytes = 2.0 * n * (sizeof(K) + 4);
+  TRY(klaunch(ctx, "k_bitonic_tile", k_bitonic_tile<K>, grid, pass_bytes, nullptr, 0.0, kin, vin, t0k, t0v, n));
+  K* sk = t0k;
+  u32* sv = t0v;
+  K* dk = t1k;
+  u32* dv = t1v;
+  for (u64 run = MERGE_TILE; run < n; run *= 2) {
+    TRY(klaunch(ctx, "k_merge_pass", k_merge_pass<K>, grid, pass_bytes, nullptr, 0.0, (const K*)sk, (const u32*)sv,
+                dk, dv, n, run));
+    std::swap(sk, dk);
+    std::swap(sv, dv);
+  }
+  return klaunch(ctx, "k_copy", kd_copy_pairs<K>, cap_grid((n + 255) / 256, 4096), pass_bytes, nullptr, 0.0,
+                 (const K*)sk, (const u32*)sv, kout, vout, n);
+}
+
 // Group boundaries: head counts per tile, their exclusive scan, then ids
 // and offsets; the group count goes to *count_d (device).
 template <class K>
@@ -1431,6 +1461,16 @@ int fmcs_sort_pairs(fmcs_ctx* ctx, uint32_t bank, uint32_t keybits, uint64_t n, 
     case 2: return radix_sort<u16>(ctx, (u16*)keys_d, oids_d, (u16*)keys_d, oids_d, n, keybits);
     case 4: return radix_sort<u32>(ctx, (u32*)keys_d, oids_d, (u32*)keys_d, oids_d, n, keybits);
     case 8: return radix_sort<u64>(ctx, (u64*)keys_d, oids_d, (u64*)keys_d, oids_d, n, keybits);
+    default: return fail(ctx, FMCS_E_ARG, "bank must be 2, 4 or 8");
+  }
+}
+
+int fmcs_sort_pairs_merge(fmcs_ctx* ctx, uint32_t bank, uint64_t n, void* keys_d, uint32_t* oids_d) {
+  if (!ctx) return FMCS_E_ARG;
+  switch (bank) {
+    case 2: return merge_sort<u16>(ctx, (u16*)keys_d, oids_d, (u16*)keys_d, oids_d, n);
+    case 4: return merge_sort<u32>(ctx, (u32*)keys_d, oids_d, (u32*)keys_d, oids_d, n);
+    case 8: return merge_sort<u64>(ctx, (u64*)keys_d, oids_d, (u64*)keys_d, oids_d, n);
     default: return fail(ctx, FMCS_E_ARG, "bank must be 2, 4 or 8");
   }
 }

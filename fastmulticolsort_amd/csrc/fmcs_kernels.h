@@ -55,6 +55,7 @@ constexpr int TILESORT_E = 4;
 constexpr int TILESORT_T = 256 * TILESORT_E;
 constexpr int SCAN_ITEMS = 8;
 constexpr u32 TSCAN_CT = 32;  // tiles per chunk of the per-segment tile-count scan
+constexpr u32 MERGE_TILE = 2048;  // rows per bitonic tile / per merge-pass block
 constexpr int SCAN_TILE = 256 * SCAN_ITEMS;
 
 __global__ void k_stitch(StitchArgs a);
@@ -100,6 +101,12 @@ template <class K>
 __global__ void k_fill_random(K*, u64, u64, u32);
 __global__ void k_uniform_offsets(u32*, u32, u32, u64);
 
+
+// Comparison sort (fmcs_merge.hip).
+template <class K>
+__global__ void k_bitonic_tile(const K*, const u32*, K*, u32*, u64);
+template <class K>
+__global__ void k_merge_pass(const K*, const u32*, K*, u32*, u64, u64);
 
 // Device-driven list kernels (fmcs_dd.hip): sizes come from device words.
 __global__ void kd_set4(u32*, u32, u32, u32, u32);

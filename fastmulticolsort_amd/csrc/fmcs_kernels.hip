@@ -1106,11 +1106,22 @@ __device__ __forceinline__ u64 mix64(u64 x) {
 }
 
 // table has cmask+1 >= 2n slots, so a free or matching slot always exists
-// and every probe sequence ends.
+// and every probe sequence ends.  The first (smallest) row of every slot is
+// kept per block in a small LDS table and added to `first` with one global
+// atomicMin per block and slot, so a column with few distinct values does
+// not serialize millions of atomics on a handful of words; slots that do
+// not fit the LDS table go straight to `first`.
+constexpr u32 HASH_LDS = 1024, HASH_LDS_PROBES = 8;
 template <class K>
 __global__ __launch_bounds__(256) void k_hash_insert(const K* __restrict__ keys, const u32* __restrict__ seg, u64 n,
                                                      u32* __restrict__ table, u32 cmask, u32* __restrict__ slot_of,
                                                      u32* __restrict__ first) {
+  __shared__ u32 s_slot[HASH_LDS], s_min[HASH_LDS];
+  for (u32 j = threadIdx.x; j < HASH_LDS; j += blockDim.x) {
+    s_slot[j] = HASH_EMPTY;
+    s_min[j] = HASH_EMPTY;
+  }
+  __syncthreads();
   for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) {
     const K k = keys[i];
     const u32 g = seg ? seg[i] : 0u;
@@ -1126,8 +1137,22 @@ __global__ __launch_bounds__(256) void k_hash_insert(const K* __restrict__ keys,
       h = (h + 1) & cmask;
     }
     slot_of[i] = h;
-    atomicMin(first + h, (u32)i);
+    // block-local first row of slot h
+    u32 e = (h * 0x9E3779B1u) >> 22;  // HASH_LDS = 2^10 buckets
+    bool done = false;
+    for (u32 p = 0; p < HASH_LDS_PROBES && !done; ++p, e = (e + 1) & (HASH_LDS - 1)) {
+      u32 cur = s_slot[e];
+      if (cur == HASH_EMPTY) cur = atomicCAS(&s_slot[e], HASH_EMPTY, h), cur = (cur == HASH_EMPTY) ? h : cur;
+      if (cur == h) {
+        atomicMin(&s_min[e], (u32)i);
+        done = true;
+      }
+    }
+    if (!done) atomicMin(first + h, (u32)i);
   }
+  __syncthreads();
+  for (u32 j = threadIdx.x; j < HASH_LDS; j += blockDim.x)
+    if (s_slot[j] != HASH_EMPTY) atomicMin(first + s_slot[j], s_min[j]);
 }
 
 __global__ void k_hash_heads(const u32* __restrict__ slot_of, const u32* __restrict__ first, u64 n,

@@ -130,6 +130,10 @@ class Context:
     def sort_pairs(self, keys, oids, keybits):
         self._chk(self.lib.fmcs_sort_pairs(self.h, bank_of(keys), keybits, keys.numel(), _p(keys), _p(oids)))
 
+    def sort_pairs_merge(self, keys, oids):
+        """Bitonic LDS tiles + merge-path passes; the same result as sort_pairs."""
+        self._chk(self.lib.fmcs_sort_pairs_merge(self.h, bank_of(keys), keys.numel(), _p(keys), _p(oids)))
+
     def segmented_sort(self, keys, oids, seg_offsets, nseg, keybits, mode=MODE_BATCHED, want_info=False):
         info = GroupInfo() if want_info else None
         self._chk(self.lib.fmcs_segmented_sort(self.h, bank_of(keys), keybits, keys.numel(), _p(keys), _p(oids),

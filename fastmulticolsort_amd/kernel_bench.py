@@ -1,0 +1,201 @@
+"""GPU measurements of the path's kernels on the other workloads, beside
+bench.py's headline (the batched segmented sort, config 3).
+
+    python -m fastmulticolsort_amd.kernel_bench [--configs 1,2,4] > out.jsonl
+
+Prints one JSON object per measurement:
+  * config 1 (2^24 rows, c1 17-bit u32 over 8192 codes, c2 33-bit u64): every
+    split of the two-round sweep (TwoRoundsExhaustive,
+    src/stitch_composer.cpp:4197-4258) run end to end through the composer,
+    the analytic model's pick and its regret;
+  * config 2 (2^26 rows, 12/20/30-bit columns stitched into one 62-bit u64
+    key): the stitch, the LSD radix sort, the bitonic + merge-path sort and
+    the payload gathers, with their HBM roofline fractions;
+  * config 4 (2^25 rows, 2/1/12/27-bit columns): the seeded PlanEnumRRS
+    plans plus the 1- and 2-round plans, end to end, and the hash GROUP BY
+    with its exact group count.
+Every result is checked (sorted order / group count) before it is printed.
+Kernel times are HIP-event timers on the context's stream; `achieved` is
+algorithmic bytes / kernel time against the 8 TB/s HBM peak.
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _emit(d):
+    print(json.dumps(d), flush=True)
+
+
+def _kernel_rows(stats, reps):
+    out = {}
+    for k, v in stats.items():
+        ms = v["ms"] / max(1, reps)
+        gbs = v["bytes"] / (v["ms"] / 1e3) / 1e9 if v["ms"] > 0 and v["bytes"] > 0 else None
+        out[k] = {"ms": round(ms, 4), "launches": v["launches"] // max(1, reps),
+                  "GBs": round(gbs, 1) if gbs else None, "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None}
+    return out
+
+
+def _timed(ctx, fn, reps=3):
+    """Wall time per call (ms, after a warm-up) and the per-kernel timers."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    ctx.set_profiling(True)
+    ctx.kernel_stats(reset=True)
+    for _ in range(reps):
+        fn()
+    stats = ctx.kernel_stats(reset=True)
+    ctx.set_profiling(False)
+    return ms, _kernel_rows(stats, reps)
+
+
+def config1(ctx, n=1 << 24):
+    import torch
+    from . import datagen, planner
+    from .engine import MODE_DEVICE
+    (c1, c2), bws = datagen.config1(n)
+    cols = [_dev(c1), _dev(c2)]
+    exp = np.lexsort((c2, c1)).astype(np.uint32)
+    order = torch.empty(n, dtype=torch.uint32, device="cuda")
+    rows = []
+    for s in range(-bws[1], bws[0] + 1):
+        split = [w for w in (bws[0] - s, bws[1] + s) if w > 0]
+        ms, _ = _timed(ctx, lambda: planner.run_plan(ctx, cols, bws, [0, 1], split, mode=MODE_DEVICE, order=order),
+                       reps=2)
+        ok = bool(np.array_equal(order.cpu().numpy(), exp))
+        rows.append({"split": split, "ms": round(ms, 3), "ok": ok})
+    model = planner.default_cost_model()
+    card = [8192, datagen.exact_cardinality(c2)]
+    _, pick, _, _ = planner.optimize(model, n, bws, card)
+    best = min(rows, key=lambda r: r["ms"])
+    picked = next((r for r in rows if r["split"] == pick), None)
+    _emit({"config": 1, "what": "two-round sweep, every split end to end (composer, device-driven batched)",
+           "rows": n, "plans": len(rows), "all_ok": all(r["ok"] for r in rows),
+           "best": best, "worst": max(rows, key=lambda r: r["ms"]),
+           "best_Mrows_s": round(n / best["ms"] / 1e3, 1),
+           "analytic_pick": pick, "analytic_pick_ms": picked["ms"] if picked else None,
+           "analytic_regret": round(picked["ms"] / best["ms"] - 1, 4) if picked else None, "sweep": rows})
+
+
+def config2(ctx, n=1 << 26):
+    import torch
+    from . import datagen
+    from .engine import construct_entries
+    cols, bws = datagen.config2(n)
+    src = [_dev(c) for c in cols]
+    ents = construct_entries(bws, [0, 1, 2], [62])
+    key = torch.empty(n, dtype=torch.uint64, device="cuda")
+    ms, kern = _timed(ctx, lambda: ctx.stitch(src, bws, ents, [key]))
+    sbytes = n * (2 + 4 + 4 + 8)
+    _emit({"config": 2, "what": "stitch 12|20|30 -> one 62-bit u64 key", "rows": n, "ms": round(ms, 3),
+           "achieved_GBs": round(sbytes / (ms / 1e3) / 1e9, 1), "frac": round(sbytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+           "kernels": kern})
+    k_host = ((cols[0].astype(np.uint64) << np.uint64(50)) | (cols[1].astype(np.uint64) << np.uint64(30))
+              | cols[2].astype(np.uint64))
+    assert np.array_equal(key.cpu().numpy(), k_host), "stitch mismatch"
+    exp = np.argsort(k_host, kind="stable").astype(np.uint32)
+    iota = _dev(np.arange(n, dtype=np.uint32))
+    kk = torch.empty_like(key)
+    oo = torch.empty_like(iota)
+
+    def lsd():
+        kk.copy_(key)
+        oo.copy_(iota)
+        ctx.sort_pairs(kk, oo, 62)
+
+    def merge():
+        kk.copy_(key)
+        oo.copy_(iota)
+        ctx.sort_pairs_merge(kk, oo)
+
+    res = {}
+    for name, fn in (("lsd_radix", lsd), ("bitonic_merge_path", merge)):
+        ms, kern = _timed(ctx, fn)
+        ok = bool(np.array_equal(oo.cpu().numpy(), exp))
+        res[name] = {"ms": round(ms, 3), "Mrows_s": round(n / ms / 1e3, 1), "ok": ok, "kernels": kern}
+    _emit({"config": 2, "what": "62-bit single-round sort of (key, oid): LSD radix vs bitonic tiles + merge path",
+           "rows": n, **res})
+    # oid payload gather of the three source columns in sorted order
+    outs = [torch.empty_like(c) for c in src]
+    ms, kern = _timed(ctx, lambda: [ctx.gather(c, oo, out=o) for c, o in zip(src, outs)])
+    ok = all(np.array_equal(o.cpu().numpy(), c[exp]) for o, c in zip(outs, cols))
+    gbytes = n * (3 * 4 + 2 * (2 + 4 + 4))
+    _emit({"config": 2, "what": "payload gather of the 3 source columns by the sorted oids", "rows": n,
+           "ms": round(ms, 3), "ok": ok, "achieved_GBs": round(gbytes / (ms / 1e3) / 1e9, 1), "kernels": kern})
+
+
+def config4(ctx, n=1 << 25, max_rrs=24):
+    import torch
+    from . import datagen, planner
+    from .engine import MODE_DEVICE
+    cols, bws = datagen.config4(n)
+    dcols = [_dev(c) for c in cols]
+    stacked = [c.astype(np.uint64) for c in cols]
+    exp = np.lexsort(tuple(reversed(stacked))).astype(np.uint32)
+    model = planner.default_cost_model()
+    card = [datagen.exact_cardinality(c) for c in cols]
+    plans = [[42]] + planner.split_exhaustive(42, 2, 64) + \
+        planner.plan_enum_rrs(model, n, bws, card, [0, 1, 2, 3], 6, 1)[:max_rrs]
+    seen, uniq = set(), []
+    for p in plans:
+        t = tuple(int(w) for w in p)
+        if t not in seen:
+            seen.add(t)
+            uniq.append(list(t))
+    order = torch.empty(n, dtype=torch.uint32, device="cuda")
+    rows = []
+    for split in uniq:
+        ms, _ = _timed(ctx, lambda: planner.run_plan(ctx, dcols, bws, [0, 1, 2, 3], split, mode=MODE_DEVICE,
+                                                     order=order), reps=2)
+        rows.append({"split": split, "ms": round(ms, 3), "ok": bool(np.array_equal(order.cpu().numpy(), exp))})
+    best = min(rows, key=lambda r: r["ms"])
+    _emit({"config": 4, "what": "ORDER BY over 1-, 2-round and seeded PlanEnumRRS plans, end to end",
+           "rows": n, "plans": len(rows), "all_ok": all(r["ok"] for r in rows), "best": best,
+           "median_ms": float(np.median([r["ms"] for r in rows])),
+           "best_Mrows_s": round(n / best["ms"] / 1e3, 1), "plans_run": rows})
+    srt = [s[exp] for s in stacked]
+    exact = 1 + int(np.count_nonzero(np.any(np.stack([np.diff(s) != 0 for s in srt]), axis=0)))
+    res = {}
+    for ht in (1, 0):
+        box = {}
+
+        def run():
+            box["r"] = ctx.multi_round_hash(dcols, hash_type=ht)
+
+        ms, kern = _timed(ctx, run)
+        res[f"hash_type_{ht}"] = {"ms": round(ms, 3), "ngroups": box["r"][2], "exact": box["r"][2] == exact,
+                                  "kernels": kern}
+    _emit({"config": 4, "what": "hash GROUP BY (flag, status, date, key)", "rows": n, "exact_groups": exact, **res})
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--configs", default="1,2,4")
+    args = ap.parse_args(argv)
+    from .engine import Context
+    ctx = Context(0)
+    for c in [int(x) for x in args.configs.split(",") if x]:
+        t0 = time.time()
+        {1: config1, 2: config2, 4: config4}[c](ctx)
+        print(f"config {c} done in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+
+
+if __name__ == "__main__":
+    main()

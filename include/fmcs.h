@@ -64,11 +64,12 @@ typedef struct {
   uint32_t ngroups_gtOne;
 } fmcs_group_info;
 
-/* Tunables of the batched segmented sort. */
+/* Tunables of the batched segmented sort and of the oid gather. */
 typedef struct {
-  uint32_t small_max; /* segments up to this many rows go to packed LDS tiles */
-  uint32_t window;    /* rows per packing window (<= small_max + 1) */
-  uint32_t msd_bits;  /* digit width of one MSD level (1..8) */
+  uint32_t small_max;        /* segments up to this many rows go to packed LDS tiles */
+  uint32_t window;           /* rows per packing window (<= small_max + 1) */
+  uint32_t msd_bits;         /* digit width of one MSD level (1..8) */
+  uint32_t gather_window_kb; /* 0: one gather pass; else passes over source windows of this many KiB */
 } fmcs_tuning;
 
 /* ---- context -------------------------------------------------------- */
@@ -124,6 +125,13 @@ int fmcs_extract_group_info(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void
  * (keys_d[i], oids_d[i]) by the low keybits bits of the key, in place.  Ties
  * keep their incoming order (LSD radix sort, stable). */
 int fmcs_sort_pairs(fmcs_ctx* ctx, uint32_t bank, uint32_t keybits, uint64_t n, void* keys_d, uint32_t* oids_d);
+
+/* The same sort by comparisons, the structure of avxmergesort_anytype
+ * (src/mergesort.h:2878-3096: sorting networks over blocks, then a 2-way
+ * merge tree): bitonic networks over 2048-row LDS tiles, then merge-path
+ * passes.  Pairs are ordered by (key, oid), i.e. the stable order; the
+ * result equals fmcs_sort_pairs' for keys within keybits. */
+int fmcs_sort_pairs_merge(fmcs_ctx* ctx, uint32_t bank, uint64_t n, void* keys_d, uint32_t* oids_d);
 
 /* The per-group loop of Composer::sortByGroup (src/composer.cpp:1280-1335):
  * sorts (keys, oids) within each segment [off[s], off[s+1]), in place, by

@@ -133,6 +133,22 @@ def test_sort_pairs(ctx, dt, bits, n):
     assert np.array_equal(host(kd), k[exp])
 
 
+@pytest.mark.parametrize("dt,bits,n", SORT_CASES + [(np.uint64, 62, 3 * 2048 + 5), (np.uint32, 31, 2048 * 9)])
+def test_sort_pairs_merge(ctx, dt, bits, n):
+    # bitonic tiles + merge-path passes: odd run counts, a partial last
+    # tile, heavy ties (ordered by oid), incoming oids not ascending
+    log("sort_pairs_merge", dt, bits, n)
+    k = datagen.uniform_bits(22, n, bits).astype(dt)
+    if n > 10:
+        k[: n // 3] = k[0]
+    oids = datagen.permutation(23, n).astype(np.uint32)
+    kd, od = dev(k), dev(oids)
+    ctx.sort_pairs_merge(kd, od)
+    exp = np.lexsort((oids, k))
+    assert np.array_equal(host(od), oids[exp])
+    assert np.array_equal(host(kd), k[exp])
+
+
 def test_sort_pairs_matches_reference_mergesort(ctx):
     if ob.ref() is None:
         pytest.skip("reference library not built here")
