@@ -26,7 +26,7 @@ EXPORTS = [
     "fmcs_ctx_create", "fmcs_ctx_destroy", "fmcs_ctx_set_stream", "fmcs_ctx_sync",
     "fmcs_ctx_set_tuning", "fmcs_ctx_get_tuning", "fmcs_ctx_trim", "fmcs_last_error",
     "fmcs_bank_bytes", "fmcs_construct_entries", "fmcs_stitch", "fmcs_gather",
-    "fmcs_extract_group_info", "fmcs_sort_pairs", "fmcs_sort_pairs_merge", "fmcs_segmented_sort",
+    "fmcs_extract_group_info", "fmcs_sort_pairs", "fmcs_sort_pairs_merge", "fmcs_fill_column", "fmcs_segmented_sort",
     "fmcs_sort_by_group",
     "fmcs_multi_round_sort", "fmcs_last_breakdown", "fmcs_hash_round", "fmcs_multi_round_hash",
     "fmcs_ctx_set_profiling", "fmcs_kernel_stats", "fmcs_kernel_stats_reset",
@@ -75,6 +75,7 @@ def load():
         "fmcs_ctx_sync": (i32, [vp]),
         "fmcs_ctx_set_tuning": (i32, [vp, P(Tuning)]),
         "fmcs_sort_pairs_merge": (i32, [vp, u32, u64, vp, vp]),
+        "fmcs_fill_column": (i32, [vp, u32, u64, u64, u64, i32, u32, vp]),
         "fmcs_ctx_get_tuning": (i32, [vp, P(Tuning)]),
         "fmcs_ctx_trim": (i32, [vp]),
         "fmcs_last_error": (C.c_char_p, [vp]),

@@ -4,6 +4,7 @@
 // the context's stream; temporary device memory comes from a per-context
 // workspace that grows on demand and is reused across calls.
 #include <algorithm>
+#include <cmath>
 #include <map>
 #include <tuple>
 #include <utility>
@@ -517,12 +518,9 @@ int seg_sort_batched(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, cons
   for (;;) {
     u32 sm = small_max;
     u32 tkb = kb;
-    if (kb + TS_SEGBITS > 64) {
-      // too many key bits to pack a segment index: only single rows are
-      // packed (their key bits need no comparing), the rest go to MSD.
-      sm = 1;
-      tkb = 0;
-    }
+    // keys too wide to pack a segment index beside them: refused tiles go
+    // to the comparison-rank fallback instead of the radix tile sort
+    const bool packable = kb + TS_SEGBITS <= 64;
     const u32 Wl = std::min<u32>(W, sm + 1);
     const u32 wpt = (ONESWEEP_TILE + Wl - 1) / Wl;
     const u64 nw = plo ? (u64)pntiles * wpt : (n + Wl - 1) / Wl;
@@ -567,7 +565,16 @@ int seg_sort_batched(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, cons
       TRY(fill(ctx, fb, nw, 0));
       TRY(tile_bucket_launch<K>(ctx, a, fb, nwin, (double)tile_rows_est, T));
       a.only = fb;
-      TRY(tile_sort_launch<K>(ctx, a, nwin, 0.0, T));
+      if (packable) {
+        TRY(tile_sort_launch<K>(ctx, a, nwin, 0.0, T));
+      } else if (nwin) {
+        Prof pf(ctx, "k_tile_rank", 0.0);
+        if (T <= 512)
+          hipLaunchKernelGGL((k_tile_rank<K, 2>), dim3(nwin), dim3(256), 0, ctx->stream, a);
+        else
+          hipLaunchKernelGGL((k_tile_rank<K, 4>), dim3(nwin), dim3(256), 0, ctx->stream, a);
+        TRY(launch_check(ctx, "k_tile_rank"));
+      }
     }
     if (nlarge == 0) break;
     u32 *llo, *lhi, *ltiles, *tbase;
@@ -855,7 +862,7 @@ int boundary_dd(fmcs_ctx* ctx, const K* v, const u32* in_group, u64 n, u32* out_
 
 struct DdLevel {
   u32 kb, sm, tkb, Wl, wpt, D, shift;
-  bool last;
+  bool last, packable;
   u64 nseg_max, nw_max, nlarge_max, ntiles_max;
 };
 
@@ -880,10 +887,7 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
       d.kb = kb;
       d.sm = small_max;
       d.tkb = kb;
-      if (kb + TS_SEGBITS > 64) {
-        d.sm = 1;
-        d.tkb = 0;
-      }
+      d.packable = kb + TS_SEGBITS <= 64;
       d.Wl = std::min<u32>(W, d.sm + 1);
       d.wpt = (ONESWEEP_TILE + d.Wl - 1) / d.Wl;
       d.nseg_max = cur_max;
@@ -976,7 +980,9 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
       TileSortArgs<K> b = a;
       b.wcount = nullptr;
       const u64 g = persist_grid(ctx, fn_tsort, d.nw_max);
-      if (T <= 512)
+      if (!d.packable)
+        TRY(klaunch(ctx, "k_tile_rank", T <= 512 ? k_tile_rank<K, 2> : k_tile_rank<K, 4>, g, 0.0, nullptr, 0.0, b));
+      else if (T <= 512)
         TRY(klaunch(ctx, "k_tile_sort", k_tile_sort<K, 2>, g, 0.0, nullptr, 0.0, b));
       else
         TRY(klaunch(ctx, "k_tile_sort", k_tile_sort<K, 4>, g, 0.0, nullptr, 0.0, b));
@@ -1308,7 +1314,8 @@ int fmcs_ctx_sync(fmcs_ctx* ctx) {
 
 int fmcs_ctx_set_tuning(fmcs_ctx* ctx, const fmcs_tuning* t) {
   if (!ctx || !t) return FMCS_E_ARG;
-  if (t->small_max < 1 || t->small_max > (u32)TILESORT_T / 2 || t->window < 1 || t->window > t->small_max + 1 ||
+  // small_max >= 64 bounds the device-driven path's segment lists by 256 / 65 entries per row
+  if (t->small_max < 64 || t->small_max > (u32)TILESORT_T / 2 || t->window < 1 || t->window > t->small_max + 1 ||
       t->msd_bits < 1 || t->msd_bits > (u32)RADIX_BITS || t->gather_window_kb > (1u << 24))
     return fail(ctx, FMCS_E_ARG, "tuning out of range");
   ctx->tune = *t;
@@ -1461,6 +1468,24 @@ int fmcs_sort_pairs(fmcs_ctx* ctx, uint32_t bank, uint32_t keybits, uint64_t n, 
     case 2: return radix_sort<u16>(ctx, (u16*)keys_d, oids_d, (u16*)keys_d, oids_d, n, keybits);
     case 4: return radix_sort<u32>(ctx, (u32*)keys_d, oids_d, (u32*)keys_d, oids_d, n, keybits);
     case 8: return radix_sort<u64>(ctx, (u64*)keys_d, oids_d, (u64*)keys_d, oids_d, n, keybits);
+    default: return fail(ctx, FMCS_E_ARG, "bank must be 2, 4 or 8");
+  }
+}
+
+int fmcs_fill_column(fmcs_ctx* ctx, uint32_t bank, uint64_t n, uint64_t seed, uint64_t card, int zipf,
+                     uint32_t width, void* out_d) {
+  if (!ctx || !out_d) return FMCS_E_ARG;
+  if (width == 0 || width > 8 * bank || card == 0) return fail(ctx, FMCS_E_ARG, "fill_column: bad width or card");
+  if (n == 0) return FMCS_OK;
+  const u64 mask = width >= 64 ? ~0ull : ((1ull << width) - 1);
+  const u64 span = width >= 64 ? ~0ull : (1ull << width);
+  const u64 step = std::max<u64>(1, span / card);
+  const double lc = std::log2((double)card);
+  const u64 grid = cap_grid((n + 255) / 256, 8192);
+  switch (bank) {
+    case 2: return klaunch(ctx, "k_fill_column", k_fill_column<u16>, grid, 0.0, nullptr, 0.0, (u16*)out_d, n, seed, card, (u32)(zipf != 0), lc, step, mask);
+    case 4: return klaunch(ctx, "k_fill_column", k_fill_column<u32>, grid, 0.0, nullptr, 0.0, (u32*)out_d, n, seed, card, (u32)(zipf != 0), lc, step, mask);
+    case 8: return klaunch(ctx, "k_fill_column", k_fill_column<u64>, grid, 0.0, nullptr, 0.0, (u64*)out_d, n, seed, card, (u32)(zipf != 0), lc, step, mask);
     default: return fail(ctx, FMCS_E_ARG, "bank must be 2, 4 or 8");
   }
 }

@@ -77,6 +77,8 @@ template <class K, int E>
 __global__ void k_tile_sort(TileSortArgs<K>);
 template <class K, int E>
 __global__ void k_tile_bucket(TileSortArgs<K>, u32*);
+template <class K, int E>
+__global__ void k_tile_rank(TileSortArgs<K>);
 __global__ void k_windows(const u32*, const u32*, u32, u32, u32, const u32*, const u32*, u32, u32*, u32*, u32*, u32*,
                           u32*, u32*);
 __global__ void k_offsets_to_list(const u32*, u32, u32*, u32*);
@@ -99,6 +101,8 @@ __global__ void k_hash_rank_slots(const u32*, const u32*, const u32*, u64, u32*)
 __global__ void k_hash_assign(const u32*, const u32*, u64, u32*);
 template <class K>
 __global__ void k_fill_random(K*, u64, u64, u32);
+template <class T>
+__global__ void k_fill_column(T*, u64, u64, u64, u32, double, u64, u64);
 __global__ void k_uniform_offsets(u32*, u32, u32, u64);
 
 

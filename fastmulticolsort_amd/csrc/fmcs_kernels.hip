@@ -889,6 +889,86 @@ __device__ __forceinline__ void tile_bucket_one(const TileSortArgs<K>& a, u32 w,
   }
 }
 
+// Fallback of the bucket tile sort for keys too wide to pack a segment index
+// beside them (kb + 11 > 64 bits): every row's place inside its segment is
+// the number of rows of the segment ordered before it by (key, oid), found
+// by comparing against the whole segment (segments hold at most small_max
+// rows; only tiles the bucket sort refused come here).
+template <class K, int E>
+__device__ __forceinline__ void tile_rank_one(const TileSortArgs<K>& a, u32 w) {
+  constexpr int NT = 256, T = NT * E;
+  __shared__ K s_k[T];
+  __shared__ u32 s_o[T];
+  __shared__ u16 s_sid[T];
+  __shared__ u16 s_start[T + 1];
+  __shared__ uint8_t s_f[T];
+  __shared__ u32 s_tmp[NT / WAVE + 1];
+  const u32 tid = threadIdx.x;
+  if (a.only && a.only[w] == 0) return;
+  const u32 lo = a.tile_lo[w], hi = a.tile_hi[w];
+  if (lo >= hi) return;
+  const u32 s0 = a.tile_s0[w], s1 = a.tile_s1[w];
+  const u32 cnt = hi - lo;
+  const K* kin = a.kin[a.src_sel];
+  const u32* vin = a.vin[a.src_sel];
+  for (u32 i = tid; i < (u32)T; i += NT) s_f[i] = 0;
+  __syncthreads();
+  for (u32 i = tid; i < cnt; i += NT) {
+    s_k[i] = kin[lo + i];
+    s_o[i] = vin[lo + i];
+  }
+  for (u32 s = s0 + tid; s <= s1; s += NT) {
+    const u32 st = a.seg_start[s];
+    if (st >= lo && st < hi) s_f[st - lo] = 1;
+  }
+  if (tid == 0) s_f[0] = 1;
+  __syncthreads();
+  u32 f[E];
+  u32 sum = 0;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    f[e] = s_f[tid * E + e];
+    sum += f[e];
+  }
+  u32 nsegs;
+  u32 run = block_excl_scan<NT>(sum, s_tmp, &nsegs);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    run += f[e];
+    const u32 p = tid * E + e;
+    s_sid[p] = (u16)(run - 1);
+    if (f[e] && p < cnt) s_start[run - 1] = (u16)p;
+  }
+  if (tid == 0) s_start[nsegs] = (u16)cnt;
+  __syncthreads();
+  for (u32 p = tid; p < cnt; p += NT) {
+    const u32 sg = s_sid[p];
+    const u32 b0 = s_start[sg], b1 = s_start[sg + 1];
+    const K kp = s_k[p];
+    const u32 op = s_o[p];
+    u32 r = b0;
+    for (u32 q = b0; q < b1; ++q) {
+      const K kq = s_k[q];
+      r += (kq < kp) || (kq == kp && s_o[q] < op);
+    }
+    if (a.kout) a.kout[lo + r] = kp;
+    a.vout[lo + r] = op;
+  }
+}
+
+template <class K, int E>
+__global__ __launch_bounds__(256) void k_tile_rank(TileSortArgs<K> a) {
+  if (a.fbcount) {
+    const u32 cnt = *a.fbcount;
+    for (u32 b = blockIdx.x; b < cnt; b += gridDim.x) {
+      tile_rank_one<K, E>(a, a.fblist[b]);
+      __syncthreads();
+    }
+  } else {
+    tile_rank_one<K, E>(a, a.wlist ? a.wlist[blockIdx.x] : blockIdx.x);
+  }
+}
+
 // One window per block, or (a.wcount set) a persistent loop over the
 // compact window list, which lists windows roughly in row order: die x
 // (blocks b with b % XCDS == x) takes the x-th contiguous part, so the
@@ -1185,6 +1265,32 @@ __global__ void k_fill_random(K* __restrict__ p, u64 n, u64 seed, u32 bits) {
   }
 }
 
+// Synthetic column (the seeded generator of the benchmarks; the reference
+// writes its inputs with src/generator.h:139-227 and src/zipf.h:33-69):
+// row i draws u = SplitMix64(seed, i); its rank is u mod card (uniform) or
+// floor(card^x) - 1 with x = u / 2^64 (density ~ 1/(rank+1), Zipf s = 1);
+// the value is (rank * step) & mask.
+template <class T>
+__global__ void k_fill_column(T* __restrict__ p, u64 n, u64 seed, u64 card, u32 zipf, double log2card, u64 step,
+                              u64 mask) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) {
+    u64 z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    u64 rank;
+    if (zipf) {
+      const double x = (double)(z >> 11) * 0x1.0p-53;
+      const double v = floor(exp2(x * log2card));
+      rank = v >= 1.0 ? (u64)v - 1 : 0;
+      if (rank >= card) rank = card - 1;
+    } else {
+      rank = z % card;
+    }
+    p[i] = (T)((rank * step) & mask);
+  }
+}
+
 __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64 n) {
   for (u64 s = (u64)blockIdx.x * blockDim.x + threadIdx.x; s <= nseg; s += (u64)gridDim.x * blockDim.x)
     off[s] = (u32)min((u64)s * size, n);
@@ -1203,6 +1309,8 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
                                              u32, const u32*, const u32*, const u32*, u32, u32, const u32*, const u32*);                                \
   template __global__ void k_tile_sort<K, 4>(TileSortArgs<K>);                                                      \
   template __global__ void k_tile_sort<K, 2>(TileSortArgs<K>);                                                      \
+  template __global__ void k_tile_rank<K, 4>(TileSortArgs<K>);                                                      \
+  template __global__ void k_tile_rank<K, 2>(TileSortArgs<K>);                                                      \
   template __global__ void k_tile_bucket<K, 4>(TileSortArgs<K>, u32*);                                             \
   template __global__ void k_tile_bucket<K, 2>(TileSortArgs<K>, u32*);                                                      \
   template __global__ void k_copy_segs<K>(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, u32,  \
@@ -1210,7 +1318,8 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
   template __global__ void k_composite<K>(const K*, const u32*, u64, u32, u64*);                                    \
   template __global__ void k_strip_composite<K>(const u64*, u64, u64, K*);                                          \
   template __global__ void k_hash_insert<K>(const K*, const u32*, u64, u32*, u32, u32*, u32*);                    \
-  template __global__ void k_fill_random<K>(K*, u64, u64, u32);
+  template __global__ void k_fill_random<K>(K*, u64, u64, u32);                                                    \
+  template __global__ void k_fill_column<K>(K*, u64, u64, u64, u32, double, u64, u64);
 
 FMCS_INST_K(u16)
 FMCS_INST_K(u32)

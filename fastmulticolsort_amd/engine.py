@@ -130,6 +130,12 @@ class Context:
     def sort_pairs(self, keys, oids, keybits):
         self._chk(self.lib.fmcs_sort_pairs(self.h, bank_of(keys), keybits, keys.numel(), _p(keys), _p(oids)))
 
+    def fill_column(self, out, seed, card, zipf, width):
+        """Seeded synthetic column into the device tensor `out` (see fmcs_fill_column)."""
+        self._chk(self.lib.fmcs_fill_column(self.h, bank_of(out), out.numel(), seed, card, int(bool(zipf)), width,
+                                             _p(out)))
+        return out
+
     def sort_pairs_merge(self, keys, oids):
         """Bitonic LDS tiles + merge-path passes; the same result as sort_pairs."""
         self._chk(self.lib.fmcs_sort_pairs_merge(self.h, bank_of(keys), keys.numel(), _p(keys), _p(oids)))

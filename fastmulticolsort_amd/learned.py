@@ -1,10 +1,15 @@
 """Learned plan picker: an MLP cost model trained on GPU-measured plan times.
 
-Per (request, plan) the features are the row count, the plan's rounds (width,
-bank, and the estimated group size before the round from the base
-cardinalities, as the analytic model computes it), padded to MAX_ROUNDS.
-The network predicts log(ms); the planner runs the candidate plans'
-features through the MLP forward kernel and takes the argmin.  Training
+Per (request, plan) the features are the row count, the analytic cost
+model's log-estimate of the plan relative to the mean over the request's
+candidates, and the plan's rounds (width, bank, and the estimated group size
+before the round from the base cardinalities, as the analytic model computes
+it), padded to MAX_ROUNDS.  The network predicts log(ms) relative to the
+mean over the request's candidates (only the ranking inside a request
+matters); the planner runs the candidate plans' features through the MLP
+forward kernel and takes the argmin.  With the analytic estimate as an
+input the network learns a correction of the analytic model from measured
+times.  Training
 uses the project's own forward / backward / Adam kernels (include/fmcs_plan.h).
 The analytic model plus exhaustive measurement over the candidates stays the
 fallback and the judge of the learned pick's regret; the chosen plan never
@@ -20,7 +25,7 @@ import numpy as np
 from . import _lib, planner
 
 MAX_ROUNDS = 6
-NFEAT = 3 + 5 * MAX_ROUNDS
+NFEAT = 4 + 5 * MAX_ROUNDS
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
@@ -82,14 +87,17 @@ def round_groups(n, bws, cards, col_order, split):
     return G
 
 
-def features(n, bws, cards, col_order, split):
+def features(n, bws, cards, col_order, split, rel_est=0.0):
+    """rel_est: the analytic log-estimate of this plan minus the mean over
+    the request's candidates."""
     f = np.zeros(NFEAT, np.float32)
     f[0] = math.log2(n) / 24.0
     f[1] = len(split) / MAX_ROUNDS
     f[2] = sum(bws) / 240.0
+    f[3] = max(-4.0, min(4.0, rel_est)) / 4.0
     G = round_groups(n, bws, cards, col_order, split)
     for r, w in enumerate(split[:MAX_ROUNDS]):
-        b = 3 + 5 * r
+        b = 4 + 5 * r
         f[b] = w / 64.0
         f[b + 1 + bank_type(w)] = 1.0
         f[b + 4] = math.log2(max(1.0, n / max(1.0, G[r]))) / 24.0
@@ -200,10 +208,24 @@ def candidate_plans(model, n, bws, cards, col_order, seed=0, max_plans=48):
     return out
 
 
-def pick(mlp, n, bws, cards, col_order, plans):
+def plan_features(model, n, bws, cards, col_order, plans):
+    """Feature rows of a request's candidate plans and the analytic estimates."""
+    est = np.array([planner.estimate_plan(model, n, bws, cards, col_order, p)[0] for p in plans])
+    le = np.log(np.maximum(est, 1e-9))
+    rel = le - le.mean()
+    return np.stack([features(n, bws, cards, col_order, p, r) for p, r in zip(plans, rel)]), est
+
+
+def relative_log(ms):
+    """Training target: log time relative to the mean over the request's plans."""
+    lt = np.log(np.maximum(np.asarray(ms, np.float64), 1e-3))
+    return (lt - lt.mean()).astype(np.float32)
+
+
+def pick(mlp, model, n, bws, cards, col_order, plans):
     import torch
-    X = torch.from_numpy(np.stack([features(n, bws, cards, col_order, p) for p in plans])).cuda()
-    y = mlp.forward(X).cpu().numpy()
+    X, _ = plan_features(model, n, bws, cards, col_order, plans)
+    y = mlp.forward(torch.from_numpy(X).cuda()).cpu().numpy()
     return int(np.argmin(y)), y
 
 
@@ -267,11 +289,11 @@ def collect(ctx, nreq, seed=5, max_log2=22, max_plans=32, log=None):
         order = list(range(len(req["bws"])))
         plans = candidate_plans(model, req["n"], req["bws"], req["cards"], order, seed=i, max_plans=max_plans)
         ms = measure_plans(ctx, req, plans)
-        for p, t in zip(plans, ms):
-            X.append(features(req["n"], req["bws"], req["cards"], order, p))
-            Y.append(math.log(max(t, 1e-3)))
-            R.append(i)
-            P.append(",".join(map(str, p)))
+        feats, _ = plan_features(model, req["n"], req["bws"], req["cards"], order, plans)
+        X.extend(list(feats))
+        Y.extend(list(relative_log(ms)))
+        R.extend([i] * len(plans))
+        P.extend(",".join(map(str, p)) for p in plans)
         reqs.append(req)
         if log:
             log(f"request {i}: n={req['n']} bws={req['bws']} plans={len(plans)} best={min(ms):.3f} ms")
@@ -281,7 +303,7 @@ def collect(ctx, nreq, seed=5, max_log2=22, max_plans=32, log=None):
 
 def regret_table(pred, y, req):
     """Per request: measured time of the picked plan over the best measured
-    plan, minus one."""
+    plan, minus one (y: log times, absolute or relative within a request)."""
     out = []
     for r in np.unique(req):
         m = req == r

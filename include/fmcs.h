@@ -66,7 +66,7 @@ typedef struct {
 
 /* Tunables of the batched segmented sort and of the oid gather. */
 typedef struct {
-  uint32_t small_max;        /* segments up to this many rows go to packed LDS tiles */
+  uint32_t small_max;        /* segments up to this many rows go to packed LDS tiles (64..512) */
   uint32_t window;           /* rows per packing window (<= small_max + 1) */
   uint32_t msd_bits;         /* digit width of one MSD level (1..8) */
   uint32_t gather_window_kb; /* 0: one gather pass; else passes over source windows of this many KiB */
@@ -132,6 +132,14 @@ int fmcs_sort_pairs(fmcs_ctx* ctx, uint32_t bank, uint32_t keybits, uint64_t n, 
  * passes.  Pairs are ordered by (key, oid), i.e. the stable order; the
  * result equals fmcs_sort_pairs' for keys within keybits. */
 int fmcs_sort_pairs_merge(fmcs_ctx* ctx, uint32_t bank, uint64_t n, void* keys_d, uint32_t* oids_d);
+
+/* Seeded synthetic column on the device, in place of the reference's
+ * generator (src/generator.h:139-227, src/zipf.h:33-69; time-seeded there):
+ * row i draws u = SplitMix64(seed, i); rank = u mod card, or with zipf
+ * floor(card^(u/2^64)) - 1 (density ~ 1/(rank+1)); value = (rank * step) &
+ * (2^width - 1), step = max(1, 2^width / card). */
+int fmcs_fill_column(fmcs_ctx* ctx, uint32_t bank, uint64_t n, uint64_t seed, uint64_t card, int zipf,
+                     uint32_t width, void* out_d);
 
 /* The per-group loop of Composer::sortByGroup (src/composer.cpp:1280-1335):
  * sorts (keys, oids) within each segment [off[s], off[s+1]), in place, by

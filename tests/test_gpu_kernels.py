@@ -253,3 +253,29 @@ def test_multi_round_matches_reference(ctx):
     order, _ = ctx.multi_round_sort([dev(c1), dev(c2)], bws)
     keys = np.stack([c1[ro], c2[ro]])
     assert np.array_equal(host(order), ob.canonical_order(keys, ro))
+
+
+# ---- synthetic column generator ----------------------------------------------
+
+@pytest.mark.parametrize("dt,w,card", [(np.uint16, 12, 2526), (np.uint32, 27, 1 << 20), (np.uint64, 40, 3),
+                                       (np.uint64, 40, 1 << 40)])
+def test_fill_column(ctx, dt, w, card):
+    n = 100003
+    out = torch.empty(n, dtype={np.uint16: torch.uint16, np.uint32: torch.uint32, np.uint64: torch.uint64}[dt],
+                      device="cuda")
+    u = datagen.splitmix64(77, n)
+    step = max(1, (1 << w) // card)
+    mask = np.uint64((1 << w) - 1)
+    # uniform: exact
+    ctx.fill_column(out, 77, card, False, w)
+    exp = ((u % np.uint64(card)) * np.uint64(step)) & mask
+    assert np.array_equal(host(out).astype(np.uint64), exp)
+    # Zipf(1): floor(card^x) - 1 in double precision (a last-ulp difference of
+    # exp2 may move a row across a rank edge: allow a handful)
+    ctx.fill_column(out, 77, card, True, w)
+    x = (u >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    rank = np.minimum(np.maximum(np.floor(np.exp2(x * np.log2(float(card)))), 1).astype(np.uint64) - 1,
+                      np.uint64(card - 1))
+    got = host(out).astype(np.uint64)
+    assert (got != ((rank * np.uint64(step)) & mask)).sum() <= 5
+    assert got.max() <= int(mask)
