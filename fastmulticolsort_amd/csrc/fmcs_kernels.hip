@@ -712,8 +712,35 @@ __global__ __launch_bounds__(256) void k_tile_sort(TileSortArgs<K> a) {
 // ---------------------------------------------------------------------------
 constexpr u32 BUCKET_LIMIT = 24;
 
+// A window's range and its rows in registers (row p = e * 256 + tid).
 template <class K, int E>
-__device__ __forceinline__ void tile_bucket_one(const TileSortArgs<K>& a, u32 w, u32* __restrict__ fallback) {
+struct WinRows {
+  u32 w, lo, hi;
+  K kk[E];
+  u32 oo[E];
+};
+
+template <class K, int E>
+__device__ __forceinline__ void load_window(const TileSortArgs<K>& a, u32 w, WinRows<K, E>& r) {
+  r.w = w;
+  r.lo = a.tile_lo[w];
+  r.hi = a.tile_hi[w];
+  const u32 cnt = r.hi > r.lo ? r.hi - r.lo : 0u;
+  const K* kin = a.kin[a.src_sel];
+  const u32* vin = a.vin[a.src_sel];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const u32 p = e * 256 + threadIdx.x;
+    if (p < cnt) {
+      r.kk[e] = kin[r.lo + p];
+      r.oo[e] = vin[r.lo + p];
+    }
+  }
+}
+
+template <class K, int E>
+__device__ __forceinline__ void tile_bucket_body(const TileSortArgs<K>& a, const WinRows<K, E>& r,
+                                                 u32* __restrict__ fallback) {
   constexpr int NT = 256, T = NT * E, NB = 2 * T, BPT = NB / NT, SE = T / NT;
   __shared__ K s_k[T];          // rows in bin order, then in final order
   __shared__ u32 s_o[T];
@@ -726,25 +753,14 @@ __device__ __forceinline__ void tile_bucket_one(const TileSortArgs<K>& a, u32 w,
   __shared__ u32 s_tmp[NT / WAVE + 1];
   __shared__ u32 s_over;
   const u32 tid = threadIdx.x;
-  const u32 lo = a.tile_lo[w], hi = a.tile_hi[w];
+  const u32 w = r.w, lo = r.lo, hi = r.hi;
   if (lo >= hi) return;
   const u32 s0 = a.tile_s0[w], s1 = a.tile_s1[w];
   const u32 cnt = hi - lo;
   const u32 kb = a.kb;
   const u64 kmask = kb >= 64 ? ~0ull : ((1ull << kb) - 1);
-  const K* kin = a.kin[a.src_sel];
-  const u32* vin = a.vin[a.src_sel];
-  // rows straight into registers, row p = e * NT + tid (coalesced)
-  K kk[E];
-  u32 oo[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const u32 p = e * NT + tid;
-    if (p < cnt) {
-      kk[e] = kin[lo + p];
-      oo[e] = vin[lo + p];
-    }
-  }
+  const K (&kk)[E] = r.kk;
+  const u32 (&oo)[E] = r.oo;
   for (u32 i = tid; i < (u32)T; i += NT) s_f[i] = 0;
   for (u32 i = tid; i < (u32)NB; i += NT) s_cnt[i] = 0;
   if (tid == 0) s_over = 0;
@@ -977,15 +993,33 @@ __global__ __launch_bounds__(256) void k_tile_rank(TileSortArgs<K> a) {
 template <class K, int E>
 __global__ __launch_bounds__(256) void k_tile_bucket(TileSortArgs<K> a, u32* __restrict__ fallback) {
   if (a.wcount) {
+    // Two register sets in turn: the next window's rows are loaded before
+    // this window is sorted, so their memory latency overlaps the LDS work
+    // (a barrier waits for LDS operations only, not for these loads).
     const u32 cnt = *a.wcount;
     const u32 xcd = blockIdx.x % XCDS, per_x = gridDim.x / XCDS;
     const u32 hi = (u32)((u64)cnt * (xcd + 1) / XCDS);
-    for (u32 b = (u32)((u64)cnt * xcd / XCDS) + blockIdx.x / XCDS; b < hi; b += per_x) {
-      tile_bucket_one<K, E>(a, a.wlist[b], fallback);
+    u32 b = (u32)((u64)cnt * xcd / XCDS) + blockIdx.x / XCDS;
+    if (b >= hi) return;
+    WinRows<K, E> ra, rb;
+    load_window<K, E>(a, a.wlist[b], ra);
+    for (;;) {
+      const u32 b1 = b + per_x;
+      if (b1 < hi) load_window<K, E>(a, a.wlist[b1], rb);
+      tile_bucket_body<K, E>(a, ra, fallback);
       __syncthreads();
+      if (b1 >= hi) break;
+      const u32 b2 = b1 + per_x;
+      if (b2 < hi) load_window<K, E>(a, a.wlist[b2], ra);
+      tile_bucket_body<K, E>(a, rb, fallback);
+      __syncthreads();
+      if (b2 >= hi) break;
+      b = b2;
     }
   } else {
-    tile_bucket_one<K, E>(a, a.wlist ? a.wlist[blockIdx.x] : blockIdx.x, fallback);
+    WinRows<K, E> r;
+    load_window<K, E>(a, a.wlist ? a.wlist[blockIdx.x] : blockIdx.x, r);
+    tile_bucket_body<K, E>(a, r, fallback);
   }
 }
 
