@@ -276,19 +276,20 @@ int klaunch(fmcs_ctx* ctx, const char* name, void (*kern)(KA...), u64 grid, doub
 
 // Segmented tile-offset scan of the per-tile digit counts (written by
 // kd_seg_hist into tcount over the same tiling) and the scatter pass.
-template <class K>
+template <class K, int RB = RADIX_BITS>
 int scatter_pass(fmcs_ctx* ctx, const char* name, const K* kin, const u32* vin, K* kout, u32* vout, const u32* lo,
                  const u32* hi, const u32* tbase, const u32* nsegs_d, const u32* ntiles_d, u64 ntiles_max,
                  const u32* dbase, u32* tcount, u32 shift, u32 dbits, double bytes, const u32* dcount, double per) {
+  constexpr u32 R = 1u << RB;
   const u64 nch = std::max<u64>(1, (ntiles_max + TSCAN_CT - 1) / TSCAN_CT);
   u32* carry;
-  TRY(getbuf_t(ctx, WS_CARRY, nch * (RADIX + 1), &carry));
-  u32* cflag = carry + nch * RADIX;
-  TRY(klaunch(ctx, "k_tile_scan", kd_tscan_local, cap_grid(nch, 2048), 0.0, nullptr, 0.0, tcount, tbase, nsegs_d,
+  TRY(getbuf_t(ctx, WS_CARRY, nch * (R + 1), &carry));
+  u32* cflag = carry + nch * R;
+  TRY(klaunch(ctx, "k_tile_scan", kd_tscan_local<RB>, cap_grid(nch, 2048), 0.0, nullptr, 0.0, tcount, tbase, nsegs_d,
               ntiles_d, carry, cflag));
-  TRY(klaunch(ctx, "k_tile_scan", kd_tscan_carry, RADIX, 0.0, nullptr, 0.0, carry, (const u32*)cflag, ntiles_d));
-  return klaunch(ctx, name, k_onesweep<K, ONESWEEP_ITEMS>,
-                 onesweep_grid(ctx, (const void*)k_onesweep<K, ONESWEEP_ITEMS>, ntiles_max), bytes, dcount, per, kin, vin,
+  TRY(klaunch(ctx, "k_tile_scan", kd_tscan_carry<RB>, R, 0.0, nullptr, 0.0, carry, (const u32*)cflag, ntiles_d));
+  return klaunch(ctx, name, k_onesweep<K, ONESWEEP_ITEMS, RB>,
+                 onesweep_grid(ctx, (const void*)k_onesweep<K, ONESWEEP_ITEMS, RB>, ntiles_max), bytes, dcount, per, kin, vin,
                  kout, vout, lo, hi, tbase, 0u, dbase, (const u32*)tcount, (const u32*)carry, shift, dbits, nsegs_d,
                  ntiles_d);
 }
@@ -383,9 +384,9 @@ int radix_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, 
       dk = (i & 1) ? t1k : t0k;
       dv = (i & 1) ? t1v : t0v;
     }
-    TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<K, ONESWEEP_ITEMS>, cap_grid(ntiles, 2048), (double)n * sizeof(K),
-                nullptr, 0.0, sk, cst, cst + 1, cst + 2, cst + 4, cst + 3, p * RADIX_BITS, (u32)RADIX_BITS,
-                (u32*)nullptr, tcount));
+    TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<K, ONESWEEP_ITEMS, RADIX_BITS>, cap_grid(ntiles, 2048),
+                (double)n * sizeof(K), nullptr, 0.0, sk, cst, cst + 1, cst + 2, cst + 4, cst + 3, p * RADIX_BITS,
+                (u32)RADIX_BITS, (u32*)nullptr, tcount, (u64)0, (u32*)nullptr));
     TRY(scatter_pass<K>(ctx, "k_onesweep", sk, sv, dk, dv, cst, cst + 1, cst + 2, cst + 4, cst + 3, ntiles,
                         dbase + (u64)p * RADIX, tcount, p * RADIX_BITS, (u32)RADIX_BITS, 2.0 * n * (sizeof(K) + 4),
                         nullptr, 0.0));
@@ -610,9 +611,9 @@ int seg_sort_batched(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, cons
     TRY(fill(ctx, hist, (u64)nlarge * RADIX, 0));
     const u32* nlarge_d = ctx->small_d + 48;
     const u32* ntiles_d = ctx->small_d + 49;
-    TRY(klaunch(ctx, "k_seg_hist", kd_seg_hist<K, ONESWEEP_ITEMS>, cap_grid(ntiles, 2048),
+    TRY(klaunch(ctx, "k_seg_hist", kd_seg_hist<K, ONESWEEP_ITEMS, RADIX_BITS>, cap_grid(ntiles, 2048),
                 (double)large_rows * sizeof(K), nullptr, 0.0, skey, (const u32*)llo, (const u32*)lhi,
-                (const u32*)tbase, nlarge_d, ntiles_d, shift, D, hist, tcount));
+                (const u32*)tbase, nlarge_d, ntiles_d, shift, D, hist, tcount, (u64)0, (u32*)nullptr));
     const int nxt = cur ^ 1;
     u32 *nst, *nsz;
     TRY(getbuf_t(ctx, lst[nxt], (u64)nlarge * RADIX + 1, &nst));
@@ -792,29 +793,29 @@ int dd_scan(fmcs_ctx* ctx, const u32* in, u32* out, const u32* n_d, u64 n_max, u
 // LSD radix sort over every pass (no host-side trivial-pass skipping).
 // Keys above keybits set the error word dd[DD_ERR].  vin == nullptr stands
 // for the identity order (values = row indices), generated in the first pass.
-template <class K>
-int radix_sort_dd(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, u64 n, u32 keybits, u32* dd) {
+template <class K, int RB>
+int radix_sort_dd_t(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, u64 n, u32 keybits, u32* dd) {
+  constexpr u32 R = 1u << RB;
   if (n == 0) return FMCS_OK;
   keybits = std::min<u32>(keybits, 8 * sizeof(K));
-  const u32 npasses = (keybits + RADIX_BITS - 1) / RADIX_BITS;
+  const u32 npasses = (keybits + RB - 1) / RB;
   if (n == 1 || npasses == 0) {
     if ((const void*)kout != (const void*)kin || vout != vin)
       return klaunch(ctx, "k_copy", kd_copy_pairs<K>, 1, 0.0, nullptr, 0.0, kin, vin, kout, vout, n);
     return FMCS_OK;
   }
-  const u32 last_bits = keybits - (npasses - 1) * RADIX_BITS;
   const u64 ntiles = (n + ONESWEEP_TILE - 1) / ONESWEEP_TILE;
   u32 *hist, *dbase, *tcount;
-  TRY(getbuf_t(ctx, WS_HIST, (u64)npasses * RADIX, &hist));
-  TRY(getbuf_t(ctx, WS_DBASE, (u64)npasses * RADIX, &dbase));
-  TRY(getbuf_t(ctx, WS_TCOUNT, ntiles * RADIX, &tcount));
+  TRY(getbuf_t(ctx, WS_HIST, (u64)npasses * R, &hist));
+  TRY(getbuf_t(ctx, WS_DBASE, (u64)npasses * R, &dbase));
+  TRY(getbuf_t(ctx, WS_TCOUNT, ntiles * R, &tcount));
   K *t0k, *t1k;
   u32 *t0v, *t1v;
   TRY(getbuf_t(ctx, WS_T0K, n, &t0k));
   TRY(getbuf_t(ctx, WS_T0V, n, &t0v));
   TRY(getbuf_t(ctx, WS_T1K, n, &t1k));
   TRY(getbuf_t(ctx, WS_T1V, n, &t1v));
-  TRY(fill(ctx, hist, (u64)npasses * RADIX, 0));
+  TRY(fill(ctx, hist, (u64)npasses * R, 0));
   // one segment [0, n) of ntiles tiles: seg_lo, seg_hi, tile_base[0], tile_base[1] (= ntiles); nsegs = 1
   u32* cst = dd + DD_CONST;
   TRY(klaunch(ctx, "k_set", kd_set4, 1, 0.0, nullptr, 0.0, cst, 0u, (u32)n, 0u, (u32)ntiles));
@@ -833,18 +834,18 @@ int radix_sort_dd(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vou
       dk = (p & 1) ? t1k : t0k;
       dv = (p & 1) ? t1v : t0v;
     }
-    u32* hp = hist + (u64)p * RADIX;
-    TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<K, ONESWEEP_ITEMS>, cap_grid(ntiles, 2048), (double)n * sizeof(K),
+    u32* hp = hist + (u64)p * R;
+    // the first pass also flags keys with bits above keybits
+    const u64 bad = (p == 0 && keybits < 8 * sizeof(K)) ? ~((1ull << keybits) - 1) : 0ull;
+    TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<K, ONESWEEP_ITEMS, RB>, cap_grid(ntiles, 2048), (double)n * sizeof(K),
                 nullptr, 0.0, sk, (const u32*)cst, (const u32*)cst + 1, (const u32*)cst + 2, one, (const u32*)cst + 3,
-                p * RADIX_BITS, (u32)RADIX_BITS, hp, tcount));
-    TRY(klaunch(ctx, "k_digit_base", kd_digit_base, 1, 0.0, nullptr, 0.0, (const u32*)hp, (const u32*)cst,
-                dbase + (u64)p * RADIX, (u32*)nullptr, (u32*)nullptr, one));
-    if (last && last_bits < (u32)RADIX_BITS)
-      TRY(klaunch(ctx, "k_check_digits", kd_check_digits, 1, 0.0, nullptr, 0.0, (const u32*)hp, last_bits,
-                  dd + DD_ERR));
-    TRY(scatter_pass<K>(ctx, "k_onesweep", sk, sv, dk, dv, cst, cst + 1, cst + 2, one, cst + 3, ntiles,
-                        dbase + (u64)p * RADIX, tcount, p * RADIX_BITS, (u32)RADIX_BITS, 2.0 * n * (sizeof(K) + 4),
-                        nullptr, 0.0));
+                p * RB, (u32)RB, hp, tcount, bad, dd + DD_ERR));
+    TRY(klaunch(ctx, "k_digit_base", kd_digit_base<RB>, 1, 0.0, nullptr, 0.0, (const u32*)hp, (const u32*)cst,
+                dbase + (u64)p * R, (u32*)nullptr, (u32*)nullptr, one));
+    const int rc = scatter_pass<K, RB>(ctx, "k_onesweep", sk, sv, dk, dv, cst, cst + 1, cst + 2, one, cst + 3, ntiles,
+                                       dbase + (u64)p * R, tcount, p * RB, (u32)RB, 2.0 * n * (sizeof(K) + 4),
+                                       nullptr, 0.0);
+    if (rc != FMCS_OK) return rc;
     sk = dk;
     sv = dv;
   }
@@ -852,6 +853,14 @@ int radix_sort_dd(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vou
     TRY(klaunch(ctx, "k_copy", kd_copy_pairs<K>, cap_grid((n + 255) / 256, 4096), 2.0 * n * (sizeof(K) + 4), nullptr,
                 0.0, sk, sv, kout, vout, n));
   return FMCS_OK;
+}
+
+// 8-bit digits.  (10-bit digits halve a 20-bit key's passes to two, but a
+// 1024-digit scatter pass measured 1.75x an 8-bit one here: 4-row digit
+// runs per 4096-row tile and two resident tiles per CU instead of three.)
+template <class K>
+int radix_sort_dd(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, u64 n, u32 keybits, u32* dd) {
+  return radix_sort_dd_t<K, RADIX_BITS>(ctx, kin, vin, kout, vout, n, keybits, dd);
 }
 
 template <class K>
@@ -1004,12 +1013,12 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
       break;
     }
     TRY(dd_fill(ctx, hist, S + LV_NLARGE, RADIX, 1, 0, 0, d.nlarge_max * RADIX));
-    TRY(klaunch(ctx, "k_seg_hist", kd_seg_hist<K, ONESWEEP_ITEMS>, cap_grid(d.ntiles_max, 2048), 0.0,
+    TRY(klaunch(ctx, "k_seg_hist", kd_seg_hist<K, ONESWEEP_ITEMS, RADIX_BITS>, cap_grid(d.ntiles_max, 2048), 0.0,
                 S + LV_ROWS_LARGE, (double)sizeof(K), (const K*)skey, (const u32*)llo, (const u32*)lhi,
                 (const u32*)tbase, (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES), d.shift, d.D, hist,
-                tcount));
+                tcount, (u64)0, (u32*)nullptr));
     const int nxt = cur ^ 1;
-    TRY(klaunch(ctx, "k_digit_base", kd_digit_base, cap_grid(d.nlarge_max, 2048), 0.0, nullptr, 0.0,
+    TRY(klaunch(ctx, "k_digit_base", kd_digit_base<RADIX_BITS>, cap_grid(d.nlarge_max, 2048), 0.0, nullptr, 0.0,
                 (const u32*)hist, (const u32*)llo, dbase, lst[nxt], lsz[nxt], (const u32*)(S + LV_NLARGE)));
     TRY(scatter_pass<K>(ctx, "k_onesweep_msd", skey, sval, dkey, dval, llo, lhi, tbase, S + LV_NLARGE,
                         S + LV_NTILES, d.ntiles_max, dbase, tcount, d.shift, d.D, 0.0, S + LV_ROWS_LARGE,

@@ -202,12 +202,14 @@ __global__ void kd_large_compact(const u32* __restrict__ st, const u32* __restri
 // (the whole array in an LSD pass) sees one global atomic per block and
 // digit.  Full tiles that start on a 16-byte boundary are read with 16-byte
 // loads (the rate of 4-byte loads is the limit for 16- and 32-bit keys).
-template <class K, int ITEMS>
+// A key with a bit in bad_mask (above the declared width) sets *err.
+template <class K, int ITEMS, int RB>
 __global__ __launch_bounds__(256) void kd_seg_hist(const K* __restrict__ keys, const u32* __restrict__ seg_lo,
                                                    const u32* __restrict__ seg_hi, const u32* __restrict__ tile_base,
                                                    const u32* __restrict__ nsegs_d, const u32* __restrict__ ntiles_d,
                                                    u32 shift, u32 dbits, u32* __restrict__ hist,
-                                                   u32* __restrict__ tile_counts) {
+                                                   u32* __restrict__ tile_counts, u64 bad_mask,
+                                                   u32* __restrict__ err) {
   constexpr int TILE = 256 * ITEMS;
   constexpr int PV = 16 / sizeof(K);           // keys per 16-byte vector
   constexpr int NV = TILE / PV / 256;          // vectors per thread in a full tile
@@ -215,22 +217,29 @@ __global__ __launch_bounds__(256) void kd_seg_hist(const K* __restrict__ keys, c
   // instruction that share a digit mostly hit different copies, so a skewed
   // key does not serialize the LDS atomics.  Copies are HSTRIDE words apart
   // so the copies of one digit sit in different banks.
-  constexpr u32 HCOPY = 8, HSTRIDE = RADIX + 1;
+  constexpr u32 R = 1u << RB, DPT = R / 256;  // RB-bit digits; thread t owns digits t + 256 q
+  constexpr u32 HCOPY = RB > 8 ? 4 : 8, HSTRIDE = R + 1;
   __shared__ u32 sh[HCOPY * HSTRIDE];
   __shared__ u32 s_seg;
   const u32 nsegs = *nsegs_d, ntiles = *ntiles_d;
   const u32 mask = (1u << dbits) - 1;
   u32* myh = sh + (threadIdx.x % HCOPY) * HSTRIDE;
-  u32 acc = 0;
+  u32 acc[DPT];
+#pragma unroll
+  for (u32 q = 0; q < DPT; ++q) acc[q] = 0;
   u32 cur = 0xffffffffu;
+  u64 bad = 0;
   for (u32 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     for (u32 i = threadIdx.x; i < HCOPY * HSTRIDE; i += 256) sh[i] = 0;
     if (threadIdx.x == 0) s_seg = upper_bound_m1(tile_base, nsegs, tile);
     __syncthreads();
     const u32 seg = s_seg;
     if (seg != cur) {
-      if (hist && acc) atomicAdd(&hist[(u64)cur * RADIX + threadIdx.x], acc);
-      acc = 0;
+#pragma unroll
+      for (u32 q = 0; q < DPT; ++q) {
+        if (hist && acc[q]) atomicAdd(&hist[(u64)cur * R + threadIdx.x + 256 * q], acc[q]);
+        acc[q] = 0;
+      }
       cur = seg;
     }
     const u64 row0 = (u64)seg_lo[seg] + (u64)(tile - tile_base[seg]) * TILE;
@@ -245,7 +254,10 @@ __global__ __launch_bounds__(256) void kd_seg_hist(const K* __restrict__ keys, c
         K k[PV];
         unpack16<K>(x[v], k);
 #pragma unroll
-        for (int i = 0; i < PV; ++i) atomicAdd(&myh[digit_of(k[i], shift, mask)], 1u);
+        for (int i = 0; i < PV; ++i) {
+          atomicAdd(&myh[digit_of(k[i], shift, mask)], 1u);
+          bad |= (u64)k[i] & bad_mask;
+        }
       }
     } else {
       K k[ITEMS];
@@ -256,17 +268,27 @@ __global__ __launch_bounds__(256) void kd_seg_hist(const K* __restrict__ keys, c
       }
 #pragma unroll
       for (int j = 0; j < ITEMS; ++j)
-        if (row0 + (u64)j * 256 + threadIdx.x < rowend) atomicAdd(&myh[digit_of(k[j], shift, mask)], 1u);
+        if (row0 + (u64)j * 256 + threadIdx.x < rowend) {
+          atomicAdd(&myh[digit_of(k[j], shift, mask)], 1u);
+          bad |= (u64)k[j] & bad_mask;
+        }
     }
     __syncthreads();
-    u32 c = 0;
 #pragma unroll
-    for (u32 h = 0; h < HCOPY; ++h) c += sh[h * HSTRIDE + threadIdx.x];
-    acc += c;
-    if (tile_counts) tile_counts[(u64)tile * RADIX + threadIdx.x] = c;
+    for (u32 q = 0; q < DPT; ++q) {
+      const u32 d = threadIdx.x + 256 * q;
+      u32 c = 0;
+#pragma unroll
+      for (u32 h = 0; h < HCOPY; ++h) c += sh[h * HSTRIDE + d];
+      acc[q] += c;
+      if (tile_counts) tile_counts[(u64)tile * R + d] = c;
+    }
     __syncthreads();
   }
-  if (hist && acc) atomicAdd(&hist[(u64)cur * RADIX + threadIdx.x], acc);
+#pragma unroll
+  for (u32 q = 0; q < DPT; ++q)
+    if (hist && acc[q]) atomicAdd(&hist[(u64)cur * R + threadIdx.x + 256 * q], acc[q]);
+  if (err && __ballot(bad != 0) && __lane_id() == 0) atomicOr(err, 1u);
 }
 
 // Per-segment exclusive scan of the tile counts along the tiles, in two
@@ -274,46 +296,52 @@ __global__ __launch_bounds__(256) void kd_seg_hist(const K* __restrict__ keys, c
 // replaces counts[t][d] by the sum of counts[t'][d] over earlier tiles t' of
 // the chunk in the same segment; carry[c][d] = that running sum at the chunk
 // end, cflag[c] = 1 when a segment starts inside chunk c.
+template <int RB>
 __global__ __launch_bounds__(256) void kd_tscan_local(u32* __restrict__ counts, const u32* __restrict__ tile_base,
                                                       const u32* __restrict__ nsegs_d,
                                                       const u32* __restrict__ ntiles_d, u32* __restrict__ carry,
                                                       u32* __restrict__ cflag) {
+  constexpr u32 R = 1u << RB;
   __shared__ u32 s_first[TSCAN_CT];
   const u32 nsegs = *nsegs_d, ntiles = *ntiles_d;
   const u32 nch = (ntiles + TSCAN_CT - 1) / TSCAN_CT;
-  const u32 d = threadIdx.x;
   for (u32 c = blockIdx.x; c < nch; c += gridDim.x) {
     const u32 t0 = c * TSCAN_CT;
     const u32 m = min(ntiles - t0, TSCAN_CT);
-    if (d < m) s_first[d] = tile_base[upper_bound_m1(tile_base, nsegs, t0 + d)] == t0 + d;
+    if (threadIdx.x < m) s_first[threadIdx.x] = tile_base[upper_bound_m1(tile_base, nsegs, t0 + threadIdx.x)] == t0 + threadIdx.x;
     __syncthreads();
-    u32 v[TSCAN_CT];
+    u32 any = 0;
+    for (u32 d = threadIdx.x; d < R; d += 256) {
+      u32 v[TSCAN_CT];
 #pragma unroll
-    for (u32 j = 0; j < TSCAN_CT; ++j) v[j] = j < m ? counts[(u64)(t0 + j) * RADIX + d] : 0u;
-    u32 run = 0, any = 0;
+      for (u32 j = 0; j < TSCAN_CT; ++j) v[j] = j < m ? counts[(u64)(t0 + j) * R + d] : 0u;
+      u32 run = 0;
 #pragma unroll
-    for (u32 j = 0; j < TSCAN_CT; ++j) {
-      if (j < m) {
-        if (s_first[j]) {
-          run = 0;
-          any = 1;
+      for (u32 j = 0; j < TSCAN_CT; ++j) {
+        if (j < m) {
+          if (s_first[j]) {
+            run = 0;
+            any = 1;
+          }
+          counts[(u64)(t0 + j) * R + d] = run;
+          run += v[j];
         }
-        counts[(u64)(t0 + j) * RADIX + d] = run;
-        run += v[j];
       }
+      carry[(u64)c * R + d] = run;
     }
-    carry[(u64)c * RADIX + d] = run;
-    if (d == 0) cflag[c] = any;
+    if (threadIdx.x == 0) cflag[c] = any;
     __syncthreads();
   }
 }
 
-// kd_tscan_carry: block d (grid = RADIX) turns carry[.][d] into carry_in:
+// kd_tscan_carry: block d (grid = 2^RB) turns carry[.][d] into carry_in:
 // the sum over earlier chunks back to (and including the tail of) the chunk
 // where the segment of the chunk's first tile starts; a segmented exclusive
 // scan over the chunks with resets at chunks that hold a segment start.
+template <int RB>
 __global__ __launch_bounds__(256) void kd_tscan_carry(u32* __restrict__ carry, const u32* __restrict__ cflag,
                                                       const u32* __restrict__ ntiles_d) {
+  constexpr u32 R = 1u << RB;
   __shared__ u32 s_f[256], s_v[256];
   const u32 d = blockIdx.x, tid = threadIdx.x;
   const u32 nch = (*ntiles_d + TSCAN_CT - 1) / TSCAN_CT;
@@ -322,7 +350,7 @@ __global__ __launch_bounds__(256) void kd_tscan_carry(u32* __restrict__ carry, c
   // this thread's chunks reduced to (has reset, sum since the last reset)
   u32 f = 0, v = 0;
   for (u32 c = c0; c < c1; ++c) {
-    const u32 x = carry[(u64)c * RADIX + d];
+    const u32 x = carry[(u64)c * R + d];
     if (cflag[c]) {
       f = 1;
       v = x;
@@ -348,29 +376,40 @@ __global__ __launch_bounds__(256) void kd_tscan_carry(u32* __restrict__ carry, c
   }
   u32 acc = tid ? s_v[tid - 1] : 0u;
   for (u32 c = c0; c < c1; ++c) {
-    const u32 x = carry[(u64)c * RADIX + d];
-    carry[(u64)c * RADIX + d] = acc;
+    const u32 x = carry[(u64)c * R + d];
+    carry[(u64)c * R + d] = acc;
     acc = cflag[c] ? x : acc + x;
   }
 }
 
 // digit_base[s][d] = lo[s] + prefix; bucket list (start, size) in (s, d)
 // order; loops over *nsets_d sets.
+template <int RB>
 __global__ __launch_bounds__(256) void kd_digit_base(const u32* __restrict__ hist, const u32* __restrict__ lo,
                                                      u32* __restrict__ digit_base, u32* __restrict__ bstart,
                                                      u32* __restrict__ bsize, const u32* __restrict__ nsets_d) {
+  constexpr u32 R = 1u << RB, DPT = R / 256;  // thread t owns digits t * DPT + q
   __shared__ u32 tmp[8];
   const u32 nsets = *nsets_d;
-  const u32 d = threadIdx.x;
   for (u32 s = blockIdx.x; s < nsets; s += gridDim.x) {
-    u32 c = hist[(u64)s * RADIX + d];
+    u32 c[DPT];
+    u32 sum = 0;
+#pragma unroll
+    for (u32 q = 0; q < DPT; ++q) {
+      c[q] = hist[(u64)s * R + threadIdx.x * DPT + q];
+      sum += c[q];
+    }
     u32 tot;
-    u32 ex = block_excl_scan<256>(c, tmp, &tot);
-    u32 b = lo[s] + ex;
-    digit_base[(u64)s * RADIX + d] = b;
-    if (bstart) {
-      bstart[(u64)s * RADIX + d] = b;
-      bsize[(u64)s * RADIX + d] = c;
+    u32 b = lo[s] + block_excl_scan<256>(sum, tmp, &tot);
+#pragma unroll
+    for (u32 q = 0; q < DPT; ++q) {
+      const u32 d = threadIdx.x * DPT + q;
+      digit_base[(u64)s * R + d] = b;
+      if (bstart) {
+        bstart[(u64)s * R + d] = b;
+        bsize[(u64)s * R + d] = c[q];
+      }
+      b += c[q];
     }
   }
 }
@@ -396,13 +435,6 @@ __global__ void kd_copy_segs(const K* __restrict__ kin, const u32* __restrict__ 
   }
 }
 
-// Keys of the last LSD pass must fit its `bits`: a count in a digit at or
-// above 2^bits means a key exceeds the declared bit width; flag it.
-__global__ void kd_check_digits(const u32* __restrict__ hist, u32 bits, u32* __restrict__ err) {
-  const u32 d = threadIdx.x;
-  if (d >= (1u << bits) && d < RADIX && hist[d] != 0) atomicOr(err, 1u);
-}
-
 template <class K>
 __global__ void kd_copy_pairs(const K* __restrict__ kin, const u32* __restrict__ vin, K* __restrict__ kout,
                               u32* __restrict__ vout, u64 n) {
@@ -418,8 +450,8 @@ __global__ void kd_copy1(const u32* __restrict__ src, u32* __restrict__ dst) {
 }
 
 #define FMCS_DD_INST(K)                                                                                            \
-  template __global__ void kd_seg_hist<K, ONESWEEP_ITEMS>(const K*, const u32*, const u32*, const u32*, const u32*,          \
-                                              const u32*, u32, u32, u32*, u32*);                                       \
+  template __global__ void kd_seg_hist<K, ONESWEEP_ITEMS, 8>(const K*, const u32*, const u32*, const u32*,           \
+                                              const u32*, const u32*, u32, u32, u32*, u32*, u64, u32*);            \
   template __global__ void kd_copy_segs<K>(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*,   \
                                            const u32*, const u32*, u32);                            \
   template __global__ void kd_copy_pairs<K>(const K*, const u32*, K*, u32*, u64);
@@ -427,5 +459,8 @@ FMCS_DD_INST(u16)
 FMCS_DD_INST(u32)
 FMCS_DD_INST(u64)
 template __global__ void kd_scan_excl<8>(const u32*, u32*, const u32*, u32*, u32*, u32*);
+template __global__ void kd_tscan_local<8>(u32*, const u32*, const u32*, const u32*, u32*, u32*);
+template __global__ void kd_tscan_carry<8>(u32*, const u32*, const u32*);
+template __global__ void kd_digit_base<8>(const u32*, const u32*, u32*, u32*, u32*, const u32*);
 
 }  // namespace fmcs

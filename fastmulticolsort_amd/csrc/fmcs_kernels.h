@@ -70,7 +70,7 @@ __global__ void k_bwrite(const K*, const u32*, u64, const u32*, u32*, u32*, u32*
 template <class K>
 __global__ void k_hist_all(const K*, u64, u32, u32, u32*);
 __global__ void k_digit_base(const u32*, const u32*, u32*, u32*, u32*);
-template <class K, int ITEMS>
+template <class K, int ITEMS, int RB>
 __global__ void k_onesweep(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, u32, const u32*,
                            const u32*, const u32*, u32, u32, const u32*, const u32*);
 template <class K, int E>
@@ -123,17 +123,19 @@ __global__ void kd_large_flags(const u32*, const u32*, u32, u32*, u32*);
 template <int ITEMS>
 __global__ void kd_scan_excl(const u32*, u32*, const u32*, u32*, u32*, u32*);
 __global__ void kd_large_compact(const u32*, const u32*, const u32*, u32, const u32*, u32, u32*, u32*, u32*);
-template <class K, int ITEMS>
+template <class K, int ITEMS, int RB>
 __global__ void kd_seg_hist(const K*, const u32*, const u32*, const u32*, const u32*, const u32*, u32, u32, u32*,
-                            u32*);
+                            u32*, u64, u32*);
+template <int RB>
 __global__ void kd_tscan_local(u32*, const u32*, const u32*, const u32*, u32*, u32*);
+template <int RB>
 __global__ void kd_tscan_carry(u32*, const u32*, const u32*);
+template <int RB>
 __global__ void kd_digit_base(const u32*, const u32*, u32*, u32*, u32*, const u32*);
 template <class K>
 __global__ void kd_copy_segs(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, const u32*,
                              const u32*, u32);
 __global__ void kd_copy1(const u32*, u32*);
-__global__ void kd_check_digits(const u32*, u32, u32*);
 template <class K>
 __global__ void kd_copy_pairs(const K*, const u32*, K*, u32*, u64);
 

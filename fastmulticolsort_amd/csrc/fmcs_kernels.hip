@@ -364,7 +364,7 @@ __global__ __launch_bounds__(256) void k_digit_base(const u32* __restrict__ hist
 // lane l is row wave_base + j*64 + l, so the ranking order (j, lane) is the
 // row order and the pass is stable.
 // ---------------------------------------------------------------------------
-template <class K, int ITEMS>
+template <class K, int ITEMS, int RB>
 __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, const u32* __restrict__ vin,
                                                   K* __restrict__ kout, u32* __restrict__ vout,
                                                   const u32* __restrict__ seg_lo, const u32* __restrict__ seg_hi,
@@ -372,11 +372,13 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
                                                   const u32* __restrict__ digit_base, const u32* __restrict__ tile_pre,
                                                   const u32* __restrict__ carry_in, u32 shift, u32 dbits,
                                                   const u32* __restrict__ nsegs_d, const u32* __restrict__ ntiles_d) {
+  // RB-bit digits (R of them); every thread owns DPT consecutive digits
   constexpr int NT = 256, NW = NT / WAVE, TILE = NT * ITEMS, WITEMS = WAVE * ITEMS;
+  constexpr int R = 1 << RB, DPT = R / NT;
   __shared__ u32 s_seg;
-  __shared__ u32 s_whist[NW][RADIX];
-  __shared__ u32 s_tdig[RADIX];
-  __shared__ u32 s_goff[RADIX];
+  __shared__ u32 s_whist[NW][R];
+  __shared__ u32 s_tdig[R];
+  __shared__ u32 s_goff[R];
   __shared__ u32 s_tmp[NW + 1];
   __shared__ K s_keys[TILE];
   __shared__ u32 s_vals[TILE];
@@ -393,7 +395,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
   const u32 xcd = blockIdx.x % XCDS, per_x = gridDim.x / XCDS;
   const u32 t_hi = (u32)((u64)ntiles * (xcd + 1) / XCDS);
   for (u32 tile = (u32)((u64)ntiles * xcd / XCDS) + blockIdx.x / XCDS; tile < t_hi; tile += per_x) {
-  for (u32 i = tid; i < NW * RADIX; i += NT) (&s_whist[0][0])[i] = 0;
+  for (u32 i = tid; i < NW * R; i += NT) (&s_whist[0][0])[i] = 0;
   if (tid == 0) s_seg = upper_bound_m1(tile_base, nsegs, tile);
   __syncthreads();
   const u32 seg = s_seg;
@@ -414,12 +416,17 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
     keys[j] = ok ? kin[r] : (K)0;
     vals[j] = ok ? (vin ? vin[r] : (u32)r) : 0u;  // no vin: the row index (identity order)
   }
-  // this tile's global offset of digit d = tid, loaded now so the loads
-  // overlap the ranking
-  const u32 d = tid;
+  // this tile's global offsets of digits tid * DPT + q, loaded now so the
+  // loads overlap the ranking
   const u32 chunk = tile / TSCAN_CT;
-  u32 gbase = digit_base[(u64)seg * RADIX + d] + tile_pre[(u64)tile * RADIX + d];
-  if (first_tile < chunk * TSCAN_CT) gbase += carry_in[(u64)chunk * RADIX + d];
+  const bool carry = first_tile < chunk * TSCAN_CT;
+  u32 gbase[DPT];
+#pragma unroll
+  for (int q = 0; q < DPT; ++q) {
+    const u32 d = tid * DPT + q;
+    gbase[q] = digit_base[(u64)seg * R + d] + tile_pre[(u64)tile * R + d];
+    if (carry) gbase[q] += carry_in[(u64)chunk * R + d];
+  }
   // Stable ranking: lanes with equal digits are matched by ballots; the
   // lowest such lane adds the match count to the wave's LDS counter with one
   // returning atomic.  The atomics of all items are issued back to back (LDS
@@ -446,18 +453,30 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) rank[j] = __shfl(old[j], (int)(rank[j] >> 8)) + (rank[j] & 0xffu);
   __syncthreads();
-  // per-digit: wave offsets within the tile and the tile's count
-  u32 run = 0;
-  for (int w = 0; w < NW; ++w) {
-    u32 c = s_whist[w][d];
-    s_whist[w][d] = run;
-    run += c;
+  // per digit: wave offsets within the tile and the tile's count
+  u32 tcount[DPT];
+  u32 tsum = 0;
+#pragma unroll
+  for (int q = 0; q < DPT; ++q) {
+    const u32 d = tid * DPT + q;
+    u32 run = 0;
+    for (int w = 0; w < NW; ++w) {
+      u32 c = s_whist[w][d];
+      s_whist[w][d] = run;
+      run += c;
+    }
+    tcount[q] = run;
+    tsum += run;
   }
-  const u32 tcount = run;
   u32 ttot;
-  u32 tdig = block_excl_scan<NT>(tcount, s_tmp, &ttot);
-  s_tdig[d] = tdig;
-  s_goff[d] = gbase - tdig;
+  u32 tdig = block_excl_scan<NT>(tsum, s_tmp, &ttot);
+#pragma unroll
+  for (int q = 0; q < DPT; ++q) {
+    const u32 d = tid * DPT + q;
+    s_tdig[d] = tdig;
+    s_goff[d] = gbase[q] - tdig;
+    tdig += tcount[q];
+  }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
@@ -1339,8 +1358,9 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
   template __global__ void k_bcount<K, BOUNDARY_ITEMS>(const K*, const u32*, u64, u32*);                            \
   template __global__ void k_bwrite<K, BOUNDARY_ITEMS>(const K*, const u32*, u64, const u32*, u32*, u32*, u32*);      \
   template __global__ void k_hist_all<K>(const K*, u64, u32, u32, u32*);                                            \
-  template __global__ void k_onesweep<K, ONESWEEP_ITEMS>(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*,    \
-                                             u32, const u32*, const u32*, const u32*, u32, u32, const u32*, const u32*);                                \
+  template __global__ void k_onesweep<K, ONESWEEP_ITEMS, 8>(const K*, const u32*, K*, u32*, const u32*, const u32*,   \
+                                             const u32*, u32, const u32*, const u32*, const u32*, u32, u32,       \
+                                             const u32*, const u32*);                                \
   template __global__ void k_tile_sort<K, 4>(TileSortArgs<K>);                                                      \
   template __global__ void k_tile_sort<K, 2>(TileSortArgs<K>);                                                      \
   template __global__ void k_tile_rank<K, 4>(TileSortArgs<K>);                                                      \
