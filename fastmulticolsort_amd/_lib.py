@@ -45,7 +45,7 @@ class GroupInfo(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("small_max", C.c_uint32), ("window", C.c_uint32), ("msd_bits", C.c_uint32),
-                ("gather_window_kb", C.c_uint32)]
+                ("gather_window_kb", C.c_uint32), ("carry_next", C.c_uint32)]
 
 
 class FmcsError(RuntimeError):

@@ -3,11 +3,13 @@
 Sweeps the bin thresholds and tile shapes of fmcs_tuning (small_max: the
 largest segment packed into an LDS tile, which also picks the 512- or
 1024-row tile; window: rows per packing window; msd_bits: digit width of an
-MSD level) and the source-window size of the oid gather (gather_window_kb,
-KiB, 0 = one pass) on a workload, timing whole multi-round sorts, and writes the
-fastest setting as JSON.  The gather window is swept first with the sort
-settings fixed, then the sort grid with the best window.  Results never
-depend on the tuning, only the time does.
+MSD level), how the second round's column reaches the first round's order
+(carry_next: moved along by the first round's radix passes, or gathered, in
+one pass or in passes over source windows of gather_window_kb KiB) on a
+workload, timing whole multi-round sorts, and writes the fastest setting as
+JSON.  The column move is swept first with the sort settings fixed, then the
+sort grid with the best move.  Results never depend on the tuning, only the
+time does.
 """
 import itertools
 import json
@@ -31,7 +33,8 @@ def candidates(grid=GRID):
 
 
 def apply(ctx, setting):
-    ctx.set_tuning(setting["small_max"], setting["window"], setting["msd_bits"], setting.get("gather_window_kb", 0))
+    ctx.set_tuning(setting["small_max"], setting["window"], setting["msd_bits"], setting.get("gather_window_kb", 0),
+                   setting.get("carry_next", 1))
 
 
 def time_setting(ctx, run, setting, reps=3):
@@ -58,9 +61,12 @@ def tune(ctx, run, grid=GRID, gather_kb=GATHER_KB, reps=3, log=None):
         return ms
 
     base = {k: v for k, v in ctx.tuning().items()}
-    gmb = min(gather_kb, key=lambda g: one(dict(base, gather_window_kb=g)))
+    # how round 2's column reaches round-1 order: carried by round 1's
+    # radix passes, or gathered (in one pass or source windows)
+    moves = [dict(carry_next=1, gather_window_kb=0)] + [dict(carry_next=0, gather_window_kb=g) for g in gather_kb]
+    mv = min(moves, key=lambda m: one(dict(base, **m)))
     for st in candidates(grid):
-        one(dict(st, gather_window_kb=gmb))
+        one(dict(st, **mv))
     best = min(table, key=lambda r: r["ms"])
     apply(ctx, best)
     return best, table

@@ -25,7 +25,7 @@ enum Slot {
   WS_KA, WS_KB, WS_VB, WS_T0K, WS_T0V, WS_T1K, WS_T1V, WS_STATUS, WS_HIST, WS_DBASE, WS_SEGOFF, WS_SEGOFF2,
   WS_ST, WS_SZ, WS_BST, WS_BSZ, WS_FLAG, WS_POS, WS_LLO, WS_LHI, WS_LTILES, WS_TBASE, WS_TLO, WS_THI, WS_TS0,
   WS_TS1, WS_COMP, WS_COMP2, WS_GROUPTMP, WS_MISC, WS_HASH, WS_HASH2, WS_HASH3, WS_HT_TABLE, WS_HT_FIRST,
-  WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_DD, WS_SCANST, WS_FBLIST, WS_TCOUNT, WS_CARRY, WS_BCNT, WS_USER0,
+  WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_DD, WS_SCANST, WS_FBLIST, WS_TCOUNT, WS_CARRY, WS_BCNT, WS_T0P, WS_T1P, WS_PAY, WS_USER0,
   WS_COUNT = WS_USER0 + 24
 };
 
@@ -45,7 +45,7 @@ struct fmcs_ctx {
   Buf ws[WS_COUNT];
   u32* small_d = nullptr;   // 64 words of device scratch (counters, totals, consts)
   u32* host_pin = nullptr;  // PIN_WORDS words pinned for readbacks
-  fmcs_tuning tune = {512, 512, 8, 0};
+  fmcs_tuning tune = {512, 512, 8, 0, 1};
   hipEvent_t ev[8] = {};
   double breakdown[4] = {0, 0, 0, 0};
   // per-kernel event timing (off unless enabled)
@@ -276,10 +276,12 @@ int klaunch(fmcs_ctx* ctx, const char* name, void (*kern)(KA...), u64 grid, doub
 
 // Segmented tile-offset scan of the per-tile digit counts (written by
 // kd_seg_hist into tcount over the same tiling) and the scatter pass.
-template <class K, int RB = RADIX_BITS>
+// A payload column (pin -> pout) moves with the pairs unless P is NoPay.
+template <class K, int RB = RADIX_BITS, class P = NoPay>
 int scatter_pass(fmcs_ctx* ctx, const char* name, const K* kin, const u32* vin, K* kout, u32* vout, const u32* lo,
                  const u32* hi, const u32* tbase, const u32* nsegs_d, const u32* ntiles_d, u64 ntiles_max,
-                 const u32* dbase, u32* tcount, u32 shift, u32 dbits, double bytes, const u32* dcount, double per) {
+                 const u32* dbase, u32* tcount, u32 shift, u32 dbits, double bytes, const u32* dcount, double per,
+                 const P* pin = nullptr, P* pout = nullptr) {
   constexpr u32 R = 1u << RB;
   const u64 nch = std::max<u64>(1, (ntiles_max + TSCAN_CT - 1) / TSCAN_CT);
   u32* carry;
@@ -288,10 +290,10 @@ int scatter_pass(fmcs_ctx* ctx, const char* name, const K* kin, const u32* vin, 
   TRY(klaunch(ctx, "k_tile_scan", kd_tscan_local<RB>, cap_grid(nch, 2048), 0.0, nullptr, 0.0, tcount, tbase, nsegs_d,
               ntiles_d, carry, cflag));
   TRY(klaunch(ctx, "k_tile_scan", kd_tscan_carry<RB>, R, 0.0, nullptr, 0.0, carry, (const u32*)cflag, ntiles_d));
-  return klaunch(ctx, name, k_onesweep<K, ONESWEEP_ITEMS, RB>,
-                 onesweep_grid(ctx, (const void*)k_onesweep<K, ONESWEEP_ITEMS, RB>, ntiles_max), bytes, dcount, per, kin, vin,
-                 kout, vout, lo, hi, tbase, 0u, dbase, (const u32*)tcount, (const u32*)carry, shift, dbits, nsegs_d,
-                 ntiles_d);
+  return klaunch(ctx, name, k_onesweep<K, ONESWEEP_ITEMS, RB, P>,
+                 onesweep_grid(ctx, (const void*)k_onesweep<K, ONESWEEP_ITEMS, RB, P>, ntiles_max), bytes, dcount, per,
+                 kin, vin, kout, vout, lo, hi, tbase, 0u, dbase, (const u32*)tcount, (const u32*)carry, shift, dbits,
+                 nsegs_d, ntiles_d, pin, pout);
 }
 
 // exclusive scan of n u32 values -> out; *total_h (host) if non-null
@@ -793,13 +795,19 @@ int dd_scan(fmcs_ctx* ctx, const u32* in, u32* out, const u32* n_d, u64 n_max, u
 // LSD radix sort over every pass (no host-side trivial-pass skipping).
 // Keys above keybits set the error word dd[DD_ERR].  vin == nullptr stands
 // for the identity order (values = row indices), generated in the first pass.
-template <class K, int RB>
-int radix_sort_dd_t(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, u64 n, u32 keybits, u32* dd) {
+// Unless P is NoPay, the payload column pin moves with the rows into pout.
+template <class K, int RB, class P = NoPay>
+int radix_sort_dd_t(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, u64 n, u32 keybits, u32* dd,
+                    const P* pin = nullptr, P* pout = nullptr) {
   constexpr u32 R = 1u << RB;
+  constexpr bool HASP = !std::is_same<P, NoPay>::value;
   if (n == 0) return FMCS_OK;
   keybits = std::min<u32>(keybits, 8 * sizeof(K));
   const u32 npasses = (keybits + RB - 1) / RB;
   if (n == 1 || npasses == 0) {
+    if (HASP && (const void*)pin != (const void*)pout)
+      TRY(klaunch(ctx, "k_copy", kd_copy_col<P>, cap_grid((n + 255) / 256, 4096), 2.0 * n * sizeof(P), nullptr, 0.0,
+                  pin, pout, n));
     if ((const void*)kout != (const void*)kin || vout != vin)
       return klaunch(ctx, "k_copy", kd_copy_pairs<K>, 1, 0.0, nullptr, 0.0, kin, vin, kout, vout, n);
     return FMCS_OK;
@@ -815,6 +823,11 @@ int radix_sort_dd_t(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* v
   TRY(getbuf_t(ctx, WS_T0V, n, &t0v));
   TRY(getbuf_t(ctx, WS_T1K, n, &t1k));
   TRY(getbuf_t(ctx, WS_T1V, n, &t1v));
+  P *t0p = nullptr, *t1p = nullptr;
+  if (HASP) {
+    if (npasses > 1) TRY(getbuf_t(ctx, WS_T0P, n, &t0p));
+    if (npasses > 2) TRY(getbuf_t(ctx, WS_T1P, n, &t1p));
+  }
   TRY(fill(ctx, hist, (u64)npasses * R, 0));
   // one segment [0, n) of ntiles tiles: seg_lo, seg_hi, tile_base[0], tile_base[1] (= ntiles); nsegs = 1
   u32* cst = dd + DD_CONST;
@@ -823,16 +836,26 @@ int radix_sort_dd_t(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* v
   const u32* one = cst + 4;
   const K* sk = kin;
   const u32* sv = vin;
+  const P* sp = pin;
   for (u32 p = 0; p < npasses; ++p) {
     const bool last = (p + 1 == npasses);
     K* dk;
     u32* dv;
-    if (last && (const void*)kout != (const void*)sk && vout != sv) {
+    P* dp = nullptr;
+    if (last && (const void*)kout != (const void*)sk && vout != sv &&
+        (!HASP || (const void*)pout != (const void*)sp)) {
       dk = kout;
       dv = vout;
+      dp = pout;
     } else {
       dk = (p & 1) ? t1k : t0k;
       dv = (p & 1) ? t1v : t0v;
+      if (HASP) {
+        // a last pass that cannot write pout directly needs a second buffer
+        if (!t0p) TRY(getbuf_t(ctx, WS_T0P, n, &t0p));
+        if ((p & 1) && !t1p) TRY(getbuf_t(ctx, WS_T1P, n, &t1p));
+        dp = (p & 1) ? t1p : t0p;
+      }
     }
     u32* hp = hist + (u64)p * R;
     // the first pass also flags keys with bits above keybits
@@ -842,16 +865,21 @@ int radix_sort_dd_t(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* v
                 p * RB, (u32)RB, hp, tcount, bad, dd + DD_ERR));
     TRY(klaunch(ctx, "k_digit_base", kd_digit_base<RB>, 1, 0.0, nullptr, 0.0, (const u32*)hp, (const u32*)cst,
                 dbase + (u64)p * R, (u32*)nullptr, (u32*)nullptr, one));
-    const int rc = scatter_pass<K, RB>(ctx, "k_onesweep", sk, sv, dk, dv, cst, cst + 1, cst + 2, one, cst + 3, ntiles,
-                                       dbase + (u64)p * R, tcount, p * RB, (u32)RB, 2.0 * n * (sizeof(K) + 4),
-                                       nullptr, 0.0);
+    const double pb = HASP ? 2.0 * n * sizeof(P) : 0.0;
+    const int rc = scatter_pass<K, RB, P>(ctx, "k_onesweep", sk, sv, dk, dv, cst, cst + 1, cst + 2, one, cst + 3,
+                                          ntiles, dbase + (u64)p * R, tcount, p * RB, (u32)RB,
+                                          2.0 * n * (sizeof(K) + 4) + pb, nullptr, 0.0, sp, dp);
     if (rc != FMCS_OK) return rc;
     sk = dk;
     sv = dv;
+    sp = dp;
   }
   if ((const void*)sk != (const void*)kout || sv != vout)
     TRY(klaunch(ctx, "k_copy", kd_copy_pairs<K>, cap_grid((n + 255) / 256, 4096), 2.0 * n * (sizeof(K) + 4), nullptr,
                 0.0, sk, sv, kout, vout, n));
+  if (HASP && (const void*)sp != (const void*)pout)
+    TRY(klaunch(ctx, "k_copy", kd_copy_col<P>, cap_grid((n + 255) / 256, 4096), 2.0 * n * sizeof(P), nullptr, 0.0,
+                sp, pout, n));
   return FMCS_OK;
 }
 
@@ -1065,21 +1093,43 @@ int gather_launch(fmcs_ctx* ctx, const K* column, const u32* order, K* out, u64 
   return FMCS_OK;
 }
 
-template <class K>
+// One round.  carried: this round's column already in the previous
+// round's order (moved by that round's radix passes), sorted in place; else
+// the column is gathered into WS_KA.  Round 0 moves pnext (the next round's
+// column) into pout unless P is NoPay.
+template <class K, class P = NoPay>
 int dd_round(fmcs_ctx* ctx, u32 r, u32 keybits, u64 n, const K* column, u32* order, const u32* gin,
-             const u32* soff, u32* gout, u32* oout, bool want_groups, u32* dd) {
-  K* ka;
-  TRY(getbuf_t(ctx, WS_KA, n, &ka));
+             const u32* soff, u32* gout, u32* oout, bool want_groups, u32* dd, K* carried = nullptr,
+             const P* pnext = nullptr, P* pout = nullptr) {
+  K* ka = carried;
+  if (!ka) TRY(getbuf_t(ctx, WS_KA, n, &ka));
   if (r == 0) {
     // the identity order of round 1 (src/composer.cpp:60-63) comes from the first pass
-    TRY(radix_sort_dd<K>(ctx, column, nullptr, ka, order, n, keybits, dd));
+    TRY((radix_sort_dd_t<K, RADIX_BITS, P>(ctx, column, nullptr, ka, order, n, keybits, dd, pnext, pout)));
   } else {
-    TRY(gather_launch<K>(ctx, column, order, ka, n));
+    if (!carried) TRY(gather_launch<K>(ctx, column, order, ka, n));
     u32* lv = dd + DD_HDR + (u64)r * DD_LVL_MAX * DD_LVL_WORDS;
     TRY(seg_sort_dd<K>(ctx, ka, order, n, keybits, soff, dd + (r - 1), n, lv));
   }
   if (want_groups) TRY(boundary_dd<K>(ctx, ka, gin, n, gout, oout, dd + r));
   return FMCS_OK;
+}
+
+// Round 0 with the next round's column (bank pbank bytes) carried into pout.
+template <class K>
+int dd_round0_carry(fmcs_ctx* ctx, u32 keybits, u64 n, const K* column, u32* order, u32* gout, u32* oout,
+                    bool want_groups, u32* dd, u32 pbank, const void* pnext, void* pout) {
+  switch (pbank) {
+    case 2:
+      return dd_round<K, u16>(ctx, 0, keybits, n, column, order, nullptr, nullptr, gout, oout, want_groups, dd,
+                              nullptr, (const u16*)pnext, (u16*)pout);
+    case 4:
+      return dd_round<K, u32>(ctx, 0, keybits, n, column, order, nullptr, nullptr, gout, oout, want_groups, dd,
+                              nullptr, (const u32*)pnext, (u32*)pout);
+    default:
+      return dd_round<K, u64>(ctx, 0, keybits, n, column, order, nullptr, nullptr, gout, oout, want_groups, dd,
+                              nullptr, (const u64*)pnext, (u64*)pout);
+  }
 }
 
 // The whole multi-round sort, enqueued without a host synchronisation.
@@ -1093,6 +1143,11 @@ int mrs_dd(fmcs_ctx* ctx, u32 nrounds, const void* const* cols, const u32* banks
   TRY(getbuf_t(ctx, WS_HASH2, n, &gb));
   TRY(getbuf_t(ctx, WS_SEGOFF, n + 1, &offa));
   TRY(getbuf_t(ctx, WS_SEGOFF2, n + 1, &offb));
+  // The first round's scatter passes can carry the second round's column
+  // along: it then arrives in round-1 order with no gather (a coalesced
+  // read and write per pass instead of one random read per row).
+  void* carried = nullptr;
+  if (ctx->tune.carry_next && nrounds >= 2 && n > 1) TRY(getbuf(ctx, WS_PAY, (size_t)n * banks[1], &carried));
   const u32* gin = nullptr;
   const u32* soff = nullptr;
   for (u32 r = 0; r < nrounds; ++r) {
@@ -1102,11 +1157,20 @@ int mrs_dd(fmcs_ctx* ctx, u32 nrounds, const void* const* cols, const u32* banks
     // dense ids of this round feed only the next round's boundary
     if (!last && !(r + 2 < nrounds || groups_d)) gout = nullptr;
     u32* oout = (soff == offa) ? offb : offa;
+    void* kc = (r == 1) ? carried : nullptr;
     int rc;
-    switch (banks[r]) {
-      case 2: rc = dd_round<u16>(ctx, r, bw[r], n, (const u16*)cols[r], order, gin, soff, gout, oout, want_groups, dd); break;
-      case 4: rc = dd_round<u32>(ctx, r, bw[r], n, (const u32*)cols[r], order, gin, soff, gout, oout, want_groups, dd); break;
-      default: rc = dd_round<u64>(ctx, r, bw[r], n, (const u64*)cols[r], order, gin, soff, gout, oout, want_groups, dd); break;
+    if (r == 0 && carried) {
+      switch (banks[0]) {
+        case 2: rc = dd_round0_carry<u16>(ctx, bw[0], n, (const u16*)cols[0], order, gout, oout, want_groups, dd, banks[1], cols[1], carried); break;
+        case 4: rc = dd_round0_carry<u32>(ctx, bw[0], n, (const u32*)cols[0], order, gout, oout, want_groups, dd, banks[1], cols[1], carried); break;
+        default: rc = dd_round0_carry<u64>(ctx, bw[0], n, (const u64*)cols[0], order, gout, oout, want_groups, dd, banks[1], cols[1], carried); break;
+      }
+    } else {
+      switch (banks[r]) {
+        case 2: rc = dd_round<u16>(ctx, r, bw[r], n, (const u16*)cols[r], order, gin, soff, gout, oout, want_groups, dd, (u16*)kc); break;
+        case 4: rc = dd_round<u32>(ctx, r, bw[r], n, (const u32*)cols[r], order, gin, soff, gout, oout, want_groups, dd, (u32*)kc); break;
+        default: rc = dd_round<u64>(ctx, r, bw[r], n, (const u64*)cols[r], order, gin, soff, gout, oout, want_groups, dd, (u64*)kc); break;
+      }
     }
     if (rc != FMCS_OK) return rc;
     gin = gout;
@@ -1141,7 +1205,7 @@ void graph_drop(fmcs_ctx* ctx) {
 int mrs_graph(fmcs_ctx* ctx, u32 nrounds, const void* const* cols, const u32* banks, const u32* bw, u64 n, u32* order,
               u32* groups_d) {
   std::vector<u64> key = {(u64)nrounds, n, (u64)(uintptr_t)order, (u64)(uintptr_t)groups_d, ctx->tune.small_max,
-                          ctx->tune.window, ctx->tune.msd_bits, ctx->tune.gather_window_kb,
+                          ctx->tune.window, ctx->tune.msd_bits, ctx->tune.gather_window_kb, ctx->tune.carry_next,
                           (u64)(uintptr_t)ctx->stream};
   for (u32 r = 0; r < nrounds; ++r) {
     key.push_back((u64)(uintptr_t)cols[r]);
@@ -1325,7 +1389,7 @@ int fmcs_ctx_set_tuning(fmcs_ctx* ctx, const fmcs_tuning* t) {
   if (!ctx || !t) return FMCS_E_ARG;
   // small_max >= 64 bounds the device-driven path's segment lists by 256 / 65 entries per row
   if (t->small_max < 64 || t->small_max > (u32)TILESORT_T / 2 || t->window < 1 || t->window > t->small_max + 1 ||
-      t->msd_bits < 1 || t->msd_bits > (u32)RADIX_BITS || t->gather_window_kb > (1u << 24))
+      t->msd_bits < 1 || t->msd_bits > (u32)RADIX_BITS || t->gather_window_kb > (1u << 24) || t->carry_next > 1)
     return fail(ctx, FMCS_E_ARG, "tuning out of range");
   ctx->tune = *t;
   return FMCS_OK;

@@ -306,6 +306,8 @@ int fmcs_run_plan_hash(fmcs_ctx* ctx, uint32_t ncols, const void* const* cols_d,
 int fmcs_calibrate(fmcs_ctx* ctx, uint64_t nrows, fmcs_cost_model* out) {
   if (!ctx || !out || nrows < 4096 || nrows >= (1ull << 30)) return FMCS_E_ARG;
   fmcs_cost_model_default(out);
+  fmcs_tuning tune;
+  if (fmcs_ctx_get_tuning(ctx, &tune) == FMCS_OK) out->carry_next = tune.carry_next;
   const uint64_t n = nrows;
   CalArgs a;
   std::memset(&a, 0, sizeof(a));

@@ -444,6 +444,11 @@ __global__ void kd_copy_pairs(const K* __restrict__ kin, const u32* __restrict__
   }
 }
 
+template <class T>
+__global__ void kd_copy_col(const T* __restrict__ in, T* __restrict__ out, u64 n) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) out[i] = in[i];
+}
+
 // ---- profiling helper: snapshot a device counter into a record slot
 __global__ void kd_copy1(const u32* __restrict__ src, u32* __restrict__ dst) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *dst = *src;
@@ -454,7 +459,8 @@ __global__ void kd_copy1(const u32* __restrict__ src, u32* __restrict__ dst) {
                                               const u32*, const u32*, u32, u32, u32*, u32*, u64, u32*);            \
   template __global__ void kd_copy_segs<K>(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*,   \
                                            const u32*, const u32*, u32);                            \
-  template __global__ void kd_copy_pairs<K>(const K*, const u32*, K*, u32*, u64);
+  template __global__ void kd_copy_pairs<K>(const K*, const u32*, K*, u32*, u64);                                  \
+  template __global__ void kd_copy_col<K>(const K*, K*, u64);
 FMCS_DD_INST(u16)
 FMCS_DD_INST(u32)
 FMCS_DD_INST(u64)

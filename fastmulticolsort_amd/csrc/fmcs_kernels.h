@@ -2,6 +2,8 @@
 #pragma once
 #include "fmcs_common.h"
 
+#include <type_traits>
+
 namespace fmcs {
 
 constexpr int STITCH_MAX_SRC = 16;
@@ -70,9 +72,11 @@ __global__ void k_bwrite(const K*, const u32*, u64, const u32*, u32*, u32*, u32*
 template <class K>
 __global__ void k_hist_all(const K*, u64, u32, u32, u32*);
 __global__ void k_digit_base(const u32*, const u32*, u32*, u32*, u32*);
-template <class K, int ITEMS, int RB>
+// P = NoPay: no payload column
+struct NoPay {};
+template <class K, int ITEMS, int RB, class P>
 __global__ void k_onesweep(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, u32, const u32*,
-                           const u32*, const u32*, u32, u32, const u32*, const u32*);
+                           const u32*, const u32*, u32, u32, const u32*, const u32*, const P*, P*);
 template <class K, int E>
 __global__ void k_tile_sort(TileSortArgs<K>);
 template <class K, int E>
@@ -138,5 +142,7 @@ __global__ void kd_copy_segs(const K*, const u32*, K*, u32*, const u32*, const u
 __global__ void kd_copy1(const u32*, u32*);
 template <class K>
 __global__ void kd_copy_pairs(const K*, const u32*, K*, u32*, u64);
+template <class T>
+__global__ void kd_copy_col(const T*, T*, u64);
 
 }  // namespace fmcs

@@ -364,24 +364,32 @@ __global__ __launch_bounds__(256) void k_digit_base(const u32* __restrict__ hist
 // lane l is row wave_base + j*64 + l, so the ranking order (j, lane) is the
 // row order and the pass is stable.
 // ---------------------------------------------------------------------------
-template <class K, int ITEMS, int RB>
+// A payload column P (pin -> pout, none when P is NoPay) moves with the
+// pairs: each row's payload lands where its (key, value) lands.  It goes
+// through the same LDS reorder after the pairs are written, in the LDS that
+// held them.
+template <class K, int ITEMS, int RB, class P>
 __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, const u32* __restrict__ vin,
                                                   K* __restrict__ kout, u32* __restrict__ vout,
                                                   const u32* __restrict__ seg_lo, const u32* __restrict__ seg_hi,
                                                   const u32* __restrict__ tile_base, u32 nsegs,
                                                   const u32* __restrict__ digit_base, const u32* __restrict__ tile_pre,
                                                   const u32* __restrict__ carry_in, u32 shift, u32 dbits,
-                                                  const u32* __restrict__ nsegs_d, const u32* __restrict__ ntiles_d) {
+                                                  const u32* __restrict__ nsegs_d, const u32* __restrict__ ntiles_d,
+                                                  const P* __restrict__ pin, P* __restrict__ pout) {
   // RB-bit digits (R of them); every thread owns DPT consecutive digits
   constexpr int NT = 256, NW = NT / WAVE, TILE = NT * ITEMS, WITEMS = WAVE * ITEMS;
   constexpr int R = 1 << RB, DPT = R / NT;
+  constexpr bool HASP = !std::is_same<P, NoPay>::value;
+  constexpr int KVB = TILE * (int)(sizeof(K) + sizeof(u32)), PB = HASP ? TILE * (int)sizeof(P) : 0;
   __shared__ u32 s_seg;
   __shared__ u32 s_whist[NW][R];
   __shared__ u32 s_tdig[R];
   __shared__ u32 s_goff[R];
   __shared__ u32 s_tmp[NW + 1];
-  __shared__ K s_keys[TILE];
-  __shared__ u32 s_vals[TILE];
+  __shared__ __align__(16) unsigned char s_raw[KVB > PB ? KVB : PB];
+  K* s_keys = (K*)s_raw;
+  u32* s_vals = (u32*)(s_raw + TILE * sizeof(K));
 
   u32 tid = threadIdx.x, wv = tid / WAVE, lane = tid & (WAVE - 1);
   if (nsegs_d) nsegs = *nsegs_d;
@@ -486,11 +494,46 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
       u32 pos = s_tdig[dd] + s_whist[wv][dd] + rank[j];
       s_keys[pos] = keys[j];
       s_vals[pos] = vals[j];
+      if (HASP) rank[j] = pos;
     }
   }
   __syncthreads();
   const u32 cnt = (u32)(rowend - row0);
-  if (cnt == (u32)TILE) {
+  if constexpr (HASP) {
+    // pairs out, keeping each LDS row's global position; then the payload
+    // rows, loaded in row order, through the same LDS positions
+    P pv[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      u64 r = wbase + (u64)j * WAVE + lane;
+      pv[j] = r < rowend ? pin[r] : (P)0;
+    }
+    u32 gpos[ITEMS];
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+      const u32 i = it * NT + tid;
+      if (i < cnt) {
+        const K k = s_keys[i];
+        const u32 g = s_goff[digit_of(k, shift, mask)] + i;
+        kout[g] = k;
+        vout[g] = s_vals[i];
+        gpos[it] = g;
+      }
+    }
+    __syncthreads();
+    P* s_pay = (P*)s_raw;
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      u64 r = wbase + (u64)j * WAVE + lane;
+      if (r < rowend) s_pay[rank[j]] = pv[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+      const u32 i = it * NT + tid;
+      if (i < cnt) pout[gpos[it]] = s_pay[i];
+    }
+  } else if (cnt == (u32)TILE) {
     // full tile: independent iterations, so the LDS reads of all rows
     // overlap
 #pragma unroll
@@ -1352,15 +1395,21 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
 // ---------------------------------------------------------------------------
 // Explicit instantiations used by the launchers.
 // ---------------------------------------------------------------------------
+#define FMCS_INST_ONESWEEP(K, P)                                                                               \
+  template __global__ void k_onesweep<K, ONESWEEP_ITEMS, 8, P>(const K*, const u32*, K*, u32*, const u32*,           \
+                                                               const u32*, const u32*, u32, const u32*, const u32*,  \
+                                                               const u32*, u32, u32, const u32*, const u32*,         \
+                                                               const P*, P*);
 #define FMCS_INST_K(K)                                                                                              \
   template __global__ void k_gather<K>(const K*, const u32*, K*, u64);                                              \
   template __global__ void k_gather_win<K>(const K*, const u32*, K*, u64, u32, u32);                                \
   template __global__ void k_bcount<K, BOUNDARY_ITEMS>(const K*, const u32*, u64, u32*);                            \
   template __global__ void k_bwrite<K, BOUNDARY_ITEMS>(const K*, const u32*, u64, const u32*, u32*, u32*, u32*);      \
   template __global__ void k_hist_all<K>(const K*, u64, u32, u32, u32*);                                            \
-  template __global__ void k_onesweep<K, ONESWEEP_ITEMS, 8>(const K*, const u32*, K*, u32*, const u32*, const u32*,   \
-                                             const u32*, u32, const u32*, const u32*, const u32*, u32, u32,       \
-                                             const u32*, const u32*);                                \
+  FMCS_INST_ONESWEEP(K, NoPay)                                                                                      \
+  FMCS_INST_ONESWEEP(K, u16)                                                                                        \
+  FMCS_INST_ONESWEEP(K, u32)                                                                                        \
+  FMCS_INST_ONESWEEP(K, u64)                                                                                        \
   template __global__ void k_tile_sort<K, 4>(TileSortArgs<K>);                                                      \
   template __global__ void k_tile_sort<K, 2>(TileSortArgs<K>);                                                      \
   template __global__ void k_tile_rank<K, 4>(TileSortArgs<K>);                                                      \

@@ -107,15 +107,22 @@ double estimate(const fmcs_cost_model* m, double n, uint32_t ncols, const uint32
     G[r] = acc;
     acc *= cardv[r];
   }
+  // A carried second column makes every round-1 pass move its bank too
+  // (the passes are HBM-bound: time scales with the bytes moved per row,
+  // key + oid + payload against key + oid) and removes round 2's gather.
+  const bool carry = m->carry_next && R >= 2 && n > 1;
   double sort_ms = 0, gather_ms = 0, extract_ms = 0;
   for (size_t r = 0; r < R; ++r) {
     int bt = bank_type(split[r]);
     if (r == 0) {
-      sort_ms += radix_ms(m, bt, split[0], n);
+      double scale = 1.0;
+      if (carry)
+        scale += (double)fmcs_bank_bytes(split[1], 0) / ((double)fmcs_bank_bytes(split[0], 0) + 4.0);
+      sort_ms += radix_ms(m, bt, split[0], n) * scale;
     } else {
       double g = std::max(1.0, n / std::min(G[r], n));
       sort_ms += single_column_cost(m, n, g, bt);
-      gather_ms += n * m->gather_ns[bt] * 1e-6;
+      if (!(carry && r == 1)) gather_ms += n * m->gather_ns[bt] * 1e-6;
     }
     if (r + 1 < R) extract_ms += n * m->boundary_ns[bt] * 1e-6;
   }
@@ -444,6 +451,7 @@ int fmcs_cost_model_default(fmcs_cost_model* m) {
   }
   m->stitch_ns_per_byte = 1.0 / 5000.0;  // 5 TB/s
   m->round_overhead_ms = 0.05;
+  m->carry_next = 1;
   return FMCS_OK;
 }
 

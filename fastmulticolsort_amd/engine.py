@@ -91,12 +91,15 @@ class Context:
         t = Tuning()
         self._chk(self.lib.fmcs_ctx_get_tuning(self.h, C.byref(t)))
         return {"small_max": t.small_max, "window": t.window, "msd_bits": t.msd_bits,
-                "gather_window_kb": t.gather_window_kb}
+                "gather_window_kb": t.gather_window_kb, "carry_next": t.carry_next}
 
-    def set_tuning(self, small_max, window, msd_bits, gather_window_kb=None):
+    def set_tuning(self, small_max, window, msd_bits, gather_window_kb=None, carry_next=None):
+        cur = self.tuning()
         if gather_window_kb is None:
-            gather_window_kb = self.tuning()["gather_window_kb"]
-        t = Tuning(small_max, window, msd_bits, gather_window_kb)
+            gather_window_kb = cur["gather_window_kb"]
+        if carry_next is None:
+            carry_next = cur["carry_next"]
+        t = Tuning(small_max, window, msd_bits, gather_window_kb, int(carry_next))
         self._chk(self.lib.fmcs_ctx_set_tuning(self.h, C.byref(t)))
 
     # ---- kernels -------------------------------------------------------

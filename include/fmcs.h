@@ -70,6 +70,8 @@ typedef struct {
   uint32_t window;           /* rows per packing window (<= small_max + 1) */
   uint32_t msd_bits;         /* digit width of one MSD level (1..8) */
   uint32_t gather_window_kb; /* 0: one gather pass; else passes over source windows of this many KiB */
+  uint32_t carry_next;       /* device-driven modes: 1 = the first round's radix passes carry the second
+                                round's column along (no gather before round 2); 0 = gather it */
 } fmcs_tuning;
 
 /* ---- context -------------------------------------------------------- */

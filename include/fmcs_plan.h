@@ -30,6 +30,9 @@ typedef struct {
   double boundary_ns[3];                       /* group boundary extraction */
   double stitch_ns_per_byte;                   /* fused stitch, per byte moved */
   double round_overhead_ms;                    /* fixed cost per round */
+  uint32_t carry_next;                         /* 1: round 2's column is carried through round 1's radix
+                                                  passes (fmcs_tuning.carry_next of the device-driven
+                                                  modes), no gather before round 2; 0: gathered */
 } fmcs_cost_model;
 
 int fmcs_cost_model_default(fmcs_cost_model* m);

@@ -72,19 +72,56 @@ def test_graph_replays_match_eager_and_oracle(ctx):
 def test_graph_recaptures_on_tuning_and_shape_change(ctx):
     old = ctx.tuning()
     try:
-        for (n, sm, w, mb, gkb) in [(1 << 16, 512, 512, 8, 0), (1 << 16, 256, 128, 6, 64), (50001, 256, 128, 6, 0),
-                                    (50001, 512, 512, 8, 96)]:
+        for (n, sm, w, mb, gkb, cn) in [(1 << 16, 512, 512, 8, 0, 1), (1 << 16, 256, 128, 6, 64, 0),
+                                        (1 << 16, 256, 128, 6, 64, 1), (50001, 256, 128, 6, 0, 0),
+                                        (50001, 512, 512, 8, 96, 0), (50001, 512, 512, 8, 96, 1)]:
             (c1, c2), bws = datagen.config3(n, ngroups=1 << 8)
-            ctx.set_tuning(sm, w, mb, gkb)
+            ctx.set_tuning(sm, w, mb, gkb, cn)
             d1, d2 = dev(c1), dev(c2)
             order = torch.empty(n, dtype=torch.uint32, device="cuda")
-            log("graph recapture", n, sm, w, mb, gkb)
+            log("graph recapture", n, sm, w, mb, gkb, cn)
             for rep in range(3):
                 ctx.multi_round_sort([d1, d2], bws, order=order, mode=MODE_GRAPH, want_count=False)
                 torch.cuda.synchronize()
                 assert np.array_equal(host(order), lex([c1, c2])), (n, sm, rep)
     finally:
-        ctx.set_tuning(old["small_max"], old["window"], old["msd_bits"], old["gather_window_kb"])
+        ctx.set_tuning(old["small_max"], old["window"], old["msd_bits"], old["gather_window_kb"], old["carry_next"])
+
+
+@pytest.mark.parametrize("mode", [MODE_DEVICE, MODE_GRAPH])
+def test_carried_second_column_matches_gathered(ctx, mode):
+    # Round 1's radix passes carry round 2's column (every key / payload bank
+    # pair; 1, 2 and 3 passes; tiny inputs; a third round gathered after a
+    # carried second): same order and group ids as gathering the column.
+    old = ctx.tuning()
+    cases = [([4, 12], 5000), ([12, 30], 70001), ([20, 40], 1 << 17), ([30, 16], 33333), ([40, 60], 20000),
+             ([16, 2], 4097), ([8, 8], 3), ([1, 1], 2), ([20, 40, 10], 40000), ([50, 9, 33], 30001)]
+    try:
+        for bws, n in cases:
+            cols = []
+            for i, w in enumerate(bws):
+                c = datagen.uniform_bits(300 + i, n, w)
+                if i == 0:
+                    c = c % np.uint64(max(2, n // 37)) if w > 8 else c  # groups of ~37 rows
+                cols.append(c.astype(datagen.bank_dtype(w)))
+            exp = lex(cols)
+            s = [c[exp].astype(np.uint64) for c in cols]
+            head = np.r_[True, np.any(np.stack([x[1:] != x[:-1] for x in s]), axis=0)] if n > 1 else np.ones(n, bool)
+            egroups = (np.cumsum(head) - 1).astype(np.uint32)
+            dc = [dev(c) for c in cols]
+            for cn in (1, 0):
+                ctx.set_tuning(old["small_max"], old["window"], old["msd_bits"], old["gather_window_kb"], cn)
+                order = torch.empty(n, dtype=torch.uint32, device="cuda")
+                groups = torch.empty(n, dtype=torch.uint32, device="cuda")
+                log("carried column", mode, bws, n, cn)
+                for rep in range(2):
+                    _, ng = ctx.multi_round_sort(dc, bws, order=order, groups=groups, mode=mode)
+                    torch.cuda.synchronize()
+                    assert np.array_equal(host(order), exp), (bws, n, cn, rep)
+                    assert np.array_equal(host(groups), egroups), (bws, n, cn, rep)
+                    assert ng == int(head.sum())
+    finally:
+        ctx.set_tuning(old["small_max"], old["window"], old["msd_bits"], old["gather_window_kb"], old["carry_next"])
 
 
 @pytest.mark.parametrize("mode", [MODE_DEVICE, MODE_GRAPH])

@@ -90,7 +90,7 @@ def test_gather_windowed(ctx, dt, kb):
         out = ctx.gather(dev(v), dev(order))
         assert np.array_equal(host(out), v[order])
     finally:
-        ctx.set_tuning(old["small_max"], old["window"], old["msd_bits"], old["gather_window_kb"])
+        ctx.set_tuning(old["small_max"], old["window"], old["msd_bits"], old["gather_window_kb"], old["carry_next"])
 
 
 @pytest.mark.parametrize("dt", [np.uint16, np.uint32, np.uint64])

@@ -66,9 +66,9 @@ def test_learned_picker_beats_analytic_on_measured_plans(ctx):
     """Train on GPU-measured plan times of seeded requests, then compare plan
     regret (measured time of the pick / best measured - 1) of the MLP pick
     and of the analytic cost model's pick on held-out requests."""
-    data = learned.collect(ctx, nreq=40, seed=11, max_log2=20, max_plans=24)
+    data = learned.collect(ctx, nreq=64, seed=11, max_log2=20, max_plans=24)
     X, y, req = data["X"], data["y"], data["req"]
-    train = req < 28
+    train = req < 44
     m = learned.Mlp(ctx, seed=0)
     m.train(torch.from_numpy(X[train]).cuda(), torch.from_numpy(y[train]).cuda(), epochs=200, batch=64, lr=3e-3)
     test = ~train

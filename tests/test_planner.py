@@ -80,9 +80,25 @@ def test_estimate_structure():
     # single round: no gather, no extraction
     c1, p1 = planner.estimate_plan(m, n, [17, 33], [8192, 1 << 33], [0, 1], [50])
     assert p1["reconstruct"] == 0 and p1["extract"] == 0 and p1["sort"] > 0
-    # two rounds: one gather, one extraction
+    # two rounds, second column gathered: one gather, one extraction
+    m.carry_next = 0
     c2, p2 = planner.estimate_plan(m, n, [17, 33], [8192, 1 << 33], [0, 1], [17, 33])
     assert p2["reconstruct"] > 0 and p2["extract"] > 0
+    # carried through round 1's three passes instead: no gather, and each
+    # u32-key pass also moves the u64 bank (16 bytes a row against 8)
+    m.carry_next = 1
+    c3, p3 = planner.estimate_plan(m, n, [17, 33], [8192, 1 << 33], [0, 1], [17, 33])
+    assert p3["reconstruct"] == 0 and p3["extract"] == p2["extract"]
+    assert p3["sort"] - p2["sort"] == pytest.approx(n * 3 * m.radix_pass_ns[1] * 1e-6 * (8 / 8), rel=1e-9)
+    # a 16-bit key carrying a u16 bank: 2 bytes more on 6
+    _, q0 = planner.estimate_plan(m, n, [10, 12], [1024, 4096], [0, 1], [10, 12])
+    m.carry_next = 0
+    _, q1 = planner.estimate_plan(m, n, [10, 12], [1024, 4096], [0, 1], [10, 12])
+    assert q0["sort"] - q1["sort"] == pytest.approx(n * 2 * m.radix_pass_ns[0] * 1e-6 * (2 / 6), rel=1e-9)
+    # a third round is still gathered
+    m.carry_next = 1
+    _, p4 = planner.estimate_plan(m, n, [17, 33], [8192, 1 << 33], [0, 1], [17, 20, 13])
+    assert p4["reconstruct"] > 0
     # group counts follow card_per_bit: more leading bits -> smaller groups
     # -> cheaper later rounds under a cost table that grows with group size
     with pytest.raises(Exception):
