@@ -1,8 +1,8 @@
 """Autotuner for the batched segmented sort.
 
 Sweeps the bin thresholds and tile shapes of fmcs_tuning (small_max: the
-largest segment packed into an LDS tile, which also picks the 512- or
-1024-row tile; window: rows per packing window; msd_bits: digit width of an
+largest segment packed into an LDS tile, which also picks the 512-, 1024-
+or 2048-row tile; window: rows per packing window; msd_bits: digit width of an
 MSD level), how the second round's column reaches the first round's order
 (carry_next: moved along by the first round's radix passes, or gathered, in
 one pass or in passes over source windows of gather_window_kb KiB) on a
@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_PATH = os.path.join(HERE, "tuning_mi355x.json")
 
 GRID = {
-    "small_max": [128, 256, 512],
+    "small_max": [256, 512, 1024],
     "window_frac": [1.0, 0.5],
     "msd_bits": [6, 7, 8],
 }

@@ -483,7 +483,7 @@ int tile_bucket_launch(fmcs_ctx* ctx, TileSortArgs<K> a, u32* fb, u32 grid, doub
   return launch_check(ctx, "k_tile_bucket");
 }
 
-constexpr u32 TS_SEGBITS = 11;  // local segment index of a TILESORT_T = 1024-row tile, with headroom
+constexpr u32 TS_SEGBITS = 11;  // local segment index of a tile of up to TILESORT_TMAX = 2048 rows
 
 // ------------------------------------------------------------------------
 // Batched, size-binned segmented sort of (keys, vals) within segments.
@@ -909,8 +909,10 @@ template <class K>
 int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* seg_off, const u32* nseg_d,
                 u64 nseg_max, u32* lv) {
   if (n <= 1 || nseg_max == 0) return FMCS_OK;
-  const u32 small_max = std::min<u32>(ctx->tune.small_max, (u32)TILESORT_T / 2);
-  const u32 T = small_max <= 256 ? 512u : (u32)TILESORT_T;
+  // tiles of 512, 1024 or 2048 rows (2, 4 or 8 per thread) for segments of
+  // up to 256, 512 or 1024 rows
+  const u32 small_max = std::min<u32>(ctx->tune.small_max, (u32)TILESORT_TMAX / 2);
+  const u32 T = small_max <= 256 ? 512u : (small_max <= 512 ? 1024u : 2048u);
   u32 W = std::max<u32>(1, std::min<u32>(ctx->tune.window, small_max + 1));
   if (W + small_max > T) W = T - small_max;
   const u32 msd_bits = std::max<u32>(1, std::min<u32>(ctx->tune.msd_bits, RADIX_BITS));
@@ -979,8 +981,9 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
   TRY(getbuf_t(ctx, WS_DBASE, m_large * RADIX, &dbase));
   TRY(getbuf_t(ctx, WS_TCOUNT, m_tiles * RADIX, &tcount));
   // the tile kernels' grid: resident blocks of the chosen tile shape
-  const void* fn_bucket = T <= 512 ? (const void*)k_tile_bucket<K, 2> : (const void*)k_tile_bucket<K, 4>;
-  const void* fn_tsort = T <= 512 ? (const void*)k_tile_sort<K, 2> : (const void*)k_tile_sort<K, 4>;
+  auto fn_bucket = T <= 512 ? k_tile_bucket<K, 2> : (T <= 1024 ? k_tile_bucket<K, 4> : k_tile_bucket<K, 8>);
+  auto fn_tsort = T <= 512 ? k_tile_sort<K, 2> : (T <= 1024 ? k_tile_sort<K, 4> : k_tile_sort<K, 8>);
+  auto fn_trank = T <= 512 ? k_tile_rank<K, 2> : (T <= 1024 ? k_tile_rank<K, 4> : k_tile_rank<K, 8>);
   TRY(klaunch(ctx, "k_offsets_to_list", kd_offsets_to_list, cap_grid((nseg_max + 255) / 256, 2048), 0.0, nullptr, 0.0,
               seg_off, nseg_d, lst[0], lsz[0]));
   const u32* plo = nullptr;
@@ -1006,23 +1009,18 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
     a.src_sel = (u32)(l & 1); a.kb = d.tkb; a.pad_seg = T; a.direct = 0; a.wlist = wlist;
     a.wcount = S + LV_WCOUNT; a.fblist = fblist; a.fbcount = S + LV_FB;
     {
-      const u64 g = (persist_grid(ctx, fn_bucket, d.nw_max) + XCDS - 1) / XCDS * XCDS;
+      const u64 g = (persist_grid(ctx, (const void*)fn_bucket, d.nw_max) + XCDS - 1) / XCDS * XCDS;
       const double per = 2.0 * (sizeof(K) + 4);
-      if (T <= 512)
-        TRY(klaunch(ctx, "k_tile_bucket", k_tile_bucket<K, 2>, g, 0.0, S + LV_ROWS_SMALL, per, a, fb));
-      else
-        TRY(klaunch(ctx, "k_tile_bucket", k_tile_bucket<K, 4>, g, 0.0, S + LV_ROWS_SMALL, per, a, fb));
+      TRY(klaunch(ctx, "k_tile_bucket", fn_bucket, g, 0.0, S + LV_ROWS_SMALL, per, a, fb));
     }
     {
       TileSortArgs<K> b = a;
       b.wcount = nullptr;
-      const u64 g = persist_grid(ctx, fn_tsort, d.nw_max);
+      const u64 g = persist_grid(ctx, (const void*)fn_tsort, d.nw_max);
       if (!d.packable)
-        TRY(klaunch(ctx, "k_tile_rank", T <= 512 ? k_tile_rank<K, 2> : k_tile_rank<K, 4>, g, 0.0, nullptr, 0.0, b));
-      else if (T <= 512)
-        TRY(klaunch(ctx, "k_tile_sort", k_tile_sort<K, 2>, g, 0.0, nullptr, 0.0, b));
+        TRY(klaunch(ctx, "k_tile_rank", fn_trank, g, 0.0, nullptr, 0.0, b));
       else
-        TRY(klaunch(ctx, "k_tile_sort", k_tile_sort<K, 4>, g, 0.0, nullptr, 0.0, b));
+        TRY(klaunch(ctx, "k_tile_sort", fn_tsort, g, 0.0, nullptr, 0.0, b));
     }
     if (d.nlarge_max == 0) break;
     const K* skey = (l & 1) ? kb_ : keys;
@@ -1388,7 +1386,7 @@ int fmcs_ctx_sync(fmcs_ctx* ctx) {
 int fmcs_ctx_set_tuning(fmcs_ctx* ctx, const fmcs_tuning* t) {
   if (!ctx || !t) return FMCS_E_ARG;
   // small_max >= 64 bounds the device-driven path's segment lists by 256 / 65 entries per row
-  if (t->small_max < 64 || t->small_max > (u32)TILESORT_T / 2 || t->window < 1 || t->window > t->small_max + 1 ||
+  if (t->small_max < 64 || t->small_max > (u32)TILESORT_TMAX / 2 || t->window < 1 || t->window > t->small_max + 1 ||
       t->msd_bits < 1 || t->msd_bits > (u32)RADIX_BITS || t->gather_window_kb > (1u << 24) || t->carry_next > 1)
     return fail(ctx, FMCS_E_ARG, "tuning out of range");
   ctx->tune = *t;

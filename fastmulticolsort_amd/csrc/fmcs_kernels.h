@@ -55,6 +55,7 @@ constexpr int BOUNDARY_ITEMS = 32;
 constexpr int BOUNDARY_TILE = 256 * BOUNDARY_ITEMS;
 constexpr int TILESORT_E = 4;
 constexpr int TILESORT_T = 256 * TILESORT_E;
+constexpr int TILESORT_TMAX = 2048;  // largest tile of the device-driven path (256 threads x 8 rows)
 constexpr int SCAN_ITEMS = 8;
 constexpr u32 TSCAN_CT = 32;  // tiles per chunk of the per-segment tile-count scan
 constexpr u32 MERGE_TILE = 2048;  // rows per bitonic tile / per merge-pass block

@@ -1415,7 +1415,10 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
   template __global__ void k_tile_rank<K, 4>(TileSortArgs<K>);                                                      \
   template __global__ void k_tile_rank<K, 2>(TileSortArgs<K>);                                                      \
   template __global__ void k_tile_bucket<K, 4>(TileSortArgs<K>, u32*);                                             \
-  template __global__ void k_tile_bucket<K, 2>(TileSortArgs<K>, u32*);                                                      \
+  template __global__ void k_tile_bucket<K, 2>(TileSortArgs<K>, u32*);                                             \
+  template __global__ void k_tile_bucket<K, 8>(TileSortArgs<K>, u32*);                                             \
+  template __global__ void k_tile_sort<K, 8>(TileSortArgs<K>);                                                      \
+  template __global__ void k_tile_rank<K, 8>(TileSortArgs<K>);                                                      \
   template __global__ void k_copy_segs<K>(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, u32,  \
                                           u32);                                                                     \
   template __global__ void k_composite<K>(const K*, const u32*, u64, u32, u64*);                                    \

@@ -66,7 +66,9 @@ typedef struct {
 
 /* Tunables of the batched segmented sort and of the oid gather. */
 typedef struct {
-  uint32_t small_max;        /* segments up to this many rows go to packed LDS tiles (64..512) */
+  uint32_t small_max;        /* segments up to this many rows go to packed LDS tiles (64..1024; tiles of
+                                512 / 1024 / 2048 rows for up to 256 / 512 / 1024; the eager batched mode
+                                uses at most 512) */
   uint32_t window;           /* rows per packing window (<= small_max + 1) */
   uint32_t msd_bits;         /* digit width of one MSD level (1..8) */
   uint32_t gather_window_kb; /* 0: one gather pass; else passes over source windows of this many KiB */
