@@ -30,8 +30,10 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "config3_2p27_traffic.json")
 # the bench's kernel timer names -> the kernel instance this workload runs
 PROFILE_NAME = {"k_gather": "k_gather<unsigned long>", "k_tile_bucket": "k_tile_bucket<unsigned long, 4>",
-                "k_onesweep": "k_onesweep<unsigned int, 16>", "k_bwrite": "k_bwrite<unsigned int, 32>",
-                "k_bcount": "k_bcount<unsigned int, 32>", "k_tile_hist": "kd_seg_hist<unsigned int, 16>"}
+                "k_onesweep": "k_onesweep<unsigned int, 16, 8, unsigned long>",
+                "k_onesweep_msd": "k_onesweep<unsigned long, 16, 8, NoPay>", "k_bwrite": "k_bwrite<unsigned int, 32>",
+                "k_bcount": "k_bcount<unsigned int, 32>", "k_tile_hist": "kd_seg_hist<unsigned int, 16, 8>",
+                "k_seg_hist": "kd_seg_hist<unsigned long, 16, 8>"}
 
 
 def measured_traffic(kernel):

@@ -4,8 +4,9 @@ Sweeps the bin thresholds and tile shapes of fmcs_tuning (small_max: the
 largest segment packed into an LDS tile, which also picks the 512-, 1024-
 or 2048-row tile; window: rows per packing window; msd_bits: digit width of an
 MSD level), how the second round's column reaches the first round's order
-(carry_next: moved along by the first round's radix passes, or gathered, in
-one pass or in passes over source windows of gather_window_kb KiB) on a
+(carry_next: moved along by the first round's radix passes, packed into
+composite keys or as its own column, or gathered, in one pass or in passes
+over source windows of gather_window_kb KiB) on a
 workload, timing whole multi-round sorts, and writes the fastest setting as
 JSON.  The column move is swept first with the sort settings fixed, then the
 sort grid with the best move.  Results never depend on the tuning, only the
@@ -63,7 +64,8 @@ def tune(ctx, run, grid=GRID, gather_kb=GATHER_KB, reps=3, log=None):
     base = {k: v for k, v in ctx.tuning().items()}
     # how round 2's column reaches round-1 order: carried by round 1's
     # radix passes, or gathered (in one pass or source windows)
-    moves = [dict(carry_next=1, gather_window_kb=0)] + [dict(carry_next=0, gather_window_kb=g) for g in gather_kb]
+    moves = [dict(carry_next=1, gather_window_kb=0), dict(carry_next=2, gather_window_kb=0)] + \
+        [dict(carry_next=0, gather_window_kb=g) for g in gather_kb]
     mv = min(moves, key=lambda m: one(dict(base, **m)))
     for st in candidates(grid):
         one(dict(st, **mv))

@@ -276,12 +276,15 @@ int klaunch(fmcs_ctx* ctx, const char* name, void (*kern)(KA...), u64 grid, doub
 
 // Segmented tile-offset scan of the per-tile digit counts (written by
 // kd_seg_hist into tcount over the same tiling) and the scatter pass.
-// A payload column (pin -> pout) moves with the pairs unless P is NoPay.
-template <class K, int RB = RADIX_BITS, class P = NoPay>
+// A payload column (pin -> pout) moves with the pairs unless P is NoPay;
+// FUSE writes (key << fshift) | payload composites to pout instead of keys.
+// kside (no payload): also write every key >> fshift as a u32.
+template <class K, int RB = RADIX_BITS, class P = NoPay, bool FUSE = false>
 int scatter_pass(fmcs_ctx* ctx, const char* name, const K* kin, const u32* vin, K* kout, u32* vout, const u32* lo,
                  const u32* hi, const u32* tbase, const u32* nsegs_d, const u32* ntiles_d, u64 ntiles_max,
                  const u32* dbase, u32* tcount, u32 shift, u32 dbits, double bytes, const u32* dcount, double per,
-                 const P* pin = nullptr, P* pout = nullptr) {
+                 const P* pin = nullptr, PayOut<P, FUSE>* pout = nullptr, u32 fshift = 0, u64 pbad = 0,
+                 u32* err = nullptr, u32* kside = nullptr) {
   constexpr u32 R = 1u << RB;
   const u64 nch = std::max<u64>(1, (ntiles_max + TSCAN_CT - 1) / TSCAN_CT);
   u32* carry;
@@ -290,10 +293,10 @@ int scatter_pass(fmcs_ctx* ctx, const char* name, const K* kin, const u32* vin, 
   TRY(klaunch(ctx, "k_tile_scan", kd_tscan_local<RB>, cap_grid(nch, 2048), 0.0, nullptr, 0.0, tcount, tbase, nsegs_d,
               ntiles_d, carry, cflag));
   TRY(klaunch(ctx, "k_tile_scan", kd_tscan_carry<RB>, R, 0.0, nullptr, 0.0, carry, (const u32*)cflag, ntiles_d));
-  return klaunch(ctx, name, k_onesweep<K, ONESWEEP_ITEMS, RB, P>,
-                 onesweep_grid(ctx, (const void*)k_onesweep<K, ONESWEEP_ITEMS, RB, P>, ntiles_max), bytes, dcount, per,
-                 kin, vin, kout, vout, lo, hi, tbase, 0u, dbase, (const u32*)tcount, (const u32*)carry, shift, dbits,
-                 nsegs_d, ntiles_d, pin, pout);
+  auto kern = k_onesweep<K, ONESWEEP_ITEMS, RB, P, FUSE>;
+  return klaunch(ctx, name, kern, onesweep_grid(ctx, (const void*)kern, ntiles_max), bytes, dcount, per, kin, vin,
+                 kout, vout, lo, hi, tbase, 0u, dbase, (const u32*)tcount, (const u32*)carry, shift, dbits, nsegs_d,
+                 ntiles_d, pin, pout, fshift, pbad, err, kside);
 }
 
 // exclusive scan of n u32 values -> out; *total_h (host) if non-null
@@ -434,15 +437,15 @@ int merge_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, 
 // and offsets; the group count goes to *count_d (device).
 template <class K>
 int boundary_launch(fmcs_ctx* ctx, const K* v, const u32* in_group, u64 n, u32* out_group, u32* seg_off,
-                    u32* count_d) {
+                    u32* count_d, u32 shift = 0) {
   const u64 ntiles = (n + BOUNDARY_TILE - 1) / BOUNDARY_TILE;
   u32* tc;
   TRY(getbuf_t(ctx, WS_BCNT, 2 * ntiles, &tc));
   const double rd = (double)n * (sizeof(K) + (in_group ? 4 : 0));
-  TRY(klaunch(ctx, "k_bcount", k_bcount<K, BOUNDARY_ITEMS>, ntiles, rd, nullptr, 0.0, v, in_group, n, tc));
+  TRY(klaunch(ctx, "k_bcount", k_bcount<K, BOUNDARY_ITEMS>, ntiles, rd, nullptr, 0.0, v, in_group, n, shift, tc));
   TRY(scan_excl(ctx, tc, tc + ntiles, ntiles, nullptr));
   return klaunch(ctx, "k_bwrite", k_bwrite<K, BOUNDARY_ITEMS>, ntiles, rd + (out_group ? 4.0 * n : 0.0), nullptr,
-                 0.0, v, in_group, n, (const u32*)(tc + ntiles), out_group, seg_off, count_d);
+                 0.0, v, in_group, n, shift, (const u32*)(tc + ntiles), out_group, seg_off, count_d);
 }
 
 // boundary: dense ids + optional segment offsets; returns count on host
@@ -892,9 +895,76 @@ int radix_sort_dd(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vou
 }
 
 template <class K>
-int boundary_dd(fmcs_ctx* ctx, const K* v, const u32* in_group, u64 n, u32* out_group, u32* seg_off, u32* count_d) {
+int boundary_dd(fmcs_ctx* ctx, const K* v, const u32* in_group, u64 n, u32* out_group, u32* seg_off, u32* count_d,
+                u32 shift = 0) {
   if (n == 0) return FMCS_OK;
-  return boundary_launch<K>(ctx, v, in_group, n, out_group, seg_off, count_d);
+  return boundary_launch<K>(ctx, v, in_group, n, out_group, seg_off, count_d, shift);
+}
+
+// Round 1 with round 2's u64 column packed beside it: one LSD sort of
+// (c1 << pbits) | c2 composites on their c1 bits.  The first pass reads
+// both columns and writes the composites; the later passes move u64
+// composites and oids (12 bytes a row against 16 for a key, an oid and a
+// carried u64).  With more than one pass, the last one also writes the sorted
+// c1 keys to side (u32), which a boundary pass reads at half the bytes of
+// the composites.  Keys above keybits, and c2 values above pbits, set
+// dd[DD_ERR].  Needs n >= 2, keybits >= 1 and keybits + pbits <= 64.
+template <class K>
+int radix_sort_dd_fused(fmcs_ctx* ctx, const K* kin, const u64* pin, u32 keybits, u32 pbits, u64* cout, u32* vout,
+                        u32* side, u64 n, u32* dd) {
+  constexpr u32 RB = RADIX_BITS, R = 1u << RB;
+  const u32 npasses = (keybits + RB - 1) / RB;
+  const u64 ntiles = (n + ONESWEEP_TILE - 1) / ONESWEEP_TILE;
+  u32 *hist, *dbase, *tcount, *t0v, *t1v;
+  u64 *t0c = nullptr, *t1c = nullptr;
+  TRY(getbuf_t(ctx, WS_HIST, (u64)npasses * R, &hist));
+  TRY(getbuf_t(ctx, WS_DBASE, (u64)npasses * R, &dbase));
+  TRY(getbuf_t(ctx, WS_TCOUNT, ntiles * R, &tcount));
+  TRY(getbuf_t(ctx, WS_T0V, n, &t0v));
+  TRY(getbuf_t(ctx, WS_T1V, n, &t1v));
+  if (npasses > 1) TRY(getbuf_t(ctx, WS_T0P, n, &t0c));
+  if (npasses > 2) TRY(getbuf_t(ctx, WS_T1P, n, &t1c));
+  TRY(fill(ctx, hist, (u64)npasses * R, 0));
+  u32* cst = dd + DD_CONST;
+  TRY(klaunch(ctx, "k_set", kd_set4, 1, 0.0, nullptr, 0.0, cst, 0u, (u32)n, 0u, (u32)ntiles));
+  TRY(klaunch(ctx, "k_set", kd_set4, 1, 0.0, nullptr, 0.0, cst + 4, 1u, 0u, 0u, 0u));
+  const u32* one = cst + 4;
+  const u64 kbad = keybits < 8 * sizeof(K) ? ~((1ull << keybits) - 1) : 0ull;
+  const u64 pbad = pbits < 64 ? ~((1ull << pbits) - 1) : 0ull;
+  const u64* sc = nullptr;
+  const u32* sv = nullptr;
+  for (u32 p = 0; p < npasses; ++p) {
+    const bool last = (p + 1 == npasses);
+    u64* dc = last ? cout : ((p & 1) ? t1c : t0c);
+    u32* dv = last ? vout : ((p & 1) ? t1v : t0v);
+    const u32 dbits = std::min<u32>(RB, keybits - p * RB);
+    u32* hp = hist + (u64)p * R;
+    if (p == 0) {
+      TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<K, ONESWEEP_ITEMS, RB>, cap_grid(ntiles, 2048),
+                  (double)n * sizeof(K), nullptr, 0.0, kin, (const u32*)cst, (const u32*)cst + 1,
+                  (const u32*)cst + 2, one, (const u32*)cst + 3, 0u, dbits, hp, tcount, kbad, dd + DD_ERR));
+    } else {
+      TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<u64, ONESWEEP_ITEMS, RB>, cap_grid(ntiles, 2048), (double)n * 8,
+                  nullptr, 0.0, sc, (const u32*)cst, (const u32*)cst + 1, (const u32*)cst + 2, one,
+                  (const u32*)cst + 3, pbits + p * RB, dbits, hp, tcount, 0ull, (u32*)nullptr));
+    }
+    TRY(klaunch(ctx, "k_digit_base", kd_digit_base<RB>, 1, 0.0, nullptr, 0.0, (const u32*)hp, (const u32*)cst,
+                dbase + (u64)p * R, (u32*)nullptr, (u32*)nullptr, one));
+    if (p == 0) {
+      TRY((scatter_pass<K, RB, u64, true>(ctx, "k_onesweep", kin, (const u32*)nullptr, (K*)nullptr, dv, cst, cst + 1,
+                                          cst + 2, one, cst + 3, ntiles, dbase, tcount, 0u, dbits,
+                                          (double)n * (sizeof(K) + 8 + 8 + 4), nullptr, 0.0, pin, dc, pbits, pbad,
+                                          dd + DD_ERR)));
+    } else {
+      TRY(scatter_pass<u64>(ctx, "k_onesweep", sc, sv, dc, dv, cst, cst + 1, cst + 2, one, cst + 3, ntiles,
+                            dbase + (u64)p * R, tcount, pbits + p * RB, dbits,
+                            2.0 * n * (8 + 4) + (last ? 4.0 * n : 0.0), nullptr, 0.0, (const NoPay*)nullptr,
+                            (NoPay*)nullptr, pbits, 0ull, (u32*)nullptr, last ? side : (u32*)nullptr));
+    }
+    sc = dc;
+    sv = dv;
+  }
+  return FMCS_OK;
 }
 
 struct DdLevel {
@@ -1143,8 +1213,12 @@ int mrs_dd(fmcs_ctx* ctx, u32 nrounds, const void* const* cols, const u32* banks
   TRY(getbuf_t(ctx, WS_SEGOFF2, n + 1, &offb));
   // The first round's scatter passes can carry the second round's column
   // along: it then arrives in round-1 order with no gather (a coalesced
-  // read and write per pass instead of one random read per row).
+  // read and write per pass instead of one random read per row).  A u64
+  // second column that fits beside a first column of at most 32 bits is
+  // packed with it into one u64 composite key (carry_next 1).
   void* carried = nullptr;
+  const bool fused = ctx->tune.carry_next == 1 && nrounds >= 2 && n > 1 && banks[0] <= 4 && banks[1] == 8 &&
+                     bw[0] + bw[1] <= 64;
   if (ctx->tune.carry_next && nrounds >= 2 && n > 1) TRY(getbuf(ctx, WS_PAY, (size_t)n * banks[1], &carried));
   const u32* gin = nullptr;
   const u32* soff = nullptr;
@@ -1157,7 +1231,19 @@ int mrs_dd(fmcs_ctx* ctx, u32 nrounds, const void* const* cols, const u32* banks
     u32* oout = (soff == offa) ? offb : offa;
     void* kc = (r == 1) ? carried : nullptr;
     int rc;
-    if (r == 0 && carried) {
+    if (r == 0 && fused) {
+      u64* comp = (u64*)carried;
+      u32* side;
+      TRY(getbuf_t(ctx, WS_KA, n, &side));
+      rc = banks[0] == 2 ? radix_sort_dd_fused<u16>(ctx, (const u16*)cols[0], (const u64*)cols[1], bw[0], bw[1],
+                                                    comp, order, side, n, dd)
+                         : radix_sort_dd_fused<u32>(ctx, (const u32*)cols[0], (const u64*)cols[1], bw[0], bw[1],
+                                                    comp, order, side, n, dd);
+      // round-1 groups: the sorted c1 keys (one pass: the composites' c1 part)
+      if (rc == FMCS_OK && want_groups)
+        rc = bw[0] > RADIX_BITS ? boundary_dd<u32>(ctx, side, nullptr, n, gout, oout, dd + 0)
+                                : boundary_dd<u64>(ctx, comp, nullptr, n, gout, oout, dd + 0, bw[1]);
+    } else if (r == 0 && carried) {
       switch (banks[0]) {
         case 2: rc = dd_round0_carry<u16>(ctx, bw[0], n, (const u16*)cols[0], order, gout, oout, want_groups, dd, banks[1], cols[1], carried); break;
         case 4: rc = dd_round0_carry<u32>(ctx, bw[0], n, (const u32*)cols[0], order, gout, oout, want_groups, dd, banks[1], cols[1], carried); break;
@@ -1387,7 +1473,7 @@ int fmcs_ctx_set_tuning(fmcs_ctx* ctx, const fmcs_tuning* t) {
   if (!ctx || !t) return FMCS_E_ARG;
   // small_max >= 64 bounds the device-driven path's segment lists by 256 / 65 entries per row
   if (t->small_max < 64 || t->small_max > (u32)TILESORT_TMAX / 2 || t->window < 1 || t->window > t->small_max + 1 ||
-      t->msd_bits < 1 || t->msd_bits > (u32)RADIX_BITS || t->gather_window_kb > (1u << 24) || t->carry_next > 1)
+      t->msd_bits < 1 || t->msd_bits > (u32)RADIX_BITS || t->gather_window_kb > (1u << 24) || t->carry_next > 2)
     return fail(ctx, FMCS_E_ARG, "tuning out of range");
   ctx->tune = *t;
   return FMCS_OK;

@@ -67,17 +67,22 @@ __global__ void k_gather(const T*, const u32*, T*, u64);
 template <class T>
 __global__ void k_gather_win(const T*, const u32*, T*, u64, u32, u32);
 template <class K, int ITEMS>
-__global__ void k_bcount(const K*, const u32*, u64, u32*);
+__global__ void k_bcount(const K*, const u32*, u64, u32, u32*);
 template <class K, int ITEMS>
-__global__ void k_bwrite(const K*, const u32*, u64, const u32*, u32*, u32*, u32*);
+__global__ void k_bwrite(const K*, const u32*, u64, u32, const u32*, u32*, u32*, u32*);
 template <class K>
 __global__ void k_hist_all(const K*, u64, u32, u32, u32*);
 __global__ void k_digit_base(const u32*, const u32*, u32*, u32*, u32*);
 // P = NoPay: no payload column
 struct NoPay {};
-template <class K, int ITEMS, int RB, class P>
+// payload output of a scatter pass: the payload, or the u64 composite
+// (key << fshift) | payload when fused
+template <class P, bool FUSE>
+using PayOut = typename std::conditional<FUSE, u64, P>::type;
+template <class K, int ITEMS, int RB, class P, bool FUSE>
 __global__ void k_onesweep(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, u32, const u32*,
-                           const u32*, const u32*, u32, u32, const u32*, const u32*, const P*, P*);
+                           const u32*, const u32*, u32, u32, const u32*, const u32*, const P*, PayOut<P, FUSE>*, u32,
+                           u64, u32*, u32*);
 template <class K, int E>
 __global__ void k_tile_sort(TileSortArgs<K>);
 template <class K, int E>

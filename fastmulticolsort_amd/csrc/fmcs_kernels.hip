@@ -132,7 +132,8 @@ __global__ __launch_bounds__(256) void k_gather(const T* __restrict__ vals, cons
 // extractGroupInfo (src/composer.cpp:630-678): dense group ids and group
 // start offsets (offsets[ngroups] = n) of a sorted column, in two passes
 // with no inter-tile waiting.  Row r is a head when
-// r == 0 or its value (or incoming group id) differs from row r - 1.
+// r == 0 or its value >> shift (or incoming group id) differs from row r - 1
+// (shift > 0: the column is the high part of a composite key).
 // k_bcount counts the heads of every tile; after an exclusive scan of the
 // counts, k_bwrite recomputes the heads and writes ids / offsets.
 // ---------------------------------------------------------------------------
@@ -142,7 +143,7 @@ __global__ __launch_bounds__(256) void k_gather(const T* __restrict__ vals, cons
 // lane's first row is the next lane down's last row; lane 0 loads it.
 template <class K, int ITEMS>
 __device__ __forceinline__ void tile_heads(const K* __restrict__ v, const u32* __restrict__ in_group, u64 n,
-                                           u64 base, bool vec_ok, u32 (&hm)[ITEMS * sizeof(K) / 16]) {
+                                           u32 shift, u64 base, bool vec_ok, u32 (&hm)[ITEMS * sizeof(K) / 16]) {
   constexpr int PV = 16 / sizeof(K), NV = ITEMS / PV;
   const u32 tid = threadIdx.x, lane = tid & (WAVE - 1);
   K k[NV][PV];
@@ -194,7 +195,7 @@ __device__ __forceinline__ void tile_heads(const K* __restrict__ v, const u32* _
     for (int i = 0; i < PV; ++i) {
       const u64 pk = i ? (u64)k[vv][i - 1] : pv;
       const u32 pgi = i ? g[vv][i - 1] : pg;
-      const bool head = (e0 + i < n) && (e0 + i == 0 || (u64)k[vv][i] != pk || g[vv][i] != pgi);
+      const bool head = (e0 + i < n) && (e0 + i == 0 || ((u64)k[vv][i] >> shift) != (pk >> shift) || g[vv][i] != pgi);
       m |= (head ? 1u : 0u) << i;
     }
     hm[vv] = m;
@@ -203,13 +204,13 @@ __device__ __forceinline__ void tile_heads(const K* __restrict__ v, const u32* _
 
 template <class K, int ITEMS>
 __global__ __launch_bounds__(256) void k_bcount(const K* __restrict__ v, const u32* __restrict__ in_group, u64 n,
-                                                u32* __restrict__ tcnt) {
+                                                u32 shift, u32* __restrict__ tcnt) {
   constexpr int NT = 256, NW = NT / WAVE, TILE = NT * ITEMS, NV = ITEMS * sizeof(K) / 16;
   __shared__ u32 s_w[NW];
   const u32 tid = threadIdx.x;
   const bool vec_ok = ((uintptr_t)v & 15) == 0 && ((uintptr_t)in_group & 15) == 0;
   u32 hm[NV];
-  tile_heads<K, ITEMS>(v, in_group, n, (u64)blockIdx.x * TILE, vec_ok, hm);
+  tile_heads<K, ITEMS>(v, in_group, n, shift, (u64)blockIdx.x * TILE, vec_ok, hm);
   u32 c = 0;
 #pragma unroll
   for (int vv = 0; vv < NV; ++vv) c += (u32)__popc(hm[vv]);
@@ -226,7 +227,7 @@ __global__ __launch_bounds__(256) void k_bcount(const K* __restrict__ v, const u
 
 template <class K, int ITEMS>
 __global__ __launch_bounds__(256) void k_bwrite(const K* __restrict__ v, const u32* __restrict__ in_group, u64 n,
-                                                const u32* __restrict__ tpre, u32* __restrict__ out_group,
+                                                u32 shift, const u32* __restrict__ tpre, u32* __restrict__ out_group,
                                                 u32* __restrict__ seg_off, u32* __restrict__ ngroups_out) {
   constexpr int NT = 256, NW = NT / WAVE, TILE = NT * ITEMS, PV = 16 / sizeof(K), NV = ITEMS / PV;
   __shared__ u32 s_cnt[NV][NW];
@@ -235,7 +236,7 @@ __global__ __launch_bounds__(256) void k_bwrite(const K* __restrict__ v, const u
   const u64 base = (u64)tile * TILE;
   const bool vec_ok = ((uintptr_t)v & 15) == 0 && ((uintptr_t)in_group & 15) == 0;
   u32 hm[NV], lex[NV];
-  tile_heads<K, ITEMS>(v, in_group, n, base, vec_ok, hm);
+  tile_heads<K, ITEMS>(v, in_group, n, shift, base, vec_ok, hm);
 #pragma unroll
   for (int vv = 0; vv < NV; ++vv) {
     const u32 c = (u32)__popc(hm[vv]);
@@ -367,8 +368,12 @@ __global__ __launch_bounds__(256) void k_digit_base(const u32* __restrict__ hist
 // A payload column P (pin -> pout, none when P is NoPay) moves with the
 // pairs: each row's payload lands where its (key, value) lands.  It goes
 // through the same LDS reorder after the pairs are written, in the LDS that
-// held them.
-template <class K, int ITEMS, int RB, class P>
+// held them.  A payload with a bit in pbad (above its declared width) sets
+// *err.  FUSE: the key and the payload leave as one u64 composite
+// (key << fshift) | payload in pout, and no keys are written to kout.
+// Without a payload, kside (if set) also gets every key >> fshift as a u32:
+// the high part of composite keys, for a boundary pass over that part alone.
+template <class K, int ITEMS, int RB, class P, bool FUSE>
 __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, const u32* __restrict__ vin,
                                                   K* __restrict__ kout, u32* __restrict__ vout,
                                                   const u32* __restrict__ seg_lo, const u32* __restrict__ seg_hi,
@@ -376,12 +381,15 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
                                                   const u32* __restrict__ digit_base, const u32* __restrict__ tile_pre,
                                                   const u32* __restrict__ carry_in, u32 shift, u32 dbits,
                                                   const u32* __restrict__ nsegs_d, const u32* __restrict__ ntiles_d,
-                                                  const P* __restrict__ pin, P* __restrict__ pout) {
+                                                  const P* __restrict__ pin, PayOut<P, FUSE>* __restrict__ pout,
+                                                  u32 fshift, u64 pbad, u32* __restrict__ err,
+                                                  u32* __restrict__ kside) {
+  using PO = PayOut<P, FUSE>;
   // RB-bit digits (R of them); every thread owns DPT consecutive digits
   constexpr int NT = 256, NW = NT / WAVE, TILE = NT * ITEMS, WITEMS = WAVE * ITEMS;
   constexpr int R = 1 << RB, DPT = R / NT;
   constexpr bool HASP = !std::is_same<P, NoPay>::value;
-  constexpr int KVB = TILE * (int)(sizeof(K) + sizeof(u32)), PB = HASP ? TILE * (int)sizeof(P) : 0;
+  constexpr int KVB = TILE * (int)(sizeof(K) + sizeof(u32)), PB = HASP ? TILE * (int)sizeof(PO) : 0;
   __shared__ u32 s_seg;
   __shared__ u32 s_whist[NW][R];
   __shared__ u32 s_tdig[R];
@@ -515,17 +523,24 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
       if (i < cnt) {
         const K k = s_keys[i];
         const u32 g = s_goff[digit_of(k, shift, mask)] + i;
-        kout[g] = k;
+        if (!FUSE) kout[g] = k;
         vout[g] = s_vals[i];
         gpos[it] = g;
       }
     }
     __syncthreads();
-    P* s_pay = (P*)s_raw;
+    PO* s_pay = (PO*)s_raw;
+    u64 bad = 0;
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
       u64 r = wbase + (u64)j * WAVE + lane;
-      if (r < rowend) s_pay[rank[j]] = pv[j];
+      if (r < rowend) {
+        bad |= (u64)pv[j] & pbad;
+        if constexpr (FUSE)
+          s_pay[rank[j]] = ((u64)keys[j] << fshift) | (u64)pv[j];
+        else
+          s_pay[rank[j]] = pv[j];
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -533,6 +548,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
       const u32 i = it * NT + tid;
       if (i < cnt) pout[gpos[it]] = s_pay[i];
     }
+    if (err && __ballot(bad != 0) && __lane_id() == 0) atomicOr(err, 1u);
   } else if (cnt == (u32)TILE) {
     // full tile: independent iterations, so the LDS reads of all rows
     // overlap
@@ -543,6 +559,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
       const u32 g = s_goff[digit_of(k, shift, mask)] + i;
       kout[g] = k;
       vout[g] = s_vals[i];
+      if (kside) kside[g] = (u32)((u64)k >> fshift);
     }
   } else {
     for (u32 i = tid; i < cnt; i += NT) {
@@ -550,6 +567,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
       u32 g = s_goff[digit_of(k, shift, mask)] + i;
       kout[g] = k;
       vout[g] = s_vals[i];
+      if (kside) kside[g] = (u32)((u64)k >> fshift);
     }
   }
   __syncthreads();
@@ -1395,16 +1413,18 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
 // ---------------------------------------------------------------------------
 // Explicit instantiations used by the launchers.
 // ---------------------------------------------------------------------------
-#define FMCS_INST_ONESWEEP(K, P)                                                                               \
-  template __global__ void k_onesweep<K, ONESWEEP_ITEMS, 8, P>(const K*, const u32*, K*, u32*, const u32*,           \
-                                                               const u32*, const u32*, u32, const u32*, const u32*,  \
-                                                               const u32*, u32, u32, const u32*, const u32*,         \
-                                                               const P*, P*);
+#define FMCS_INST_ONESWEEP_F(K, P, F)                                                                          \
+  template __global__ void k_onesweep<K, ONESWEEP_ITEMS, 8, P, F>(const K*, const u32*, K*, u32*, const u32*,        \
+                                                                  const u32*, const u32*, u32, const u32*,          \
+                                                                  const u32*, const u32*, u32, u32, const u32*,      \
+                                                                  const u32*, const P*, PayOut<P, F>*, u32, u64,    \
+                                                                  u32*, u32*);
+#define FMCS_INST_ONESWEEP(K, P) FMCS_INST_ONESWEEP_F(K, P, false)
 #define FMCS_INST_K(K)                                                                                              \
   template __global__ void k_gather<K>(const K*, const u32*, K*, u64);                                              \
   template __global__ void k_gather_win<K>(const K*, const u32*, K*, u64, u32, u32);                                \
-  template __global__ void k_bcount<K, BOUNDARY_ITEMS>(const K*, const u32*, u64, u32*);                            \
-  template __global__ void k_bwrite<K, BOUNDARY_ITEMS>(const K*, const u32*, u64, const u32*, u32*, u32*, u32*);      \
+  template __global__ void k_bcount<K, BOUNDARY_ITEMS>(const K*, const u32*, u64, u32, u32*);                       \
+  template __global__ void k_bwrite<K, BOUNDARY_ITEMS>(const K*, const u32*, u64, u32, const u32*, u32*, u32*, u32*); \
   template __global__ void k_hist_all<K>(const K*, u64, u32, u32, u32*);                                            \
   FMCS_INST_ONESWEEP(K, NoPay)                                                                                      \
   FMCS_INST_ONESWEEP(K, u16)                                                                                        \
@@ -1429,6 +1449,8 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
 
 FMCS_INST_K(u16)
 FMCS_INST_K(u32)
+FMCS_INST_ONESWEEP_F(u16, u64, true)
+FMCS_INST_ONESWEEP_F(u32, u64, true)
 template __global__ void k_hash_insert<uint8_t>(const uint8_t*, const u32*, u64, u32*, u32, u32*, u32*);
 template __global__ void k_gather<uint8_t>(const uint8_t*, const u32*, uint8_t*, u64);
 template __global__ void k_fill_random<uint8_t>(uint8_t*, u64, u64, u32);

@@ -110,15 +110,19 @@ double estimate(const fmcs_cost_model* m, double n, uint32_t ncols, const uint32
   // A carried second column makes every round-1 pass move its bank too
   // (the passes are HBM-bound: time scales with the bytes moved per row,
   // key + oid + payload against key + oid) and removes round 2's gather.
+  // Packed into u64 composites, a round-1 pass moves what a u64-bank pass
+  // moves (composite + oid).
   const bool carry = m->carry_next && R >= 2 && n > 1;
+  const bool fused = carry && m->carry_next == 1 && fmcs_bank_bytes(split[0], 0) <= 4 &&
+                     fmcs_bank_bytes(split[1], 0) == 8 && split[0] + split[1] <= 64;
   double sort_ms = 0, gather_ms = 0, extract_ms = 0;
   for (size_t r = 0; r < R; ++r) {
     int bt = bank_type(split[r]);
     if (r == 0) {
       double scale = 1.0;
-      if (carry)
+      if (carry && !fused)
         scale += (double)fmcs_bank_bytes(split[1], 0) / ((double)fmcs_bank_bytes(split[0], 0) + 4.0);
-      sort_ms += radix_ms(m, bt, split[0], n) * scale;
+      sort_ms += fused ? radix_ms(m, 2, split[0], n) : radix_ms(m, bt, split[0], n) * scale;
     } else {
       double g = std::max(1.0, n / std::min(G[r], n));
       sort_ms += single_column_cost(m, n, g, bt);

@@ -72,8 +72,12 @@ typedef struct {
   uint32_t window;           /* rows per packing window (<= small_max + 1) */
   uint32_t msd_bits;         /* digit width of one MSD level (1..8) */
   uint32_t gather_window_kb; /* 0: one gather pass; else passes over source windows of this many KiB */
-  uint32_t carry_next;       /* device-driven modes: 1 = the first round's radix passes carry the second
-                                round's column along (no gather before round 2); 0 = gather it */
+  uint32_t carry_next;       /* device-driven modes, how round 2's column reaches round-1 order: 1 = the
+                                first round's radix passes carry it along, packed with the first column
+                                into one u64 composite key when the first is at most 32 bits, the second
+                                is in a u64 bank and the two widths add up to at most 64 (this form also
+                                reports second-column values above their width); 2 = carried as a
+                                separate column; 0 = gathered after round 1 */
 } fmcs_tuning;
 
 /* ---- context -------------------------------------------------------- */

@@ -30,9 +30,10 @@ typedef struct {
   double boundary_ns[3];                       /* group boundary extraction */
   double stitch_ns_per_byte;                   /* fused stitch, per byte moved */
   double round_overhead_ms;                    /* fixed cost per round */
-  uint32_t carry_next;                         /* 1: round 2's column is carried through round 1's radix
-                                                  passes (fmcs_tuning.carry_next of the device-driven
-                                                  modes), no gather before round 2; 0: gathered */
+  uint32_t carry_next;                         /* fmcs_tuning.carry_next of the executor: 1 and 2 price
+                                                  round 2's column as carried through round 1's radix
+                                                  passes (1: packed into u64 composites where it fits),
+                                                  no gather before round 2; 0: as gathered */
 } fmcs_cost_model;
 
 int fmcs_cost_model_default(fmcs_cost_model* m);

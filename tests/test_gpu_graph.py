@@ -92,10 +92,15 @@ def test_graph_recaptures_on_tuning_and_shape_change(ctx):
 def test_carried_second_column_matches_gathered(ctx, mode):
     # Round 1's radix passes carry round 2's column (every key / payload bank
     # pair; 1, 2 and 3 passes; tiny inputs; a third round gathered after a
-    # carried second): same order and group ids as gathering the column.
+    # carried second), packed into u64 composites where the widths allow
+    # (carry_next 1: u16/u32 first column, u64 second, widths summing to at
+    # most 64, 1 to 4 passes) or as a separate column (carry_next 2): same
+    # order and group ids as gathering the column (carry_next 0).
     old = ctx.tuning()
     cases = [([4, 12], 5000), ([12, 30], 70001), ([20, 40], 1 << 17), ([30, 16], 33333), ([40, 60], 20000),
-             ([16, 2], 4097), ([8, 8], 3), ([1, 1], 2), ([20, 40, 10], 40000), ([50, 9, 33], 30001)]
+             ([16, 2], 4097), ([8, 8], 3), ([1, 1], 2), ([20, 40, 10], 40000), ([50, 9, 33], 30001),
+             ([16, 48], 50001), ([8, 56], 20000), ([31, 33], 40000), ([20, 44], 3), ([24, 40, 10], 30001),
+             ([1, 63], 4097)]
     try:
         for bws, n in cases:
             cols = []
@@ -109,7 +114,7 @@ def test_carried_second_column_matches_gathered(ctx, mode):
             head = np.r_[True, np.any(np.stack([x[1:] != x[:-1] for x in s]), axis=0)] if n > 1 else np.ones(n, bool)
             egroups = (np.cumsum(head) - 1).astype(np.uint32)
             dc = [dev(c) for c in cols]
-            for cn in (1, 0):
+            for cn in (1, 2, 0):
                 ctx.set_tuning(old["small_max"], old["window"], old["msd_bits"], old["gather_window_kb"], cn)
                 order = torch.empty(n, dtype=torch.uint32, device="cuda")
                 groups = torch.empty(n, dtype=torch.uint32, device="cuda")
@@ -187,6 +192,21 @@ def test_device_modes_wide_keys_and_single_rows(ctx, mode):
         assert np.array_equal(host(order), lex([c1, c2])), n
         eo, _, eng, _ = ob.multi_round_sort([c1, c2])
         assert ng == eng
+
+
+def test_packed_composite_reports_second_column_wider_than_declared(ctx):
+    # the packed form (20-bit u32 first column, 40-bit u64 second) checks the
+    # second column too: a 41-bit value would corrupt the first column's bits
+    n = 10000
+    c1 = datagen.uniform_bits(8, n, 20).astype(np.uint32)
+    c2 = datagen.uniform_bits(9, n, 40).astype(np.uint64)
+    c2[n // 2] |= np.uint64(1 << 40)
+    order = torch.empty(n, dtype=torch.uint32, device="cuda")
+    with pytest.raises(FmcsError):
+        ctx.multi_round_sort([dev(c1), dev(c2)], [20, 40], order=order, mode=MODE_DEVICE)
+    c2[n // 2] &= np.uint64((1 << 40) - 1)
+    ctx.multi_round_sort([dev(c1), dev(c2)], [20, 40], order=order, mode=MODE_DEVICE)
+    assert np.array_equal(host(order), lex([c1, c2]))
 
 
 def test_device_mode_reports_keys_wider_than_declared(ctx):

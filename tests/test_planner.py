@@ -86,10 +86,21 @@ def test_estimate_structure():
     assert p2["reconstruct"] > 0 and p2["extract"] > 0
     # carried through round 1's three passes instead: no gather, and each
     # u32-key pass also moves the u64 bank (16 bytes a row against 8)
-    m.carry_next = 1
+    m.carry_next = 2
     c3, p3 = planner.estimate_plan(m, n, [17, 33], [8192, 1 << 33], [0, 1], [17, 33])
     assert p3["reconstruct"] == 0 and p3["extract"] == p2["extract"]
     assert p3["sort"] - p2["sort"] == pytest.approx(n * 3 * m.radix_pass_ns[1] * 1e-6 * (8 / 8), rel=1e-9)
+    # packed into u64 composites (17 + 33 <= 64): three u64-bank passes
+    m.carry_next = 1
+    _, p5 = planner.estimate_plan(m, n, [17, 33], [8192, 1 << 33], [0, 1], [17, 33])
+    assert p5["reconstruct"] == 0
+    assert p5["sort"] - p2["sort"] == pytest.approx(n * 3 * (m.radix_pass_ns[2] - m.radix_pass_ns[1]) * 1e-6,
+                                                    rel=1e-9)
+    # too wide to pack (40 + 33 > 64): carried separately
+    _, p6 = planner.estimate_plan(m, n, [40, 33], [1 << 40, 1 << 33], [0, 1], [40, 33])
+    m.carry_next = 2
+    _, p7 = planner.estimate_plan(m, n, [40, 33], [1 << 40, 1 << 33], [0, 1], [40, 33])
+    assert p6["sort"] == pytest.approx(p7["sort"], rel=1e-12)
     # a 16-bit key carrying a u16 bank: 2 bytes more on 6
     _, q0 = planner.estimate_plan(m, n, [10, 12], [1024, 4096], [0, 1], [10, 12])
     m.carry_next = 0
