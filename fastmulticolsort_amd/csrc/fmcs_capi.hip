@@ -975,9 +975,11 @@ struct DdLevel {
 
 // Batched segmented sort of (keys, vals) within the *nseg_d segments given
 // by seg_off; results in place.  lv: DD_LVL_MAX level-state records, zero.
+// keys_out false: only vals are wanted; the tile sorts do not write the
+// sorted keys back (keys ends unspecified).
 template <class K>
 int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* seg_off, const u32* nseg_d,
-                u64 nseg_max, u32* lv) {
+                u64 nseg_max, u32* lv, bool keys_out = true) {
   if (n <= 1 || nseg_max == 0) return FMCS_OK;
   // tiles of 512, 1024 or 2048 rows (2, 4 or 8 per thread) for segments of
   // up to 256, 512 or 1024 rows
@@ -1074,13 +1076,14 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
     TileSortArgs<K> a;
     std::memset(&a, 0, sizeof(a));
     a.kin[0] = keys; a.vin[0] = vals; a.kin[1] = kb_; a.vin[1] = vb_;
-    a.kout = keys; a.vout = vals;
+    a.kout = keys_out ? keys : nullptr; a.vout = vals;
     a.tile_lo = tlo; a.tile_hi = thi; a.tile_s0 = ts0; a.tile_s1 = ts1; a.seg_start = cst;
     a.src_sel = (u32)(l & 1); a.kb = d.tkb; a.pad_seg = T; a.direct = 0; a.wlist = wlist;
     a.wcount = S + LV_WCOUNT; a.fblist = fblist; a.fbcount = S + LV_FB;
     {
       const u64 g = (persist_grid(ctx, (const void*)fn_bucket, d.nw_max) + XCDS - 1) / XCDS * XCDS;
-      const double per = 2.0 * (sizeof(K) + 4);
+      // rows read (key, oid) and written (oid, and the key when wanted)
+      const double per = keys_out ? 2.0 * (sizeof(K) + 4) : sizeof(K) + 8.0;
       TRY(klaunch(ctx, "k_tile_bucket", fn_bucket, g, 0.0, S + LV_ROWS_SMALL, per, a, fb));
     }
     {
@@ -1177,7 +1180,8 @@ int dd_round(fmcs_ctx* ctx, u32 r, u32 keybits, u64 n, const K* column, u32* ord
   } else {
     if (!carried) TRY(gather_launch<K>(ctx, column, order, ka, n));
     u32* lv = dd + DD_HDR + (u64)r * DD_LVL_MAX * DD_LVL_WORDS;
-    TRY(seg_sort_dd<K>(ctx, ka, order, n, keybits, soff, dd + (r - 1), n, lv));
+    // the sorted keys feed only this round's boundary
+    TRY(seg_sort_dd<K>(ctx, ka, order, n, keybits, soff, dd + (r - 1), n, lv, want_groups));
   }
   if (want_groups) TRY(boundary_dd<K>(ctx, ka, gin, n, gout, oout, dd + r));
   return FMCS_OK;
