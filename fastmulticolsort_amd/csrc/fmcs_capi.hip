@@ -25,7 +25,7 @@ enum Slot {
   WS_KA, WS_KB, WS_VB, WS_T0K, WS_T0V, WS_T1K, WS_T1V, WS_STATUS, WS_HIST, WS_DBASE, WS_SEGOFF, WS_SEGOFF2,
   WS_ST, WS_SZ, WS_BST, WS_BSZ, WS_FLAG, WS_POS, WS_LLO, WS_LHI, WS_LTILES, WS_TBASE, WS_TLO, WS_THI, WS_TS0,
   WS_TS1, WS_COMP, WS_COMP2, WS_GROUPTMP, WS_MISC, WS_HASH, WS_HASH2, WS_HASH3, WS_HT_TABLE, WS_HT_FIRST,
-  WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_DD, WS_SCANST, WS_FBLIST, WS_TCOUNT, WS_CARRY, WS_BCNT, WS_T0P, WS_T1P, WS_PAY, WS_USER0,
+  WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_DD, WS_SCANST, WS_FBLIST, WS_TCOUNT, WS_CARRY, WS_BCNT, WS_T0P, WS_T1P, WS_PAY, WS_SGL, WS_FBLIST2, WS_USER0,
   WS_COUNT = WS_USER0 + 24
 };
 
@@ -777,7 +777,8 @@ constexpr u32 DD_HDR = 256;  // [0, 64) group count per round, [72] key error, [
 constexpr u32 DD_ERR = 72;
 constexpr u32 DD_CONST = 76;
 constexpr u64 DD_WORDS = DD_HDR + (u64)DD_ROUNDS_MAX * DD_LVL_MAX * DD_LVL_WORDS;
-enum { LV_NSEG, LV_WCOUNT, LV_ROWS_SMALL, LV_ROWS_LARGE, LV_NLARGE, LV_NTILES, LV_SCAN_A, LV_SCAN_B, LV_OS, LV_FB };
+enum { LV_NSEG, LV_WCOUNT, LV_ROWS_SMALL, LV_ROWS_LARGE, LV_NLARGE, LV_NTILES, LV_SCAN_A, LV_SCAN_B, LV_OS, LV_FB,
+       LV_NSINGLE, LV_FB2 };
 
 // p[0 .. ceil(*cnt * mult / div) + add) = v, bounded by max_words
 int dd_fill(fmcs_ctx* ctx, u32* p, const u32* cnt, u32 mult, u32 div, u32 add, u32 v, u64 max_words) {
@@ -968,9 +969,9 @@ int radix_sort_dd_fused(fmcs_ctx* ctx, const K* kin, const u64* pin, u32 keybits
 }
 
 struct DdLevel {
-  u32 kb, sm, tkb, Wl, wpt, D, shift;
+  u32 kb, sm, tkb, Wl, wpt, D, shift, single_max;
   bool last, packable;
-  u64 nseg_max, nw_max, nlarge_max, ntiles_max;
+  u64 nseg_max, nw_max, nlarge_max, ntiles_max, nsingle_max;
 };
 
 // Batched segmented sort of (keys, vals) within the *nseg_d segments given
@@ -1003,7 +1004,11 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
       d.wpt = (ONESWEEP_TILE + d.Wl - 1) / d.Wl;
       d.nseg_max = cur_max;
       d.nw_max = L.empty() ? (n + d.Wl - 1) / d.Wl : prev_tiles * d.wpt;
-      d.nlarge_max = std::min<u64>(cur_max, n / ((u64)d.sm + 1));
+      // segments of sm + 1 .. T rows fill a tile alone ("single" windows);
+      // larger ones take an MSD level
+      d.single_max = T;
+      d.nsingle_max = std::min<u64>(cur_max, n / ((u64)d.sm + 1));
+      d.nlarge_max = std::min<u64>(cur_max, n / ((u64)d.single_max + 1));
       d.ntiles_max = std::min<u64>(n / ONESWEEP_TILE + d.nlarge_max, n);
       d.last = (kb == 0);
       if (!d.last) {
@@ -1018,8 +1023,9 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
       kb -= d.D;
     }
   }
-  u64 m_list[2] = {0, 0}, m_nw = 1, m_flag = 1, m_large = 1, m_tiles = 1;
+  u64 m_list[2] = {0, 0}, m_nw = 1, m_flag = 1, m_large = 1, m_tiles = 1, m_single = 1;
   for (size_t l = 0; l < L.size(); ++l) {
+    m_single = std::max<u64>(m_single, L[l].nsingle_max);
     m_list[l & 1] = std::max<u64>(m_list[l & 1], L[l].nseg_max + 1);
     m_nw = std::max<u64>(m_nw, L[l].nw_max);
     m_flag = std::max<u64>(m_flag, L[l].nseg_max + 1);
@@ -1052,6 +1058,10 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
   TRY(getbuf_t(ctx, WS_HIST, m_large * RADIX, &hist));
   TRY(getbuf_t(ctx, WS_DBASE, m_large * RADIX, &dbase));
   TRY(getbuf_t(ctx, WS_TCOUNT, m_tiles * RADIX, &tcount));
+  u32 *sgl, *fblist2;
+  TRY(getbuf_t(ctx, WS_SGL, 3 * m_single, &sgl));
+  TRY(getbuf_t(ctx, WS_FBLIST2, m_single, &fblist2));
+  u32 *slo = sgl, *shi = sgl + m_single, *sseg = sgl + 2 * m_single;
   // the tile kernels' grid: resident blocks of the chosen tile shape
   auto fn_bucket = T <= 512 ? k_tile_bucket<K, 2> : (T <= 1024 ? k_tile_bucket<K, 4> : k_tile_bucket<K, 8>);
   auto fn_tsort = T <= 512 ? k_tile_sort<K, 2> : (T <= 1024 ? k_tile_sort<K, 4> : k_tile_sort<K, 8>);
@@ -1069,9 +1079,9 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
     u32* csz = lsz[cur];
     TRY(klaunch(ctx, "k_windows", kd_windows, cap_grid((d.nseg_max + 2047) / 2048, 2048), 0.0, nullptr, 0.0,
                 (const u32*)cst, (const u32*)csz, nseg_p, d.Wl, d.sm, plo, ptb, d.wpt, tlo, thi, ts0, ts1, wlist,
-                S + LV_WCOUNT));
+                S + LV_WCOUNT, d.single_max, slo, shi, sseg, S + LV_NSINGLE));
     TRY(klaunch(ctx, "k_large_flags", kd_large_flags, cap_grid((d.nseg_max + 255) / 256, 1024), 0.0, nullptr, 0.0,
-                (const u32*)csz, nseg_p, d.sm, flag, S + LV_ROWS_SMALL));
+                (const u32*)csz, nseg_p, d.single_max, flag, S + LV_ROWS_SMALL));
     TRY(dd_scan(ctx, flag, pos, nseg_p, d.nseg_max, S + LV_SCAN_A, S + LV_NLARGE));
     TileSortArgs<K> a;
     std::memset(&a, 0, sizeof(a));
@@ -1095,6 +1105,20 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
       else
         TRY(klaunch(ctx, "k_tile_sort", fn_tsort, g, 0.0, nullptr, 0.0, b));
     }
+    if (d.nsingle_max) {
+      // the single-segment windows, listed in segment order
+      TileSortArgs<K> c = a;
+      c.tile_lo = slo; c.tile_hi = shi; c.tile_s0 = sseg; c.tile_s1 = sseg;
+      c.wlist = nullptr; c.wcount = S + LV_NSINGLE; c.fblist = fblist2; c.fbcount = S + LV_FB2;
+      const u64 g = (persist_grid(ctx, (const void*)fn_bucket, d.nsingle_max) + XCDS - 1) / XCDS * XCDS;
+      TRY(klaunch(ctx, "k_tile_bucket", fn_bucket, g, 0.0, nullptr, 0.0, c, fb));
+      c.wcount = nullptr;
+      const u64 g2 = persist_grid(ctx, (const void*)fn_tsort, d.nsingle_max);
+      if (!d.packable)
+        TRY(klaunch(ctx, "k_tile_rank", fn_trank, g2, 0.0, nullptr, 0.0, c));
+      else
+        TRY(klaunch(ctx, "k_tile_sort", fn_tsort, g2, 0.0, nullptr, 0.0, c));
+    }
     if (d.nlarge_max == 0) break;
     const K* skey = (l & 1) ? kb_ : keys;
     const u32* sval = (l & 1) ? vb_ : vals;
@@ -1102,7 +1126,8 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
     u32* dval = (l & 1) ? vals : vb_;
     if (d.last && !(l & 1)) break;  // every key bit consumed; large segments already in place
     TRY(klaunch(ctx, "k_large_compact", kd_large_compact, cap_grid((d.nseg_max + 255) / 256, 2048), 0.0, nullptr, 0.0,
-                (const u32*)cst, (const u32*)csz, nseg_p, d.sm, (const u32*)pos, (u32)ONESWEEP_TILE, llo, lhi, ltiles));
+                (const u32*)cst, (const u32*)csz, nseg_p, d.single_max, (const u32*)pos, (u32)ONESWEEP_TILE, llo,
+                lhi, ltiles));
     TRY(dd_scan(ctx, ltiles, tbase, S + LV_NLARGE, d.nlarge_max, S + LV_SCAN_B, S + LV_NTILES));
     if (d.last) {
       // odd level, every key bit consumed: move the large segments back

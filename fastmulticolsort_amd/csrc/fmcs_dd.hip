@@ -49,12 +49,17 @@ __device__ __forceinline__ u32 window_of_d(const u32* st, u32 i, u32 W, const u3
 // LDS atomic per wave) and appends them to the compact list with one global
 // atomic per chunk, so the list counter sees few contended updates.
 constexpr u32 WIN_CHUNK = 2048;
+// Segments of small_max + 1 .. single_max rows (single_max > small_max: a
+// whole tile) are listed apart, each one its own window: (slo, shi, sseg)
+// appended at *scount, one global atomic per wave.
 __global__ __launch_bounds__(256) void kd_windows(const u32* __restrict__ st, const u32* __restrict__ sz,
                                                   const u32* __restrict__ nseg_d, u32 W, u32 small_max,
                                                   const u32* __restrict__ plo, const u32* __restrict__ pwb, u32 wpt,
                                                   u32* __restrict__ tlo, u32* __restrict__ thi, u32* __restrict__ ts0,
                                                   u32* __restrict__ ts1, u32* __restrict__ wlist,
-                                                  u32* __restrict__ wcount) {
+                                                  u32* __restrict__ wcount, u32 single_max, u32* __restrict__ slo,
+                                                  u32* __restrict__ shi, u32* __restrict__ sseg,
+                                                  u32* __restrict__ scount) {
   __shared__ u32 s_w[WIN_CHUNK];
   __shared__ u32 s_n, s_base;
   const u32 nseg = *nseg_d;
@@ -64,17 +69,33 @@ __global__ __launch_bounds__(256) void kd_windows(const u32* __restrict__ st, co
     __syncthreads();
     for (u32 j = 0; j < WIN_CHUNK; j += 256) {
       const u64 i64 = c0 + j + threadIdx.x;
-      bool first = false;
-      u32 w = 0;
+      bool first = false, single = false;
+      u32 w = 0, i = 0, size = 0;
       if (i64 < nseg) {
-        const u32 i = (u32)i64;
-        const u32 size = sz[i];
+        i = (u32)i64;
+        size = sz[i];
         if (size <= small_max) {
           w = window_of_d(st, i, W, plo, pwb, wpt);
           first = (i == 0) || (window_of_d(st, i - 1, W, plo, pwb, wpt) != w);
           bool last = (i + 1 == nseg) || (window_of_d(st, i + 1, W, plo, pwb, wpt) != w) || (sz[i + 1] > small_max);
           if (first) { tlo[w] = st[i]; ts0[w] = i; }
           if (last) { thi[w] = st[i] + size; ts1[w] = i; }
+        } else {
+          single = size <= single_max;
+        }
+      }
+      const u64 ms = __ballot(single);
+      if (ms) {
+        const u32 leader = (u32)(__ffsll((long long)ms) - 1);
+        u32 b = 0;
+        if (lane == leader) b = atomicAdd(scount, (u32)__popcll(ms));
+        b = __shfl(b, (int)leader);
+        if (single) {
+          const u32 k = b + (u32)__popcll(ms & lanemask_lt());
+          const u32 s0 = st[i];
+          slo[k] = s0;
+          shi[k] = s0 + size;
+          sseg[k] = i;
         }
       }
       const u64 m = __ballot(first);

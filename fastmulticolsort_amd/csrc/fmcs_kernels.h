@@ -128,7 +128,7 @@ __global__ void kd_mul(const u32*, u32, u32*);
 __global__ void kd_fill(u32*, const u32*, u32, u32, u32, u32);
 __global__ void kd_offsets_to_list(const u32*, const u32*, u32*, u32*);
 __global__ void kd_windows(const u32*, const u32*, const u32*, u32, u32, const u32*, const u32*, u32, u32*, u32*,
-                           u32*, u32*, u32*, u32*);
+                           u32*, u32*, u32*, u32*, u32, u32*, u32*, u32*, u32*);
 __global__ void kd_large_flags(const u32*, const u32*, u32, u32*, u32*);
 template <int ITEMS>
 __global__ void kd_scan_excl(const u32*, u32*, const u32*, u32*, u32*, u32*);

@@ -1081,16 +1081,17 @@ __global__ __launch_bounds__(256) void k_tile_bucket(TileSortArgs<K> a, u32* __r
     const u32 hi = (u32)((u64)cnt * (xcd + 1) / XCDS);
     u32 b = (u32)((u64)cnt * xcd / XCDS) + blockIdx.x / XCDS;
     if (b >= hi) return;
+    // no window list: window b itself
     WinRows<K, E> ra, rb;
-    load_window<K, E>(a, a.wlist[b], ra);
+    load_window<K, E>(a, a.wlist ? a.wlist[b] : b, ra);
     for (;;) {
       const u32 b1 = b + per_x;
-      if (b1 < hi) load_window<K, E>(a, a.wlist[b1], rb);
+      if (b1 < hi) load_window<K, E>(a, a.wlist ? a.wlist[b1] : b1, rb);
       tile_bucket_body<K, E>(a, ra, fallback);
       __syncthreads();
       if (b1 >= hi) break;
       const u32 b2 = b1 + per_x;
-      if (b2 < hi) load_window<K, E>(a, a.wlist[b2], ra);
+      if (b2 < hi) load_window<K, E>(a, a.wlist ? a.wlist[b2] : b2, ra);
       tile_bucket_body<K, E>(a, rb, fallback);
       __syncthreads();
       if (b2 >= hi) break;
