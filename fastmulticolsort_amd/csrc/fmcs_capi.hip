@@ -1004,9 +1004,10 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
       d.wpt = (ONESWEEP_TILE + d.Wl - 1) / d.Wl;
       d.nseg_max = cur_max;
       d.nw_max = L.empty() ? (n + d.Wl - 1) / d.Wl : prev_tiles * d.wpt;
-      // segments of sm + 1 .. T rows fill a tile alone ("single" windows);
-      // larger ones take an MSD level
-      d.single_max = T;
+      // segments of sm + 1 .. 2T rows (at most the largest tile) sort alone
+      // in an LDS tile of T or 2T rows ("single" windows); larger ones take
+      // an MSD level
+      d.single_max = std::min<u32>(2 * T, (u32)TILESORT_TMAX);
       d.nsingle_max = std::min<u64>(cur_max, n / ((u64)d.sm + 1));
       d.nlarge_max = std::min<u64>(cur_max, n / ((u64)d.single_max + 1));
       d.ntiles_max = std::min<u64>(n / ONESWEEP_TILE + d.nlarge_max, n);
@@ -1066,6 +1067,11 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
   auto fn_bucket = T <= 512 ? k_tile_bucket<K, 2> : (T <= 1024 ? k_tile_bucket<K, 4> : k_tile_bucket<K, 8>);
   auto fn_tsort = T <= 512 ? k_tile_sort<K, 2> : (T <= 1024 ? k_tile_sort<K, 4> : k_tile_sort<K, 8>);
   auto fn_trank = T <= 512 ? k_tile_rank<K, 2> : (T <= 1024 ? k_tile_rank<K, 4> : k_tile_rank<K, 8>);
+  // single windows above T rows: the next tile size
+  const u32 T2 = std::min<u32>(2 * T, (u32)TILESORT_TMAX);
+  auto fn_bucket2 = T2 <= 512 ? k_tile_bucket<K, 2> : (T2 <= 1024 ? k_tile_bucket<K, 4> : k_tile_bucket<K, 8>);
+  auto fn_tsort2 = T2 <= 512 ? k_tile_sort<K, 2> : (T2 <= 1024 ? k_tile_sort<K, 4> : k_tile_sort<K, 8>);
+  auto fn_trank2 = T2 <= 512 ? k_tile_rank<K, 2> : (T2 <= 1024 ? k_tile_rank<K, 4> : k_tile_rank<K, 8>);
   TRY(klaunch(ctx, "k_offsets_to_list", kd_offsets_to_list, cap_grid((nseg_max + 255) / 256, 2048), 0.0, nullptr, 0.0,
               seg_off, nseg_d, lst[0], lsz[0]));
   const u32* plo = nullptr;
@@ -1090,6 +1096,7 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
     a.tile_lo = tlo; a.tile_hi = thi; a.tile_s0 = ts0; a.tile_s1 = ts1; a.seg_start = cst;
     a.src_sel = (u32)(l & 1); a.kb = d.tkb; a.pad_seg = T; a.direct = 0; a.wlist = wlist;
     a.wcount = S + LV_WCOUNT; a.fblist = fblist; a.fbcount = S + LV_FB;
+    a.size_lo = 0; a.size_hi = ~0u;
     {
       const u64 g = (persist_grid(ctx, (const void*)fn_bucket, d.nw_max) + XCDS - 1) / XCDS * XCDS;
       // rows read (key, oid) and written (oid, and the key when wanted)
@@ -1106,18 +1113,26 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
         TRY(klaunch(ctx, "k_tile_sort", fn_tsort, g, 0.0, nullptr, 0.0, b));
     }
     if (d.nsingle_max) {
-      // the single-segment windows, listed in segment order
+      // the single-segment windows (in segment order): up to T rows in
+      // T-row tiles, then the larger ones in 2T-row tiles; refused windows of
+      // either size go to the 2T-row radix tile sort
       TileSortArgs<K> c = a;
       c.tile_lo = slo; c.tile_hi = shi; c.tile_s0 = sseg; c.tile_s1 = sseg;
       c.wlist = nullptr; c.wcount = S + LV_NSINGLE; c.fblist = fblist2; c.fbcount = S + LV_FB2;
+      c.size_lo = 0; c.size_hi = T;
       const u64 g = (persist_grid(ctx, (const void*)fn_bucket, d.nsingle_max) + XCDS - 1) / XCDS * XCDS;
       TRY(klaunch(ctx, "k_tile_bucket", fn_bucket, g, 0.0, nullptr, 0.0, c, fb));
+      if (d.single_max > T) {
+        c.size_lo = T; c.size_hi = d.single_max;
+        const u64 gb = (persist_grid(ctx, (const void*)fn_bucket2, d.nsingle_max) + XCDS - 1) / XCDS * XCDS;
+        TRY(klaunch(ctx, "k_tile_bucket", fn_bucket2, gb, 0.0, nullptr, 0.0, c, fb));
+      }
       c.wcount = nullptr;
-      const u64 g2 = persist_grid(ctx, (const void*)fn_tsort, d.nsingle_max);
+      const u64 g2 = persist_grid(ctx, (const void*)fn_tsort2, d.nsingle_max);
       if (!d.packable)
-        TRY(klaunch(ctx, "k_tile_rank", fn_trank, g2, 0.0, nullptr, 0.0, c));
+        TRY(klaunch(ctx, "k_tile_rank", fn_trank2, g2, 0.0, nullptr, 0.0, c));
       else
-        TRY(klaunch(ctx, "k_tile_sort", fn_tsort, g2, 0.0, nullptr, 0.0, c));
+        TRY(klaunch(ctx, "k_tile_sort", fn_tsort2, g2, 0.0, nullptr, 0.0, c));
     }
     if (d.nlarge_max == 0) break;
     const K* skey = (l & 1) ? kb_ : keys;

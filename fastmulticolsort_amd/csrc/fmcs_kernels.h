@@ -46,6 +46,7 @@ struct TileSortArgs {
   const u32* wcount;  // if set: persistent loop over wlist[0 .. *wcount)
   u32* fblist;        // bucket tiles refused -> appended here (with fbcount)
   u32* fbcount;       // radix tile sort: persistent loop over fblist[0 .. *fbcount)
+  u32 size_lo, size_hi;  // bucket sort: only windows of size_lo + 1 .. size_hi rows (size_hi 0: all)
 };
 
 constexpr u32 XCDS = 8;  // accelerator dies of an MI355X; workgroup b runs on die b % XCDS

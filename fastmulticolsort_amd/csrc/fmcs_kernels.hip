@@ -805,7 +805,11 @@ __device__ __forceinline__ void load_window(const TileSortArgs<K>& a, u32 w, Win
   r.w = w;
   r.lo = a.tile_lo[w];
   r.hi = a.tile_hi[w];
-  const u32 cnt = r.hi > r.lo ? r.hi - r.lo : 0u;
+  u32 cnt = r.hi > r.lo ? r.hi - r.lo : 0u;
+  if (a.size_hi && (cnt <= a.size_lo || cnt > a.size_hi)) {
+    r.hi = r.lo;  // another launch's window: skipped
+    cnt = 0;
+  }
   const K* kin = a.kin[a.src_sel];
   const u32* vin = a.vin[a.src_sel];
 #pragma unroll

@@ -130,18 +130,21 @@ def test_carried_second_column_matches_gathered(ctx, mode):
 
 
 @pytest.mark.parametrize("mode", [MODE_DEVICE, MODE_GRAPH])
-def test_2048_row_tiles(ctx, mode):
+def test_2048_row_tiles_and_single_windows(ctx, mode):
     # small_max 1024 selects 2048-row tiles: segments of 513..1024 rows are
-    # packed into LDS tiles.  Distinct keys (bucket sort), heavy duplicates
-    # (the radix tile sort takes the refused tiles) and 60-bit keys (no room
-    # for a packed segment index: comparison rank) are all checked.
+    # packed into LDS tiles.  With small_max 512 (1024-row tiles), segments
+    # of 513..1024 and 1025..2048 rows are sorted alone in 1024- and
+    # 2048-row tiles.  Distinct keys (bucket sort), heavy duplicates (the
+    # radix tile sort takes the refused tiles) and 60-bit keys (no room for a
+    # packed segment index: comparison rank) are all checked.
     old = ctx.tuning()
     n = 1 << 18
-    g = datagen.uniform_below(21, n, n // 800).astype(np.uint32)  # ~800-row groups
-    cases = [("zipf", *datagen.config3(n, ngroups=1 << 8)[0], [20, 40]),
-             ("distinct", g, datagen.uniform_bits(22, n, 40), [20, 40]),
-             ("duplicates", g, datagen.uniform_below(23, n, 3).astype(np.uint64), [20, 40]),
-             ("wide", g, datagen.uniform_bits(24, n, 60), [20, 60])]
+    cases = [("zipf", *datagen.config3(n, ngroups=1 << 8)[0], [20, 40])]
+    for gs in (800, 1500):
+        g = datagen.uniform_below(21 + gs, n, n // gs).astype(np.uint32)
+        cases += [(f"distinct{gs}", g, datagen.uniform_bits(22, n, 40), [20, 40]),
+                  (f"duplicates{gs}", g, datagen.uniform_below(23, n, 3).astype(np.uint64), [20, 40]),
+                  (f"wide{gs}", g, datagen.uniform_bits(24, n, 60), [20, 60])]
     try:
         for name, c1, c2, bws in cases:
             c1 = c1.astype(np.uint32)
@@ -150,10 +153,10 @@ def test_2048_row_tiles(ctx, mode):
             eo, _, eng, _ = ob.multi_round_sort([c1, c2])
             assert np.array_equal(exp, eo)
             d1, d2 = dev(c1), dev(c2)
-            for w in (1024, 512):
-                ctx.set_tuning(1024, w, 8, 0, 1)
+            for sm, w in ((1024, 1024), (1024, 512), (512, 512)):
+                ctx.set_tuning(sm, w, 8, 0, 1)
                 order = torch.empty(n, dtype=torch.uint32, device="cuda")
-                log("2048-row tiles", mode, name, w)
+                log("2048-row tiles / single windows", mode, name, sm, w)
                 for rep in range(2):
                     _, ng = ctx.multi_round_sort([d1, d2], bws, order=order, mode=mode)
                     torch.cuda.synchronize()
