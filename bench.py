@@ -2,9 +2,10 @@
 groups (c1: 20-bit code in a u32 bank, c2: 40-bit uniform in a u64 bank).
 
 One step = one full two-round sort on the GPU, inputs resident in HBM:
-identity order, round-1 LSD radix sort of c1 with the oid payload, group
-boundaries, oid gather of c2, round-2 batched segmented sort of c2 within
-the c1 groups (the reference's Composer::multiRoundSorting,
+identity order, round-1 LSD radix sort of c1 with the oid payload (c2
+travelling with it, packed beside c1 in u64 composite keys, so no oid
+gather of c2 is needed), group boundaries, round-2 batched segmented sort of
+c2 within the c1 groups (the reference's Composer::multiRoundSorting,
 src/composer.cpp:48-135).  The timed steps replay the device-driven round
 as one hipGraph (--mode graph, the default); per-kernel HIP-event times come
 from a second pass over the same kernels launched eagerly.  The
@@ -30,9 +31,11 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "config3_2p27_traffic.json")
 # the bench's kernel timer names -> the kernel instance this workload runs
 PROFILE_NAME = {"k_gather": "k_gather<unsigned long>", "k_tile_bucket": "k_tile_bucket<unsigned long, 4>",
-                "k_onesweep": "k_onesweep<unsigned int, 16, 8, unsigned long>",
-                "k_onesweep_msd": "k_onesweep<unsigned long, 16, 8, NoPay>", "k_bwrite": "k_bwrite<unsigned int, 32>",
-                "k_bcount": "k_bcount<unsigned int, 32>", "k_tile_hist": "kd_seg_hist<unsigned int, 16, 8>",
+                "k_tile_bucket_2t": "k_tile_bucket<unsigned long, 8>",
+                "k_onesweep": "k_onesweep<unsigned long, 16, 8, NoPay, false, true>",
+                "k_onesweep_fuse": "k_onesweep<unsigned int, 16, 8, unsigned long, true, true>",
+                "k_onesweep_msd": "k_onesweep<unsigned long, 16, 8, NoPay, false, false>",
+                "k_bwrite": "k_bwrite<unsigned int, 32>", "k_bcount": "k_bcount<unsigned int, 32>",
                 "k_seg_hist": "kd_seg_hist<unsigned long, 16, 8>"}
 
 
@@ -115,6 +118,8 @@ def main():
     ap.add_argument("--groups-log2", type=int, default=20)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-comparators", action="store_true",
+                    help="skip the host-synchronised batched and unbinned runs (a profile of the timed kernels alone)")
     ap.add_argument("--cpu-sample-log2", type=int, default=23)
     ap.add_argument("--autotune", action="store_true",
                     help="sweep the batched sort's bin thresholds and tile shapes first and save the best")
@@ -208,7 +213,7 @@ def main():
 
     # host-synchronised batched form (one readback per level)
     batched_ms = None
-    if main_mode != MODE_BATCHED:
+    if main_mode != MODE_BATCHED and not args.no_comparators:
         step(MODE_BATCHED)
         torch.cuda.synchronize()
         tb0 = time.perf_counter()
@@ -219,15 +224,17 @@ def main():
         log(f"batched (host-synchronised): {batched_ms:.3f} ms")
 
     # comparators
-    ctx.multi_round_sort([d1, d2], bws, order=order, mode=MODE_UNBINNED)
-    torch.cuda.synchronize()
-    tu0 = time.perf_counter()
-    ctx.multi_round_sort([d1, d2], bws, order=order, mode=MODE_UNBINNED)
-    torch.cuda.synchronize()
-    unb_ms = (time.perf_counter() - tu0) * 1e3
-    log(f"unbinned: {unb_ms:.3f} ms")
+    unb_ms = None
+    if not args.no_comparators:
+        ctx.multi_round_sort([d1, d2], bws, order=order, mode=MODE_UNBINNED)
+        torch.cuda.synchronize()
+        tu0 = time.perf_counter()
+        ctx.multi_round_sort([d1, d2], bws, order=order, mode=MODE_UNBINNED)
+        torch.cuda.synchronize()
+        unb_ms = (time.perf_counter() - tu0) * 1e3
+        log(f"unbinned: {unb_ms:.3f} ms")
     plain_ms = None
-    if not args.no_plain:
+    if not args.no_plain and not args.no_comparators:
         tp0 = time.perf_counter()
         ctx.multi_round_sort([d1, d2], bws, order=order, mode=MODE_PLAIN)
         torch.cuda.synchronize()
@@ -259,10 +266,10 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name),
                      "avg_launch_ms": round(avg_ms, 5), "bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
-        "comparators": {"plain_per_round_ms": plain_ms, "unbinned_ms": round(unb_ms, 3),
+        "comparators": {"plain_per_round_ms": plain_ms, "unbinned_ms": round(unb_ms, 3) if unb_ms else None,
                         "host_synchronised_batched_ms": (round(batched_ms, 3) if batched_ms else None),
                         "batched_vs_plain": (round(plain_ms / ms, 2) if plain_ms else None),
-                        "batched_vs_unbinned": round(unb_ms / ms, 3)},
+                        "batched_vs_unbinned": round(unb_ms / ms, 3) if unb_ms else None},
         "mode": args.mode,
         "kernel_ms_per_step": round(kernel_ms, 4),
         "tuning": tuning or ctx.tuning(),

@@ -278,8 +278,9 @@ int klaunch(fmcs_ctx* ctx, const char* name, void (*kern)(KA...), u64 grid, doub
 // kd_seg_hist into tcount over the same tiling) and the scatter pass.
 // A payload column (pin -> pout) moves with the pairs unless P is NoPay;
 // FUSE writes (key << fshift) | payload composites to pout instead of keys.
-// kside (no payload): also write every key >> fshift as a u32.
-template <class K, int RB = RADIX_BITS, class P = NoPay, bool FUSE = false>
+// kside (no payload): also write every key >> fshift as a u32.  LSD: the
+// pass covers one segment (the kernel skips the tile -> segment search).
+template <class K, int RB = RADIX_BITS, class P = NoPay, bool FUSE = false, bool LSD = false>
 int scatter_pass(fmcs_ctx* ctx, const char* name, const K* kin, const u32* vin, K* kout, u32* vout, const u32* lo,
                  const u32* hi, const u32* tbase, const u32* nsegs_d, const u32* ntiles_d, u64 ntiles_max,
                  const u32* dbase, u32* tcount, u32 shift, u32 dbits, double bytes, const u32* dcount, double per,
@@ -293,7 +294,7 @@ int scatter_pass(fmcs_ctx* ctx, const char* name, const K* kin, const u32* vin, 
   TRY(klaunch(ctx, "k_tile_scan", kd_tscan_local<RB>, cap_grid(nch, 2048), 0.0, nullptr, 0.0, tcount, tbase, nsegs_d,
               ntiles_d, carry, cflag));
   TRY(klaunch(ctx, "k_tile_scan", kd_tscan_carry<RB>, R, 0.0, nullptr, 0.0, carry, (const u32*)cflag, ntiles_d));
-  auto kern = k_onesweep<K, ONESWEEP_ITEMS, RB, P, FUSE>;
+  auto kern = k_onesweep<K, ONESWEEP_ITEMS, RB, P, FUSE, LSD>;
   return klaunch(ctx, name, kern, onesweep_grid(ctx, (const void*)kern, ntiles_max), bytes, dcount, per, kin, vin,
                  kout, vout, lo, hi, tbase, 0u, dbase, (const u32*)tcount, (const u32*)carry, shift, dbits, nsegs_d,
                  ntiles_d, pin, pout, fshift, pbad, err, kside);
@@ -392,7 +393,8 @@ int radix_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, 
     TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<K, ONESWEEP_ITEMS, RADIX_BITS>, cap_grid(ntiles, 2048),
                 (double)n * sizeof(K), nullptr, 0.0, sk, cst, cst + 1, cst + 2, cst + 4, cst + 3, p * RADIX_BITS,
                 (u32)RADIX_BITS, (u32*)nullptr, tcount, (u64)0, (u32*)nullptr));
-    TRY(scatter_pass<K>(ctx, "k_onesweep", sk, sv, dk, dv, cst, cst + 1, cst + 2, cst + 4, cst + 3, ntiles,
+    TRY((scatter_pass<K, RADIX_BITS, NoPay, false, true>)(ctx, "k_onesweep", sk, sv, dk, dv, cst, cst + 1, cst + 2,
+                                                         cst + 4, cst + 3, ntiles,
                         dbase + (u64)p * RADIX, tcount, p * RADIX_BITS, (u32)RADIX_BITS, 2.0 * n * (sizeof(K) + 4),
                         nullptr, 0.0));
     sk = dk;
@@ -870,7 +872,7 @@ int radix_sort_dd_t(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* v
     TRY(klaunch(ctx, "k_digit_base", kd_digit_base<RB>, 1, 0.0, nullptr, 0.0, (const u32*)hp, (const u32*)cst,
                 dbase + (u64)p * R, (u32*)nullptr, (u32*)nullptr, one));
     const double pb = HASP ? 2.0 * n * sizeof(P) : 0.0;
-    const int rc = scatter_pass<K, RB, P>(ctx, "k_onesweep", sk, sv, dk, dv, cst, cst + 1, cst + 2, one, cst + 3,
+    const int rc = scatter_pass<K, RB, P, false, true>(ctx, "k_onesweep", sk, sv, dk, dv, cst, cst + 1, cst + 2, one, cst + 3,
                                           ntiles, dbase + (u64)p * R, tcount, p * RB, (u32)RB,
                                           2.0 * n * (sizeof(K) + 4) + pb, nullptr, 0.0, sp, dp);
     if (rc != FMCS_OK) return rc;
@@ -952,12 +954,12 @@ int radix_sort_dd_fused(fmcs_ctx* ctx, const K* kin, const u64* pin, u32 keybits
     TRY(klaunch(ctx, "k_digit_base", kd_digit_base<RB>, 1, 0.0, nullptr, 0.0, (const u32*)hp, (const u32*)cst,
                 dbase + (u64)p * R, (u32*)nullptr, (u32*)nullptr, one));
     if (p == 0) {
-      TRY((scatter_pass<K, RB, u64, true>(ctx, "k_onesweep", kin, (const u32*)nullptr, (K*)nullptr, dv, cst, cst + 1,
+      TRY((scatter_pass<K, RB, u64, true, true>(ctx, "k_onesweep_fuse", kin, (const u32*)nullptr, (K*)nullptr, dv, cst, cst + 1,
                                           cst + 2, one, cst + 3, ntiles, dbase, tcount, 0u, dbits,
                                           (double)n * (sizeof(K) + 8 + 8 + 4), nullptr, 0.0, pin, dc, pbits, pbad,
                                           dd + DD_ERR)));
     } else {
-      TRY(scatter_pass<u64>(ctx, "k_onesweep", sc, sv, dc, dv, cst, cst + 1, cst + 2, one, cst + 3, ntiles,
+      TRY((scatter_pass<u64, RB, NoPay, false, true>)(ctx, "k_onesweep", sc, sv, dc, dv, cst, cst + 1, cst + 2, one, cst + 3, ntiles,
                             dbase + (u64)p * R, tcount, pbits + p * RB, dbits,
                             2.0 * n * (8 + 4) + (last ? 4.0 * n : 0.0), nullptr, 0.0, (const NoPay*)nullptr,
                             (NoPay*)nullptr, pbits, 0ull, (u32*)nullptr, last ? side : (u32*)nullptr));
@@ -1125,7 +1127,7 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
       if (d.single_max > T) {
         c.size_lo = T; c.size_hi = d.single_max;
         const u64 gb = (persist_grid(ctx, (const void*)fn_bucket2, d.nsingle_max) + XCDS - 1) / XCDS * XCDS;
-        TRY(klaunch(ctx, "k_tile_bucket", fn_bucket2, gb, 0.0, nullptr, 0.0, c, fb));
+        TRY(klaunch(ctx, "k_tile_bucket_2t", fn_bucket2, gb, 0.0, nullptr, 0.0, c, fb));
       }
       c.wcount = nullptr;
       const u64 g2 = persist_grid(ctx, (const void*)fn_tsort2, d.nsingle_max);

@@ -38,10 +38,15 @@ __global__ void kd_offsets_to_list(const u32* __restrict__ off, const u32* __res
   }
 }
 
-__device__ __forceinline__ u32 window_of_d(const u32* st, u32 i, u32 W, const u32* plo, const u32* pwb, u32 wpt) {
-  if (!plo) return st[i] / W;
+// window_of (fmcs_kernels.hip) over device-sized lists: empty segments are
+// placed at start - 1 within their parent.
+__device__ __forceinline__ u32 window_of_d(const u32* st, const u32* sz, u32 i, u32 W, const u32* plo,
+                                           const u32* pwb, u32 wpt) {
+  const u32 s = st[i];
+  if (!plo) return ((sz[i] == 0 && s > 0) ? s - 1 : s) / W;
   u32 p = i >> RADIX_BITS;
-  return pwb[p] * wpt + (st[i] - plo[p]) / W;
+  const u32 lo = plo[p];
+  return pwb[p] * wpt + (((sz[i] == 0 && s > lo) ? s - 1 : s) - lo) / W;
 }
 
 // Window packing (see k_windows).  A block takes WIN_CHUNK consecutive
@@ -75,9 +80,9 @@ __global__ __launch_bounds__(256) void kd_windows(const u32* __restrict__ st, co
         i = (u32)i64;
         size = sz[i];
         if (size <= small_max) {
-          w = window_of_d(st, i, W, plo, pwb, wpt);
-          first = (i == 0) || (window_of_d(st, i - 1, W, plo, pwb, wpt) != w);
-          bool last = (i + 1 == nseg) || (window_of_d(st, i + 1, W, plo, pwb, wpt) != w) || (sz[i + 1] > small_max);
+          w = window_of_d(st, sz, i, W, plo, pwb, wpt);
+          first = (i == 0) || (window_of_d(st, sz, i - 1, W, plo, pwb, wpt) != w);
+          bool last = (i + 1 == nseg) || (window_of_d(st, sz, i + 1, W, plo, pwb, wpt) != w) || (sz[i + 1] > small_max);
           if (first) { tlo[w] = st[i]; ts0[w] = i; }
           if (last) { thi[w] = st[i] + size; ts1[w] = i; }
         } else {

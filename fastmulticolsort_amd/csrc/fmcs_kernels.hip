@@ -373,7 +373,9 @@ __global__ __launch_bounds__(256) void k_digit_base(const u32* __restrict__ hist
 // (key << fshift) | payload in pout, and no keys are written to kout.
 // Without a payload, kside (if set) also gets every key >> fshift as a u32:
 // the high part of composite keys, for a boundary pass over that part alone.
-template <class K, int ITEMS, int RB, class P, bool FUSE>
+// LSD: one segment [seg_lo[0], seg_hi[0]) tiled from tile 0 (no search for
+// a tile's segment).
+template <class K, int ITEMS, int RB, class P, bool FUSE, bool LSD>
 __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, const u32* __restrict__ vin,
                                                   K* __restrict__ kout, u32* __restrict__ vout,
                                                   const u32* __restrict__ seg_lo, const u32* __restrict__ seg_hi,
@@ -412,9 +414,9 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
   const u32 t_hi = (u32)((u64)ntiles * (xcd + 1) / XCDS);
   for (u32 tile = (u32)((u64)ntiles * xcd / XCDS) + blockIdx.x / XCDS; tile < t_hi; tile += per_x) {
   for (u32 i = tid; i < NW * R; i += NT) (&s_whist[0][0])[i] = 0;
-  if (tid == 0) s_seg = upper_bound_m1(tile_base, nsegs, tile);
+  if (!LSD && tid == 0) s_seg = upper_bound_m1(tile_base, nsegs, tile);
   __syncthreads();
-  const u32 seg = s_seg;
+  const u32 seg = LSD ? 0u : s_seg;
   const u32 first_tile = tile_base[seg];
   const u64 row0 = (u64)seg_lo[seg] + (u64)(tile - first_tile) * TILE;
   const u64 rowend = min((u64)seg_hi[seg], row0 + TILE);
@@ -1114,11 +1116,20 @@ __global__ __launch_bounds__(256) void k_tile_bucket(TileSortArgs<K> a, u32* __r
 // small segments of a window are contiguous and precede it.  Lists of MSD
 // buckets (RADIX per parent segment, parents not adjacent in general) use
 // windows relative to their parent: parent_wbase[p] * wpt + (start - lo[p]) / W,
-// so a window never spans the gap between two parents.
-__device__ __forceinline__ u32 window_of(const u32* st, u32 i, u32 W, const u32* plo, const u32* pwb, u32 wpt) {
-  if (!plo) return st[i] / W;
+// so a window never spans the gap between two parents.  An empty segment
+// is placed at start - 1 (never before its parent's first row): the empty
+// buckets after a parent's last row would otherwise take the window id one
+// past the parent's range (the next parent's first window, when the parent
+// fills its tiles exactly) and stretch that window back over the gap.
+__device__ __forceinline__ u32 window_pos(const u32* st, const u32* sz, u32 i, u32 lo) {
+  const u32 s = st[i];
+  return (sz[i] == 0 && s > lo) ? s - 1 : s;
+}
+__device__ __forceinline__ u32 window_of(const u32* st, const u32* sz, u32 i, u32 W, const u32* plo, const u32* pwb,
+                                         u32 wpt) {
+  if (!plo) return window_pos(st, sz, i, 0u) / W;
   u32 p = i >> RADIX_BITS;
-  return pwb[p] * wpt + (st[i] - plo[p]) / W;
+  return pwb[p] * wpt + (window_pos(st, sz, i, plo[p]) - plo[p]) / W;
 }
 
 __global__ void k_windows(const u32* __restrict__ st, const u32* __restrict__ sz, u32 nseg, u32 W, u32 small_max,
@@ -1131,9 +1142,9 @@ __global__ void k_windows(const u32* __restrict__ st, const u32* __restrict__ sz
   if (i < nseg) {
     u32 size = sz[i];
     if (size <= small_max) {
-      w = window_of(st, i, W, plo, pwb, wpt);
-      first = (i == 0) || (window_of(st, i - 1, W, plo, pwb, wpt) != w);
-      bool last = (i + 1 == nseg) || (window_of(st, i + 1, W, plo, pwb, wpt) != w) || (sz[i + 1] > small_max);
+      w = window_of(st, sz, i, W, plo, pwb, wpt);
+      first = (i == 0) || (window_of(st, sz, i - 1, W, plo, pwb, wpt) != w);
+      bool last = (i + 1 == nseg) || (window_of(st, sz, i + 1, W, plo, pwb, wpt) != w) || (sz[i + 1] > small_max);
       if (first) { tlo[w] = st[i]; ts0[w] = i; }
       if (last) { thi[w] = st[i] + size; ts1[w] = i; }
     }
@@ -1418,19 +1429,20 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
 // ---------------------------------------------------------------------------
 // Explicit instantiations used by the launchers.
 // ---------------------------------------------------------------------------
-#define FMCS_INST_ONESWEEP_F(K, P, F)                                                                          \
-  template __global__ void k_onesweep<K, ONESWEEP_ITEMS, 8, P, F>(const K*, const u32*, K*, u32*, const u32*,        \
-                                                                  const u32*, const u32*, u32, const u32*,          \
-                                                                  const u32*, const u32*, u32, u32, const u32*,      \
-                                                                  const u32*, const P*, PayOut<P, F>*, u32, u64,    \
-                                                                  u32*, u32*);
-#define FMCS_INST_ONESWEEP(K, P) FMCS_INST_ONESWEEP_F(K, P, false)
+#define FMCS_INST_ONESWEEP_F(K, P, F, L)                                                                       \
+  template __global__ void k_onesweep<K, ONESWEEP_ITEMS, 8, P, F, L>(const K*, const u32*, K*, u32*, const u32*,     \
+                                                                     const u32*, const u32*, u32, const u32*,       \
+                                                                     const u32*, const u32*, u32, u32, const u32*,   \
+                                                                     const u32*, const P*, PayOut<P, F>*, u32, u64, \
+                                                                     u32*, u32*);
+#define FMCS_INST_ONESWEEP(K, P) FMCS_INST_ONESWEEP_F(K, P, false, true)
 #define FMCS_INST_K(K)                                                                                              \
   template __global__ void k_gather<K>(const K*, const u32*, K*, u64);                                              \
   template __global__ void k_gather_win<K>(const K*, const u32*, K*, u64, u32, u32);                                \
   template __global__ void k_bcount<K, BOUNDARY_ITEMS>(const K*, const u32*, u64, u32, u32*);                       \
   template __global__ void k_bwrite<K, BOUNDARY_ITEMS>(const K*, const u32*, u64, u32, const u32*, u32*, u32*, u32*); \
   template __global__ void k_hist_all<K>(const K*, u64, u32, u32, u32*);                                            \
+  FMCS_INST_ONESWEEP_F(K, NoPay, false, false)                                                                      \
   FMCS_INST_ONESWEEP(K, NoPay)                                                                                      \
   FMCS_INST_ONESWEEP(K, u16)                                                                                        \
   FMCS_INST_ONESWEEP(K, u32)                                                                                        \
@@ -1454,8 +1466,8 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
 
 FMCS_INST_K(u16)
 FMCS_INST_K(u32)
-FMCS_INST_ONESWEEP_F(u16, u64, true)
-FMCS_INST_ONESWEEP_F(u32, u64, true)
+FMCS_INST_ONESWEEP_F(u16, u64, true, true)
+FMCS_INST_ONESWEEP_F(u32, u64, true, true)
 template __global__ void k_hash_insert<uint8_t>(const uint8_t*, const u32*, u64, u32*, u32, u32*, u32*);
 template __global__ void k_gather<uint8_t>(const uint8_t*, const u32*, uint8_t*, u64);
 template __global__ void k_fill_random<uint8_t>(uint8_t*, u64, u64, u32);

@@ -81,6 +81,7 @@ def config1(ctx, n=1 << 24):
                        reps=2)
         ok = bool(np.array_equal(order.cpu().numpy(), exp))
         rows.append({"split": split, "ms": round(ms, 3), "ok": ok})
+        print(f"config 1 split {split}: {ms:.3f} ms ok={ok}", file=sys.stderr, flush=True)
     model = planner.default_cost_model()
     card = [8192, datagen.exact_cardinality(c2)]
     _, pick, _, _ = planner.optimize(model, n, bws, card)
@@ -165,6 +166,7 @@ def config4(ctx, n=1 << 25, max_rrs=24):
         ms, _ = _timed(ctx, lambda: planner.run_plan(ctx, dcols, bws, [0, 1, 2, 3], split, mode=MODE_DEVICE,
                                                      order=order), reps=2)
         rows.append({"split": split, "ms": round(ms, 3), "ok": bool(np.array_equal(order.cpu().numpy(), exp))})
+        print(f"config 4 plan {split}: {ms:.3f} ms ok={rows[-1]['ok']}", file=sys.stderr, flush=True)
     best = min(rows, key=lambda r: r["ms"])
     _emit({"config": 4, "what": "ORDER BY over 1-, 2-round and seeded PlanEnumRRS plans, end to end",
            "rows": n, "plans": len(rows), "all_ok": all(r["ok"] for r in rows), "best": best,

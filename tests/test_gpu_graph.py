@@ -166,6 +166,35 @@ def test_2048_row_tiles_and_single_windows(ctx, mode):
         ctx.set_tuning(old["small_max"], old["window"], old["msd_bits"], old["gather_window_kb"], old["carry_next"])
 
 
+@pytest.mark.parametrize("mode", [MODE_BATCHED, MODE_DEVICE, MODE_GRAPH])
+def test_packing_window_after_a_parent_that_fills_its_tiles(ctx, mode):
+    # Regression: a second-level parent bucket of exactly 4096 rows (one
+    # whole scatter tile) whose digits leave empty buckets at its end.  Those
+    # empty buckets start at the parent's end; their packing window must stay
+    # in the parent's range, not take the next parent's first window (which
+    # would stretch that window back over the rows between the two parents,
+    # past the LDS tile).
+    rng = np.random.default_rng(1)
+
+    def part(m, top, lim):
+        return ((np.uint64(top) << np.uint64(32)) | (rng.integers(0, lim, m).astype(np.uint64) << np.uint64(24))
+                | rng.integers(0, 1 << 24, m).astype(np.uint64))
+
+    for lead in (0, 5):  # lead: rows of a small group before the first parent
+        c2 = np.concatenate([rng.integers(0, 1 << 40, lead).astype(np.uint64), part(4096, 0x10, 0x80),
+                             rng.integers(0, 1 << 40, 1000).astype(np.uint64), part(3000, 0x20, 0x100)])
+        c1 = np.concatenate([np.full(lead, 0), np.full(4096, 1), np.full(1000, 2), np.full(3000, 3)]).astype(np.uint32)
+        n = c1.size
+        order = torch.empty(n, dtype=torch.uint32, device="cuda")
+        log("packing window after a tile-filling parent", mode, lead)
+        _, ng = ctx.multi_round_sort([dev(c1), dev(c2)], [20, 40], order=order, mode=mode)
+        torch.cuda.synchronize()
+        exp = lex([c1, c2])
+        assert np.array_equal(host(order), exp)
+        eo, _, eng, _ = ob.multi_round_sort([c1, c2])
+        assert np.array_equal(host(order), eo) and ng == eng
+
+
 @pytest.mark.parametrize("mode", [MODE_DEVICE, MODE_GRAPH])
 def test_device_modes_four_rounds_config4_shape(ctx, mode):
     n = 1 << 17
