@@ -366,10 +366,50 @@ int fmcs_calibrate(fmcs_ctx* ctx, uint64_t nrows, fmcs_cost_model* out) {
         CalArgs* q = (CalArgs*)x;
         int r = reset_pairs(c, q);
         return r ? r : fmcs_segmented_sort(c, q->bank, q->bits, q->n, q->keys, q->oids, q->off, q->nseg,
-                                           FMCS_MODE_BATCHED, nullptr);
+                                           FMCS_MODE_DEVICE, nullptr);
       }, &a);
       out->segsort_ns[b][i] = std::max(1e-6, t - t_copy) * 1e6 / n;
     }
+  }
+  // Fixed costs (launches, list set-up) of the device-driven plans, from
+  // sorts of a few rows: per LSD pass (a 40-bit against a 16-bit first
+  // round: 5 passes against 2), per level of a later round (a 48-bit against
+  // a 16-bit second round: 7 levels against 3) and per round (two rounds
+  // against one, less the second round's levels).
+  {
+    a.n = 4096;
+    if ((rc = fmcs::internal::fill_random(ctx, k0, 8, a.n, 31, 40)) || (rc = fmcs::internal::fill_random(ctx, k, 8, a.n, 32, 48)) ||
+        (rc = fmcs::internal::fill_random(ctx, v2, 8, a.n, 33, 16)))
+      return rc;
+    // u64 banks: keys0 40-bit, keys 48-bit, vals2 16-bit values
+    struct Run {
+      static int go(fmcs_ctx* c, CalArgs* q, uint32_t rounds, const void* c0, uint32_t w0, const void* c1, uint32_t w1) {
+        const void* cols[2] = {c0, c1};
+        const uint32_t bk[2] = {8, 8}, bw[2] = {w0, w1};
+        return fmcs_multi_round_sort(c, rounds, cols, bk, bw, q->n, q->oids, nullptr, FMCS_MODE_DEVICE, nullptr);
+      }
+    };
+    double t20 = time_ms(ctx, 5, [](fmcs_ctx* c, void* x) {
+      CalArgs* q = (CalArgs*)x;
+      return Run::go(c, q, 1, q->vals2, 16, nullptr, 0);
+    }, &a);
+    double t40 = time_ms(ctx, 5, [](fmcs_ctx* c, void* x) {
+      CalArgs* q = (CalArgs*)x;
+      return Run::go(c, q, 1, q->keys0, 40, nullptr, 0);
+    }, &a);
+    double t2a = time_ms(ctx, 5, [](fmcs_ctx* c, void* x) {
+      CalArgs* q = (CalArgs*)x;
+      return Run::go(c, q, 2, q->vals2, 16, q->vals2, 16);
+    }, &a);
+    double t2b = time_ms(ctx, 5, [](fmcs_ctx* c, void* x) {
+      CalArgs* q = (CalArgs*)x;
+      return Run::go(c, q, 2, q->vals2, 16, q->keys, 48);
+    }, &a);
+    out->pass_overhead_ms = std::max(0.001, (t40 - t20) / 3.0);  // 5 passes against 2
+    out->level_overhead_ms = std::max(0.002, (t2b - t2a) / 4.0);  // 7 levels against 3
+    out->round_overhead_ms = std::max(0.005, t2a - t20 - 3.0 * out->level_overhead_ms);
+    if ((rc = fmcs_ctx_sync(ctx))) return rc;
+    a.n = n;
   }
   // stitch: two u32 sources into one u64 bank
   {

@@ -123,9 +123,14 @@ double estimate(const fmcs_cost_model* m, double n, uint32_t ncols, const uint32
       if (carry && !fused)
         scale += (double)fmcs_bank_bytes(split[1], 0) / ((double)fmcs_bank_bytes(split[0], 0) + 4.0);
       sort_ms += fused ? radix_ms(m, 2, split[0], n) : radix_ms(m, bt, split[0], n) * scale;
+      sort_ms += ((split[0] + 7) / 8) * m->pass_overhead_ms;
     } else {
       double g = std::max(1.0, n / std::min(G[r], n));
       sort_ms += single_column_cost(m, n, g, bt);
+      // the device-driven level schedule: one level per 8 key bits and a last
+      // one, unless no segment can exceed a tile (one level)
+      const double levels = n <= 2048.0 ? 1.0 : (double)((split[r] + 7) / 8 + 1);
+      sort_ms += levels * m->level_overhead_ms;
       if (!(carry && r == 1)) gather_ms += n * m->gather_ns[bt] * 1e-6;
     }
     if (r + 1 < R) extract_ms += n * m->boundary_ns[bt] * 1e-6;
@@ -455,6 +460,8 @@ int fmcs_cost_model_default(fmcs_cost_model* m) {
   }
   m->stitch_ns_per_byte = 1.0 / 5000.0;  // 5 TB/s
   m->round_overhead_ms = 0.05;
+  m->pass_overhead_ms = 0.015;
+  m->level_overhead_ms = 0.05;
   m->carry_next = 1;
   return FMCS_OK;
 }

@@ -25,7 +25,8 @@ import numpy as np
 from . import _lib, planner
 
 MAX_ROUNDS = 6
-NFEAT = 4 + 5 * MAX_ROUNDS
+PER_ROUND = 7
+NFEAT = 4 + PER_ROUND * MAX_ROUNDS
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
@@ -97,10 +98,14 @@ def features(n, bws, cards, col_order, split, rel_est=0.0):
     f[3] = max(-4.0, min(4.0, rel_est)) / 4.0
     G = round_groups(n, bws, cards, col_order, split)
     for r, w in enumerate(split[:MAX_ROUNDS]):
-        b = 4 + 5 * r
+        b = 4 + PER_ROUND * r
         f[b] = w / 64.0
         f[b + 1 + bank_type(w)] = 1.0
         f[b + 4] = math.log2(max(1.0, n / max(1.0, G[r]))) / 24.0
+        # fixed-cost drivers: LSD passes (round 1) or segmented-sort levels,
+        # and keys too wide for a packed segment index (comparison rank)
+        f[b + 5] = ((w + 7) // 8 + (1 if r else 0)) / 9.0
+        f[b + 6] = 1.0 if w > 53 else 0.0
     return f
 
 

@@ -15,6 +15,7 @@ class CostModel(C.Structure):
     _fields_ = [("radix_pass_ns", C.c_double * 3), ("segsort_ns", (C.c_double * GB) * 3),
                 ("gather_ns", C.c_double * 3), ("boundary_ns", C.c_double * 3),
                 ("stitch_ns_per_byte", C.c_double), ("round_overhead_ms", C.c_double),
+                ("pass_overhead_ms", C.c_double), ("level_overhead_ms", C.c_double),
                 ("carry_next", C.c_uint32)]
 
     def to_dict(self):
@@ -22,6 +23,7 @@ class CostModel(C.Structure):
                 "segsort_ns": [list(r) for r in self.segsort_ns],
                 "gather_ns": list(self.gather_ns), "boundary_ns": list(self.boundary_ns),
                 "stitch_ns_per_byte": self.stitch_ns_per_byte, "round_overhead_ms": self.round_overhead_ms,
+                "pass_overhead_ms": self.pass_overhead_ms, "level_overhead_ms": self.level_overhead_ms,
                 "carry_next": self.carry_next}
 
     @classmethod
@@ -35,6 +37,8 @@ class CostModel(C.Structure):
                 m.segsort_ns[i][j] = d["segsort_ns"][i][j]
         m.stitch_ns_per_byte = d["stitch_ns_per_byte"]
         m.round_overhead_ms = d["round_overhead_ms"]
+        m.pass_overhead_ms = d.get("pass_overhead_ms", 0.015)
+        m.level_overhead_ms = d.get("level_overhead_ms", 0.05)
         m.carry_next = d.get("carry_next", 1)
         return m
 

@@ -89,7 +89,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--requests", type=int, default=10000)
     ap.add_argument("--batch", type=int, default=16)
-    ap.add_argument("--train", type=int, default=96)
+    ap.add_argument("--train", type=int, default=160)
     ap.add_argument("--eval", type=int, default=48)
     ap.add_argument("--max-plans", type=int, default=24)
     ap.add_argument("--train-max-log2", type=int, default=22)

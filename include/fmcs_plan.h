@@ -30,6 +30,9 @@ typedef struct {
   double boundary_ns[3];                       /* group boundary extraction */
   double stitch_ns_per_byte;                   /* fused stitch, per byte moved */
   double round_overhead_ms;                    /* fixed cost per round */
+  double pass_overhead_ms;                     /* fixed cost per round-1 LSD pass */
+  double level_overhead_ms;                    /* fixed cost per level of a later round's segmented sort
+                                                  (one level per 8 key bits, plus one) */
   uint32_t carry_next;                         /* fmcs_tuning.carry_next of the executor: 1 and 2 price
                                                   round 2's column as carried through round 1's radix
                                                   passes (1: packed into u64 composites where it fits),
