@@ -25,7 +25,7 @@ enum Slot {
   WS_KA, WS_KB, WS_VB, WS_T0K, WS_T0V, WS_T1K, WS_T1V, WS_STATUS, WS_HIST, WS_DBASE, WS_SEGOFF, WS_SEGOFF2,
   WS_ST, WS_SZ, WS_BST, WS_BSZ, WS_FLAG, WS_POS, WS_LLO, WS_LHI, WS_LTILES, WS_TBASE, WS_TLO, WS_THI, WS_TS0,
   WS_TS1, WS_COMP, WS_COMP2, WS_GROUPTMP, WS_MISC, WS_HASH, WS_HASH2, WS_HASH3, WS_HT_TABLE, WS_HT_FIRST,
-  WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_DD, WS_SCANST, WS_FBLIST, WS_TCOUNT, WS_CARRY, WS_BCNT, WS_T0P, WS_T1P, WS_PAY, WS_SGL, WS_FBLIST2, WS_USER0,
+  WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_DD, WS_SCANST, WS_FBLIST, WS_TCOUNT, WS_CARRY, WS_BCNT, WS_T0P, WS_T1P, WS_PAY, WS_SGL, WS_FBLIST2, WS_RKL, WS_RKL2, WS_USER0,
   WS_COUNT = WS_USER0 + 24
 };
 
@@ -780,7 +780,7 @@ constexpr u32 DD_ERR = 72;
 constexpr u32 DD_CONST = 76;
 constexpr u64 DD_WORDS = DD_HDR + (u64)DD_ROUNDS_MAX * DD_LVL_MAX * DD_LVL_WORDS;
 enum { LV_NSEG, LV_WCOUNT, LV_ROWS_SMALL, LV_ROWS_LARGE, LV_NLARGE, LV_NTILES, LV_SCAN_A, LV_SCAN_B, LV_OS, LV_FB,
-       LV_NSINGLE, LV_FB2 };
+       LV_NSINGLE, LV_FB2, LV_RK, LV_RK2 };
 
 // p[0 .. ceil(*cnt * mult / div) + add) = v, bounded by max_words
 int dd_fill(fmcs_ctx* ctx, u32* p, const u32* cnt, u32 mult, u32 div, u32 add, u32 v, u64 max_words) {
@@ -1065,6 +1065,9 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
   TRY(getbuf_t(ctx, WS_SGL, 3 * m_single, &sgl));
   TRY(getbuf_t(ctx, WS_FBLIST2, m_single, &fblist2));
   u32 *slo = sgl, *shi = sgl + m_single, *sseg = sgl + 2 * m_single;
+  u32 *rkl, *rkl2;
+  TRY(getbuf_t(ctx, WS_RKL, m_nw, &rkl));
+  TRY(getbuf_t(ctx, WS_RKL2, m_single, &rkl2));
   // the tile kernels' grid: resident blocks of the chosen tile shape
   auto fn_bucket = T <= 512 ? k_tile_bucket<K, 2> : (T <= 1024 ? k_tile_bucket<K, 4> : k_tile_bucket<K, 8>);
   auto fn_tsort = T <= 512 ? k_tile_sort<K, 2> : (T <= 1024 ? k_tile_sort<K, 4> : k_tile_sort<K, 8>);
@@ -1106,13 +1109,18 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
       TRY(klaunch(ctx, "k_tile_bucket", fn_bucket, g, 0.0, S + LV_ROWS_SMALL, per, a, fb));
     }
     {
+      // refused windows: the radix tile sort over the bits that vary inside
+      // each; where even those leave no room for the segment index (keys
+      // wider than 53 bits), the comparison rank
       TileSortArgs<K> b = a;
       b.wcount = nullptr;
+      b.rklist = rkl; b.rkcount = S + LV_RK;
       const u64 g = persist_grid(ctx, (const void*)fn_tsort, d.nw_max);
-      if (!d.packable)
+      TRY(klaunch(ctx, "k_tile_sort", fn_tsort, g, 0.0, nullptr, 0.0, b));
+      if (!d.packable) {
+        b.fblist = rkl; b.fbcount = S + LV_RK;
         TRY(klaunch(ctx, "k_tile_rank", fn_trank, g, 0.0, nullptr, 0.0, b));
-      else
-        TRY(klaunch(ctx, "k_tile_sort", fn_tsort, g, 0.0, nullptr, 0.0, b));
+      }
     }
     if (d.nsingle_max) {
       // the single-segment windows (in segment order): up to T rows in
@@ -1130,11 +1138,13 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
         TRY(klaunch(ctx, "k_tile_bucket_2t", fn_bucket2, gb, 0.0, nullptr, 0.0, c, fb));
       }
       c.wcount = nullptr;
+      c.rklist = rkl2; c.rkcount = S + LV_RK2;
       const u64 g2 = persist_grid(ctx, (const void*)fn_tsort2, d.nsingle_max);
-      if (!d.packable)
+      TRY(klaunch(ctx, "k_tile_sort", fn_tsort2, g2, 0.0, nullptr, 0.0, c));
+      if (!d.packable) {
+        c.fblist = rkl2; c.fbcount = S + LV_RK2;
         TRY(klaunch(ctx, "k_tile_rank", fn_trank2, g2, 0.0, nullptr, 0.0, c));
-      else
-        TRY(klaunch(ctx, "k_tile_sort", fn_tsort2, g2, 0.0, nullptr, 0.0, c));
+      }
     }
     if (d.nlarge_max == 0) break;
     const K* skey = (l & 1) ? kb_ : keys;

@@ -47,6 +47,8 @@ struct TileSortArgs {
   u32* fblist;        // bucket tiles refused -> appended here (with fbcount)
   u32* fbcount;       // radix tile sort: persistent loop over fblist[0 .. *fbcount)
   u32 size_lo, size_hi;  // bucket sort: only windows of size_lo + 1 .. size_hi rows (size_hi 0: all)
+  u32* rklist;        // radix tile sort: windows too wide to pack -> appended here
+  u32* rkcount;       //   (with rkcount), for the comparison rank
 };
 
 constexpr u32 XCDS = 8;  // accelerator dies of an MI355X; workgroup b runs on die b % XCDS

@@ -690,6 +690,32 @@ __device__ __forceinline__ void tile_sort_one(const TileSortArgs<K>& a, u32 w) {
   }
   u32 nsegs;
   u32 run = block_excl_scan<NT>(sum, s_tmp, &nsegs);
+  // Key bits above the highest one that varies inside the window are the
+  // same in every row: the composite needs only the kbe bits up to it (the
+  // rest is restored from s_top with the segment's partitioned top bits).
+  u64 kor = 0, kand = ~0ull;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const u32 p = tid * E + e;
+    if (p < cnt) {
+      const u64 k = s_c[p] & kmask;
+      kor |= k;
+      kand &= k;
+    }
+  }
+  u64 kcor, kcand;
+  block_or_and<NT>(kor, kand, s_red, &kcor, &kcand);  // contains __syncthreads
+  const u64 kvary = kcor ^ kcand;
+  const u32 kbe = kvary ? 64u - (u32)__clzll((long long)kvary) : 0u;
+  u32 segbits = 0;
+  while (segbits < 32 && (1u << segbits) < nsegs) ++segbits;
+  if (kbe + segbits > 64) {
+    // no room beside the key for the segment index: the comparison rank
+    // (k_tile_rank over a.rklist) sorts this window
+    if (tid == 0 && a.rklist) a.rklist[atomicAdd(a.rkcount, 1u)] = w;
+    return;
+  }
+  const u64 emask = kbe >= 64 ? ~0ull : ((1ull << kbe) - 1);
   u64 vor = 0, vand = ~0ull;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
@@ -698,8 +724,8 @@ __device__ __forceinline__ void tile_sort_one(const TileSortArgs<K>& a, u32 w) {
     if (p < cnt) {
       u64 k = s_c[p];
       u32 sg = run - 1;
-      if (f[e]) s_top[sg] = k & ~kmask;
-      u64 c = ((u64)sg << kb) | (k & kmask);
+      if (f[e]) s_top[sg] = k & ~emask;
+      u64 c = (kbe >= 64 ? 0ull : ((u64)sg << kbe)) | (k & emask);
       s_c[p] = c;
       vor |= c;
       vand &= c;
@@ -708,9 +734,7 @@ __device__ __forceinline__ void tile_sort_one(const TileSortArgs<K>& a, u32 w) {
   u64 cor, cand;
   block_or_and<NT>(vor, vand, s_red, &cor, &cand);  // contains __syncthreads
   const u64 vary = cor ^ cand;
-  u32 segbits = 0;
-  while (segbits < 32 && (1u << segbits) < nsegs) ++segbits;
-  const u32 B = kb + segbits;
+  const u32 B = kbe + segbits;
   // Rows are spread over all waves: Ee items per lane, so the ranking work
   // follows the rows actually in the tile, not its capacity.
   const u32 Ee = (cnt + NT - 1) / NT;
@@ -758,9 +782,9 @@ __device__ __forceinline__ void tile_sort_one(const TileSortArgs<K>& a, u32 w) {
   }
   for (u32 i = tid; i < cnt; i += NT) {
     u64 c = s_c[i];
-    u32 sg = (u32)(c >> kb);
-    if (kb >= 64) sg = 0;
-    u64 k = s_top[sg] | (c & kmask);
+    u32 sg = (u32)(c >> kbe);
+    if (kbe >= 64) sg = 0;
+    u64 k = s_top[sg] | (c & emask);
     if (a.kout) a.kout[lo + i] = (K)k;
     a.vout[lo + i] = s_o[i];
   }

@@ -142,9 +142,17 @@ def test_2048_row_tiles_and_single_windows(ctx, mode):
     cases = [("zipf", *datagen.config3(n, ngroups=1 << 8)[0], [20, 40])]
     for gs in (800, 1500):
         g = datagen.uniform_below(21 + gs, n, n // gs).astype(np.uint32)
+        # wide keys with heavy duplicates: windows the bucket sort refuses go
+        # to the radix tile sort over the bits that vary inside them (a
+        # constant high part), or to the comparison rank (three distinct
+        # full-width values)
+        wide_dup = (np.uint64(0xABCDE) << np.uint64(40)) | datagen.uniform_below(25, n, 3).astype(np.uint64)
+        three = datagen.splitmix64(26, 3)
         cases += [(f"distinct{gs}", g, datagen.uniform_bits(22, n, 40), [20, 40]),
                   (f"duplicates{gs}", g, datagen.uniform_below(23, n, 3).astype(np.uint64), [20, 40]),
-                  (f"wide{gs}", g, datagen.uniform_bits(24, n, 60), [20, 60])]
+                  (f"wide{gs}", g, datagen.uniform_bits(24, n, 60), [20, 60]),
+                  (f"wide_dup{gs}", g, wide_dup, [20, 60]),
+                  (f"wide_dup64_{gs}", g, three[datagen.uniform_below(27, n, 3).astype(np.int64)], [20, 64])]
     try:
         for name, c1, c2, bws in cases:
             c1 = c1.astype(np.uint32)
