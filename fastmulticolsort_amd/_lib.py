@@ -29,7 +29,7 @@ EXPORTS = [
     "fmcs_extract_group_info", "fmcs_sort_pairs", "fmcs_sort_pairs_merge", "fmcs_fill_column", "fmcs_segmented_sort",
     "fmcs_sort_by_group",
     "fmcs_multi_round_sort", "fmcs_last_breakdown", "fmcs_hash_round", "fmcs_multi_round_hash",
-    "fmcs_ctx_set_profiling", "fmcs_kernel_stats", "fmcs_kernel_stats_reset",
+    "fmcs_ctx_set_profiling", "fmcs_kernel_stats", "fmcs_kernel_stats_reset", "fmcs_distinct_count",
 ]
 
 
@@ -91,6 +91,7 @@ def load():
         "fmcs_last_breakdown": (i32, [vp, P(C.c_double)]),
         "fmcs_hash_round": (i32, [vp, u32, u64, vp, vp, vp, vp, vp, vp, i32, P(u32)]),
         "fmcs_multi_round_hash": (i32, [vp, u32, P(vp), P(u32), u64, i32, vp, vp, P(u32)]),
+        "fmcs_distinct_count": (i32, [vp, u32, u64, vp, P(u64)]),
         "fmcs_ctx_set_profiling": (i32, [vp, i32]),
         "fmcs_kernel_stats": (i32, [vp, C.c_char_p, u64]),
         "fmcs_kernel_stats_reset": (i32, [vp]),

@@ -1892,6 +1892,18 @@ int fmcs_hash_round(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* values
   return FMCS_OK;
 }
 
+int fmcs_distinct_count(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* col_d, uint64_t* count) {
+  if (!ctx || !count || (n && !col_d) || !(bank == 1 || bank == 2 || bank == 4 || bank == 8))
+    return FMCS_E_ARG;
+  if (n >= (1ull << 30)) return fail(ctx, FMCS_E_UNSUPPORTED, "at most 2^30 - 1 rows");
+  u32* gid;
+  TRY(getbuf_t(ctx, WS_HASH3, std::max<u64>(n, 1), &gid));
+  u32 ng = 0;
+  TRY(hash_ids(ctx, bank, col_d, nullptr, n, gid, &ng));
+  *count = ng;
+  return FMCS_OK;
+}
+
 int fmcs_multi_round_hash(fmcs_ctx* ctx, uint32_t nrounds, const void* const* cols_d, const uint32_t* banks,
                           uint64_t n, int hash_type, uint32_t* order_d, uint32_t* gid_d, uint32_t* ngroups) {
   if (!ctx || !cols_d || !banks || !order_d || !gid_d || nrounds == 0) return FMCS_E_ARG;

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -281,6 +282,50 @@ int fmcs_run_plan(fmcs_ctx* ctx, uint32_t ncols, const void* const* cols_d, cons
   StitchComposer sc(ctx, ncols, nrows, p, cols);
   std::vector<uint32_t> sp(split, split + nsplit);
   return sc.run_single_plan(col_order, sp, false, order_d, groups_d, ms6, total_ms, ngroups);
+}
+
+int fmcs_optimize_and_run(fmcs_ctx* ctx, const fmcs_cost_model* m, uint32_t ncols, const void* const* cols_d,
+                          const uint32_t* banks, const uint32_t* bw, const uint64_t* card, int ordered, int mode,
+                          uint64_t nrows, uint32_t* order_d, uint32_t* groups_d, uint32_t* best_order,
+                          uint32_t* best_split, uint32_t* best_nsplit, double* opt_ms, double* ms6, double* total_ms,
+                          uint32_t* ngroups) {
+  if (!ctx || !cols_d || !banks || !bw || !order_d || !best_order || !best_split || !best_nsplit || ncols == 0 ||
+      ncols > 64)
+    return FMCS_E_ARG;
+  auto t0 = std::chrono::steady_clock::now();
+  fmcs_cost_model dm;
+  if (!m) {
+    fmcs_cost_model_default(&dm);
+    m = &dm;
+  }
+  // base column cardinalities (StitchComposer's constructor,
+  // src/stitch_composer.h:22-37): given, or counted on the device
+  std::vector<uint64_t> cv(ncols);
+  for (uint32_t c = 0; c < ncols; ++c) {
+    if (card) {
+      cv[c] = card[c];
+    } else {
+      int rc = fmcs_distinct_count(ctx, banks[c], nrows, cols_d[c], &cv[c]);
+      if (rc != FMCS_OK) return rc;
+    }
+    cv[c] = std::max<uint64_t>(cv[c], 1);
+  }
+  // the search may return up to ceil((2S - 2) / 16) rounds; the executor
+  // runs at most 16
+  uint32_t S = 0;
+  for (uint32_t c = 0; c < ncols; ++c) S += bw[c];
+  std::vector<uint32_t> ord(ncols), split((2 * S) / 16 + 64);
+  uint32_t ns = 0;
+  double cost = 0;
+  int rc = fmcs_optimize(m, nrows, ncols, bw, cv.data(), ordered, ord.data(), split.data(), &ns, &cost, nullptr);
+  if (rc != FMCS_OK) return set_error(ctx, rc, "plan search failed");
+  if (ns > 16) return set_error(ctx, FMCS_E_UNSUPPORTED, "best plan has more than 16 rounds");
+  if (opt_ms) *opt_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  std::copy(ord.begin(), ord.end(), best_order);
+  std::copy(split.begin(), split.begin() + ns, best_split);
+  *best_nsplit = ns;
+  return fmcs_run_plan(ctx, ncols, cols_d, banks, bw, ord.data(), ns, split.data(), mode, nrows, order_d, groups_d,
+                       ms6, total_ms, ngroups);
 }
 
 int fmcs_run_plan_hash(fmcs_ctx* ctx, uint32_t ncols, const void* const* cols_d, const uint32_t* banks,

@@ -208,6 +208,12 @@ class Context:
                                            _p(gid), _p(oo), _p(og), int(last_round), C.byref(ng)))
         return gid, oo, og, ng.value
 
+    def distinct_count(self, col):
+        """Exact number of distinct values of a device column (hash grouping)."""
+        c = C.c_uint64(0)
+        self._chk(self.lib.fmcs_distinct_count(self.h, bank_of(col), col.numel(), _p(col), C.byref(c)))
+        return c.value
+
     def multi_round_hash(self, cols, hash_type=1):
         n = cols[0].numel()
         dev = cols[0].device

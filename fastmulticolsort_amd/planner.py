@@ -68,6 +68,8 @@ def lib():
             "fmcs_run_plan_hash": [vp, u32, P(vp), P(u32), P(u32), P(u32), u32, P(u32), i32, u64, vp, vp,
                                    P(C.c_double), P(u32)],
             "fmcs_calibrate": [vp, u64, P(CostModel)],
+            "fmcs_optimize_and_run": [vp, P(CostModel), u32, P(vp), P(u32), P(u32), P(u64), i32, i32, u64, vp, vp,
+                                      P(u32), P(u32), P(u32), P(C.c_double), P(C.c_double), P(C.c_double), P(u32)],
         }
         for name, args in sigs.items():
             fn = getattr(L, name)
@@ -188,6 +190,56 @@ def run_plan(ctx, cols, bws, col_order, split, mode=0, order=None, groups=None):
                              C.byref(ng))
     ctx._chk(rc)
     return order, list(bd), tot.value, ng.value
+
+
+def optimize_and_run(ctx, cols, bws, cards=None, ordered=True, model=None, mode=0, order=None, groups=None,
+                     mlp=None):
+    """StitchComposer::optimizeAndRun (src/stitch_composer.cpp:3839-4013) on
+    device tensors: search the plans, run the best.  cards: distinct values
+    per column (None: counted on the device).  model: a cost model (None:
+    the default; calibrate(ctx) gives the device's).  mlp: a trained
+    learned.Mlp picks among the learned module's candidate plans instead of
+    the analytic search (ordered only).  Returns a dict with the order, the
+    chosen column order and split, the search time and the plan's GPU time."""
+    import time
+    import torch
+    from .engine import bank_of
+    n = cols[0].numel()
+    if order is None:
+        order = torch.empty(n, dtype=torch.uint32, device=cols[0].device)
+    k = len(cols)
+    if mlp is not None:
+        if not ordered:
+            raise ValueError("the learned picker covers ORDER BY (ordered) only")
+        from . import learned
+        t0 = time.perf_counter()
+        if cards is None:
+            cards = [ctx.distinct_count(c) for c in cols]
+        m = model if model is not None else default_cost_model()
+        ids = list(range(k))
+        plans = learned.candidate_plans(m, n, bws, cards, ids, seed=0)
+        pick, _ = learned.pick(mlp, m, n, bws, cards, ids, plans)
+        opt_ms = (time.perf_counter() - t0) * 1e3
+        _, bd, tot, ng = run_plan(ctx, cols, bws, ids, plans[pick], mode=mode, order=order, groups=groups)
+        return {"order": order, "col_order": ids, "split": list(plans[pick]), "opt_ms": opt_ms, "breakdown": bd,
+                "total_ms": tot, "ngroups": ng}
+    m = model if model is not None else default_cost_model()
+    cp = (C.c_void_p * k)(*[c.data_ptr() for c in cols])
+    bo = (C.c_uint32 * k)()
+    bs = (C.c_uint32 * 64)()
+    ns = C.c_uint32(0)
+    opt = C.c_double(0)
+    bd = (C.c_double * 6)()
+    tot = C.c_double(0)
+    ng = C.c_uint32(0)
+    rc = lib().fmcs_optimize_and_run(ctx.h, C.byref(m), k, cp, _u32([bank_of(c) for c in cols]), _u32(bws),
+                                     _u64(cards) if cards is not None else None, int(bool(ordered)), mode, n,
+                                     C.c_void_p(order.data_ptr()),
+                                     C.c_void_p(groups.data_ptr()) if groups is not None else None, bo, bs,
+                                     C.byref(ns), C.byref(opt), bd, C.byref(tot), C.byref(ng))
+    ctx._chk(rc)
+    return {"order": order, "col_order": list(bo), "split": list(bs[: ns.value]), "opt_ms": opt.value,
+            "breakdown": list(bd), "total_ms": tot.value, "ngroups": ng.value}
 
 
 def run_plan_hash(ctx, cols, bws, col_order, split, hash_type=1):

@@ -220,6 +220,13 @@ int fmcs_hash_round(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* values
 int fmcs_multi_round_hash(fmcs_ctx* ctx, uint32_t nrounds, const void* const* cols_d, const uint32_t* banks,
                           uint64_t n, int hash_type, uint32_t* order_d, uint32_t* gid_d, uint32_t* ngroups);
 
+/* Column::GetCardinality (src/column.h:18, :61, :81-88): the number of
+ * distinct values of a column, which the planner's card_per_bit statistics
+ * start from.  The reference stores it with the column when generating the
+ * data; here it is counted exactly from the data on the device (hash
+ * grouping, bank 1, 2, 4 or 8 bytes). */
+int fmcs_distinct_count(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* col_d, uint64_t* count);
+
 #ifdef __cplusplus
 }
 #endif

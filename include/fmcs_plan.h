@@ -101,6 +101,21 @@ int fmcs_run_plan_hash(fmcs_ctx* ctx, uint32_t ncols, const void* const* cols_d,
                        int hash_type, uint64_t nrows, uint32_t* order_d, uint32_t* gid_d, double* total_ms,
                        uint32_t* ngroups);
 
+/* StitchComposer::optimizeAndRun (src/stitch_composer.cpp:3839-4013): the
+ * plan search of fmcs_optimize (column orders: all permutations when
+ * ordered == 0, the GROUP BY case), costed by m (NULL: the default model;
+ * fmcs_calibrate gives the device's), then run_single_plan of the best plan,
+ * as fmcs_run_plan.  card: distinct values per column, or NULL to count them
+ * on the device (fmcs_distinct_count).  best_order (ncols entries),
+ * best_split (64 entries) and *best_nsplit receive the plan; *opt_ms the
+ * host time of the search (and of the counting), ms6 / *total_ms the plan's
+ * breakdown and GPU time. */
+int fmcs_optimize_and_run(fmcs_ctx* ctx, const fmcs_cost_model* m, uint32_t ncols, const void* const* cols_d,
+                          const uint32_t* banks, const uint32_t* bw, const uint64_t* card, int ordered, int mode,
+                          uint64_t nrows, uint32_t* order_d, uint32_t* groups_d, uint32_t* best_order,
+                          uint32_t* best_split, uint32_t* best_nsplit, double* opt_ms, double* ms6, double* total_ms,
+                          uint32_t* ngroups);
+
 /* Measures the cost model on this device with synthetic banks of nrows rows
  * (radix passes, gather, boundary, stitch, and segmented sorts over
  * groups of 2^k rows). */

@@ -6,7 +6,7 @@ import torch
 
 import oracle_bind as ob
 from fastmulticolsort_amd import datagen, planner
-from fastmulticolsort_amd.engine import MODE_BATCHED, MODE_UNBINNED
+from fastmulticolsort_amd.engine import MODE_BATCHED, MODE_DEVICE, MODE_UNBINNED
 
 pytestmark = pytest.mark.gpu
 
@@ -78,3 +78,67 @@ def test_calibrate_produces_positive_costs(ctx):
         assert d["radix_pass_ns"][b] > 0 and d["gather_ns"][b] > 0 and d["boundary_ns"][b] > 0
         assert all(x > 0 for x in d["segsort_ns"][b][:21])
     assert d["stitch_ns_per_byte"] > 0
+
+
+@pytest.mark.parametrize("dt", [np.uint8, np.uint16, np.uint32, np.uint64])
+@pytest.mark.parametrize("n", [0, 1, 5000, 100003])
+def test_distinct_count_is_exact(ctx, dt, n):
+    bits = 8 * np.dtype(dt).itemsize
+    v = datagen.uniform_bits(31 + n, n, bits).astype(dt) if n else np.zeros(0, dt)
+    if n > 10:
+        v[::3] = v[0]  # many repeats
+        v = v % dt(max(2, n // 7)) if bits > 8 else v
+    assert ctx.distinct_count(dev(v)) == np.unique(v).size
+
+
+@pytest.mark.parametrize("mode", [MODE_BATCHED, MODE_DEVICE])
+def test_optimize_and_run_order_by(ctx, mode):
+    # StitchComposer::optimizeAndRun: search, then run the best plan; with
+    # cardinalities given and counted on the device
+    n = 1 << 16
+    (c1, c2), bws = datagen.config1(n)
+    exp = np.lexsort((c2, c1)).astype(np.uint32)
+    cols = [dev(c1), dev(c2)]
+    for cards in ([8192, 1 << 33], None):
+        r = planner.optimize_and_run(ctx, cols, bws, cards=cards, ordered=True, mode=mode)
+        assert np.array_equal(r["order"].cpu().numpy(), exp)
+        assert sum(r["split"]) == sum(bws) and r["col_order"] == [0, 1]
+        assert r["opt_ms"] >= 0 and r["total_ms"] > 0
+
+
+def test_optimize_and_run_group_by(ctx):
+    # ordered = 0 (GROUP BY): the search may permute the columns; the rows
+    # come out sorted by the chosen column order, and the groups are the
+    # distinct tuples
+    n = 1 << 16
+    c0 = datagen.uniform_below(41, n, 3).astype(np.uint16)
+    c1 = datagen.uniform_below(42, n, 2000).astype(np.uint32)
+    c2 = datagen.uniform_below(43, n, 5).astype(np.uint16)
+    cols_h = [c0, c1, c2]
+    bws = [2, 11, 3]
+    groups = torch.empty(n, dtype=torch.uint32, device="cuda")
+    r = planner.optimize_and_run(ctx, [dev(c) for c in cols_h], bws, ordered=False, groups=groups)
+    co = r["col_order"]
+    assert sorted(co) == [0, 1, 2]
+    o = r["order"].cpu().numpy()
+    perm = [cols_h[c].astype(np.uint64) for c in co]
+    assert np.array_equal(o, np.lexsort(tuple(reversed(perm))).astype(np.uint32))
+    s = [p[o] for p in perm]
+    head = np.r_[True, np.any(np.stack([x[1:] != x[:-1] for x in s]), axis=0)]
+    assert np.array_equal(groups.cpu().numpy(), (np.cumsum(head) - 1).astype(np.uint32))
+    assert r["ngroups"] == int(head.sum()) == len({(a, b, c) for a, b, c in zip(c0, c1, c2)})
+
+
+def test_optimize_and_run_learned_picker(ctx):
+    # the learned picker chooses among the candidates; any choice gives the
+    # same order
+    from fastmulticolsort_amd import learned
+    n = 1 << 15
+    (c1, c2), bws = datagen.config1(n)
+    exp = np.lexsort((c2, c1)).astype(np.uint32)
+    mlp = learned.Mlp(ctx, seed=0)
+    r = planner.optimize_and_run(ctx, [dev(c1), dev(c2)], bws, mlp=mlp, mode=MODE_DEVICE)
+    assert np.array_equal(r["order"].cpu().numpy(), exp)
+    assert sum(r["split"]) == sum(bws)
+    with pytest.raises(ValueError):
+        planner.optimize_and_run(ctx, [dev(c1), dev(c2)], bws, mlp=mlp, ordered=False)
