@@ -1605,6 +1605,12 @@ int fmcs_construct_entries(uint32_t ncols, const uint32_t* bw, const uint32_t* c
 int fmcs_stitch(fmcs_ctx* ctx, uint64_t n, uint32_t nsrc, const void* const* src_d, const uint32_t* src_bank,
                 const uint32_t* src_bw, uint32_t nent, const fmcs_entry* entries, uint32_t ndst, void* const* dst_d,
                 const uint32_t* dst_bank) {
+  return fmcs_stitch_dir(ctx, n, nsrc, src_d, src_bank, src_bw, nent, entries, ndst, dst_d, dst_bank, nullptr);
+}
+
+int fmcs_stitch_dir(fmcs_ctx* ctx, uint64_t n, uint32_t nsrc, const void* const* src_d, const uint32_t* src_bank,
+                    const uint32_t* src_bw, uint32_t nent, const fmcs_entry* entries, uint32_t ndst,
+                    void* const* dst_d, const uint32_t* dst_bank, const uint8_t* src_desc) {
   if (!ctx) return FMCS_E_ARG;
   if (nsrc == 0 || nsrc > STITCH_MAX_SRC || ndst == 0 || ndst > STITCH_MAX_DST || nent == 0 || nent > STITCH_MAX_ENT)
     return fail(ctx, FMCS_E_ARG, "stitch: too many columns or entries");
@@ -1618,6 +1624,7 @@ int fmcs_stitch(fmcs_ctx* ctx, uint64_t n, uint32_t nsrc, const void* const* src
     if (((uintptr_t)src_d[s] & 15) != 0) return fail(ctx, FMCS_E_ARG, "stitch: source not 16-byte aligned");
     a.src[s] = src_d[s];
     a.src_bank[s] = src_bank[s];
+    if (src_desc && src_desc[s]) a.invert |= 1u << s;
   }
   std::vector<u32> width(ndst, 0);
   u32 prev = 0;

@@ -46,6 +46,7 @@ struct setting_t {
 // compose_params_t (src/types.h:169-186), the fields this path uses.
 struct compose_params_t {
   std::vector<uint32_t> bitwidth;
+  std::vector<uint8_t> descending;  // per column, empty: all ascending (the reference: ascending only)
   uint32_t ordered = 1;
   uint32_t hash_type = 1;
   int mode = FMCS_MODE_BATCHED;
@@ -183,8 +184,9 @@ class StitchComposer : public Composer {
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess || hipEventCreate(&e2) != hipSuccess)
       return set_error(ctx_, FMCS_E_HIP, "event creation failed");
     (void)hipEventRecord(e0, st);
-    rc = fmcs_stitch(ctx_, num_rows_, num_columns_, src.data(), sbank.data(), params_.bitwidth.data(),
-                     (uint32_t)entries.size(), entries.data(), nn, dst.data(), dbank.data());
+    rc = fmcs_stitch_dir(ctx_, num_rows_, num_columns_, src.data(), sbank.data(), params_.bitwidth.data(),
+                         (uint32_t)entries.size(), entries.data(), nn, dst.data(), dbank.data(),
+                         params_.descending.empty() ? nullptr : params_.descending.data());
     (void)hipEventRecord(e1, st);
     std::vector<std::unique_ptr<Column>> owned;
     std::vector<Column*> newcols;
@@ -268,6 +270,14 @@ extern "C" {
 int fmcs_run_plan(fmcs_ctx* ctx, uint32_t ncols, const void* const* cols_d, const uint32_t* banks, const uint32_t* bw,
                   const uint32_t* col_order, uint32_t nsplit, const uint32_t* split, int mode, uint64_t nrows,
                   uint32_t* order_d, uint32_t* groups_d, double* ms6, double* total_ms, uint32_t* ngroups) {
+  return fmcs_run_plan_dir(ctx, ncols, cols_d, banks, bw, nullptr, col_order, nsplit, split, mode, nrows, order_d,
+                           groups_d, ms6, total_ms, ngroups);
+}
+
+int fmcs_run_plan_dir(fmcs_ctx* ctx, uint32_t ncols, const void* const* cols_d, const uint32_t* banks,
+                      const uint32_t* bw, const uint8_t* descending, const uint32_t* col_order, uint32_t nsplit,
+                      const uint32_t* split, int mode, uint64_t nrows, uint32_t* order_d, uint32_t* groups_d,
+                      double* ms6, double* total_ms, uint32_t* ngroups) {
   if (!ctx || !cols_d || !banks || !bw || !col_order || !split || !order_d || ncols == 0 || ncols > 64)
     return FMCS_E_ARG;
   std::vector<std::unique_ptr<Column>> owned;
@@ -278,6 +288,7 @@ int fmcs_run_plan(fmcs_ctx* ctx, uint32_t ncols, const void* const* cols_d, cons
   }
   compose_params_t p;
   p.bitwidth.assign(bw, bw + ncols);
+  if (descending) p.descending.assign(descending, descending + ncols);
   p.mode = mode;
   StitchComposer sc(ctx, ncols, nrows, p, cols);
   std::vector<uint32_t> sp(split, split + nsplit);

@@ -22,6 +22,7 @@ struct StitchArgs {
   uint8_t e_shift[STITCH_MAX_ENT];  // bw_src - end
   uint8_t e_width[STITCH_MAX_ENT];  // end - start + 1
   u32 ndst;
+  u32 invert;  // bit s: source s sorts descending (its value bits are inverted)
   u64 n;
 };
 

@@ -103,8 +103,11 @@ __global__ __launch_bounds__(256) void k_stitch(StitchArgs a) {
       u64 v[4];
       load4(a.src[s], a.src_bank[s], base, full, a.n, v);
       u64 m = w >= 64 ? ~0ull : ((1ull << w) - 1);
+      // a descending source: its bits inverted within their width, so the
+      // ascending sort of the new column orders it from the largest value
+      const u64 flip = ((a.invert >> s) & 1u) ? m : 0ull;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[j] = (w >= 64 ? 0 : (acc[j] << w)) | ((v[j] >> sh) & m);
+      for (int j = 0; j < 4; ++j) acc[j] = (w >= 64 ? 0 : (acc[j] << w)) | (((v[j] >> sh) & m) ^ flip);
     }
     store4(a.dst[c], a.dst_bank[c], base, full, a.n, acc);
   }

@@ -65,6 +65,8 @@ def lib():
                               P(u64)],
             "fmcs_run_plan": [vp, u32, P(vp), P(u32), P(u32), P(u32), u32, P(u32), i32, u64, vp, vp,
                               P(C.c_double), P(C.c_double), P(u32)],
+            "fmcs_run_plan_dir": [vp, u32, P(vp), P(u32), P(u32), P(C.c_uint8), P(u32), u32, P(u32), i32, u64, vp,
+                                  vp, P(C.c_double), P(C.c_double), P(u32)],
             "fmcs_run_plan_hash": [vp, u32, P(vp), P(u32), P(u32), P(u32), u32, P(u32), i32, u64, vp, vp,
                                    P(C.c_double), P(u32)],
             "fmcs_calibrate": [vp, u64, P(CostModel)],
@@ -171,9 +173,10 @@ def optimize(model, nrows, bw, card, ordered=True):
 
 # ---- composer-level execution (GPU) -------------------------------------
 
-def run_plan(ctx, cols, bws, col_order, split, mode=0, order=None, groups=None):
+def run_plan(ctx, cols, bws, col_order, split, mode=0, order=None, groups=None, descending=None):
     """StitchComposer::run_single_plan on device tensors; returns
-    (order, breakdown_ms[6], total_ms, ngroups)."""
+    (order, breakdown_ms[6], total_ms, ngroups).  descending: a flag per
+    column (None: all ascending)."""
     import torch
     from .engine import bank_of
     n = cols[0].numel()
@@ -184,10 +187,17 @@ def run_plan(ctx, cols, bws, col_order, split, mode=0, order=None, groups=None):
     bd = (C.c_double * 6)()
     tot = C.c_double(0)
     ng = C.c_uint32(0)
-    rc = lib().fmcs_run_plan(ctx.h, k, cp, _u32([bank_of(c) for c in cols]), _u32(bws), _u32(col_order),
-                             len(split), _u32(split), mode, n, C.c_void_p(order.data_ptr()),
-                             C.c_void_p(groups.data_ptr()) if groups is not None else None, bd, C.byref(tot),
-                             C.byref(ng))
+    if descending is None:
+        rc = lib().fmcs_run_plan(ctx.h, k, cp, _u32([bank_of(c) for c in cols]), _u32(bws), _u32(col_order),
+                                 len(split), _u32(split), mode, n, C.c_void_p(order.data_ptr()),
+                                 C.c_void_p(groups.data_ptr()) if groups is not None else None, bd, C.byref(tot),
+                                 C.byref(ng))
+    else:
+        desc = (C.c_uint8 * k)(*[1 if d else 0 for d in descending])
+        rc = lib().fmcs_run_plan_dir(ctx.h, k, cp, _u32([bank_of(c) for c in cols]), _u32(bws), desc,
+                                     _u32(col_order), len(split), _u32(split), mode, n, C.c_void_p(order.data_ptr()),
+                                     C.c_void_p(groups.data_ptr()) if groups is not None else None, bd,
+                                     C.byref(tot), C.byref(ng))
     ctx._chk(rc)
     return order, list(bd), tot.value, ng.value
 

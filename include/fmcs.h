@@ -116,6 +116,16 @@ int fmcs_stitch(fmcs_ctx* ctx, uint64_t n, uint32_t nsrc, const void* const* src
                 const uint32_t* src_bw, uint32_t nent, const fmcs_entry* entries, uint32_t ndst,
                 void* const* dst_d, const uint32_t* dst_bank);
 
+/* fmcs_stitch with a sort direction per source column (src_desc[i] != 0:
+ * descending; NULL: all ascending).  A descending column's value bits are
+ * inverted within its width while it is stitched, so sorting the new banks
+ * ascending orders it from the largest value; ties keep the ascending oid
+ * order.  The reference parses a direction but sorts ascending only
+ * (src/composer.cpp:561-563). */
+int fmcs_stitch_dir(fmcs_ctx* ctx, uint64_t n, uint32_t nsrc, const void* const* src_d, const uint32_t* src_bank,
+                    const uint32_t* src_bw, uint32_t nent, const fmcs_entry* entries, uint32_t ndst,
+                    void* const* dst_d, const uint32_t* dst_bank, const uint8_t* src_desc);
+
 /* Composer::BATproject<T> (src/composer.cpp:608-628): out[i] = values[order[i]]. */
 int fmcs_gather(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* values_d, const uint32_t* order_d,
                 void* out_d);

@@ -94,6 +94,15 @@ int fmcs_run_plan(fmcs_ctx* ctx, uint32_t ncols, const void* const* cols_d, cons
                   uint64_t nrows, uint32_t* order_d, uint32_t* groups_d, double* ms6, double* total_ms,
                   uint32_t* ngroups);
 
+/* fmcs_run_plan with a sort direction per column (descending[i] != 0:
+ * descending, through fmcs_stitch_dir; NULL: all ascending).  Ties keep the
+ * ascending oid order.  The reference sorts ascending only
+ * (src/composer.cpp:561-563). */
+int fmcs_run_plan_dir(fmcs_ctx* ctx, uint32_t ncols, const void* const* cols_d, const uint32_t* banks,
+                      const uint32_t* bw, const uint8_t* descending, const uint32_t* col_order, uint32_t nsplit,
+                      const uint32_t* split, int mode, uint64_t nrows, uint32_t* order_d, uint32_t* groups_d,
+                      double* ms6, double* total_ms, uint32_t* ngroups);
+
 /* StitchComposer::run_single_plan_hash (src/stitch_composer.cpp:808-1079):
  * the same with u8 banks allowed and hash grouping per round. */
 int fmcs_run_plan_hash(fmcs_ctx* ctx, uint32_t ncols, const void* const* cols_d, const uint32_t* banks,

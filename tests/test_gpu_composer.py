@@ -142,3 +142,24 @@ def test_optimize_and_run_learned_picker(ctx):
     assert sum(r["split"]) == sum(bws)
     with pytest.raises(ValueError):
         planner.optimize_and_run(ctx, [dev(c1), dev(c2)], bws, mlp=mlp, ordered=False)
+
+
+@pytest.mark.parametrize("mode", [MODE_BATCHED, MODE_DEVICE])
+def test_descending_columns(ctx, mode):
+    # per-column direction through the stitch (a descending column's bits
+    # inverted within its width); plans whose rounds mix an inverted and a
+    # plain source; ties keep ascending oids
+    n = 1 << 16
+    (c1, c2), bws = datagen.config1(n)
+    c2 = c2.copy()
+    c2[::5] = c2[0]  # ties in the second column too
+    cols = [dev(c1), dev(c2)]
+    m1, m2 = np.uint64((1 << bws[0]) - 1), np.uint64((1 << bws[1]) - 1)
+    k1, k2 = c1.astype(np.uint64), c2.astype(np.uint64)
+    for desc in ([True, False], [False, True], [True, True]):
+        a = m1 - k1 if desc[0] else k1
+        b = m2 - k2 if desc[1] else k2
+        exp = np.lexsort((b, a)).astype(np.uint32)
+        for split in ([50], [17, 33], [21, 29], [9, 41], [16, 16, 18]):
+            order, _, _, _ = planner.run_plan(ctx, cols, bws, [0, 1], split, mode=mode, descending=desc)
+            assert np.array_equal(order.cpu().numpy(), exp), (desc, split)
