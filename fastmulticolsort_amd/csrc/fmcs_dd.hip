@@ -239,6 +239,7 @@ __global__ __launch_bounds__(256) void kd_seg_hist(const K* __restrict__ keys, c
   constexpr int TILE = 256 * ITEMS;
   constexpr int PV = 16 / sizeof(K);           // keys per 16-byte vector
   constexpr int NV = TILE / PV / 256;          // vectors per thread in a full tile
+  constexpr bool VEC = TILE % (PV * 256) == 0;  // a full tile is whole vectors per thread
   // HCOPY sub-histograms, lane l counting into copy l % HCOPY: lanes of one
   // instruction that share a digit mostly hit different copies, so a skewed
   // key does not serialize the LDS atomics.  Copies are HSTRIDE words apart
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(256) void kd_seg_hist(const K* __restrict__ keys, c
     }
     const u64 row0 = (u64)seg_lo[seg] + (u64)(tile - tile_base[seg]) * TILE;
     const u64 rowend = min((u64)seg_hi[seg], row0 + TILE);
-    if (rowend - row0 == (u64)TILE && ((uintptr_t)(keys + row0) & 15) == 0) {
+    if (VEC && rowend - row0 == (u64)TILE && ((uintptr_t)(keys + row0) & 15) == 0) {
       const uint4* vp = (const uint4*)(keys + row0);
       uint4 x[NV];
 #pragma unroll

@@ -283,10 +283,12 @@ def measure_plans(ctx, req, plans, reps=2):
     return out
 
 
-def collect(ctx, nreq, seed=5, max_log2=22, max_plans=32, log=None):
-    """Measured (features, ms) over seeded requests; returns a dict of arrays."""
+def collect(ctx, nreq, seed=5, max_log2=22, max_plans=32, log=None, model=None):
+    """Measured (features, ms) over seeded requests; returns a dict of arrays.
+    model: the cost model of the analytic-estimate feature (None: default)."""
     rng = np.random.default_rng(seed)
-    model = planner.default_cost_model()
+    if model is None:
+        model = planner.default_cost_model()
     X, Y, R, P = [], [], [], []
     reqs = []
     for i in range(nreq):
