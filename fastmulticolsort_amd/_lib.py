@@ -65,6 +65,11 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    # torch ships its own HIP runtime; importing it first makes the library
+    # bind to that one.  Loaded first, the library would bring in the ROCm
+    # install's runtime beside torch's, and one process with two runtimes
+    # fails to open the GPU in whichever initialises second.
+    import torch  # noqa: F401
     lib = C.CDLL(LIB_PATH)
     vp, u32, u64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
     P = C.POINTER

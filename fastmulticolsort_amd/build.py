@@ -9,7 +9,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libfmcs.so")
 
 HIP_SOURCES = ["fmcs_kernels.hip", "fmcs_dd.hip", "fmcs_merge.hip", "fmcs_capi.hip", "fmcs_mlp.hip"]
-HOST_SOURCES = ["fmcs_plan.cpp", "fmcs_composer.cpp"]
+HOST_SOURCES = ["fmcs_plan.cpp", "fmcs_composer.cpp", "fmcs_io.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -1472,25 +1472,46 @@ int uniform_offsets(fmcs_ctx* ctx, uint32_t* off, uint32_t nseg, uint64_t n, uin
 // =========================================================================
 // C ABI
 // =========================================================================
+// Message of the calling thread's last failed fmcs_ctx_create (no context
+// exists to hold it): fmcs_last_error(NULL) returns it.
+thread_local std::string g_create_err;
+int create_fail(int code, const std::string& msg) {
+  g_create_err = msg;
+  return code;
+}
+
 extern "C" {
 
 int fmcs_ctx_create(int device, fmcs_ctx** out) {
-  if (!out) return FMCS_E_ARG;
+  if (!out) return create_fail(FMCS_E_ARG, "null output pointer");
   *out = nullptr;
   int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return FMCS_E_HIP;
-  if (device < 0 || device >= ndev) return FMCS_E_ARG;
-  if (hipSetDevice(device) != hipSuccess) return FMCS_E_HIP;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess) return create_fail(FMCS_E_HIP, std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+  if (ndev == 0) return create_fail(FMCS_E_HIP, "no GPU visible");
+  if (device < 0 || device >= ndev) return create_fail(FMCS_E_ARG, "device index out of range");
+  e = hipSetDevice(device);
+  if (e != hipSuccess) return create_fail(FMCS_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
   fmcs_ctx* c = new fmcs_ctx();
   c->device = device;
-  if (hipMalloc(&c->small_d, 64 * sizeof(u32)) != hipSuccess) { delete c; return FMCS_E_NOMEM; }
-  if (hipHostMalloc((void**)&c->host_pin, PIN_WORDS * sizeof(u32), hipHostMallocDefault) != hipSuccess) {
+  e = hipMalloc(&c->small_d, 64 * sizeof(u32));
+  if (e != hipSuccess) {
+    delete c;
+    return create_fail(FMCS_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  e = hipHostMalloc((void**)&c->host_pin, PIN_WORDS * sizeof(u32), hipHostMallocDefault);
+  if (e != hipSuccess) {
     (void)hipFree(c->small_d);
     delete c;
-    return FMCS_E_NOMEM;
+    return create_fail(FMCS_E_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
   }
-  for (auto& e : c->ev)
-    if (hipEventCreate(&e) != hipSuccess) return FMCS_E_HIP;
+  for (auto& ev : c->ev) {
+    e = hipEventCreate(&ev);
+    if (e != hipSuccess) {
+      fmcs_ctx_destroy(c);
+      return create_fail(FMCS_E_HIP, std::string("hipEventCreate: ") + hipGetErrorString(e));
+    }
+  }
   *out = c;
   return FMCS_OK;
 }
@@ -1554,7 +1575,7 @@ int fmcs_ctx_trim(fmcs_ctx* ctx) {
   return FMCS_OK;
 }
 
-const char* fmcs_last_error(fmcs_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* fmcs_last_error(fmcs_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
 
 uint32_t fmcs_bank_bytes(uint32_t width, int allow_u8) {
   if (allow_u8 && width <= 8) return 1;

@@ -59,7 +59,8 @@ class Context:
         h = C.c_void_p()
         rc = self.lib.fmcs_ctx_create(device, C.byref(h))
         if rc != 0:
-            raise FmcsError(rc, "fmcs_ctx_create failed")
+            msg = self.lib.fmcs_last_error(None)
+            raise FmcsError(rc, "fmcs_ctx_create failed: " + (msg.decode() if msg else ""))
         self.h = h
 
     def close(self):

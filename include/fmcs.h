@@ -29,6 +29,8 @@ extern "C" {
 #define FMCS_E_NOMEM 2
 #define FMCS_E_HIP 3
 #define FMCS_E_UNSUPPORTED 4
+#define FMCS_E_IO 5     /* a file cannot be opened, read or written (fmcs_io.h) */
+#define FMCS_E_FORMAT 6 /* a file's contents do not parse (fmcs_io.h) */
 
 /* Segmented-sort forms for rounds > 1. */
 #define FMCS_MODE_BATCHED 0  /* size-binned: packed LDS tiles + segmented MSD radix */

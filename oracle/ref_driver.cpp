@@ -8,6 +8,10 @@
 // these wrappers call its per-round members directly and keep the outputs.
 #include "composer.cpp"        // Composer::sortByGroup / subsort_mergesort / hashByGroup_* templates
 #include "chain_composer.cpp"  // ChainComposer (concrete Composer)
+#include <functional>          // standard headers generator.h uses without including them
+#include <limits>
+#include <random>
+#include "generator.h"         // DataGenerator::ReadFromFile / ReadFromFile_tpch
 
 #include <cstring>
 #include <cstdint>
@@ -145,6 +149,34 @@ int ref_stitch(uint64_t n, const void *const *src, const uint32_t *src_bank, con
     delete dc;
   }
   for (uint32_t s = 0; s < nsrc; ++s) { free(srcd[s]); delete srcc[s]; }
+  return 0;
+}
+
+// DataGenerator::ReadFromFile (tpch == 0) / ReadFromFile_tpch
+// (src/generator.h:67-136): copies column c's values into cols[c] (its
+// bank: <= 16 bits u16, <= 32 u32, else u64) and its std::set cardinality
+// into card[c]; *nrows is the caller's count, or gets the TPC-H header's.
+// The reference exits on a missing file, so the caller passes existing
+// files only.
+int ref_read_text(const char *path, int tpch, uint32_t ncols, const uint32_t *widths, uint64_t *nrows,
+                  void *const *cols, uint64_t *card) {
+  if (ncols > 64) return -1;
+  Column *colv[64];
+  uint32_t w[64];
+  for (uint32_t c = 0; c < ncols; ++c) w[c] = widths[c];
+  DataGenerator g;
+  uint64_t n = *nrows;
+  if (tpch)
+    g.ReadFromFile_tpch(colv, std::string(path), ncols, n, w);
+  else
+    g.ReadFromFile(colv, std::string(path), ncols, n, w);
+  *nrows = n;
+  for (uint32_t c = 0; c < ncols; ++c) {
+    if (cols[c]) std::memcpy(cols[c], colv[c]->GetColumn(), n * colv[c]->GetWidth());
+    card[c] = colv[c]->GetCardinality();
+    free(colv[c]->GetColumn());
+    delete colv[c];
+  }
   return 0;
 }
 

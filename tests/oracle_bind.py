@@ -203,6 +203,26 @@ def ref_mergesort(keys, oids):
     return keys, oids
 
 
+def ref_read_text(path, widths, nrows=0, tpch=False):
+    """DataGenerator::ReadFromFile / ReadFromFile_tpch (src/generator.h:67-136)
+    of the reference itself: (columns, cardinalities).  nrows: the plain
+    layout's row count (the TPC-H layout reads its header)."""
+    k = len(widths)
+    if tpch:
+        # the header gives the row count; read it first to size the outputs
+        with open(path) as f:
+            nrows = int(f.read(64).split()[0])
+    dts = [np.uint16 if w <= 16 else (np.uint32 if w <= 32 else np.uint64) for w in widths]
+    cols = [np.zeros(nrows, dt) for dt in dts]
+    card = np.zeros(k, np.uint64)
+    nr = C.c_uint64(nrows)
+    w = _u32(widths)
+    rc = ref().ref_read_text(str(path).encode(), C.c_int(int(bool(tpch))), C.c_uint32(k), _p(w),
+                             C.byref(nr), _ptrs(cols), _p(card))
+    assert rc == 0 and nr.value == nrows
+    return cols, [int(x) for x in card]
+
+
 def canonical_order(full_key_sorted_runs, order):
     """Sort oids ascending within runs of equal full keys: the reference's
     tie order is arbitrary, the contract is the oid tie-break."""
