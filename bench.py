@@ -109,6 +109,33 @@ def cpu_baseline(c1, c2, sample_rows):
                       f"1 warm-up + median of 2, one pinned thread"}
 
 
+def cpu_baseline_all_cores(c1, c2, sample_rows, max_threads=16):
+    """The same round loop on every core of this process's share (at most
+    16): the reference's kernels under a parallel driver (oracle/_ref), the
+    all-core comparator; None where the reference build is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+    if ob.ref() is None:
+        return None
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        cores = os.cpu_count() or 1
+    nt = max(1, min(max_threads, cores))
+    s1 = np.ascontiguousarray(c1[:sample_rows])
+    s2 = np.ascontiguousarray(c2[:sample_rows])
+    times = []
+    for it in range(4):
+        t0 = time.perf_counter()
+        ob.ref_sort_rounds_mt([s1, s2], [20, 40], nt)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times[1:]))
+    return {"value": sample_rows / t / 1e6, "unit": "Mrows/s", "cores": nt, "kind": "reference",
+            "sample": f"first {sample_rows} rows of the same c1/c2 columns, 2 rounds, {nt} threads "
+                      f"(radix partition + the reference's merge sort per part, groups over threads), "
+                      f"1 warm-up + median of 3"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,10 +268,12 @@ def main():
         plain_ms = (time.perf_counter() - tp0) * 1e3
         log(f"plain per-round: {plain_ms:.1f} ms")
 
-    cpu = None
+    cpu = cpu_all = None
     if not args.no_cpu:
         cpu = cpu_baseline(c1, c2, min(n, 1 << args.cpu_sample_log2))
         log(f"cpu baseline: {cpu}")
+        cpu_all = cpu_baseline_all_cores(c1, c2, min(n, 1 << args.cpu_sample_log2))
+        log(f"cpu baseline, all cores: {cpu_all}")
 
     out = {
         "metric": "Mrows/s, batched segmented sort vs plain per-round launches; % of HBM peak",
@@ -266,6 +295,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name),
                      "avg_launch_ms": round(avg_ms, 5), "bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
         "comparators": {"plain_per_round_ms": plain_ms, "unbinned_ms": round(unb_ms, 3) if unb_ms else None,
                         "host_synchronised_batched_ms": (round(batched_ms, 3) if batched_ms else None),
                         "batched_vs_plain": (round(plain_ms / ms, 2) if plain_ms else None),

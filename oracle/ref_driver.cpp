@@ -13,8 +13,12 @@
 #include <random>
 #include "generator.h"         // DataGenerator::ReadFromFile / ReadFromFile_tpch
 
+#include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <cstdint>
+#include <thread>
+#include <vector>
 
 using namespace multiAttrSort;
 
@@ -42,6 +46,148 @@ void *copy_bank(uint32_t bank, const void *src, uint64_t n) {
     case 4: return aligned_copy<uint32_t>(src, n);
     default: return aligned_copy<uint64_t>(src, n);
   }
+}
+
+// ---- the round loop on several threads, over the reference's own kernels ----
+
+template <class F>
+void par(uint32_t nt, F f) {
+  std::vector<std::thread> th;
+  for (uint32_t t = 0; t < nt; ++t) th.emplace_back(f, t);
+  for (auto &x : th) x.join();
+}
+
+// Rows [start, start + m) of bat sorted by their key: a stable radix
+// partition on the key's top bits (per-thread histograms, the partitioning
+// phase of the reference's pthread sort, src/mergesort.h:2603-2779), then
+// every part sorted by the reference's subsort (avxmergesort_anytype), the
+// parts spread over the threads.
+template <class T>
+void psort_range(ChainComposer &comp, BAT_t<T> *bat, uint64_t start, uint64_t m, uint32_t bitwidth, uint32_t nt,
+                 T *tv, surrogate_t *to) {
+  const uint32_t pb = bitwidth < 8 ? bitwidth : 8;
+  const uint32_t P = 1u << pb;
+  const uint32_t shift = bitwidth - pb;
+  T *v = bat->values + start;
+  surrogate_t *o = bat->oids + start;
+  std::vector<uint64_t> hist((size_t)nt * P, 0), off((size_t)nt * P, 0), pstart(P + 1, 0);
+  par(nt, [&](uint32_t t) {
+    const uint64_t a = m * t / nt, b = m * (t + 1) / nt;
+    uint64_t *h = hist.data() + (size_t)t * P;
+    for (uint64_t i = a; i < b; ++i) ++h[((uint64_t)v[i] >> shift) & (P - 1)];
+  });
+  uint64_t run = 0;
+  for (uint32_t p = 0; p < P; ++p) {
+    pstart[p] = run;
+    for (uint32_t t = 0; t < nt; ++t) {
+      off[(size_t)t * P + p] = run;
+      run += hist[(size_t)t * P + p];
+    }
+  }
+  pstart[P] = m;
+  par(nt, [&](uint32_t t) {
+    const uint64_t a = m * t / nt, b = m * (t + 1) / nt;
+    uint64_t *ofs = off.data() + (size_t)t * P;
+    for (uint64_t i = a; i < b; ++i) {
+      const uint64_t d = ofs[((uint64_t)v[i] >> shift) & (P - 1)]++;
+      tv[d] = v[i];
+      to[d] = o[i];
+    }
+  });
+  par(nt, [&](uint32_t t) {
+    const uint64_t a = m * t / nt, b = m * (t + 1) / nt;
+    std::memcpy(v + a, tv + a, (b - a) * sizeof(T));
+    std::memcpy(o + a, to + a, (b - a) * sizeof(surrogate_t));
+  });
+  // largest parts first, so a heavy part does not start last
+  std::vector<uint32_t> ord(P);
+  for (uint32_t p = 0; p < P; ++p) ord[p] = p;
+  std::sort(ord.begin(), ord.end(),
+            [&](uint32_t x, uint32_t y) { return pstart[x + 1] - pstart[x] > pstart[y + 1] - pstart[y]; });
+  std::atomic<uint32_t> next(0);
+  par(nt, [&](uint32_t) {
+    for (uint32_t j; (j = next.fetch_add(1)) < P;) {
+      const uint32_t p = ord[j];
+      const uint64_t np = pstart[p + 1] - pstart[p];
+      if (np > 1) comp.subsort<T>(bat, start + pstart[p], np, true, bitwidth);
+    }
+  });
+}
+
+// One round of Composer::sortByGroup<T> (src/composer.cpp:1222-1408) on nt
+// threads: BATproject, the per-group sorts (groups above n / (8 nt) rows
+// partitioned as above, the others spread over the threads in chunks), and
+// extractGroupInfo (a two-pass parallel scan) unless it is the last round.
+template <class T>
+void round_mt(ChainComposer &comp, uint32_t nt, const T *col, uint64_t n, surrogate_t *order, surrogate_t *inGroup,
+              surrogate_t *outGroup, uint32_t bitwidth, bool last) {
+  T *vals = (T *)malloc_aligned(n * sizeof(T));
+  T *tv = (T *)malloc_aligned(n * sizeof(T));
+  surrogate_t *to = (surrogate_t *)malloc_aligned(n * sizeof(surrogate_t));
+  if (inGroup)
+    par(nt, [&](uint32_t t) {
+      const uint64_t a = n * t / nt, b = n * (t + 1) / nt;
+      for (uint64_t i = a; i < b; ++i) vals[i] = col[order[i]];
+    });
+  else
+    std::memcpy(vals, col, n * sizeof(T));
+  BAT_t<T> bat{order, vals, n};
+  if (!inGroup) {
+    psort_range<T>(comp, &bat, 0, n, bitwidth, nt, tv, to);
+  } else {
+    // group starts, found per thread chunk, then concatenated in order
+    std::vector<std::vector<uint64_t>> st(nt);
+    par(nt, [&](uint32_t t) {
+      const uint64_t a = n * t / nt, b = n * (t + 1) / nt;
+      for (uint64_t i = a; i < b; ++i)
+        if (i == 0 || inGroup[i] != inGroup[i - 1]) st[t].push_back(i);
+    });
+    std::vector<uint64_t> starts;
+    for (auto &x : st) starts.insert(starts.end(), x.begin(), x.end());
+    starts.push_back(n);
+    const uint64_t big = std::max<uint64_t>(4096, n / (8 * (uint64_t)nt));
+    std::vector<uint64_t> small;
+    for (size_t g = 0; g + 1 < starts.size(); ++g) {
+      const uint64_t m = starts[g + 1] - starts[g];
+      if (m >= big)
+        psort_range<T>(comp, &bat, starts[g], m, bitwidth, nt, tv + starts[g], to + starts[g]);
+      else if (m > 1)
+        small.push_back(g);
+    }
+    std::atomic<size_t> next(0);
+    const size_t CH = 64;
+    par(nt, [&](uint32_t) {
+      for (size_t c; (c = next.fetch_add(CH)) < small.size();)
+        for (size_t j = c; j < std::min(small.size(), c + CH); ++j) {
+          const uint64_t g = small[j];
+          comp.subsort<T>(&bat, starts[g], starts[g + 1] - starts[g], true, bitwidth);
+        }
+    });
+  }
+  if (!last) {
+    std::vector<uint64_t> heads(nt + 1, 0);
+    auto head = [&](uint64_t i) -> uint32_t {
+      return i > 0 && (vals[i] != vals[i - 1] || (inGroup && inGroup[i] != inGroup[i - 1]));
+    };
+    par(nt, [&](uint32_t t) {
+      const uint64_t a = n * t / nt, b = n * (t + 1) / nt;
+      uint64_t c = 0;
+      for (uint64_t i = a; i < b; ++i) c += head(i);
+      heads[t + 1] = c;
+    });
+    for (uint32_t t = 0; t < nt; ++t) heads[t + 1] += heads[t];
+    par(nt, [&](uint32_t t) {
+      const uint64_t a = n * t / nt, b = n * (t + 1) / nt;
+      uint64_t g = heads[t];
+      for (uint64_t i = a; i < b; ++i) {
+        g += head(i);
+        outGroup[i] = (surrogate_t)g;
+      }
+    });
+  }
+  free(vals);
+  free(tv);
+  free(to);
 }
 
 }  // namespace
@@ -109,6 +255,49 @@ int ref_sort_rounds(uint32_t nrounds, const void *const *cols, const uint32_t *b
   free(inGroup);
   free(outGroup);
   for (uint32_t r = 0; r < nrounds; ++r) { free(data[r]); delete colv[r]; }
+  return 0;
+}
+
+// The round loop of ref_sort_rounds on nthreads threads (the reference's
+// kernels, a parallel driver written here: its active path is single
+// threaded, and its pthread sort, src/mergesort.h:290-441, has no caller).
+// Orders agree with ref_sort_rounds up to the order of equal keys.
+int ref_sort_rounds_mt(uint32_t nthreads, uint32_t nrounds, const void *const *cols, const uint32_t *banks,
+                       const uint32_t *bitwidths, uint64_t n, uint32_t *order_out, uint32_t *groups_out) {
+  if (nrounds == 0 || nrounds > 64 || nthreads == 0 || n == 0) return -1;
+  setting_t s;
+  compose_params_t p;
+  bool asc[64];
+  uint32_t bws[64];
+  for (uint32_t r = 0; r < nrounds; ++r) {
+    asc[r] = true;
+    bws[r] = bitwidths[r];
+  }
+  p.column_asc_desc = asc;
+  p.bitwidth = bws;
+  ChainComposer comp(s, nrounds, n, p, nullptr);
+  surrogate_t *order = (surrogate_t *)malloc_aligned(n * sizeof(surrogate_t));
+  par(nthreads, [&](uint32_t t) {
+    const uint64_t a = n * t / nthreads, b = n * (t + 1) / nthreads;
+    for (uint64_t i = a; i < b; ++i) order[i] = (surrogate_t)i;
+  });
+  surrogate_t *outGroup = (surrogate_t *)malloc_aligned(n * sizeof(surrogate_t));
+  surrogate_t *inGroup = nullptr;
+  for (uint32_t r = 0; r < nrounds; ++r) {
+    const bool last = (r == nrounds - 1);
+    switch (banks[r]) {
+      case 2: round_mt<uint16_t>(comp, nthreads, (const uint16_t *)cols[r], n, order, inGroup, outGroup, bws[r], last); break;
+      case 4: round_mt<uint32_t>(comp, nthreads, (const uint32_t *)cols[r], n, order, inGroup, outGroup, bws[r], last); break;
+      default: round_mt<uint64_t>(comp, nthreads, (const uint64_t *)cols[r], n, order, inGroup, outGroup, bws[r], last); break;
+    }
+    if (!last && groups_out) std::memcpy(groups_out + (uint64_t)r * n, outGroup, n * sizeof(surrogate_t));
+    if (!inGroup) inGroup = (surrogate_t *)malloc_aligned(n * sizeof(surrogate_t));
+    swap<surrogate_t>(&inGroup, &outGroup);
+  }
+  std::memcpy(order_out, order, n * sizeof(surrogate_t));
+  free(order);
+  free(inGroup);
+  free(outGroup);
   return 0;
 }
 

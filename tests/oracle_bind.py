@@ -195,6 +195,22 @@ def ref_sort_rounds(cols, bitwidths):
     return order, groups[: (r - 1) * n].reshape(r - 1, n)
 
 
+def ref_sort_rounds_mt(cols, bitwidths, nthreads):
+    """ref_sort_rounds on nthreads threads (the reference's kernels under a
+    parallel driver): (order, groups of the rounds before the last)."""
+    n = len(cols[0])
+    cols = [np.ascontiguousarray(c) for c in cols]
+    banks = _u32([c.dtype.itemsize for c in cols])
+    bws = _u32(bitwidths)
+    order = np.zeros(n, np.uint32)
+    r = len(cols)
+    groups = np.zeros(max(r - 1, 1) * n, np.uint32)
+    rc = ref().ref_sort_rounds_mt(C.c_uint32(nthreads), C.c_uint32(r), _ptrs(cols), _p(banks), _p(bws),
+                                  C.c_uint64(n), _p(order), _p(groups))
+    assert rc == 0
+    return order, groups[: (r - 1) * n].reshape(r - 1, n)
+
+
 def ref_mergesort(keys, oids):
     keys = keys.copy(); oids = oids.copy()
     rc = ref().ref_mergesort(C.c_uint32(keys.dtype.itemsize), _p(keys), _p(oids),

@@ -198,6 +198,30 @@ def test_oracle_matches_reference_rounds():
 
 
 @pytest.mark.skipif(ob.ref() is None, reason="reference library not built")
+@pytest.mark.parametrize("nthreads", [1, 3, 8])
+def test_multithreaded_reference_rounds_match_the_single_threaded(nthreads):
+    """The all-core CPU comparator: the reference's kernels under a parallel
+    driver give the single-threaded round loop's groups, and its order up to
+    the order of equal keys; config-1 and a Zipf first column with groups
+    large enough to take the partitioned path."""
+    cases = [datagen.config1(1 << 15)]
+    z = datagen.zipf_column(11, 1 << 16, 64, 1.1, np.arange(64, dtype=np.uint32) * np.uint32(977))
+    cases.append(([z, datagen.uniform_bits(12, 1 << 16, 40)], [16, 40]))
+    cases.append(([datagen.uniform_bits(13, 1 << 16, 12).astype(np.uint16),
+                   datagen.uniform_bits(14, 1 << 16, 20).astype(np.uint32),
+                   datagen.uniform_bits(15, 1 << 16, 30).astype(np.uint32)], [12, 20, 30]))
+    for cols, bws in cases:
+        cols = [np.ascontiguousarray(c) for c in cols]
+        ro, rg = ob.ref_sort_rounds(cols, bws)
+        mo, mg = ob.ref_sort_rounds_mt(cols, bws, nthreads)
+        keys = np.stack([c[mo].astype(np.uint64) for c in cols])
+        keys_r = np.stack([c[ro].astype(np.uint64) for c in cols])
+        assert np.array_equal(keys, keys_r)  # the same sorted tuples
+        assert np.array_equal(ob.canonical_order(keys, mo), ob.canonical_order(keys_r, ro))
+        assert np.array_equal(mg, rg)
+
+
+@pytest.mark.skipif(ob.ref() is None, reason="reference library not built")
 @pytest.mark.parametrize("bws,split", [([20, 36], [24, 32]), ([12, 20, 30], [62]), ([17, 33], [10, 40]),
                                        ([2, 1, 12, 27], [16, 16, 10])])
 def test_oracle_stitch_matches_reference(bws, split):
