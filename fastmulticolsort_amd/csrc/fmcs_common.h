@@ -61,14 +61,30 @@ __device__ __forceinline__ u64 lanemask_lt() {
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// Inclusive wave scan of u32.
+// Inclusive wave scan of u32 through DPP lane moves (no LDS crossbar
+// instructions, which a shuffle costs): row_shr 1, 2, 4, 8 inside each
+// 16-lane row, then row_bcast:15 (row r's last lane into row r + 1) and
+// row_bcast:31 (lane 31 into rows 2 and 3).  A lane whose source lies
+// outside its row or below it ignores the moved value.
+template <int CTRL>
+__device__ __forceinline__ u32 dpp_move(u32 v) {
+  return (u32)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xf, 0xf, false);
+}
 __device__ __forceinline__ u32 wave_incl_scan(u32 v) {
-  u32 lane = __lane_id();
-#pragma unroll
-  for (int o = 1; o < WAVE; o <<= 1) {
-    u32 t = __shfl_up(v, o);
-    if (lane >= (u32)o) v += t;
-  }
+  const u32 lane = __lane_id(), rl = lane & 15u;
+  u32 t;
+  t = dpp_move<0x111>(v);  // row_shr:1
+  if (rl >= 1) v += t;
+  t = dpp_move<0x112>(v);  // row_shr:2
+  if (rl >= 2) v += t;
+  t = dpp_move<0x114>(v);  // row_shr:4
+  if (rl >= 4) v += t;
+  t = dpp_move<0x118>(v);  // row_shr:8
+  if (rl >= 8) v += t;
+  t = dpp_move<0x142>(v);  // row_bcast:15
+  if ((lane & 31u) >= 16) v += t;
+  t = dpp_move<0x143>(v);  // row_bcast:31
+  if (lane >= 32) v += t;
   return v;
 }
 
