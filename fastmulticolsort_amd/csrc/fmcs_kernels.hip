@@ -217,9 +217,8 @@ __global__ __launch_bounds__(256) void k_bcount(const K* __restrict__ v, const u
   u32 c = 0;
 #pragma unroll
   for (int vv = 0; vv < NV; ++vv) c += (u32)__popc(hm[vv]);
-#pragma unroll
-  for (int o = WAVE / 2; o > 0; o >>= 1) c += __shfl_down(c, o);
-  if ((tid & (WAVE - 1)) == 0) s_w[tid / WAVE] = c;
+  c = wave_incl_scan(c);  // lane 63 holds the wave's total
+  if ((tid & (WAVE - 1)) == WAVE - 1) s_w[tid / WAVE] = c;
   __syncthreads();
   if (tid == 0) {
     u32 t = 0;
