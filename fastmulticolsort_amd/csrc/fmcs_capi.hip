@@ -25,7 +25,7 @@ enum Slot {
   WS_KA, WS_KB, WS_VB, WS_T0K, WS_T0V, WS_T1K, WS_T1V, WS_STATUS, WS_HIST, WS_DBASE, WS_SEGOFF, WS_SEGOFF2,
   WS_ST, WS_SZ, WS_BST, WS_BSZ, WS_FLAG, WS_POS, WS_LLO, WS_LHI, WS_LTILES, WS_TBASE, WS_TLO, WS_THI, WS_TS0,
   WS_TS1, WS_COMP, WS_COMP2, WS_GROUPTMP, WS_MISC, WS_HASH, WS_HASH2, WS_HASH3, WS_HT_TABLE, WS_HT_FIRST,
-  WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_DD, WS_SCANST, WS_FBLIST, WS_TCOUNT, WS_CARRY, WS_BCNT, WS_T0P, WS_T1P, WS_PAY, WS_SGL, WS_FBLIST2, WS_RKL, WS_RKL2, WS_USER0,
+  WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_DD, WS_SCANST, WS_FBLIST, WS_TCOUNT, WS_CARRY, WS_BCNT, WS_T0P, WS_T1P, WS_PAY, WS_SGL, WS_FBLIST2, WS_RKL, WS_RKL2, WS_DNEXT, WS_USER0,
   WS_COUNT = WS_USER0 + 24
 };
 
@@ -285,7 +285,7 @@ int scatter_pass(fmcs_ctx* ctx, const char* name, const K* kin, const u32* vin, 
                  const u32* hi, const u32* tbase, const u32* nsegs_d, const u32* ntiles_d, u64 ntiles_max,
                  const u32* dbase, u32* tcount, u32 shift, u32 dbits, double bytes, const u32* dcount, double per,
                  const P* pin = nullptr, PayOut<P, FUSE>* pout = nullptr, u32 fshift = 0, u64 pbad = 0,
-                 u32* err = nullptr, u32* kside = nullptr) {
+                 u32* err = nullptr, u32* kside = nullptr, uint8_t* dnext = nullptr, u32 nshift = 0) {
   constexpr u32 R = 1u << RB;
   const u64 nch = std::max<u64>(1, (ntiles_max + TSCAN_CT - 1) / TSCAN_CT);
   u32* carry;
@@ -297,7 +297,7 @@ int scatter_pass(fmcs_ctx* ctx, const char* name, const K* kin, const u32* vin, 
   auto kern = k_onesweep<K, ONESWEEP_ITEMS, RB, P, FUSE, LSD>;
   return klaunch(ctx, name, kern, onesweep_grid(ctx, (const void*)kern, ntiles_max), bytes, dcount, per, kin, vin,
                  kout, vout, lo, hi, tbase, 0u, dbase, (const u32*)tcount, (const u32*)carry, shift, dbits, nsegs_d,
-                 ntiles_d, pin, pout, fshift, pbad, err, kside);
+                 ntiles_d, pin, pout, fshift, pbad, err, kside, dnext, nshift);
 }
 
 // exclusive scan of n u32 values -> out; *total_h (host) if non-null
@@ -927,6 +927,12 @@ int radix_sort_dd_fused(fmcs_ctx* ctx, const K* kin, const u64* pin, u32 keybits
   TRY(getbuf_t(ctx, WS_T1V, n, &t1v));
   if (npasses > 1) TRY(getbuf_t(ctx, WS_T0P, n, &t0c));
   if (npasses > 2) TRY(getbuf_t(ctx, WS_T1P, n, &t1c));
+  // every pass but the last writes each row's digit of the next pass as a
+  // byte beside it, so the next pass's tile histogram reads 1 byte a row
+  // instead of the 8-byte composite (read before the next scatter rewrites
+  // the bytes, so one buffer serves every pass)
+  uint8_t* dnx = nullptr;
+  if (npasses > 1) TRY(getbuf_t(ctx, WS_DNEXT, n, &dnx));
   TRY(fill(ctx, hist, (u64)npasses * R, 0));
   u32* cst = dd + DD_CONST;
   TRY(klaunch(ctx, "k_set", kd_set4, 1, 0.0, nullptr, 0.0, cst, 0u, (u32)n, 0u, (u32)ntiles));
@@ -947,22 +953,25 @@ int radix_sort_dd_fused(fmcs_ctx* ctx, const K* kin, const u64* pin, u32 keybits
                   (double)n * sizeof(K), nullptr, 0.0, kin, (const u32*)cst, (const u32*)cst + 1,
                   (const u32*)cst + 2, one, (const u32*)cst + 3, 0u, dbits, hp, tcount, kbad, dd + DD_ERR));
     } else {
-      TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<u64, ONESWEEP_ITEMS, RB>, cap_grid(ntiles, 2048), (double)n * 8,
-                  nullptr, 0.0, sc, (const u32*)cst, (const u32*)cst + 1, (const u32*)cst + 2, one,
-                  (const u32*)cst + 3, pbits + p * RB, dbits, hp, tcount, 0ull, (u32*)nullptr));
+      TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<uint8_t, ONESWEEP_ITEMS, RB>, cap_grid(ntiles, 2048),
+                  (double)n, nullptr, 0.0, (const uint8_t*)dnx, (const u32*)cst, (const u32*)cst + 1,
+                  (const u32*)cst + 2, one, (const u32*)cst + 3, 0u, dbits, hp, tcount, 0ull, (u32*)nullptr));
     }
     TRY(klaunch(ctx, "k_digit_base", kd_digit_base<RB>, 1, 0.0, nullptr, 0.0, (const u32*)hp, (const u32*)cst,
                 dbase + (u64)p * R, (u32*)nullptr, (u32*)nullptr, one));
     if (p == 0) {
+      // the next digit: c1's bits RB .. 2 RB - 1
       TRY((scatter_pass<K, RB, u64, true, true>(ctx, "k_onesweep_fuse", kin, (const u32*)nullptr, (K*)nullptr, dv, cst, cst + 1,
                                           cst + 2, one, cst + 3, ntiles, dbase, tcount, 0u, dbits,
-                                          (double)n * (sizeof(K) + 8 + 8 + 4), nullptr, 0.0, pin, dc, pbits, pbad,
-                                          dd + DD_ERR)));
+                                          (double)n * (sizeof(K) + 8 + 8 + 4 + (last ? 0 : 1)), nullptr, 0.0, pin, dc,
+                                          pbits, pbad, dd + DD_ERR, (u32*)nullptr, last ? (uint8_t*)nullptr : dnx,
+                                          RB)));
     } else {
       TRY((scatter_pass<u64, RB, NoPay, false, true>)(ctx, "k_onesweep", sc, sv, dc, dv, cst, cst + 1, cst + 2, one, cst + 3, ntiles,
                             dbase + (u64)p * R, tcount, pbits + p * RB, dbits,
-                            2.0 * n * (8 + 4) + (last ? 4.0 * n : 0.0), nullptr, 0.0, (const NoPay*)nullptr,
-                            (NoPay*)nullptr, pbits, 0ull, (u32*)nullptr, last ? side : (u32*)nullptr));
+                            2.0 * n * (8 + 4) + (last ? 4.0 * n : 1.0 * n), nullptr, 0.0, (const NoPay*)nullptr,
+                            (NoPay*)nullptr, pbits, 0ull, (u32*)nullptr, last ? side : (u32*)nullptr,
+                            last ? (uint8_t*)nullptr : dnx, pbits + (p + 1) * RB));
     }
     sc = dc;
     sv = dv;

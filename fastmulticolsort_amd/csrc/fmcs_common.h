@@ -129,6 +129,9 @@ __device__ __forceinline__ void unpack16(const uint4& x, K (&k)[16 / sizeof(K)])
   } else if constexpr (sizeof(K) == 4) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) k[i] = (K)w[i];
+  } else if constexpr (sizeof(K) == 1) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) k[i] = (K)(w[i / 4] >> (8 * (i % 4)));
   } else {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {

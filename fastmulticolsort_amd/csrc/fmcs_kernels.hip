@@ -387,7 +387,8 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
                                                   const u32* __restrict__ nsegs_d, const u32* __restrict__ ntiles_d,
                                                   const P* __restrict__ pin, PayOut<P, FUSE>* __restrict__ pout,
                                                   u32 fshift, u64 pbad, u32* __restrict__ err,
-                                                  u32* __restrict__ kside) {
+                                                  u32* __restrict__ kside, uint8_t* __restrict__ dnext,
+                                                  u32 nshift) {
   using PO = PayOut<P, FUSE>;
   // RB-bit digits (R of them); every thread owns DPT consecutive digits
   constexpr int NT = 256, NW = NT / WAVE, TILE = NT * ITEMS, WITEMS = WAVE * ITEMS;
@@ -529,6 +530,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
         const u32 g = s_goff[digit_of(k, shift, mask)] + i;
         if (!FUSE) kout[g] = k;
         vout[g] = s_vals[i];
+        if (dnext) dnext[g] = (uint8_t)((u64)k >> nshift);
         gpos[it] = g;
       }
     }
@@ -564,6 +566,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
       kout[g] = k;
       vout[g] = s_vals[i];
       if (kside) kside[g] = (u32)((u64)k >> fshift);
+      if (dnext) dnext[g] = (uint8_t)((u64)k >> nshift);
     }
   } else {
     for (u32 i = tid; i < cnt; i += NT) {
@@ -572,6 +575,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
       kout[g] = k;
       vout[g] = s_vals[i];
       if (kside) kside[g] = (u32)((u64)k >> fshift);
+      if (dnext) dnext[g] = (uint8_t)((u64)k >> nshift);
     }
   }
   __syncthreads();
@@ -1460,7 +1464,7 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
                                                                      const u32*, const u32*, u32, const u32*,       \
                                                                      const u32*, const u32*, u32, u32, const u32*,   \
                                                                      const u32*, const P*, PayOut<P, F>*, u32, u64, \
-                                                                     u32*, u32*);
+                                                                     u32*, u32*, uint8_t*, u32);
 #define FMCS_INST_ONESWEEP(K, P) FMCS_INST_ONESWEEP_F(K, P, false, true)
 #define FMCS_INST_K(K)                                                                                              \
   template __global__ void k_gather<K>(const K*, const u32*, K*, u64);                                              \
