@@ -857,7 +857,7 @@ __device__ __forceinline__ void load_window(const TileSortArgs<K>& a, u32 w, Win
 template <class K, int E>
 __device__ __forceinline__ void tile_bucket_body(const TileSortArgs<K>& a, const WinRows<K, E>& r,
                                                  u32* __restrict__ fallback) {
-  constexpr int NT = 256, T = NT * E, NB = 2 * T, BPT = NB / NT, SE = T / NT;
+  constexpr int NT = 256, T = NT * E, NB = T, BPT = NB / NT, SE = T / NT;
   __shared__ K s_k[T];          // rows in bin order, then in final order
   __shared__ u32 s_o[T];
   __shared__ u32 s_cnt[NB];     // bin counts -> bin cursors -> bin ends
@@ -916,7 +916,7 @@ __device__ __forceinline__ void tile_bucket_body(const TileSortArgs<K>& a, const
       u32 lb = 0;
       if (sg < nsegs) {
         const u32 m = (u32)s_segstart[sg + 1] - (u32)s_segstart[sg];
-        lb = m <= 1 ? 0u : 32u - __clz(m - 1);
+        lb = m <= 2 ? 0u : 31u - __clz(m - 1);  // 2^lb >= m / 2: about two rows a bin
         bsum += 1u << lb;
       }
       lbs[e] = lb;
@@ -1105,33 +1105,22 @@ __global__ __launch_bounds__(256) void k_tile_rank(TileSortArgs<K> a) {
 // compact window list, which lists windows roughly in row order: die x
 // (blocks b with b % XCDS == x) takes the x-th contiguous part, so the
 // shared lines at window edges are written from one L2.  The persistent
-// grid is a multiple of XCDS.
+// grid is a multiple of XCDS.  The kernel is bound by the latency of a
+// window's chain of barrier phases, so what matters is how many windows a
+// CU holds at once: one register set of rows (no prefetch of the next
+// window) and half as many bins as rows keep it at 62 VGPRs and 24.6 KB of
+// LDS for 1024-row tiles, 6 blocks a CU.
 template <class K, int E>
 __global__ __launch_bounds__(256) void k_tile_bucket(TileSortArgs<K> a, u32* __restrict__ fallback) {
   if (a.wcount) {
-    // Two register sets in turn: the next window's rows are loaded before
-    // this window is sorted, so their memory latency overlaps the LDS work
-    // (a barrier waits for LDS operations only, not for these loads).
     const u32 cnt = *a.wcount;
     const u32 xcd = blockIdx.x % XCDS, per_x = gridDim.x / XCDS;
     const u32 hi = (u32)((u64)cnt * (xcd + 1) / XCDS);
-    u32 b = (u32)((u64)cnt * xcd / XCDS) + blockIdx.x / XCDS;
-    if (b >= hi) return;
-    // no window list: window b itself
-    WinRows<K, E> ra, rb;
-    load_window<K, E>(a, a.wlist ? a.wlist[b] : b, ra);
-    for (;;) {
-      const u32 b1 = b + per_x;
-      if (b1 < hi) load_window<K, E>(a, a.wlist ? a.wlist[b1] : b1, rb);
-      tile_bucket_body<K, E>(a, ra, fallback);
+    for (u32 b = (u32)((u64)cnt * xcd / XCDS) + blockIdx.x / XCDS; b < hi; b += per_x) {
+      WinRows<K, E> r;
+      load_window<K, E>(a, a.wlist ? a.wlist[b] : b, r);
+      tile_bucket_body<K, E>(a, r, fallback);
       __syncthreads();
-      if (b1 >= hi) break;
-      const u32 b2 = b1 + per_x;
-      if (b2 < hi) load_window<K, E>(a, a.wlist ? a.wlist[b2] : b2, ra);
-      tile_bucket_body<K, E>(a, rb, fallback);
-      __syncthreads();
-      if (b2 >= hi) break;
-      b = b2;
     }
   } else {
     WinRows<K, E> r;
