@@ -427,26 +427,34 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
   const u64 wbase = row0 + (u64)wv * WITEMS;
   const u64 lt = lanemask_lt();
 
+  // Loads in the order they are needed: the keys (for the ranking), this
+  // tile's global digit offsets (after the tile scan), then the oids (for
+  // the staging).  Memory operations complete in order, so the ranking
+  // waits for the keys only while the rest is in flight; every load is
+  // unconditional (a load under a branch makes the compiler wait for all).
+  // Rows past the tile's end load the last row instead (rowend > row0): no
+  // branch around the loads; those rows are never ranked or written.
+  const u64 rlast = rowend - 1;
   K keys[ITEMS];
   u32 vals[ITEMS];
   u32 rank[ITEMS];
 #pragma unroll
-  for (int j = 0; j < ITEMS; ++j) {
-    u64 r = wbase + (u64)j * WAVE + lane;
-    bool ok = r < rowend;
-    keys[j] = ok ? kin[r] : (K)0;
-    vals[j] = ok ? (vin ? vin[r] : (u32)r) : 0u;  // no vin: the row index (identity order)
-  }
-  // this tile's global offsets of digits tid * DPT + q, loaded now so the
-  // loads overlap the ranking
+  for (int j = 0; j < ITEMS; ++j) keys[j] = kin[min(wbase + (u64)j * WAVE + lane, rlast)];
   const u32 chunk = tile / TSCAN_CT;
   const bool carry = first_tile < chunk * TSCAN_CT;
   u32 gbase[DPT];
 #pragma unroll
   for (int q = 0; q < DPT; ++q) {
     const u32 d = tid * DPT + q;
-    gbase[q] = digit_base[(u64)seg * R + d] + tile_pre[(u64)tile * R + d];
-    if (carry) gbase[q] += carry_in[(u64)chunk * R + d];
+    const u32 c = carry_in[(u64)chunk * R + d];
+    gbase[q] = digit_base[(u64)seg * R + d] + tile_pre[(u64)tile * R + d] + (carry ? c : 0u);
+  }
+  if (vin) {
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) vals[j] = vin[min(wbase + (u64)j * WAVE + lane, rlast)];
+  } else {
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) vals[j] = (u32)(wbase + (u64)j * WAVE + lane);  // the identity order
   }
   // Stable ranking: lanes with equal digits are matched by ballots; the
   // lowest such lane adds the match count to the wave's LDS counter with one
