@@ -523,23 +523,36 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
   if constexpr (HASP) {
     // pairs out, keeping each LDS row's global position; then the payload
     // rows, loaded in row order, through the same LDS positions
+    // payload loads without branches (rows past the end load the last
+    // row), and a full tile's pair stores without per-row conditions: the
+    // wait for the payload then does not also wait for those stores
     P pv[ITEMS];
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-      u64 r = wbase + (u64)j * WAVE + lane;
-      pv[j] = r < rowend ? pin[r] : (P)0;
-    }
+    for (int j = 0; j < ITEMS; ++j) pv[j] = pin[min(wbase + (u64)j * WAVE + lane, rlast)];
     u32 gpos[ITEMS];
+    if (cnt == (u32)TILE) {
 #pragma unroll
-    for (int it = 0; it < ITEMS; ++it) {
-      const u32 i = it * NT + tid;
-      if (i < cnt) {
+      for (int it = 0; it < ITEMS; ++it) {
+        const u32 i = it * NT + tid;
         const K k = s_keys[i];
         const u32 g = s_goff[digit_of(k, shift, mask)] + i;
         if (!FUSE) kout[g] = k;
         vout[g] = s_vals[i];
         if (dnext) dnext[g] = (uint8_t)((u64)k >> nshift);
         gpos[it] = g;
+      }
+    } else {
+#pragma unroll
+      for (int it = 0; it < ITEMS; ++it) {
+        const u32 i = it * NT + tid;
+        if (i < cnt) {
+          const K k = s_keys[i];
+          const u32 g = s_goff[digit_of(k, shift, mask)] + i;
+          if (!FUSE) kout[g] = k;
+          vout[g] = s_vals[i];
+          if (dnext) dnext[g] = (uint8_t)((u64)k >> nshift);
+          gpos[it] = g;
+        }
       }
     }
     __syncthreads();
