@@ -287,12 +287,11 @@ __global__ __launch_bounds__(256) void kd_seg_hist(const K* __restrict__ keys, c
         }
       }
     } else {
+      // rows past the tile's end load its last row (no branch around the
+      // loads, so their waits stay per load) and are not counted
       K k[ITEMS];
 #pragma unroll
-      for (int j = 0; j < ITEMS; ++j) {
-        const u64 r = row0 + (u64)j * 256 + threadIdx.x;
-        k[j] = r < rowend ? keys[r] : (K)0;
-      }
+      for (int j = 0; j < ITEMS; ++j) k[j] = keys[min(row0 + (u64)j * 256 + threadIdx.x, rowend - 1)];
 #pragma unroll
       for (int j = 0; j < ITEMS; ++j)
         if (row0 + (u64)j * 256 + threadIdx.x < rowend) {
