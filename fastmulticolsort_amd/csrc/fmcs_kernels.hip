@@ -879,16 +879,26 @@ template <class K, int E>
 __device__ __forceinline__ void tile_bucket_body(const TileSortArgs<K>& a, const WinRows<K, E>& r,
                                                  u32* __restrict__ fallback) {
   constexpr int NT = 256, T = NT * E, NB = T, BPT = NB / NT, SE = T / NT;
-  __shared__ K s_k[T];          // rows in bin order, then in final order
-  __shared__ u32 s_o[T];
+  // The segment arrays are dead once the rows are scattered into their
+  // bins, so they share LDS with the row buffers (fewer bytes a block, more
+  // windows in flight a CU): region A holds the rows' keys from the scatter
+  // on and, before it, the local segment ids [0, 2T), the segments' first
+  // bins [2T, 4T), their log2 bin counts [4T, 5T) and the segment-start
+  // flags [5T, 6T); region B holds the oids from the scatter on and, before
+  // it, the segment starts.
+  constexpr int AB = (int)sizeof(K) * T > 6 * T ? (int)sizeof(K) * T : 6 * T;
+  constexpr int BB = 4 * T;
+  __shared__ __align__(16) unsigned char s_raw[AB + BB];
   __shared__ u32 s_cnt[NB];     // bin counts -> bin cursors -> bin ends
-  __shared__ u16 s_sid[T];      // local segment of every row position
-  __shared__ u16 s_segstart[T + 1];
-  __shared__ u16 s_segbase[T];  // first bin of every segment
-  __shared__ uint8_t s_lb[T];   // log2 bins of every segment
-  __shared__ uint8_t s_f[T];    // segment-start flags
   __shared__ u32 s_tmp[NT / WAVE + 1];
   __shared__ u32 s_over;
+  K* s_k = (K*)s_raw;                           // rows in bin order, then in final order
+  u32* s_o = (u32*)(s_raw + AB);
+  u16* s_sid = (u16*)s_raw;                     // local segment of every row position
+  u16* s_segbase = (u16*)(s_raw + 2 * T);       // first bin of every segment
+  uint8_t* s_lb = s_raw + 4 * T;                // log2 bins of every segment
+  uint8_t* s_f = s_raw + 5 * T;                 // segment-start flags
+  u16* s_segstart = (u16*)(s_raw + AB);         // T + 1 entries
   const u32 tid = threadIdx.x;
   const u32 w = r.w, lo = r.lo, hi = r.hi;
   if (lo >= hi) return;
