@@ -793,9 +793,11 @@ int dd_scan(fmcs_ctx* ctx, const u32* in, u32* out, const u32* n_d, u64 n_max, u
   const u64 tiles_max = std::max<u64>(1, (n_max + SCAN_TILE - 1) / SCAN_TILE);
   TRY(getbuf_t(ctx, WS_SCANST, tiles_max, &st));
   TRY(dd_fill(ctx, st, n_d, 1, SCAN_TILE, 0, 0, tiles_max));
+  // blocks take tiles from the counter, so a few hundred cover any list:
+  // a larger grid only adds launch time (most lists are far below their bound)
   return klaunch(ctx, "k_scan", kd_scan_excl<SCAN_ITEMS>,
-                 cap_grid(tiles_max, resident(ctx, (const void*)kd_scan_excl<SCAN_ITEMS>)), 0.0, nullptr, 0.0, in, out,
-                 n_d, st, counter, total);
+                 cap_grid(tiles_max, std::min<u64>(512, resident(ctx, (const void*)kd_scan_excl<SCAN_ITEMS>))), 0.0,
+                 nullptr, 0.0, in, out, n_d, st, counter, total);
 }
 
 // LSD radix sort over every pass (no host-side trivial-pass skipping).
@@ -1100,7 +1102,8 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
     TRY(klaunch(ctx, "k_windows", kd_windows, cap_grid((d.nseg_max + 2047) / 2048, 2048), 0.0, nullptr, 0.0,
                 (const u32*)cst, (const u32*)csz, nseg_p, d.Wl, d.sm, plo, ptb, d.wpt, tlo, thi, ts0, ts1, wlist,
                 S + LV_WCOUNT, d.single_max, slo, shi, sseg, S + LV_NSINGLE));
-    TRY(klaunch(ctx, "k_large_flags", kd_large_flags, cap_grid((d.nseg_max + 255) / 256, 1024), 0.0, nullptr, 0.0,
+    // grid-stride over a device count far below its host bound: 512 blocks
+    TRY(klaunch(ctx, "k_large_flags", kd_large_flags, cap_grid((d.nseg_max + 255) / 256, 512), 0.0, nullptr, 0.0,
                 (const u32*)csz, nseg_p, d.single_max, flag, S + LV_ROWS_SMALL));
     TRY(dd_scan(ctx, flag, pos, nseg_p, d.nseg_max, S + LV_SCAN_A, S + LV_NLARGE));
     TileSortArgs<K> a;
