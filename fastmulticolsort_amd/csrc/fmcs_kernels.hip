@@ -151,47 +151,67 @@ __device__ __forceinline__ void tile_heads(const K* __restrict__ v, const u32* _
   const u32 tid = threadIdx.x, lane = tid & (WAVE - 1);
   K k[NV][PV];
   u32 g[NV][PV];
+  // a whole aligned tile: straight-line vector loads (no per-vector branch)
+  const bool full = vec_ok && base + (u64)ITEMS * 256 <= n;
+  // the wave's first thread: lane 0's predecessor row is wave-uniform, a
+  // scalar load (its wait is not the vector loads' in-order one)
+  const u32 wtid = __builtin_amdgcn_readfirstlane(tid);
+  // one vector of the keys (and the group ids) at element e0
+  auto vec_load = [&](int vv, u64 e0) {
+    unpack16<K>(*(const uint4*)(v + e0), k[vv]);
+    if (in_group) {
+      if constexpr (PV >= 4) {
 #pragma unroll
-  for (int vv = 0; vv < NV; ++vv) {
-    const u64 e0 = base + ((u64)vv * 256 + tid) * PV;
-    if (vec_ok && e0 + PV <= n) {
-      unpack16<K>(*(const uint4*)(v + e0), k[vv]);
-      if (in_group) {
-        if constexpr (PV >= 4) {
+        for (int i = 0; i < PV; i += 4) {
+          const uint4 x = *(const uint4*)(in_group + e0 + i);
+          g[vv][i] = x.x;
+          g[vv][i + 1] = x.y;
+          g[vv][i + 2] = x.z;
+          g[vv][i + 3] = x.w;
+        }
+      } else {
+        const uint2 x = *(const uint2*)(in_group + e0);
+        g[vv][0] = x.x;
+        g[vv][1] = x.y;
+      }
+    }
+  };
+  if (full) {
 #pragma unroll
-          for (int i = 0; i < PV; i += 4) {
-            const uint4 x = *(const uint4*)(in_group + e0 + i);
-            g[vv][i] = x.x;
-            g[vv][i + 1] = x.y;
-            g[vv][i + 2] = x.z;
-            g[vv][i + 3] = x.w;
-          }
-        } else {
-          const uint2 x = *(const uint2*)(in_group + e0);
-          g[vv][0] = x.x;
-          g[vv][1] = x.y;
+    for (int vv = 0; vv < NV; ++vv) vec_load(vv, base + ((u64)vv * 256 + tid) * PV);
+  } else {
+#pragma unroll
+    for (int vv = 0; vv < NV; ++vv) {
+      const u64 e0 = base + ((u64)vv * 256 + tid) * PV;
+      if (vec_ok && e0 + PV <= n) {
+        vec_load(vv, e0);
+      } else {
+#pragma unroll
+        for (int i = 0; i < PV; ++i) {
+          k[vv][i] = e0 + i < n ? v[e0 + i] : (K)0;
+          if (in_group) g[vv][i] = e0 + i < n ? in_group[e0 + i] : 0u;
         }
       }
-    } else {
-#pragma unroll
-      for (int i = 0; i < PV; ++i) {
-        k[vv][i] = e0 + i < n ? v[e0 + i] : (K)0;
-        if (in_group) g[vv][i] = e0 + i < n ? in_group[e0 + i] : 0u;
-      }
     }
-    if (!in_group) {
+  }
+  if (!in_group) {
+#pragma unroll
+    for (int vv = 0; vv < NV; ++vv)
 #pragma unroll
       for (int i = 0; i < PV; ++i) g[vv][i] = 0;
-    }
   }
 #pragma unroll
   for (int vv = 0; vv < NV; ++vv) {
     const u64 e0 = base + ((u64)vv * 256 + tid) * PV;
     u64 pv = (u64)__shfl_up((unsigned long long)k[vv][PV - 1], 1);
     u32 pg = __shfl_up(g[vv][PV - 1], 1);
+    const u64 ew = base + ((u64)vv * 256 + wtid) * PV;  // lane 0's first row
+    const u64 ep = ew > 0 && ew < n ? ew - 1 : 0;
+    const u64 pvw = (u64)v[ep];
+    const u32 pgw = in_group ? in_group[ep] : 0u;
     if (lane == 0 && e0 > 0 && e0 < n) {
-      pv = (u64)v[e0 - 1];
-      if (in_group) pg = in_group[e0 - 1];
+      pv = pvw;
+      pg = pgw;
     }
     u32 m = 0;
 #pragma unroll
