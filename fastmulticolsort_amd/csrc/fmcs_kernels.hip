@@ -144,8 +144,8 @@ __global__ __launch_bounds__(256) void k_gather(const T* __restrict__ vals, cons
 // r = base + (v * 256 + tid) * PV + i, PV = 16 / sizeof(K) keys per vector.
 // hm[v] gets the head bits of the lane's PV rows.  The predecessor of a
 // lane's first row is the next lane down's last row; lane 0 loads it.
-template <class K, int ITEMS>
-__device__ __forceinline__ void tile_heads(const K* __restrict__ v, const u32* __restrict__ in_group, u64 n,
+template <class K, int ITEMS, bool HASG>
+__device__ __forceinline__ void tile_heads_g(const K* __restrict__ v, const u32* __restrict__ in_group, u64 n,
                                            u32 shift, u64 base, bool vec_ok, u32 (&hm)[ITEMS * sizeof(K) / 16]) {
   constexpr int PV = 16 / sizeof(K), NV = ITEMS / PV;
   const u32 tid = threadIdx.x, lane = tid & (WAVE - 1);
@@ -159,7 +159,7 @@ __device__ __forceinline__ void tile_heads(const K* __restrict__ v, const u32* _
   // one vector of the keys (and the group ids) at element e0
   auto vec_load = [&](int vv, u64 e0) {
     unpack16<K>(*(const uint4*)(v + e0), k[vv]);
-    if (in_group) {
+    if constexpr (HASG) {
       if constexpr (PV >= 4) {
 #pragma unroll
         for (int i = 0; i < PV; i += 4) {
@@ -189,12 +189,12 @@ __device__ __forceinline__ void tile_heads(const K* __restrict__ v, const u32* _
 #pragma unroll
         for (int i = 0; i < PV; ++i) {
           k[vv][i] = e0 + i < n ? v[e0 + i] : (K)0;
-          if (in_group) g[vv][i] = e0 + i < n ? in_group[e0 + i] : 0u;
+          if constexpr (HASG) g[vv][i] = e0 + i < n ? in_group[e0 + i] : 0u;
         }
       }
     }
   }
-  if (!in_group) {
+  if constexpr (!HASG) {
 #pragma unroll
     for (int vv = 0; vv < NV; ++vv)
 #pragma unroll
@@ -208,7 +208,8 @@ __device__ __forceinline__ void tile_heads(const K* __restrict__ v, const u32* _
     const u64 ew = base + ((u64)vv * 256 + wtid) * PV;  // lane 0's first row
     const u64 ep = ew > 0 && ew < n ? ew - 1 : 0;
     const u64 pvw = (u64)v[ep];
-    const u32 pgw = in_group ? in_group[ep] : 0u;
+    u32 pgw = 0;
+    if constexpr (HASG) pgw = in_group[ep];
     if (lane == 0 && e0 > 0 && e0 < n) {
       pv = pvw;
       pg = pgw;
@@ -223,6 +224,16 @@ __device__ __forceinline__ void tile_heads(const K* __restrict__ v, const u32* _
     }
     hm[vv] = m;
   }
+}
+
+// the group ids' presence as a template flag: no branch between the loads
+template <class K, int ITEMS>
+__device__ __forceinline__ void tile_heads(const K* __restrict__ v, const u32* __restrict__ in_group, u64 n,
+                                           u32 shift, u64 base, bool vec_ok, u32 (&hm)[ITEMS * sizeof(K) / 16]) {
+  if (in_group)
+    tile_heads_g<K, ITEMS, true>(v, in_group, n, shift, base, vec_ok, hm);
+  else
+    tile_heads_g<K, ITEMS, false>(v, in_group, n, shift, base, vec_ok, hm);
 }
 
 template <class K, int ITEMS>
