@@ -876,33 +876,47 @@ __global__ __launch_bounds__(256) void k_tile_sort(TileSortArgs<K> a) {
 // ---------------------------------------------------------------------------
 constexpr u32 BUCKET_LIMIT = 24;
 
-// A window's range and its rows in registers (row p = e * 256 + tid).
+// A window's range, its segment range and the starts of its first 256 E
+// segments (sst[e]: segment s0 + e * 256 + tid), and its rows in registers
+// (row p = e * 256 + tid).
 template <class K, int E>
 struct WinRows {
-  u32 w, lo, hi;
+  u32 w, lo, hi, s0, s1;
+  u32 sst[E];
   K kk[E];
   u32 oo[E];
 };
 
+// The segment starts are loaded before the rows: the flag phase needs only
+// them, and memory operations complete in order, so it need not wait for
+// the rows.  Both without per-row branches: indices past the end repeat the
+// last segment / row (a flag set twice; rows past cnt are never used).
 template <class K, int E>
 __device__ __forceinline__ void load_window(const TileSortArgs<K>& a, u32 w, WinRows<K, E>& r) {
   r.w = w;
   r.lo = a.tile_lo[w];
   r.hi = a.tile_hi[w];
+  r.s0 = a.tile_s0[w];
+  r.s1 = a.tile_s1[w];
   u32 cnt = r.hi > r.lo ? r.hi - r.lo : 0u;
   if (a.size_hi && (cnt <= a.size_lo || cnt > a.size_hi)) {
     r.hi = r.lo;  // another launch's window: skipped
     cnt = 0;
   }
+  if (cnt == 0 || r.s1 < r.s0) {
+    r.hi = r.lo;
+    return;
+  }
+  const u32 nsg = r.s1 - r.s0 + 1;
   const K* kin = a.kin[a.src_sel];
   const u32* vin = a.vin[a.src_sel];
 #pragma unroll
+  for (int e = 0; e < E; ++e) r.sst[e] = a.seg_start[r.s0 + min(e * 256 + threadIdx.x, nsg - 1)];
+#pragma unroll
   for (int e = 0; e < E; ++e) {
-    const u32 p = e * 256 + threadIdx.x;
-    if (p < cnt) {
-      r.kk[e] = kin[r.lo + p];
-      r.oo[e] = vin[r.lo + p];
-    }
+    const u32 p = min(e * 256 + threadIdx.x, cnt - 1);
+    r.kk[e] = kin[r.lo + p];
+    r.oo[e] = vin[r.lo + p];
   }
 }
 
@@ -933,7 +947,7 @@ __device__ __forceinline__ void tile_bucket_body(const TileSortArgs<K>& a, const
   const u32 tid = threadIdx.x;
   const u32 w = r.w, lo = r.lo, hi = r.hi;
   if (lo >= hi) return;
-  const u32 s0 = a.tile_s0[w], s1 = a.tile_s1[w];
+  const u32 s0 = r.s0, s1 = r.s1;
   const u32 cnt = hi - lo;
   const u32 kb = a.kb;
   const u64 kmask = kb >= 64 ? ~0ull : ((1ull << kb) - 1);
@@ -943,7 +957,12 @@ __device__ __forceinline__ void tile_bucket_body(const TileSortArgs<K>& a, const
   for (u32 i = tid; i < (u32)NB; i += NT) s_cnt[i] = 0;
   if (tid == 0) s_over = 0;
   __syncthreads();
-  for (u32 s = s0 + tid; s <= s1; s += NT) {
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const u32 st = r.sst[e];
+    if (st >= lo && st < hi) s_f[st - lo] = 1;
+  }
+  for (u32 s = s0 + T + tid; s <= s1; s += NT) {  // segments beyond the loaded ones
     const u32 st = a.seg_start[s];
     if (st >= lo && st < hi) s_f[st - lo] = 1;
   }
