@@ -482,9 +482,11 @@ int tile_bucket_launch(fmcs_ctx* ctx, TileSortArgs<K> a, u32* fb, u32 grid, doub
   if (grid == 0) return FMCS_OK;
   Prof pf(ctx, "k_tile_bucket", 2.0 * rows * (sizeof(K) + 4));
   if (T <= 512)
-    hipLaunchKernelGGL((k_tile_bucket<K, 2>), dim3(grid), dim3(256), 0, ctx->stream, a, fb);
+    hipLaunchKernelGGL((k_tile_bucket<K, 2>), dim3(grid), dim3(256), 0, ctx->stream, a, fb, a.tile_lo, a.tile_hi,
+                       a.tile_s0, a.tile_s1, a.wlist);
   else
-    hipLaunchKernelGGL((k_tile_bucket<K, 4>), dim3(grid), dim3(256), 0, ctx->stream, a, fb);
+    hipLaunchKernelGGL((k_tile_bucket<K, 4>), dim3(grid), dim3(256), 0, ctx->stream, a, fb, a.tile_lo, a.tile_hi,
+                       a.tile_s0, a.tile_s1, a.wlist);
   return launch_check(ctx, "k_tile_bucket");
 }
 
@@ -1118,7 +1120,8 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
       const u64 g = (persist_grid(ctx, (const void*)fn_bucket, d.nw_max) + XCDS - 1) / XCDS * XCDS;
       // rows read (key, oid) and written (oid, and the key when wanted)
       const double per = keys_out ? 2.0 * (sizeof(K) + 4) : sizeof(K) + 8.0;
-      TRY(klaunch(ctx, "k_tile_bucket", fn_bucket, g, 0.0, S + LV_ROWS_SMALL, per, a, fb));
+      TRY(klaunch(ctx, "k_tile_bucket", fn_bucket, g, 0.0, S + LV_ROWS_SMALL, per, a, fb, a.tile_lo, a.tile_hi,
+                  a.tile_s0, a.tile_s1, a.wlist));
     }
     {
       // refused windows: the radix tile sort over the bits that vary inside
@@ -1143,11 +1146,13 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
       c.wlist = nullptr; c.wcount = S + LV_NSINGLE; c.fblist = fblist2; c.fbcount = S + LV_FB2;
       c.size_lo = 0; c.size_hi = T;
       const u64 g = (persist_grid(ctx, (const void*)fn_bucket, d.nsingle_max) + XCDS - 1) / XCDS * XCDS;
-      TRY(klaunch(ctx, "k_tile_bucket", fn_bucket, g, 0.0, nullptr, 0.0, c, fb));
+      TRY(klaunch(ctx, "k_tile_bucket", fn_bucket, g, 0.0, nullptr, 0.0, c, fb, c.tile_lo, c.tile_hi, c.tile_s0,
+                  c.tile_s1, c.wlist));
       if (d.single_max > T) {
         c.size_lo = T; c.size_hi = d.single_max;
         const u64 gb = (persist_grid(ctx, (const void*)fn_bucket2, d.nsingle_max) + XCDS - 1) / XCDS * XCDS;
-        TRY(klaunch(ctx, "k_tile_bucket_2t", fn_bucket2, gb, 0.0, nullptr, 0.0, c, fb));
+        TRY(klaunch(ctx, "k_tile_bucket_2t", fn_bucket2, gb, 0.0, nullptr, 0.0, c, fb, c.tile_lo, c.tile_hi,
+                    c.tile_s0, c.tile_s1, c.wlist));
       }
       c.wcount = nullptr;
       c.rklist = rkl2; c.rkcount = S + LV_RK2;

@@ -89,8 +89,10 @@ __global__ void k_onesweep(const K*, const u32*, K*, u32*, const u32*, const u32
                            u64, u32*, u32*, uint8_t*, u32);
 template <class K, int E>
 __global__ void k_tile_sort(TileSortArgs<K>);
+// the window descriptor arrays (tile_lo, tile_hi, tile_s0, tile_s1, wlist of
+// the argument block) are passed again as restrict pointers
 template <class K, int E>
-__global__ void k_tile_bucket(TileSortArgs<K>, u32*);
+__global__ void k_tile_bucket(TileSortArgs<K>, u32*, const u32*, const u32*, const u32*, const u32*, const u32*);
 template <class K, int E>
 __global__ void k_tile_rank(TileSortArgs<K>);
 __global__ void k_windows(const u32*, const u32*, u32, u32, u32, const u32*, const u32*, u32, u32*, u32*, u32*, u32*,

@@ -891,13 +891,19 @@ struct WinRows {
 // them, and memory operations complete in order, so it need not wait for
 // the rows.  Both without per-row branches: indices past the end repeat the
 // last segment / row (a flag set twice; rows past cnt are never used).
+// The descriptor arrays come in as restrict kernel parameters (they are
+// the same as a.tile_lo ... a.tile_s1): the compiler then knows that the
+// kernel's stores leave them alone and reads them with scalar loads, whose
+// waits do not hold the vector loads up.
 template <class K, int E>
-__device__ __forceinline__ void load_window(const TileSortArgs<K>& a, u32 w, WinRows<K, E>& r) {
+__device__ __forceinline__ void load_window(const TileSortArgs<K>& a, u32 w, const u32* __restrict__ tlo,
+                                            const u32* __restrict__ thi, const u32* __restrict__ ts0,
+                                            const u32* __restrict__ ts1, WinRows<K, E>& r) {
   r.w = w;
-  r.lo = a.tile_lo[w];
-  r.hi = a.tile_hi[w];
-  r.s0 = a.tile_s0[w];
-  r.s1 = a.tile_s1[w];
+  r.lo = tlo[w];
+  r.hi = thi[w];
+  r.s0 = ts0[w];
+  r.s1 = ts1[w];
   u32 cnt = r.hi > r.lo ? r.hi - r.lo : 0u;
   if (a.size_hi && (cnt <= a.size_lo || cnt > a.size_hi)) {
     r.hi = r.lo;  // another launch's window: skipped
@@ -1192,20 +1198,23 @@ __global__ __launch_bounds__(256) void k_tile_rank(TileSortArgs<K> a) {
 // window) and half as many bins as rows keep it at 62 VGPRs and 24.6 KB of
 // LDS for 1024-row tiles, 6 blocks a CU.
 template <class K, int E>
-__global__ __launch_bounds__(256) void k_tile_bucket(TileSortArgs<K> a, u32* __restrict__ fallback) {
+__global__ __launch_bounds__(256) void k_tile_bucket(TileSortArgs<K> a, u32* __restrict__ fallback,
+                                                     const u32* __restrict__ tlo, const u32* __restrict__ thi,
+                                                     const u32* __restrict__ ts0, const u32* __restrict__ ts1,
+                                                     const u32* __restrict__ wl) {
   if (a.wcount) {
     const u32 cnt = *a.wcount;
     const u32 xcd = blockIdx.x % XCDS, per_x = gridDim.x / XCDS;
     const u32 hi = (u32)((u64)cnt * (xcd + 1) / XCDS);
     for (u32 b = (u32)((u64)cnt * xcd / XCDS) + blockIdx.x / XCDS; b < hi; b += per_x) {
       WinRows<K, E> r;
-      load_window<K, E>(a, a.wlist ? a.wlist[b] : b, r);
+      load_window<K, E>(a, wl ? wl[b] : b, tlo, thi, ts0, ts1, r);
       tile_bucket_body<K, E>(a, r, fallback);
       __syncthreads();
     }
   } else {
     WinRows<K, E> r;
-    load_window<K, E>(a, a.wlist ? a.wlist[blockIdx.x] : blockIdx.x, r);
+    load_window<K, E>(a, wl ? wl[blockIdx.x] : blockIdx.x, tlo, thi, ts0, ts1, r);
     tile_bucket_body<K, E>(a, r, fallback);
   }
 }
@@ -1551,9 +1560,12 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
   template __global__ void k_tile_sort<K, 2>(TileSortArgs<K>);                                                      \
   template __global__ void k_tile_rank<K, 4>(TileSortArgs<K>);                                                      \
   template __global__ void k_tile_rank<K, 2>(TileSortArgs<K>);                                                      \
-  template __global__ void k_tile_bucket<K, 4>(TileSortArgs<K>, u32*);                                             \
-  template __global__ void k_tile_bucket<K, 2>(TileSortArgs<K>, u32*);                                             \
-  template __global__ void k_tile_bucket<K, 8>(TileSortArgs<K>, u32*);                                             \
+  template __global__ void k_tile_bucket<K, 4>(TileSortArgs<K>, u32*, const u32*, const u32*, const u32*,          \
+                                               const u32*, const u32*);                                           \
+  template __global__ void k_tile_bucket<K, 2>(TileSortArgs<K>, u32*, const u32*, const u32*, const u32*,          \
+                                               const u32*, const u32*);                                           \
+  template __global__ void k_tile_bucket<K, 8>(TileSortArgs<K>, u32*, const u32*, const u32*, const u32*,          \
+                                               const u32*, const u32*);                                           \
   template __global__ void k_tile_sort<K, 8>(TileSortArgs<K>);                                                      \
   template __global__ void k_tile_rank<K, 8>(TileSortArgs<K>);                                                      \
   template __global__ void k_copy_segs<K>(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, u32,  \
