@@ -784,17 +784,11 @@ constexpr u64 DD_WORDS = DD_HDR + (u64)DD_ROUNDS_MAX * DD_LVL_MAX * DD_LVL_WORDS
 enum { LV_NSEG, LV_WCOUNT, LV_ROWS_SMALL, LV_ROWS_LARGE, LV_NLARGE, LV_NTILES, LV_SCAN_A, LV_SCAN_B, LV_OS, LV_FB,
        LV_NSINGLE, LV_FB2, LV_RK, LV_RK2 };
 
-// p[0 .. ceil(*cnt * mult / div) + add) = v, bounded by max_words
-int dd_fill(fmcs_ctx* ctx, u32* p, const u32* cnt, u32 mult, u32 div, u32 add, u32 v, u64 max_words) {
-  return klaunch(ctx, "k_fill", kd_fill, cap_grid((max_words + 255) / 256, 1024), 0.0, nullptr, 0.0, p, cnt, mult,
-                 div, add, v);
-}
-
-int dd_scan(fmcs_ctx* ctx, const u32* in, u32* out, const u32* n_d, u64 n_max, u32* counter, u32* total) {
-  u32* st;
+// Exclusive scan of a device-sized list.  st holds ceil(n_max / SCAN_TILE)
+// look-back status words that the kernel before it cleared (the first
+// ceil(*n_d / SCAN_TILE) of them are enough).
+int dd_scan(fmcs_ctx* ctx, const u32* in, u32* out, const u32* n_d, u64 n_max, u32* st, u32* counter, u32* total) {
   const u64 tiles_max = std::max<u64>(1, (n_max + SCAN_TILE - 1) / SCAN_TILE);
-  TRY(getbuf_t(ctx, WS_SCANST, tiles_max, &st));
-  TRY(dd_fill(ctx, st, n_d, 1, SCAN_TILE, 0, 0, tiles_max));
   // blocks take tiles from the counter, so a few hundred cover any list:
   // a larger grid only adds launch time (most lists are far below their bound)
   return klaunch(ctx, "k_scan", kd_scan_excl<SCAN_ITEMS>,
@@ -1077,6 +1071,9 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
   u32 *sgl, *fblist2;
   TRY(getbuf_t(ctx, WS_SGL, 3 * m_single, &sgl));
   TRY(getbuf_t(ctx, WS_FBLIST2, m_single, &fblist2));
+  // both scans of a level: kd_large_flags and kd_large_compact clear the words
+  u32* scan_st;
+  TRY(getbuf_t(ctx, WS_SCANST, (m_flag + SCAN_TILE - 1) / SCAN_TILE, &scan_st));
   u32 *slo = sgl, *shi = sgl + m_single, *sseg = sgl + 2 * m_single;
   u32 *rkl, *rkl2;
   TRY(getbuf_t(ctx, WS_RKL, m_nw, &rkl));
@@ -1106,8 +1103,8 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
                 S + LV_WCOUNT, d.single_max, slo, shi, sseg, S + LV_NSINGLE));
     // grid-stride over a device count far below its host bound: 512 blocks
     TRY(klaunch(ctx, "k_large_flags", kd_large_flags, cap_grid((d.nseg_max + 255) / 256, 512), 0.0, nullptr, 0.0,
-                (const u32*)csz, nseg_p, d.single_max, flag, S + LV_ROWS_SMALL));
-    TRY(dd_scan(ctx, flag, pos, nseg_p, d.nseg_max, S + LV_SCAN_A, S + LV_NLARGE));
+                (const u32*)csz, nseg_p, d.single_max, flag, S + LV_ROWS_SMALL, scan_st));
+    TRY(dd_scan(ctx, flag, pos, nseg_p, d.nseg_max, scan_st, S + LV_SCAN_A, S + LV_NLARGE));
     TileSortArgs<K> a;
     std::memset(&a, 0, sizeof(a));
     a.kin[0] = keys; a.vin[0] = vals; a.kin[1] = kb_; a.vin[1] = vb_;
@@ -1169,10 +1166,12 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
     K* dkey = (l & 1) ? keys : kb_;
     u32* dval = (l & 1) ? vals : vb_;
     if (d.last && !(l & 1)) break;  // every key bit consumed; large segments already in place
-    TRY(klaunch(ctx, "k_large_compact", kd_large_compact, cap_grid((d.nseg_max + 255) / 256, 2048), 0.0, nullptr, 0.0,
-                (const u32*)cst, (const u32*)csz, nseg_p, d.single_max, (const u32*)pos, (u32)ONESWEEP_TILE, llo,
-                lhi, ltiles));
-    TRY(dd_scan(ctx, ltiles, tbase, S + LV_NLARGE, d.nlarge_max, S + LV_SCAN_B, S + LV_NTILES));
+    // also clears the scan status below and (levels that scatter) the digit counts
+    TRY(klaunch(ctx, "k_large_compact", kd_large_compact,
+                cap_grid(std::max<u64>(d.nseg_max, (d.last ? 0 : d.nlarge_max * RADIX)) / 256 + 1, 2048), 0.0,
+                nullptr, 0.0, (const u32*)cst, (const u32*)csz, nseg_p, d.single_max, (const u32*)pos,
+                (u32)ONESWEEP_TILE, llo, lhi, ltiles, (const u32*)(S + LV_NLARGE), scan_st, d.last ? nullptr : hist));
+    TRY(dd_scan(ctx, ltiles, tbase, S + LV_NLARGE, d.nlarge_max, scan_st, S + LV_SCAN_B, S + LV_NTILES));
     if (d.last) {
       // odd level, every key bit consumed: move the large segments back
       TRY(klaunch(ctx, "k_copy_segs", kd_copy_segs<K>, cap_grid(d.ntiles_max, 2048), 0.0, S + LV_ROWS_LARGE,
@@ -1180,7 +1179,6 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
                   (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES), (u32)ONESWEEP_TILE));
       break;
     }
-    TRY(dd_fill(ctx, hist, S + LV_NLARGE, RADIX, 1, 0, 0, d.nlarge_max * RADIX));
     TRY(klaunch(ctx, "k_seg_hist", kd_seg_hist<K, ONESWEEP_ITEMS, RADIX_BITS>, cap_grid(d.ntiles_max, 2048), 0.0,
                 S + LV_ROWS_LARGE, (double)sizeof(K), (const K*)skey, (const u32*)llo, (const u32*)lhi,
                 (const u32*)tbase, (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES), d.shift, d.D, hist,

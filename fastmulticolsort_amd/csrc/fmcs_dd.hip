@@ -22,12 +22,6 @@ __global__ void kd_mul(const u32* __restrict__ a, u32 m, u32* __restrict__ out) 
   if (blockIdx.x == 0 && threadIdx.x == 0) *out = *a * m;
 }
 
-// p[0 .. ceil(*cnt * mult / div) + add) = v
-__global__ void kd_fill(u32* __restrict__ p, const u32* __restrict__ cnt, u32 mult, u32 div, u32 add, u32 v) {
-  const u64 words = ((u64)*cnt * mult + div - 1) / div + add;
-  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (u64)gridDim.x * blockDim.x) p[i] = v;
-}
-
 __global__ void kd_offsets_to_list(const u32* __restrict__ off, const u32* __restrict__ nseg_d, u32* __restrict__ st,
                                    u32* __restrict__ sz) {
   const u32 nseg = *nseg_d;
@@ -120,10 +114,14 @@ __global__ __launch_bounds__(256) void kd_windows(const u32* __restrict__ st, co
   }
 }
 
-// flag[i] = segment i is large; rows[0] += small rows, rows[1] += large rows
+// flag[i] = segment i is large; rows[0] += small rows, rows[1] += large rows.
+// Also clears the look-back status words of the scan over flag that
+// follows (one per SCAN_TILE segments), so that scan needs no fill launch.
 __global__ void kd_large_flags(const u32* __restrict__ sz, const u32* __restrict__ n_d, u32 small_max,
-                               u32* __restrict__ flag, u32* __restrict__ rows) {
+                               u32* __restrict__ flag, u32* __restrict__ rows, u32* __restrict__ scan_status) {
   const u32 nseg = *n_d;
+  const u32 nst = (nseg + SCAN_TILE - 1) / SCAN_TILE;
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < nst; i += gridDim.x * blockDim.x) scan_status[i] = 0;
   u32 a = 0, b = 0;
   for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < nseg; i += gridDim.x * blockDim.x) {
     u32 s = sz[i];
@@ -205,10 +203,21 @@ __global__ __launch_bounds__(256) void kd_scan_excl(const u32* __restrict__ in, 
   }
 }
 
+// The large segments' (lo, hi, tiles) at their scanned positions.  Also
+// clears what the next kernels of the level expect zeroed, sized by the
+// large-segment count *nl_d: the look-back status words of the scan over
+// ltiles and (hist set) the nl x 256 per-segment digit counts.
 __global__ void kd_large_compact(const u32* __restrict__ st, const u32* __restrict__ sz, const u32* __restrict__ n_d,
                                  u32 small_max, const u32* __restrict__ pos, u32 tile_rows, u32* __restrict__ llo,
-                                 u32* __restrict__ lhi, u32* __restrict__ ltiles) {
+                                 u32* __restrict__ lhi, u32* __restrict__ ltiles, const u32* __restrict__ nl_d,
+                                 u32* __restrict__ scan_status, u32* __restrict__ hist) {
   const u32 nseg = *n_d;
+  const u32 nl = *nl_d;
+  const u32 gtid = blockIdx.x * blockDim.x + threadIdx.x, gstride = gridDim.x * blockDim.x;
+  const u32 nst = (nl + SCAN_TILE - 1) / SCAN_TILE;
+  for (u32 i = gtid; i < nst; i += gstride) scan_status[i] = 0;
+  if (hist)
+    for (u64 i = gtid; i < (u64)nl * RADIX; i += gstride) hist[i] = 0;
   for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < nseg; i += gridDim.x * blockDim.x) {
     u32 s = sz[i];
     if (s <= small_max) continue;
