@@ -786,14 +786,17 @@ enum { LV_NSEG, LV_WCOUNT, LV_ROWS_SMALL, LV_ROWS_LARGE, LV_NLARGE, LV_NTILES, L
 
 // Exclusive scan of a device-sized list.  st holds ceil(n_max / SCAN_TILE)
 // look-back status words that the kernel before it cleared (the first
-// ceil(*n_d / SCAN_TILE) of them are enough).
-int dd_scan(fmcs_ctx* ctx, const u32* in, u32* out, const u32* n_d, u64 n_max, u32* st, u32* counter, u32* total) {
+// ceil(*n_d / SCAN_TILE) of them are enough).  flags: in holds segment
+// sizes, the scan is over (size > small_max), and rows (if set) [0..1] get
+// the small and large rows.
+int dd_scan(fmcs_ctx* ctx, const u32* in, u32* out, const u32* n_d, u64 n_max, u32* st, u32* counter, u32* total,
+            bool flags = false, u32 small_max = 0, u32* rows = nullptr) {
   const u64 tiles_max = std::max<u64>(1, (n_max + SCAN_TILE - 1) / SCAN_TILE);
   // blocks take tiles from the counter, so a few hundred cover any list:
   // a larger grid only adds launch time (most lists are far below their bound)
-  return klaunch(ctx, "k_scan", kd_scan_excl<SCAN_ITEMS>,
-                 cap_grid(tiles_max, std::min<u64>(512, resident(ctx, (const void*)kd_scan_excl<SCAN_ITEMS>))), 0.0,
-                 nullptr, 0.0, in, out, n_d, st, counter, total);
+  auto fn = flags ? kd_scan_excl<SCAN_ITEMS, true> : kd_scan_excl<SCAN_ITEMS, false>;
+  return klaunch(ctx, "k_scan", fn, cap_grid(tiles_max, std::min<u64>(512, resident(ctx, (const void*)fn))), 0.0,
+                 nullptr, 0.0, in, out, n_d, st, counter, total, small_max, rows);
 }
 
 // LSD radix sort over every pass (no host-side trivial-pass skipping).
@@ -868,7 +871,7 @@ int radix_sort_dd_t(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* v
                 nullptr, 0.0, sk, (const u32*)cst, (const u32*)cst + 1, (const u32*)cst + 2, one, (const u32*)cst + 3,
                 p * RB, (u32)RB, hp, tcount, bad, dd + DD_ERR));
     TRY(klaunch(ctx, "k_digit_base", kd_digit_base<RB>, 1, 0.0, nullptr, 0.0, (const u32*)hp, (const u32*)cst,
-                dbase + (u64)p * R, (u32*)nullptr, (u32*)nullptr, one));
+                dbase + (u64)p * R, (u32*)nullptr, (u32*)nullptr, one, (u32*)nullptr));
     const double pb = HASP ? 2.0 * n * sizeof(P) : 0.0;
     const int rc = scatter_pass<K, RB, P, false, true>(ctx, "k_onesweep", sk, sv, dk, dv, cst, cst + 1, cst + 2, one, cst + 3,
                                           ntiles, dbase + (u64)p * R, tcount, p * RB, (u32)RB,
@@ -956,7 +959,7 @@ int radix_sort_dd_fused(fmcs_ctx* ctx, const K* kin, const u64* pin, u32 keybits
                   (const u32*)cst + 2, one, (const u32*)cst + 3, 0u, dbits, hp, tcount, 0ull, (u32*)nullptr));
     }
     TRY(klaunch(ctx, "k_digit_base", kd_digit_base<RB>, 1, 0.0, nullptr, 0.0, (const u32*)hp, (const u32*)cst,
-                dbase + (u64)p * R, (u32*)nullptr, (u32*)nullptr, one));
+                dbase + (u64)p * R, (u32*)nullptr, (u32*)nullptr, one, (u32*)nullptr));
     if (p == 0) {
       // the next digit: c1's bits RB .. 2 RB - 1
       TRY((scatter_pass<K, RB, u64, true, true>(ctx, "k_onesweep_fuse", kin, (const u32*)nullptr, (K*)nullptr, dv, cst, cst + 1,
@@ -1051,7 +1054,7 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
   TRY(getbuf_t(ctx, WS_SZ, std::max<u64>(m_list[0], 1), &lsz[0]));
   TRY(getbuf_t(ctx, WS_BST, std::max<u64>(m_list[1], 1), &lst[1]));
   TRY(getbuf_t(ctx, WS_BSZ, std::max<u64>(m_list[1], 1), &lsz[1]));
-  u32 *tlo, *thi, *ts0, *ts1, *wlist, *fb, *fblist, *flag, *pos, *llo, *lhi, *ltiles, *tbase, *hist, *dbase, *tcount;
+  u32 *tlo, *thi, *ts0, *ts1, *wlist, *fb, *fblist, *pos, *llo, *lhi, *ltiles, *tbase, *hist, *dbase, *tcount;
   TRY(getbuf_t(ctx, WS_TLO, m_nw, &tlo));
   TRY(getbuf_t(ctx, WS_THI, m_nw, &thi));
   TRY(getbuf_t(ctx, WS_TS0, m_nw, &ts0));
@@ -1059,7 +1062,6 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
   TRY(getbuf_t(ctx, WS_WLIST, m_nw, &wlist));
   TRY(getbuf_t(ctx, WS_FALLBACK, m_nw, &fb));
   TRY(getbuf_t(ctx, WS_FBLIST, m_nw, &fblist));
-  TRY(getbuf_t(ctx, WS_FLAG, m_flag, &flag));
   TRY(getbuf_t(ctx, WS_POS, m_flag, &pos));
   TRY(getbuf_t(ctx, WS_LLO, m_large, &llo));
   TRY(getbuf_t(ctx, WS_LHI, m_large, &lhi));
@@ -1071,7 +1073,7 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
   u32 *sgl, *fblist2;
   TRY(getbuf_t(ctx, WS_SGL, 3 * m_single, &sgl));
   TRY(getbuf_t(ctx, WS_FBLIST2, m_single, &fblist2));
-  // both scans of a level: kd_large_flags and kd_large_compact clear the words
+  // both scans of a level: kd_windows and kd_large_compact clear the words
   u32* scan_st;
   TRY(getbuf_t(ctx, WS_SCANST, (m_flag + SCAN_TILE - 1) / SCAN_TILE, &scan_st));
   u32 *slo = sgl, *shi = sgl + m_single, *sseg = sgl + 2 * m_single;
@@ -1100,11 +1102,11 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
     u32* csz = lsz[cur];
     TRY(klaunch(ctx, "k_windows", kd_windows, cap_grid((d.nseg_max + 2047) / 2048, 2048), 0.0, nullptr, 0.0,
                 (const u32*)cst, (const u32*)csz, nseg_p, d.Wl, d.sm, plo, ptb, d.wpt, tlo, thi, ts0, ts1, wlist,
-                S + LV_WCOUNT, d.single_max, slo, shi, sseg, S + LV_NSINGLE));
-    // grid-stride over a device count far below its host bound: 512 blocks
-    TRY(klaunch(ctx, "k_large_flags", kd_large_flags, cap_grid((d.nseg_max + 255) / 256, 512), 0.0, nullptr, 0.0,
-                (const u32*)csz, nseg_p, d.single_max, flag, S + LV_ROWS_SMALL, scan_st));
-    TRY(dd_scan(ctx, flag, pos, nseg_p, d.nseg_max, scan_st, S + LV_SCAN_A, S + LV_NLARGE));
+                S + LV_WCOUNT, d.single_max, slo, shi, sseg, S + LV_NSINGLE, scan_st));
+    // each segment's place among the large ones (above single_max rows);
+    // the row counts only feed the per-kernel byte figures of a profiled run
+    TRY(dd_scan(ctx, csz, pos, nseg_p, d.nseg_max, scan_st, S + LV_SCAN_A, S + LV_NLARGE, true, d.single_max,
+                (ctx->prof && !ctx->capturing) ? S + LV_ROWS_SMALL : nullptr));
     TileSortArgs<K> a;
     std::memset(&a, 0, sizeof(a));
     a.kin[0] = keys; a.vin[0] = vals; a.kin[1] = kb_; a.vin[1] = vb_;
@@ -1185,13 +1187,11 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
                 tcount, (u64)0, (u32*)nullptr));
     const int nxt = cur ^ 1;
     TRY(klaunch(ctx, "k_digit_base", kd_digit_base<RADIX_BITS>, cap_grid(d.nlarge_max, 2048), 0.0, nullptr, 0.0,
-                (const u32*)hist, (const u32*)llo, dbase, lst[nxt], lsz[nxt], (const u32*)(S + LV_NLARGE)));
+                (const u32*)hist, (const u32*)llo, dbase, lst[nxt], lsz[nxt], (const u32*)(S + LV_NLARGE),
+                l + 1 < L.size() ? S + DD_LVL_WORDS + LV_NSEG : nullptr));
     TRY(scatter_pass<K>(ctx, "k_onesweep_msd", skey, sval, dkey, dval, llo, lhi, tbase, S + LV_NLARGE,
                         S + LV_NTILES, d.ntiles_max, dbase, tcount, d.shift, d.D, 0.0, S + LV_ROWS_LARGE,
                         2.0 * (sizeof(K) + 4)));
-    if (l + 1 < L.size())
-      TRY(klaunch(ctx, "k_set", kd_mul, 1, 0.0, nullptr, 0.0, (const u32*)(S + LV_NLARGE), (u32)RADIX,
-                  S + DD_LVL_WORDS + LV_NSEG));
     plo = llo;
     ptb = tbase;
   }

@@ -17,11 +17,6 @@ __global__ void kd_set4(u32* __restrict__ p, u32 a, u32 b, u32 c, u32 d) {
   }
 }
 
-// *out = *a * m
-__global__ void kd_mul(const u32* __restrict__ a, u32 m, u32* __restrict__ out) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *out = *a * m;
-}
-
 __global__ void kd_offsets_to_list(const u32* __restrict__ off, const u32* __restrict__ nseg_d, u32* __restrict__ st,
                                    u32* __restrict__ sz) {
   const u32 nseg = *nseg_d;
@@ -58,11 +53,14 @@ __global__ __launch_bounds__(256) void kd_windows(const u32* __restrict__ st, co
                                                   u32* __restrict__ ts1, u32* __restrict__ wlist,
                                                   u32* __restrict__ wcount, u32 single_max, u32* __restrict__ slo,
                                                   u32* __restrict__ shi, u32* __restrict__ sseg,
-                                                  u32* __restrict__ scount) {
+                                                  u32* __restrict__ scount, u32* __restrict__ scan_status) {
   __shared__ u32 s_w[WIN_CHUNK];
   __shared__ u32 s_n, s_base;
   const u32 nseg = *nseg_d;
   const u32 lane = __lane_id();
+  // the look-back status words of the level's scan over the segment sizes
+  const u32 nst = (nseg + SCAN_TILE - 1) / SCAN_TILE;
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < nst; i += gridDim.x * blockDim.x) scan_status[i] = 0;
   for (u64 c0 = (u64)blockIdx.x * WIN_CHUNK; c0 < nseg; c0 += (u64)gridDim.x * WIN_CHUNK) {
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
@@ -114,47 +112,17 @@ __global__ __launch_bounds__(256) void kd_windows(const u32* __restrict__ st, co
   }
 }
 
-// flag[i] = segment i is large; rows[0] += small rows, rows[1] += large rows.
-// Also clears the look-back status words of the scan over flag that
-// follows (one per SCAN_TILE segments), so that scan needs no fill launch.
-__global__ void kd_large_flags(const u32* __restrict__ sz, const u32* __restrict__ n_d, u32 small_max,
-                               u32* __restrict__ flag, u32* __restrict__ rows, u32* __restrict__ scan_status) {
-  const u32 nseg = *n_d;
-  const u32 nst = (nseg + SCAN_TILE - 1) / SCAN_TILE;
-  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < nst; i += gridDim.x * blockDim.x) scan_status[i] = 0;
-  u32 a = 0, b = 0;
-  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < nseg; i += gridDim.x * blockDim.x) {
-    u32 s = sz[i];
-    bool large = s > small_max;
-    flag[i] = large ? 1u : 0u;
-    a += large ? 0u : s;
-    b += large ? s : 0u;
-  }
-#pragma unroll
-  for (int o = WAVE / 2; o > 0; o >>= 1) {
-    a += __shfl_down(a, o);
-    b += __shfl_down(b, o);
-  }
-  __shared__ u32 s_ab[2];
-  if (threadIdx.x == 0) s_ab[0] = s_ab[1] = 0;
-  __syncthreads();
-  if ((threadIdx.x & (WAVE - 1)) == 0) {
-    if (a) atomicAdd(&s_ab[0], a);
-    if (b) atomicAdd(&s_ab[1], b);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (s_ab[0]) atomicAdd(rows, s_ab[0]);
-    if (s_ab[1]) atomicAdd(rows + 1, s_ab[1]);
-  }
-}
-
 // Exclusive scan with decoupled look-back, persistent: blocks take tiles from
 // the counter until ceil(n / TILE) are done.  *total = the sum (0 when n = 0).
-template <int ITEMS>
+// FLAGS: in holds segment sizes and the scanned value is (size > small_max),
+// so out[i] is segment i's place among the large segments and *total their
+// count; rows (if set) [0] and [1] get the small and the large segments'
+// rows.
+template <int ITEMS, bool FLAGS>
 __global__ __launch_bounds__(256) void kd_scan_excl(const u32* __restrict__ in, u32* __restrict__ out,
                                                     const u32* __restrict__ n_d, u32* __restrict__ status,
-                                                    u32* __restrict__ counter, u32* __restrict__ total) {
+                                                    u32* __restrict__ counter, u32* __restrict__ total,
+                                                    u32 small_max, u32* __restrict__ rows) {
   constexpr int NT = 256, TILE = NT * ITEMS;
   __shared__ u32 s_tile, s_excl;
   __shared__ u32 s_tmp[NT / WAVE + 1];
@@ -165,6 +133,9 @@ __global__ __launch_bounds__(256) void kd_scan_excl(const u32* __restrict__ in, 
     if (blockIdx.x == 0 && tid == 0) *total = 0;
     return;
   }
+  u32 rs = 0, rl = 0;  // FLAGS: this thread's small and large rows
+  __shared__ u32 s_rows[2];
+  if (FLAGS && tid == 0) s_rows[0] = s_rows[1] = 0;
   for (;;) {
     if (tid == 0) s_tile = atomicAdd(counter, 1u);
     __syncthreads();
@@ -176,6 +147,12 @@ __global__ __launch_bounds__(256) void kd_scan_excl(const u32* __restrict__ in, 
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
       v[j] = (base + j < n) ? in[base + j] : 0u;
+      if constexpr (FLAGS) {
+        const bool large = v[j] > small_max;
+        rs += large ? 0u : v[j];
+        rl += large ? v[j] : 0u;
+        v[j] = large ? 1u : 0u;
+      }
       sum += v[j];
     }
     u32 tot;
@@ -200,6 +177,23 @@ __global__ __launch_bounds__(256) void kd_scan_excl(const u32* __restrict__ in, 
       run += v[j];
     }
     __syncthreads();
+  }
+  if constexpr (FLAGS) {
+    // every thread left the loop together (s_tile is block-uniform); one
+    // global atomic per block and counter
+    if (rows) {
+      rs = __shfl(wave_incl_scan(rs), WAVE - 1);
+      rl = __shfl(wave_incl_scan(rl), WAVE - 1);
+      if ((tid & (WAVE - 1)) == 0) {
+        atomicAdd(&s_rows[0], rs);
+        atomicAdd(&s_rows[1], rl);
+      }
+      __syncthreads();
+      if (tid == 0) {
+        if (s_rows[0]) atomicAdd(rows, s_rows[0]);
+        if (s_rows[1]) atomicAdd(rows + 1, s_rows[1]);
+      }
+    }
   }
 }
 
@@ -422,10 +416,13 @@ __global__ __launch_bounds__(256) void kd_tscan_carry(u32* __restrict__ carry, c
 template <int RB>
 __global__ __launch_bounds__(256) void kd_digit_base(const u32* __restrict__ hist, const u32* __restrict__ lo,
                                                      u32* __restrict__ digit_base, u32* __restrict__ bstart,
-                                                     u32* __restrict__ bsize, const u32* __restrict__ nsets_d) {
+                                                     u32* __restrict__ bsize, const u32* __restrict__ nsets_d,
+                                                     u32* __restrict__ nbuckets) {
   constexpr u32 R = 1u << RB, DPT = R / 256;  // thread t owns digits t * DPT + q
   __shared__ u32 tmp[8];
   const u32 nsets = *nsets_d;
+  // (if set) the bucket count, the next level's segment count
+  if (nbuckets && blockIdx.x == 0 && threadIdx.x == 0) *nbuckets = nsets * R;
   for (u32 s = blockIdx.x; s < nsets; s += gridDim.x) {
     u32 c[DPT];
     u32 sum = 0;
@@ -503,9 +500,10 @@ template __global__ void kd_seg_hist<uint8_t, ONESWEEP_ITEMS, 8>(const uint8_t*,
 FMCS_DD_INST(u16)
 FMCS_DD_INST(u32)
 FMCS_DD_INST(u64)
-template __global__ void kd_scan_excl<8>(const u32*, u32*, const u32*, u32*, u32*, u32*);
+template __global__ void kd_scan_excl<8, false>(const u32*, u32*, const u32*, u32*, u32*, u32*, u32, u32*);
+template __global__ void kd_scan_excl<8, true>(const u32*, u32*, const u32*, u32*, u32*, u32*, u32, u32*);
 template __global__ void kd_tscan_local<8>(u32*, const u32*, const u32*, const u32*, u32*, u32*);
 template __global__ void kd_tscan_carry<8>(u32*, const u32*, const u32*);
-template __global__ void kd_digit_base<8>(const u32*, const u32*, u32*, u32*, u32*, const u32*);
+template __global__ void kd_digit_base<8>(const u32*, const u32*, u32*, u32*, u32*, const u32*, u32*);
 
 }  // namespace fmcs

@@ -130,13 +130,11 @@ __global__ void k_merge_pass(const K*, const u32*, K*, u32*, u64, u64);
 
 // Device-driven list kernels (fmcs_dd.hip): sizes come from device words.
 __global__ void kd_set4(u32*, u32, u32, u32, u32);
-__global__ void kd_mul(const u32*, u32, u32*);
 __global__ void kd_offsets_to_list(const u32*, const u32*, u32*, u32*);
 __global__ void kd_windows(const u32*, const u32*, const u32*, u32, u32, const u32*, const u32*, u32, u32*, u32*,
-                           u32*, u32*, u32*, u32*, u32, u32*, u32*, u32*, u32*);
-__global__ void kd_large_flags(const u32*, const u32*, u32, u32*, u32*, u32*);
-template <int ITEMS>
-__global__ void kd_scan_excl(const u32*, u32*, const u32*, u32*, u32*, u32*);
+                           u32*, u32*, u32*, u32*, u32, u32*, u32*, u32*, u32*, u32*);
+template <int ITEMS, bool FLAGS>
+__global__ void kd_scan_excl(const u32*, u32*, const u32*, u32*, u32*, u32*, u32, u32*);
 __global__ void kd_large_compact(const u32*, const u32*, const u32*, u32, const u32*, u32, u32*, u32*, u32*,
                                  const u32*, u32*, u32*);
 template <class K, int ITEMS, int RB>
@@ -147,7 +145,7 @@ __global__ void kd_tscan_local(u32*, const u32*, const u32*, const u32*, u32*, u
 template <int RB>
 __global__ void kd_tscan_carry(u32*, const u32*, const u32*);
 template <int RB>
-__global__ void kd_digit_base(const u32*, const u32*, u32*, u32*, u32*, const u32*);
+__global__ void kd_digit_base(const u32*, const u32*, u32*, u32*, u32*, const u32*, u32*);
 template <class K>
 __global__ void kd_copy_segs(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, const u32*,
                              const u32*, u32);
