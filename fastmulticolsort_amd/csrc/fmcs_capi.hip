@@ -25,7 +25,7 @@ enum Slot {
   WS_KA, WS_KB, WS_VB, WS_T0K, WS_T0V, WS_T1K, WS_T1V, WS_STATUS, WS_HIST, WS_DBASE, WS_SEGOFF, WS_SEGOFF2,
   WS_ST, WS_SZ, WS_BST, WS_BSZ, WS_FLAG, WS_POS, WS_LLO, WS_LHI, WS_LTILES, WS_TBASE, WS_TLO, WS_THI, WS_TS0,
   WS_TS1, WS_COMP, WS_COMP2, WS_GROUPTMP, WS_MISC, WS_HASH, WS_HASH2, WS_HASH3, WS_HT_TABLE, WS_HT_FIRST,
-  WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_DD, WS_SCANST, WS_FBLIST, WS_TCOUNT, WS_CARRY, WS_BCNT, WS_T0P, WS_T1P, WS_PAY, WS_SGL, WS_FBLIST2, WS_RKL, WS_RKL2, WS_DNEXT, WS_USER0,
+  WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_DD, WS_SCANST, WS_FBLIST, WS_TCOUNT, WS_CARRY, WS_BCNT, WS_T0P, WS_T1P, WS_PAY, WS_SGL, WS_FBLIST2, WS_RKL, WS_RKL2, WS_DNEXT, WS_TSEG, WS_USER0,
   WS_COUNT = WS_USER0 + 24
 };
 
@@ -285,19 +285,20 @@ int scatter_pass(fmcs_ctx* ctx, const char* name, const K* kin, const u32* vin, 
                  const u32* hi, const u32* tbase, const u32* nsegs_d, const u32* ntiles_d, u64 ntiles_max,
                  const u32* dbase, u32* tcount, u32 shift, u32 dbits, double bytes, const u32* dcount, double per,
                  const P* pin = nullptr, PayOut<P, FUSE>* pout = nullptr, u32 fshift = 0, u64 pbad = 0,
-                 u32* err = nullptr, u32* kside = nullptr, uint8_t* dnext = nullptr, u32 nshift = 0) {
+                 u32* err = nullptr, u32* kside = nullptr, uint8_t* dnext = nullptr, u32 nshift = 0,
+                 const u32* tseg = nullptr) {
   constexpr u32 R = 1u << RB;
   const u64 nch = std::max<u64>(1, (ntiles_max + TSCAN_CT - 1) / TSCAN_CT);
   u32* carry;
   TRY(getbuf_t(ctx, WS_CARRY, nch * (R + 1), &carry));
   u32* cflag = carry + nch * R;
   TRY(klaunch(ctx, "k_tile_scan", kd_tscan_local<RB>, cap_grid(nch, 2048), 0.0, nullptr, 0.0, tcount, tbase, nsegs_d,
-              ntiles_d, carry, cflag));
+              ntiles_d, carry, cflag, tseg));
   TRY(klaunch(ctx, "k_tile_scan", kd_tscan_carry<RB>, R, 0.0, nullptr, 0.0, carry, (const u32*)cflag, ntiles_d));
   auto kern = k_onesweep<K, ONESWEEP_ITEMS, RB, P, FUSE, LSD>;
   return klaunch(ctx, name, kern, onesweep_grid(ctx, (const void*)kern, ntiles_max), bytes, dcount, per, kin, vin,
                  kout, vout, lo, hi, tbase, 0u, dbase, (const u32*)tcount, (const u32*)carry, shift, dbits, nsegs_d,
-                 ntiles_d, pin, pout, fshift, pbad, err, kside, dnext, nshift);
+                 ntiles_d, pin, pout, fshift, pbad, err, kside, dnext, nshift, tseg);
 }
 
 // exclusive scan of n u32 values -> out; *total_h (host) if non-null
@@ -392,7 +393,7 @@ int radix_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, 
     }
     TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<K, ONESWEEP_ITEMS, RADIX_BITS>, cap_grid(ntiles, 2048),
                 (double)n * sizeof(K), nullptr, 0.0, sk, cst, cst + 1, cst + 2, cst + 4, cst + 3, p * RADIX_BITS,
-                (u32)RADIX_BITS, (u32*)nullptr, tcount, (u64)0, (u32*)nullptr));
+                (u32)RADIX_BITS, (u32*)nullptr, tcount, (u64)0, (u32*)nullptr, (u32*)nullptr));
     TRY((scatter_pass<K, RADIX_BITS, NoPay, false, true>)(ctx, "k_onesweep", sk, sv, dk, dv, cst, cst + 1, cst + 2,
                                                          cst + 4, cst + 3, ntiles,
                         dbase + (u64)p * RADIX, tcount, p * RADIX_BITS, (u32)RADIX_BITS, 2.0 * n * (sizeof(K) + 4),
@@ -622,7 +623,7 @@ int seg_sort_batched(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, cons
     const u32* ntiles_d = ctx->small_d + 49;
     TRY(klaunch(ctx, "k_seg_hist", kd_seg_hist<K, ONESWEEP_ITEMS, RADIX_BITS>, cap_grid(ntiles, 2048),
                 (double)large_rows * sizeof(K), nullptr, 0.0, skey, (const u32*)llo, (const u32*)lhi,
-                (const u32*)tbase, nlarge_d, ntiles_d, shift, D, hist, tcount, (u64)0, (u32*)nullptr));
+                (const u32*)tbase, nlarge_d, ntiles_d, shift, D, hist, tcount, (u64)0, (u32*)nullptr, (u32*)nullptr));
     const int nxt = cur ^ 1;
     u32 *nst, *nsz;
     TRY(getbuf_t(ctx, lst[nxt], (u64)nlarge * RADIX + 1, &nst));
@@ -869,7 +870,7 @@ int radix_sort_dd_t(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* v
     const u64 bad = (p == 0 && keybits < 8 * sizeof(K)) ? ~((1ull << keybits) - 1) : 0ull;
     TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<K, ONESWEEP_ITEMS, RB>, cap_grid(ntiles, 2048), (double)n * sizeof(K),
                 nullptr, 0.0, sk, (const u32*)cst, (const u32*)cst + 1, (const u32*)cst + 2, one, (const u32*)cst + 3,
-                p * RB, (u32)RB, hp, tcount, bad, dd + DD_ERR));
+                p * RB, (u32)RB, hp, tcount, bad, dd + DD_ERR, (u32*)nullptr));
     TRY(klaunch(ctx, "k_digit_base", kd_digit_base<RB>, 1, 0.0, nullptr, 0.0, (const u32*)hp, (const u32*)cst,
                 dbase + (u64)p * R, (u32*)nullptr, (u32*)nullptr, one, (u32*)nullptr));
     const double pb = HASP ? 2.0 * n * sizeof(P) : 0.0;
@@ -952,11 +953,13 @@ int radix_sort_dd_fused(fmcs_ctx* ctx, const K* kin, const u64* pin, u32 keybits
     if (p == 0) {
       TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<K, ONESWEEP_ITEMS, RB>, cap_grid(ntiles, 2048),
                   (double)n * sizeof(K), nullptr, 0.0, kin, (const u32*)cst, (const u32*)cst + 1,
-                  (const u32*)cst + 2, one, (const u32*)cst + 3, 0u, dbits, hp, tcount, kbad, dd + DD_ERR));
+                  (const u32*)cst + 2, one, (const u32*)cst + 3, 0u, dbits, hp, tcount, kbad, dd + DD_ERR,
+                  (u32*)nullptr));
     } else {
       TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<uint8_t, ONESWEEP_ITEMS, RB>, cap_grid(ntiles, 2048),
                   (double)n, nullptr, 0.0, (const uint8_t*)dnx, (const u32*)cst, (const u32*)cst + 1,
-                  (const u32*)cst + 2, one, (const u32*)cst + 3, 0u, dbits, hp, tcount, 0ull, (u32*)nullptr));
+                  (const u32*)cst + 2, one, (const u32*)cst + 3, 0u, dbits, hp, tcount, 0ull, (u32*)nullptr,
+                  (u32*)nullptr));
     }
     TRY(klaunch(ctx, "k_digit_base", kd_digit_base<RB>, 1, 0.0, nullptr, 0.0, (const u32*)hp, (const u32*)cst,
                 dbase + (u64)p * R, (u32*)nullptr, (u32*)nullptr, one, (u32*)nullptr));
@@ -1070,6 +1073,9 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
   TRY(getbuf_t(ctx, WS_HIST, m_large * RADIX, &hist));
   TRY(getbuf_t(ctx, WS_DBASE, m_large * RADIX, &dbase));
   TRY(getbuf_t(ctx, WS_TCOUNT, m_tiles * RADIX, &tcount));
+  // every MSD tile's segment: written by the histogram, read by the tile scan and the scatter
+  u32* tseg;
+  TRY(getbuf_t(ctx, WS_TSEG, m_tiles, &tseg));
   u32 *sgl, *fblist2;
   TRY(getbuf_t(ctx, WS_SGL, 3 * m_single, &sgl));
   TRY(getbuf_t(ctx, WS_FBLIST2, m_single, &fblist2));
@@ -1184,14 +1190,15 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
     TRY(klaunch(ctx, "k_seg_hist", kd_seg_hist<K, ONESWEEP_ITEMS, RADIX_BITS>, cap_grid(d.ntiles_max, 2048), 0.0,
                 S + LV_ROWS_LARGE, (double)sizeof(K), (const K*)skey, (const u32*)llo, (const u32*)lhi,
                 (const u32*)tbase, (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES), d.shift, d.D, hist,
-                tcount, (u64)0, (u32*)nullptr));
+                tcount, (u64)0, (u32*)nullptr, tseg));
     const int nxt = cur ^ 1;
     TRY(klaunch(ctx, "k_digit_base", kd_digit_base<RADIX_BITS>, cap_grid(d.nlarge_max, 2048), 0.0, nullptr, 0.0,
                 (const u32*)hist, (const u32*)llo, dbase, lst[nxt], lsz[nxt], (const u32*)(S + LV_NLARGE),
                 l + 1 < L.size() ? S + DD_LVL_WORDS + LV_NSEG : nullptr));
     TRY(scatter_pass<K>(ctx, "k_onesweep_msd", skey, sval, dkey, dval, llo, lhi, tbase, S + LV_NLARGE,
                         S + LV_NTILES, d.ntiles_max, dbase, tcount, d.shift, d.D, 0.0, S + LV_ROWS_LARGE,
-                        2.0 * (sizeof(K) + 4)));
+                        2.0 * (sizeof(K) + 4), (const NoPay*)nullptr, (NoPay*)nullptr, 0u, 0ull, (u32*)nullptr,
+                        (u32*)nullptr, (uint8_t*)nullptr, 0u, tseg));
     plo = llo;
     ptb = tbase;
   }

@@ -232,13 +232,14 @@ __global__ void kd_large_compact(const u32* __restrict__ st, const u32* __restri
 // digit.  Full tiles that start on a 16-byte boundary are read with 16-byte
 // loads (the rate of 4-byte loads is the limit for 16- and 32-bit keys).
 // A key with a bit in bad_mask (above the declared width) sets *err.
+// tile_seg (if set) gets every tile's segment, for the passes after it.
 template <class K, int ITEMS, int RB>
 __global__ __launch_bounds__(256) void kd_seg_hist(const K* __restrict__ keys, const u32* __restrict__ seg_lo,
                                                    const u32* __restrict__ seg_hi, const u32* __restrict__ tile_base,
                                                    const u32* __restrict__ nsegs_d, const u32* __restrict__ ntiles_d,
                                                    u32 shift, u32 dbits, u32* __restrict__ hist,
                                                    u32* __restrict__ tile_counts, u64 bad_mask,
-                                                   u32* __restrict__ err) {
+                                                   u32* __restrict__ err, u32* __restrict__ tile_seg) {
   constexpr int TILE = 256 * ITEMS;
   constexpr int PV = 16 / sizeof(K);           // keys per 16-byte vector
   constexpr int NV = TILE / PV / 256;          // vectors per thread in a full tile
@@ -261,7 +262,10 @@ __global__ __launch_bounds__(256) void kd_seg_hist(const K* __restrict__ keys, c
   u64 bad = 0;
   for (u32 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     for (u32 i = threadIdx.x; i < HCOPY * HSTRIDE; i += 256) sh[i] = 0;
-    if (threadIdx.x == 0) s_seg = upper_bound_m1(tile_base, nsegs, tile);
+    if (threadIdx.x == 0) {
+      s_seg = upper_bound_m1(tile_base, nsegs, tile);
+      if (tile_seg) tile_seg[tile] = s_seg;
+    }
     __syncthreads();
     const u32 seg = s_seg;
     if (seg != cur) {
@@ -324,12 +328,13 @@ __global__ __launch_bounds__(256) void kd_seg_hist(const K* __restrict__ keys, c
 // steps.  kd_tscan_local: for every chunk of TSCAN_CT tiles, thread d
 // replaces counts[t][d] by the sum of counts[t'][d] over earlier tiles t' of
 // the chunk in the same segment; carry[c][d] = that running sum at the chunk
-// end, cflag[c] = 1 when a segment starts inside chunk c.
+// end, cflag[c] = 1 when a segment starts inside chunk c.  tile_seg (if set)
+// holds every tile's segment (else found by search over tile_base).
 template <int RB>
 __global__ __launch_bounds__(256) void kd_tscan_local(u32* __restrict__ counts, const u32* __restrict__ tile_base,
                                                       const u32* __restrict__ nsegs_d,
                                                       const u32* __restrict__ ntiles_d, u32* __restrict__ carry,
-                                                      u32* __restrict__ cflag) {
+                                                      u32* __restrict__ cflag, const u32* __restrict__ tile_seg) {
   constexpr u32 R = 1u << RB;
   __shared__ u32 s_first[TSCAN_CT];
   const u32 nsegs = *nsegs_d, ntiles = *ntiles_d;
@@ -337,7 +342,11 @@ __global__ __launch_bounds__(256) void kd_tscan_local(u32* __restrict__ counts, 
   for (u32 c = blockIdx.x; c < nch; c += gridDim.x) {
     const u32 t0 = c * TSCAN_CT;
     const u32 m = min(ntiles - t0, TSCAN_CT);
-    if (threadIdx.x < m) s_first[threadIdx.x] = tile_base[upper_bound_m1(tile_base, nsegs, t0 + threadIdx.x)] == t0 + threadIdx.x;
+    if (threadIdx.x < m) {
+      const u32 t = t0 + threadIdx.x;
+      s_first[threadIdx.x] = tile_seg ? (t == 0 || tile_seg[t] != tile_seg[t - 1])
+                                      : tile_base[upper_bound_m1(tile_base, nsegs, t)] == t;
+    }
     __syncthreads();
     u32 any = 0;
     for (u32 d = threadIdx.x; d < R; d += 256) {
@@ -488,7 +497,7 @@ __global__ void kd_copy1(const u32* __restrict__ src, u32* __restrict__ dst) {
 
 #define FMCS_DD_INST(K)                                                                                            \
   template __global__ void kd_seg_hist<K, ONESWEEP_ITEMS, 8>(const K*, const u32*, const u32*, const u32*,           \
-                                              const u32*, const u32*, u32, u32, u32*, u32*, u64, u32*);            \
+                                              const u32*, const u32*, u32, u32, u32*, u32*, u64, u32*, u32*);      \
   template __global__ void kd_copy_segs<K>(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*,   \
                                            const u32*, const u32*, u32);                            \
   template __global__ void kd_copy_pairs<K>(const K*, const u32*, K*, u32*, u64);                                  \
@@ -496,13 +505,13 @@ __global__ void kd_copy1(const u32* __restrict__ src, u32* __restrict__ dst) {
 // the next-digit bytes of the packed round-1 passes
 template __global__ void kd_seg_hist<uint8_t, ONESWEEP_ITEMS, 8>(const uint8_t*, const u32*, const u32*, const u32*,
                                                                  const u32*, const u32*, u32, u32, u32*, u32*, u64,
-                                                                 u32*);
+                                                                 u32*, u32*);
 FMCS_DD_INST(u16)
 FMCS_DD_INST(u32)
 FMCS_DD_INST(u64)
 template __global__ void kd_scan_excl<8, false>(const u32*, u32*, const u32*, u32*, u32*, u32*, u32, u32*);
 template __global__ void kd_scan_excl<8, true>(const u32*, u32*, const u32*, u32*, u32*, u32*, u32, u32*);
-template __global__ void kd_tscan_local<8>(u32*, const u32*, const u32*, const u32*, u32*, u32*);
+template __global__ void kd_tscan_local<8>(u32*, const u32*, const u32*, const u32*, u32*, u32*, const u32*);
 template __global__ void kd_tscan_carry<8>(u32*, const u32*, const u32*);
 template __global__ void kd_digit_base<8>(const u32*, const u32*, u32*, u32*, u32*, const u32*, u32*);
 

@@ -86,7 +86,7 @@ using PayOut = typename std::conditional<FUSE, u64, P>::type;
 template <class K, int ITEMS, int RB, class P, bool FUSE, bool LSD>
 __global__ void k_onesweep(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, u32, const u32*,
                            const u32*, const u32*, u32, u32, const u32*, const u32*, const P*, PayOut<P, FUSE>*, u32,
-                           u64, u32*, u32*, uint8_t*, u32);
+                           u64, u32*, u32*, uint8_t*, u32, const u32*);
 template <class K, int E>
 __global__ void k_tile_sort(TileSortArgs<K>);
 // the window descriptor arrays (tile_lo, tile_hi, tile_s0, tile_s1, wlist of
@@ -139,9 +139,9 @@ __global__ void kd_large_compact(const u32*, const u32*, const u32*, u32, const 
                                  const u32*, u32*, u32*);
 template <class K, int ITEMS, int RB>
 __global__ void kd_seg_hist(const K*, const u32*, const u32*, const u32*, const u32*, const u32*, u32, u32, u32*,
-                            u32*, u64, u32*);
+                            u32*, u64, u32*, u32*);
 template <int RB>
-__global__ void kd_tscan_local(u32*, const u32*, const u32*, const u32*, u32*, u32*);
+__global__ void kd_tscan_local(u32*, const u32*, const u32*, const u32*, u32*, u32*, const u32*);
 template <int RB>
 __global__ void kd_tscan_carry(u32*, const u32*, const u32*);
 template <int RB>

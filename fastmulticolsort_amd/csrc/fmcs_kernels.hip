@@ -407,7 +407,8 @@ __global__ __launch_bounds__(256) void k_digit_base(const u32* __restrict__ hist
 // Without a payload, kside (if set) also gets every key >> fshift as a u32:
 // the high part of composite keys, for a boundary pass over that part alone.
 // LSD: one segment [seg_lo[0], seg_hi[0]) tiled from tile 0 (no search for
-// a tile's segment).
+// a tile's segment).  Else tile_seg (if set) holds every tile's segment;
+// without it the segment is found by search over tile_base.
 template <class K, int ITEMS, int RB, class P, bool FUSE, bool LSD>
 __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, const u32* __restrict__ vin,
                                                   K* __restrict__ kout, u32* __restrict__ vout,
@@ -419,7 +420,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
                                                   const P* __restrict__ pin, PayOut<P, FUSE>* __restrict__ pout,
                                                   u32 fshift, u64 pbad, u32* __restrict__ err,
                                                   u32* __restrict__ kside, uint8_t* __restrict__ dnext,
-                                                  u32 nshift) {
+                                                  u32 nshift, const u32* __restrict__ tile_seg) {
   using PO = PayOut<P, FUSE>;
   // RB-bit digits (R of them); every thread owns DPT consecutive digits
   constexpr int NT = 256, NW = NT / WAVE, TILE = NT * ITEMS, WITEMS = WAVE * ITEMS;
@@ -447,10 +448,11 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
   const u32 xcd = blockIdx.x % XCDS, per_x = gridDim.x / XCDS;
   const u32 t_hi = (u32)((u64)ntiles * (xcd + 1) / XCDS);
   for (u32 tile = (u32)((u64)ntiles * xcd / XCDS) + blockIdx.x / XCDS; tile < t_hi; tile += per_x) {
+  const u32 tseg = (!LSD && tile_seg) ? tile_seg[tile] : 0u;
   for (u32 i = tid; i < NW * R; i += NT) (&s_whist[0][0])[i] = 0;
-  if (!LSD && tid == 0) s_seg = upper_bound_m1(tile_base, nsegs, tile);
+  if (!LSD && !tile_seg && tid == 0) s_seg = upper_bound_m1(tile_base, nsegs, tile);
   __syncthreads();
-  const u32 seg = LSD ? 0u : s_seg;
+  const u32 seg = LSD ? 0u : (tile_seg ? tseg : s_seg);
   const u32 first_tile = tile_base[seg];
   const u64 row0 = (u64)seg_lo[seg] + (u64)(tile - first_tile) * TILE;
   const u64 rowend = min((u64)seg_hi[seg], row0 + TILE);
@@ -1543,7 +1545,7 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
                                                                      const u32*, const u32*, u32, const u32*,       \
                                                                      const u32*, const u32*, u32, u32, const u32*,   \
                                                                      const u32*, const P*, PayOut<P, F>*, u32, u64, \
-                                                                     u32*, u32*, uint8_t*, u32);
+                                                                     u32*, u32*, uint8_t*, u32, const u32*);
 #define FMCS_INST_ONESWEEP(K, P) FMCS_INST_ONESWEEP_F(K, P, false, true)
 #define FMCS_INST_K(K)                                                                                              \
   template __global__ void k_gather<K>(const K*, const u32*, K*, u64);                                              \
