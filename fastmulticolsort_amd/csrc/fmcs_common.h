@@ -25,37 +25,6 @@ __device__ __forceinline__ void status_store(u32* p, u32 v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Walks back from tile `t` (exclusive) over status words status[p*stride]
-// for p = t-1, t-2, ... >= stop, summing aggregates until an inclusive
-// prefix is found.  Every predecessor took its tile id from an atomic
-// counter before this tile did, so it is resident and publishes its
-// aggregate without waiting on anything: the spin always ends.
-// Status words are fetched LB_BATCH predecessors at a time so their
-// cross-XCD load latencies overlap.
-constexpr u32 LB_BATCH = 8;
-__device__ __forceinline__ u32 lookback(const u32* status, u32 t, u32 stop, u32 stride) {
-  u32 excl = 0;
-  u32 p = t;
-  while (p > stop) {
-    const u32 m = min(LB_BATCH, p - stop);
-    u32 s[LB_BATCH];
-#pragma unroll
-    for (u32 j = 0; j < LB_BATCH; ++j) s[j] = (j < m) ? status_load(status + (u64)(p - 1 - j) * stride) : 0u;
-    bool done = false;
-#pragma unroll
-    for (u32 j = 0; j < LB_BATCH; ++j) {
-      if (done || j >= m) continue;
-      u32 v = s[j];
-      while ((v & ~ST_VAL) == 0) v = status_load(status + (u64)(p - 1 - j) * stride);
-      excl += v & ST_VAL;
-      if (v & ST_INC) done = true;
-    }
-    if (done) break;
-    p -= m;
-  }
-  return excl;
-}
-
 __device__ __forceinline__ u64 lanemask_lt() {
   u32 lane = __lane_id();
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -86,6 +55,39 @@ __device__ __forceinline__ u32 wave_incl_scan(u32 v) {
   t = dpp_move<0x143>(v);  // row_bcast:31
   if (lane >= 32) v += t;
   return v;
+}
+
+// Decoupled look-back by a whole wave (64 lanes, all active, the same t >
+// 0): walks back from tile t (exclusive) over status[0 .. t), each round
+// loading the 64 nearest unvisited predecessors (lane l the (l+1)-th
+// nearest).  It waits until every word below the nearest inclusive prefix
+// holds a flag, then adds the values up to and including that prefix (all
+// 64 when there is none, and goes on).  Every predecessor took its tile id
+// from an atomic counter before this tile did, so it is resident and
+// publishes its aggregate without waiting on anything: the spin always
+// ends.  Returns the exclusive prefix in every lane.
+__device__ __forceinline__ u32 lookback_wave(const u32* status, u32 t) {
+  const u32 lane = __lane_id();
+  u32 excl = 0;
+  for (u32 p = t; p > 0; p = p > (u32)WAVE ? p - (u32)WAVE : 0u) {
+    const bool in = lane < p;  // lanes past tile 0 read as an inclusive 0
+    const u32* a = status + (in ? p - 1 - lane : 0u);
+    u32 v = in ? status_load(a) : ST_INC;
+    u64 inc;
+    for (;;) {
+      const u64 wait = __ballot((v & ~ST_VAL) == 0);
+      inc = __ballot((v & ST_INC) != 0);
+      const u64 below = inc ? (inc & (~inc + 1)) - 1 : ~0ull;
+      if ((wait & below) == 0) break;
+      if ((v & ~ST_VAL) == 0) v = status_load(a);
+    }
+    // lanes up to and including the nearest inclusive one
+    const u64 upto = inc ? ((inc & (~inc + 1)) << 1) - 1 : ~0ull;
+    const u32 c = ((upto >> lane) & 1ull) ? (v & ST_VAL) : 0u;
+    excl += (u32)__builtin_amdgcn_readlane((int)wave_incl_scan(c), WAVE - 1);
+    if (inc) break;
+  }
+  return excl;
 }
 
 // Exclusive scan over a block of NT threads (NT/64 waves), one value each.

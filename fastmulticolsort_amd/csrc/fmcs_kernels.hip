@@ -1365,17 +1365,20 @@ __global__ __launch_bounds__(256) void k_scan_excl(const u32* __restrict__ in, u
   }
   u32 tot;
   u32 ex = block_excl_scan<NT>(sum, s_tmp, &tot);
-  if (tid == 0) {
+  if (tid < WAVE) {
+    const u32 lane = tid;
     u32 e = 0;
     if (tile == 0) {
-      status_store(status, ST_INC | tot);
+      if (lane == 0) status_store(status, ST_INC | tot);
     } else {
-      status_store(status + tile, ST_AGG | tot);
-      e = lookback(status, tile, 0, 1);
-      status_store(status + tile, ST_INC | (e + tot));
+      if (lane == 0) status_store(status + tile, ST_AGG | tot);
+      e = lookback_wave(status, tile);
+      if (lane == 0) status_store(status + tile, ST_INC | (e + tot));
     }
-    s_excl = e;
-    if ((u64)(tile + 1) * TILE >= n && total) *total = e + tot;
+    if (lane == 0) {
+      s_excl = e;
+      if ((u64)(tile + 1) * TILE >= n && total) *total = e + tot;
+    }
   }
   __syncthreads();
   u32 run = s_excl + ex;
