@@ -27,25 +27,17 @@ __global__ void kd_offsets_to_list(const u32* __restrict__ off, const u32* __res
   }
 }
 
-// window_of (fmcs_kernels.hip) over device-sized lists: empty segments are
-// placed at start - 1 within their parent.
-__device__ __forceinline__ u32 window_of_d(const u32* st, const u32* sz, u32 i, u32 W, const u32* plo,
-                                           const u32* pwb, u32 wpt) {
-  const u32 s = st[i];
-  if (!plo) return ((sz[i] == 0 && s > 0) ? s - 1 : s) / W;
-  u32 p = i >> RADIX_BITS;
-  const u32 lo = plo[p];
-  return pwb[p] * wpt + (((sz[i] == 0 && s > lo) ? s - 1 : s) - lo) / W;
-}
-
-// Window packing (see k_windows).  A block takes WIN_CHUNK consecutive
-// segments at a time, collects the windows that start a run in LDS (one
-// LDS atomic per wave) and appends them to the compact list with one global
-// atomic per chunk, so the list counter sees few contended updates.
+// Window packing (see k_windows) over device-sized lists; an empty segment
+// is placed at start - 1 within its parent.  A block takes WIN_CHUNK
+// consecutive segments at a time, collects the windows that start a run in
+// LDS (one LDS atomic per wave) and appends them to the compact list with
+// one global atomic per chunk, so the list counter sees few contended
+// updates.
 constexpr u32 WIN_CHUNK = 2048;
 // Segments of small_max + 1 .. single_max rows (single_max > small_max: a
 // whole tile) are listed apart, each one its own window: (slo, shi, sseg)
-// appended at *scount, one global atomic per wave.
+// appended at *scount, collected in LDS like the windows (one global atomic
+// per chunk).
 __global__ __launch_bounds__(256) void kd_windows(const u32* __restrict__ st, const u32* __restrict__ sz,
                                                   const u32* __restrict__ nseg_d, u32 W, u32 small_max,
                                                   const u32* __restrict__ plo, const u32* __restrict__ pwb, u32 wpt,
@@ -54,29 +46,67 @@ __global__ __launch_bounds__(256) void kd_windows(const u32* __restrict__ st, co
                                                   u32* __restrict__ wcount, u32 single_max, u32* __restrict__ slo,
                                                   u32* __restrict__ shi, u32* __restrict__ sseg,
                                                   u32* __restrict__ scount, u32* __restrict__ scan_status) {
+  constexpr u32 PER = WIN_CHUNK / 256;
   __shared__ u32 s_w[WIN_CHUNK];
-  __shared__ u32 s_n, s_base;
+  // the chunk's segments' windows and sizes at 1 .. WIN_CHUNK, the segment
+  // before the chunk at 0 and the one after it at WIN_CHUNK + 1
+  __shared__ u32 s_wid[WIN_CHUNK + 2];
+  __shared__ u32 s_sz[WIN_CHUNK + 2];
+  __shared__ u32 s_si[WIN_CHUNK];  // the chunk's single segments
+  __shared__ u32 s_n, s_base, s_ns, s_sbase;
   const u32 nseg = *nseg_d;
-  const u32 lane = __lane_id();
+  const u32 tid = threadIdx.x, lane = __lane_id();
   // the look-back status words of the level's scan over the segment sizes
   const u32 nst = (nseg + SCAN_TILE - 1) / SCAN_TILE;
-  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < nst; i += gridDim.x * blockDim.x) scan_status[i] = 0;
+  for (u32 i = blockIdx.x * blockDim.x + tid; i < nst; i += gridDim.x * blockDim.x) scan_status[i] = 0;
+  // window of a segment from its start s, size z and (below the first level)
+  // its parent's first row lo and first window pb
+  auto window = [&](u32 s, u32 z, u32 lo, u32 pb) -> u32 {
+    if (!plo) return ((z == 0 && s > 0) ? s - 1 : s) / W;
+    return pb * wpt + (((z == 0 && s > lo) ? s - 1 : s) - lo) / W;
+  };
   for (u64 c0 = (u64)blockIdx.x * WIN_CHUNK; c0 < nseg; c0 += (u64)gridDim.x * WIN_CHUNK) {
-    if (threadIdx.x == 0) s_n = 0;
+    if (tid == 0) s_n = s_ns = 0;
+    // every load of the chunk first, at clamped indices and without
+    // branches, so they are in flight together
+    u32 sv[PER], zv[PER], lv[PER], bv[PER];
+#pragma unroll
+    for (u32 j = 0; j < PER; ++j) {
+      const u32 ic = (u32)min(c0 + j * 256 + tid, (u64)nseg - 1);
+      sv[j] = st[ic];
+      zv[j] = sz[ic];
+      lv[j] = plo ? plo[ic >> RADIX_BITS] : 0u;
+      bv[j] = plo ? pwb[ic >> RADIX_BITS] : 0u;
+    }
+    // threads 0 and 1: the segments either side of the chunk
+    const u32 ie = tid == 0 ? (u32)(c0 > 0 ? c0 - 1 : 0) : (u32)min(c0 + WIN_CHUNK, (u64)nseg - 1);
+    const u32 es = st[ie], ez = sz[ie];
+    const u32 el = plo ? plo[ie >> RADIX_BITS] : 0u, eb = plo ? pwb[ie >> RADIX_BITS] : 0u;
+#pragma unroll
+    for (u32 j = 0; j < PER; ++j) {
+      s_wid[1 + j * 256 + tid] = window(sv[j], zv[j], lv[j], bv[j]);
+      s_sz[1 + j * 256 + tid] = zv[j];
+    }
+    if (tid < 2) {
+      s_wid[tid == 0 ? 0 : WIN_CHUNK + 1] = window(es, ez, el, eb);
+      s_sz[tid == 0 ? 0 : WIN_CHUNK + 1] = ez;
+    }
     __syncthreads();
-    for (u32 j = 0; j < WIN_CHUNK; j += 256) {
-      const u64 i64 = c0 + j + threadIdx.x;
+#pragma unroll
+    for (u32 j = 0; j < PER; ++j) {
+      const u64 i64 = c0 + j * 256 + tid;
+      const u32 k = 1 + j * 256 + tid;
       bool first = false, single = false;
-      u32 w = 0, i = 0, size = 0;
+      u32 w = 0, size = 0;
+      const u32 i = (u32)i64;
       if (i64 < nseg) {
-        i = (u32)i64;
-        size = sz[i];
+        size = s_sz[k];
         if (size <= small_max) {
-          w = window_of_d(st, sz, i, W, plo, pwb, wpt);
-          first = (i == 0) || (window_of_d(st, sz, i - 1, W, plo, pwb, wpt) != w);
-          bool last = (i + 1 == nseg) || (window_of_d(st, sz, i + 1, W, plo, pwb, wpt) != w) || (sz[i + 1] > small_max);
-          if (first) { tlo[w] = st[i]; ts0[w] = i; }
-          if (last) { thi[w] = st[i] + size; ts1[w] = i; }
+          w = s_wid[k];
+          first = (i == 0) || (s_wid[k - 1] != w);
+          const bool last = (i + 1 == nseg) || (s_wid[k + 1] != w) || (s_sz[k + 1] > small_max);
+          if (first) { tlo[w] = sv[j]; ts0[w] = i; }
+          if (last) { thi[w] = sv[j] + size; ts1[w] = i; }
         } else {
           single = size <= single_max;
         }
@@ -85,15 +115,9 @@ __global__ __launch_bounds__(256) void kd_windows(const u32* __restrict__ st, co
       if (ms) {
         const u32 leader = (u32)(__ffsll((long long)ms) - 1);
         u32 b = 0;
-        if (lane == leader) b = atomicAdd(scount, (u32)__popcll(ms));
+        if (lane == leader) b = atomicAdd(&s_ns, (u32)__popcll(ms));
         b = __shfl(b, (int)leader);
-        if (single) {
-          const u32 k = b + (u32)__popcll(ms & lanemask_lt());
-          const u32 s0 = st[i];
-          slo[k] = s0;
-          shi[k] = s0 + size;
-          sseg[k] = i;
-        }
+        if (single) s_si[b + (u32)__popcll(ms & lanemask_lt())] = i;
       }
       const u64 m = __ballot(first);
       if (m) {
@@ -105,9 +129,19 @@ __global__ __launch_bounds__(256) void kd_windows(const u32* __restrict__ st, co
       }
     }
     __syncthreads();
-    if (threadIdx.x == 0) s_base = s_n ? atomicAdd(wcount, s_n) : 0u;
+    if (tid == 0) {
+      s_base = s_n ? atomicAdd(wcount, s_n) : 0u;
+      s_sbase = s_ns ? atomicAdd(scount, s_ns) : 0u;
+    }
     __syncthreads();
-    for (u32 k = threadIdx.x; k < s_n; k += 256) wlist[s_base + k] = s_w[k];
+    for (u32 k = tid; k < s_n; k += 256) wlist[s_base + k] = s_w[k];
+    for (u32 k = tid; k < s_ns; k += 256) {
+      const u32 i = s_si[k];
+      const u32 s0 = st[i];
+      slo[s_sbase + k] = s0;
+      shi[s_sbase + k] = s0 + s_sz[1 + (i - (u32)c0)];
+      sseg[s_sbase + k] = i;
+    }
     __syncthreads();
   }
 }
