@@ -787,17 +787,21 @@ enum { LV_NSEG, LV_WCOUNT, LV_ROWS_SMALL, LV_ROWS_LARGE, LV_NLARGE, LV_NTILES, L
 
 // Exclusive scan of a device-sized list.  st holds ceil(n_max / SCAN_TILE)
 // look-back status words that the kernel before it cleared (the first
-// ceil(*n_d / SCAN_TILE) of them are enough).  flags: in holds segment
-// sizes, the scan is over (size > small_max), and rows (if set) [0..1] get
+// ceil(*n_d / SCAN_TILE) of them are enough).  flags: in holds the sizes of
+// segments starting at seg_start, the scan is over (size > small_max), the
+// large segments' (lo, hi, tiles) go to their places in llo, lhi, ltiles with
+// their digit counts in hist (if set) zeroed, and rows (if set) [0..1] get
 // the small and large rows.
 int dd_scan(fmcs_ctx* ctx, const u32* in, u32* out, const u32* n_d, u64 n_max, u32* st, u32* counter, u32* total,
-            bool flags = false, u32 small_max = 0, u32* rows = nullptr) {
+            bool flags = false, u32 small_max = 0, u32* rows = nullptr, const u32* seg_start = nullptr,
+            u32* llo = nullptr, u32* lhi = nullptr, u32* ltiles = nullptr, u32* hist = nullptr) {
   const u64 tiles_max = std::max<u64>(1, (n_max + SCAN_TILE - 1) / SCAN_TILE);
   // blocks take tiles from the counter, so a few hundred cover any list:
   // a larger grid only adds launch time (most lists are far below their bound)
   auto fn = flags ? kd_scan_excl<SCAN_ITEMS, true> : kd_scan_excl<SCAN_ITEMS, false>;
   return klaunch(ctx, "k_scan", fn, cap_grid(tiles_max, std::min<u64>(512, resident(ctx, (const void*)fn))), 0.0,
-                 nullptr, 0.0, in, out, n_d, st, counter, total, small_max, rows);
+                 nullptr, 0.0, in, out, n_d, st, counter, total, small_max, rows, seg_start, (u32)ONESWEEP_TILE, llo,
+                 lhi, ltiles, hist);
 }
 
 // LSD radix sort over every pass (no host-side trivial-pass skipping).
@@ -1057,7 +1061,7 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
   TRY(getbuf_t(ctx, WS_SZ, std::max<u64>(m_list[0], 1), &lsz[0]));
   TRY(getbuf_t(ctx, WS_BST, std::max<u64>(m_list[1], 1), &lst[1]));
   TRY(getbuf_t(ctx, WS_BSZ, std::max<u64>(m_list[1], 1), &lsz[1]));
-  u32 *tlo, *thi, *ts0, *ts1, *wlist, *fb, *fblist, *pos, *llo, *lhi, *ltiles, *tbase, *hist, *dbase, *tcount;
+  u32 *tlo, *thi, *ts0, *ts1, *wlist, *fb, *fblist, *llo, *lhi, *ltiles, *tbase, *hist, *dbase, *tcount;
   TRY(getbuf_t(ctx, WS_TLO, m_nw, &tlo));
   TRY(getbuf_t(ctx, WS_THI, m_nw, &thi));
   TRY(getbuf_t(ctx, WS_TS0, m_nw, &ts0));
@@ -1065,7 +1069,6 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
   TRY(getbuf_t(ctx, WS_WLIST, m_nw, &wlist));
   TRY(getbuf_t(ctx, WS_FALLBACK, m_nw, &fb));
   TRY(getbuf_t(ctx, WS_FBLIST, m_nw, &fblist));
-  TRY(getbuf_t(ctx, WS_POS, m_flag, &pos));
   TRY(getbuf_t(ctx, WS_LLO, m_large, &llo));
   TRY(getbuf_t(ctx, WS_LHI, m_large, &lhi));
   TRY(getbuf_t(ctx, WS_LTILES, m_large, &ltiles));
@@ -1079,9 +1082,11 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
   u32 *sgl, *fblist2;
   TRY(getbuf_t(ctx, WS_SGL, 3 * m_single, &sgl));
   TRY(getbuf_t(ctx, WS_FBLIST2, m_single, &fblist2));
-  // both scans of a level: kd_windows and kd_large_compact clear the words
+  // a region for each of a level's two scans, both cleared by kd_windows
+  const u64 st_words = (m_flag + SCAN_TILE - 1) / SCAN_TILE;
   u32* scan_st;
-  TRY(getbuf_t(ctx, WS_SCANST, (m_flag + SCAN_TILE - 1) / SCAN_TILE, &scan_st));
+  TRY(getbuf_t(ctx, WS_SCANST, 2 * st_words, &scan_st));
+  u32* scan_st2 = scan_st + st_words;
   u32 *slo = sgl, *shi = sgl + m_single, *sseg = sgl + 2 * m_single;
   u32 *rkl, *rkl2;
   TRY(getbuf_t(ctx, WS_RKL, m_nw, &rkl));
@@ -1108,11 +1113,13 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
     u32* csz = lsz[cur];
     TRY(klaunch(ctx, "k_windows", kd_windows, cap_grid((d.nseg_max + 2047) / 2048, 2048), 0.0, nullptr, 0.0,
                 (const u32*)cst, (const u32*)csz, nseg_p, d.Wl, d.sm, plo, ptb, d.wpt, tlo, thi, ts0, ts1, wlist,
-                S + LV_WCOUNT, d.single_max, slo, shi, sseg, S + LV_NSINGLE, scan_st));
-    // each segment's place among the large ones (above single_max rows);
-    // the row counts only feed the per-kernel byte figures of a profiled run
-    TRY(dd_scan(ctx, csz, pos, nseg_p, d.nseg_max, scan_st, S + LV_SCAN_A, S + LV_NLARGE, true, d.single_max,
-                (ctx->prof && !ctx->capturing) ? S + LV_ROWS_SMALL : nullptr));
+                S + LV_WCOUNT, d.single_max, slo, shi, sseg, S + LV_NSINGLE, scan_st, scan_st2));
+    // the large segments (above single_max rows) compacted, with their digit
+    // counts zeroed on the levels that scatter; the row counts only feed the
+    // per-kernel byte figures of a profiled run
+    TRY(dd_scan(ctx, csz, nullptr, nseg_p, d.nseg_max, scan_st, S + LV_SCAN_A, S + LV_NLARGE, true, d.single_max,
+                (ctx->prof && !ctx->capturing) ? S + LV_ROWS_SMALL : nullptr, cst, llo, lhi, ltiles,
+                d.last ? nullptr : hist));
     TileSortArgs<K> a;
     std::memset(&a, 0, sizeof(a));
     a.kin[0] = keys; a.vin[0] = vals; a.kin[1] = kb_; a.vin[1] = vb_;
@@ -1174,12 +1181,7 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
     K* dkey = (l & 1) ? keys : kb_;
     u32* dval = (l & 1) ? vals : vb_;
     if (d.last && !(l & 1)) break;  // every key bit consumed; large segments already in place
-    // also clears the scan status below and (levels that scatter) the digit counts
-    TRY(klaunch(ctx, "k_large_compact", kd_large_compact,
-                cap_grid(std::max<u64>(d.nseg_max, (d.last ? 0 : d.nlarge_max * RADIX)) / 256 + 1, 2048), 0.0,
-                nullptr, 0.0, (const u32*)cst, (const u32*)csz, nseg_p, d.single_max, (const u32*)pos,
-                (u32)ONESWEEP_TILE, llo, lhi, ltiles, (const u32*)(S + LV_NLARGE), scan_st, d.last ? nullptr : hist));
-    TRY(dd_scan(ctx, ltiles, tbase, S + LV_NLARGE, d.nlarge_max, scan_st, S + LV_SCAN_B, S + LV_NTILES));
+    TRY(dd_scan(ctx, ltiles, tbase, S + LV_NLARGE, d.nlarge_max, scan_st2, S + LV_SCAN_B, S + LV_NTILES));
     if (d.last) {
       // odd level, every key bit consumed: move the large segments back
       TRY(klaunch(ctx, "k_copy_segs", kd_copy_segs<K>, cap_grid(d.ntiles_max, 2048), 0.0, S + LV_ROWS_LARGE,

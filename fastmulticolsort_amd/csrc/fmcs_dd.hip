@@ -45,7 +45,8 @@ __global__ __launch_bounds__(256) void kd_windows(const u32* __restrict__ st, co
                                                   u32* __restrict__ ts1, u32* __restrict__ wlist,
                                                   u32* __restrict__ wcount, u32 single_max, u32* __restrict__ slo,
                                                   u32* __restrict__ shi, u32* __restrict__ sseg,
-                                                  u32* __restrict__ scount, u32* __restrict__ scan_status) {
+                                                  u32* __restrict__ scount, u32* __restrict__ scan_status,
+                                                  u32* __restrict__ scan_status2) {
   constexpr u32 PER = WIN_CHUNK / 256;
   __shared__ u32 s_w[WIN_CHUNK];
   // the chunk's segments' windows and sizes at 1 .. WIN_CHUNK, the segment
@@ -56,9 +57,13 @@ __global__ __launch_bounds__(256) void kd_windows(const u32* __restrict__ st, co
   __shared__ u32 s_n, s_base, s_ns, s_sbase;
   const u32 nseg = *nseg_d;
   const u32 tid = threadIdx.x, lane = __lane_id();
-  // the look-back status words of the level's scan over the segment sizes
+  // the look-back status words of the level's two scans: over the segment
+  // sizes, and over the large segments' tile counts (no more of those)
   const u32 nst = (nseg + SCAN_TILE - 1) / SCAN_TILE;
-  for (u32 i = blockIdx.x * blockDim.x + tid; i < nst; i += gridDim.x * blockDim.x) scan_status[i] = 0;
+  for (u32 i = blockIdx.x * blockDim.x + tid; i < nst; i += gridDim.x * blockDim.x) {
+    scan_status[i] = 0;
+    scan_status2[i] = 0;
+  }
   // window of a segment from its start s, size z and (below the first level)
   // its parent's first row lo and first window pb
   auto window = [&](u32 s, u32 z, u32 lo, u32 pb) -> u32 {
@@ -148,18 +153,24 @@ __global__ __launch_bounds__(256) void kd_windows(const u32* __restrict__ st, co
 
 // Exclusive scan with decoupled look-back, persistent: blocks take tiles from
 // the counter until ceil(n / TILE) are done.  *total = the sum (0 when n = 0).
-// FLAGS: in holds segment sizes and the scanned value is (size > small_max),
-// so out[i] is segment i's place among the large segments and *total their
-// count; rows (if set) [0] and [1] get the small and the large segments'
-// rows.
+// FLAGS: in holds the sizes of segments starting at seg_start and the
+// scanned value is (size > small_max): *total is the large segments' count,
+// and large segment i, the p-th, gets llo[p] = its first row, lhi[p] = its
+// end, ltiles[p] = its tiles of tile_rows rows, and (hist set) zeroed digit
+// counts hist[p * 256 .. + 256); out is not written.  rows (if set) [0] and
+// [1] get the small and the large segments' rows.
 template <int ITEMS, bool FLAGS>
 __global__ __launch_bounds__(256) void kd_scan_excl(const u32* __restrict__ in, u32* __restrict__ out,
                                                     const u32* __restrict__ n_d, u32* __restrict__ status,
                                                     u32* __restrict__ counter, u32* __restrict__ total,
-                                                    u32 small_max, u32* __restrict__ rows) {
+                                                    u32 small_max, u32* __restrict__ rows,
+                                                    const u32* __restrict__ seg_start, u32 tile_rows,
+                                                    u32* __restrict__ llo, u32* __restrict__ lhi,
+                                                    u32* __restrict__ ltiles, u32* __restrict__ hist) {
   constexpr int NT = 256, TILE = NT * ITEMS;
   __shared__ u32 s_tile, s_excl;
   __shared__ u32 s_tmp[NT / WAVE + 1];
+  __shared__ u32 s_lpos[FLAGS ? TILE : 1];  // FLAGS: the tile's large segments' places
   const u32 n = *n_d;
   const u32 ntiles = (n + TILE - 1) / TILE;
   const u32 tid = threadIdx.x;
@@ -177,11 +188,18 @@ __global__ __launch_bounds__(256) void kd_scan_excl(const u32* __restrict__ in, 
     if (tile >= ntiles) break;
     const u64 base = (u64)tile * TILE + (u64)tid * ITEMS;
     u32 v[ITEMS];
+    u32 zs[FLAGS ? ITEMS : 1], ss[FLAGS ? ITEMS : 1];  // FLAGS: sizes and starts
     u32 sum = 0;
 #pragma unroll
+    for (int j = 0; j < ITEMS; ++j) v[j] = (base + j < n) ? in[base + j] : 0u;
+    if constexpr (FLAGS) {
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j) ss[j] = seg_start[min(base + j, (u64)n - 1)];
+    }
+#pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
-      v[j] = (base + j < n) ? in[base + j] : 0u;
       if constexpr (FLAGS) {
+        zs[j] = v[j];
         const bool large = v[j] > small_max;
         rs += large ? 0u : v[j];
         rl += large ? v[j] : 0u;
@@ -208,10 +226,29 @@ __global__ __launch_bounds__(256) void kd_scan_excl(const u32* __restrict__ in, 
     }
     __syncthreads();
     u32 run = s_excl + ex;
+    if constexpr (FLAGS) {
+      // rows past n were read as size 0: never large
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-      if (base + j < n) out[base + j] = run;
-      run += v[j];
+      for (int j = 0; j < ITEMS; ++j) {
+        if (v[j]) {
+          llo[run] = ss[j];
+          lhi[run] = ss[j] + zs[j];
+          ltiles[run] = (zs[j] + tile_rows - 1) / tile_rows;
+          s_lpos[run - s_excl] = run;
+        }
+        run += v[j];
+      }
+      if (hist) {
+        __syncthreads();
+        // one 256-count row per large segment, a row per block step
+        for (u32 k = 0; k < tot; ++k) hist[(u64)s_lpos[k] * RADIX + tid] = 0;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j) {
+        if (base + j < n) out[base + j] = run;
+        run += v[j];
+      }
     }
     __syncthreads();
   }
@@ -231,31 +268,6 @@ __global__ __launch_bounds__(256) void kd_scan_excl(const u32* __restrict__ in, 
         if (s_rows[1]) atomicAdd(rows + 1, s_rows[1]);
       }
     }
-  }
-}
-
-// The large segments' (lo, hi, tiles) at their scanned positions.  Also
-// clears what the next kernels of the level expect zeroed, sized by the
-// large-segment count *nl_d: the look-back status words of the scan over
-// ltiles and (hist set) the nl x 256 per-segment digit counts.
-__global__ void kd_large_compact(const u32* __restrict__ st, const u32* __restrict__ sz, const u32* __restrict__ n_d,
-                                 u32 small_max, const u32* __restrict__ pos, u32 tile_rows, u32* __restrict__ llo,
-                                 u32* __restrict__ lhi, u32* __restrict__ ltiles, const u32* __restrict__ nl_d,
-                                 u32* __restrict__ scan_status, u32* __restrict__ hist) {
-  const u32 nseg = *n_d;
-  const u32 nl = *nl_d;
-  const u32 gtid = blockIdx.x * blockDim.x + threadIdx.x, gstride = gridDim.x * blockDim.x;
-  const u32 nst = (nl + SCAN_TILE - 1) / SCAN_TILE;
-  for (u32 i = gtid; i < nst; i += gstride) scan_status[i] = 0;
-  if (hist)
-    for (u64 i = gtid; i < (u64)nl * RADIX; i += gstride) hist[i] = 0;
-  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < nseg; i += gridDim.x * blockDim.x) {
-    u32 s = sz[i];
-    if (s <= small_max) continue;
-    u32 p = pos[i];
-    llo[p] = st[i];
-    lhi[p] = st[i] + s;
-    ltiles[p] = (s + tile_rows - 1) / tile_rows;
   }
 }
 
@@ -546,8 +558,11 @@ template __global__ void kd_seg_hist<uint8_t, ONESWEEP_ITEMS, 8>(const uint8_t*,
 FMCS_DD_INST(u16)
 FMCS_DD_INST(u32)
 FMCS_DD_INST(u64)
-template __global__ void kd_scan_excl<8, false>(const u32*, u32*, const u32*, u32*, u32*, u32*, u32, u32*);
-template __global__ void kd_scan_excl<8, true>(const u32*, u32*, const u32*, u32*, u32*, u32*, u32, u32*);
+#define FMCS_SCAN_INST(F)                                                                                         \
+  template __global__ void kd_scan_excl<8, F>(const u32*, u32*, const u32*, u32*, u32*, u32*, u32, u32*, const u32*, \
+                                              u32, u32*, u32*, u32*, u32*);
+FMCS_SCAN_INST(false)
+FMCS_SCAN_INST(true)
 template __global__ void kd_tscan_local<8>(u32*, const u32*, const u32*, const u32*, u32*, u32*, const u32*);
 template __global__ void kd_tscan_carry<8>(u32*, const u32*, const u32*);
 template __global__ void kd_digit_base<8>(const u32*, const u32*, u32*, u32*, u32*, const u32*, u32*);

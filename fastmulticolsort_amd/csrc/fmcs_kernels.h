@@ -132,11 +132,10 @@ __global__ void k_merge_pass(const K*, const u32*, K*, u32*, u64, u64);
 __global__ void kd_set4(u32*, u32, u32, u32, u32);
 __global__ void kd_offsets_to_list(const u32*, const u32*, u32*, u32*);
 __global__ void kd_windows(const u32*, const u32*, const u32*, u32, u32, const u32*, const u32*, u32, u32*, u32*,
-                           u32*, u32*, u32*, u32*, u32, u32*, u32*, u32*, u32*, u32*);
+                           u32*, u32*, u32*, u32*, u32, u32*, u32*, u32*, u32*, u32*, u32*);
 template <int ITEMS, bool FLAGS>
-__global__ void kd_scan_excl(const u32*, u32*, const u32*, u32*, u32*, u32*, u32, u32*);
-__global__ void kd_large_compact(const u32*, const u32*, const u32*, u32, const u32*, u32, u32*, u32*, u32*,
-                                 const u32*, u32*, u32*);
+__global__ void kd_scan_excl(const u32*, u32*, const u32*, u32*, u32*, u32*, u32, u32*, const u32*, u32, u32*, u32*,
+                             u32*, u32*);
 template <class K, int ITEMS, int RB>
 __global__ void kd_seg_hist(const K*, const u32*, const u32*, const u32*, const u32*, const u32*, u32, u32, u32*,
                             u32*, u64, u32*, u32*);
