@@ -26,7 +26,7 @@ EXPORTS = [
     "fmcs_ctx_create", "fmcs_ctx_destroy", "fmcs_ctx_set_stream", "fmcs_ctx_sync",
     "fmcs_ctx_set_tuning", "fmcs_ctx_get_tuning", "fmcs_ctx_trim", "fmcs_last_error",
     "fmcs_bank_bytes", "fmcs_construct_entries", "fmcs_stitch", "fmcs_stitch_dir", "fmcs_gather",
-    "fmcs_extract_group_info", "fmcs_sort_pairs", "fmcs_sort_pairs_merge", "fmcs_fill_column", "fmcs_segmented_sort",
+    "fmcs_extract_group_info", "fmcs_sort_pairs", "fmcs_sort_pairs_noalloc", "fmcs_sort_pairs_merge", "fmcs_fill_column", "fmcs_segmented_sort",
     "fmcs_sort_by_group",
     "fmcs_multi_round_sort", "fmcs_last_breakdown", "fmcs_hash_round", "fmcs_multi_round_hash",
     "fmcs_ctx_set_profiling", "fmcs_kernel_stats", "fmcs_kernel_stats_reset", "fmcs_distinct_count",
@@ -92,6 +92,7 @@ def load():
         "fmcs_gather": (i32, [vp, u32, u64, vp, vp, vp]),
         "fmcs_extract_group_info": (i32, [vp, u32, u64, vp, vp, vp, vp, P(u32)]),
         "fmcs_sort_pairs": (i32, [vp, u32, u32, u64, vp, vp]),
+        "fmcs_sort_pairs_noalloc": (i32, [vp, u32, u32, u64, vp, vp]),
         "fmcs_segmented_sort": (i32, [vp, u32, u32, u64, vp, vp, vp, u32, i32, P(GroupInfo)]),
         "fmcs_sort_by_group": (i32, [vp, u32, u32, u64, vp, vp, vp, vp, i32, i32, vp, P(GroupInfo), P(u32)]),
         "fmcs_multi_round_sort": (i32, [vp, u32, P(vp), P(u32), P(u32), u64, vp, vp, i32, P(u32)]),

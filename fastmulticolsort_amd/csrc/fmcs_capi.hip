@@ -75,6 +75,17 @@ struct fmcs_ctx {
   u32 ncu = 0;
   std::map<const void*, u32> resident;
   bool dd_pending = false;  // a device-driven call's key-error word is not yet checked
+  // Arena mode (the allocation-free entry point): while arena is set, every
+  // workspace slot is carved out of [arena, arena + arena_cap) for the
+  // current call instead of being allocated; a slot asked for again with no
+  // more bytes gets the same memory.
+  char* arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
+  size_t arena_off[WS_COUNT] = {}, arena_len[WS_COUNT] = {};
+  // this device's copy of the static scratch array (fmcs_noalloc.hip), or
+  // null with the reason in scratch_err
+  void* scratch_ptr = nullptr;
+  std::string scratch_err;
 };
 
 #define HIPCHK(x)                                                                          \
@@ -98,6 +109,18 @@ int fail(fmcs_ctx* ctx, int code, const std::string& msg) {
 
 int getbuf(fmcs_ctx* ctx, Slot s, size_t bytes, void** out) {
   bytes = std::max<size_t>(bytes, 256);
+  if (ctx->arena) {
+    if (ctx->arena_len[s] < bytes) {
+      const size_t off = (ctx->arena_used + 255) & ~(size_t)255;
+      if (off > ctx->arena_cap || bytes > ctx->arena_cap - off)
+        return fail(ctx, FMCS_E_ARG, "scratch of " + std::to_string(ctx->arena_cap) + " bytes exhausted");
+      ctx->arena_off[s] = off;
+      ctx->arena_len[s] = bytes;
+      ctx->arena_used = off + bytes;
+    }
+    *out = ctx->arena + ctx->arena_off[s];
+    return FMCS_OK;
+  }
   Buf& b = ctx->ws[s];
   if (b.bytes < bytes) {
     if (ctx->capturing) return fail(ctx, FMCS_E_HIP, "workspace growth during graph capture");
@@ -240,7 +263,7 @@ struct Prof {
   fmcs_ctx* ctx;
   int idx = -1;
   Prof(fmcs_ctx* c, const char* name, double bytes, const u32* dcount = nullptr, double per_unit = 0) : ctx(c) {
-    if (!ctx->prof || ctx->capturing) return;
+    if (!ctx->prof || ctx->capturing || ctx->arena) return;
     if (ctx->recs.size() > 8192) prof_flush(ctx);
     fmcs_ctx::Rec r{name, pool_get(ctx), pool_get(ctx), bytes, dcount, per_unit};
     if (r.e0) (void)hipEventRecord(r.e0, ctx->stream);
@@ -371,8 +394,16 @@ int radix_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, 
   u32 *t0v, *t1v;
   TRY(getbuf_t(ctx, WS_T0K, n, &t0k));
   TRY(getbuf_t(ctx, WS_T0V, n, &t0v));
-  TRY(getbuf_t(ctx, WS_T1K, n, &t1k));
-  TRY(getbuf_t(ctx, WS_T1V, n, &t1v));
+  if (ctx->arena && (const void*)kin == (const void*)kout && vin == vout) {
+    // in place from scratch: the caller's arrays are the second buffer
+    // (odd passes write them, even passes the scratch pair; after an odd
+    // number of passes the result is copied home below)
+    t1k = kout;
+    t1v = vout;
+  } else {
+    TRY(getbuf_t(ctx, WS_T1K, n, &t1k));
+    TRY(getbuf_t(ctx, WS_T1V, n, &t1v));
+  }
   // constants: seg_lo=0, seg_hi=n, tile_base={0, ntiles}, nsegs = 1
   u32 consts[5] = {0u, (u32)n, 0u, (u32)ntiles, 1u};
   TRY(upload(ctx, ctx->small_d + 16, consts, 5));
@@ -1490,6 +1521,31 @@ int uniform_offsets(fmcs_ctx* ctx, uint32_t* off, uint32_t nseg, uint64_t n, uin
   return launch_check(ctx, "k_uniform_offsets");
 }
 
+int scratch_of(fmcs_ctx* ctx, void** ptr) {
+  if (!ctx->scratch_ptr) return fail(ctx, FMCS_E_HIP, "no scratch array on this device: " + ctx->scratch_err);
+  *ptr = ctx->scratch_ptr;
+  return FMCS_OK;
+}
+
+// fmcs_sort_pairs in place with every workspace slot carved out of
+// [scratch, scratch + cap): no allocation, the same kernels and passes.
+int sort_pairs_in_scratch(fmcs_ctx* ctx, uint32_t bank, uint32_t keybits, uint64_t n, void* keys, uint32_t* oids,
+                          void* scratch, size_t cap) {
+  ctx->arena = (char*)scratch;
+  ctx->arena_cap = cap;
+  ctx->arena_used = 0;
+  for (int s = 0; s < WS_COUNT; ++s) ctx->arena_len[s] = 0;
+  int rc;
+  switch (bank) {
+    case 2: rc = radix_sort<u16>(ctx, (u16*)keys, oids, (u16*)keys, oids, n, keybits); break;
+    case 4: rc = radix_sort<u32>(ctx, (u32*)keys, oids, (u32*)keys, oids, n, keybits); break;
+    default: rc = radix_sort<u64>(ctx, (u64*)keys, oids, (u64*)keys, oids, n, keybits); break;
+  }
+  ctx->arena = nullptr;
+  ctx->arena_cap = 0;
+  return rc;
+}
+
 }  // namespace internal
 }  // namespace fmcs
 
@@ -1518,6 +1574,7 @@ int fmcs_ctx_create(int device, fmcs_ctx** out) {
   if (e != hipSuccess) return create_fail(FMCS_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
   fmcs_ctx* c = new fmcs_ctx();
   c->device = device;
+  if (fmcs::internal::noalloc_setup(device, &c->scratch_ptr, &c->scratch_err) != FMCS_OK) c->scratch_ptr = nullptr;
   e = hipMalloc(&c->small_d, 64 * sizeof(u32));
   if (e != hipSuccess) {
     delete c;

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "../../include/fmcs.h"
 
 namespace fmcs {
@@ -19,6 +21,17 @@ int scratch(fmcs_ctx* ctx, int slot, size_t bytes, void** out);
 int fill_random(fmcs_ctx* ctx, void* p, uint32_t bank, uint64_t n, uint64_t seed, uint32_t bits);
 // off[s] = min(s * size, n) for s in [0, nseg].
 int uniform_offsets(fmcs_ctx* ctx, uint32_t* off, uint32_t nseg, uint64_t n, uint32_t size);
+
+// Address of the current device's copy of the static scratch array
+// (fmcs_noalloc.hip), looked up once per device; the context's creation
+// calls it with its device current.  On failure *err holds the reason.
+int noalloc_setup(int device, void** ptr, std::string* err);
+// The context's scratch address (FMCS_E_HIP if the lookup failed).
+int scratch_of(fmcs_ctx* ctx, void** ptr);
+// In-place LSD sort of (keys, oids) with every buffer carved out of
+// [scratch, scratch + cap): no device allocation.
+int sort_pairs_in_scratch(fmcs_ctx* ctx, uint32_t bank, uint32_t keybits, uint64_t n, void* keys, uint32_t* oids,
+                          void* scratch, size_t cap);
 
 }  // namespace internal
 }  // namespace fmcs

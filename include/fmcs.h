@@ -146,6 +146,22 @@ int fmcs_extract_group_info(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void
  * keep their incoming order (LSD radix sort, stable). */
 int fmcs_sort_pairs(fmcs_ctx* ctx, uint32_t bank, uint32_t keybits, uint64_t n, void* keys_d, uint32_t* oids_d);
 
+/* fmcs_sort_pairs without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, kernels and result as fmcs_sort_pairs, for
+ * n <= FMCS_NOALLOC_MAX_ROWS (larger n: FMCS_E_ARG).  Its buffers live in a
+ * static device array of FMCS_NOALLOC_SCRATCH_BYTES that each GPU holds
+ * until the process ends, shared by every call on that GPU: calls that
+ * overlap on one GPU (two streams, two threads) race and must not be issued.
+ * Like fmcs_sort_pairs it synchronizes the context's stream once (the digit
+ * histogram is read back to skip passes in which every key has the same
+ * digit). */
+#define FMCS_NOALLOC_MAX_ROWS (1u << 20)
+#define FMCS_NOALLOC_SCRATCH_BYTES (16u << 20)
+int fmcs_sort_pairs_noalloc(fmcs_ctx* ctx, uint32_t bank, uint32_t keybits, uint64_t n, void* keys_d,
+                            uint32_t* oids_d);
+
 /* The same sort by comparisons, the structure of avxmergesort_anytype
  * (src/mergesort.h:2878-3096: sorting networks over blocks, then a 2-way
  * merge tree): bitonic networks over 2048-row LDS tiles, then merge-path
