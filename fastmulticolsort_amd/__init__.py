@@ -5,4 +5,4 @@ libfmcs.so), with the reference's composer / plan API on the host.
 """
 from . import _lib  # noqa: F401
 
-__all__ = ["engine", "datagen", "composer"]
+__all__ = ["engine", "datagen", "planner", "learned", "columnar", "query", "autotune"]

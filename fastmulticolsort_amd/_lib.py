@@ -30,7 +30,15 @@ EXPORTS = [
     "fmcs_sort_by_group",
     "fmcs_multi_round_sort", "fmcs_last_breakdown", "fmcs_hash_round", "fmcs_multi_round_hash",
     "fmcs_ctx_set_profiling", "fmcs_kernel_stats", "fmcs_kernel_stats_reset", "fmcs_distinct_count",
+    "fmcs_classify_memory", "fmcs_pointer_kind", "fmcs_ctx_set_host_staging",
 ]
+
+# fmcs_pointer_kind / fmcs_classify_memory
+MEM_DEVICE = 0
+MEM_PINNED = 1
+MEM_MANAGED = 2
+MEM_HOST = 3
+MEM_OTHER = 4
 
 
 class Entry(C.Structure):
@@ -103,6 +111,9 @@ def load():
         "fmcs_ctx_set_profiling": (i32, [vp, i32]),
         "fmcs_kernel_stats": (i32, [vp, C.c_char_p, u64]),
         "fmcs_kernel_stats_reset": (i32, [vp]),
+        "fmcs_classify_memory": (u32, [i32, i32, i32, i32]),
+        "fmcs_pointer_kind": (i32, [vp, vp, P(u32)]),
+        "fmcs_ctx_set_host_staging": (i32, [vp, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -26,7 +26,8 @@ enum Slot {
   WS_ST, WS_SZ, WS_BST, WS_BSZ, WS_FLAG, WS_POS, WS_LLO, WS_LHI, WS_LTILES, WS_TBASE, WS_TLO, WS_THI, WS_TS0,
   WS_TS1, WS_COMP, WS_COMP2, WS_GROUPTMP, WS_MISC, WS_HASH, WS_HASH2, WS_HASH3, WS_HT_TABLE, WS_HT_FIRST,
   WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_DD, WS_SCANST, WS_FBLIST, WS_TCOUNT, WS_CARRY, WS_BCNT, WS_T0P, WS_T1P, WS_PAY, WS_SGL, WS_FBLIST2, WS_RKL, WS_RKL2, WS_DNEXT, WS_TSEG, WS_USER0,
-  WS_COUNT = WS_USER0 + 24
+  WS_STAGE0 = WS_USER0 + 24,  // device copies of staged host buffers (one slot per argument)
+  WS_COUNT = WS_STAGE0 + 72
 };
 
 struct Buf {
@@ -86,12 +87,21 @@ struct fmcs_ctx {
   // null with the reason in scratch_err
   void* scratch_ptr = nullptr;
   std::string scratch_err;
+  // stage pageable host buffers through device memory (else they are
+  // rejected with FMCS_E_ARG)
+  bool host_staging = false;
 };
 
 #define HIPCHK(x)                                                                          \
   do {                                                                                     \
     hipError_t e_ = (x);                                                                   \
     if (e_ != hipSuccess) return fail(ctx, FMCS_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define HIPCHK_C(c, x)                                                                      \
+  do {                                                                                      \
+    hipError_t e_ = (x);                                                                    \
+    if (e_ != hipSuccess) return fail(c, FMCS_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
 #define TRY(x)              \
@@ -529,7 +539,8 @@ constexpr u32 TS_SEGBITS = 11;  // local segment index of a tile of up to TILESO
 // Level 0 data in (keys, vals); results in place.
 // ------------------------------------------------------------------------
 template <class K>
-int seg_sort_batched(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* seg_off, u32 nseg) {
+int seg_sort_batched(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* seg_off, u32 nseg,
+                     u32* err = nullptr) {
   if (n <= 1 || nseg == 0) return FMCS_OK;
   const u32 small_max = std::min<u32>(ctx->tune.small_max, (u32)TILESORT_T / 2);
   const u32 T = small_max <= 256 ? 512u : (u32)TILESORT_T;
@@ -600,6 +611,10 @@ int seg_sort_batched(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, cons
     a.kout = keys; a.vout = vals;
     a.tile_lo = tlo; a.tile_hi = thi; a.tile_s0 = ts0; a.tile_s1 = ts1; a.seg_start = cst;
     a.src_sel = level & 1; a.kb = tkb; a.pad_seg = T; a.direct = 0; a.wlist = wlist;
+    if (level == 0 && err && keybits < 8 * sizeof(K)) {  // every row is a window row or a large-segment row here
+      a.bad = ~((1ull << keybits) - 1);
+      a.err = err;
+    }
     {
       // bucket fast path; tiles it refuses go to the radix tile sort
       u32* fb;
@@ -654,7 +669,7 @@ int seg_sort_batched(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, cons
     const u32* ntiles_d = ctx->small_d + 49;
     TRY(klaunch(ctx, "k_seg_hist", kd_seg_hist<K, ONESWEEP_ITEMS, RADIX_BITS>, cap_grid(ntiles, 2048),
                 (double)large_rows * sizeof(K), nullptr, 0.0, skey, (const u32*)llo, (const u32*)lhi,
-                (const u32*)tbase, nlarge_d, ntiles_d, shift, D, hist, tcount, (u64)0, (u32*)nullptr, (u32*)nullptr));
+                (const u32*)tbase, nlarge_d, ntiles_d, shift, D, hist, tcount, a.bad, a.err, (u32*)nullptr));
     const int nxt = cur ^ 1;
     u32 *nst, *nsz;
     TRY(getbuf_t(ctx, lst[nxt], (u64)nlarge * RADIX + 1, &nst));
@@ -702,7 +717,8 @@ int seg_sort_plain(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const 
 
 // single unbinned segmented sort: LSD radix over (group id, key) composites
 template <class K>
-int seg_sort_unbinned(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* group, u32 ngroups) {
+int seg_sort_unbinned(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* group, u32 ngroups,
+                      u32* err) {
   u32 gbits = 0;
   while (gbits < 32 && (1ull << gbits) < ngroups) ++gbits;
   if (gbits + keybits > 64) return fail(ctx, FMCS_E_UNSUPPORTED, "composite key exceeds 64 bits");
@@ -710,7 +726,7 @@ int seg_sort_unbinned(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, con
   TRY(getbuf_t(ctx, WS_COMP, n, &c));
   TRY(getbuf_t(ctx, WS_COMP2, n, &c2));
   u32 blocks = (u32)std::min<u64>((n + 255) / 256, 8192);
-  hipLaunchKernelGGL((k_composite<K>), dim3(blocks), dim3(256), 0, ctx->stream, keys, group, n, keybits, c);
+  hipLaunchKernelGGL((k_composite<K>), dim3(blocks), dim3(256), 0, ctx->stream, keys, group, n, keybits, c, err);
   TRY(launch_check(ctx, "k_composite"));
   TRY(radix_sort<u64>(ctx, c, vals, c2, vals, n, gbits + keybits));
   u64 km = keybits >= 64 ? ~0ull : ((1ull << keybits) - 1);
@@ -721,19 +737,41 @@ int seg_sort_unbinned(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, con
 template <class K>
 int seg_sort_dd_host(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* seg_off, u32 nseg);
 
+// Keys with bits above keybits are an error in every mode: the eager modes
+// return FMCS_E_ARG (the plain form before it sorts anything); the
+// device-driven ones set the context's error word, reported by the next
+// call that returns a count or by fmcs_ctx_sync.
 template <class K>
 int seg_sort(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* seg_off, u32 nseg, int mode,
              const u32* group) {
+  if (mode == FMCS_MODE_DEVICE || mode == FMCS_MODE_GRAPH)
+    return seg_sort_dd_host<K>(ctx, keys, vals, n, keybits, seg_off, nseg);
+  if (mode != FMCS_MODE_BATCHED && mode != FMCS_MODE_PLAIN && mode != FMCS_MODE_UNBINNED)
+    return fail(ctx, FMCS_E_ARG, "unknown mode");
+  if (mode == FMCS_MODE_UNBINNED && !group) return fail(ctx, FMCS_E_ARG, "unbinned mode needs group ids");
+  if (n <= 1 || nseg == 0) return FMCS_OK;
+  keybits = std::min<u32>(keybits, 8 * sizeof(K));
+  const bool check = keybits < 8 * sizeof(K);
+  u32* err = ctx->small_d + 60;
+  if (check) TRY(fill(ctx, err, 1, 0));
+  u32 e = 0;
   switch (mode) {
-    case FMCS_MODE_BATCHED: return seg_sort_batched<K>(ctx, keys, vals, n, keybits, seg_off, nseg);
-    case FMCS_MODE_PLAIN: return seg_sort_plain<K>(ctx, keys, vals, n, keybits, seg_off, nseg);
-    case FMCS_MODE_UNBINNED:
-      if (!group) return fail(ctx, FMCS_E_ARG, "unbinned mode needs group ids");
-      return seg_sort_unbinned<K>(ctx, keys, vals, n, keybits, group, nseg);
-    case FMCS_MODE_DEVICE:
-    case FMCS_MODE_GRAPH: return seg_sort_dd_host<K>(ctx, keys, vals, n, keybits, seg_off, nseg);
-    default: return fail(ctx, FMCS_E_ARG, "unknown mode");
+    case FMCS_MODE_BATCHED: TRY(seg_sort_batched<K>(ctx, keys, vals, n, keybits, seg_off, nseg, check ? err : nullptr)); break;
+    case FMCS_MODE_PLAIN:
+      if (check) {
+        TRY(klaunch(ctx, "k_key_check", k_key_check<K>, cap_grid((n + 255) / 256, 2048), (double)n * sizeof(K),
+                    nullptr, 0.0, (const K*)keys, n, ~((1ull << keybits) - 1), err));
+        TRY(readback(ctx, err, 1, &e));
+        if (e) return fail(ctx, FMCS_E_ARG, "keys exceed keybits");
+      }
+      return seg_sort_plain<K>(ctx, keys, vals, n, keybits, seg_off, nseg);
+    default: TRY(seg_sort_unbinned<K>(ctx, keys, vals, n, keybits, group, nseg, err)); break;
   }
+  if (check) {
+    TRY(readback(ctx, err, 1, &e));
+    if (e) return fail(ctx, FMCS_E_ARG, "keys exceed keybits");
+  }
+  return FMCS_OK;
 }
 
 int group_info_from_offsets(fmcs_ctx* ctx, u32 bank, const u32* seg_off, u32 nseg, fmcs_group_info* info) {
@@ -815,6 +853,23 @@ constexpr u32 DD_CONST = 76;
 constexpr u64 DD_WORDS = DD_HDR + (u64)DD_ROUNDS_MAX * DD_LVL_MAX * DD_LVL_WORDS;
 enum { LV_NSEG, LV_WCOUNT, LV_ROWS_SMALL, LV_ROWS_LARGE, LV_NLARGE, LV_NTILES, LV_SCAN_A, LV_SCAN_B, LV_OS, LV_FB,
        LV_NSINGLE, LV_FB2, LV_RK, LV_RK2 };
+
+// The device-driven word area.  Its key-error word dd[DD_ERR] is sticky:
+// cleared once when the area is allocated and again only when an error
+// has been read back and reported (dd_finish), so an unsynchronised call's
+// error survives later calls until a sync or a count readback reports it.
+int dd_get(fmcs_ctx* ctx, u32** dd) {
+  const bool fresh = ctx->ws[WS_DD].p == nullptr;
+  TRY(getbuf_t(ctx, WS_DD, DD_WORDS, dd));
+  if (fresh) TRY(fill(ctx, *dd + DD_ERR, 1, 0));
+  return FMCS_OK;
+}
+
+// Clears the first `words` words of the area, all but the error word.
+int dd_clear(fmcs_ctx* ctx, u32* dd, u64 words) {
+  TRY(fill(ctx, dd, DD_ERR, 0));
+  return fill(ctx, dd + DD_ERR + 1, words - DD_ERR - 1, 0);
+}
 
 // Exclusive scan of a device-sized list.  st holds ceil(n_max / SCAN_TILE)
 // look-back status words that the kernel before it cleared (the first
@@ -1030,7 +1085,7 @@ struct DdLevel {
 // sorted keys back (keys ends unspecified).
 template <class K>
 int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* seg_off, const u32* nseg_d,
-                u64 nseg_max, u32* lv, bool keys_out = true) {
+                u64 nseg_max, u32* lv, bool keys_out = true, u32* err = nullptr) {
   if (n <= 1 || nseg_max == 0) return FMCS_OK;
   // tiles of 512, 1024 or 2048 rows (2, 4 or 8 per thread) for segments of
   // up to 256, 512 or 1024 rows
@@ -1159,6 +1214,12 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
     a.src_sel = (u32)(l & 1); a.kb = d.tkb; a.pad_seg = T; a.direct = 0; a.wlist = wlist;
     a.wcount = S + LV_WCOUNT; a.fblist = fblist; a.fbcount = S + LV_FB;
     a.size_lo = 0; a.size_hi = ~0u;
+    if (l == 0 && err && keybits < 8 * sizeof(K)) {
+      // level 0 reads every row (window rows here, large-segment rows in its
+      // histogram), so it checks the declared width of every key
+      a.bad = ~((1ull << keybits) - 1);
+      a.err = err;
+    }
     {
       const u64 g = (persist_grid(ctx, (const void*)fn_bucket, d.nw_max) + XCDS - 1) / XCDS * XCDS;
       // rows read (key, oid) and written (oid, and the key when wanted)
@@ -1223,7 +1284,7 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
     TRY(klaunch(ctx, "k_seg_hist", kd_seg_hist<K, ONESWEEP_ITEMS, RADIX_BITS>, cap_grid(d.ntiles_max, 2048), 0.0,
                 S + LV_ROWS_LARGE, (double)sizeof(K), (const K*)skey, (const u32*)llo, (const u32*)lhi,
                 (const u32*)tbase, (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES), d.shift, d.D, hist,
-                tcount, (u64)0, (u32*)nullptr, tseg));
+                tcount, a.bad, a.err, tseg));
     const int nxt = cur ^ 1;
     TRY(klaunch(ctx, "k_digit_base", kd_digit_base<RADIX_BITS>, cap_grid(d.nlarge_max, 2048), 0.0, nullptr, 0.0,
                 (const u32*)hist, (const u32*)llo, dbase, lst[nxt], lsz[nxt], (const u32*)(S + LV_NLARGE),
@@ -1244,10 +1305,10 @@ template <class K>
 int seg_sort_dd_host(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* seg_off, u32 nseg) {
   if (n <= 1 || nseg == 0) return FMCS_OK;
   u32* dd;
-  TRY(getbuf_t(ctx, WS_DD, DD_WORDS, &dd));
-  TRY(fill(ctx, dd, DD_HDR + (u64)DD_LVL_MAX * DD_LVL_WORDS, 0));
+  TRY(dd_get(ctx, &dd));
+  TRY(dd_clear(ctx, dd, DD_HDR + (u64)DD_LVL_MAX * DD_LVL_WORDS));
   TRY(klaunch(ctx, "k_set", kd_set4, 1, 0.0, nullptr, 0.0, dd + 88, nseg, 0u, 0u, 0u));
-  TRY(seg_sort_dd<K>(ctx, keys, vals, n, keybits, seg_off, dd + 88, nseg, dd + DD_HDR));
+  TRY(seg_sort_dd<K>(ctx, keys, vals, n, keybits, seg_off, dd + 88, nseg, dd + DD_HDR, true, dd + DD_ERR));
   if (ctx->prof) TRY(prof_flush(ctx));
   return FMCS_OK;
 }
@@ -1278,7 +1339,7 @@ int gather_launch(fmcs_ctx* ctx, const K* column, const u32* order, K* out, u64 
 template <class K, class P = NoPay>
 int dd_round(fmcs_ctx* ctx, u32 r, u32 keybits, u64 n, const K* column, u32* order, const u32* gin,
              const u32* soff, u32* gout, u32* oout, bool want_groups, u32* dd, K* carried = nullptr,
-             const P* pnext = nullptr, P* pout = nullptr) {
+             const P* pnext = nullptr, P* pout = nullptr, bool check_keys = true) {
   K* ka = carried;
   if (!ka) TRY(getbuf_t(ctx, WS_KA, n, &ka));
   if (r == 0) {
@@ -1288,7 +1349,8 @@ int dd_round(fmcs_ctx* ctx, u32 r, u32 keybits, u64 n, const K* column, u32* ord
     if (!carried) TRY(gather_launch<K>(ctx, column, order, ka, n));
     u32* lv = dd + DD_HDR + (u64)r * DD_LVL_MAX * DD_LVL_WORDS;
     // the sorted keys feed only this round's boundary
-    TRY(seg_sort_dd<K>(ctx, ka, order, n, keybits, soff, dd + (r - 1), n, lv, want_groups));
+    TRY(seg_sort_dd<K>(ctx, ka, order, n, keybits, soff, dd + (r - 1), n, lv, want_groups,
+                       check_keys ? dd + DD_ERR : nullptr));
   }
   if (want_groups) TRY(boundary_dd<K>(ctx, ka, gin, n, gout, oout, dd + r));
   return FMCS_OK;
@@ -1315,8 +1377,8 @@ int dd_round0_carry(fmcs_ctx* ctx, u32 keybits, u64 n, const K* column, u32* ord
 int mrs_dd(fmcs_ctx* ctx, u32 nrounds, const void* const* cols, const u32* banks, const u32* bw, u64 n, u32* order,
            u32* groups_d) {
   u32* dd;
-  TRY(getbuf_t(ctx, WS_DD, DD_WORDS, &dd));
-  TRY(fill(ctx, dd, DD_WORDS, 0));
+  TRY(dd_get(ctx, &dd));
+  TRY(dd_clear(ctx, dd, DD_WORDS));
   u32 *ga, *gb, *offa, *offb;
   TRY(getbuf_t(ctx, WS_HASH, n, &ga));
   TRY(getbuf_t(ctx, WS_HASH2, n, &gb));
@@ -1361,10 +1423,13 @@ int mrs_dd(fmcs_ctx* ctx, u32 nrounds, const void* const* cols, const u32* banks
         default: rc = dd_round0_carry<u64>(ctx, bw[0], n, (const u64*)cols[0], order, gout, oout, want_groups, dd, banks[1], cols[1], carried); break;
       }
     } else {
+      // round 2 of the packed form sorts composites on their low bits (the
+      // first column above them is no error; the second was checked when packed)
+      const bool check = !(fused && r == 1);
       switch (banks[r]) {
-        case 2: rc = dd_round<u16>(ctx, r, bw[r], n, (const u16*)cols[r], order, gin, soff, gout, oout, want_groups, dd, (u16*)kc); break;
-        case 4: rc = dd_round<u32>(ctx, r, bw[r], n, (const u32*)cols[r], order, gin, soff, gout, oout, want_groups, dd, (u32*)kc); break;
-        default: rc = dd_round<u64>(ctx, r, bw[r], n, (const u64*)cols[r], order, gin, soff, gout, oout, want_groups, dd, (u64*)kc); break;
+        case 2: rc = dd_round<u16>(ctx, r, bw[r], n, (const u16*)cols[r], order, gin, soff, gout, oout, want_groups, dd, (u16*)kc, (const NoPay*)nullptr, (NoPay*)nullptr, check); break;
+        case 4: rc = dd_round<u32>(ctx, r, bw[r], n, (const u32*)cols[r], order, gin, soff, gout, oout, want_groups, dd, (u32*)kc, (const NoPay*)nullptr, (NoPay*)nullptr, check); break;
+        default: rc = dd_round<u64>(ctx, r, bw[r], n, (const u64*)cols[r], order, gin, soff, gout, oout, want_groups, dd, (u64*)kc, (const NoPay*)nullptr, (NoPay*)nullptr, check); break;
       }
     }
     if (rc != FMCS_OK) return rc;
@@ -1381,7 +1446,10 @@ int dd_finish(fmcs_ctx* ctx, u32 nrounds, bool have_groups, u32* ngroups) {
   u32 h[DD_ERR + 1];
   TRY(readback(ctx, dd, DD_ERR + 1, h));
   ctx->dd_pending = false;
-  if (h[DD_ERR]) return fail(ctx, FMCS_E_ARG, "keys exceed keybits");
+  if (h[DD_ERR]) {
+    TRY(fill(ctx, dd + DD_ERR, 1, 0));  // reported once
+    return fail(ctx, FMCS_E_ARG, "keys exceed keybits");
+  }
   if (ngroups) *ngroups = have_groups ? h[nrounds - 1] : (nrounds >= 2 ? h[nrounds - 2] : 0u);
   return FMCS_OK;
 }
@@ -1546,6 +1614,40 @@ int sort_pairs_in_scratch(fmcs_ctx* ctx, uint32_t bank, uint32_t keybits, uint64
   return rc;
 }
 
+int Stager::arg_void(void** p, uint64_t bytes, Dir dir, const char* name) {
+  if (!*p || bytes == 0) return FMCS_OK;
+  uint32_t kind = FMCS_MEM_OTHER;
+  TRY(fmcs_pointer_kind(ctx_, *p, &kind));
+  if (kind == FMCS_MEM_DEVICE || kind == FMCS_MEM_PINNED || kind == FMCS_MEM_MANAGED) return FMCS_OK;
+  if (kind == FMCS_MEM_OTHER)
+    return fail(ctx_, FMCS_E_ARG, std::string(name) + " is not memory this context's GPU can use");
+  if (!allow_ || !ctx_->host_staging)
+    return fail(ctx_, FMCS_E_ARG,
+                std::string(name) + " is pageable host memory: pass device memory, or stage it with "
+                                    "fmcs_ctx_set_host_staging" + (allow_ ? "" : " (not with this entry point)"));
+  if (WS_STAGE0 + next_ >= WS_COUNT) return fail(ctx_, FMCS_E_UNSUPPORTED, "too many staged buffers in one call");
+  void* dev;
+  TRY(getbuf(ctx_, (Slot)(WS_STAGE0 + next_++), bytes, &dev));
+  if (dir & IN) HIPCHK_C(ctx_, hipMemcpyAsync(dev, *p, bytes, hipMemcpyHostToDevice, ctx_->stream));
+  if (dir & OUT) outs_.push_back({*p, dev, bytes});
+  *p = dev;
+  return FMCS_OK;
+}
+
+void Stager::set_out_bytes(const void* host, uint64_t bytes) {
+  for (Out& o : outs_)
+    if (o.host == host) o.bytes = std::min(o.bytes, bytes);
+}
+
+int Stager::finish() {
+  if (outs_.empty()) return FMCS_OK;
+  for (const Out& o : outs_)
+    if (o.bytes) HIPCHK_C(ctx_, hipMemcpyAsync(o.host, o.dev, o.bytes, hipMemcpyDeviceToHost, ctx_->stream));
+  HIPCHK_C(ctx_, hipStreamSynchronize(ctx_->stream));
+  outs_.clear();
+  return FMCS_OK;
+}
+
 }  // namespace internal
 }  // namespace fmcs
 
@@ -1554,11 +1656,20 @@ int sort_pairs_in_scratch(fmcs_ctx* ctx, uint32_t bank, uint32_t keybits, uint64
 // =========================================================================
 // Message of the calling thread's last failed fmcs_ctx_create (no context
 // exists to hold it): fmcs_last_error(NULL) returns it.
+using fmcs::internal::Stager;
+
 thread_local std::string g_create_err;
 int create_fail(int code, const std::string& msg) {
   g_create_err = msg;
   return code;
 }
+
+namespace {
+// fmcs_multi_round_sort after its buffer arguments are checked (and staged)
+int mrs_body(fmcs_ctx* ctx, uint32_t nrounds, const void* const* cols_d, const uint32_t* banks,
+             const uint32_t* bitwidths, uint64_t n, uint32_t* order_d, uint32_t* groups_d, int mode,
+             uint32_t* ngroups);
+}  // namespace
 
 extern "C" {
 
@@ -1646,6 +1757,8 @@ int fmcs_ctx_get_tuning(fmcs_ctx* ctx, fmcs_tuning* t) {
 int fmcs_ctx_trim(fmcs_ctx* ctx) {
   if (!ctx) return FMCS_E_ARG;
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  // an unreported device-driven key error goes with the word area: report it now
+  const int pending = ctx->dd_pending ? dd_finish(ctx, 1, true, nullptr) : FMCS_OK;
   graph_drop(ctx);
   ctx->ws_epoch++;
   for (auto& b : ctx->ws) {
@@ -1653,10 +1766,38 @@ int fmcs_ctx_trim(fmcs_ctx* ctx) {
     b.p = nullptr;
     b.bytes = 0;
   }
-  return FMCS_OK;
+  return pending;
 }
 
 const char* fmcs_last_error(fmcs_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+
+uint32_t fmcs_classify_memory(int query_status, int memory_type, int owner_device, int ctx_device) {
+  if (query_status != 0) return FMCS_MEM_HOST;  // unknown to the runtime: pageable host memory
+  switch (memory_type) {
+    case hipMemoryTypeUnregistered: return FMCS_MEM_HOST;
+    case hipMemoryTypeHost: return FMCS_MEM_PINNED;
+    case hipMemoryTypeDevice: return owner_device == ctx_device ? FMCS_MEM_DEVICE : FMCS_MEM_OTHER;
+    case hipMemoryTypeManaged:
+    case hipMemoryTypeUnified: return FMCS_MEM_MANAGED;
+    default: return FMCS_MEM_OTHER;
+  }
+}
+
+int fmcs_pointer_kind(fmcs_ctx* ctx, const void* p, uint32_t* kind) {
+  if (!ctx || !kind) return FMCS_E_ARG;
+  hipPointerAttribute_t a;
+  std::memset(&a, 0, sizeof(a));
+  const hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) (void)hipGetLastError();  // not an error of the call: the answer is "host"
+  *kind = fmcs_classify_memory((int)e, (int)a.type, a.device, ctx->device);
+  return FMCS_OK;
+}
+
+int fmcs_ctx_set_host_staging(fmcs_ctx* ctx, int on) {
+  if (!ctx) return FMCS_E_ARG;
+  ctx->host_staging = on != 0;
+  return FMCS_OK;
+}
 
 uint32_t fmcs_bank_bytes(uint32_t width, int allow_u8) {
   if (allow_u8 && width <= 8) return 1;
@@ -1716,6 +1857,24 @@ int fmcs_stitch_dir(fmcs_ctx* ctx, uint64_t n, uint32_t nsrc, const void* const*
   if (!ctx) return FMCS_E_ARG;
   if (nsrc == 0 || nsrc > STITCH_MAX_SRC || ndst == 0 || ndst > STITCH_MAX_DST || nent == 0 || nent > STITCH_MAX_ENT)
     return fail(ctx, FMCS_E_ARG, "stitch: too many columns or entries");
+  if (!src_d || !src_bank || !src_bw || !entries || !dst_d || !dst_bank) return FMCS_E_ARG;
+  for (u32 s = 0; s < nsrc; ++s)
+    if (src_bank[s] > 8) return fail(ctx, FMCS_E_ARG, "stitch: bad source bank/width");
+  for (u32 c = 0; c < ndst; ++c)
+    if (dst_bank[c] > 8) return fail(ctx, FMCS_E_ARG, "stitch: destination too narrow");
+  Stager st(ctx);
+  const void* srcs[STITCH_MAX_SRC];
+  void* dsts[STITCH_MAX_DST];
+  for (u32 s = 0; s < nsrc; ++s) {
+    srcs[s] = src_d[s];
+    TRY(st.arg(srcs[s], n * src_bank[s], Stager::IN, "src_d"));
+  }
+  for (u32 c = 0; c < ndst; ++c) {
+    dsts[c] = dst_d[c];
+    TRY(st.arg(dsts[c], n * dst_bank[c], Stager::OUT, "dst_d"));
+  }
+  src_d = srcs;
+  dst_d = dsts;
   StitchArgs a;
   std::memset(&a, 0, sizeof(a));
   a.n = n;
@@ -1756,7 +1915,8 @@ int fmcs_stitch_dir(fmcs_ctx* ctx, uint64_t n, uint32_t nsrc, const void* const*
   u64 threads = (n + 3) / 4;
   u32 blocks = (u32)((threads + 255) / 256);
   hipLaunchKernelGGL(k_stitch, dim3(blocks), dim3(256), 0, ctx->stream, a);
-  return launch_check(ctx, "k_stitch");
+  TRY(launch_check(ctx, "k_stitch"));
+  return st.finish();
 }
 
 int fmcs_gather(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* values_d, const uint32_t* order_d,
@@ -1764,69 +1924,98 @@ int fmcs_gather(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* values_d, 
   if (!ctx) return FMCS_E_ARG;
   if (!bank_ok(bank)) return fail(ctx, FMCS_E_ARG, "gather: bank must be 2, 4 or 8");
   if (n == 0) return FMCS_OK;
+  Stager st(ctx);
+  TRY(st.arg(values_d, n * bank, Stager::IN, "values_d"));
+  TRY(st.arg(order_d, n * 4, Stager::IN, "order_d"));
+  TRY(st.arg(out_d, n * bank, Stager::OUT, "out_d"));
   if (((uintptr_t)order_d & 15) != 0) return fail(ctx, FMCS_E_ARG, "gather: order must be 16-byte aligned");
   switch (bank) {
-    case 2: return gather_launch<u16>(ctx, (const u16*)values_d, order_d, (u16*)out_d, n);
-    case 4: return gather_launch<u32>(ctx, (const u32*)values_d, order_d, (u32*)out_d, n);
-    default: return gather_launch<u64>(ctx, (const u64*)values_d, order_d, (u64*)out_d, n);
+    case 2: TRY(gather_launch<u16>(ctx, (const u16*)values_d, order_d, (u16*)out_d, n)); break;
+    case 4: TRY(gather_launch<u32>(ctx, (const u32*)values_d, order_d, (u32*)out_d, n)); break;
+    default: TRY(gather_launch<u64>(ctx, (const u64*)values_d, order_d, (u64*)out_d, n)); break;
   }
+  return st.finish();
 }
 
 int fmcs_extract_group_info(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* sorted_d, const uint32_t* in_group_d,
                             uint32_t* out_group_d, uint32_t* seg_offsets_d, uint32_t* ngroups) {
   if (!ctx || !ngroups) return FMCS_E_ARG;
+  if (!bank_ok(bank)) return fail(ctx, FMCS_E_ARG, "bank must be 2, 4 or 8");
+  Stager st(ctx);
+  uint32_t* const off_caller = seg_offsets_d;
+  TRY(st.arg(sorted_d, n * bank, Stager::IN, "sorted_d"));
+  TRY(st.arg(in_group_d, n * 4, Stager::IN, "in_group_d"));
+  TRY(st.arg(out_group_d, n * 4, Stager::OUT, "out_group_d"));
+  TRY(st.arg(seg_offsets_d, (n + 1) * 4, Stager::OUT, "seg_offsets_d"));
   switch (bank) {
-    case 2: return boundary<u16>(ctx, (const u16*)sorted_d, in_group_d, n, out_group_d, seg_offsets_d, ngroups);
-    case 4: return boundary<u32>(ctx, (const u32*)sorted_d, in_group_d, n, out_group_d, seg_offsets_d, ngroups);
-    case 8: return boundary<u64>(ctx, (const u64*)sorted_d, in_group_d, n, out_group_d, seg_offsets_d, ngroups);
-    default: return fail(ctx, FMCS_E_ARG, "bank must be 2, 4 or 8");
+    case 2: TRY(boundary<u16>(ctx, (const u16*)sorted_d, in_group_d, n, out_group_d, seg_offsets_d, ngroups)); break;
+    case 4: TRY(boundary<u32>(ctx, (const u32*)sorted_d, in_group_d, n, out_group_d, seg_offsets_d, ngroups)); break;
+    default: TRY(boundary<u64>(ctx, (const u64*)sorted_d, in_group_d, n, out_group_d, seg_offsets_d, ngroups)); break;
   }
+  st.set_out_bytes(off_caller, ((u64)*ngroups + 1) * 4);  // the caller's array holds ngroups + 1 words
+  return st.finish();
 }
 
 int fmcs_sort_pairs(fmcs_ctx* ctx, uint32_t bank, uint32_t keybits, uint64_t n, void* keys_d, uint32_t* oids_d) {
   if (!ctx) return FMCS_E_ARG;
+  if (!bank_ok(bank)) return fail(ctx, FMCS_E_ARG, "bank must be 2, 4 or 8");
   if (keybits == 0 || keybits > 8 * bank) return fail(ctx, FMCS_E_ARG, "bad keybits");
+  Stager st(ctx);
+  TRY(st.arg(keys_d, n * bank, Stager::INOUT, "keys_d"));
+  TRY(st.arg(oids_d, n * 4, Stager::INOUT, "oids_d"));
   switch (bank) {
-    case 2: return radix_sort<u16>(ctx, (u16*)keys_d, oids_d, (u16*)keys_d, oids_d, n, keybits);
-    case 4: return radix_sort<u32>(ctx, (u32*)keys_d, oids_d, (u32*)keys_d, oids_d, n, keybits);
-    case 8: return radix_sort<u64>(ctx, (u64*)keys_d, oids_d, (u64*)keys_d, oids_d, n, keybits);
-    default: return fail(ctx, FMCS_E_ARG, "bank must be 2, 4 or 8");
+    case 2: TRY(radix_sort<u16>(ctx, (u16*)keys_d, oids_d, (u16*)keys_d, oids_d, n, keybits)); break;
+    case 4: TRY(radix_sort<u32>(ctx, (u32*)keys_d, oids_d, (u32*)keys_d, oids_d, n, keybits)); break;
+    default: TRY(radix_sort<u64>(ctx, (u64*)keys_d, oids_d, (u64*)keys_d, oids_d, n, keybits)); break;
   }
+  return st.finish();
 }
 
 int fmcs_fill_column(fmcs_ctx* ctx, uint32_t bank, uint64_t n, uint64_t seed, uint64_t card, int zipf,
                      uint32_t width, void* out_d) {
   if (!ctx || !out_d) return FMCS_E_ARG;
+  if (!bank_ok(bank)) return fail(ctx, FMCS_E_ARG, "bank must be 2, 4 or 8");
   if (width == 0 || width > 8 * bank || card == 0) return fail(ctx, FMCS_E_ARG, "fill_column: bad width or card");
   if (n == 0) return FMCS_OK;
+  Stager st(ctx);
+  TRY(st.arg(out_d, n * bank, Stager::OUT, "out_d"));
   const u64 mask = width >= 64 ? ~0ull : ((1ull << width) - 1);
   const u64 span = width >= 64 ? ~0ull : (1ull << width);
   const u64 step = std::max<u64>(1, span / card);
   const double lc = std::log2((double)card);
   const u64 grid = cap_grid((n + 255) / 256, 8192);
   switch (bank) {
-    case 2: return klaunch(ctx, "k_fill_column", k_fill_column<u16>, grid, 0.0, nullptr, 0.0, (u16*)out_d, n, seed, card, (u32)(zipf != 0), lc, step, mask);
-    case 4: return klaunch(ctx, "k_fill_column", k_fill_column<u32>, grid, 0.0, nullptr, 0.0, (u32*)out_d, n, seed, card, (u32)(zipf != 0), lc, step, mask);
-    case 8: return klaunch(ctx, "k_fill_column", k_fill_column<u64>, grid, 0.0, nullptr, 0.0, (u64*)out_d, n, seed, card, (u32)(zipf != 0), lc, step, mask);
-    default: return fail(ctx, FMCS_E_ARG, "bank must be 2, 4 or 8");
+    case 2: TRY(klaunch(ctx, "k_fill_column", k_fill_column<u16>, grid, 0.0, nullptr, 0.0, (u16*)out_d, n, seed, card, (u32)(zipf != 0), lc, step, mask)); break;
+    case 4: TRY(klaunch(ctx, "k_fill_column", k_fill_column<u32>, grid, 0.0, nullptr, 0.0, (u32*)out_d, n, seed, card, (u32)(zipf != 0), lc, step, mask)); break;
+    default: TRY(klaunch(ctx, "k_fill_column", k_fill_column<u64>, grid, 0.0, nullptr, 0.0, (u64*)out_d, n, seed, card, (u32)(zipf != 0), lc, step, mask)); break;
   }
+  return st.finish();
 }
 
 int fmcs_sort_pairs_merge(fmcs_ctx* ctx, uint32_t bank, uint64_t n, void* keys_d, uint32_t* oids_d) {
   if (!ctx) return FMCS_E_ARG;
+  if (!bank_ok(bank)) return fail(ctx, FMCS_E_ARG, "bank must be 2, 4 or 8");
+  Stager st(ctx);
+  TRY(st.arg(keys_d, n * bank, Stager::INOUT, "keys_d"));
+  TRY(st.arg(oids_d, n * 4, Stager::INOUT, "oids_d"));
   switch (bank) {
-    case 2: return merge_sort<u16>(ctx, (u16*)keys_d, oids_d, (u16*)keys_d, oids_d, n);
-    case 4: return merge_sort<u32>(ctx, (u32*)keys_d, oids_d, (u32*)keys_d, oids_d, n);
-    case 8: return merge_sort<u64>(ctx, (u64*)keys_d, oids_d, (u64*)keys_d, oids_d, n);
-    default: return fail(ctx, FMCS_E_ARG, "bank must be 2, 4 or 8");
+    case 2: TRY(merge_sort<u16>(ctx, (u16*)keys_d, oids_d, (u16*)keys_d, oids_d, n)); break;
+    case 4: TRY(merge_sort<u32>(ctx, (u32*)keys_d, oids_d, (u32*)keys_d, oids_d, n)); break;
+    default: TRY(merge_sort<u64>(ctx, (u64*)keys_d, oids_d, (u64*)keys_d, oids_d, n)); break;
   }
+  return st.finish();
 }
 
 int fmcs_segmented_sort(fmcs_ctx* ctx, uint32_t bank, uint32_t keybits, uint64_t n, void* keys_d, uint32_t* oids_d,
                         const uint32_t* seg_offsets_d, uint32_t nseg, int mode, fmcs_group_info* info) {
   if (!ctx) return FMCS_E_ARG;
+  if (!bank_ok(bank)) return fail(ctx, FMCS_E_ARG, "bank must be 2, 4 or 8");
   if (keybits == 0 || keybits > 8 * bank) return fail(ctx, FMCS_E_ARG, "bad keybits");
   if (n >= (1ull << 30)) return fail(ctx, FMCS_E_UNSUPPORTED, "n < 2^30 required");
+  Stager st(ctx);
+  TRY(st.arg(keys_d, n * bank, Stager::INOUT, "keys_d"));
+  TRY(st.arg(oids_d, n * 4, Stager::INOUT, "oids_d"));
+  TRY(st.arg(seg_offsets_d, ((u64)nseg + 1) * 4, Stager::IN, "seg_offsets_d"));
   if (info) TRY(group_info_from_offsets(ctx, bank, seg_offsets_d, nseg, info));
   const u32* grp = nullptr;
   if (mode == FMCS_MODE_UNBINNED) {
@@ -1844,11 +2033,14 @@ int fmcs_segmented_sort(fmcs_ctx* ctx, uint32_t bank, uint32_t keybits, uint64_t
     grp = g;
   }
   switch (bank) {
-    case 2: return seg_sort<u16>(ctx, (u16*)keys_d, oids_d, n, keybits, seg_offsets_d, nseg, mode, grp);
-    case 4: return seg_sort<u32>(ctx, (u32*)keys_d, oids_d, n, keybits, seg_offsets_d, nseg, mode, grp);
-    case 8: return seg_sort<u64>(ctx, (u64*)keys_d, oids_d, n, keybits, seg_offsets_d, nseg, mode, grp);
-    default: return fail(ctx, FMCS_E_ARG, "bank must be 2, 4 or 8");
+    case 2: TRY(seg_sort<u16>(ctx, (u16*)keys_d, oids_d, n, keybits, seg_offsets_d, nseg, mode, grp)); break;
+    case 4: TRY(seg_sort<u32>(ctx, (u32*)keys_d, oids_d, n, keybits, seg_offsets_d, nseg, mode, grp)); break;
+    default: TRY(seg_sort<u64>(ctx, (u64*)keys_d, oids_d, n, keybits, seg_offsets_d, nseg, mode, grp)); break;
   }
+  // the device-driven forms report keys above keybits through the context's
+  // error word, read by the next call that returns a count or by fmcs_ctx_sync
+  if ((mode == FMCS_MODE_DEVICE || mode == FMCS_MODE_GRAPH) && n > 1 && nseg) ctx->dd_pending = true;
+  return st.finish();
 }
 
 int fmcs_sort_by_group(fmcs_ctx* ctx, uint32_t bank, uint32_t keybits, uint64_t n, const void* column_d,
@@ -1858,12 +2050,25 @@ int fmcs_sort_by_group(fmcs_ctx* ctx, uint32_t bank, uint32_t keybits, uint64_t 
   if (keybits == 0 || keybits > 8 * bank) return fail(ctx, FMCS_E_ARG, "bad keybits");
   if (!last_round && !out_group_d) return fail(ctx, FMCS_E_ARG, "out_group required unless last round");
   if (n >= (1ull << 30)) return fail(ctx, FMCS_E_UNSUPPORTED, "n < 2^30 required");
+  if (!bank_ok(bank)) return fail(ctx, FMCS_E_ARG, "bank must be 2, 4 or 8");
+  // the reference's buffers (src/composer.cpp:1222-1261): the column and
+  // the previous groups are read, the order is read and rewritten, the new
+  // groups (and the sorted values) are written
+  Stager st(ctx);
+  TRY(st.arg(column_d, n * bank, Stager::IN, "column_d"));
+  TRY(st.arg(order_d, n * 4, Stager::INOUT, "order_d"));
+  TRY(st.arg(in_group_d, n * 4, Stager::IN, "in_group_d"));
+  TRY(st.arg(out_group_d, n * 4, Stager::OUT, "out_group_d"));
+  TRY(st.arg(sorted_d, n * bank, Stager::OUT, "sorted_d"));
   switch (bank) {
-    case 2: return sort_by_group_t<u16>(ctx, keybits, n, (const u16*)column_d, order_d, in_group_d, out_group_d, last_round, mode, (u16*)sorted_d, info, ngroups, nullptr, 0, nullptr, nullptr, nullptr, nullptr);
-    case 4: return sort_by_group_t<u32>(ctx, keybits, n, (const u32*)column_d, order_d, in_group_d, out_group_d, last_round, mode, (u32*)sorted_d, info, ngroups, nullptr, 0, nullptr, nullptr, nullptr, nullptr);
-    case 8: return sort_by_group_t<u64>(ctx, keybits, n, (const u64*)column_d, order_d, in_group_d, out_group_d, last_round, mode, (u64*)sorted_d, info, ngroups, nullptr, 0, nullptr, nullptr, nullptr, nullptr);
-    default: return fail(ctx, FMCS_E_ARG, "bank must be 2, 4 or 8");
+    case 2: TRY(sort_by_group_t<u16>(ctx, keybits, n, (const u16*)column_d, order_d, in_group_d, out_group_d, last_round, mode, (u16*)sorted_d, info, ngroups, nullptr, 0, nullptr, nullptr, nullptr, nullptr)); break;
+    case 4: TRY(sort_by_group_t<u32>(ctx, keybits, n, (const u32*)column_d, order_d, in_group_d, out_group_d, last_round, mode, (u32*)sorted_d, info, ngroups, nullptr, 0, nullptr, nullptr, nullptr, nullptr)); break;
+    default: TRY(sort_by_group_t<u64>(ctx, keybits, n, (const u64*)column_d, order_d, in_group_d, out_group_d, last_round, mode, (u64*)sorted_d, info, ngroups, nullptr, 0, nullptr, nullptr, nullptr, nullptr)); break;
   }
+  // a later round's device-driven segmented sort reports keys above keybits
+  // through the error word (the next count readback or fmcs_ctx_sync)
+  if (in_group_d && n > 1 && (mode == FMCS_MODE_DEVICE || mode == FMCS_MODE_GRAPH)) ctx->dd_pending = true;
+  return st.finish();
 }
 
 int fmcs_multi_round_sort(fmcs_ctx* ctx, uint32_t nrounds, const void* const* cols_d, const uint32_t* banks,
@@ -1875,6 +2080,24 @@ int fmcs_multi_round_sort(fmcs_ctx* ctx, uint32_t nrounds, const void* const* co
     if (!bank_ok(banks[r]) || bitwidths[r] == 0 || bitwidths[r] > 8 * banks[r])
       return fail(ctx, FMCS_E_ARG, "bad bank/bitwidth");
   if (n == 0) return FMCS_OK;
+  // the reference's column banks (src/composer.cpp:48-135) are read; the
+  // final order and groups are written
+  Stager st(ctx);
+  std::vector<const void*> cols(cols_d, cols_d + nrounds);
+  for (u32 r = 0; r < nrounds; ++r) TRY(st.arg(cols[r], n * banks[r], Stager::IN, "cols_d"));
+  TRY(st.arg(order_d, n * 4, Stager::OUT, "order_d"));
+  TRY(st.arg(groups_d, n * 4, Stager::OUT, "groups_d"));
+  TRY(mrs_body(ctx, nrounds, cols.data(), banks, bitwidths, n, order_d, groups_d, mode, ngroups));
+  return st.finish();
+}
+
+}  // extern "C"
+
+namespace {
+
+int mrs_body(fmcs_ctx* ctx, uint32_t nrounds, const void* const* cols_d, const uint32_t* banks,
+             const uint32_t* bitwidths, uint64_t n, uint32_t* order_d, uint32_t* groups_d, int mode,
+             uint32_t* ngroups) {
   if (mode == FMCS_MODE_DEVICE || mode == FMCS_MODE_GRAPH) {
     if (nrounds > DD_ROUNDS_MAX) return fail(ctx, FMCS_E_UNSUPPORTED, "device-driven path supports at most 64 rounds");
     if (((uintptr_t)order_d & 15) != 0) return fail(ctx, FMCS_E_ARG, "order must be 16-byte aligned");
@@ -1927,6 +2150,10 @@ int fmcs_multi_round_sort(fmcs_ctx* ctx, uint32_t nrounds, const void* const* co
   return FMCS_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
 int fmcs_ctx_set_profiling(fmcs_ctx* ctx, int on) {
   if (!ctx) return FMCS_E_ARG;
   TRY(prof_flush(ctx));
@@ -1973,6 +2200,13 @@ int fmcs_hash_round(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* values
   if (!(bank == 1 || bank_ok(bank))) return fail(ctx, FMCS_E_ARG, "bank must be 1, 2, 4 or 8");
   if (!last_round && (!out_order_d || !out_group_d || !order_d))
     return fail(ctx, FMCS_E_ARG, "order, out_order and out_group required unless last round");
+  Stager st(ctx);
+  TRY(st.arg(values_d, n * bank, Stager::IN, "values_d"));
+  TRY(st.arg(order_d, n * 4, Stager::IN, "order_d"));
+  TRY(st.arg(in_group_d, n * 4, Stager::IN, "in_group_d"));
+  TRY(st.arg(gid_d, n * 4, Stager::OUT, "gid_d"));
+  TRY(st.arg(out_order_d, n * 4, Stager::OUT, "out_order_d"));
+  TRY(st.arg(out_group_d, n * 4, Stager::OUT, "out_group_d"));
   const void* keys = values_d;
   if (in_group_d && n) {
     if (!order_d) return fail(ctx, FMCS_E_ARG, "order required after round 1");
@@ -1998,13 +2232,15 @@ int fmcs_hash_round(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* values
     TRY(radix_sort<u32>(ctx, gid_d, order_d, out_group_d, out_order_d, n, bits));
   }
   if (ngroups) *ngroups = ng;
-  return FMCS_OK;
+  return st.finish();
 }
 
 int fmcs_distinct_count(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* col_d, uint64_t* count) {
   if (!ctx || !count || (n && !col_d) || !(bank == 1 || bank == 2 || bank == 4 || bank == 8))
     return FMCS_E_ARG;
   if (n >= (1ull << 30)) return fail(ctx, FMCS_E_UNSUPPORTED, "at most 2^30 - 1 rows");
+  Stager st(ctx);
+  TRY(st.arg(col_d, n * bank, Stager::IN, "col_d"));
   u32* gid;
   TRY(getbuf_t(ctx, WS_HASH3, std::max<u64>(n, 1), &gid));
   u32 ng = 0;
@@ -2023,6 +2259,12 @@ int fmcs_multi_round_hash(fmcs_ctx* ctx, uint32_t nrounds, const void* const* co
     if (ngroups) *ngroups = 0;
     return FMCS_OK;
   }
+  Stager st(ctx);
+  std::vector<const void*> colv(cols_d, cols_d + nrounds);
+  for (u32 r = 0; r < nrounds; ++r) TRY(st.arg(colv[r], n * banks[r], Stager::IN, "cols_d"));
+  cols_d = colv.data();
+  TRY(st.arg(order_d, n * 4, Stager::OUT, "order_d"));
+  TRY(st.arg(gid_d, n * 4, Stager::OUT, "gid_d"));
   hipLaunchKernelGGL(k_iota_u32, dim3((u32)std::min<u64>((n + 255) / 256, 8192)), dim3(256), 0, ctx->stream, order_d, n);
   TRY(launch_check(ctx, "k_iota_u32"));
   HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
@@ -2066,7 +2308,7 @@ int fmcs_multi_round_hash(fmcs_ctx* ctx, uint32_t nrounds, const void* const* co
   ctx->breakdown[2] = 0;
   ctx->breakdown[3] = 0;
   if (ngroups) *ngroups = ng;
-  return FMCS_OK;
+  return st.finish();
 }
 
 }  // extern "C"

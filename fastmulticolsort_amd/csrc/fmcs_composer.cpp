@@ -265,6 +265,26 @@ int reset_pairs(fmcs_ctx* ctx, CalArgs* a) {
 
 using namespace fmcs::host;
 
+namespace {
+
+// Checks (and, if the context stages host buffers, stages) the column banks
+// and the order / group outputs of a plan run.
+int stage_plan_args(fmcs::internal::Stager& st, uint32_t ncols, const void* const* cols_d, const uint32_t* banks,
+                    uint64_t nrows, std::vector<const void*>& cols, uint32_t*& order_d, uint32_t*& groups_d) {
+  using fmcs::internal::Stager;
+  cols.assign(cols_d, cols_d + ncols);
+  for (uint32_t i = 0; i < ncols; ++i) {
+    if (banks[i] != 2 && banks[i] != 4 && banks[i] != 8) return FMCS_E_ARG;
+    int rc = st.arg(cols[i], nrows * banks[i], Stager::IN, "cols_d");
+    if (rc != FMCS_OK) return rc;
+  }
+  int rc = st.arg(order_d, nrows * 4, Stager::OUT, "order_d");
+  if (rc == FMCS_OK) rc = st.arg(groups_d, nrows * 4, Stager::OUT, "groups_d");
+  return rc;
+}
+
+}  // namespace
+
 extern "C" {
 
 int fmcs_run_plan(fmcs_ctx* ctx, uint32_t ncols, const void* const* cols_d, const uint32_t* banks, const uint32_t* bw,
@@ -280,10 +300,14 @@ int fmcs_run_plan_dir(fmcs_ctx* ctx, uint32_t ncols, const void* const* cols_d, 
                       double* ms6, double* total_ms, uint32_t* ngroups) {
   if (!ctx || !cols_d || !banks || !bw || !col_order || !split || !order_d || ncols == 0 || ncols > 64)
     return FMCS_E_ARG;
+  fmcs::internal::Stager st(ctx);
+  std::vector<const void*> cv;
+  int rc = stage_plan_args(st, ncols, cols_d, banks, nrows, cv, order_d, groups_d);
+  if (rc != FMCS_OK) return rc;
   std::vector<std::unique_ptr<Column>> owned;
   std::vector<Column*> cols;
   for (uint32_t i = 0; i < ncols; ++i) {
-    owned.push_back(make_column(banks[i], const_cast<void*>(cols_d[i])));
+    owned.push_back(make_column(banks[i], const_cast<void*>(cv[i])));
     cols.push_back(owned.back().get());
   }
   compose_params_t p;
@@ -292,7 +316,8 @@ int fmcs_run_plan_dir(fmcs_ctx* ctx, uint32_t ncols, const void* const* cols_d, 
   p.mode = mode;
   StitchComposer sc(ctx, ncols, nrows, p, cols);
   std::vector<uint32_t> sp(split, split + nsplit);
-  return sc.run_single_plan(col_order, sp, false, order_d, groups_d, ms6, total_ms, ngroups);
+  rc = sc.run_single_plan(col_order, sp, false, order_d, groups_d, ms6, total_ms, ngroups);
+  return rc != FMCS_OK ? rc : st.finish();
 }
 
 int fmcs_optimize_and_run(fmcs_ctx* ctx, const fmcs_cost_model* m, uint32_t ncols, const void* const* cols_d,
@@ -303,6 +328,11 @@ int fmcs_optimize_and_run(fmcs_ctx* ctx, const fmcs_cost_model* m, uint32_t ncol
   if (!ctx || !cols_d || !banks || !bw || !order_d || !best_order || !best_split || !best_nsplit || ncols == 0 ||
       ncols > 64)
     return FMCS_E_ARG;
+  fmcs::internal::Stager st(ctx);
+  std::vector<const void*> staged;
+  int src = stage_plan_args(st, ncols, cols_d, banks, nrows, staged, order_d, groups_d);
+  if (src != FMCS_OK) return src;
+  cols_d = staged.data();
   auto t0 = std::chrono::steady_clock::now();
   fmcs_cost_model dm;
   if (!m) {
@@ -335,8 +365,9 @@ int fmcs_optimize_and_run(fmcs_ctx* ctx, const fmcs_cost_model* m, uint32_t ncol
   std::copy(ord.begin(), ord.end(), best_order);
   std::copy(split.begin(), split.begin() + ns, best_split);
   *best_nsplit = ns;
-  return fmcs_run_plan(ctx, ncols, cols_d, banks, bw, ord.data(), ns, split.data(), mode, nrows, order_d, groups_d,
-                       ms6, total_ms, ngroups);
+  rc = fmcs_run_plan(ctx, ncols, cols_d, banks, bw, ord.data(), ns, split.data(), mode, nrows, order_d, groups_d, ms6,
+                     total_ms, ngroups);
+  return rc != FMCS_OK ? rc : st.finish();
 }
 
 int fmcs_run_plan_hash(fmcs_ctx* ctx, uint32_t ncols, const void* const* cols_d, const uint32_t* banks,
@@ -345,10 +376,14 @@ int fmcs_run_plan_hash(fmcs_ctx* ctx, uint32_t ncols, const void* const* cols_d,
                        uint32_t* ngroups) {
   if (!ctx || !cols_d || !banks || !bw || !col_order || !split || !order_d || !gid_d || ncols == 0 || ncols > 64)
     return FMCS_E_ARG;
+  fmcs::internal::Stager st(ctx);
+  std::vector<const void*> cv;
+  int rc = stage_plan_args(st, ncols, cols_d, banks, nrows, cv, order_d, gid_d);
+  if (rc != FMCS_OK) return rc;
   std::vector<std::unique_ptr<Column>> owned;
   std::vector<Column*> cols;
   for (uint32_t i = 0; i < ncols; ++i) {
-    owned.push_back(make_column(banks[i], const_cast<void*>(cols_d[i])));
+    owned.push_back(make_column(banks[i], const_cast<void*>(cv[i])));
     cols.push_back(owned.back().get());
   }
   compose_params_t p;
@@ -356,7 +391,8 @@ int fmcs_run_plan_hash(fmcs_ctx* ctx, uint32_t ncols, const void* const* cols_d,
   p.hash_type = (uint32_t)hash_type;
   StitchComposer sc(ctx, ncols, nrows, p, cols);
   std::vector<uint32_t> sp(split, split + nsplit);
-  return sc.run_single_plan(col_order, sp, true, order_d, gid_d, nullptr, total_ms, ngroups);
+  rc = sc.run_single_plan(col_order, sp, true, order_d, gid_d, nullptr, total_ms, ngroups);
+  return rc != FMCS_OK ? rc : st.finish();
 }
 
 int fmcs_calibrate(fmcs_ctx* ctx, uint64_t nrows, fmcs_cost_model* out) {

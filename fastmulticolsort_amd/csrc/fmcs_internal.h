@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/fmcs.h"
 
@@ -32,6 +33,43 @@ int scratch_of(fmcs_ctx* ctx, void** ptr);
 // [scratch, scratch + cap): no device allocation.
 int sort_pairs_in_scratch(fmcs_ctx* ctx, uint32_t bank, uint32_t keybits, uint64_t n, void* keys, uint32_t* oids,
                           void* scratch, size_t cap);
+
+// Buffer arguments of an entry point.  Every non-null *_d pointer is
+// classified (fmcs_pointer_kind) before anything is launched: device,
+// pinned and managed memory pass; another GPU's memory fails with
+// FMCS_E_ARG; pageable host memory fails with FMCS_E_ARG too, unless the
+// context stages host buffers (fmcs_ctx_set_host_staging) and the entry
+// point allows it, in which case the pointer is replaced by a device copy
+// in a workspace slot: inputs are copied in at once, outputs are copied
+// back (and the stream synchronized) by finish().
+class Stager {
+ public:
+  enum Dir { IN = 1, OUT = 2, INOUT = 3 };
+  explicit Stager(fmcs_ctx* ctx, bool allow_staging = true) : ctx_(ctx), allow_(allow_staging) {}
+  template <class T>
+  int arg(T*& p, uint64_t bytes, Dir dir, const char* name) {
+    void* v = (void*)p;
+    const int rc = arg_void(&v, bytes, dir, name);
+    p = (T*)v;
+    return rc;
+  }
+  int arg_void(void** p, uint64_t bytes, Dir dir, const char* name);
+  // the copy-back of the output staged for the caller's pointer `host`
+  // takes only `bytes` (an output whose length the call determines)
+  void set_out_bytes(const void* host, uint64_t bytes);
+  int finish();
+
+ private:
+  struct Out {
+    void* host;
+    void* dev;
+    uint64_t bytes;
+  };
+  fmcs_ctx* ctx_;
+  bool allow_;
+  int next_ = 0;
+  std::vector<Out> outs_;
+};
 
 }  // namespace internal
 }  // namespace fmcs

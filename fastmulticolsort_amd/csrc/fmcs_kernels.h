@@ -50,6 +50,8 @@ struct TileSortArgs {
   u32 size_lo, size_hi;  // bucket sort: only windows of size_lo + 1 .. size_hi rows (size_hi 0: all)
   u32* rklist;        // radix tile sort: windows too wide to pack -> appended here
   u32* rkcount;       //   (with rkcount), for the comparison rank
+  u64 bad;            // bucket sort, if err is set: key bits above the declared width;
+  u32* err;           //   a window holding such a key sets *err
 };
 
 constexpr u32 XCDS = 8;  // accelerator dies of an MI355X; workgroup b runs on die b % XCDS
@@ -107,7 +109,9 @@ __global__ void k_scan_excl(const u32*, u32*, u64, u32*, u32*, u32*);
 __global__ void k_fill_u32(u32*, u64, u32);
 __global__ void k_iota_u32(u32*, u64);
 template <class K>
-__global__ void k_composite(const K*, const u32*, u64, u32, u64*);
+__global__ void k_composite(const K*, const u32*, u64, u32, u64*, u32*);
+template <class K>
+__global__ void k_key_check(const K*, u64, u64, u32*);
 template <class K>
 __global__ void k_strip_composite(const u64*, u64, u64, K*);
 template <class K>

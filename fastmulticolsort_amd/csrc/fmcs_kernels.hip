@@ -955,6 +955,12 @@ __device__ __forceinline__ void tile_bucket_body(const TileSortArgs<K>& a, const
   const u32 tid = threadIdx.x;
   const u32 w = r.w, lo = r.lo, hi = r.hi;
   if (lo >= hi) return;
+  if (a.err) {  // rows past the window's end repeat its last row
+    u64 b = 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) b |= (u64)r.kk[e] & a.bad;
+    if (__ballot(b != 0) && __lane_id() == 0) atomicOr(a.err, 1u);
+  }
   const u32 s0 = r.s0, s1 = r.s1;
   const u32 cnt = hi - lo;
   const u32 kb = a.kb;
@@ -1397,11 +1403,29 @@ __global__ void k_iota_u32(u32* __restrict__ p, u64 n) {
   for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) p[i] = (u32)i;
 }
 
-// Composite key for the single unbinned segmented sort: (group << kb) | key.
+// Composite key for the single unbinned segmented sort: (group << kb) |
+// key.  A key with bits above kb sets *err (and is masked, so its row stays
+// in its group).
 template <class K>
-__global__ void k_composite(const K* __restrict__ k, const u32* __restrict__ g, u64 n, u32 kb, u64* __restrict__ out) {
+__global__ void k_composite(const K* __restrict__ k, const u32* __restrict__ g, u64 n, u32 kb, u64* __restrict__ out,
+                            u32* __restrict__ err) {
+  const u64 kmask = kb >= 64 ? ~0ull : ((1ull << kb) - 1);
+  u64 bad = 0;
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) {
+    const u64 key = (u64)k[i];
+    bad |= key & ~kmask;
+    out[i] = ((u64)g[i] << kb) | (key & kmask);
+  }
+  if (__ballot(bad != 0) && __lane_id() == 0) atomicOr(err, 1u);
+}
+
+// *err |= 1 if any key has a bit in bad_mask.
+template <class K>
+__global__ void k_key_check(const K* __restrict__ k, u64 n, u64 bad_mask, u32* __restrict__ err) {
+  u64 bad = 0;
   for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x)
-    out[i] = ((u64)g[i] << kb) | (u64)k[i];
+    bad |= (u64)k[i] & bad_mask;
+  if (__ballot(bad != 0) && __lane_id() == 0) atomicOr(err, 1u);
 }
 
 template <class K>
@@ -1575,7 +1599,8 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
   template __global__ void k_tile_rank<K, 8>(TileSortArgs<K>);                                                      \
   template __global__ void k_copy_segs<K>(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, u32,  \
                                           u32);                                                                     \
-  template __global__ void k_composite<K>(const K*, const u32*, u64, u32, u64*);                                    \
+  template __global__ void k_composite<K>(const K*, const u32*, u64, u32, u64*, u32*);                              \
+  template __global__ void k_key_check<K>(const K*, u64, u64, u32*);                                                \
   template __global__ void k_strip_composite<K>(const u64*, u64, u64, K*);                                          \
   template __global__ void k_hash_insert<K>(const K*, const u32*, u64, u32*, u32, u32*, u32*);                    \
   template __global__ void k_fill_random<K>(K*, u64, u64, u32);                                                    \

@@ -14,6 +14,9 @@
 
 #include "../../include/fmcs.h"
 #include "../../include/fmcs_plan.h"
+#include <initializer_list>
+#include <utility>
+
 #include "fmcs_internal.h"
 
 namespace fmcs {
@@ -198,6 +201,22 @@ bool shape_ok(const fmcs_mlp_shape* s) {
 
 using namespace fmcs;
 
+namespace {
+
+// The network's buffers are device memory the caller trains in place: they
+// are checked (no staging) before any launch.
+int check_bufs(fmcs_ctx* ctx, std::initializer_list<std::pair<const void*, const char*>> bufs) {
+  internal::Stager st(ctx, false);
+  for (const auto& b : bufs) {
+    void* p = const_cast<void*>(b.first);
+    const int rc = st.arg(p, 1, internal::Stager::IN, b.second);
+    if (rc != FMCS_OK) return rc;
+  }
+  return FMCS_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int fmcs_mlp_param_count(const fmcs_mlp_shape* s, uint32_t* count) {
@@ -211,6 +230,8 @@ int fmcs_mlp_forward(fmcs_ctx* ctx, const fmcs_mlp_shape* s, const float* params
                      uint32_t batch, float* y_d, float* acts_d) {
   if (!ctx || !shape_ok(s) || !params_d || !x_d || !y_d) return FMCS_E_ARG;
   if (batch == 0) return FMCS_OK;
+  const int crc = check_bufs(ctx, {{params_d, "params_d"}, {x_d, "x_d"}, {y_d, "y_d"}, {acts_d, "acts_d"}});
+  if (crc != FMCS_OK) return crc;
   Shape sh{s->nin, s->h1, s->h2};
   hipLaunchKernelGGL(k_mlp_forward, dim3((batch + FWD_NT - 1) / FWD_NT), dim3(FWD_NT), 0,
                      internal::stream_of(ctx), sh, params_d, x_d, batch, y_d, acts_d);
@@ -223,6 +244,9 @@ int fmcs_mlp_backward(fmcs_ctx* ctx, const fmcs_mlp_shape* s, const float* param
                       float* loss_d) {
   if (!ctx || !shape_ok(s) || !params_d || !x_d || !acts_d || !y_d || !target_d || !grad_d || batch == 0)
     return FMCS_E_ARG;
+  const int crc = check_bufs(ctx, {{params_d, "params_d"}, {x_d, "x_d"}, {acts_d, "acts_d"}, {y_d, "y_d"},
+                                   {target_d, "target_d"}, {grad_d, "grad_d"}, {loss_d, "loss_d"}});
+  if (crc != FMCS_OK) return crc;
   Shape sh{s->nin, s->h1, s->h2};
   const uint32_t width = sh.count() + 1;
   const uint32_t nblk = (batch + BWD_SB - 1) / BWD_SB;
@@ -247,6 +271,8 @@ int fmcs_mlp_adam(fmcs_ctx* ctx, uint32_t nparams, float* params_d, const float*
                   float lr, float beta1, float beta2, float eps, uint32_t step) {
   if (!ctx || !params_d || !grad_d || !m_d || !v_d || step == 0) return FMCS_E_ARG;
   if (nparams == 0) return FMCS_OK;
+  const int crc = check_bufs(ctx, {{params_d, "params_d"}, {grad_d, "grad_d"}, {m_d, "m_d"}, {v_d, "v_d"}});
+  if (crc != FMCS_OK) return crc;
   float c1 = 1.f - powf(beta1, (float)step), c2 = 1.f - powf(beta2, (float)step);
   hipLaunchKernelGGL(k_mlp_adam, dim3((nparams + 255) / 256), dim3(256), 0, internal::stream_of(ctx), params_d,
                      grad_d, m_d, v_d, nparams, lr, beta1, beta2, eps, c1, c2);

@@ -70,8 +70,13 @@ extern "C" int fmcs_sort_pairs_noalloc(fmcs_ctx* ctx, uint32_t bank, uint32_t ke
     return set_error(ctx, FMCS_E_ARG, "fmcs_sort_pairs_noalloc: n exceeds FMCS_NOALLOC_MAX_ROWS (1048576)");
   if (n && (!keys_d || !oids_d)) return set_error(ctx, FMCS_E_ARG, "null keys or oids");
   if (n == 0) return FMCS_OK;
+  // checked, never staged: staging would allocate
+  Stager st(ctx, false);
+  int rc = st.arg(keys_d, n * bank, Stager::INOUT, "keys_d");
+  if (rc == FMCS_OK) rc = st.arg(oids_d, n * 4, Stager::INOUT, "oids_d");
+  if (rc != FMCS_OK) return rc;
   void* p;
-  const int rc = scratch_of(ctx, &p);
+  rc = scratch_of(ctx, &p);
   if (rc != FMCS_OK) return rc;
   return sort_pairs_in_scratch(ctx, bank, keybits, n, keys_d, oids_d, p, FMCS_NOALLOC_SCRATCH_BYTES);
 }

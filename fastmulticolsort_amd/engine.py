@@ -82,6 +82,16 @@ class Context:
     def sync(self):
         self._chk(self.lib.fmcs_ctx_sync(self.h))
 
+    def pointer_kind(self, ptr):
+        """_lib.MEM_* kind of the memory at address ptr (an int)."""
+        k = C.c_uint32(0)
+        self._chk(self.lib.fmcs_pointer_kind(self.h, C.c_void_p(ptr), C.byref(k)))
+        return k.value
+
+    def set_host_staging(self, on):
+        """Stage pageable host buffers through device memory (else they are rejected)."""
+        self._chk(self.lib.fmcs_ctx_set_host_staging(self.h, int(bool(on))))
+
     def set_stream(self, stream):
         self._chk(self.lib.fmcs_ctx_set_stream(self.h, C.c_void_p(stream)))
 

@@ -266,19 +266,37 @@ def request_columns(req):
     return cols
 
 
-def measure_plans(ctx, req, plans, reps=2):
-    """GPU time (ms, HIP events) of every plan of a request."""
+class PlanMismatch(AssertionError):
+    """A plan's order differs from the lexicographic order of the columns."""
+
+
+def expected_order(host_cols):
+    """ORDER BY the columns with the oid tie-break (numpy's stable lexsort)."""
+    return np.lexsort(tuple(reversed(host_cols))).astype(np.uint32)
+
+
+def measure_plans(ctx, req, plans, reps=2, verify=True, mode=None):
+    """GPU time (ms, HIP events) of every plan of a request, in the
+    device-driven mode the request queue runs (mode None).  verify: every
+    plan's order is checked against numpy's lexsort of the request's columns
+    (PlanMismatch on a difference); the plan never changes the result."""
     import torch
-    cols = [torch.from_numpy(c).cuda() for c in request_columns(req)]
+    from .engine import MODE_DEVICE
+    mode = MODE_DEVICE if mode is None else mode
+    host = request_columns(req)
+    cols = [torch.from_numpy(c).cuda() for c in host]
     order = torch.empty(req["n"], dtype=torch.uint32, device="cuda")
     order_cols = list(range(len(req["bws"])))
+    exp = expected_order(host) if verify else None
     out = []
-    planner.run_plan(ctx, cols, req["bws"], order_cols, plans[0], order=order)  # warm-up
+    planner.run_plan(ctx, cols, req["bws"], order_cols, plans[0], mode=mode, order=order)  # warm-up
     for p in plans:
         best = float("inf")
         for _ in range(reps):
-            _, _, tot, _ = planner.run_plan(ctx, cols, req["bws"], order_cols, p, order=order)
+            _, _, tot, _ = planner.run_plan(ctx, cols, req["bws"], order_cols, p, mode=mode, order=order)
             best = min(best, tot)
+        if verify and not np.array_equal(order.cpu().numpy(), exp):
+            raise PlanMismatch(f"plan {p} of request n={req['n']} bws={req['bws']}: order differs from lexsort")
         out.append(best)
     return out
 

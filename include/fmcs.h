@@ -4,8 +4,11 @@
  * The reference (FastMultiColSort) is a C++ class library with no C ABI;
  * each entry point below replaces one of its hot-path seams, cited as
  * file:line in the reference tree.  Conventions:
- *   - every array argument named *_d is DEVICE memory owned by the caller;
- *     host arrays (lists of pointers, widths, entries) are plain host memory;
+ *   - every array argument named *_d is owned by the caller and is memory
+ *     the GPU can read (device, pinned or managed); pageable host memory is
+ *     rejected with FMCS_E_ARG before anything runs, unless the context
+ *     stages host buffers (fmcs_ctx_set_host_staging, below); host arrays
+ *     (lists of pointers, widths, entries) are plain host memory;
  *   - results go to caller-named outputs; nothing is swapped or freed;
  *   - calls are ordered on the context's stream; functions that return a
  *     count to the host synchronize that stream;
@@ -93,6 +96,34 @@ int fmcs_ctx_get_tuning(fmcs_ctx* ctx, fmcs_tuning* t);
 /* Frees the context's cached workspace. */
 int fmcs_ctx_trim(fmcs_ctx* ctx);
 const char* fmcs_last_error(fmcs_ctx* ctx);
+
+/* ---- buffer arguments ------------------------------------------------- */
+
+/* Every entry point classifies its non-null *_d pointers before it launches
+ * anything.  Device memory of the context's GPU, pinned host memory
+ * (hipHostMalloc) and managed memory are used in place.  Pageable host
+ * memory -- the reference's malloc_aligned buffers (src/common.h:19-31) --
+ * is rejected with FMCS_E_ARG, unless the context stages host buffers:
+ * then each such argument is copied into a device buffer of the context
+ * before the call's work and, if it is an output, copied back and the
+ * stream synchronized before the call returns.  Another GPU's memory is
+ * rejected with FMCS_E_ARG. */
+#define FMCS_MEM_DEVICE 0  /* device memory of the context's GPU */
+#define FMCS_MEM_PINNED 1  /* host memory mapped for the GPU (hipHostMalloc) */
+#define FMCS_MEM_MANAGED 2 /* managed / unified memory */
+#define FMCS_MEM_HOST 3    /* pageable host memory (malloc): the GPU cannot read it */
+#define FMCS_MEM_OTHER 4   /* another GPU's memory, or an array */
+
+/* The classification itself, from what hipPointerGetAttributes reported:
+ * query_status (its hipError_t; non-zero means the runtime does not know
+ * the pointer, i.e. pageable host memory), memory_type (hipMemoryType),
+ * owner_device (the attribute's device) and the context's device. */
+uint32_t fmcs_classify_memory(int query_status, int memory_type, int owner_device, int ctx_device);
+/* The kind of the memory p points into, for this context's device. */
+int fmcs_pointer_kind(fmcs_ctx* ctx, const void* p, uint32_t* kind);
+/* on != 0: stage pageable host buffers through device memory (off by
+ * default).  Results equal those of the same call on device buffers. */
+int fmcs_ctx_set_host_staging(fmcs_ctx* ctx, int on);
 
 /* ---- host-side plan helpers ----------------------------------------- */
 

@@ -317,9 +317,6 @@ uint32_t orc_multi_round_sort(uint32_t nrounds, const void *const *cols, const u
 
 /* ---- a10/a11/a12: hash grouping --------------------------------------- */
 
-/* mix_int / mix_lng (src/hashing.h:21-24). */
-uint32_t orc_mix_int(uint32_t x) { return (x >> 7) ^ (x >> 13) ^ (x >> 21) ^ x; }
-uint32_t orc_mix_lng(uint64_t x) { return orc_mix_int((uint32_t)(x ^ (x >> 32))); }
 
 /* Hashing::HASHmask (src/hashing.h:442-465). */
 uint64_t orc_hashmask(uint64_t cnt)

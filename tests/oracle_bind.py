@@ -41,10 +41,6 @@ def lib():
         _lib.orc_multi_round_hash.restype = C.c_uint32
         _lib.orc_hashmask.restype = C.c_uint64
         _lib.orc_hashmask.argtypes = [C.c_uint64]
-        _lib.orc_mix_int.restype = C.c_uint32
-        _lib.orc_mix_int.argtypes = [C.c_uint32]
-        _lib.orc_mix_lng.restype = C.c_uint32
-        _lib.orc_mix_lng.argtypes = [C.c_uint64]
         _lib.orc_split_exhaustive.restype = C.c_uint64
     return _lib
 
@@ -161,14 +157,6 @@ def multi_round_hash(cols):
 
 def hashmask(cnt):
     return int(lib().orc_hashmask(C.c_uint64(cnt)))
-
-
-def mix_int(x):
-    return int(lib().orc_mix_int(C.c_uint32(x)))
-
-
-def mix_lng(x):
-    return int(lib().orc_mix_lng(C.c_uint64(x)))
 
 
 def split_exhaustive(S, rounds, maxv=64, cap=200000):

@@ -46,3 +46,37 @@ def test_construct_entries_rejects_bad_split():
         construct_entries([17, 33], [0, 1], [20, 31])
     with pytest.raises(Exception):
         construct_entries([17, 33], [0, 0], [20, 30])
+
+
+def _hip_memory_types():
+    """hipMemoryType's values, read from the installed HIP header."""
+    txt = open("/opt/rocm/include/hip/hip_runtime_api.h").read()
+    body = re.search(r"typedef enum hipMemoryType \{(.*?)\} hipMemoryType;", txt, flags=re.S).group(1)
+    return {m.group(1): int(m.group(2)) for m in re.finditer(r"(hipMemoryType\w+)\s*=\s*(\d+)", body)}
+
+
+def test_memory_classification():
+    # the guard every entry point applies to its *_d pointers, as a pure
+    # function of what hipPointerGetAttributes reports (no GPU needed)
+    t = _hip_memory_types()
+    f = _lib.load().fmcs_classify_memory
+    ctx_dev = 0
+    assert f(1, 0, 0, ctx_dev) == _lib.MEM_HOST  # the runtime does not know it: pageable host memory
+    assert f(0, t["hipMemoryTypeUnregistered"], 0, ctx_dev) == _lib.MEM_HOST
+    assert f(0, t["hipMemoryTypeHost"], 0, ctx_dev) == _lib.MEM_PINNED
+    assert f(0, t["hipMemoryTypeHost"], 3, ctx_dev) == _lib.MEM_PINNED  # pinned memory serves every GPU
+    assert f(0, t["hipMemoryTypeDevice"], 0, ctx_dev) == _lib.MEM_DEVICE
+    assert f(0, t["hipMemoryTypeDevice"], 1, ctx_dev) == _lib.MEM_OTHER  # another GPU's memory
+    assert f(0, t["hipMemoryTypeDevice"], 1, 1) == _lib.MEM_DEVICE
+    assert f(0, t["hipMemoryTypeManaged"], 0, ctx_dev) == _lib.MEM_MANAGED
+    assert f(0, t["hipMemoryTypeUnified"], 0, ctx_dev) == _lib.MEM_MANAGED
+    assert f(0, t["hipMemoryTypeArray"], 0, ctx_dev) == _lib.MEM_OTHER
+
+
+def test_entry_points_refuse_a_null_context():
+    import ctypes as C
+    lib = _lib.load()
+    k = C.c_uint32(0)
+    assert lib.fmcs_pointer_kind(None, None, C.byref(k)) == _lib.FMCS_E_ARG
+    assert lib.fmcs_ctx_set_host_staging(None, 1) == _lib.FMCS_E_ARG
+    assert lib.fmcs_sort_pairs_noalloc(None, 4, 10, 0, None, None) == _lib.FMCS_E_ARG

@@ -156,15 +156,6 @@ def test_hashmask_closed_form():
         assert (m == 2 * lo) == (lo + lo - cnt < 2 * (cnt - lo))
 
 
-def test_mix_functions():
-    # src/hashing.h:21-24
-    for x in [0, 1, 0x12345678, 0xFFFFFFFF]:
-        assert ob.mix_int(x) == ((x >> 7) ^ (x >> 13) ^ (x >> 21) ^ x) & 0xFFFFFFFF
-    x = 0x123456789ABCDEF0
-    f = (x ^ (x >> 32)) & 0xFFFFFFFF
-    assert ob.mix_lng(x) == ((f >> 7) ^ (f >> 13) ^ (f >> 21) ^ f)
-
-
 @pytest.mark.parametrize("S,r", [(42, 1), (42, 2), (42, 3), (42, 4), (50, 2), (10, 5)])
 def test_split_enumeration_counts(S, r):
     cnt, plans = ob.split_exhaustive(S, r, 64)
