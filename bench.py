@@ -76,13 +76,45 @@ def check_sorted(c1, c2, order):
     return not (tie & (o[1:] < o[:-1])).any()
 
 
-def cpu_baseline(c1, c2, sample_rows):
+def cpu_info():
+    """CPU model and core counts of the host the baseline runs on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        share = len(os.sched_getaffinity(0))
+    except Exception:
+        share = None
+    return {"cpu_model": model, "nproc": os.cpu_count(), "cores_in_affinity": share}
+
+
+WARMUPS, RUNS = 5, 5  # the reference's practice (src/stitch_composer.cpp:4214-4223), median of 5
+
+
+def cpu_baseline(c1, c2, sample_rows, full=True):
     """The reference's own AVX2 round loop (oracle/_ref, built from the
-    reference sources) on the first `sample_rows` rows, one pinned thread.
-    Falls back to the C restatement (oracle) where the reference build is
-    absent."""
+    reference sources), one pinned thread: 5 warm-ups and the median of 5
+    runs on the first `sample_rows` rows, and one run on all the rows
+    beside it.  Falls back to the C restatement (oracle) where the
+    reference build is absent."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as ob
+    kind = "reference" if ob.ref() is not None else "port"
+
+    def run(a, b):
+        t0 = time.perf_counter()
+        if kind == "reference":
+            ob.ref_sort_rounds([a, b], [20, 40])
+        else:
+            ob.multi_round_sort([a, b])
+        return time.perf_counter() - t0
+
     s1 = np.ascontiguousarray(c1[:sample_rows])
     s2 = np.ascontiguousarray(c2[:sample_rows])
     try:
@@ -90,23 +122,22 @@ def cpu_baseline(c1, c2, sample_rows):
         os.sched_setaffinity(0, {sorted(old)[0]})
     except Exception:
         old = None
-    kind = "reference" if ob.ref() is not None else "port"
-    times = []
     try:
-        for it in range(3):
-            t0 = time.perf_counter()
-            if kind == "reference":
-                ob.ref_sort_rounds([s1, s2], [20, 40])
-            else:
-                ob.multi_round_sort([s1, s2])
-            times.append(time.perf_counter() - t0)
+        for _ in range(WARMUPS):
+            run(s1, s2)
+        t = float(np.median([run(s1, s2) for _ in range(RUNS)]))
+        t_full = run(np.ascontiguousarray(c1), np.ascontiguousarray(c2)) if full and len(c1) > sample_rows else None
     finally:
         if old is not None:
             os.sched_setaffinity(0, old)
-    t = float(np.median(times[1:]))
-    return {"value": sample_rows / t / 1e6, "unit": "Mrows/s", "cores": 1, "kind": kind,
-            "sample": f"first {sample_rows} rows of the same c1/c2 columns, 2 rounds (c1 u32, c2 u64), "
-                      f"1 warm-up + median of 2, one pinned thread"}
+    out = {"value": sample_rows / t / 1e6, "unit": "Mrows/s", "cores": 1, "kind": kind,
+           "sample": f"first {sample_rows} rows of the same c1/c2 columns, 2 rounds (c1 u32, c2 u64), "
+                     f"{WARMUPS} warm-ups + median of {RUNS}, one pinned thread"}
+    if t_full is not None:
+        out["full_size"] = {"rows": len(c1), "value": len(c1) / t_full / 1e6, "unit": "Mrows/s",
+                            "runs": 1, "seconds": round(t_full, 2)}
+    out.update(cpu_info())
+    return out
 
 
 def cpu_baseline_all_cores(c1, c2, sample_rows, max_threads=16):
@@ -125,15 +156,15 @@ def cpu_baseline_all_cores(c1, c2, sample_rows, max_threads=16):
     s1 = np.ascontiguousarray(c1[:sample_rows])
     s2 = np.ascontiguousarray(c2[:sample_rows])
     times = []
-    for it in range(4):
+    for it in range(WARMUPS + RUNS):
         t0 = time.perf_counter()
         ob.ref_sort_rounds_mt([s1, s2], [20, 40], nt)
         times.append(time.perf_counter() - t0)
-    t = float(np.median(times[1:]))
+    t = float(np.median(times[WARMUPS:]))
     return {"value": sample_rows / t / 1e6, "unit": "Mrows/s", "cores": nt, "kind": "reference",
             "sample": f"first {sample_rows} rows of the same c1/c2 columns, 2 rounds, {nt} threads "
                       f"(radix partition + the reference's merge sort per part, groups over threads), "
-                      f"1 warm-up + median of 3"}
+                      f"{WARMUPS} warm-ups + median of {RUNS}"}
 
 
 def main():
@@ -148,6 +179,7 @@ def main():
     ap.add_argument("--no-comparators", action="store_true",
                     help="skip the host-synchronised batched and unbinned runs (a profile of the timed kernels alone)")
     ap.add_argument("--cpu-sample-log2", type=int, default=23)
+    ap.add_argument("--no-cpu-full", action="store_true", help="skip the CPU baseline's full-size run")
     ap.add_argument("--autotune", action="store_true",
                     help="sweep the batched sort's bin thresholds and tile shapes first and save the best")
     ap.add_argument("--mode", choices=["graph", "device", "batched"], default="graph")
@@ -226,11 +258,15 @@ def main():
     if not ok:
         raise SystemExit("profiled pass failed the sortedness check")
 
-    # dominant kernel and its roofline (algorithmic bytes / avg duration)
+    # dominant kernel and its roofline: algorithmic bytes per launch (for a
+    # radix digit pass 2(w + 4) per row: key and oid read and written) over
+    # the average launch; side outputs that later kernels of this design read
+    # (a next-digit byte, the sorted c1 column) are reported apart
     dom = max(stats.items(), key=lambda kv: kv[1]["ms"] if kv[1]["bytes"] > 0 else 0.0)
     name, st = dom
     avg_ms = st["ms"] / st["launches"]
     bytes_per_launch = st["bytes"] / st["launches"]
+    extra_per_launch = st.get("extra_bytes", 0.0) / st["launches"]
     achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
     kernels = {k: {"launches": v["launches"], "ms_per_step": v["ms"] / args.steps,
                    "GBs": (v["bytes"] / (v["ms"] / 1e3) / 1e9) if v["ms"] > 0 else None}
@@ -270,7 +306,7 @@ def main():
 
     cpu = cpu_all = None
     if not args.no_cpu:
-        cpu = cpu_baseline(c1, c2, min(n, 1 << args.cpu_sample_log2))
+        cpu = cpu_baseline(c1, c2, min(n, 1 << args.cpu_sample_log2), full=not args.no_cpu_full)
         log(f"cpu baseline: {cpu}")
         cpu_all = cpu_baseline_all_cores(c1, c2, min(n, 1 << args.cpu_sample_log2))
         log(f"cpu baseline, all cores: {cpu_all}")
@@ -293,7 +329,10 @@ def main():
                    "rows": n, "groups": ng},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name),
-                     "avg_launch_ms": round(avg_ms, 5), "bytes_per_launch": bytes_per_launch},
+                     "avg_launch_ms": round(avg_ms, 5), "bytes_per_launch": bytes_per_launch,
+                     "extra_bytes_per_launch": extra_per_launch,
+                     "frac_with_extra": round((bytes_per_launch + extra_per_launch) / (avg_ms / 1e3) / 1e9
+                                              / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
         "comparators": {"plain_per_round_ms": plain_ms, "unbinned_ms": round(unb_ms, 3) if unb_ms else None,

@@ -56,11 +56,14 @@ struct fmcs_ctx {
     double bytes;
     const u32* dcount;  // if set: bytes += per_unit * (*dcount), read when the records are flushed
     double per_unit;
+    double extra;       // side outputs a later kernel of this design reads (not the pass's own bytes)
   };
   struct Agg {
     u64 launches = 0;
-    double ms = 0, bytes = 0;
+    double ms = 0, bytes = 0, extra = 0;
   };
+  // extra bytes of the next profiled launch (set just before it)
+  double extra_next = 0;
   bool prof = false;
   std::vector<Rec> recs;
   std::vector<hipEvent_t> pool;
@@ -260,6 +263,7 @@ int prof_flush(fmcs_ctx* ctx) {
       a.launches++;
       a.ms += ms;
       a.bytes += bytes;
+      a.extra += r.extra;
     }
     if (r.e0) ctx->pool.push_back(r.e0);
     if (r.e1) ctx->pool.push_back(r.e1);
@@ -273,9 +277,11 @@ struct Prof {
   fmcs_ctx* ctx;
   int idx = -1;
   Prof(fmcs_ctx* c, const char* name, double bytes, const u32* dcount = nullptr, double per_unit = 0) : ctx(c) {
+    const double extra = ctx->extra_next;
+    ctx->extra_next = 0;
     if (!ctx->prof || ctx->capturing || ctx->arena) return;
     if (ctx->recs.size() > 8192) prof_flush(ctx);
-    fmcs_ctx::Rec r{name, pool_get(ctx), pool_get(ctx), bytes, dcount, per_unit};
+    fmcs_ctx::Rec r{name, pool_get(ctx), pool_get(ctx), bytes, dcount, per_unit, extra};
     if (r.e0) (void)hipEventRecord(r.e0, ctx->stream);
     ctx->recs.push_back(r);
     idx = (int)ctx->recs.size() - 1;
@@ -321,6 +327,8 @@ int scatter_pass(fmcs_ctx* ctx, const char* name, const K* kin, const u32* vin, 
                  u32* err = nullptr, u32* kside = nullptr, uint8_t* dnext = nullptr, u32 nshift = 0,
                  const u32* tseg = nullptr) {
   constexpr u32 R = 1u << RB;
+  const double extra = ctx->extra_next;  // belongs to the scatter launch, not the scans before it
+  ctx->extra_next = 0;
   const u64 nch = std::max<u64>(1, (ntiles_max + TSCAN_CT - 1) / TSCAN_CT);
   u32* carry;
   TRY(getbuf_t(ctx, WS_CARRY, nch * (R + 1), &carry));
@@ -329,6 +337,7 @@ int scatter_pass(fmcs_ctx* ctx, const char* name, const K* kin, const u32* vin, 
               ntiles_d, carry, cflag, tseg));
   TRY(klaunch(ctx, "k_tile_scan", kd_tscan_carry<RB>, R, 0.0, nullptr, 0.0, carry, (const u32*)cflag, ntiles_d));
   auto kern = k_onesweep<K, ONESWEEP_ITEMS, RB, P, FUSE, LSD>;
+  ctx->extra_next = extra;
   return klaunch(ctx, name, kern, onesweep_grid(ctx, (const void*)kern, ntiles_max), bytes, dcount, per, kin, vin,
                  kout, vout, lo, hi, tbase, 0u, dbase, (const u32*)tcount, (const u32*)carry, shift, dbits, nsegs_d,
                  ntiles_d, pin, pout, fshift, pbad, err, kside, dnext, nshift, tseg);
@@ -1054,16 +1063,21 @@ int radix_sort_dd_fused(fmcs_ctx* ctx, const K* kin, const u64* pin, u32 keybits
     TRY(klaunch(ctx, "k_digit_base", kd_digit_base<RB>, 1, 0.0, nullptr, 0.0, (const u32*)hp, (const u32*)cst,
                 dbase + (u64)p * R, (u32*)nullptr, (u32*)nullptr, one, (u32*)nullptr));
     if (p == 0) {
-      // the next digit: c1's bits RB .. 2 RB - 1
+      // the next digit: c1's bits RB .. 2 RB - 1 (a byte a row beside the
+      // pass's own bytes: c1 and c2 read, composite and oid written)
+      ctx->extra_next = last ? 0.0 : 1.0 * n;
       TRY((scatter_pass<K, RB, u64, true, true>(ctx, "k_onesweep_fuse", kin, (const u32*)nullptr, (K*)nullptr, dv, cst, cst + 1,
                                           cst + 2, one, cst + 3, ntiles, dbase, tcount, 0u, dbits,
-                                          (double)n * (sizeof(K) + 8 + 8 + 4 + (last ? 0 : 1)), nullptr, 0.0, pin, dc,
+                                          (double)n * (sizeof(K) + 8 + 8 + 4), nullptr, 0.0, pin, dc,
                                           pbits, pbad, dd + DD_ERR, (u32*)nullptr, last ? (uint8_t*)nullptr : dnx,
                                           RB)));
     } else {
+      // 2(w + 4) a row for the composite pass; the next digit (a byte a
+      // row) or, on the last pass, the sorted c1 (4 bytes a row) besides
+      ctx->extra_next = last ? 4.0 * n : 1.0 * n;
       TRY((scatter_pass<u64, RB, NoPay, false, true>)(ctx, "k_onesweep", sc, sv, dc, dv, cst, cst + 1, cst + 2, one, cst + 3, ntiles,
                             dbase + (u64)p * R, tcount, pbits + p * RB, dbits,
-                            2.0 * n * (8 + 4) + (last ? 4.0 * n : 1.0 * n), nullptr, 0.0, (const NoPay*)nullptr,
+                            2.0 * n * (8 + 4), nullptr, 0.0, (const NoPay*)nullptr,
                             (NoPay*)nullptr, pbits, 0ull, (u32*)nullptr, last ? side : (u32*)nullptr,
                             last ? (uint8_t*)nullptr : dnx, pbits + (p + 1) * RB));
     }
@@ -2168,9 +2182,10 @@ int fmcs_kernel_stats(fmcs_ctx* ctx, char* buf, uint64_t cap) {
   bool first = true;
   for (auto& kv : ctx->agg) {
     char line[256];
-    std::snprintf(line, sizeof(line), "%s\"%s\": {\"launches\": %llu, \"ms\": %.6f, \"bytes\": %.0f}",
+    std::snprintf(line, sizeof(line),
+                  "%s\"%s\": {\"launches\": %llu, \"ms\": %.6f, \"bytes\": %.0f, \"extra_bytes\": %.0f}",
                   first ? "" : ", ", kv.first.c_str(), (unsigned long long)kv.second.launches, kv.second.ms,
-                  kv.second.bytes);
+                  kv.second.bytes, kv.second.extra);
     js += line;
     first = false;
   }

@@ -3,7 +3,8 @@
     python -m fastmulticolsort_amd.rocprof_summary stats DIR CMD > kernel_stats.md
     python -m fastmulticolsort_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR CMD_F CMD_W > traffic.json
 
-`stats` reads DIR/*kernel_stats.csv (--kernel-trace --stats) and prints a
+`stats` reads DIR/*kernel_stats.csv (--kernel-trace --stats), or the
+rocpd database DIR/*.db that rocprofv3 writes by default, and prints a
 markdown table of calls / total / average per kernel.  `traffic` reads the
 per-dispatch FETCH_SIZE and WRITE_SIZE counters (KiB) of two --pmc runs and
 scales each by what the same counter reads for the plain device copy the
@@ -15,6 +16,7 @@ import glob
 import json
 import os
 import re
+import sqlite3
 import sys
 from collections import defaultdict
 
@@ -34,19 +36,48 @@ def _one(pattern):
     return files[0]
 
 
+def _db(d):
+    files = sorted(glob.glob(os.path.join(d, "*.db")))
+    return sqlite3.connect(files[0]) if files else None
+
+
+def kernel_rows(d):
+    """(name, calls, total ns, average ns, percent) per kernel, CSV or rocpd."""
+    csvs = sorted(glob.glob(os.path.join(d, "*kernel_stats.csv")))
+    if csvs:
+        return [(r["Name"], int(r["Calls"]), float(r["TotalDurationNs"]), float(r["AverageNs"]),
+                 float(r["Percentage"])) for r in csv.DictReader(open(csvs[0]))]
+    db = _db(d)
+    if db is None:
+        raise SystemExit(f"no kernel statistics under {d}")
+    # the database's durations are nanoseconds per dispatch
+    rows = db.execute("select name, count(*), sum(duration), avg(duration) from kernels group by name "
+                      "order by sum(duration) desc").fetchall()
+    tot = sum(r[2] for r in rows) or 1.0
+    return [(n, c, float(t), float(a), 100.0 * t / tot) for n, c, t, a in rows]
+
+
 def stats(d, cmd):
-    rows = list(csv.DictReader(open(_one(os.path.join(d, "*kernel_stats.csv")))))
     out = [f"Command: `{cmd}`", "", "| kernel | calls | total ms | avg us | % of GPU time |", "|---|---|---|---|---|"]
-    for r in rows:
-        out.append(f"| {short(r['Name'])} | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
-                   f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['Percentage']):.2f} |")
+    for name, calls, tot, avg, pct in kernel_rows(d):
+        out.append(f"| {short(name)} | {calls} | {tot / 1e6:.3f} | {avg / 1e3:.1f} | {pct:.2f} |")
     return "\n".join(out)
 
 
 def _pmc(d):
     per = defaultdict(list)
-    for r in csv.DictReader(open(_one(os.path.join(d, "*counter_collection.csv")))):
-        per[short(r["Kernel_Name"])].append((int(r["Grid_Size"]), float(r["Counter_Value"])))
+    csvs = sorted(glob.glob(os.path.join(d, "*counter_collection.csv")))
+    if csvs:
+        for r in csv.DictReader(open(csvs[0])):
+            per[short(r["Kernel_Name"])].append((int(r["Grid_Size"]), float(r["Counter_Value"])))
+        return per
+    db = _db(d)
+    if db is None:
+        raise SystemExit(f"no counter collection under {d}")
+    # one row per dispatch and counter; a counter summed over its instances
+    for name, grid, val in db.execute("select kernel_name, grid_size, sum(value) from counters_collection "
+                                      "group by dispatch_id, counter_name"):
+        per[short(name)].append((int(grid), float(val)))
     return per
 
 
