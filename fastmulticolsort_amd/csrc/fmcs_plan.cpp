@@ -9,11 +9,14 @@
 #include <limits>
 #include <numeric>
 #include <random>
+#include <functional>
 #include <set>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/fmcs.h"
 #include "../../include/fmcs_plan.h"
+#include "fmcs_rrs.h"
 
 namespace {
 
@@ -39,13 +42,26 @@ double radix_ms(const fmcs_cost_model* m, int bt, uint32_t bits, double n) {
   return n * passes * m->radix_pass_ns[bt] * 1e-6;
 }
 
-// StitchComposer::calculate_single_column_cost (src/stitch_composer.cpp:
-// 2186-2226): (n/g) groups times the sort cost of one group, scaled by
-// f(g) = 1/(1 + 1/(g-1)) (groups of one row need no sort).  With per-row
-// GPU costs this is n * f(g) * per_row(g).
+// The bank type (0: u16, 1: u32, 2: u64) of a new column's width.
+// calculate_single_column_cost (src/stitch_composer.cpp:2186-2226,
+// src/plan_enum_rrs.cpp:7-35): (n/g) groups times the cost of sorting one
+// group of g rows, scaled by f(g) = 1/(1 + 1/(g-1)) (a group of one row
+// needs no sort).  `one_group` prices one group.
+template <class F>
+double column_cost(double n, double g, F one_group) {
+  const double group_num = n / g;
+  const double factor = (g - 1.0) < 0.000001 ? 0.0 : 1.0 / (1.0 + 1.0 / (g - 1.0));
+  return group_num * factor * one_group(g);
+}
+
+// The same with per-row GPU times: one group of g rows costs g * per_row(g).
 double single_column_cost(const fmcs_cost_model* m, double n, double g, int bt) {
-  double f = (g - 1.0) < 1e-6 ? 0.0 : 1.0 / (1.0 + 1.0 / (g - 1.0));
-  return n * f * segsort_ns(m, bt, g) * 1e-6;
+  return column_cost(n, g, [&](double x) { return x * segsort_ns(m, bt, x) * 1e-6; });
+}
+
+// ... and with caller-supplied primitives (the reference's sort estimators).
+double single_column_cost_prim(const fmcs_cost_primitives* p, double n, double g, int bt) {
+  return column_cost(n, g, [&](double x) { return p->sort_cost(p->user, x, (uint32_t)bt); });
 }
 
 struct Stats {
@@ -53,16 +69,30 @@ struct Stats {
   uint32_t shift[3];        // entries per destination bank
 };
 
+// The conversion of a double to uint64 that the reference's cardinalities
+// go through (`uint64 *= double`, src/plan_enum_rrs.cpp:393) on x86-64:
+// truncation below 2^63, exact in [2^63, 2^64), 0 from 2^64 up and 2^63
+// for NaN (cvttsd2si with its out-of-range value, and the 2^63 offset).
+uint64_t u64_of(double v) {
+  if (std::isnan(v)) return 1ull << 63;
+  if (v >= 18446744073709551616.0) return 0;
+  if (v >= 9223372036854775808.0) return (uint64_t)(int64_t)(v - 9223372036854775808.0) ^ (1ull << 63);
+  if (v < -9223372036854775808.0) return 1ull << 63;
+  return (uint64_t)(int64_t)v;
+}
+
 // getCardinalityFromStatistics (src/stitch_composer.cpp:288-553) and
 // PlanEnumRRS::GetStatistics (src/plan_enum_rrs.cpp:269-396): walk the
 // construct entries of the split, count assemble/shift operations by bank,
-// and multiply card_per_bit^width into each new column's cardinality.
-// truncate=true reproduces the uint64 cardinalities of the RRS variant.
+// and multiply card_per_bit^width into each new column's cardinality (all
+// but the last).  Card is double (the optimizer's variant) or uint64_t (the
+// RRS variant, converted after every entry).
+template <class Card>
 void plan_stats(uint32_t ncols, const uint32_t* bw, const uint64_t* card, const uint32_t* order, const Split& split,
-                std::vector<double>& cardinality, Stats& st, bool truncate) {
+                std::vector<Card>& cardinality, Stats& st) {
   std::memset(&st, 0, sizeof(st));
   const size_t nn = split.size();
-  cardinality.assign(nn, 1.0);
+  cardinality.assign(nn, (Card)1);
   std::vector<double> cpb(ncols);
   for (uint32_t c = 0; c < ncols; ++c) cpb[c] = std::pow((double)card[c], 1.0 / (double)bw[c]);
   uint32_t oi = 0, ni = 0;
@@ -77,8 +107,11 @@ void plan_stats(uint32_t ncols, const uint32_t* bw, const uint64_t* card, const 
     st.shift[to]++;
     st.assemble[from][to]++;
     if (ni + 1 < nn) {
-      double v = cardinality[ni] * std::pow(cpb[src], (double)m);
-      cardinality[ni] = truncate ? (double)(uint64_t)v : v;
+      const double v = (double)cardinality[ni] * std::pow(cpb[src], (double)m);
+      if constexpr (std::is_same<Card, uint64_t>::value)
+        cardinality[ni] = u64_of(v);
+      else
+        cardinality[ni] = v;
     }
     if (ow == 0) {
       ++oi;
@@ -91,22 +124,61 @@ void plan_stats(uint32_t ncols, const uint32_t* bw, const uint64_t* card, const 
   }
 }
 
+// Groups before each round from the new columns' cardinalities: G_0 = 1,
+// G_r = card_0 * ... * card_{r-1} (src/stitch_composer.cpp:2306-2318; in
+// uint64 with wrap-around in the RRS variant, src/plan_enum_rrs.cpp:149-156).
+template <class Card>
+std::vector<double> round_groups(const std::vector<Card>& cardinality) {
+  std::vector<double> G(cardinality.size());
+  Card acc = cardinality.empty() ? (Card)1 : cardinality[0];
+  if (!G.empty()) G[0] = 1.0;
+  for (size_t r = 1; r < cardinality.size(); ++r) {
+    G[r] = (double)acc;
+    acc *= cardinality[r];
+  }
+  return G;
+}
+
+std::vector<double> plan_groups(uint32_t ncols, const uint32_t* bw, const uint64_t* card, const uint32_t* order,
+                                const Split& split, bool truncate, Stats& st) {
+  if (truncate) {
+    std::vector<uint64_t> cu;
+    plan_stats(ncols, bw, card, order, split, cu, st);
+    return round_groups(cu);
+  }
+  std::vector<double> cd;
+  plan_stats(ncols, bw, card, order, split, cd, st);
+  return round_groups(cd);
+}
+
 // estimate_plan_collection_greedy's per-plan costing (src/stitch_composer.cpp:
-// 2228-2378) with GPU terms: stitch, tuple reconstruction (gather), multi-pass
-// sort and group extraction.
+// 2228-2378) and PlanEnumRRS::GetCost (src/plan_enum_rrs.cpp:94-197) with
+// caller-supplied primitives: stitch + tuple reconstruction + (group
+// extraction, 0 there) + the per-round sorts, in that order.
+double estimate_prim(const fmcs_cost_primitives* p, double n, uint32_t ncols, const uint32_t* bw,
+                     const uint64_t* card, const uint32_t* order, const Split& split, bool truncate) {
+  Stats st;
+  const std::vector<double> G = plan_groups(ncols, bw, card, order, split, truncate, st);
+  const size_t R = split.size();
+  std::vector<uint32_t> bytes(R);
+  double sort = 0.0;
+  for (size_t r = 0; r < R; ++r) {
+    bytes[r] = fmcs_bank_bytes(split[r], 0);
+    sort += single_column_cost_prim(p, n, n / G[r], bank_type(split[r]));
+  }
+  const double stitch = p->stitch_cost(p->user, &st.assemble[0][0], st.shift, (uint64_t)n);
+  const double recon = p->reconstruct_cost(p->user, bytes.data(), (uint32_t)R, (uint64_t)n);
+  const double extract = 0.0;
+  return stitch + recon + extract + sort;
+}
+
+// The GPU model: stitch, tuple reconstruction (gather), multi-pass sort and
+// group extraction, with the executor's fixed costs.
 double estimate(const fmcs_cost_model* m, double n, uint32_t ncols, const uint32_t* bw, const uint64_t* card,
                 const uint32_t* order, const Split& split, bool truncate, double* parts) {
-  std::vector<double> cardv;
   Stats st;
-  plan_stats(ncols, bw, card, order, split, cardv, st, truncate);
+  const std::vector<double> G = plan_groups(ncols, bw, card, order, split, truncate, st);
   const size_t R = split.size();
-  // groups before each round: G_0 = 1, G_r = prod_{i<r} card_i
-  std::vector<double> G(R, 1.0);
-  double acc = 1.0;
-  for (size_t r = 0; r < R; ++r) {
-    G[r] = acc;
-    acc *= cardv[r];
-  }
   // A carried second column makes every round-1 pass move its bank too
   // (the passes are HBM-bound: time scales with the bytes moved per row,
   // key + oid + payload against key + oid) and removes round 2's gather.
@@ -209,9 +281,12 @@ double estimate_group_num(uint32_t bit_position, uint32_t ncols, const uint32_t*
 
 // find_greedy_plans (src/stitch_composer.cpp:1992-2125): per bank pattern,
 // every cut is placed where the NEXT round's sort is cheapest
-// (determineBitRange :1736-1763, determineOptBitPosition_v2 :2155-2184).
-std::vector<Split> greedy_plans(const fmcs_cost_model* m, double n, uint32_t ncols, const uint32_t* bw,
-                                const uint64_t* card, const uint32_t* order, uint32_t rounds) {
+// (determineBitRange :1736-1763, determineOptBitPosition_v2 :2155-2184; a
+// tie goes to the later position).  colcost(n, g, bank type) prices a
+// round over groups of g rows.
+template <class ColCost>
+std::vector<Split> greedy_plans(ColCost colcost, double n, uint32_t ncols, const uint32_t* bw, const uint64_t* card,
+                                const uint32_t* order, uint32_t rounds) {
   std::vector<Split> out;
   uint32_t S = 0;
   for (uint32_t c = 0; c < ncols; ++c) S += bw[c];
@@ -231,8 +306,8 @@ std::vector<Split> greedy_plans(const fmcs_cost_model* m, double n, uint32_t nco
       double best = std::numeric_limits<double>::max();
       int opt = -1;
       for (int pos = lo; pos <= hi; ++pos) {
-        double gsz = n / estimate_group_num((uint32_t)pos, ncols, bw, card, order);
-        double c = single_column_cost(m, n, std::max(1.0, gsz), (int)cate[r + 1]);
+        const double gsz = n / estimate_group_num((uint32_t)pos, ncols, bw, card, order);
+        const double c = colcost(n, gsz, (int)cate[r + 1]);
         if (c <= best) {
           best = c;
           opt = pos;
@@ -257,93 +332,36 @@ std::vector<Split> greedy_plans(const fmcs_cost_model* m, double n, uint32_t nco
   return out;
 }
 
-// ---- Recursive Random Search (Ye et al. 2003; src/rrs.h:80-170) ------------
+std::vector<Split> greedy_plans(const fmcs_cost_model* m, double n, uint32_t ncols, const uint32_t* bw,
+                                const uint64_t* card, const uint32_t* order, uint32_t rounds) {
+  auto cc = [m](double nn, double g, int bt) { return single_column_cost(m, nn, std::max(1.0, g), bt); };
+  return greedy_plans(cc, n, ncols, bw, card, order, rounds);
+}
 
-class PlanEnumRRS {
+// ---- Recursive Random Search over massaging plans ---------------------------
+//
+// PlanEnumRRS (src/plan_enum_rrs.cpp:37-267, parameters src/plan_enum_rrs.h:29):
+// a plan is a split (round widths summing to S, each at most 64), costed with
+// the RRS variant of the cardinalities (truncated to uint64).  The samples and
+// neighbours follow the reference's draws one for one, so a seed visits the
+// reference's plans: the sampler draws a round count once before its retry
+// loop (src/plan_enum_rrs.cpp:48), and a neighbour move off either end of
+// the split appends a new last round (:225-231).
+class PlanEnumRRS : public fmcs::rrs::Search<Split, double> {
  public:
-  PlanEnumRRS(const fmcs_cost_model* m, double n, uint32_t S, uint32_t max_round, uint32_t ncols,
-              const uint32_t* bw, const uint64_t* card, const uint32_t* order, uint64_t seed)
-      : m_(m), n_(n), S_(S), max_round_(max_round), ncols_(ncols), bw_(bw), card_(card), order_(order),
-        gen_(seed), dist_(0.0, 1.0) {}
-
-  // RRS<P,C>::Optimize with (p,r,q,v,c,st) = (0.99,0.1,0.99,0.8,0.5,0.001)
-  // (src/plan_enum_rrs.h:29)
-  void Optimize() {
-    const double p = 0.99, r = 0.1, q = 0.99, v = 0.8, c = 0.5, st = 0.001;
-    const size_t nsamp = (size_t)std::ceil(std::log(1 - p) / std::log(1 - r));
-    const size_t l = (size_t)std::ceil(std::log(1 - q) / std::log(1 - v));
-    opt_point_ = Sample();
-    opt_cost_ = Cost(opt_point_);
-    for (size_t i = 1; i < nsamp; ++i) {
-      Split x = Sample();
-      double cx = Cost(x);
-      if (cx < opt_cost_) {
-        opt_cost_ = cx;
-        opt_point_ = x;
-      }
-    }
-    Split x0 = opt_point_;
-    double cost0 = opt_cost_, yr = opt_cost_;
-    std::vector<double> F{opt_cost_};
-    bool exploit = true;
-    size_t i = 0;
-    while (!Stop()) {
-      if (exploit) {
-        size_t j = 0;
-        double fc = Cost(x0);
-        Split xl = x0;
-        double ro = r;
-        while (ro > st) {
-          Split x1 = Neighbor(xl, ro);
-          double c1 = Cost(x1);
-          if (c1 < fc) {
-            xl = x1;
-            fc = c1;
-            j = 0;
-          } else {
-            ++j;
-          }
-          if (j == l) {
-            ro = c * ro;
-            j = 0;
-          }
-        }
-        exploit = false;
-        if (fc < opt_cost_) {
-          opt_cost_ = fc;
-          opt_point_ = xl;
-        }
-      }
-      Split x1 = Sample();
-      double c1 = Cost(x1);
-      if (c1 < cost0) {
-        x0 = x1;
-        cost0 = c1;
-      }
-      if (c1 < yr) exploit = true;
-      if (i == nsamp) {
-        F.push_back(cost0);
-        yr = std::accumulate(F.begin(), F.end(), 0.0) / F.size();
-        i = 0;
-      }
-      ++i;
-    }
-  }
+  PlanEnumRRS(std::function<double(const Split&)> cost, uint32_t S, uint32_t max_round, uint64_t seed)
+      : Search<Split, double>(seed), cost_(std::move(cost)), S_(S), max_round_(max_round) {}
 
   const std::set<Split>& Plans() const { return plans_; }
-  const Split& Best() const { return opt_point_; }
-  double BestCost() const { return opt_cost_; }
 
- private:
-  double Dice() { return dist_(gen_); }
-
-  // PlanEnumRRS::GetRandomSample (src/plan_enum_rrs.cpp:37-92)
-  Split Sample() {
+ protected:
+  // GetRandomSample (src/plan_enum_rrs.cpp:37-92)
+  Split Sample() override {
+    (void)Dice();  // the reference's first round-count draw, overwritten inside its loop
     Split s;
     for (;;) {
       s.clear();
-      uint32_t rounds = std::min<uint32_t>((uint32_t)std::ceil(Dice() * max_round_), max_round_);
-      if (rounds == 0) rounds = 1;
+      const uint32_t rounds = std::min<uint32_t>((uint32_t)std::ceil(Dice() * max_round_), max_round_);
       uint32_t avail = S_, cur = 0;
       while (cur + 1 < rounds) {
         uint32_t w = 1 + (uint32_t)std::floor(Dice() * (double)(avail - 1));
@@ -355,31 +373,25 @@ class PlanEnumRRS {
       }
       if (avail > 0 && avail <= 64) {
         s.push_back(avail);
-        if (s.size() == rounds) {
-          plans_.insert(s);
-          return s;
-        }
+        plans_.insert(s);
+        return s;
       }
     }
   }
 
-  // PlanEnumRRS::GetCost (src/plan_enum_rrs.cpp:94-197): the uint64
-  // truncating cardinality variant.
-  double Cost(const Split& s) { return estimate(m_, n_, ncols_, bw_, card_, order_, s, true, nullptr); }
+  double Evaluate(const Split& s) override { return cost_(s); }
 
-  // PlanEnumRRS::StopCritera (src/plan_enum_rrs.cpp:199-204)
-  bool Stop() { return ++counter_ > 10; }
+  // StopCritera (src/plan_enum_rrs.cpp:199-204)
+  bool Stop() override { return ++counter_ > 10; }
 
-  // PlanEnumRRS::GetRandomNeighbor (src/plan_enum_rrs.cpp:206-267): move
-  // single bits between adjacent rounds; a move off either end appends a new
-  // last round, as the reference does.
-  Split Neighbor(const Split& pt, double ro) {
+  // GetRandomNeighbor (src/plan_enum_rrs.cpp:206-267): move single bits
+  // between adjacent rounds, ceil(S * ro) moves
+  Split Neighbor(const Split& pt, double ro) override {
     Split t(pt.begin(), pt.end());
     uint32_t steps = (uint32_t)std::ceil(S_ * ro);
     while (steps > 0) {
       int dec = (int)std::floor(Dice() * t.size());
-      if (dec >= (int)t.size()) dec = (int)t.size() - 1;
-      uint32_t coin = ((uint32_t)(100.0 * Dice())) % 2;
+      const uint32_t coin = ((uint32_t)(100.0 * Dice())) % 2;
       int inc = coin ? dec - 1 : dec + 1;
       if (inc == -1 || inc == (int)t.size()) {
         t.push_back(0);
@@ -398,18 +410,11 @@ class PlanEnumRRS {
     return out;
   }
 
-  const fmcs_cost_model* m_;
-  double n_;
-  uint32_t S_, max_round_, ncols_;
-  const uint32_t* bw_;
-  const uint64_t* card_;
-  const uint32_t* order_;
-  std::mt19937_64 gen_;
-  std::uniform_real_distribution<double> dist_;
+ private:
+  std::function<double(const Split&)> cost_;
+  uint32_t S_, max_round_;
   size_t counter_ = 0;
   std::set<Split> plans_;
-  Split opt_point_;
-  double opt_cost_ = 0;
 };
 
 int write_plans(const std::vector<Split>& plans, uint32_t* out, uint64_t cap_words, uint64_t* nplans,
@@ -482,6 +487,112 @@ int fmcs_estimate_plan(const fmcs_cost_model* m, uint64_t nrows, uint32_t ncols,
   return FMCS_OK;
 }
 
+namespace {
+bool split_ok(uint32_t ncols, const uint32_t* bw, uint32_t nsplit, const uint32_t* split) {
+  if (!split || nsplit == 0) return false;
+  uint64_t S = 0, T = 0;
+  for (uint32_t c = 0; c < ncols; ++c) S += bw[c];
+  for (uint32_t r = 0; r < nsplit; ++r) {
+    if (split[r] == 0 || split[r] > 64) return false;
+    T += split[r];
+  }
+  return S == T;
+}
+bool order_ok(uint32_t ncols, const uint32_t* order) {
+  if (!order) return false;
+  std::vector<char> seen(ncols, 0);
+  for (uint32_t i = 0; i < ncols; ++i) {
+    if (order[i] >= ncols || seen[order[i]]) return false;
+    seen[order[i]] = 1;
+  }
+  return true;
+}
+bool prims_ok(const fmcs_cost_primitives* p) { return p && p->sort_cost && p->stitch_cost && p->reconstruct_cost; }
+}  // namespace
+
+int fmcs_plan_statistics(uint32_t ncols, const uint32_t* bw, const uint64_t* card, const uint32_t* col_order,
+                         uint32_t nsplit, const uint32_t* split, double* card_out, uint32_t* assemble9,
+                         uint32_t* shift3) {
+  if (!inputs_ok(ncols, bw, card) || !order_ok(ncols, col_order) || !split_ok(ncols, bw, nsplit, split))
+    return FMCS_E_ARG;
+  std::vector<double> cd;
+  Stats st;
+  plan_stats(ncols, bw, card, col_order, Split(split, split + nsplit), cd, st);
+  if (card_out) std::copy(cd.begin(), cd.end(), card_out);
+  if (assemble9) std::memcpy(assemble9, st.assemble, sizeof(st.assemble));
+  if (shift3) std::memcpy(shift3, st.shift, sizeof(st.shift));
+  return FMCS_OK;
+}
+
+int fmcs_plan_statistics_u64(uint32_t ncols, const uint32_t* bw, const uint64_t* card, const uint32_t* col_order,
+                             uint32_t nsplit, const uint32_t* split, uint64_t* card_out, uint64_t* groups_out) {
+  if (!inputs_ok(ncols, bw, card) || !order_ok(ncols, col_order) || !split_ok(ncols, bw, nsplit, split))
+    return FMCS_E_ARG;
+  std::vector<uint64_t> cu;
+  Stats st;
+  plan_stats(ncols, bw, card, col_order, Split(split, split + nsplit), cu, st);
+  if (card_out) std::copy(cu.begin(), cu.end(), card_out);
+  if (groups_out && !cu.empty()) {
+    uint64_t acc = cu[0];
+    groups_out[0] = 1;
+    for (size_t r = 1; r < cu.size(); ++r) {
+      groups_out[r] = acc;
+      acc *= cu[r];
+    }
+  }
+  return FMCS_OK;
+}
+
+int fmcs_estimate_group_num(uint32_t bit_position, uint32_t ncols, const uint32_t* bw, const uint64_t* card,
+                            const uint32_t* col_order, double* groups) {
+  if (!groups || !inputs_ok(ncols, bw, card) || !order_ok(ncols, col_order)) return FMCS_E_ARG;
+  *groups = estimate_group_num(bit_position, ncols, bw, card, col_order);
+  return FMCS_OK;
+}
+
+int fmcs_column_cost_prim(const fmcs_cost_primitives* p, uint64_t nrows, double group_rows, uint32_t bank_type,
+                          double* cost) {
+  if (!prims_ok(p) || !cost || bank_type > 2) return FMCS_E_ARG;
+  *cost = single_column_cost_prim(p, (double)nrows, group_rows, (int)bank_type);
+  return FMCS_OK;
+}
+
+int fmcs_estimate_plan_prim(const fmcs_cost_primitives* p, uint64_t nrows, uint32_t ncols, const uint32_t* bw,
+                            const uint64_t* card, const uint32_t* col_order, uint32_t nsplit, const uint32_t* split,
+                            int truncate, double* cost) {
+  if (!prims_ok(p) || !cost || !inputs_ok(ncols, bw, card) || !order_ok(ncols, col_order) ||
+      !split_ok(ncols, bw, nsplit, split))
+    return FMCS_E_ARG;
+  *cost = estimate_prim(p, (double)nrows, ncols, bw, card, col_order, Split(split, split + nsplit), truncate != 0);
+  return FMCS_OK;
+}
+
+int fmcs_greedy_plans_prim(const fmcs_cost_primitives* p, uint64_t nrows, uint32_t ncols, const uint32_t* bw,
+                           const uint64_t* card, const uint32_t* col_order, uint32_t rounds, uint32_t* out,
+                           uint64_t cap_words, uint64_t* nplans, uint64_t* words) {
+  if (!prims_ok(p) || rounds < 1 || rounds > 20 || !inputs_ok(ncols, bw, card) || !order_ok(ncols, col_order))
+    return FMCS_E_ARG;
+  auto cc = [p](double nn, double g, int bt) { return single_column_cost_prim(p, nn, g, bt); };
+  return write_plans(greedy_plans(cc, (double)nrows, ncols, bw, card, col_order, rounds), out, cap_words, nplans,
+                     words);
+}
+
+int fmcs_plan_enum_rrs_prim(const fmcs_cost_primitives* p, uint64_t nrows, uint32_t ncols, const uint32_t* bw,
+                            const uint64_t* card, const uint32_t* col_order, uint32_t max_round, uint64_t seed,
+                            uint32_t* out, uint64_t cap_words, uint64_t* nplans, uint64_t* words) {
+  if (!prims_ok(p) || max_round == 0 || !inputs_ok(ncols, bw, card) || !order_ok(ncols, col_order))
+    return FMCS_E_ARG;
+  uint32_t S = 0;
+  for (uint32_t c = 0; c < ncols; ++c) S += bw[c];
+  if (max_round * 64u < S) return FMCS_E_ARG;
+  const double n = (double)nrows;
+  PlanEnumRRS e([=](const Split& sp) { return estimate_prim(p, n, ncols, bw, card, col_order, sp, true); }, S,
+                max_round, seed);
+  e.Optimize();
+  std::vector<Split> v(e.Plans().begin(), e.Plans().end());
+  return write_plans(v, out, cap_words, nplans, words);
+}
+
 int fmcs_split_exhaustive(uint32_t S, uint32_t rounds, uint32_t maxv, uint32_t* out, uint64_t cap_plans,
                           uint64_t* count) {
   if (!count) return FMCS_E_ARG;
@@ -518,7 +629,9 @@ int fmcs_plan_enum_rrs(const fmcs_cost_model* m, uint64_t nrows, uint32_t ncols,
   uint32_t S = 0;
   for (uint32_t c = 0; c < ncols; ++c) S += bw[c];
   if (max_round * 64u < S) return FMCS_E_ARG;
-  PlanEnumRRS e(m, (double)nrows, S, max_round, ncols, bw, card, col_order, seed);
+  const double n = (double)nrows;
+  PlanEnumRRS e([=](const Split& sp) { return estimate(m, n, ncols, bw, card, col_order, sp, true, nullptr); }, S,
+                max_round, seed);
   e.Optimize();
   std::vector<Split> v(e.Plans().begin(), e.Plans().end());
   return write_plans(v, out, cap_words, nplans, words);

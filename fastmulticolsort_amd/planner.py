@@ -43,6 +43,24 @@ class CostModel(C.Structure):
         return m
 
 
+SortCostFn = C.CFUNCTYPE(C.c_double, C.c_void_p, C.c_double, C.c_uint32)
+StitchCostFn = C.CFUNCTYPE(C.c_double, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_uint64)
+ReconCostFn = C.CFUNCTYPE(C.c_double, C.c_void_p, C.POINTER(C.c_uint32), C.c_uint32, C.c_uint64)
+
+
+class CostPrimitives(C.Structure):
+    """fmcs_cost_primitives: sort_cost(user, group_rows, bank_type),
+    stitch_cost(user, assemble9, shift3, nrows), reconstruct_cost(user,
+    bytewidths, nrounds, nrows)."""
+    _fields_ = [("sort_cost", SortCostFn), ("stitch_cost", StitchCostFn), ("reconstruct_cost", ReconCostFn),
+                ("user", C.c_void_p)]
+
+    @classmethod
+    def from_addresses(cls, sort, stitch, recon):
+        """Primitives at raw C function addresses (e.g. the reference's estimators)."""
+        return cls(C.cast(sort, SortCostFn), C.cast(stitch, StitchCostFn), C.cast(recon, ReconCostFn), None)
+
+
 _sig_done = False
 
 
@@ -72,6 +90,16 @@ def lib():
             "fmcs_calibrate": [vp, u64, P(CostModel)],
             "fmcs_optimize_and_run": [vp, P(CostModel), u32, P(vp), P(u32), P(u32), P(u64), i32, i32, u64, vp, vp,
                                       P(u32), P(u32), P(u32), P(C.c_double), P(C.c_double), P(C.c_double), P(u32)],
+            "fmcs_plan_statistics": [u32, P(u32), P(u64), P(u32), u32, P(u32), P(C.c_double), P(u32), P(u32)],
+            "fmcs_plan_statistics_u64": [u32, P(u32), P(u64), P(u32), u32, P(u32), P(u64), P(u64)],
+            "fmcs_estimate_group_num": [u32, u32, P(u32), P(u64), P(u32), P(C.c_double)],
+            "fmcs_column_cost_prim": [P(CostPrimitives), u64, C.c_double, u32, P(C.c_double)],
+            "fmcs_estimate_plan_prim": [P(CostPrimitives), u64, u32, P(u32), P(u64), P(u32), u32, P(u32), i32,
+                                        P(C.c_double)],
+            "fmcs_greedy_plans_prim": [P(CostPrimitives), u64, u32, P(u32), P(u64), P(u32), u32, P(u32), u64,
+                                       P(u64), P(u64)],
+            "fmcs_plan_enum_rrs_prim": [P(CostPrimitives), u64, u32, P(u32), P(u64), P(u32), u32, u64, P(u32), u64,
+                                        P(u64), P(u64)],
         }
         for name, args in sigs.items():
             fn = getattr(L, name)
@@ -172,6 +200,67 @@ def optimize(model, nrows, bw, card, ordered=True):
 
 
 # ---- composer-level execution (GPU) -------------------------------------
+
+def plan_statistics(bw, card, col_order, split):
+    """(cardinality of each new column, assemble[3][3] flat, shift[3]) as
+    getCardinalityFromStatistics computes them."""
+    k = len(split)
+    cd = (C.c_double * k)()
+    a = (C.c_uint32 * 9)()
+    sh = (C.c_uint32 * 3)()
+    _chk(lib().fmcs_plan_statistics(len(bw), _u32(bw), _u64(card), _u32(col_order), k, _u32(split), cd, a, sh),
+         "plan_statistics")
+    return list(cd), list(a), list(sh)
+
+
+def plan_statistics_u64(bw, card, col_order, split):
+    """(uint64 cardinalities as PlanEnumRRS holds them, groups before each round)."""
+    k = len(split)
+    cu = (C.c_uint64 * k)()
+    g = (C.c_uint64 * k)()
+    _chk(lib().fmcs_plan_statistics_u64(len(bw), _u32(bw), _u64(card), _u32(col_order), k, _u32(split), cu, g),
+         "plan_statistics_u64")
+    return list(cu), list(g)
+
+
+def estimate_group_num(bitpos, bw, card, col_order):
+    out = C.c_double()
+    _chk(lib().fmcs_estimate_group_num(bitpos, len(bw), _u32(bw), _u64(card), _u32(col_order), C.byref(out)),
+         "estimate_group_num")
+    return out.value
+
+
+def column_cost_prim(prims, nrows, g, bank_type):
+    out = C.c_double()
+    _chk(lib().fmcs_column_cost_prim(C.byref(prims), nrows, g, bank_type, C.byref(out)), "column_cost_prim")
+    return out.value
+
+
+def estimate_plan_prim(prims, nrows, bw, card, col_order, split, truncate=False):
+    out = C.c_double()
+    _chk(lib().fmcs_estimate_plan_prim(C.byref(prims), nrows, len(bw), _u32(bw), _u64(card), _u32(col_order),
+                                       len(split), _u32(split), int(bool(truncate)), C.byref(out)),
+         "estimate_plan_prim")
+    return out.value
+
+
+def greedy_plans_prim(prims, nrows, bw, card, col_order, rounds):
+    cap = 1 << 16
+    buf = (C.c_uint32 * cap)()
+    npl, words = C.c_uint64(), C.c_uint64()
+    _chk(lib().fmcs_greedy_plans_prim(C.byref(prims), nrows, len(bw), _u32(bw), _u64(card), _u32(col_order), rounds,
+                                      buf, cap, C.byref(npl), C.byref(words)), "greedy_plans_prim")
+    return _unflatten(buf, words.value)
+
+
+def plan_enum_rrs_prim(prims, nrows, bw, card, col_order, max_round, seed):
+    cap = 1 << 19
+    buf = (C.c_uint32 * cap)()
+    npl, words = C.c_uint64(), C.c_uint64()
+    _chk(lib().fmcs_plan_enum_rrs_prim(C.byref(prims), nrows, len(bw), _u32(bw), _u64(card), _u32(col_order),
+                                       max_round, seed, buf, cap, C.byref(npl), C.byref(words)), "plan_enum_rrs_prim")
+    return _unflatten(buf, words.value)
+
 
 def run_plan(ctx, cols, bws, col_order, split, mode=0, order=None, groups=None, descending=None):
     """StitchComposer::run_single_plan on device tensors; returns

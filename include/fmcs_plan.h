@@ -41,6 +41,61 @@ typedef struct {
 
 int fmcs_cost_model_default(fmcs_cost_model* m);
 
+/* Cost primitives supplied by the caller, for the planner's structure with
+ * someone else's constants -- for instance the reference's own estimators
+ * (StitchComposer::estimate_mergesort_uint{16,32,64}, estimate_stitch,
+ * estimate_tuple_reconstruct; src/stitch_composer.h:54-72), so that a CPU
+ * executor keeps its cost tables and this planner prices plans exactly as
+ * the reference does.  Units are the caller's. */
+typedef struct {
+  /* cost of sorting one group of group_rows rows in a bank of type bank_type (0: u16, 1: u32, 2: u64) */
+  double (*sort_cost)(void* user, double group_rows, uint32_t bank_type);
+  /* stitch cost from assemble[from bank][to bank] (row-major, 9 words) and shift[to bank] entry counts */
+  double (*stitch_cost)(void* user, const uint32_t* assemble9, const uint32_t* shift3, uint64_t nrows);
+  /* tuple-reconstruction cost of a plan's new banks (bytes per round, rounds 1.. gathered) */
+  double (*reconstruct_cost)(void* user, const uint32_t* bytewidths, uint32_t nrounds, uint64_t nrows);
+  void* user;
+} fmcs_cost_primitives;
+
+/* getCardinalityFromStatistics (src/stitch_composer.cpp:288-553): the
+ * cardinality of every new column but the last (card_per_bit^width products,
+ * card_per_bit = card^(1/bw)), card_out[nsplit - 1] = 1, and the assemble /
+ * shift statistics of the split's construct entries. */
+int fmcs_plan_statistics(uint32_t ncols, const uint32_t* bw, const uint64_t* card, const uint32_t* col_order,
+                         uint32_t nsplit, const uint32_t* split, double* card_out, uint32_t* assemble9,
+                         uint32_t* shift3);
+/* PlanEnumRRS::GetStatistics (src/plan_enum_rrs.cpp:269-396): the same
+ * cardinalities held in uint64 and converted after every entry (x86-64
+ * conversion), and groups_out (optional) the groups before each round,
+ * their uint64 products with wrap-around (src/plan_enum_rrs.cpp:149-156). */
+int fmcs_plan_statistics_u64(uint32_t ncols, const uint32_t* bw, const uint64_t* card, const uint32_t* col_order,
+                             uint32_t nsplit, const uint32_t* split, uint64_t* card_out, uint64_t* groups_out);
+/* estimateGroupNum (src/stitch_composer.cpp:1845-1867): groups of the first
+ * bit_position bits of the ordered columns. */
+int fmcs_estimate_group_num(uint32_t bit_position, uint32_t ncols, const uint32_t* bw, const uint64_t* card,
+                            const uint32_t* col_order, double* groups);
+/* calculate_single_column_cost (src/stitch_composer.cpp:2186-2226): (n / g)
+ * groups x f(g) x sort_cost(g), f(g) = 1 / (1 + 1/(g - 1)), 0 for g <= 1. */
+int fmcs_column_cost_prim(const fmcs_cost_primitives* p, uint64_t nrows, double group_rows, uint32_t bank_type,
+                          double* cost);
+/* One plan's cost with the primitives: estimate_plan_collection_greedy's
+ * costing (truncate 0; src/stitch_composer.cpp:2228-2378) or
+ * PlanEnumRRS::GetCost's (truncate 1, uint64 cardinalities;
+ * src/plan_enum_rrs.cpp:94-197). */
+int fmcs_estimate_plan_prim(const fmcs_cost_primitives* p, uint64_t nrows, uint32_t ncols, const uint32_t* bw,
+                            const uint64_t* card, const uint32_t* col_order, uint32_t nsplit, const uint32_t* split,
+                            int truncate, double* cost);
+/* fmcs_greedy_plans with the primitives' column cost. */
+int fmcs_greedy_plans_prim(const fmcs_cost_primitives* p, uint64_t nrows, uint32_t ncols, const uint32_t* bw,
+                           const uint64_t* card, const uint32_t* col_order, uint32_t rounds, uint32_t* out,
+                           uint64_t cap_words, uint64_t* nplans, uint64_t* words);
+/* fmcs_plan_enum_rrs costed by fmcs_estimate_plan_prim (truncate 1): with the
+ * reference's estimators and the seed the reference's RRS took from
+ * time(0) (src/rrs.h:68), the same visited plans as PlanEnumRRS. */
+int fmcs_plan_enum_rrs_prim(const fmcs_cost_primitives* p, uint64_t nrows, uint32_t ncols, const uint32_t* bw,
+                            const uint64_t* card, const uint32_t* col_order, uint32_t max_round, uint64_t seed,
+                            uint32_t* out, uint64_t cap_words, uint64_t* nplans, uint64_t* words);
+
 /* Estimated cost of one plan (estimate_plan_collection_greedy's per-plan
  * costing, src/stitch_composer.cpp:2228-2378; truncate_cardinality selects
  * PlanEnumRRS::GetCost's uint64 cardinalities, src/plan_enum_rrs.cpp:129-153).

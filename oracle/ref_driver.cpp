@@ -12,6 +12,9 @@
 #include <limits>
 #include <random>
 #include "generator.h"         // DataGenerator::ReadFromFile / ReadFromFile_tpch
+#include "stitch_composer.cpp" // StitchComposer's estimators, greedy plans, cardinality statistics
+#include "plan_enum_rrs.cpp"   // PlanEnumRRS (RRS over massaging plans)
+#include <ctime>
 
 #include <algorithm>
 #include <atomic>
@@ -367,6 +370,228 @@ int ref_read_text(const char *path, int tpch, uint32_t ncols, const uint32_t *wi
     delete colv[c];
   }
   return 0;
+}
+
+}  // extern "C"
+
+// ---- the reference's planner ------------------------------------------------
+//
+// Its statistics, group-count estimate, column cost, greedy plans and plan
+// costing are private members of StitchComposer / PlanEnumRRS.  A pointer to
+// a private member may be named in an explicit instantiation (access is not
+// checked there), and a friend defined by that instantiation hands it out.
+namespace {
+
+template <class Tag, typename Tag::type M>
+struct Reach {
+  friend typename Tag::type reach(Tag) { return M; }
+};
+
+struct CardStats {
+  typedef void (StitchComposer::*type)(uint32_t *, std::vector<uint32_t> &, double *, uint32_t (*)[3], uint32_t *);
+  friend type reach(CardStats);
+};
+template struct Reach<CardStats, &StitchComposer::getCardinalityFromStatistics>;
+
+struct GroupNum {
+  typedef double (StitchComposer::*type)(uint32_t, uint32_t *);
+  friend type reach(GroupNum);
+};
+template struct Reach<GroupNum, &StitchComposer::estimateGroupNum>;
+
+struct ColumnCost {
+  typedef double (StitchComposer::*type)(double, uint32_t);
+  friend type reach(ColumnCost);
+};
+template struct Reach<ColumnCost, &StitchComposer::calculate_single_column_cost>;
+
+struct Greedy {
+  typedef void (StitchComposer::*type)(uint32_t, std::vector<std::vector<uint32_t> > &, uint32_t *);
+  friend type reach(Greedy);
+};
+template struct Reach<Greedy, &StitchComposer::find_greedy_plans>;
+
+struct PlanCost {
+  typedef void (StitchComposer::*type)(uint32_t *, std::vector<std::vector<uint32_t> > &, int &, double &);
+  friend type reach(PlanCost);
+};
+template struct Reach<PlanCost, &StitchComposer::estimate_plan_collection_greedy>;
+
+struct RrsStats {
+  typedef void (PlanEnumRRS::*type)(const std::vector<uint32_t> &, uint64_t *, uint32_t (*)[3], uint32_t *);
+  friend type reach(RrsStats);
+};
+template struct Reach<RrsStats, &PlanEnumRRS::GetStatistics>;
+
+// A StitchComposer over columns that carry only their cardinality (the
+// planner reads no values).
+struct RefPlanner {
+  std::vector<uint32_t> bw;
+  std::vector<SpecialiedColumn<uint64_t> *> cols;
+  std::vector<Column *> colp;
+  StitchComposer *sc = nullptr;
+  RefPlanner(uint64_t n, uint32_t ncols, const uint32_t *bws, const uint64_t *card) : bw(bws, bws + ncols) {
+    uint32_t S = 0;
+    for (uint32_t c = 0; c < ncols; ++c) {
+      auto *col = new SpecialiedColumn<uint64_t>();
+      col->SetColumn(nullptr);
+      col->SetWidth(8);
+      col->SetCardinality(card[c]);
+      cols.push_back(col);
+      colp.push_back(col);
+      S += bws[c];
+    }
+    compose_params_t p;
+    p.bitwidth = bw.data();
+    p.colwidth_sum = S;
+    setting_t st;
+    sc = new StitchComposer(st, ncols, n, p, colp.data());
+  }
+  ~RefPlanner() {
+    delete sc;
+    for (auto *c : cols) delete c;
+  }
+};
+
+struct RefRrs : PlanEnumRRS {
+  using PlanEnumRRS::PlanEnumRRS;
+  using PlanEnumRRS::GetCost;
+};
+
+std::vector<uint32_t> vec(const uint32_t *a, uint32_t n) { return std::vector<uint32_t>(a, a + n); }
+
+int put_plans(const std::vector<std::vector<uint32_t> > &plans, uint32_t *out, uint64_t cap, uint64_t *nplans,
+              uint64_t *words) {
+  uint64_t w = 0;
+  for (const auto &p : plans) {
+    if (w + 1 + p.size() > cap) return -1;
+    out[w++] = (uint32_t)p.size();
+    for (uint32_t x : p) out[w++] = x;
+  }
+  *nplans = plans.size();
+  *words = w;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void *ref_planner_new(uint64_t n, uint32_t ncols, const uint32_t *bw, const uint64_t *card) {
+  return new RefPlanner(n, ncols, bw, card);
+}
+void ref_planner_free(void *h) { delete (RefPlanner *)h; }
+
+// getCardinalityFromStatistics (src/stitch_composer.cpp:288-553)
+void ref_card_stats(void *h, const uint32_t *order, uint32_t nsplit, const uint32_t *split, double *card,
+                    uint32_t *assemble9, uint32_t *shift3) {
+  RefPlanner *r = (RefPlanner *)h;
+  std::vector<uint32_t> ord(order, order + r->bw.size()), sp = vec(split, nsplit);
+  uint32_t a[3][3] = {{0}};
+  uint32_t sh[3] = {0, 0, 0};
+  for (uint32_t i = 0; i < nsplit; ++i) card[i] = 1;
+  (r->sc->*reach(CardStats()))(ord.data(), sp, card, a, sh);
+  std::memcpy(assemble9, a, sizeof(a));
+  std::memcpy(shift3, sh, sizeof(sh));
+}
+
+// estimateGroupNum (src/stitch_composer.cpp:1845-1867)
+double ref_group_num(void *h, uint32_t bitpos, const uint32_t *order) {
+  RefPlanner *r = (RefPlanner *)h;
+  std::vector<uint32_t> ord(order, order + r->bw.size());
+  return (r->sc->*reach(GroupNum()))(bitpos, ord.data());
+}
+
+// calculate_single_column_cost (src/stitch_composer.cpp:2186-2226)
+double ref_column_cost(void *h, double g, uint32_t bank_type) {
+  return (((RefPlanner *)h)->sc->*reach(ColumnCost()))(g, bank_type);
+}
+
+// find_greedy_plans (src/stitch_composer.cpp:1992-2125), rounds > 2
+int ref_greedy_plans(void *h, uint32_t rounds, const uint32_t *order, uint32_t *out, uint64_t cap, uint64_t *nplans,
+                     uint64_t *words) {
+  RefPlanner *r = (RefPlanner *)h;
+  std::vector<uint32_t> ord(order, order + r->bw.size());
+  std::vector<std::vector<uint32_t> > plans;
+  (r->sc->*reach(Greedy()))(rounds, plans, ord.data());
+  return put_plans(plans, out, cap, nplans, words);
+}
+
+// One plan through estimate_plan_collection_greedy (src/stitch_composer.cpp:2228-2378)
+double ref_plan_cost(void *h, const uint32_t *order, uint32_t nsplit, const uint32_t *split) {
+  RefPlanner *r = (RefPlanner *)h;
+  std::vector<uint32_t> ord(order, order + r->bw.size());
+  std::vector<std::vector<uint32_t> > plans{vec(split, nsplit)};
+  int best = -1;
+  double cost = 0;
+  (r->sc->*reach(PlanCost()))(ord.data(), plans, best, cost);
+  return cost;
+}
+
+// PlanEnumRRS::GetStatistics and GetCost (src/plan_enum_rrs.cpp:94-197, :269-396)
+double ref_rrs_cost(uint64_t n, uint32_t max_round, const uint32_t *order, uint32_t ncols, const uint32_t *bw,
+                    const uint64_t *card, uint32_t nsplit, const uint32_t *split, uint64_t *card_out,
+                    uint32_t *assemble9, uint32_t *shift3) {
+  uint32_t S = 0;
+  std::vector<uint32_t> c32(ncols);
+  for (uint32_t c = 0; c < ncols; ++c) {
+    S += bw[c];
+    c32[c] = (uint32_t)card[c];
+  }
+  RefRrs e(n, S, max_round, vec(order, ncols), ncols, vec(bw, ncols), c32, 1e18);
+  std::vector<uint32_t> sp = vec(split, nsplit);
+  if (card_out) {
+    uint32_t a[3][3] = {{0}};
+    uint32_t sh[3] = {0, 0, 0};
+    for (uint32_t i = 0; i < nsplit; ++i) card_out[i] = 1;
+    (e.*reach(RrsStats()))(sp, card_out, a, sh);
+    std::memcpy(assemble9, a, sizeof(a));
+    std::memcpy(shift3, sh, sizeof(sh));
+  }
+  return e.GetCost(sp);
+}
+
+// PlanEnumRRS::Optimize and ObtainPlans (src/plan_enum_rrs.h:20-79): its engine
+// is seeded from time(0) in the constructor (src/rrs.h:68); the second read
+// before and after the construction tells the seed (retried when they differ).
+int ref_plan_enum_rrs(uint64_t n, uint32_t max_round, const uint32_t *order, uint32_t ncols, const uint32_t *bw,
+                      const uint64_t *card, uint64_t *seed, uint32_t *out, uint64_t cap, uint64_t *nplans,
+                      uint64_t *words) {
+  uint32_t S = 0;
+  std::vector<uint32_t> c32(ncols);
+  for (uint32_t c = 0; c < ncols; ++c) {
+    S += bw[c];
+    c32[c] = (uint32_t)card[c];
+  }
+  for (;;) {
+    const std::time_t t0 = std::time(0);
+    RefRrs e(n, S, max_round, vec(order, ncols), ncols, vec(bw, ncols), c32, 1e18);
+    const std::time_t t1 = std::time(0);
+    if (t0 != t1) continue;
+    e.Optimize();
+    const auto visited = e.ObtainPlans();
+    *seed = (uint64_t)t0;
+    return put_plans(std::vector<std::vector<uint32_t> >(visited.begin(), visited.end()), out, cap, nplans, words);
+  }
+}
+
+// The reference's static estimators with the signatures of the product's
+// fmcs_cost_primitives callbacks (src/stitch_composer.h:54-72).
+double ref_cb_sort(void *, double g, uint32_t bank_type) {
+  switch (bank_type) {
+    case 0: return StitchComposer::estimate_mergesort_uint16(g);
+    case 1: return StitchComposer::estimate_mergesort_uint32(g);
+    default: return StitchComposer::estimate_mergesort_uint64(g);
+  }
+}
+double ref_cb_stitch(void *, const uint32_t *assemble9, const uint32_t *shift3, uint64_t n) {
+  uint32_t a[3][3], sh[3];
+  std::memcpy(a, assemble9, sizeof(a));
+  std::memcpy(sh, shift3, sizeof(sh));
+  return StitchComposer::estimate_stitch(a, sh, n);  // nitems is uint32_t there, as in its callers
+}
+double ref_cb_reconstruct(void *, const uint32_t *bytewidths, uint32_t nrounds, uint64_t n) {
+  return StitchComposer::estimate_tuple_reconstruct(bytewidths, nrounds, n);
 }
 
 }  // extern "C"

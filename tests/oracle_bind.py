@@ -243,3 +243,109 @@ def canonical_order(full_key_sorted_runs, order):
         if e - s > 1:
             out[s:e] = np.sort(out[s:e])
     return out
+
+
+# ---- the reference's planner (oracle/_ref) ----------------------------------
+
+def _ref_planner_sigs(r):
+    u32, u64, vp, dbl = C.c_uint32, C.c_uint64, C.c_void_p, C.c_double
+    P = C.POINTER
+    sigs = {
+        "ref_planner_new": (vp, [u64, u32, P(u32), P(u64)]),
+        "ref_planner_free": (None, [vp]),
+        "ref_card_stats": (None, [vp, P(u32), u32, P(u32), P(dbl), P(u32), P(u32)]),
+        "ref_group_num": (dbl, [vp, u32, P(u32)]),
+        "ref_column_cost": (dbl, [vp, dbl, u32]),
+        "ref_greedy_plans": (C.c_int, [vp, u32, P(u32), P(u32), u64, P(u64), P(u64)]),
+        "ref_plan_cost": (dbl, [vp, P(u32), u32, P(u32)]),
+        "ref_rrs_cost": (dbl, [u64, u32, P(u32), u32, P(u32), P(u64), u32, P(u32), P(u64), P(u32), P(u32)]),
+        "ref_plan_enum_rrs": (C.c_int, [u64, u32, P(u32), u32, P(u32), P(u64), P(u64), P(u32), u64, P(u64),
+                                        P(u64)]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(r, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def _a32(xs):
+    return (C.c_uint32 * len(xs))(*[int(x) for x in xs])
+
+
+def _a64(xs):
+    return (C.c_uint64 * len(xs))(*[int(x) for x in xs])
+
+
+def _unflatten(buf, words):
+    out, i = [], 0
+    while i < words:
+        k = buf[i]
+        out.append([int(x) for x in buf[i + 1:i + 1 + k]])
+        i += 1 + k
+    return out
+
+
+class RefPlanner:
+    """The reference's StitchComposer planner over columns of the given widths
+    and cardinalities (no values)."""
+
+    def __init__(self, n, bws, cards):
+        r = ref()
+        _ref_planner_sigs(r)
+        self.r, self.n, self.bws, self.cards = r, n, list(bws), list(cards)
+        self.h = r.ref_planner_new(n, len(bws), _a32(bws), _a64(cards))
+
+    def close(self):
+        if self.h:
+            self.r.ref_planner_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def card_stats(self, order, split):
+        card = (C.c_double * len(split))()
+        a = (C.c_uint32 * 9)()
+        s = (C.c_uint32 * 3)()
+        self.r.ref_card_stats(self.h, _a32(order), len(split), _a32(split), card, a, s)
+        return list(card), list(a), list(s)
+
+    def group_num(self, bitpos, order):
+        return self.r.ref_group_num(self.h, bitpos, _a32(order))
+
+    def column_cost(self, g, bank_type):
+        return self.r.ref_column_cost(self.h, g, bank_type)
+
+    def greedy_plans(self, rounds, order):
+        buf = (C.c_uint32 * 100000)()
+        npl, words = C.c_uint64(), C.c_uint64()
+        assert self.r.ref_greedy_plans(self.h, rounds, _a32(order), buf, len(buf), C.byref(npl), C.byref(words)) == 0
+        return _unflatten(buf, words.value)
+
+    def plan_cost(self, order, split):
+        return self.r.ref_plan_cost(self.h, _a32(order), len(split), _a32(split))
+
+    def rrs_cost(self, order, split, max_round):
+        card = (C.c_uint64 * len(split))()
+        a = (C.c_uint32 * 9)()
+        s = (C.c_uint32 * 3)()
+        c = self.r.ref_rrs_cost(self.n, max_round, _a32(order), len(self.bws), _a32(self.bws), _a64(self.cards),
+                                len(split), _a32(split), card, a, s)
+        return c, list(card), list(a), list(s)
+
+    def plan_enum_rrs(self, order, max_round):
+        """(seed the reference's engine took, visited plans)"""
+        buf = (C.c_uint32 * 400000)()
+        seed, npl, words = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        assert self.r.ref_plan_enum_rrs(self.n, max_round, _a32(order), len(self.bws), _a32(self.bws),
+                                        _a64(self.cards), C.byref(seed), buf, len(buf), C.byref(npl),
+                                        C.byref(words)) == 0
+        return seed.value, _unflatten(buf, words.value)
+
+
+def ref_cost_callbacks():
+    """Addresses of the reference's static estimators, in the callback
+    signatures of fmcs_cost_primitives (sort, stitch, reconstruct)."""
+    r = ref()
+    return tuple(C.cast(getattr(r, n), C.c_void_p).value for n in ("ref_cb_sort", "ref_cb_stitch",
+                                                                      "ref_cb_reconstruct"))
