@@ -25,7 +25,8 @@ enum Slot {
   WS_KA, WS_KB, WS_VB, WS_T0K, WS_T0V, WS_T1K, WS_T1V, WS_STATUS, WS_HIST, WS_DBASE, WS_SEGOFF, WS_SEGOFF2,
   WS_ST, WS_SZ, WS_BST, WS_BSZ, WS_FLAG, WS_POS, WS_LLO, WS_LHI, WS_LTILES, WS_TBASE, WS_TLO, WS_THI, WS_TS0,
   WS_TS1, WS_COMP, WS_COMP2, WS_GROUPTMP, WS_MISC, WS_HASH, WS_HASH2, WS_HASH3, WS_HT_TABLE, WS_HT_FIRST,
-  WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_DD, WS_SCANST, WS_FBLIST, WS_TCOUNT, WS_CARRY, WS_BCNT, WS_T0P, WS_T1P, WS_PAY, WS_SGL, WS_FBLIST2, WS_RKL, WS_RKL2, WS_DNEXT, WS_TSEG, WS_USER0,
+  WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_DD, WS_SCANST, WS_FBLIST, WS_TCOUNT, WS_CARRY, WS_BCNT, WS_T0P, WS_T1P, WS_PAY, WS_SGL, WS_FBLIST2, WS_RKL, WS_RKL2, WS_DNEXT, WS_TSEG, WS_HG_KEY, WS_HG_POS, WS_HG_SKEY, WS_HG_SPOS, WS_HG_F, WS_HG_AUX, WS_HG_PSTART,
+  WS_HG_TKEY, WS_HG_TFIRST, WS_HG_CHUNK, WS_USER0,
   WS_STAGE0 = WS_USER0 + 24,  // device copies of staged host buffers (one slot per argument)
   WS_COUNT = WS_STAGE0 + 72
 };
@@ -1528,11 +1529,175 @@ int mrs_graph(fmcs_ctx* ctx, u32 nrounds, const void* const* cols, const u32* ba
   return FMCS_OK;
 }
 
+// hash_ids for composites (seg << 8 bank) | key of at most 64 bits (every
+// round of 1-, 2- and 4-byte banks, and 8-byte banks without segments):
+// the first position of every row's group (F) from LDS tables over chunks,
+// in the local form when the chunks hold few distinct hashes, else
+// partitioned (fmcs_hash.hip); then heads, their scan and the ids.
+// The local form.  Its global table has room for 8x the sampled estimate
+// of the chunk-local distinct hashes; a fuller table (an estimate too low)
+// is reported by pass 1 and the pass runs again with room for every row.
+template <class K>
+int hash_first_local(fmcs_ctx* ctx, const K* keys, const u32* seg, u64 n, u32 kb, u64 distinct, u32* F) {
+  u32* marker = ctx->small_d + 62;
+  u32* overflow = ctx->small_d + 63;
+  const u64 nch = (n + 2047) / 2048;
+  auto kl = k_hash_local<K>;
+  auto kf = k_hash_local_first<K>;
+  const double rd = (double)n * (sizeof(K) + (seg ? 4 : 0));
+  u64 want = std::max<u64>(4096, 8 * distinct);
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    u64 tcap = 1;
+    while (tcap < want) tcap <<= 1;
+    u64* tkeys;
+    u32* tfirst;
+    TRY(getbuf_t(ctx, WS_HG_TKEY, tcap, &tkeys));
+    TRY(getbuf_t(ctx, WS_HG_TFIRST, tcap, &tfirst));
+    TRY(fill(ctx, (u32*)tkeys, 2 * tcap, 0xffffffffu));
+    TRY(fill(ctx, tfirst, tcap, 0xffffffffu));
+    TRY(fill(ctx, marker, 1, ~0u));
+    TRY(fill(ctx, overflow, 1, 0));
+    TRY(klaunch(ctx, "k_hash_local", kl, cap_grid(nch, 2 * resident(ctx, (const void*)kl)), rd, nullptr, 0.0, keys,
+                seg, n, kb, tkeys, tfirst, tcap - 1, marker, ctx->small_d + 61, 1u, 1u, overflow));
+    u32 over = 0;
+    TRY(readback(ctx, overflow, 1, &over));
+    if (!over)
+      return klaunch(ctx, "k_hash_local_first", kf, cap_grid(nch, 2 * resident(ctx, (const void*)kf)), rd + 4.0 * n,
+                     nullptr, 0.0, keys, seg, n, kb, (const u64*)tkeys, (const u32*)tfirst, tcap - 1,
+                     (const u32*)marker, F);
+    want = 2 * n;  // every row distinct fits
+  }
+  return fail(ctx, FMCS_E_HIP, "hash table overflow with room for every row");
+}
+
+template <class K>
+int hash_first_partitioned(fmcs_ctx* ctx, const K* keys, const u32* seg, u64 n, u32 kb, u32* F) {
+  // partitions of about 512 .. 1024 rows on average, whole 8-bit digits
+  const u32 PB = n <= (1ull << 17) ? 8u : (n <= (1ull << 26) ? 16u : 24u);
+  const u32 nparts = 1u << PB;
+  const u32 pw = (u32)std::max<u64>(1, ((u64)nparts * 1024) / n);
+  const u32 nwin = (nparts + pw - 1) / pw;
+  u64 *hk, *hs;
+  u32 *pos, *ps, *large, *pstart;
+  TRY(getbuf_t(ctx, WS_HG_KEY, n, &hk));
+  TRY(getbuf_t(ctx, WS_HG_POS, n, &pos));
+  TRY(getbuf_t(ctx, WS_HG_SKEY, n, &hs));
+  TRY(getbuf_t(ctx, WS_HG_SPOS, n, &ps));
+  TRY(getbuf_t(ctx, WS_HG_PSTART, 2 * (u64)nwin, &large));
+  TRY(getbuf_t(ctx, WS_HG_CHUNK, (u64)nparts + 1, &pstart));
+  const double rd = (double)n * (sizeof(K) + (seg ? 4 : 0));
+  TRY(klaunch(ctx, "k_hash_prep", k_hash_prep<K>, cap_grid((n + 255) / 256, 8192), rd + 12.0 * n, nullptr, 0.0, keys,
+              seg, n, kb, hk, pos));
+  // stable partition of (hash, position) on the hash's low PB bits
+  TRY(radix_sort<u64>(ctx, hk, pos, hs, ps, n, PB));
+  u32* nlarge_d = ctx->small_d + 61;
+  u32* marker = ctx->small_d + 62;
+  TRY(fill(ctx, nlarge_d, 1, 0));
+  TRY(fill(ctx, marker, 1, ~0u));
+  TRY(klaunch(ctx, "k_hash_pstart", k_hash_pstart, cap_grid(((u64)nparts + 256) / 256, 8192), 0.0, nullptr, 0.0,
+              (const u64*)hs, (u32)n, nparts - 1, pstart));
+  TRY(klaunch(ctx, "k_hash_window", k_hash_window, nwin, (double)n * (8 + 4 + 4), nullptr, 0.0, (const u64*)hs,
+              (const u32*)ps, (const u32*)pstart, nparts, pw, F, large, nlarge_d));
+  u32 nlarge = 0;
+  TRY(readback(ctx, nlarge_d, 1, &nlarge));
+  if (nlarge == 0) return FMCS_OK;
+  // windows too large for LDS: chunks of their rows through one global table
+  std::vector<u32> wl(2 * (size_t)nlarge);
+  HIPCHK(hipMemcpyAsync(wl.data(), large, wl.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  std::vector<u32> cl;
+  u64 rows = 0;
+  for (u32 k = 0; k < nlarge; ++k) {
+    const u32 lo = wl[2 * k], hi = wl[2 * k + 1];
+    rows += hi - lo;
+    for (u32 a = lo; a < hi; a += 2048) cl.push_back(a);
+  }
+  std::vector<u32> ch(cl.size());
+  {
+    size_t q = 0;
+    for (u32 k = 0; k < nlarge; ++k)
+      for (u32 a = wl[2 * k]; a < wl[2 * k + 1]; a += 2048) ch[q++] = std::min<u32>(wl[2 * k + 1], a + 2048);
+  }
+  const u32 nch = (u32)cl.size();
+  u64 tcap = 1;
+  while (tcap < 2 * rows) tcap <<= 1;
+  u64* tkeys;
+  u32 *tfirst, *cd;
+  TRY(getbuf_t(ctx, WS_HG_TKEY, tcap, &tkeys));
+  TRY(getbuf_t(ctx, WS_HG_TFIRST, tcap, &tfirst));
+  TRY(getbuf_t(ctx, WS_HG_AUX, 2 * (u64)nch, &cd));  // F's heads come later; the slot is free until then
+  HIPCHK(hipMemcpyAsync(cd, cl.data(), (size_t)nch * 4, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(cd + nch, ch.data(), (size_t)nch * 4, hipMemcpyHostToDevice, ctx->stream));
+  TRY(fill(ctx, (u32*)tkeys, 2 * tcap, 0xffffffffu));
+  TRY(fill(ctx, tfirst, tcap, 0xffffffffu));
+  const u64 g = cap_grid(nch, 2 * resident(ctx, (const void*)k_hash_large));
+  u32* overflow = ctx->small_d + 63;  // cannot be set: the table has room for every row
+  TRY(fill(ctx, overflow, 1, 0));
+  TRY(klaunch(ctx, "k_hash_large", k_hash_large, g, (double)rows * 12, nullptr, 0.0, (const u64*)hs, (const u32*)ps,
+              (const u32*)cd, (const u32*)(cd + nch), nch, tkeys, tfirst, tcap - 1, marker, overflow));
+  TRY(klaunch(ctx, "k_hash_large_first", k_hash_large_first, g, (double)rows * 16, nullptr, 0.0, (const u64*)hs,
+              (const u32*)ps, (const u32*)cd, (const u32*)(cd + nch), nch, (const u64*)tkeys, (const u32*)tfirst,
+              tcap - 1, (const u32*)marker, F));
+  HIPCHK(hipStreamSynchronize(ctx->stream));  // the chunk lists above are host memory
+  return FMCS_OK;
+}
+
+template <class K>
+int hash_ids_packed(fmcs_ctx* ctx, const K* keys, const u32* seg, u64 n, u32* gid, u32* ng) {
+  const u32 kb = 8 * sizeof(K);
+  u32 *F, *aux;
+  TRY(getbuf_t(ctx, WS_HG_F, n, &F));
+  // a sample of about 256 of the 2048-row chunks: the local form's global
+  // work is the chunks' own distinct hashes, so few of them (at most an
+  // eighth of the rows: values repeated many times within a chunk) -> the
+  // local form, its table sized from the sample's distinct count (a table
+  // too small is found and enlarged); else the partitioned one
+  const u64 nch = (n + 2047) / 2048;
+  const u32 stride = (u32)std::max<u64>(1, nch / 256);
+  const u64 sampled = (nch + stride - 1) / stride;
+  const u64 srows = std::min<u64>(n, sampled * 2048);
+  u64 scap = 1;
+  while (scap < 2 * srows) scap <<= 1;
+  u64* stab;
+  TRY(getbuf_t(ctx, WS_HG_TKEY, scap, &stab));
+  TRY(fill(ctx, (u32*)stab, 2 * scap, 0xffffffffu));
+  TRY(fill(ctx, ctx->small_d + 61, 2, 0));
+  auto kl = k_hash_local<K>;
+  TRY(klaunch(ctx, "k_hash_count", kl, cap_grid(sampled, 2 * resident(ctx, (const void*)kl)),
+              (double)srows * (sizeof(K) + (seg ? 4 : 0)), nullptr, 0.0, keys, seg, n, kb, stab, (u32*)nullptr,
+              scap - 1, (u32*)nullptr, ctx->small_d + 61, 0u, stride, (u32*)nullptr));
+  u32 sample[2] = {0, 0};
+  TRY(readback(ctx, ctx->small_d + 61, 2, sample));
+  const u64 distinct = (u64)sample[1] + 1;
+  if ((u64)sample[0] * 8 <= srows)
+    TRY(hash_first_local<K>(ctx, keys, seg, n, kb, distinct, F));
+  else
+    TRY(hash_first_partitioned<K>(ctx, keys, seg, n, kb, F));
+  const u64 grid = cap_grid((n + 255) / 256, 8192);
+  TRY(getbuf_t(ctx, WS_HG_AUX, n, &aux));
+  TRY(klaunch(ctx, "k_hash_heads", k_hash_heads_f, grid, (double)n * 8, nullptr, 0.0, (const u32*)F, n, aux));
+  u32* prefix;
+  TRY(getbuf_t(ctx, WS_HG_POS, n, &prefix));  // the partition positions are dead by now
+  TRY(scan_excl(ctx, aux, prefix, n, ctx->small_d + 56));
+  TRY(klaunch(ctx, "k_hash_assign", k_hash_assign_f, grid, (double)n * 12, nullptr, 0.0, (const u32*)F,
+              (const u32*)prefix, n, gid));
+  return readback(ctx, ctx->small_d + 56, 1, ng);
+}
+
 // First-appearance group ids of (seg[i], keys[i]) over rows in order,
 // numbered 0.. across the whole array (segments contiguous or not).
 int hash_ids(fmcs_ctx* ctx, u32 bank, const void* keys, const u32* seg, u64 n, u32* gid, u32* ng) {
   *ng = 0;
   if (n == 0) return FMCS_OK;
+  if (!seg || bank <= 4) {
+    switch (bank) {
+      case 1: return hash_ids_packed<uint8_t>(ctx, (const uint8_t*)keys, seg, n, gid, ng);
+      case 2: return hash_ids_packed<u16>(ctx, (const u16*)keys, seg, n, gid, ng);
+      case 4: return hash_ids_packed<u32>(ctx, (const u32*)keys, seg, n, gid, ng);
+      default: return hash_ids_packed<u64>(ctx, (const u64*)keys, seg, n, gid, ng);
+    }
+  }
+  // wider composites: one open-addressing table over (segment, key)
   u64 cap = 1;
   while (cap < 2 * n) cap <<= 1;
   if (cap > (1ull << 31)) return fail(ctx, FMCS_E_UNSUPPORTED, "hash table too large");

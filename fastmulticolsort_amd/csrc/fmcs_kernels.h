@@ -117,6 +117,20 @@ __global__ void k_strip_composite(const u64*, u64, u64, K*);
 template <class K>
 __global__ void k_hash_insert(const K*, const u32*, u64, u32*, u32, u32*, u32*);
 __global__ void k_hash_heads(const u32*, const u32*, u64, u32*);
+// hash grouping of composites that fit 64 bits (fmcs_hash.hip)
+template <class K>
+__global__ void k_hash_prep(const K*, const u32*, u64, u32, u64*, u32*);
+template <class K>
+__global__ void k_hash_local(const K*, const u32*, u64, u32, u64*, u32*, u64, u32*, u32*, u32, u32, u32*);
+template <class K>
+__global__ void k_hash_local_first(const K*, const u32*, u64, u32, const u64*, const u32*, u64, const u32*, u32*);
+__global__ void k_hash_pstart(const u64*, u32, u32, u32*);
+__global__ void k_hash_window(const u64*, const u32*, const u32*, u32, u32, u32*, u32*, u32*);
+__global__ void k_hash_large(const u64*, const u32*, const u32*, const u32*, u32, u64*, u32*, u64, u32*, u32*);
+__global__ void k_hash_large_first(const u64*, const u32*, const u32*, const u32*, u32, const u64*, const u32*, u64,
+                                   const u32*, u32*);
+__global__ void k_hash_heads_f(const u32*, u64, u32*);
+__global__ void k_hash_assign_f(const u32*, const u32*, u64, u32*);
 __global__ void k_hash_rank_slots(const u32*, const u32*, const u32*, u64, u32*);
 __global__ void k_hash_assign(const u32*, const u32*, u64, u32*);
 template <class K>
