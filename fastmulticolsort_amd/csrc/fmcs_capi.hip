@@ -344,16 +344,37 @@ int scatter_pass(fmcs_ctx* ctx, const char* name, const K* kin, const u32* vin, 
                  ntiles_d, pin, pout, fshift, pbad, err, kside, dnext, nshift, tseg);
 }
 
-// exclusive scan of n u32 values -> out; *total_h (host) if non-null
-int scan_excl(fmcs_ctx* ctx, const u32* in, u32* out, u64 n, u32* total_d) {
+// exclusive scan of n u32 values -> out, the sum to *total_d (device) if
+// non-null; heads: scan the head flags (in[i] == i) of a first-position
+// array instead of its values
+int scan_excl(fmcs_ctx* ctx, const u32* in, u32* out, u64 n, u32* total_d, bool heads = false) {
   if (n == 0) return FMCS_OK;
   u64 ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
   u32* status;
   TRY(getbuf_t(ctx, WS_STATUS, ntiles, &status));
+  if (n >= (1ull << 21)) {
+    // long arrays: reduce-then-scan over about 1024 chunks of C rows
+    const u64 C = ((n + 1023) / 1024 + 1023) / 1024 * 1024;
+    const u64 G = (n + C - 1) / C;  // <= 1024 <= ntiles
+    u32* sums = status;
+    {
+      Prof pf(ctx, "k_scan_reduce", 4.0 * n);
+      hipLaunchKernelGGL((heads ? k_scan_reduce<true> : k_scan_reduce<false>), dim3((u32)G), dim3(256), 0,
+                         ctx->stream, in, n, C, sums);
+      TRY(launch_check(ctx, "k_scan_reduce"));
+    }
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, ctx->stream, sums, (u32)G, total_d);
+    TRY(launch_check(ctx, "k_scan_sums"));
+    Prof pf(ctx, "k_scan_rescan", 8.0 * n);
+    hipLaunchKernelGGL((heads ? k_scan_rescan<true> : k_scan_rescan<false>), dim3((u32)G), dim3(256), 0,
+                       ctx->stream, in, out, n, C, (const u32*)sums);
+    return launch_check(ctx, "k_scan_rescan");
+  }
   TRY(fill(ctx, status, ntiles, 0));
   TRY(fill(ctx, ctx->small_d + 8, 1, 0));
-  hipLaunchKernelGGL((k_scan_excl<SCAN_ITEMS>), dim3((u32)ntiles), dim3(256), 0, ctx->stream, in, out, n, status,
-                     ctx->small_d + 8, total_d);
+  Prof pf(ctx, "k_scan_excl", 8.0 * n);
+  hipLaunchKernelGGL((heads ? k_scan_excl<SCAN_ITEMS, true> : k_scan_excl<SCAN_ITEMS, false>), dim3((u32)ntiles),
+                     dim3(256), 0, ctx->stream, in, out, n, status, ctx->small_d + 8, total_d);
   return launch_check(ctx, "k_scan_excl");
 }
 
@@ -483,7 +504,7 @@ int merge_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, 
     std::swap(sk, dk);
     std::swap(sv, dv);
   }
-  return klaunch(ctx, "k_copy", kd_copy_pairs<K>, cap_grid((n + 255) / 256, 4096), pass_bytes, nullptr, 0.0,
+  return klaunch(ctx, "kd_copy_pairs", kd_copy_pairs<K>, cap_grid((n + 255) / 256, 4096), pass_bytes, nullptr, 0.0,
                  (const K*)sk, (const u32*)sv, kout, vout, n);
 }
 
@@ -914,10 +935,10 @@ int radix_sort_dd_t(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* v
   const u32 npasses = (keybits + RB - 1) / RB;
   if (n == 1 || npasses == 0) {
     if (HASP && (const void*)pin != (const void*)pout)
-      TRY(klaunch(ctx, "k_copy", kd_copy_col<P>, cap_grid((n + 255) / 256, 4096), 2.0 * n * sizeof(P), nullptr, 0.0,
+      TRY(klaunch(ctx, "kd_copy_col", kd_copy_col<P>, cap_grid((n + 255) / 256, 4096), 2.0 * n * sizeof(P), nullptr, 0.0,
                   pin, pout, n));
     if ((const void*)kout != (const void*)kin || vout != vin)
-      return klaunch(ctx, "k_copy", kd_copy_pairs<K>, 1, 0.0, nullptr, 0.0, kin, vin, kout, vout, n);
+      return klaunch(ctx, "kd_copy_pairs", kd_copy_pairs<K>, 1, 0.0, nullptr, 0.0, kin, vin, kout, vout, n);
     return FMCS_OK;
   }
   const u64 ntiles = (n + ONESWEEP_TILE - 1) / ONESWEEP_TILE;
@@ -983,10 +1004,10 @@ int radix_sort_dd_t(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* v
     sp = dp;
   }
   if ((const void*)sk != (const void*)kout || sv != vout)
-    TRY(klaunch(ctx, "k_copy", kd_copy_pairs<K>, cap_grid((n + 255) / 256, 4096), 2.0 * n * (sizeof(K) + 4), nullptr,
+    TRY(klaunch(ctx, "kd_copy_pairs", kd_copy_pairs<K>, cap_grid((n + 255) / 256, 4096), 2.0 * n * (sizeof(K) + 4), nullptr,
                 0.0, sk, sv, kout, vout, n));
   if (HASP && (const void*)sp != (const void*)pout)
-    TRY(klaunch(ctx, "k_copy", kd_copy_col<P>, cap_grid((n + 255) / 256, 4096), 2.0 * n * sizeof(P), nullptr, 0.0,
+    TRY(klaunch(ctx, "kd_copy_col", kd_copy_col<P>, cap_grid((n + 255) / 256, 4096), 2.0 * n * sizeof(P), nullptr, 0.0,
                 sp, pout, n));
   return FMCS_OK;
 }
@@ -1291,7 +1312,7 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
     TRY(dd_scan(ctx, ltiles, tbase, S + LV_NLARGE, d.nlarge_max, scan_st2, S + LV_SCAN_B, S + LV_NTILES));
     if (d.last) {
       // odd level, every key bit consumed: move the large segments back
-      TRY(klaunch(ctx, "k_copy_segs", kd_copy_segs<K>, cap_grid(d.ntiles_max, 2048), 0.0, S + LV_ROWS_LARGE,
+      TRY(klaunch(ctx, "kd_copy_segs", kd_copy_segs<K>, cap_grid(d.ntiles_max, 2048), 0.0, S + LV_ROWS_LARGE,
                   2.0 * (sizeof(K) + 4), skey, sval, keys, vals, (const u32*)llo, (const u32*)lhi, (const u32*)tbase,
                   (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES), (u32)ONESWEEP_TILE));
       break;
@@ -1645,7 +1666,7 @@ int hash_first_partitioned(fmcs_ctx* ctx, const K* keys, const u32* seg, u64 n, 
 template <class K>
 int hash_ids_packed(fmcs_ctx* ctx, const K* keys, const u32* seg, u64 n, u32* gid, u32* ng) {
   const u32 kb = 8 * sizeof(K);
-  u32 *F, *aux;
+  u32* F;
   TRY(getbuf_t(ctx, WS_HG_F, n, &F));
   // a sample of about 256 of the 2048-row chunks: the local form's global
   // work is the chunks' own distinct hashes, so few of them (at most an
@@ -1674,12 +1695,11 @@ int hash_ids_packed(fmcs_ctx* ctx, const K* keys, const u32* seg, u64 n, u32* gi
   else
     TRY(hash_first_partitioned<K>(ctx, keys, seg, n, kb, F));
   const u64 grid = cap_grid((n + 255) / 256, 8192);
-  TRY(getbuf_t(ctx, WS_HG_AUX, n, &aux));
-  TRY(klaunch(ctx, "k_hash_heads", k_hash_heads_f, grid, (double)n * 8, nullptr, 0.0, (const u32*)F, n, aux));
   u32* prefix;
   TRY(getbuf_t(ctx, WS_HG_POS, n, &prefix));  // the partition positions are dead by now
-  TRY(scan_excl(ctx, aux, prefix, n, ctx->small_d + 56));
-  TRY(klaunch(ctx, "k_hash_assign", k_hash_assign_f, grid, (double)n * 12, nullptr, 0.0, (const u32*)F,
+  // heads (F[p] == p) numbered by the scan itself
+  TRY(scan_excl(ctx, F, prefix, n, ctx->small_d + 56, true));
+  TRY(klaunch(ctx, "k_hash_assign_f", k_hash_assign_f, grid, (double)n * 12, nullptr, 0.0, (const u32*)F,
               (const u32*)prefix, n, gid));
   return readback(ctx, ctx->small_d + 56, 1, ng);
 }
@@ -2093,8 +2113,17 @@ int fmcs_stitch_dir(fmcs_ctx* ctx, uint64_t n, uint32_t nsrc, const void* const*
   if (n == 0) return FMCS_OK;
   u64 threads = (n + 3) / 4;
   u32 blocks = (u32)((threads + 255) / 256);
-  hipLaunchKernelGGL(k_stitch, dim3(blocks), dim3(256), 0, ctx->stream, a);
-  TRY(launch_check(ctx, "k_stitch"));
+  {
+    // every source column that an entry names read once, every stitched column written once
+    u64 used = 0;
+    for (u32 i = 0; i < nent; ++i) used |= 1ull << entries[i].src;
+    double row_bytes = 0;
+    for (u32 s = 0; s < nsrc; ++s) row_bytes += ((used >> s) & 1) ? src_bank[s] : 0;
+    for (u32 c = 0; c < ndst; ++c) row_bytes += dst_bank[c];
+    Prof pf(ctx, "k_stitch", row_bytes * n);
+    hipLaunchKernelGGL(k_stitch, dim3(blocks), dim3(256), 0, ctx->stream, a);
+    TRY(launch_check(ctx, "k_stitch"));
+  }
   return st.finish();
 }
 

@@ -7,7 +7,8 @@
 // same composite: no key is ever compared through a row index.  Every
 // grouping step below produces, for every position p, the first position
 // F[p] of p's group; then p is a group head when F[p] == p, ids are the
-// exclusive scan of the heads, and gid[p] = prefix[F[p]].
+// exclusive scan of the heads (k_scan_excl's head mode reads F directly),
+// and gid[p] = prefix[F[p]].
 //
 // F comes from LDS hash tables over chunks of 2048 rows, one of two ways:
 //  * local: chunks of consecutive positions.  Each chunk's distinct hashes
@@ -380,12 +381,6 @@ __global__ __launch_bounds__(256) void k_hash_large_first(const u64* __restrict_
       if (slot[e] != HG_NONE) F[pos[e]] = slot[e] == HG_TS ? *marker : s_first[slot[e]];
     __syncthreads();
   }
-}
-
-// heads[p] = 1 when p is the first position of its group.
-__global__ __launch_bounds__(256) void k_hash_heads_f(const u32* __restrict__ F, u64 n, u32* __restrict__ heads) {
-  for (u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (u64)gridDim.x * blockDim.x)
-    heads[p] = F[p] == (u32)p ? 1u : 0u;
 }
 
 // gid[p] = the rank of p's group head among the heads.

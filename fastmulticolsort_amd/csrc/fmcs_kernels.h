@@ -104,8 +104,13 @@ __global__ void k_large_flags(const u32*, u32, u32, u32*, u32*);
 __global__ void k_large_compact(const u32*, const u32*, u32, u32, const u32*, u32, u32*, u32*, u32*);
 template <class K>
 __global__ void k_copy_segs(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, u32, u32);
-template <int ITEMS>
+template <int ITEMS, bool HEADS>
 __global__ void k_scan_excl(const u32*, u32*, u64, u32*, u32*, u32*);
+template <bool HEADS>
+__global__ void k_scan_reduce(const u32*, u64, u64, u32*);
+__global__ void k_scan_sums(u32*, u32, u32*);
+template <bool HEADS>
+__global__ void k_scan_rescan(const u32*, u32*, u64, u64, const u32*);
 __global__ void k_fill_u32(u32*, u64, u32);
 __global__ void k_iota_u32(u32*, u64);
 template <class K>
@@ -129,7 +134,6 @@ __global__ void k_hash_window(const u64*, const u32*, const u32*, u32, u32, u32*
 __global__ void k_hash_large(const u64*, const u32*, const u32*, const u32*, u32, u64*, u32*, u64, u32*, u32*);
 __global__ void k_hash_large_first(const u64*, const u32*, const u32*, const u32*, u32, const u64*, const u32*, u64,
                                    const u32*, u32*);
-__global__ void k_hash_heads_f(const u32*, u64, u32*);
 __global__ void k_hash_assign_f(const u32*, const u32*, u64, u32*);
 __global__ void k_hash_rank_slots(const u32*, const u32*, const u32*, u64, u32*);
 __global__ void k_hash_assign(const u32*, const u32*, u64, u32*);

@@ -1348,31 +1348,63 @@ __global__ void k_copy_segs(const K* __restrict__ kin, const u32* __restrict__ v
 
 // ---------------------------------------------------------------------------
 // Device scan (exclusive, u32) in one sweep with decoupled look-back; total
-// written to *total.  Values must sum below 2^30.
+// written to *total.  Values must sum below 2^30.  A tile of 256 * ITEMS
+// values is ITEMS / 4 slices of 1024; a thread holds 4 consecutive values of
+// each slice, read and written as one 16-byte vector (lane-contiguous, so
+// every load instruction covers 1 KiB without gaps), and each slice is
+// scanned across the block.  HEADS scans the head flags of a first-position
+// array instead of the input values: value i is 1 when in[i] == i.
 // ---------------------------------------------------------------------------
-template <int ITEMS>
+template <int ITEMS, bool HEADS>
 __global__ __launch_bounds__(256) void k_scan_excl(const u32* __restrict__ in, u32* __restrict__ out, u64 n,
                                                    u32* __restrict__ status, u32* __restrict__ counter,
                                                    u32* __restrict__ total) {
-  constexpr int NT = 256, TILE = NT * ITEMS;
+  constexpr int NT = 256, NW = NT / WAVE, V = ITEMS / 4, SLICE = NT * 4, TILE = NT * ITEMS;
+  static_assert(ITEMS % 4 == 0, "whole 16-byte vectors per thread");
   __shared__ u32 s_tile, s_excl;
-  __shared__ u32 s_tmp[NT / WAVE + 1];
-  u32 tid = threadIdx.x;
+  __shared__ u32 s_wave[V][NW];
+  const u32 tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
   if (tid == 0) s_tile = atomicAdd(counter, 1u);
   __syncthreads();
-  u32 tile = s_tile;
-  u64 base = (u64)tile * TILE + (u64)tid * ITEMS;
-  u32 v[ITEMS];
-  u32 sum = 0;
+  const u32 tile = s_tile;
+  const u64 t0 = (u64)tile * TILE;
+  const bool vec = ((((uintptr_t)in | (uintptr_t)out) & 15) == 0) && t0 + TILE <= n;
+  u32 v[V][4], incl[V];
 #pragma unroll
-  for (int j = 0; j < ITEMS; ++j) {
-    v[j] = (base + j < n) ? in[base + j] : 0u;
-    sum += v[j];
+  for (int s = 0; s < V; ++s) {
+    const u64 base = t0 + (u64)s * SLICE + tid * 4;
+    if (vec) {
+      const uint4 q = *reinterpret_cast<const uint4*>(in + base);
+      v[s][0] = q.x; v[s][1] = q.y; v[s][2] = q.z; v[s][3] = q.w;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[s][j] = base + j < n ? in[base + j] : 0u;
+    }
+    u32 sum = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (HEADS) v[s][j] = (base + j < n && v[s][j] == (u32)(base + j)) ? 1u : 0u;
+      sum += v[s][j];
+    }
+    incl[s] = wave_incl_scan(sum) - sum;  // exclusive within the wave from here on
+    if (lane == WAVE - 1) s_wave[s][wv] = incl[s] + sum;
   }
-  u32 tot;
-  u32 ex = block_excl_scan<NT>(sum, s_tmp, &tot);
+  __syncthreads();
+  // slice offsets inside the tile and the tile's total
+  u32 tot = 0;
+#pragma unroll
+  for (int s = 0; s < V; ++s) {
+    u32 before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const u32 c = s_wave[s][w];
+      before += (u32)w < wv ? c : 0u;
+      all += c;
+    }
+    incl[s] += tot + before;
+    tot += all;
+  }
   if (tid < WAVE) {
-    const u32 lane = tid;
     u32 e = 0;
     if (tile == 0) {
       if (lane == 0) status_store(status, ST_INC | tot);
@@ -1383,15 +1415,138 @@ __global__ __launch_bounds__(256) void k_scan_excl(const u32* __restrict__ in, u
     }
     if (lane == 0) {
       s_excl = e;
-      if ((u64)(tile + 1) * TILE >= n && total) *total = e + tot;
+      if (t0 + TILE >= n && total) *total = e + tot;
     }
   }
   __syncthreads();
-  u32 run = s_excl + ex;
+  const u32 e = s_excl;
 #pragma unroll
-  for (int j = 0; j < ITEMS; ++j) {
-    if (base + j < n) out[base + j] = run;
+  for (int s = 0; s < V; ++s) {
+    const u64 base = t0 + (u64)s * SLICE + tid * 4;
+    u32 o[4];
+    u32 run = e + incl[s];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      o[j] = run;
+      run += v[s][j];
+    }
+    if (vec) {
+      *reinterpret_cast<uint4*>(out + base) = make_uint4(o[0], o[1], o[2], o[3]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (base + j < n) out[base + j] = o[j];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Reduce-then-scan for long arrays: no block waits on another and no tile id
+// comes from a global atomic (one counter serialises its atomics, about
+// 11 ns each on gfx950, which caps a one-pass scan of 2^25 values at 16384
+// tiles near 200 us).  Block b owns rows [b*C, min(n, (b+1)*C)), C a
+// multiple of 1024; a thread reads 4 consecutive values per 1024-row step
+// (one 16-byte vector when the buffers are 16-byte aligned).  HEADS: value i
+// is 1 when in[i] == i (the head flags of a first-position array).
+// ---------------------------------------------------------------------------
+template <bool HEADS>
+__device__ __forceinline__ void scan_quad(const u32* __restrict__ in, u64 i, u64 end, bool vec, u32 v[4]) {
+  if (vec && i + 4 <= end) {
+    const uint4 q = *reinterpret_cast<const uint4*>(in + i);
+    v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = i + j < end ? in[i + j] : 0u;
+  }
+  if (HEADS) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = (i + j < end && v[j] == (u32)(i + j)) ? 1u : 0u;
+  }
+}
+
+template <bool HEADS>
+__global__ __launch_bounds__(256) void k_scan_reduce(const u32* __restrict__ in, u64 n, u64 C, u32* __restrict__ sums) {
+  const u64 b0 = (u64)blockIdx.x * C, end = min(n, b0 + C);
+  const bool vec = (((uintptr_t)in) & 15) == 0;
+  u32 acc = 0;
+  for (u64 i = b0 + threadIdx.x * 4; i < end; i += 1024) {
+    u32 v[4];
+    scan_quad<HEADS>(in, i, end, vec, v);
+    acc += v[0] + v[1] + v[2] + v[3];
+  }
+  for (int o = WAVE / 2; o > 0; o >>= 1) acc += __shfl_down(acc, o);
+  __shared__ u32 s[256 / WAVE];
+  if ((threadIdx.x & (WAVE - 1)) == 0) s[threadIdx.x / WAVE] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u32 t = 0;
+    for (int w = 0; w < 256 / WAVE; ++w) t += s[w];
+    sums[blockIdx.x] = t;
+  }
+}
+
+// Exclusive scan of G <= 4096 block sums in place (one block of 256), the
+// grand total to *total if set.
+__global__ __launch_bounds__(256) void k_scan_sums(u32* __restrict__ sums, u32 G, u32* __restrict__ total) {
+  __shared__ u32 s_tmp[256 / WAVE + 1];
+  u32 v[16];
+  u32 sum = 0;
+  const u32 b = threadIdx.x * 16;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    v[j] = b + j < G ? sums[b + j] : 0u;
+    sum += v[j];
+  }
+  u32 tot;
+  u32 run = block_excl_scan<256>(sum, s_tmp, &tot);
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    if (b + j < G) sums[b + j] = run;
     run += v[j];
+  }
+  if (threadIdx.x == 0 && total) *total = tot;
+}
+
+template <bool HEADS>
+__global__ __launch_bounds__(256) void k_scan_rescan(const u32* __restrict__ in, u32* __restrict__ out, u64 n, u64 C,
+                                                     const u32* __restrict__ offs) {
+  constexpr int NW = 256 / WAVE;
+  __shared__ u32 s_w[2][NW];
+  const u32 tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
+  const u64 b0 = (u64)blockIdx.x * C, end = min(n, b0 + C);
+  const bool vec = ((((uintptr_t)in) | ((uintptr_t)out)) & 15) == 0;
+  u32 run = offs[blockIdx.x];
+  int par = 0;
+  // every thread runs the same number of steps (C is a multiple of 1024);
+  // the wave totals alternate between two LDS rows, so one barrier a step
+  for (u64 i0 = b0; i0 < end; i0 += 1024, par ^= 1) {
+    const u64 i = i0 + tid * 4;
+    u32 v[4];
+    scan_quad<HEADS>(in, i, end, vec, v);
+    const u32 sum = v[0] + v[1] + v[2] + v[3];
+    const u32 ex = wave_incl_scan(sum) - sum;
+    if (lane == WAVE - 1) s_w[par][wv] = ex + sum;
+    __syncthreads();
+    u32 before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const u32 c = s_w[par][w];
+      before += (u32)w < wv ? c : 0u;
+      all += c;
+    }
+    u32 o[4];
+    o[0] = run + before + ex;
+    o[1] = o[0] + v[0];
+    o[2] = o[1] + v[1];
+    o[3] = o[2] + v[2];
+    if (vec && i + 4 <= end) {
+      *reinterpret_cast<uint4*>(out + i) = make_uint4(o[0], o[1], o[2], o[3]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (i + j < end) out[i + j] = o[j];
+    }
+    run += all;
   }
 }
 
@@ -1614,6 +1769,11 @@ template __global__ void k_hash_insert<uint8_t>(const uint8_t*, const u32*, u64,
 template __global__ void k_gather<uint8_t>(const uint8_t*, const u32*, uint8_t*, u64);
 template __global__ void k_fill_random<uint8_t>(uint8_t*, u64, u64, u32);
 FMCS_INST_K(u64)
-template __global__ void k_scan_excl<8>(const u32*, u32*, u64, u32*, u32*, u32*);
+template __global__ void k_scan_excl<8, false>(const u32*, u32*, u64, u32*, u32*, u32*);
+template __global__ void k_scan_excl<8, true>(const u32*, u32*, u64, u32*, u32*, u32*);
+template __global__ void k_scan_reduce<false>(const u32*, u64, u64, u32*);
+template __global__ void k_scan_reduce<true>(const u32*, u64, u64, u32*);
+template __global__ void k_scan_rescan<false>(const u32*, u32*, u64, u64, const u32*);
+template __global__ void k_scan_rescan<true>(const u32*, u32*, u64, u64, const u32*);
 
 }  // namespace fmcs

@@ -1,7 +1,8 @@
 """GPU measurements of the path's kernels on the other workloads, beside
 bench.py's headline (the batched segmented sort, config 3).
 
-    python -m fastmulticolsort_amd.kernel_bench [--configs 1,2,4] > out.jsonl
+    python -m fastmulticolsort_amd.kernel_bench [--configs 1,2,4] [--plans "16,26;42"|none]
+        [--hash-types 1,0|none] [--copy-probe] > out.jsonl
 
 Prints one JSON object per measurement:
   * config 1 (2^24 rows, c1 17-bit u32 over 8192 codes, c2 33-bit u64): every
@@ -16,7 +17,13 @@ Prints one JSON object per measurement:
     with its exact group count.
 Every result is checked (sorted order / group count) before it is printed.
 Kernel times are HIP-event timers on the context's stream; `achieved` is
-algorithmic bytes / kernel time against the 8 TB/s HBM peak.
+algorithmic bytes / kernel time against the 8 TB/s HBM peak.  Each kernel
+row also carries `calls_in_process` and `bytes_in_process`: the launches
+and algorithmic bytes of every call of the measured function in this
+process (one warm-up, the timed calls, the profiled calls), which is what a
+rocprofv3 trace of the same command sees (`rocprof_summary roofline` joins
+the two).  `--copy-probe` starts with one plain 256 MiB device copy, the
+first copy dispatch of the process, the reference for PMC traffic counters.
 """
 import argparse
 import json
@@ -39,11 +46,14 @@ def _emit(d):
 
 def _kernel_rows(stats, reps):
     out = {}
+    calls = 1 + 2 * reps  # warm-up, timed and profiled calls of the function
     for k, v in stats.items():
         ms = v["ms"] / max(1, reps)
         gbs = v["bytes"] / (v["ms"] / 1e3) / 1e9 if v["ms"] > 0 and v["bytes"] > 0 else None
         out[k] = {"ms": round(ms, 4), "launches": v["launches"] // max(1, reps),
-                  "GBs": round(gbs, 1) if gbs else None, "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None}
+                  "GBs": round(gbs, 1) if gbs else None, "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None,
+                  "calls_in_process": v["launches"] // max(1, reps) * calls,
+                  "bytes_in_process": int(v["bytes"] // max(1, reps) * calls)}
     return out
 
 
@@ -142,7 +152,7 @@ def config2(ctx, n=1 << 26):
            "ms": round(ms, 3), "ok": ok, "achieved_GBs": round(gbytes / (ms / 1e3) / 1e9, 1), "kernels": kern})
 
 
-def config4(ctx, n=1 << 25, max_rrs=24):
+def config4(ctx, n=1 << 25, max_rrs=24, plans=None, hash_types=(1, 0)):
     import torch
     from . import datagen, planner
     from .engine import MODE_DEVICE
@@ -152,8 +162,9 @@ def config4(ctx, n=1 << 25, max_rrs=24):
     exp = np.lexsort(tuple(reversed(stacked))).astype(np.uint32)
     model = planner.default_cost_model()
     card = [datagen.exact_cardinality(c) for c in cols]
-    plans = [[42]] + planner.split_exhaustive(42, 2, 64) + \
-        planner.plan_enum_rrs(model, n, bws, card, [0, 1, 2, 3], 6, 1)[:max_rrs]
+    if plans is None:
+        plans = [[42]] + planner.split_exhaustive(42, 2, 64) + \
+            planner.plan_enum_rrs(model, n, bws, card, [0, 1, 2, 3], 6, 1)[:max_rrs]
     seen, uniq = set(), []
     for p in plans:
         t = tuple(int(w) for w in p)
@@ -163,19 +174,24 @@ def config4(ctx, n=1 << 25, max_rrs=24):
     order = torch.empty(n, dtype=torch.uint32, device="cuda")
     rows = []
     for split in uniq:
-        ms, _ = _timed(ctx, lambda: planner.run_plan(ctx, dcols, bws, [0, 1, 2, 3], split, mode=MODE_DEVICE,
-                                                     order=order), reps=2)
+        ms, kern = _timed(ctx, lambda: planner.run_plan(ctx, dcols, bws, [0, 1, 2, 3], split, mode=MODE_DEVICE,
+                                                        order=order), reps=2)
         rows.append({"split": split, "ms": round(ms, 3), "ok": bool(np.array_equal(order.cpu().numpy(), exp))})
+        if len(uniq) <= 4:
+            rows[-1]["kernels"] = kern
         print(f"config 4 plan {split}: {ms:.3f} ms ok={rows[-1]['ok']}", file=sys.stderr, flush=True)
-    best = min(rows, key=lambda r: r["ms"])
-    _emit({"config": 4, "what": "ORDER BY over 1-, 2-round and seeded PlanEnumRRS plans, end to end",
-           "rows": n, "plans": len(rows), "all_ok": all(r["ok"] for r in rows), "best": best,
-           "median_ms": float(np.median([r["ms"] for r in rows])),
-           "best_Mrows_s": round(n / best["ms"] / 1e3, 1), "plans_run": rows})
+    if rows:
+        best = {k: v for k, v in min(rows, key=lambda r: r["ms"]).items() if k != "kernels"}
+        _emit({"config": 4, "what": "ORDER BY over 1-, 2-round and seeded PlanEnumRRS plans, end to end",
+               "rows": n, "plans": len(rows), "all_ok": all(r["ok"] for r in rows), "best": best,
+               "median_ms": float(np.median([r["ms"] for r in rows])),
+               "best_Mrows_s": round(n / best["ms"] / 1e3, 1), "plans_run": rows})
+    if not hash_types:
+        return
     srt = [s[exp] for s in stacked]
     exact = 1 + int(np.count_nonzero(np.any(np.stack([np.diff(s) != 0 for s in srt]), axis=0)))
     res = {}
-    for ht in (1, 0):
+    for ht in hash_types:
         box = {}
 
         def run():
@@ -190,12 +206,32 @@ def config4(ctx, n=1 << 25, max_rrs=24):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--configs", default="1,2,4")
+    ap.add_argument("--plans", default=None,
+                    help='config 4: the splits to run, "16,26;42" (default: 1- and 2-round and seeded RRS plans), or none')
+    ap.add_argument("--hash-types", default="1,0", help="config 4: hash_type values to run, or none")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="start with one plain 256 MiB device copy (the traffic reference for PMC counters)")
     args = ap.parse_args(argv)
+    if args.copy_probe:
+        import torch
+        a = torch.empty(1 << 28, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+        print("copy probe: 268435456 bytes read + 268435456 bytes written", file=sys.stderr, flush=True)
+    plans = None
+    if args.plans is not None:
+        plans = [] if args.plans == "none" else [[int(w) for w in p.split(",")] for p in args.plans.split(";")]
+    hts = [] if args.hash_types == "none" else [int(x) for x in args.hash_types.split(",")]
     from .engine import Context
     ctx = Context(0)
     for c in [int(x) for x in args.configs.split(",") if x]:
         t0 = time.time()
-        {1: config1, 2: config2, 4: config4}[c](ctx)
+        if c == 4:
+            config4(ctx, plans=plans, hash_types=hts)
+        else:
+            {1: config1, 2: config2}[c](ctx)
         print(f"config {c} done in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
 
 

@@ -2,6 +2,8 @@
 
     python -m fastmulticolsort_amd.rocprof_summary stats DIR CMD > kernel_stats.md
     python -m fastmulticolsort_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR CMD_F CMD_W > traffic.json
+    python -m fastmulticolsort_amd.rocprof_summary roofline TRACE_DIR FETCH_DIR WRITE_DIR BENCH_JSONL \
+        CMD_T CMD_F CMD_W > kernels.md
 
 `stats` reads DIR/*kernel_stats.csv (--kernel-trace --stats), or the
 rocpd database DIR/*.db that rocprofv3 writes by default, and prints a
@@ -10,6 +12,12 @@ per-dispatch FETCH_SIZE and WRITE_SIZE counters (KiB) of two --pmc runs and
 scales each by what the same counter reads for the plain device copy the
 run includes (bench.py --copy-probe: 256 MiB read + 256 MiB written by
 hipMemcpy's copy kernel); the result is HBM bytes per dispatch by kernel.
+`roofline` joins a kernel trace and the two counter runs of a
+`kernel_bench --copy-probe` command with the algorithmic bytes that the
+command printed (`bytes_in_process` / `calls_in_process` of every kernel
+row): per kernel, the traced average duration, algorithmic bytes per
+launch, their ratio against the 8 TB/s HBM peak, and the counted HBM bytes
+per dispatch.  There the probe is the process's first copy dispatch.
 """
 import csv
 import glob
@@ -64,21 +72,107 @@ def stats(d, cmd):
     return "\n".join(out)
 
 
-def _pmc(d):
+def _pmc(d, with_id=False):
+    """Per kernel: (grid size, counter value) per dispatch, or (dispatch id,
+    value) with `with_id`."""
     per = defaultdict(list)
     csvs = sorted(glob.glob(os.path.join(d, "*counter_collection.csv")))
     if csvs:
         for r in csv.DictReader(open(csvs[0])):
-            per[short(r["Kernel_Name"])].append((int(r["Grid_Size"]), float(r["Counter_Value"])))
+            first = int(r["Dispatch_Id"]) if with_id else int(r["Grid_Size"])
+            per[short(r["Kernel_Name"])].append((first, float(r["Counter_Value"])))
         return per
     db = _db(d)
     if db is None:
         raise SystemExit(f"no counter collection under {d}")
     # one row per dispatch and counter; a counter summed over its instances
-    for name, grid, val in db.execute("select kernel_name, grid_size, sum(value) from counters_collection "
-                                      "group by dispatch_id, counter_name"):
-        per[short(name)].append((int(grid), float(val)))
+    col = "dispatch_id" if with_id else "grid_size"
+    for name, first, val in db.execute(f"select kernel_name, {col}, sum(value) from counters_collection "
+                                       "group by dispatch_id, counter_name"):
+        per[short(name)].append((int(first), float(val)))
     return per
+
+
+def base(name):
+    return short(name).split("<")[0].strip()
+
+
+# The library labels some launches by their role; the kernel each runs
+LABEL_KERNEL = {"k_onesweep_msd": "k_onesweep", "k_onesweep_fuse": "k_onesweep", "k_tile_hist": "kd_seg_hist",
+                "k_seg_hist": "kd_seg_hist", "k_tile_bucket_2t": "k_tile_bucket", "k_hash_count": "k_hash_local"}
+
+
+def _algorithmic(jsonl):
+    """Launches and algorithmic bytes per kernel over the process, from the
+    kernel rows of a kernel_bench output."""
+    calls, nbytes = defaultdict(int), defaultdict(int)
+
+    def walk(o):
+        if isinstance(o, dict):
+            if "calls_in_process" in o and "bytes_in_process" in o:
+                return True
+            for k, v in o.items():
+                if k == "kernels" and isinstance(v, dict):
+                    for kn, row in v.items():
+                        kn = LABEL_KERNEL.get(kn, kn)
+                        calls[kn] += int(row["calls_in_process"])
+                        nbytes[kn] += int(row["bytes_in_process"])
+                else:
+                    walk(v)
+        elif isinstance(o, list):
+            for v in o:
+                walk(v)
+
+    for line in open(jsonl):
+        line = line.strip()
+        if line.startswith("{"):
+            walk(json.loads(line))
+    return calls, nbytes
+
+
+def roofline(tdir, fdir, wdir, jsonl, cmd_t, cmd_f, cmd_w, peak_gbs=8000.0):
+    calls, nbytes = _algorithmic(jsonl)
+    dur, cnt = defaultdict(float), defaultdict(int)
+    for name, c, tot, _, _ in kernel_rows(tdir):
+        dur[base(name)] += tot
+        cnt[base(name)] += c
+    f, w = _pmc(fdir, True), _pmc(wdir, True)
+    fc = min(f.get(COPY_KERNEL, [(0, 0.0)]))[1]
+    wc = min(w.get(COPY_KERNEL, [(0, 0.0)]))[1]
+    f_scale = COPY_BYTES / (fc * 1024) if fc else None
+    w_scale = COPY_BYTES / (wc * 1024) if wc else None
+    fb, wb = defaultdict(list), defaultdict(list)
+    for k, v in f.items():
+        fb[base(k)] += [x for _, x in v]
+    for k, v in w.items():
+        wb[base(k)] += [x for _, x in v]
+    out = [f"Kernel trace: `{cmd_t}`", "", f"FETCH_SIZE: `{cmd_f}`", "", f"WRITE_SIZE: `{cmd_w}`", "",
+           f"Counter scale from the first plain copy dispatch (256 MiB read + 256 MiB written): FETCH_SIZE read "
+           f"{fc:.0f} KiB (scale {f_scale or 0:.3f}), WRITE_SIZE read {wc:.0f} KiB (scale {w_scale or 0:.3f}).",
+           "",
+           "Per kernel (all template instances together): traced calls and average duration; algorithmic "
+           "bytes per launch from the library's accounting (`calls` there must equal the traced calls); "
+           "`frac` = algorithmic bytes / average duration / 8 TB/s; HBM = scaled FETCH_SIZE + WRITE_SIZE per "
+           "dispatch; `HBM/alg` > 1 means bytes moved beyond the algorithm's.", "",
+           "| kernel | traced calls | library calls | avg us | alg MB/launch | GB/s | frac | HBM MB/dispatch | HBM/alg |",
+           "|---|---|---|---|---|---|---|---|---|"]
+    for k in sorted(dur, key=lambda x: -dur[x]):
+        if k == COPY_KERNEL and k not in calls:
+            label = k + " (copies)"
+        else:
+            label = k
+        avg = dur[k] / cnt[k]
+        alg = nbytes[k] / calls[k] if calls.get(k) else None
+        gbs = alg / avg if alg else None  # bytes per ns = GB/s
+        hbm = None
+        if fb.get(k) and wb.get(k) and f_scale and w_scale:
+            hbm = (sum(fb[k]) / len(fb[k]) * f_scale + sum(wb[k]) / len(wb[k]) * w_scale) * 1024
+        out.append("| {} | {} | {} | {:.1f} | {} | {} | {} | {} | {} |".format(
+            label, cnt[k], calls.get(k, "—"), avg / 1e3,
+            f"{alg / 1e6:.1f}" if alg else "—", f"{gbs:.0f}" if gbs else "—",
+            f"{gbs / peak_gbs:.3f}" if gbs else "—", f"{hbm / 1e6:.1f}" if hbm else "—",
+            f"{hbm / alg:.2f}" if hbm and alg else "—"))
+    return "\n".join(out)
 
 
 def traffic(fdir, wdir, cmd_f, cmd_w):
@@ -110,6 +204,8 @@ def main(argv):
         print(stats(argv[1], argv[2]))
     elif argv[0] == "traffic":
         print(json.dumps(traffic(argv[1], argv[2], argv[3], argv[4]), indent=1))
+    elif argv[0] == "roofline":
+        print(roofline(*argv[1:8]))
     else:
         raise SystemExit(__doc__)
 

@@ -173,3 +173,21 @@ def test_local_form_table_overflow_runs_again(ctx):
     gid, _, _, ng = ctx.hash_round(dev(k), None, last_round=True)
     eg, en = first_appearance_ids(k)
     assert ng == en and np.array_equal(host(gid), eg)
+
+
+@pytest.mark.parametrize("n", [(1 << 21) + 5, 1 << 22])
+def test_long_scans_number_the_groups_in_both_forms(ctx, n):
+    # from 2^21 rows the heads are numbered by reduce-then-scan: the packed
+    # form scans its head flags (F[p] == p) directly, the chained form (u64
+    # keys under segments) scans a flag array; a ragged tail ends the last chunk
+    print("long scans", n, flush=True)
+    k32 = datagen.uniform_below(51, n, 1 << 20).astype(np.uint32)
+    gid, _, _, ng = ctx.hash_round(dev(k32), None, last_round=True)
+    eg, en = first_appearance_ids(k32)
+    assert ng == en and np.array_equal(host(gid), eg)
+    seg = datagen.uniform_below(52, n, 3).astype(np.uint32)
+    k64 = datagen.uniform_below(53, n, 1 << 18).astype(np.uint64) << np.uint64(40)
+    order = torch.arange(n, dtype=torch.int64).to(torch.int32).view(torch.uint32).cuda()
+    gid, _, _, ng = ctx.hash_round(dev(k64), order, in_group=dev(seg), last_round=True)
+    eg, en = first_appearance_ids(k64, seg)
+    assert ng == en and np.array_equal(host(gid), eg)
