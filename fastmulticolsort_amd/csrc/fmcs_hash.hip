@@ -201,10 +201,14 @@ __global__ __launch_bounds__(256) void k_hash_local(const K* __restrict__ keys, 
       ++claimed;  // the marker composite: once per sampled chunk holding it (an estimate)
       if (insert) atomicMin(marker, s_marker);
     }
-    if (mine) atomicAdd(&s_count, mine);
-    if (!insert && claimed) atomicAdd(distinct + 1, claimed);
+    if (!insert) {
+      // the counts only sample (a few hundred chunks): one counter word
+      // taking an atomic per chunk of the whole array would serialise
+      if (mine) atomicAdd(&s_count, mine);
+      if (claimed) atomicAdd(distinct + 1, claimed);
+    }
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(distinct, s_count);
+    if (!insert && threadIdx.x == 0) atomicAdd(distinct, s_count);
     __syncthreads();
   }
 }
