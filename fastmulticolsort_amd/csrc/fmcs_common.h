@@ -25,6 +25,10 @@ __device__ __forceinline__ void status_store(u32* p, u32 v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Truth table of a | (b ^ c) for v_bitop3_b32 (bit (a << 2 | b << 1 | c) of
+// the table is the result for those input bits).
+constexpr unsigned BITOP3_OR_XOR = 0xF6;
+
 __device__ __forceinline__ u64 lanemask_lt() {
   u32 lane = __lane_id();
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));

@@ -329,7 +329,7 @@ __global__ __launch_bounds__(256) void kd_seg_hist(const K* __restrict__ keys, c
     const u64 rowend = min((u64)seg_hi[seg], row0 + TILE);
     if (VEC && rowend - row0 == (u64)TILE && ((uintptr_t)(keys + row0) & 15) == 0) {
       const uint4* vp = (const uint4*)(keys + row0);
-      uint4 x[NV];
+      uint4 x[VEC ? NV : 1];
 #pragma unroll
       for (int v = 0; v < NV; ++v) x[v] = vp[v * 256 + threadIdx.x];
 #pragma unroll

@@ -427,14 +427,15 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
   constexpr int R = 1 << RB, DPT = R / NT;
   constexpr bool HASP = !std::is_same<P, NoPay>::value;
   constexpr int KVB = TILE * (int)(sizeof(K) + sizeof(u32)), PB = HASP ? TILE * (int)sizeof(PO) : 0;
-  __shared__ u32 s_seg;
   __shared__ u32 s_whist[NW][R];
-  __shared__ u32 s_tdig[R];
   __shared__ u32 s_goff[R];
-  __shared__ u32 s_tmp[NW + 1];
   __shared__ __align__(16) unsigned char s_raw[KVB > PB ? KVB : PB];
   K* s_keys = (K*)s_raw;
   u32* s_vals = (u32*)(s_raw + TILE * sizeof(K));
+  // the block scan's scratch and the tile's segment: the staging area is
+  // idle until the scan's closing barrier
+  u32* s_tmp = (u32*)s_raw;
+  u32& s_seg = *((u32*)s_raw + 8);
 
   u32 tid = threadIdx.x, wv = tid / WAVE, lane = tid & (WAVE - 1);
   if (nsegs_d) nsegs = *nsegs_d;
@@ -501,12 +502,19 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
     u64 r = wbase + (u64)j * WAVE + lane;
     bool ok = r < rowend;
     u32 dg = digit_of(keys[j], shift, mask);
-    u64 peers = __ballot(ok);
-    for (u32 b = 0; b < dbits; ++b) {
-      u64 bb = __ballot((dg >> b) & 1u);
-      peers &= ((dg >> b) & 1u) ? bb : ~bb;
+    // lanes whose digit differs from this lane's in any bit: per bit, the
+    // ballot xor the lane's own bit spread over the word (0 or all ones),
+    // or-ed into the two halves of the mask with one 3-input op each.  All RB
+    // bits, unrolled: a digit's bits above dbits are zero in every lane.
+    u32 dlo = 0, dhi = 0;
+#pragma unroll
+    for (int b = 0; b < RB; ++b) {
+      const u32 own = (u32)__builtin_amdgcn_sbfe((int)dg, b, 1);
+      const u64 bb = __ballot(own != 0);
+      dlo = __builtin_amdgcn_bitop3_b32(dlo, (u32)bb, own, BITOP3_OR_XOR);
+      dhi = __builtin_amdgcn_bitop3_b32(dhi, (u32)(bb >> 32), own, BITOP3_OR_XOR);
     }
-    if (!ok) peers = 0;
+    u64 peers = ok ? __ballot(ok) & ~(((u64)dhi << 32) | dlo) : 0ull;
     const u32 lead = ok ? (u32)(__ffsll((long long)peers) - 1) : lane;
     rank[j] = (lead << 8) | (u32)__popcll(peers & lt);
     old[j] = 0;
@@ -515,17 +523,20 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) rank[j] = __shfl(old[j], (int)(rank[j] >> 8)) + (rank[j] & 0xffu);
   __syncthreads();
-  // per digit: wave offsets within the tile and the tile's count
+  // per digit: the waves' counts and the tile's count; after the block scan
+  // of the tile counts, every wave's LDS offset of the digit (the digit's
+  // offset in the tile folded in)
+  u32 wcnt[DPT][NW];
   u32 tcount[DPT];
   u32 tsum = 0;
 #pragma unroll
   for (int q = 0; q < DPT; ++q) {
     const u32 d = tid * DPT + q;
     u32 run = 0;
+#pragma unroll
     for (int w = 0; w < NW; ++w) {
-      u32 c = s_whist[w][d];
-      s_whist[w][d] = run;
-      run += c;
+      wcnt[q][w] = s_whist[w][d];
+      run += wcnt[q][w];
     }
     tcount[q] = run;
     tsum += run;
@@ -535,7 +546,12 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
 #pragma unroll
   for (int q = 0; q < DPT; ++q) {
     const u32 d = tid * DPT + q;
-    s_tdig[d] = tdig;
+    u32 run = tdig;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      s_whist[w][d] = run;
+      run += wcnt[q][w];
+    }
     s_goff[d] = gbase[q] - tdig;
     tdig += tcount[q];
   }
@@ -545,7 +561,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
     u64 r = wbase + (u64)j * WAVE + lane;
     if (r < rowend) {
       u32 dd = digit_of(keys[j], shift, mask);
-      u32 pos = s_tdig[dd] + s_whist[wv][dd] + rank[j];
+      u32 pos = s_whist[wv][dd] + rank[j];
       s_keys[pos] = keys[j];
       s_vals[pos] = vals[j];
       if (HASP) rank[j] = pos;
