@@ -245,6 +245,16 @@ double time_ms(fmcs_ctx* ctx, int reps, int (*fn)(fmcs_ctx*, void*), void* arg) 
   return ms / reps;
 }
 
+// The least of `trials` runs of time_ms (the per-row constants: large
+// sorts, where jitter only ever adds time).  The fixed costs of small sorts
+// are means over many runs instead: plans pay the typical launch cost, and a
+// minimum would price it too low.
+double time_ms_best(fmcs_ctx* ctx, int trials, int reps, int (*fn)(fmcs_ctx*, void*), void* arg) {
+  double best = 1e300;
+  for (int t = 0; t < trials; ++t) best = std::min(best, time_ms(ctx, reps, fn, arg));
+  return best;
+}
+
 struct CalArgs {
   uint32_t bank, bits;
   uint64_t n;
@@ -424,22 +434,22 @@ int fmcs_calibrate(fmcs_ctx* ctx, uint64_t nrows, fmcs_cost_model* out) {
     a.bits = bits[b];
     if ((rc = fmcs::internal::fill_random(ctx, k0, a.bank, n, 777 + b, 8))) return rc;
     // one LSD pass (8-bit keys) including the reset copies, minus the copies
-    double t_pass = time_ms(ctx, 3, [](fmcs_ctx* c, void* x) {
+    double t_pass = time_ms_best(ctx, 3, 3, [](fmcs_ctx* c, void* x) {
       CalArgs* q = (CalArgs*)x;
       int r = reset_pairs(c, q);
       return r ? r : fmcs_sort_pairs(c, q->bank, 8, q->n, q->keys, q->oids);
     }, &a);
-    double t_copy = time_ms(ctx, 3, [](fmcs_ctx* c, void* x) { return reset_pairs(c, (CalArgs*)x); }, &a);
+    double t_copy = time_ms_best(ctx, 3, 3, [](fmcs_ctx* c, void* x) { return reset_pairs(c, (CalArgs*)x); }, &a);
     out->radix_pass_ns[b] = std::max(1e-6, (t_pass - t_copy)) * 1e6 / n;
     if ((rc = fmcs::internal::fill_random(ctx, k0, a.bank, n, 999 + b, a.bits))) return rc;
-    double t_g = time_ms(ctx, 3, [](fmcs_ctx* c, void* x) {
+    double t_g = time_ms_best(ctx, 3, 3, [](fmcs_ctx* c, void* x) {
       CalArgs* q = (CalArgs*)x;
       return fmcs_gather(c, q->bank, q->n, q->keys0, q->perm, q->keys);
     }, &a);
     out->gather_ns[b] = t_g * 1e6 / n;
     // boundary over sorted keys
     if ((rc = reset_pairs(ctx, &a)) || (rc = fmcs_sort_pairs(ctx, a.bank, a.bits, n, a.keys, a.oids))) return rc;
-    double t_b = time_ms(ctx, 3, [](fmcs_ctx* c, void* x) {
+    double t_b = time_ms_best(ctx, 3, 3, [](fmcs_ctx* c, void* x) {
       CalArgs* q = (CalArgs*)x;
       uint32_t ng;
       return fmcs_extract_group_info(c, q->bank, q->n, q->keys, q->oids, (uint32_t*)q->vals2, nullptr, &ng);
@@ -454,7 +464,7 @@ int fmcs_calibrate(fmcs_ctx* ctx, uint64_t nrows, fmcs_cost_model* out) {
       }
       a.nseg = (uint32_t)((n + g - 1) / g);
       if ((rc = fmcs::internal::uniform_offsets(ctx, off, a.nseg, n, (uint32_t)g))) return rc;
-      double t = time_ms(ctx, 2, [](fmcs_ctx* c, void* x) {
+      double t = time_ms_best(ctx, 3, 2, [](fmcs_ctx* c, void* x) {
         CalArgs* q = (CalArgs*)x;
         int r = reset_pairs(c, q);
         return r ? r : fmcs_segmented_sort(c, q->bank, q->bits, q->n, q->keys, q->oids, q->off, q->nseg,
@@ -464,7 +474,7 @@ int fmcs_calibrate(fmcs_ctx* ctx, uint64_t nrows, fmcs_cost_model* out) {
     }
   }
   // Fixed costs (launches, list set-up) of the device-driven plans, from
-  // sorts of a few rows: per LSD pass (a 40-bit against a 16-bit first
+  // sorts of a few rows (means of 60 runs each): per LSD pass (a 40-bit against a 16-bit first
   // round: 5 passes against 2), per level of a later round (a 48-bit against
   // a 16-bit second round: 7 levels against 3) and per round (two rounds
   // against one, less the second round's levels).
@@ -481,19 +491,19 @@ int fmcs_calibrate(fmcs_ctx* ctx, uint64_t nrows, fmcs_cost_model* out) {
         return fmcs_multi_round_sort(c, rounds, cols, bk, bw, q->n, q->oids, nullptr, FMCS_MODE_DEVICE, nullptr);
       }
     };
-    double t20 = time_ms(ctx, 5, [](fmcs_ctx* c, void* x) {
+    double t20 = time_ms(ctx, 60, [](fmcs_ctx* c, void* x) {
       CalArgs* q = (CalArgs*)x;
       return Run::go(c, q, 1, q->vals2, 16, nullptr, 0);
     }, &a);
-    double t40 = time_ms(ctx, 5, [](fmcs_ctx* c, void* x) {
+    double t40 = time_ms(ctx, 60, [](fmcs_ctx* c, void* x) {
       CalArgs* q = (CalArgs*)x;
       return Run::go(c, q, 1, q->keys0, 40, nullptr, 0);
     }, &a);
-    double t2a = time_ms(ctx, 5, [](fmcs_ctx* c, void* x) {
+    double t2a = time_ms(ctx, 60, [](fmcs_ctx* c, void* x) {
       CalArgs* q = (CalArgs*)x;
       return Run::go(c, q, 2, q->vals2, 16, q->vals2, 16);
     }, &a);
-    double t2b = time_ms(ctx, 5, [](fmcs_ctx* c, void* x) {
+    double t2b = time_ms(ctx, 60, [](fmcs_ctx* c, void* x) {
       CalArgs* q = (CalArgs*)x;
       return Run::go(c, q, 2, q->vals2, 16, q->keys, 48);
     }, &a);
@@ -508,7 +518,7 @@ int fmcs_calibrate(fmcs_ctx* ctx, uint64_t nrows, fmcs_cost_model* out) {
     a.bank = 4;
     if ((rc = fmcs::internal::fill_random(ctx, k0, 4, n, 5, 20)) || (rc = fmcs::internal::fill_random(ctx, k, 4, n, 6, 20)))
       return rc;
-    double t_s = time_ms(ctx, 3, [](fmcs_ctx* c, void* x) {
+    double t_s = time_ms_best(ctx, 3, 3, [](fmcs_ctx* c, void* x) {
       CalArgs* q = (CalArgs*)x;
       const void* src[2] = {q->keys0, q->keys};
       uint32_t sb[2] = {4, 4}, sw[2] = {20, 20}, db[1] = {8};

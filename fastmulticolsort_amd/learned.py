@@ -213,12 +213,19 @@ def candidate_plans(model, n, bws, cards, col_order, seed=0, max_plans=48):
     return out
 
 
+def request_features(n, bws, cards, col_order, plans, est):
+    """Feature rows of a request's candidate plans given their analytic
+    estimates (ms): each plan's log-estimate enters relative to the mean
+    over the candidates."""
+    le = np.log(np.maximum(np.asarray(est, np.float64), 1e-9))
+    rel = le - le.mean()
+    return np.stack([features(n, bws, cards, col_order, p, r) for p, r in zip(plans, rel)])
+
+
 def plan_features(model, n, bws, cards, col_order, plans):
     """Feature rows of a request's candidate plans and the analytic estimates."""
     est = np.array([planner.estimate_plan(model, n, bws, cards, col_order, p)[0] for p in plans])
-    le = np.log(np.maximum(est, 1e-9))
-    rel = le - le.mean()
-    return np.stack([features(n, bws, cards, col_order, p, r) for p, r in zip(plans, rel)]), est
+    return request_features(n, bws, cards, col_order, plans, est), est
 
 
 def relative_log(ms):

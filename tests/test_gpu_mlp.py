@@ -1,13 +1,16 @@
 """The learned cost model's kernels: forward, backward and Adam against an
 fp64 restatement (pinned itself by central finite differences), and the
-plan picker's gain over the analytic model on GPU-measured plans."""
+plan picker's regret against the analytic model's on the committed MI355X
+plan timings."""
+import json
 import math
+import os
 
 import numpy as np
 import pytest
 import torch
 
-from fastmulticolsort_amd import learned, planner
+from fastmulticolsort_amd import learned
 from mlp_ref import fwd64, grad64, loss64
 
 pytestmark = pytest.mark.gpu
@@ -62,32 +65,50 @@ def test_training_fits_a_smooth_function(ctx):
     assert hist[-1] < hist[0] / 10
 
 
-def test_learned_picker_beats_analytic_on_measured_plans(ctx):
-    """Train on GPU-measured plan times of seeded requests, then compare plan
-    regret (measured time of the pick / best measured - 1) of the MLP pick
-    and of the analytic cost model's pick on held-out requests.  Both use
-    the device-calibrated cost model; the MLP is an ensemble of three seeds
-    (mean prediction), which steadies a 56-request training set."""
-    model = planner.calibrate(ctx, nrows=1 << 22)
-    data = learned.collect(ctx, nreq=80, seed=11, max_log2=20, max_plans=24, model=model)
-    X, y, req = data["X"], data["y"], data["req"]
-    train = req < 56
-    test = ~train
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "plan_timings.npz")
+# held-out mean regret: the learned pick's at most 10 %, and at least a
+# quarter point below the analytic model's (measured on MI355X: 2.67 %
+# against 3.05 %; medians 0.43 % against 0.71 %)
+MAX_REGRET = 0.10
+MARGIN = 0.0025
+
+
+def test_learned_picker_regret_on_committed_timings(ctx):
+    """The picker trained with the project's kernels on the committed
+    MI355X plan timings (tests/golden/plan_timings.npz, written by
+    tests/golden/make_plan_timings.py): 256 training requests, 64 held out,
+    every candidate plan of every request timed.  Regret of a request = the
+    measured time of the picked plan over the best measured candidate, minus
+    one (the reference's CostModelComparison, src/stitch_composer.cpp:
+    4337-4469); the analytic pick uses the estimates stored with the timings
+    (the same calibrated constants).  The kernels are deterministic and the
+    data is fixed, so the figures repeat."""
+    d = np.load(FIXTURE, allow_pickle=False)
+    ms, est, req, plan = d["ms"], d["est"], d["req"], d["plan"]
+    reqs = json.loads(str(d["requests"]))
+    ntrain = json.loads(str(d["meta"]))["train"]
+    X = np.zeros((len(ms), learned.NFEAT), np.float32)
+    y = np.zeros(len(ms), np.float32)
+    for r in range(len(reqs)):
+        m = np.where(req == r)[0]
+        q = reqs[r]
+        plans = [[int(w) for w in plan[i].split(",")] for i in m]
+        X[m] = learned.request_features(q["n"], q["bws"], q["cards"], list(range(len(q["bws"]))), plans, est[m])
+        y[m] = learned.relative_log(ms[m])
+    train, test = req < ntrain, req >= ntrain
     Xt = torch.from_numpy(X[test]).cuda()
     pred = np.zeros(int(test.sum()), np.float64)
     for s in range(3):
-        m = learned.Mlp(ctx, seed=s)
-        m.train(torch.from_numpy(X[train]).cuda(), torch.from_numpy(y[train]).cuda(), epochs=200, batch=64, lr=3e-3)
-        pred += m.forward(Xt).cpu().numpy()
-    reg_mlp = learned.regret_table(pred, y[test], req[test])
-    # analytic picks
-    import json
-    reqs = json.loads(data["requests"])
-    est = []
-    for x_plan, r in zip(data["plan"][test], req[test]):
-        q = reqs[r]
-        p = [int(w) for w in x_plan.split(",")]
-        est.append(planner.estimate_plan(model, q["n"], q["bws"], q["cards"], list(range(len(q["bws"]))), p)[0])
-    reg_an = learned.regret_table(np.log(np.array(est)), y[test], req[test])
-    print("regret mlp mean %.3f analytic mean %.3f" % (reg_mlp.mean(), reg_an.mean()), flush=True)
-    assert reg_mlp.mean() <= reg_an.mean() + 0.02
+        mlp = learned.Mlp(ctx, seed=s)
+        mlp.train(torch.from_numpy(X[train]).cuda(), torch.from_numpy(y[train]).cuda(), epochs=30, batch=256,
+                  lr=3e-3, seed=s)
+        pred += mlp.forward(Xt).cpu().numpy()
+    logt = np.log(ms[test])
+    reg_mlp = learned.regret_table(pred, logt, req[test])
+    reg_an = learned.regret_table(np.log(est[test]), logt, req[test])
+    print("held-out regret: mlp mean %.4f median %.4f, analytic mean %.4f median %.4f"
+          % (reg_mlp.mean(), np.median(reg_mlp), reg_an.mean(), np.median(reg_an)), flush=True)
+    assert len(reg_mlp) == len(reqs) - ntrain
+    assert reg_mlp.mean() <= MAX_REGRET
+    assert reg_mlp.mean() <= reg_an.mean() - MARGIN
+    assert np.median(reg_mlp) <= np.median(reg_an)
