@@ -478,9 +478,9 @@ int radix_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, 
   return FMCS_OK;
 }
 
-// Comparison sort of (kin, vin) -> (kout, vout): bitonic LDS tiles, then
-// merge-path passes doubling the sorted run length, ping-ponging through
-// the workspace.
+// Comparison sort of (kin, vin) -> (kout, vout): 2048-row tiles (register
+// bitonic networks, merge path across waves), then merge-path passes
+// doubling the sorted run length through the workspace.
 template <class K>
 int merge_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, u64 n) {
   if (n == 0) return FMCS_OK;
@@ -493,19 +493,32 @@ int merge_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, 
   TRY(getbuf_t(ctx, WS_T1V, n, &t1v));
   const u64 grid = (n + MERGE_TILE - 1) / MERGE_TILE;
   const double pass_bytes = 2.0 * n * (sizeof(K) + 4);
-  TRY(klaunch(ctx, "k_bitonic_tile", k_bitonic_tile<K>, grid, pass_bytes, nullptr, 0.0, kin, vin, t0k, t0v, n));
+  u32 npass = 0;
+  for (u64 run = MERGE_TILE; run < n; run *= 2) ++npass;
+  // the tile sort writes the output itself when no merge pass follows (in
+  // place is safe: a block reads its whole tile before writing it); else
+  // the passes alternate between two work buffers and the last one writes
+  // the output
+  TRY(klaunch(ctx, "k_bitonic_tile", k_bitonic_tile<K>, grid, pass_bytes, nullptr, 0.0, kin, vin,
+              npass ? t0k : kout, npass ? t0v : vout, n));
+  if (!npass) return FMCS_OK;
+  u32* split;
+  TRY(getbuf_t(ctx, WS_STATUS, grid + 1, &split));
   K* sk = t0k;
   u32* sv = t0v;
   K* dk = t1k;
   u32* dv = t1v;
-  for (u64 run = MERGE_TILE; run < n; run *= 2) {
+  u32 pass = 0;
+  for (u64 run = MERGE_TILE; run < n; run *= 2, ++pass) {
+    const bool last = pass + 1 == npass;
+    TRY(klaunch(ctx, "k_merge_splits", k_merge_splits<K>, (grid + 255) / 256, 0.0, nullptr, 0.0, (const K*)sk,
+                (const u32*)sv, n, run, (u32)grid, split));
     TRY(klaunch(ctx, "k_merge_pass", k_merge_pass<K>, grid, pass_bytes, nullptr, 0.0, (const K*)sk, (const u32*)sv,
-                dk, dv, n, run));
+                last ? kout : dk, last ? vout : dv, n, run, (const u32*)split));
     std::swap(sk, dk);
     std::swap(sv, dv);
   }
-  return klaunch(ctx, "kd_copy_pairs", kd_copy_pairs<K>, cap_grid((n + 255) / 256, 4096), pass_bytes, nullptr, 0.0,
-                 (const K*)sk, (const u32*)sv, kout, vout, n);
+  return FMCS_OK;
 }
 
 // Group boundaries: head counts per tile, their exclusive scan, then ids

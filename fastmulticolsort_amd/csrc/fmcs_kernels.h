@@ -148,7 +148,9 @@ __global__ void k_uniform_offsets(u32*, u32, u32, u64);
 template <class K>
 __global__ void k_bitonic_tile(const K*, const u32*, K*, u32*, u64);
 template <class K>
-__global__ void k_merge_pass(const K*, const u32*, K*, u32*, u64, u64);
+__global__ void k_merge_pass(const K*, const u32*, K*, u32*, u64, u64, const u32*);
+template <class K>
+__global__ void k_merge_splits(const K*, const u32*, u64, u64, u32, u32*);
 
 // Device-driven list kernels (fmcs_dd.hip): sizes come from device words.
 __global__ void kd_set4(u32*, u32, u32, u32, u32);
