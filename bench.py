@@ -183,6 +183,9 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="sweep the batched sort's bin thresholds and tile shapes first and save the best")
     ap.add_argument("--mode", choices=["graph", "device", "batched"], default="graph")
+    ap.add_argument("--med-max", type=int, default=None,
+                    help="tuning med_max: 0 = level-synchronous tile sorts, 1024..16384 = the persistent "
+                         "size-binned launch with single segments up to this many rows (default: the tuning file)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one plain 256 MiB device copy (the traffic reference for PMC counters)")
     args = ap.parse_args()
@@ -226,6 +229,11 @@ def main():
         tuning = best
     else:
         tuning = autotune.load(ctx)
+    if args.med_max is not None:
+        t = ctx.tuning()
+        ctx.set_tuning(min(t["small_max"], 512) if args.med_max else t["small_max"], t["window"], t["msd_bits"],
+                       med_max=args.med_max)
+        tuning = ctx.tuning()
     log(f"tuning: {tuning or ctx.tuning()}")
 
     for _ in range(args.warmup):

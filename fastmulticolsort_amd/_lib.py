@@ -7,7 +7,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libfmcs.so")
+# FMCS_LIB selects another build of the same library (a path to a .so)
+LIB_PATH = os.environ.get("FMCS_LIB") or os.path.join(HERE, "libfmcs.so")
 
 FMCS_OK = 0
 FMCS_E_ARG = 1
@@ -53,7 +54,7 @@ class GroupInfo(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("small_max", C.c_uint32), ("window", C.c_uint32), ("msd_bits", C.c_uint32),
-                ("gather_window_kb", C.c_uint32), ("carry_next", C.c_uint32)]
+                ("gather_window_kb", C.c_uint32), ("carry_next", C.c_uint32), ("med_max", C.c_uint32)]
 
 
 class FmcsError(RuntimeError):

@@ -6,11 +6,14 @@ or 2048-row tile; window: rows per packing window; msd_bits: digit width of an
 MSD level), how the second round's column reaches the first round's order
 (carry_next: moved along by the first round's radix passes, packed into
 composite keys or as its own column, or gathered, in one pass or in passes
-over source windows of gather_window_kb KiB) on a
-workload, timing whole multi-round sorts, and writes the fastest setting as
-JSON.  The column move is swept first with the sort settings fixed, then the
-sort grid with the best move.  Results never depend on the tuning, only the
-time does.
+over source windows of gather_window_kb KiB) and the form of the batched
+sort (med_max 0: every radix level sorts its own windows; else one
+persistent size-binned launch after the levels, with single segments of up to
+med_max rows sorted by one workgroup each) on a workload, timing whole
+multi-round sorts, and writes the fastest setting as JSON.  The column move is
+swept first with the sort settings fixed, then the sort grid with the best
+move, then the persistent form's thresholds with the best sort settings.
+Results never depend on the tuning, only the time does.
 """
 import itertools
 import json
@@ -26,6 +29,7 @@ GRID = {
     "msd_bits": [6, 7, 8],
 }
 GATHER_KB = [0, 262144]
+MED_MAX = [2048, 4096, 8192, 16384]
 
 
 def candidates(grid=GRID):
@@ -35,7 +39,7 @@ def candidates(grid=GRID):
 
 def apply(ctx, setting):
     ctx.set_tuning(setting["small_max"], setting["window"], setting["msd_bits"], setting.get("gather_window_kb", 0),
-                   setting.get("carry_next", 1))
+                   setting.get("carry_next", 1), setting.get("med_max", 0))
 
 
 def time_setting(ctx, run, setting, reps=3):
@@ -50,7 +54,7 @@ def time_setting(ctx, run, setting, reps=3):
     return (time.perf_counter() - t0) * 1e3 / reps
 
 
-def tune(ctx, run, grid=GRID, gather_kb=GATHER_KB, reps=3, log=None):
+def tune(ctx, run, grid=GRID, gather_kb=GATHER_KB, med_max=MED_MAX, reps=3, log=None):
     """run(): one multi-round sort on resident data.  Returns (best, table)."""
     table = []
 
@@ -68,7 +72,12 @@ def tune(ctx, run, grid=GRID, gather_kb=GATHER_KB, reps=3, log=None):
         [dict(carry_next=0, gather_window_kb=g) for g in gather_kb]
     mv = min(moves, key=lambda m: one(dict(base, **m)))
     for st in candidates(grid):
-        one(dict(st, **mv))
+        one(dict(st, med_max=0, **mv))
+    # the persistent form (1024-row window tiles: small_max at most 512)
+    lvl = min(table, key=lambda r: r["ms"])
+    sm = min(lvl["small_max"], 512)
+    for mm in med_max:
+        one(dict(lvl, small_max=sm, window=min(lvl["window"], sm), med_max=mm))
     best = min(table, key=lambda r: r["ms"])
     apply(ctx, best)
     return best, table

@@ -18,7 +18,7 @@ OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libfmcs.so")
 
 HIP_SOURCES = ["fmcs_kernels.hip", "fmcs_dd.hip", "fmcs_merge.hip", "fmcs_capi.hip", "fmcs_mlp.hip",
-               "fmcs_hash.hip", "fmcs_noalloc.hip"]
+               "fmcs_hash.hip", "fmcs_noalloc.hip", "fmcs_bins.hip"]
 HOST_SOURCES = ["fmcs_plan.cpp", "fmcs_composer.cpp", "fmcs_io.cpp"]
 
 HIPCC_FLAGS = [

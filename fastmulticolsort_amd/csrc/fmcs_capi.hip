@@ -26,7 +26,7 @@ enum Slot {
   WS_ST, WS_SZ, WS_BST, WS_BSZ, WS_FLAG, WS_POS, WS_LLO, WS_LHI, WS_LTILES, WS_TBASE, WS_TLO, WS_THI, WS_TS0,
   WS_TS1, WS_COMP, WS_COMP2, WS_GROUPTMP, WS_MISC, WS_HASH, WS_HASH2, WS_HASH3, WS_HT_TABLE, WS_HT_FIRST,
   WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_DD, WS_SCANST, WS_FBLIST, WS_TCOUNT, WS_CARRY, WS_BCNT, WS_T0P, WS_T1P, WS_PAY, WS_SGL, WS_FBLIST2, WS_RKL, WS_RKL2, WS_DNEXT, WS_TSEG, WS_HG_KEY, WS_HG_POS, WS_HG_SKEY, WS_HG_SPOS, WS_HG_F, WS_HG_AUX, WS_HG_PSTART,
-  WS_HG_TKEY, WS_HG_TFIRST, WS_HG_CHUNK, WS_USER0,
+  WS_HG_TKEY, WS_HG_TFIRST, WS_HG_CHUNK, WS_PLIST, WS_BINSC, WS_FBWIN, WS_FBPOOL, WS_USER0,
   WS_STAGE0 = WS_USER0 + 24,  // device copies of staged host buffers (one slot per argument)
   WS_COUNT = WS_STAGE0 + 72
 };
@@ -47,7 +47,7 @@ struct fmcs_ctx {
   Buf ws[WS_COUNT];
   u32* small_d = nullptr;   // 64 words of device scratch (counters, totals, consts)
   u32* host_pin = nullptr;  // PIN_WORDS words pinned for readbacks
-  fmcs_tuning tune = {512, 512, 8, 0, 1};
+  fmcs_tuning tune = {512, 512, 8, 0, 1, 0};
   hipEvent_t ev[8] = {};
   double breakdown[4] = {0, 0, 0, 0};
   // per-kernel event timing (off unless enabled)
@@ -888,15 +888,12 @@ int sort_by_group_t(fmcs_ctx* ctx, u32 keybits, u64 n, const K* column, u32* ord
 // segments each hold more than small_max rows, so at most n / (small_max+1)
 // of them exist; their tiles number at most n / TILE + that.
 // ========================================================================
-constexpr u32 DD_LVL_WORDS = 16;
 constexpr u32 DD_LVL_MAX = 72;
 constexpr u32 DD_ROUNDS_MAX = 64;
 constexpr u32 DD_HDR = 256;  // [0, 64) group count per round, [72] key error, [76, 81) LSD consts, [88] segment count
 constexpr u32 DD_ERR = 72;
 constexpr u32 DD_CONST = 76;
 constexpr u64 DD_WORDS = DD_HDR + (u64)DD_ROUNDS_MAX * DD_LVL_MAX * DD_LVL_WORDS;
-enum { LV_NSEG, LV_WCOUNT, LV_ROWS_SMALL, LV_ROWS_LARGE, LV_NLARGE, LV_NTILES, LV_SCAN_A, LV_SCAN_B, LV_OS, LV_FB,
-       LV_NSINGLE, LV_FB2, LV_RK, LV_RK2 };
 
 // The device-driven word area.  Its key-error word dd[DD_ERR] is sticky:
 // cleared once when the area is allocated and again only when an error
@@ -1136,16 +1133,22 @@ template <class K>
 int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* seg_off, const u32* nseg_d,
                 u64 nseg_max, u32* lv, bool keys_out = true, u32* err = nullptr) {
   if (n <= 1 || nseg_max == 0) return FMCS_OK;
+  // The persistent form (tune.med_max set): the levels only split segments
+  // above med_max rows; one k_seg_bins launch then sorts every level's
+  // windows (BIN_T-row tiles) and single segments of up to med_max rows.
+  bool persist = ctx->tune.med_max != 0;
+  const u32 med_max = std::min<u32>(std::max<u32>(ctx->tune.med_max, BIN_T), BIN_MED_MAX);
   // tiles of 512, 1024 or 2048 rows (2, 4 or 8 per thread) for segments of
   // up to 256, 512 or 1024 rows
-  const u32 small_max = std::min<u32>(ctx->tune.small_max, (u32)TILESORT_TMAX / 2);
-  const u32 T = small_max <= 256 ? 512u : (small_max <= 512 ? 1024u : 2048u);
+  const u32 small_max = std::min<u32>(ctx->tune.small_max, persist ? BIN_T / 2 : (u32)TILESORT_TMAX / 2);
+  const u32 T = persist ? BIN_T : (small_max <= 256 ? 512u : (small_max <= 512 ? 1024u : 2048u));
   u32 W = std::max<u32>(1, std::min<u32>(ctx->tune.window, small_max + 1));
   if (W + small_max > T) W = T - small_max;
   const u32 msd_bits = std::max<u32>(1, std::min<u32>(ctx->tune.msd_bits, RADIX_BITS));
   // static level schedule with buffer bounds
   std::vector<DdLevel> L;
-  {
+  for (;;) {
+    L.clear();
     u32 kb = std::min<u32>(keybits, 8 * sizeof(K));
     u64 cur_max = nseg_max, prev_tiles = 0;
     for (;;) {
@@ -1160,8 +1163,8 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
       d.nw_max = L.empty() ? (n + d.Wl - 1) / d.Wl : prev_tiles * d.wpt;
       // segments of sm + 1 .. 2T rows (at most the largest tile) sort alone
       // in an LDS tile of T or 2T rows ("single" windows); larger ones take
-      // an MSD level
-      d.single_max = std::min<u32>(2 * T, (u32)TILESORT_TMAX);
+      // an MSD level.  Persistent: up to med_max rows, sorted by one block
+      d.single_max = persist ? med_max : std::min<u32>(2 * T, (u32)TILESORT_TMAX);
       d.nsingle_max = std::min<u64>(cur_max, n / ((u64)d.sm + 1));
       d.nlarge_max = std::min<u64>(cur_max, n / ((u64)d.single_max + 1));
       d.ntiles_max = std::min<u64>(n / ONESWEEP_TILE + d.nlarge_max, n);
@@ -1177,6 +1180,10 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
       prev_tiles = d.ntiles_max;
       kb -= d.D;
     }
+    // the persistent launch takes at most BIN_LVL_MAX levels (narrow MSD
+    // digits on wide keys need more): the level-synchronous form then
+    if (!persist || L.size() <= BIN_LVL_MAX) break;
+    persist = false;
   }
   u64 m_list[2] = {0, 0}, m_nw = 1, m_flag = 1, m_large = 1, m_tiles = 1, m_single = 1;
   for (size_t l = 0; l < L.size(); ++l) {
@@ -1187,6 +1194,23 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
     m_large = std::max<u64>(m_large, L[l].nlarge_max + 1);
     m_tiles = std::max<u64>(m_tiles, L[l].ntiles_max + 1);
   }
+  // persistent form: every level's segment starts, window slots and single
+  // segments stay until the one launch after the levels, each level in a
+  // region of its own
+  std::vector<u64> loff(L.size() + 1, 0), woff(L.size() + 1, 0), soff(L.size() + 1, 0);
+  for (size_t l = 0; l < L.size(); ++l) {
+    loff[l + 1] = loff[l] + L[l].nseg_max + 1;
+    woff[l + 1] = woff[l] + L[l].nw_max;
+    soff[l + 1] = soff[l] + L[l].nsingle_max;
+  }
+  if (persist) {
+    if (loff[L.size()] >= (1ull << 32) || woff[L.size()] >= (1ull << 32) || soff[L.size()] >= (1ull << 32))
+      return fail(ctx, FMCS_E_UNSUPPORTED, "segmented-sort lists exceed 2^32 entries");
+    m_nw = std::max<u64>(woff[L.size()], 1);
+    m_single = std::max<u64>(soff[L.size()], 1);
+  }
+  u32* plist = nullptr;
+  if (persist) TRY(getbuf_t(ctx, WS_PLIST, loff[L.size()], &plist));
   K* kb_;
   u32* vb_;
   TRY(getbuf_t(ctx, WS_KB, n, &kb_));
@@ -1236,7 +1260,7 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
   auto fn_tsort2 = T2 <= 512 ? k_tile_sort<K, 2> : (T2 <= 1024 ? k_tile_sort<K, 4> : k_tile_sort<K, 8>);
   auto fn_trank2 = T2 <= 512 ? k_tile_rank<K, 2> : (T2 <= 1024 ? k_tile_rank<K, 4> : k_tile_rank<K, 8>);
   TRY(klaunch(ctx, "k_offsets_to_list", kd_offsets_to_list, cap_grid((nseg_max + 255) / 256, 2048), 0.0, nullptr, 0.0,
-              seg_off, nseg_d, lst[0], lsz[0]));
+              seg_off, nseg_d, persist ? plist : lst[0], lsz[0]));
   const u32* plo = nullptr;
   const u32* ptb = nullptr;
   for (size_t l = 0; l < L.size(); ++l) {
@@ -1244,11 +1268,13 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
     u32* S = lv + l * DD_LVL_WORDS;
     const u32* nseg_p = (l == 0) ? nseg_d : S + LV_NSEG;
     const int cur = (int)(l & 1);
-    u32* cst = lst[cur];
+    u32* cst = persist ? plist + loff[l] : lst[cur];
     u32* csz = lsz[cur];
+    const u64 wo = persist ? woff[l] : 0, so = persist ? soff[l] : 0;
     TRY(klaunch(ctx, "k_windows", kd_windows, cap_grid((d.nseg_max + 2047) / 2048, 2048), 0.0, nullptr, 0.0,
-                (const u32*)cst, (const u32*)csz, nseg_p, d.Wl, d.sm, plo, ptb, d.wpt, tlo, thi, ts0, ts1, wlist,
-                S + LV_WCOUNT, d.single_max, slo, shi, sseg, S + LV_NSINGLE, scan_st, scan_st2));
+                (const u32*)cst, (const u32*)csz, nseg_p, d.Wl, d.sm, plo, ptb, d.wpt, tlo + wo, thi + wo, ts0 + wo,
+                ts1 + wo, wlist + wo, S + LV_WCOUNT, d.single_max, slo + so, shi + so, sseg + so, S + LV_NSINGLE,
+                scan_st, scan_st2));
     // the large segments (above single_max rows) compacted, with their digit
     // counts zeroed on the levels that scatter; the row counts only feed the
     // per-kernel byte figures of a profiled run
@@ -1269,14 +1295,14 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
       a.bad = ~((1ull << keybits) - 1);
       a.err = err;
     }
-    {
+    if (!persist) {
       const u64 g = (persist_grid(ctx, (const void*)fn_bucket, d.nw_max) + XCDS - 1) / XCDS * XCDS;
       // rows read (key, oid) and written (oid, and the key when wanted)
       const double per = keys_out ? 2.0 * (sizeof(K) + 4) : sizeof(K) + 8.0;
       TRY(klaunch(ctx, "k_tile_bucket", fn_bucket, g, 0.0, S + LV_ROWS_SMALL, per, a, fb, a.tile_lo, a.tile_hi,
                   a.tile_s0, a.tile_s1, a.wlist));
     }
-    {
+    if (!persist) {
       // refused windows: the radix tile sort over the bits that vary inside
       // each; where even those leave no room for the segment index (keys
       // wider than 53 bits), the comparison rank
@@ -1290,7 +1316,7 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
         TRY(klaunch(ctx, "k_tile_rank", fn_trank, g, 0.0, nullptr, 0.0, b));
       }
     }
-    if (d.nsingle_max) {
+    if (d.nsingle_max && !persist) {
       // the single-segment windows (in segment order): up to T rows in
       // T-row tiles, then the larger ones in 2T-row tiles; refused windows of
       // either size go to the 2T-row radix tile sort
@@ -1336,7 +1362,8 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
                 tcount, a.bad, a.err, tseg));
     const int nxt = cur ^ 1;
     TRY(klaunch(ctx, "k_digit_base", kd_digit_base<RADIX_BITS>, cap_grid(d.nlarge_max, 2048), 0.0, nullptr, 0.0,
-                (const u32*)hist, (const u32*)llo, dbase, lst[nxt], lsz[nxt], (const u32*)(S + LV_NLARGE),
+                (const u32*)hist, (const u32*)llo, dbase, persist ? plist + loff[l + 1] : lst[nxt], lsz[nxt],
+                (const u32*)(S + LV_NLARGE),
                 l + 1 < L.size() ? S + DD_LVL_WORDS + LV_NSEG : nullptr));
     TRY(scatter_pass<K>(ctx, "k_onesweep_msd", skey, sval, dkey, dval, llo, lhi, tbase, S + LV_NLARGE,
                         S + LV_NTILES, d.ntiles_max, dbase, tcount, d.shift, d.D, 0.0, S + LV_ROWS_LARGE,
@@ -1344,6 +1371,59 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
                         (u32*)nullptr, (uint8_t*)nullptr, 0u, tseg));
     plo = llo;
     ptb = tbase;
+  }
+  if (!persist) return FMCS_OK;
+  // one launch for every level's single segments and windows
+  auto fn_bins = k_seg_bins<K>;
+  const u64 gbins = std::max<u64>(resident(ctx, (const void*)fn_bins) / XCDS, 1) * XCDS;
+  u32* binsc;
+  TRY(getbuf_t(ctx, WS_BINSC, gbins * BIN_SCRATCH, &binsc));
+  BinArgs<K> b;
+  std::memset(&b, 0, sizeof(b));
+  b.kin[0] = keys; b.kin[1] = kb_; b.vin[0] = vals; b.vin[1] = vb_;
+  b.kbuf[0] = keys; b.kbuf[1] = kb_; b.vbuf[0] = vals; b.vbuf[1] = vb_;
+  b.kout = keys_out ? keys : nullptr; b.vout = vals;
+  b.starts = plist; b.tlo = tlo; b.thi = thi; b.ts0 = ts0; b.ts1 = ts1; b.wlist = wlist;
+  b.slo = slo; b.shi = shi; b.sseg = sseg; b.lv = lv; b.scratch = binsc;
+  b.err = err;
+  b.bad = (err && keybits < 8 * sizeof(K)) ? ~((1ull << keybits) - 1) : 0ull;
+  b.nlev = (u32)L.size(); b.W = W; b.small_max = small_max;
+  for (size_t l = 0; l < L.size(); ++l) {
+    b.loff[l] = (u32)loff[l]; b.woff[l] = (u32)woff[l]; b.soff[l] = (u32)soff[l]; b.kbs[l] = L[l].tkb;
+  }
+  // refused windows (a bin above BUCKET_LIMIT rows, so at least
+  // BUCKET_LIMIT + 1 rows each): two lists by source buffer, their counters
+  // in the zeroed level record after the last level
+  const u64 fbcap = n / (BUCKET_LIMIT + 1) + 1;
+  u32 *fbw, *fbpool;
+  TRY(getbuf_t(ctx, WS_FBWIN, 10 * fbcap, &fbw));
+  TRY(getbuf_t(ctx, WS_FBPOOL, n, &fbpool));
+  u32* fbcnt = lv + L.size() * DD_LVL_WORDS;  // [0, 2) refused, [2, 4) too wide for the radix tile sort
+  b.fbpool = fbpool; b.fblo = fbw; b.fbhi = fbw + 2 * fbcap; b.fbs0 = fbw + 4 * fbcap; b.fbs1 = fbw + 6 * fbcap;
+  b.fbcount = fbcnt; b.fbcap = (u32)fbcap;
+  // every row ends in one window or single segment: read (key, oid), written
+  // (oid, and the key when wanted)
+  const double per = keys_out ? 2.0 * (sizeof(K) + 4) : sizeof(K) + 8.0;
+  TRY(klaunch(ctx, "k_seg_bins", fn_bins, gbins, (double)n * per, nullptr, 0.0, b));
+  // the refused windows: radix tile sort over the key bits that vary inside
+  // each, the comparison rank where those leave no room for a segment index
+  const u32 kbits = std::min<u32>(keybits, 8 * sizeof(K));
+  for (u32 src = 0; src < 2; ++src) {
+    TileSortArgs<K> f;
+    std::memset(&f, 0, sizeof(f));
+    f.kin[0] = keys; f.vin[0] = vals; f.kin[1] = kb_; f.vin[1] = vb_;
+    f.kout = keys_out ? keys : nullptr; f.vout = vals;
+    f.tile_lo = b.fblo + src * fbcap; f.tile_hi = b.fbhi + src * fbcap;
+    f.tile_s0 = b.fbs0 + src * fbcap; f.tile_s1 = b.fbs1 + src * fbcap;
+    f.seg_start = fbpool; f.src_sel = src; f.kb = kbits; f.pad_seg = BIN_T;
+    f.fbcount = fbcnt + src;
+    f.rklist = fbw + 8 * fbcap + src * fbcap; f.rkcount = fbcnt + 2 + src;
+    const u64 g = persist_grid(ctx, (const void*)k_tile_sort<K, BIN_E>, fbcap);
+    TRY(klaunch(ctx, "k_tile_sort", k_tile_sort<K, BIN_E>, g, 0.0, nullptr, 0.0, f));
+    if (kbits + TS_SEGBITS > 64) {
+      f.fblist = f.rklist; f.fbcount = f.rkcount;
+      TRY(klaunch(ctx, "k_tile_rank", k_tile_rank<K, BIN_E>, g, 0.0, nullptr, 0.0, f));
+    }
   }
   return FMCS_OK;
 }
@@ -1497,6 +1577,7 @@ int dd_finish(fmcs_ctx* ctx, u32 nrounds, bool have_groups, u32* ngroups) {
   ctx->dd_pending = false;
   if (h[DD_ERR]) {
     TRY(fill(ctx, dd + DD_ERR, 1, 0));  // reported once
+    if (h[DD_ERR] & 2u) return fail(ctx, FMCS_E_HIP, "segmented sort: a block's segment stack overflowed");
     return fail(ctx, FMCS_E_ARG, "keys exceed keybits");
   }
   if (ngroups) *ngroups = have_groups ? h[nrounds - 1] : (nrounds >= 2 ? h[nrounds - 2] : 0u);
@@ -1518,6 +1599,7 @@ int mrs_graph(fmcs_ctx* ctx, u32 nrounds, const void* const* cols, const u32* ba
               u32* groups_d) {
   std::vector<u64> key = {(u64)nrounds, n, (u64)(uintptr_t)order, (u64)(uintptr_t)groups_d, ctx->tune.small_max,
                           ctx->tune.window, ctx->tune.msd_bits, ctx->tune.gather_window_kb, ctx->tune.carry_next,
+                          ctx->tune.med_max,
                           (u64)(uintptr_t)ctx->stream};
   for (u32 r = 0; r < nrounds; ++r) {
     key.push_back((u64)(uintptr_t)cols[r]);
@@ -1954,7 +2036,8 @@ int fmcs_ctx_set_tuning(fmcs_ctx* ctx, const fmcs_tuning* t) {
   if (!ctx || !t) return FMCS_E_ARG;
   // small_max >= 64 bounds the device-driven path's segment lists by 256 / 65 entries per row
   if (t->small_max < 64 || t->small_max > (u32)TILESORT_TMAX / 2 || t->window < 1 || t->window > t->small_max + 1 ||
-      t->msd_bits < 1 || t->msd_bits > (u32)RADIX_BITS || t->gather_window_kb > (1u << 24) || t->carry_next > 2)
+      t->msd_bits < 1 || t->msd_bits > (u32)RADIX_BITS || t->gather_window_kb > (1u << 24) || t->carry_next > 2 ||
+      (t->med_max != 0 && (t->med_max < BIN_T || t->med_max > BIN_MED_MAX)))
     return fail(ctx, FMCS_E_ARG, "tuning out of range");
   ctx->tune = *t;
   return FMCS_OK;

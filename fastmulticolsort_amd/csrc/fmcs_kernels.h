@@ -60,6 +60,7 @@ constexpr int ONESWEEP_TILE = 256 * ONESWEEP_ITEMS;
 constexpr int BOUNDARY_ITEMS = 32;
 constexpr int BOUNDARY_TILE = 256 * BOUNDARY_ITEMS;
 constexpr int TILESORT_E = 4;
+constexpr u32 BUCKET_LIMIT = 24;  // largest bin the bucket tile sort takes (else the radix tile sort)
 constexpr int TILESORT_T = 256 * TILESORT_E;
 constexpr int TILESORT_TMAX = 2048;  // largest tile of the device-driven path (256 threads x 8 rows)
 constexpr int SCAN_ITEMS = 8;
@@ -177,5 +178,60 @@ template <class K>
 __global__ void kd_copy_pairs(const K*, const u32*, K*, u32*, u64);
 template <class T>
 __global__ void kd_copy_col(const T*, T*, u64);
+
+// Level records of the device-driven segmented sort: DD_LVL_WORDS words per
+// level, these at their places.
+constexpr u32 DD_LVL_WORDS = 16;
+enum { LV_NSEG, LV_WCOUNT, LV_ROWS_SMALL, LV_ROWS_LARGE, LV_NLARGE, LV_NTILES, LV_SCAN_A, LV_SCAN_B, LV_OS, LV_FB,
+       LV_NSINGLE, LV_FB2, LV_RK, LV_RK2 };
+
+// The persistent size-binned launch (fmcs_bins.hip).  After the radix levels
+// have split every segment above med_max rows, one launch sorts the work that
+// every level left behind: single segments of small_max + 1 .. med_max rows
+// (those above one window tile sorted by the block alone: an MSD level of
+// its own plus its child windows) and the windows of packed small segments.
+// Window tiles are BIN_T rows.
+constexpr int BIN_E = 4;
+constexpr u32 BIN_T = 256 * BIN_E;
+constexpr u32 BIN_LVL_MAX = 12;
+constexpr u32 BIN_MED_MAX = 16384;   // largest medium segment (bounds the block's stack)
+constexpr u32 BIN_STACK = 128;       // segments above BIN_T rows pending in one block
+constexpr u32 BIN_SCRATCH = 2048;    // global words per block: child starts, stack, window list
+template <class K>
+struct BinArgs {
+  const K* kin[2];   // rows of a level by its parity (0: the caller's buffers)
+  const u32* vin[2];
+  K* kbuf[2];        // the same buffers, written by a medium segment's MSD level
+  u32* vbuf[2];
+  K* kout;           // final rows (kout null: sorted keys not wanted)
+  u32* vout;
+  const u32* starts;  // every level's segment starts, at loff[level]
+  const u32* tlo;     // window slots, at woff[level]
+  const u32* thi;
+  const u32* ts0;
+  const u32* ts1;
+  const u32* wlist;   // compact window ids, at woff[level]
+  const u32* slo;     // single segments, at soff[level]
+  const u32* shi;
+  const u32* sseg;
+  const u32* lv;      // level records (window and single counts)
+  u32* scratch;       // BIN_SCRATCH words per block
+  u32* err;           // level-0 rows: a key with a bit in bad sets *err |= 1; a full stack *err |= 2
+  u64 bad;
+  // windows the bucket sort refuses, by source buffer: list b at b * fbcap of
+  // fblo/fbhi/fbs0/fbs1, counted in fbcount[b]; their segment starts in
+  // fbpool (row-indexed, n words)
+  u32* fbpool;
+  u32* fblo;
+  u32* fbhi;
+  u32* fbs0;
+  u32* fbs1;
+  u32* fbcount;
+  u32 fbcap;
+  u32 nlev, W, small_max;
+  u32 loff[BIN_LVL_MAX], woff[BIN_LVL_MAX], soff[BIN_LVL_MAX], kbs[BIN_LVL_MAX];
+};
+template <class K>
+__global__ void k_seg_bins(BinArgs<K>);
 
 }  // namespace fmcs

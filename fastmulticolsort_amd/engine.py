@@ -102,15 +102,20 @@ class Context:
         t = Tuning()
         self._chk(self.lib.fmcs_ctx_get_tuning(self.h, C.byref(t)))
         return {"small_max": t.small_max, "window": t.window, "msd_bits": t.msd_bits,
-                "gather_window_kb": t.gather_window_kb, "carry_next": t.carry_next}
+                "gather_window_kb": t.gather_window_kb, "carry_next": t.carry_next, "med_max": t.med_max}
 
-    def set_tuning(self, small_max, window, msd_bits, gather_window_kb=None, carry_next=None):
+    def set_tuning(self, small_max, window, msd_bits, gather_window_kb=None, carry_next=None, med_max=None):
+        """med_max 0: level-synchronous batched sort; 1024..16384: the
+        persistent size-binned launch with single segments up to med_max rows
+        sorted by one workgroup each."""
         cur = self.tuning()
         if gather_window_kb is None:
             gather_window_kb = cur["gather_window_kb"]
         if carry_next is None:
             carry_next = cur["carry_next"]
-        t = Tuning(small_max, window, msd_bits, gather_window_kb, int(carry_next))
+        if med_max is None:
+            med_max = cur["med_max"]
+        t = Tuning(small_max, window, msd_bits, gather_window_kb, int(carry_next), int(med_max))
         self._chk(self.lib.fmcs_ctx_set_tuning(self.h, C.byref(t)))
 
     # ---- kernels -------------------------------------------------------

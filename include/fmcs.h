@@ -83,6 +83,12 @@ typedef struct {
                                 is in a u64 bank and the two widths add up to at most 64 (this form also
                                 reports second-column values above their width); 2 = carried as a
                                 separate column; 0 = gathered after round 1 */
+  uint32_t med_max;          /* device-driven modes: 0 = every radix level sorts its own windows (the
+                                level-synchronous form); else 1024..16384: the levels split only segments
+                                above med_max rows and one persistent launch then sorts every level's
+                                windows and single segments, those above 1024 rows each by one workgroup
+                                (an MSD level of its own, then its child windows).  The persistent form
+                                uses 1024-row window tiles (small_max at most 512). */
 } fmcs_tuning;
 
 /* ---- context -------------------------------------------------------- */
