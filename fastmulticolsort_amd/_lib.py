@@ -31,7 +31,7 @@ EXPORTS = [
     "fmcs_sort_by_group",
     "fmcs_multi_round_sort", "fmcs_last_breakdown", "fmcs_hash_round", "fmcs_multi_round_hash",
     "fmcs_ctx_set_profiling", "fmcs_kernel_stats", "fmcs_kernel_stats_reset", "fmcs_distinct_count",
-    "fmcs_classify_memory", "fmcs_pointer_kind", "fmcs_ctx_set_host_staging",
+    "fmcs_classify_memory", "fmcs_pointer_kind", "fmcs_ctx_set_host_staging", "fmcs_probe_handoff",
 ]
 
 # fmcs_pointer_kind / fmcs_classify_memory
@@ -115,6 +115,7 @@ def load():
         "fmcs_classify_memory": (u32, [i32, i32, i32, i32]),
         "fmcs_pointer_kind": (i32, [vp, vp, P(u32)]),
         "fmcs_ctx_set_host_staging": (i32, [vp, i32]),
+        "fmcs_probe_handoff": (i32, [vp, u32, u32, P(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -408,6 +408,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FMCS_BINS_W
   }
 }
 
+// Probe of what a hand-off between workgroups costs inside one launch: every
+// block copies tasks of PROBE_ROWS u64 rows; after each task mode 1 has
+// every wave release its stores at agent scope (a fence, as when each
+// wave publishes its own output), mode 2 one release after a barrier (the
+// least a task whose output another workgroup reads must do), mode 3 the
+// same after every 8th task of a block; mode 0 none.  On MI355X an
+// agent-scope release writes back the XCD's L2 (buffer_wbl2), which
+// serialises per XCD.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_handoff_probe(const u64* __restrict__ in, u64* __restrict__ out, u32 ntasks,
+                                                       u32* __restrict__ flags) {
+  for (u32 t = blockIdx.x; t < ntasks; t += gridDim.x) {
+    const u64* src = in + (u64)t * PROBE_ROWS;
+    u64* dst = out + (u64)t * PROBE_ROWS;
+#pragma unroll
+    for (u32 j = 0; j < PROBE_ROWS / 256; ++j) dst[j * 256 + threadIdx.x] = src[j * 256 + threadIdx.x] + 1;
+    if (MODE == 1) {
+      __threadfence();
+      if (threadIdx.x == 0)
+        __hip_atomic_fetch_add(flags + (blockIdx.x % XCDS) * 16, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (MODE == 2 || (MODE == 3 && (t / gridDim.x) % 8 == 7)) {
+      __syncthreads();
+      if (threadIdx.x == 0)
+        __hip_atomic_fetch_add(flags + (blockIdx.x % XCDS) * 16, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+template __global__ void k_handoff_probe<0>(const u64*, u64*, u32, u32*);
+template __global__ void k_handoff_probe<1>(const u64*, u64*, u32, u32*);
+template __global__ void k_handoff_probe<2>(const u64*, u64*, u32, u32*);
+template __global__ void k_handoff_probe<3>(const u64*, u64*, u32, u32*);
+
 template __global__ void k_seg_bins<uint16_t>(BinArgs<uint16_t>);
 template __global__ void k_seg_bins<uint32_t>(BinArgs<uint32_t>);
 template __global__ void k_seg_bins<uint64_t>(BinArgs<uint64_t>);

@@ -2540,6 +2540,44 @@ int fmcs_hash_round(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* values
   return st.finish();
 }
 
+int fmcs_probe_handoff(fmcs_ctx* ctx, uint32_t tasks, uint32_t grid, double* ms4) {
+  if (!ctx || !ms4 || tasks == 0 || tasks > (1u << 17) || grid == 0 || grid > (1u << 16))
+    return fail(ctx, FMCS_E_ARG, "probe: 1..2^17 tasks, 1..2^16 blocks");
+  const u64 rows = (u64)tasks * PROBE_ROWS;
+  void *in, *out, *fl;
+  TRY(internal::scratch(ctx, 13, rows * 8, &in));
+  TRY(internal::scratch(ctx, 14, rows * 8, &out));
+  TRY(internal::scratch(ctx, 15, 4096, &fl));
+  TRY(fill(ctx, (u32*)in, rows * 2, 0));
+  TRY(fill(ctx, (u32*)fl, 1024, 0));
+  void (*fns[4])(const u64*, u64*, u32, u32*) = {k_handoff_probe<0>, k_handoff_probe<1>, k_handoff_probe<2>,
+                                                  k_handoff_probe<3>};
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int rc = FMCS_OK;
+  for (int m = 0; m < 4 && rc == FMCS_OK; ++m) {
+    float best = 1e30f;
+    for (int r = 0; r < 5 && rc == FMCS_OK; ++r) {
+      hipError_t e = hipEventRecord(e0, ctx->stream);
+      if (e == hipSuccess) {
+        hipLaunchKernelGGL(fns[m], dim3(grid), dim3(256), 0, ctx->stream, (const u64*)in, (u64*)out, tasks, (u32*)fl);
+        e = hipGetLastError();
+      }
+      if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.0f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) rc = fail(ctx, FMCS_E_HIP, hipGetErrorString(e));
+      best = std::min(best, ms);
+    }
+    ms4[m] = best;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
 int fmcs_distinct_count(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* col_d, uint64_t* count) {
   if (!ctx || !count || (n && !col_d) || !(bank == 1 || bank == 2 || bank == 4 || bank == 8))
     return FMCS_E_ARG;

@@ -233,5 +233,8 @@ struct BinArgs {
 };
 template <class K>
 __global__ void k_seg_bins(BinArgs<K>);
+constexpr u32 PROBE_ROWS = 4096;  // u64 rows a task of k_handoff_probe (32 KiB)
+template <int MODE>
+__global__ void k_handoff_probe(const u64*, u64*, u32, u32*);
 
 }  // namespace fmcs

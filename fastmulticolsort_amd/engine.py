@@ -224,6 +224,14 @@ class Context:
                                            _p(gid), _p(oo), _p(og), int(last_round), C.byref(ng)))
         return gid, oo, og, ng.value
 
+    def probe_handoff(self, tasks=65536, grid=1024):
+        """Milliseconds of the hand-off probe (fmcs_probe_handoff): modes 0 none,
+        1 every wave releases per task, 2 one release after a barrier per
+        task, 3 one release per 8 tasks of a block."""
+        ms = (C.c_double * 4)()
+        self._chk(self.lib.fmcs_probe_handoff(self.h, tasks, grid, ms))
+        return list(ms)
+
     def distinct_count(self, col):
         """Exact number of distinct values of a device column (hash grouping)."""
         c = C.c_uint64(0)

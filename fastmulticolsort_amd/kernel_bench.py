@@ -211,7 +211,24 @@ def main(argv=None):
     ap.add_argument("--hash-types", default="1,0", help="config 4: hash_type values to run, or none")
     ap.add_argument("--copy-probe", action="store_true",
                     help="start with one plain 256 MiB device copy (the traffic reference for PMC counters)")
+    ap.add_argument("--handoff-probe", action="store_true",
+                    help="time fmcs_probe_handoff (agent-scope releases between streaming tasks) and exit")
     args = ap.parse_args(argv)
+    if args.handoff_probe:
+        from .engine import Context
+        ctx = Context(0)
+        tasks = 65536
+        bytes_moved = 2.0 * tasks * 4096 * 8
+        for grid in (1024, 2048):
+            ms = ctx.probe_handoff(tasks, grid)
+            names = ["none", "every wave releases per task", "one release per task", "one release per 8 tasks"]
+            for m, name in enumerate(names):
+                releases = [0, tasks * 4, tasks, tasks // 8][m]
+                extra_ns = (ms[m] - ms[0]) * 1e6 / releases if releases else 0.0
+                _emit({"probe": "handoff", "grid": grid, "tasks": tasks, "task_bytes": 32768, "mode": m,
+                       "what": name, "ms": round(ms[m], 4), "GBs": round(bytes_moved / ms[m] / 1e6, 1),
+                       "releases": releases, "ns_per_release": round(extra_ns, 1)})
+        return
     if args.copy_probe:
         import torch
         a = torch.empty(1 << 28, dtype=torch.uint8, device="cuda")

@@ -292,6 +292,16 @@ int fmcs_multi_round_hash(fmcs_ctx* ctx, uint32_t nrounds, const void* const* co
  * grouping, bank 1, 2, 4 or 8 bytes). */
 int fmcs_distinct_count(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* col_d, uint64_t* count);
 
+/* Measurement probe with no counterpart in the reference: the cost of a
+ * hand-off between workgroups inside one launch, the step a single-launch
+ * work queue pays whenever one workgroup's output becomes another's input.
+ * Each of `grid` blocks copies tasks of 4096 u64 rows (tasks in all, at most
+ * 2^17); ms4[m] is the best of 5 launches of mode m: 0 no release, 1 every
+ * wave releases its stores at agent scope after each task, 2 one release
+ * after a barrier per task, 3 one release per 8 tasks of a block.  Uses
+ * three of the context's scratch buffers (2 x tasks x 32 KiB). */
+int fmcs_probe_handoff(fmcs_ctx* ctx, uint32_t tasks, uint32_t grid, double* ms4);
+
 #ifdef __cplusplus
 }
 #endif

@@ -146,3 +146,11 @@ def test_persistent_multi_round_graph(ctx):
     finally:
         ctx.set_tuning(old["small_max"], old["window"], old["msd_bits"], old["gather_window_kb"], old["carry_next"],
                        old["med_max"])
+
+
+def test_probe_handoff_runs(ctx):
+    ms = ctx.probe_handoff(1024, 256)
+    log("handoff probe", ms)
+    assert len(ms) == 4 and all(m > 0 for m in ms)
+    with pytest.raises(FmcsError):
+        ctx.probe_handoff(0, 256)
