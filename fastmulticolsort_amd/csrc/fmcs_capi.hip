@@ -26,7 +26,7 @@ enum Slot {
   WS_ST, WS_SZ, WS_BST, WS_BSZ, WS_FLAG, WS_POS, WS_LLO, WS_LHI, WS_LTILES, WS_TBASE, WS_TLO, WS_THI, WS_TS0,
   WS_TS1, WS_COMP, WS_COMP2, WS_GROUPTMP, WS_MISC, WS_HASH, WS_HASH2, WS_HASH3, WS_HT_TABLE, WS_HT_FIRST,
   WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_DD, WS_SCANST, WS_FBLIST, WS_TCOUNT, WS_CARRY, WS_BCNT, WS_T0P, WS_T1P, WS_PAY, WS_SGL, WS_FBLIST2, WS_RKL, WS_RKL2, WS_DNEXT, WS_TSEG, WS_HG_KEY, WS_HG_POS, WS_HG_SKEY, WS_HG_SPOS, WS_HG_F, WS_HG_AUX, WS_HG_PSTART,
-  WS_HG_TKEY, WS_HG_TFIRST, WS_HG_CHUNK, WS_PLIST, WS_BINSC, WS_FBWIN, WS_FBPOOL, WS_USER0,
+  WS_HG_TKEY, WS_HG_TFIRST, WS_HG_CHUNK, WS_PLIST, WS_BINSC, WS_FBWIN, WS_FBPOOL, WS_UNB0, WS_UNB1, WS_UNBK, WS_USER0,
   WS_STAGE0 = WS_USER0 + 24,  // device copies of staged host buffers (one slot per argument)
   WS_COUNT = WS_STAGE0 + 72
 };
@@ -759,13 +759,55 @@ int seg_sort_plain(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const 
   return FMCS_OK;
 }
 
+template <class K>
+int gather_launch(fmcs_ctx* ctx, const K* column, const u32* order, K* out, u64 n);
+
+// The unbinned form when (group id, key) does not fit 64 bits: the same LSD
+// order in two stable sorts, by the key (carrying each row's position), then
+// by the group id of those positions; rows keep their incoming order among
+// equal (group, key) pairs, as in the composite form.
+template <class K>
+int seg_sort_unbinned_wide(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* group, u32 gbits,
+                           u32* err) {
+  const u64 grid = cap_grid((n + 255) / 256, 8192);
+  if (err)
+    TRY(klaunch(ctx, "k_key_check", k_key_check<K>, grid, (double)n * sizeof(K), nullptr, 0.0, (const K*)keys, n,
+                ~((1ull << keybits) - 1), err));
+  K* ka;
+  u32 *pos, *u0, *u1;
+  u64 *c, *c2;
+  TRY(getbuf_t(ctx, WS_UNBK, n, &ka));  // the caller's keys may live in WS_KA
+  TRY(getbuf_t(ctx, WS_UNB0, n, &pos));
+  TRY(getbuf_t(ctx, WS_UNB1, n, &u0));
+  TRY(getbuf_t(ctx, WS_COMP, n + 2, &c));
+  TRY(getbuf_t(ctx, WS_COMP2, n + 2, &c2));
+  const u64 half = (n + 3) & ~3ull;  // the second array of each buffer 16-byte aligned (gather orders)
+  u32* gA = (u32*)c;
+  u32* iA = (u32*)c + half;
+  u32* jB = (u32*)c2;
+  u1 = (u32*)c2 + half;
+  // 1. by key: ka = sorted keys, iA = their rows
+  TRY(klaunch(ctx, "k_iota", k_iota_u32, grid, 0.0, nullptr, 0.0, pos, n));
+  TRY(radix_sort<K>(ctx, keys, pos, ka, iA, n, keybits));
+  // 2. by group, stably: jB = positions in the key order
+  TRY(gather_launch<u32>(ctx, group, iA, gA, n));
+  TRY(klaunch(ctx, "k_iota", k_iota_u32, grid, 0.0, nullptr, 0.0, pos, n));
+  TRY(radix_sort<u32>(ctx, gA, pos, u0, jB, n, gbits));
+  // 3. rows out: keys from the key order, oids through the original rows
+  TRY(gather_launch<K>(ctx, ka, jB, keys, n));
+  TRY(gather_launch<u32>(ctx, iA, jB, u0, n));
+  TRY(gather_launch<u32>(ctx, vals, u0, u1, n));
+  HIPCHK(hipMemcpyAsync(vals, u1, n * sizeof(u32), hipMemcpyDeviceToDevice, ctx->stream));
+  return FMCS_OK;
+}
+
 // single unbinned segmented sort: LSD radix over (group id, key) composites
 template <class K>
 int seg_sort_unbinned(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* group, u32 ngroups,
                       u32* err) {
   u32 gbits = 0;
   while (gbits < 32 && (1ull << gbits) < ngroups) ++gbits;
-  if (gbits + keybits > 64) return fail(ctx, FMCS_E_UNSUPPORTED, "composite key exceeds 64 bits");
+  if (gbits + keybits > 64) return seg_sort_unbinned_wide<K>(ctx, keys, vals, n, keybits, group, gbits, err);
   u64 *c, *c2;
   TRY(getbuf_t(ctx, WS_COMP, n, &c));
   TRY(getbuf_t(ctx, WS_COMP2, n, &c2));
