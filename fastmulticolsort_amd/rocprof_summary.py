@@ -4,6 +4,7 @@
     python -m fastmulticolsort_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR CMD_F CMD_W > traffic.json
     python -m fastmulticolsort_amd.rocprof_summary roofline TRACE_DIR FETCH_DIR WRITE_DIR BENCH_JSONL \
         CMD_T CMD_F CMD_W > kernels.md
+    python -m fastmulticolsort_amd.rocprof_summary counters KERNELS DIR [DIR ...] > counters.md
 
 `stats` reads DIR/*kernel_stats.csv (--kernel-trace --stats), or the
 rocpd database DIR/*.db that rocprofv3 writes by default, and prints a
@@ -18,6 +19,12 @@ command printed (`bytes_in_process` / `calls_in_process` of every kernel
 row): per kernel, the traced average duration, algorithmic bytes per
 launch, their ratio against the 8 TB/s HBM peak, and the counted HBM bytes
 per dispatch.  There the probe is the process's first copy dispatch.
+`counters` reads the --pmc databases of one or more runs (each counter
+summed over its instances, averaged over a kernel's dispatches) and prints,
+for the kernels whose names contain one of the comma-separated KERNELS, the
+shares of wave time the SQ counters give: VALU / LDS instruction activity
+and waits, per-wave VALU instructions, LDS bank-conflict cycles per LDS
+instruction.
 """
 import csv
 import glob
@@ -199,8 +206,45 @@ def traffic(fdir, wdir, cmd_f, cmd_w):
             "kernels": kern}
 
 
+def counter_means(dirs):
+    """{kernel: {counter: mean over dispatches of the instance sum}} over the
+    rocpd databases of several --pmc runs."""
+    out = defaultdict(dict)
+    for d in dirs:
+        db = _db(d)
+        if db is None:
+            raise SystemExit(f"no counter database under {d}")
+        for name, cname, total, disp in db.execute(
+                "select kernel_name, counter_name, sum(value), count(distinct dispatch_id) "
+                "from counters_collection group by kernel_name, counter_name"):
+            out[short(name)][cname] = float(total) / max(1, disp)
+    return out
+
+
+def counters(kernels, dirs):
+    agg = counter_means(dirs)
+    wanted = [k.strip() for k in kernels.split(",") if k.strip()]
+    out = ["| kernel | waves | VALU instr / wave | VALU active % | LDS active % | waiting for LDS issue % | "
+           "waiting (any) % | waiting for any issue % | LDS bank-conflict cycles / LDS instr |",
+           "|---|---|---|---|---|---|---|---|---|"]
+    for name, a in sorted(agg.items()):
+        if not any(w in name for w in wanted) or "SQ_WAVE_CYCLES" not in a:
+            continue
+        wc = a["SQ_WAVE_CYCLES"] or 1.0
+        pct = lambda c: 100.0 * a.get(c, 0.0) / wc  # noqa: E731
+        waves = a.get("SQ_WAVES", 0.0)
+        vpw = a.get("SQ_INSTS_VALU", 0.0) / waves if waves else float("nan")
+        bc = a.get("SQ_LDS_BANK_CONFLICT", 0.0) / a["SQ_INSTS_LDS"] if a.get("SQ_INSTS_LDS") else float("nan")
+        out.append(f"| {name} | {waves:.0f} | {vpw:.0f} | {pct('SQ_ACTIVE_INST_VALU'):.1f} | "
+                   f"{pct('SQ_ACTIVE_INST_LDS'):.1f} | {pct('SQ_WAIT_INST_LDS'):.1f} | {pct('SQ_WAIT_ANY'):.1f} | "
+                   f"{pct('SQ_WAIT_INST_ANY'):.1f} | {bc:.2f} |")
+    return "\n".join(out)
+
+
 def main(argv):
-    if argv[0] == "stats":
+    if argv[0] == "counters":
+        print(counters(argv[1], argv[2:]))
+    elif argv[0] == "stats":
         print(stats(argv[1], argv[2]))
     elif argv[0] == "traffic":
         print(json.dumps(traffic(argv[1], argv[2], argv[3], argv[4]), indent=1))
