@@ -320,7 +320,8 @@ int klaunch(fmcs_ctx* ctx, const char* name, void (*kern)(KA...), u64 grid, doub
 // FUSE writes (key << fshift) | payload composites to pout instead of keys.
 // kside (no payload): also write every key >> fshift as a u32.  LSD: the
 // pass covers one segment (the kernel skips the tile -> segment search).
-template <class K, int RB = RADIX_BITS, class P = NoPay, bool FUSE = false, bool LSD = false>
+template <class K, int RB = RADIX_BITS, class P = NoPay, bool FUSE = false, bool LSD = false,
+          int ITEMS = ONESWEEP_ITEMS>
 int scatter_pass(fmcs_ctx* ctx, const char* name, const K* kin, const u32* vin, K* kout, u32* vout, const u32* lo,
                  const u32* hi, const u32* tbase, const u32* nsegs_d, const u32* ntiles_d, u64 ntiles_max,
                  const u32* dbase, u32* tcount, u32 shift, u32 dbits, double bytes, const u32* dcount, double per,
@@ -337,7 +338,7 @@ int scatter_pass(fmcs_ctx* ctx, const char* name, const K* kin, const u32* vin, 
   TRY(klaunch(ctx, "k_tile_scan", kd_tscan_local<RB>, cap_grid(nch, 2048), 0.0, nullptr, 0.0, tcount, tbase, nsegs_d,
               ntiles_d, carry, cflag, tseg));
   TRY(klaunch(ctx, "k_tile_scan", kd_tscan_carry<RB>, R, 0.0, nullptr, 0.0, carry, (const u32*)cflag, ntiles_d));
-  auto kern = k_onesweep<K, ONESWEEP_ITEMS, RB, P, FUSE, LSD>;
+  auto kern = k_onesweep<K, ITEMS, RB, P, FUSE, LSD>;
   ctx->extra_next = extra;
   return klaunch(ctx, name, kern, onesweep_grid(ctx, (const void*)kern, ntiles_max), bytes, dcount, per, kin, vin,
                  kout, vout, lo, hi, tbase, 0u, dbase, (const u32*)tcount, (const u32*)carry, shift, dbits, nsegs_d,
@@ -932,9 +933,11 @@ int sort_by_group_t(fmcs_ctx* ctx, u32 keybits, u64 n, const K* column, u32* ord
 // ========================================================================
 constexpr u32 DD_LVL_MAX = 72;
 constexpr u32 DD_ROUNDS_MAX = 64;
-constexpr u32 DD_HDR = 256;  // [0, 64) group count per round, [72] key error, [76, 81) LSD consts, [88] segment count
+constexpr u32 DD_HDR = 256;  // [0, 64) group count per round, [72] key error, [76, 81) LSD consts, [88] segment count,
+                             // [96, 100) LSD consts of WIDE_TILE passes
 constexpr u32 DD_ERR = 72;
 constexpr u32 DD_CONST = 76;
+constexpr u32 DD_CONST_W = 96;  // one-segment constants of the WIDE_TILE passes
 constexpr u64 DD_WORDS = DD_HDR + (u64)DD_ROUNDS_MAX * DD_LVL_MAX * DD_LVL_WORDS;
 
 // The device-driven word area.  Its key-error word dd[DD_ERR] is sticky:
@@ -963,13 +966,14 @@ int dd_clear(fmcs_ctx* ctx, u32* dd, u64 words) {
 // the small and large rows.
 int dd_scan(fmcs_ctx* ctx, const u32* in, u32* out, const u32* n_d, u64 n_max, u32* st, u32* counter, u32* total,
             bool flags = false, u32 small_max = 0, u32* rows = nullptr, const u32* seg_start = nullptr,
-            u32* llo = nullptr, u32* lhi = nullptr, u32* ltiles = nullptr, u32* hist = nullptr) {
+            u32* llo = nullptr, u32* lhi = nullptr, u32* ltiles = nullptr, u32* hist = nullptr,
+            u32 tile_rows = ONESWEEP_TILE) {
   const u64 tiles_max = std::max<u64>(1, (n_max + SCAN_TILE - 1) / SCAN_TILE);
   // blocks take tiles from the counter, so a few hundred cover any list:
   // a larger grid only adds launch time (most lists are far below their bound)
   auto fn = flags ? kd_scan_excl<SCAN_ITEMS, true> : kd_scan_excl<SCAN_ITEMS, false>;
   return klaunch(ctx, "k_scan", fn, cap_grid(tiles_max, std::min<u64>(512, resident(ctx, (const void*)fn))), 0.0,
-                 nullptr, 0.0, in, out, n_d, st, counter, total, small_max, rows, seg_start, (u32)ONESWEEP_TILE, llo,
+                 nullptr, 0.0, in, out, n_d, st, counter, total, small_max, rows, seg_start, tile_rows, llo,
                  lhi, ltiles, hist);
 }
 
@@ -1092,7 +1096,10 @@ int radix_sort_dd_fused(fmcs_ctx* ctx, const K* kin, const u64* pin, u32 keybits
                         u32* side, u64 n, u32* dd) {
   constexpr u32 RB = RADIX_BITS, R = 1u << RB;
   const u32 npasses = (keybits + RB - 1) / RB;
+  // the fused first pass on ONESWEEP_TILE-row tiles, the composite passes
+  // on WIDE_TILE-row tiles (each with its own one-segment constants)
   const u64 ntiles = (n + ONESWEEP_TILE - 1) / ONESWEEP_TILE;
+  const u64 ntiles_w = (n + WIDE_TILE - 1) / WIDE_TILE;
   u32 *hist, *dbase, *tcount, *t0v, *t1v;
   u64 *t0c = nullptr, *t1c = nullptr;
   TRY(getbuf_t(ctx, WS_HIST, (u64)npasses * R, &hist));
@@ -1110,8 +1117,10 @@ int radix_sort_dd_fused(fmcs_ctx* ctx, const K* kin, const u64* pin, u32 keybits
   if (npasses > 1) TRY(getbuf_t(ctx, WS_DNEXT, n, &dnx));
   TRY(fill(ctx, hist, (u64)npasses * R, 0));
   u32* cst = dd + DD_CONST;
+  u32* csw = dd + DD_CONST_W;
   TRY(klaunch(ctx, "k_set", kd_set4, 1, 0.0, nullptr, 0.0, cst, 0u, (u32)n, 0u, (u32)ntiles));
   TRY(klaunch(ctx, "k_set", kd_set4, 1, 0.0, nullptr, 0.0, cst + 4, 1u, 0u, 0u, 0u));
+  if (npasses > 1) TRY(klaunch(ctx, "k_set", kd_set4, 1, 0.0, nullptr, 0.0, csw, 0u, (u32)n, 0u, (u32)ntiles_w));
   const u32* one = cst + 4;
   const u64 kbad = keybits < 8 * sizeof(K) ? ~((1ull << keybits) - 1) : 0ull;
   const u64 pbad = pbits < 64 ? ~((1ull << pbits) - 1) : 0ull;
@@ -1129,9 +1138,9 @@ int radix_sort_dd_fused(fmcs_ctx* ctx, const K* kin, const u64* pin, u32 keybits
                   (const u32*)cst + 2, one, (const u32*)cst + 3, 0u, dbits, hp, tcount, kbad, dd + DD_ERR,
                   (u32*)nullptr));
     } else {
-      TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<uint8_t, ONESWEEP_ITEMS, RB>, cap_grid(ntiles, 2048),
-                  (double)n, nullptr, 0.0, (const uint8_t*)dnx, (const u32*)cst, (const u32*)cst + 1,
-                  (const u32*)cst + 2, one, (const u32*)cst + 3, 0u, dbits, hp, tcount, 0ull, (u32*)nullptr,
+      TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<uint8_t, WIDE_ITEMS, RB>, cap_grid(ntiles_w, 2048),
+                  (double)n, nullptr, 0.0, (const uint8_t*)dnx, (const u32*)csw, (const u32*)csw + 1,
+                  (const u32*)csw + 2, one, (const u32*)csw + 3, 0u, dbits, hp, tcount, 0ull, (u32*)nullptr,
                   (u32*)nullptr));
     }
     TRY(klaunch(ctx, "k_digit_base", kd_digit_base<RB>, 1, 0.0, nullptr, 0.0, (const u32*)hp, (const u32*)cst,
@@ -1149,7 +1158,7 @@ int radix_sort_dd_fused(fmcs_ctx* ctx, const K* kin, const u64* pin, u32 keybits
       // 2(w + 4) a row for the composite pass; the next digit (a byte a
       // row) or, on the last pass, the sorted c1 (4 bytes a row) besides
       ctx->extra_next = last ? 4.0 * n : 1.0 * n;
-      TRY((scatter_pass<u64, RB, NoPay, false, true>)(ctx, "k_onesweep", sc, sv, dc, dv, cst, cst + 1, cst + 2, one, cst + 3, ntiles,
+      TRY((scatter_pass<u64, RB, NoPay, false, true, WIDE_ITEMS>)(ctx, "k_onesweep", sc, sv, dc, dv, csw, csw + 1, csw + 2, one, csw + 3, ntiles_w,
                             dbase + (u64)p * R, tcount, pbits + p * RB, dbits,
                             2.0 * n * (8 + 4), nullptr, 0.0, (const NoPay*)nullptr,
                             (NoPay*)nullptr, pbits, 0ull, (u32*)nullptr, last ? side : (u32*)nullptr,
@@ -1200,7 +1209,7 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
       d.tkb = kb;
       d.packable = kb + TS_SEGBITS <= 64;
       d.Wl = std::min<u32>(W, d.sm + 1);
-      d.wpt = (ONESWEEP_TILE + d.Wl - 1) / d.Wl;
+      d.wpt = (WIDE_TILE + d.Wl - 1) / d.Wl;
       d.nseg_max = cur_max;
       d.nw_max = L.empty() ? (n + d.Wl - 1) / d.Wl : prev_tiles * d.wpt;
       // segments of sm + 1 .. 2T rows (at most the largest tile) sort alone
@@ -1209,7 +1218,7 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
       d.single_max = persist ? med_max : std::min<u32>(2 * T, (u32)TILESORT_TMAX);
       d.nsingle_max = std::min<u64>(cur_max, n / ((u64)d.sm + 1));
       d.nlarge_max = std::min<u64>(cur_max, n / ((u64)d.single_max + 1));
-      d.ntiles_max = std::min<u64>(n / ONESWEEP_TILE + d.nlarge_max, n);
+      d.ntiles_max = std::min<u64>(n / WIDE_TILE + d.nlarge_max, n);
       d.last = (kb == 0);
       if (!d.last) {
         d.D = std::min<u32>(msd_bits, kb);
@@ -1322,7 +1331,7 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
     // per-kernel byte figures of a profiled run
     TRY(dd_scan(ctx, csz, nullptr, nseg_p, d.nseg_max, scan_st, S + LV_SCAN_A, S + LV_NLARGE, true, d.single_max,
                 (ctx->prof && !ctx->capturing) ? S + LV_ROWS_SMALL : nullptr, cst, llo, lhi, ltiles,
-                d.last ? nullptr : hist));
+                d.last ? nullptr : hist, WIDE_TILE));
     TileSortArgs<K> a;
     std::memset(&a, 0, sizeof(a));
     a.kin[0] = keys; a.vin[0] = vals; a.kin[1] = kb_; a.vin[1] = vb_;
@@ -1395,10 +1404,10 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
       // odd level, every key bit consumed: move the large segments back
       TRY(klaunch(ctx, "kd_copy_segs", kd_copy_segs<K>, cap_grid(d.ntiles_max, 2048), 0.0, S + LV_ROWS_LARGE,
                   2.0 * (sizeof(K) + 4), skey, sval, keys, vals, (const u32*)llo, (const u32*)lhi, (const u32*)tbase,
-                  (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES), (u32)ONESWEEP_TILE));
+                  (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES), (u32)WIDE_TILE));
       break;
     }
-    TRY(klaunch(ctx, "k_seg_hist", kd_seg_hist<K, ONESWEEP_ITEMS, RADIX_BITS>, cap_grid(d.ntiles_max, 2048), 0.0,
+    TRY(klaunch(ctx, "k_seg_hist", kd_seg_hist<K, WIDE_ITEMS, RADIX_BITS>, cap_grid(d.ntiles_max, 2048), 0.0,
                 S + LV_ROWS_LARGE, (double)sizeof(K), (const K*)skey, (const u32*)llo, (const u32*)lhi,
                 (const u32*)tbase, (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES), d.shift, d.D, hist,
                 tcount, a.bad, a.err, tseg));
@@ -1407,7 +1416,7 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
                 (const u32*)hist, (const u32*)llo, dbase, persist ? plist + loff[l + 1] : lst[nxt], lsz[nxt],
                 (const u32*)(S + LV_NLARGE),
                 l + 1 < L.size() ? S + DD_LVL_WORDS + LV_NSEG : nullptr));
-    TRY(scatter_pass<K>(ctx, "k_onesweep_msd", skey, sval, dkey, dval, llo, lhi, tbase, S + LV_NLARGE,
+    TRY((scatter_pass<K, RADIX_BITS, NoPay, false, false, WIDE_ITEMS>)(ctx, "k_onesweep_msd", skey, sval, dkey, dval, llo, lhi, tbase, S + LV_NLARGE,
                         S + LV_NTILES, d.ntiles_max, dbase, tcount, d.shift, d.D, 0.0, S + LV_ROWS_LARGE,
                         2.0 * (sizeof(K) + 4), (const NoPay*)nullptr, (NoPay*)nullptr, 0u, 0ull, (u32*)nullptr,
                         (u32*)nullptr, (uint8_t*)nullptr, 0u, tseg));

@@ -547,14 +547,16 @@ __global__ void kd_copy1(const u32* __restrict__ src, u32* __restrict__ dst) {
 #define FMCS_DD_INST(K)                                                                                            \
   template __global__ void kd_seg_hist<K, ONESWEEP_ITEMS, 8>(const K*, const u32*, const u32*, const u32*,           \
                                               const u32*, const u32*, u32, u32, u32*, u32*, u64, u32*, u32*);      \
+  template __global__ void kd_seg_hist<K, WIDE_ITEMS, 8>(const K*, const u32*, const u32*, const u32*,               \
+                                              const u32*, const u32*, u32, u32, u32*, u32*, u64, u32*, u32*);      \
   template __global__ void kd_copy_segs<K>(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*,   \
                                            const u32*, const u32*, u32);                            \
   template __global__ void kd_copy_pairs<K>(const K*, const u32*, K*, u32*, u64);                                  \
   template __global__ void kd_copy_col<K>(const K*, K*, u64);
 // the next-digit bytes of the packed round-1 passes
-template __global__ void kd_seg_hist<uint8_t, ONESWEEP_ITEMS, 8>(const uint8_t*, const u32*, const u32*, const u32*,
-                                                                 const u32*, const u32*, u32, u32, u32*, u32*, u64,
-                                                                 u32*, u32*);
+template __global__ void kd_seg_hist<uint8_t, WIDE_ITEMS, 8>(const uint8_t*, const u32*, const u32*, const u32*,
+                                                             const u32*, const u32*, u32, u32, u32*, u32*, u64, u32*,
+                                                             u32*);
 FMCS_DD_INST(u16)
 FMCS_DD_INST(u32)
 FMCS_DD_INST(u64)

@@ -1265,12 +1265,12 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
 // ---------------------------------------------------------------------------
 // Explicit instantiations used by the launchers.
 // ---------------------------------------------------------------------------
-#define FMCS_INST_ONESWEEP_F(K, P, F, L)                                                                       \
-  template __global__ void k_onesweep<K, ONESWEEP_ITEMS, 8, P, F, L>(const K*, const u32*, K*, u32*, const u32*,     \
-                                                                     const u32*, const u32*, u32, const u32*,       \
-                                                                     const u32*, const u32*, u32, u32, const u32*,   \
-                                                                     const u32*, const P*, PayOut<P, F>*, u32, u64, \
-                                                                     u32*, u32*, uint8_t*, u32, const u32*);
+#define FMCS_INST_ONESWEEP_IF(K, I, P, F, L)                                                                    \
+  template __global__ void k_onesweep<K, I, 8, P, F, L>(const K*, const u32*, K*, u32*, const u32*, const u32*,      \
+                                                        const u32*, u32, const u32*, const u32*, const u32*, u32, u32, \
+                                                        const u32*, const u32*, const P*, PayOut<P, F>*, u32, u64,   \
+                                                        u32*, u32*, uint8_t*, u32, const u32*);
+#define FMCS_INST_ONESWEEP_F(K, P, F, L) FMCS_INST_ONESWEEP_IF(K, ONESWEEP_ITEMS, P, F, L)
 #define FMCS_INST_ONESWEEP(K, P) FMCS_INST_ONESWEEP_F(K, P, false, true)
 #define FMCS_INST_K(K)                                                                                              \
   template __global__ void k_gather<K>(const K*, const u32*, K*, u64);                                              \
@@ -1279,6 +1279,8 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
   template __global__ void k_bwrite<K, BOUNDARY_ITEMS>(const K*, const u32*, u64, u32, const u32*, u32*, u32*, u32*); \
   template __global__ void k_hist_all<K>(const K*, u64, u32, u32, u32*);                                            \
   FMCS_INST_ONESWEEP_F(K, NoPay, false, false)                                                                      \
+  FMCS_INST_ONESWEEP_IF(K, WIDE_ITEMS, NoPay, false, false)                                                         \
+  FMCS_INST_ONESWEEP_IF(K, WIDE_ITEMS, NoPay, false, true)                                                          \
   FMCS_INST_ONESWEEP(K, NoPay)                                                                                      \
   FMCS_INST_ONESWEEP(K, u16)                                                                                        \
   FMCS_INST_ONESWEEP(K, u32)                                                                                        \
