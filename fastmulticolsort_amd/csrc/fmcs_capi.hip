@@ -321,8 +321,8 @@ int klaunch(fmcs_ctx* ctx, const char* name, void (*kern)(KA...), u64 grid, doub
 // kside (no payload): also write every key >> fshift as a u32.  LSD: the
 // pass covers one segment (the kernel skips the tile -> segment search).
 template <class K, int RB = RADIX_BITS, class P = NoPay, bool FUSE = false, bool LSD = false,
-          int ITEMS = ONESWEEP_ITEMS>
-int scatter_pass(fmcs_ctx* ctx, const char* name, const K* kin, const u32* vin, K* kout, u32* vout, const u32* lo,
+          int ITEMS = ONESWEEP_ITEMS, class KO = K>
+int scatter_pass(fmcs_ctx* ctx, const char* name, const K* kin, const u32* vin, KO* kout, u32* vout, const u32* lo,
                  const u32* hi, const u32* tbase, const u32* nsegs_d, const u32* ntiles_d, u64 ntiles_max,
                  const u32* dbase, u32* tcount, u32 shift, u32 dbits, double bytes, const u32* dcount, double per,
                  const P* pin = nullptr, PayOut<P, FUSE>* pout = nullptr, u32 fshift = 0, u64 pbad = 0,
@@ -338,7 +338,7 @@ int scatter_pass(fmcs_ctx* ctx, const char* name, const K* kin, const u32* vin, 
   TRY(klaunch(ctx, "k_tile_scan", kd_tscan_local<RB>, cap_grid(nch, 2048), 0.0, nullptr, 0.0, tcount, tbase, nsegs_d,
               ntiles_d, carry, cflag, tseg));
   TRY(klaunch(ctx, "k_tile_scan", kd_tscan_carry<RB>, R, 0.0, nullptr, 0.0, carry, (const u32*)cflag, ntiles_d));
-  auto kern = k_onesweep<K, ITEMS, RB, P, FUSE, LSD>;
+  auto kern = k_onesweep<K, ITEMS, RB, P, FUSE, LSD, KO>;
   ctx->extra_next = extra;
   return klaunch(ctx, name, kern, onesweep_grid(ctx, (const void*)kern, ntiles_max), bytes, dcount, per, kin, vin,
                  kout, vout, lo, hi, tbase, 0u, dbase, (const u32*)tcount, (const u32*)carry, shift, dbits, nsegs_d,
@@ -1264,7 +1264,7 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
   if (persist) TRY(getbuf_t(ctx, WS_PLIST, loff[L.size()], &plist));
   K* kb_;
   u32* vb_;
-  TRY(getbuf_t(ctx, WS_KB, n, &kb_));
+  TRY(getbuf_t(ctx, WS_KB, n + 2, &kb_));  // + 2: two aligned u32 arrays when the keys narrow
   TRY(getbuf_t(ctx, WS_VB, n, &vb_));
   u32 *lst[2], *lsz[2];
   TRY(getbuf_t(ctx, WS_ST, std::max<u64>(m_list[0], 1), &lst[0]));
@@ -1301,15 +1301,122 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
   u32 *rkl, *rkl2;
   TRY(getbuf_t(ctx, WS_RKL, m_nw, &rkl));
   TRY(getbuf_t(ctx, WS_RKL2, m_single, &rkl2));
-  // the tile kernels' grid: resident blocks of the chosen tile shape
-  auto fn_bucket = T <= 512 ? k_tile_bucket<K, 2> : (T <= 1024 ? k_tile_bucket<K, 4> : k_tile_bucket<K, 8>);
-  auto fn_tsort = T <= 512 ? k_tile_sort<K, 2> : (T <= 1024 ? k_tile_sort<K, 4> : k_tile_sort<K, 8>);
-  auto fn_trank = T <= 512 ? k_tile_rank<K, 2> : (T <= 1024 ? k_tile_rank<K, 4> : k_tile_rank<K, 8>);
-  // single windows above T rows: the next tile size
-  const u32 T2 = std::min<u32>(2 * T, (u32)TILESORT_TMAX);
-  auto fn_bucket2 = T2 <= 512 ? k_tile_bucket<K, 2> : (T2 <= 1024 ? k_tile_bucket<K, 4> : k_tile_bucket<K, 8>);
-  auto fn_tsort2 = T2 <= 512 ? k_tile_sort<K, 2> : (T2 <= 1024 ? k_tile_sort<K, 4> : k_tile_sort<K, 8>);
-  auto fn_trank2 = T2 <= 512 ? k_tile_rank<K, 2> : (T2 <= 1024 ? k_tile_rank<K, 4> : k_tile_rank<K, 8>);
+  // Narrowed keys: in a round whose sorted keys are not wanted, with 64-bit
+  // keys of which at most 32 bits are left to sort after the first MSD level,
+  // every later level moves u32 keys (the bits still to sort): the later MSD
+  // passes and the windows after the first level read and write 4 key bytes
+  // a row instead of 8.  The first level scatters its large segments' keys
+  // into the first u32 array; later levels alternate between two u32 arrays
+  // carved from the level buffer.
+  const bool narrow = !keys_out && !persist && sizeof(K) == 8 && L.size() >= 2 && !L[0].last && L[1].kb <= 32;
+  u32* k32[2] = {nullptr, nullptr};  // the keys of levels l >= 1 are in k32[l & 1]
+  if (narrow) {
+    const u64 half = (n + 3) & ~3ull;  // the second array 16-byte aligned
+    k32[1] = (u32*)kb_;
+    k32[0] = (u32*)kb_ + half;
+  }
+  // The windows of level l (rows in the level's buffers kin0 / kin1 by
+  // parity, keys of type KL): the bucket sort, the refused windows to the
+  // radix tile sort (and the comparison rank), then the single-segment
+  // windows.
+  auto level_tiles = [&](auto* kin0, auto* kin1, size_t l, const DdLevel& d, u32* S, u32* cst, u64 bad,
+                         u32* errp) -> int {
+    using KL = typename std::remove_pointer<decltype(kin0)>::type;
+    auto fn_bucket = T <= 512 ? k_tile_bucket<KL, 2> : (T <= 1024 ? k_tile_bucket<KL, 4> : k_tile_bucket<KL, 8>);
+    auto fn_tsort = T <= 512 ? k_tile_sort<KL, 2> : (T <= 1024 ? k_tile_sort<KL, 4> : k_tile_sort<KL, 8>);
+    auto fn_trank = T <= 512 ? k_tile_rank<KL, 2> : (T <= 1024 ? k_tile_rank<KL, 4> : k_tile_rank<KL, 8>);
+    // single windows above T rows: the next tile size
+    const u32 T2 = std::min<u32>(2 * T, (u32)TILESORT_TMAX);
+    auto fn_bucket2 = T2 <= 512 ? k_tile_bucket<KL, 2> : (T2 <= 1024 ? k_tile_bucket<KL, 4> : k_tile_bucket<KL, 8>);
+    auto fn_tsort2 = T2 <= 512 ? k_tile_sort<KL, 2> : (T2 <= 1024 ? k_tile_sort<KL, 4> : k_tile_sort<KL, 8>);
+    auto fn_trank2 = T2 <= 512 ? k_tile_rank<KL, 2> : (T2 <= 1024 ? k_tile_rank<KL, 4> : k_tile_rank<KL, 8>);
+    TileSortArgs<KL> a;
+    std::memset(&a, 0, sizeof(a));
+    a.kin[0] = kin0; a.vin[0] = vals; a.kin[1] = kin1; a.vin[1] = vb_;
+    if constexpr (std::is_same<KL, K>::value) a.kout = keys_out ? keys : nullptr;
+    a.vout = vals;
+    a.tile_lo = tlo; a.tile_hi = thi; a.tile_s0 = ts0; a.tile_s1 = ts1; a.seg_start = cst;
+    a.src_sel = (u32)(l & 1); a.kb = d.tkb; a.pad_seg = T; a.direct = 0; a.wlist = wlist;
+    a.wcount = S + LV_WCOUNT; a.fblist = fblist; a.fbcount = S + LV_FB;
+    a.size_lo = 0; a.size_hi = ~0u;
+    a.bad = bad; a.err = errp;
+    {
+      const u64 g = (persist_grid(ctx, (const void*)fn_bucket, d.nw_max) + XCDS - 1) / XCDS * XCDS;
+      // rows read (key, oid) and written (oid, and the key when wanted)
+      const double per = keys_out ? 2.0 * (sizeof(KL) + 4) : sizeof(KL) + 8.0;
+      TRY(klaunch(ctx, "k_tile_bucket", fn_bucket, g, 0.0, S + LV_ROWS_SMALL, per, a, fb, a.tile_lo, a.tile_hi,
+                  a.tile_s0, a.tile_s1, a.wlist));
+    }
+    {
+      // refused windows: the radix tile sort over the bits that vary inside
+      // each; where even those leave no room for the segment index (keys
+      // wider than 53 bits), the comparison rank
+      TileSortArgs<KL> b = a;
+      b.wcount = nullptr;
+      b.rklist = rkl; b.rkcount = S + LV_RK;
+      const u64 g = persist_grid(ctx, (const void*)fn_tsort, d.nw_max);
+      TRY(klaunch(ctx, "k_tile_sort", fn_tsort, g, 0.0, nullptr, 0.0, b));
+      if (!d.packable) {
+        b.fblist = rkl; b.fbcount = S + LV_RK;
+        TRY(klaunch(ctx, "k_tile_rank", fn_trank, g, 0.0, nullptr, 0.0, b));
+      }
+    }
+    if (d.nsingle_max) {
+      // the single-segment windows (in segment order): up to T rows in
+      // T-row tiles, then the larger ones in 2T-row tiles; refused windows of
+      // either size go to the 2T-row radix tile sort
+      TileSortArgs<KL> c = a;
+      c.tile_lo = slo; c.tile_hi = shi; c.tile_s0 = sseg; c.tile_s1 = sseg;
+      c.wlist = nullptr; c.wcount = S + LV_NSINGLE; c.fblist = fblist2; c.fbcount = S + LV_FB2;
+      c.size_lo = 0; c.size_hi = T;
+      const u64 g = (persist_grid(ctx, (const void*)fn_bucket, d.nsingle_max) + XCDS - 1) / XCDS * XCDS;
+      TRY(klaunch(ctx, "k_tile_bucket", fn_bucket, g, 0.0, nullptr, 0.0, c, fb, c.tile_lo, c.tile_hi, c.tile_s0,
+                  c.tile_s1, c.wlist));
+      if (d.single_max > T) {
+        c.size_lo = T; c.size_hi = d.single_max;
+        const u64 gb = (persist_grid(ctx, (const void*)fn_bucket2, d.nsingle_max) + XCDS - 1) / XCDS * XCDS;
+        TRY(klaunch(ctx, "k_tile_bucket_2t", fn_bucket2, gb, 0.0, nullptr, 0.0, c, fb, c.tile_lo, c.tile_hi,
+                    c.tile_s0, c.tile_s1, c.wlist));
+      }
+      c.wcount = nullptr;
+      c.rklist = rkl2; c.rkcount = S + LV_RK2;
+      const u64 g2 = persist_grid(ctx, (const void*)fn_tsort2, d.nsingle_max);
+      TRY(klaunch(ctx, "k_tile_sort", fn_tsort2, g2, 0.0, nullptr, 0.0, c));
+      if (!d.packable) {
+        c.fblist = rkl2; c.fbcount = S + LV_RK2;
+        TRY(klaunch(ctx, "k_tile_rank", fn_trank2, g2, 0.0, nullptr, 0.0, c));
+      }
+    }
+    return FMCS_OK;
+  };
+  // The MSD level of level l over its large segments: keys of type KI from
+  // skey into dkey (type KO: the keys' low bits when narrower), oids from
+  // sval into dval; on an odd last level (every key bit consumed) the large
+  // segments only move back to the final rows.
+  auto level_msd = [&](const auto* skey, auto* dkey, const u32* sval, u32* dval, size_t l, const DdLevel& d, u32* S,
+                       int cur, u64 bad, u32* errp) -> int {
+    using KI = typename std::remove_const<typename std::remove_pointer<decltype(skey)>::type>::type;
+    using KO = typename std::remove_pointer<decltype(dkey)>::type;
+    if (d.last) {
+      KI* kfin = nullptr;
+      if constexpr (std::is_same<KI, K>::value) kfin = keys_out ? keys : nullptr;
+      return klaunch(ctx, "kd_copy_segs", kd_copy_segs<KI>, cap_grid(d.ntiles_max, 2048), 0.0, S + LV_ROWS_LARGE,
+                     2.0 * (sizeof(KI) + 4), skey, sval, kfin, vals, (const u32*)llo, (const u32*)lhi,
+                     (const u32*)tbase, (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES), (u32)WIDE_TILE);
+    }
+    TRY(klaunch(ctx, "k_seg_hist", kd_seg_hist<KI, WIDE_ITEMS, RADIX_BITS>, cap_grid(d.ntiles_max, 2048), 0.0,
+                S + LV_ROWS_LARGE, (double)sizeof(KI), skey, (const u32*)llo, (const u32*)lhi,
+                (const u32*)tbase, (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES), d.shift, d.D, hist,
+                tcount, bad, errp, tseg));
+    const int nxt = cur ^ 1;
+    TRY(klaunch(ctx, "k_digit_base", kd_digit_base<RADIX_BITS>, cap_grid(d.nlarge_max, 2048), 0.0, nullptr, 0.0,
+                (const u32*)hist, (const u32*)llo, dbase, persist ? plist + loff[l + 1] : lst[nxt], lsz[nxt],
+                (const u32*)(S + LV_NLARGE), l + 1 < L.size() ? S + DD_LVL_WORDS + LV_NSEG : nullptr));
+    return scatter_pass<KI, RADIX_BITS, NoPay, false, false, WIDE_ITEMS, KO>(
+        ctx, "k_onesweep_msd", skey, sval, dkey, dval, llo, lhi, tbase, S + LV_NLARGE, S + LV_NTILES, d.ntiles_max,
+        dbase, tcount, d.shift, d.D, 0.0, S + LV_ROWS_LARGE, sizeof(KI) + 8.0 + sizeof(KO), (const NoPay*)nullptr,
+        (NoPay*)nullptr, 0u, 0ull, (u32*)nullptr, (u32*)nullptr, (uint8_t*)nullptr, 0u, tseg);
+  };
   TRY(klaunch(ctx, "k_offsets_to_list", kd_offsets_to_list, cap_grid((nseg_max + 255) / 256, 2048), 0.0, nullptr, 0.0,
               seg_off, nseg_d, persist ? plist : lst[0], lsz[0]));
   const u32* plo = nullptr;
@@ -1332,94 +1439,33 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
     TRY(dd_scan(ctx, csz, nullptr, nseg_p, d.nseg_max, scan_st, S + LV_SCAN_A, S + LV_NLARGE, true, d.single_max,
                 (ctx->prof && !ctx->capturing) ? S + LV_ROWS_SMALL : nullptr, cst, llo, lhi, ltiles,
                 d.last ? nullptr : hist, WIDE_TILE));
-    TileSortArgs<K> a;
-    std::memset(&a, 0, sizeof(a));
-    a.kin[0] = keys; a.vin[0] = vals; a.kin[1] = kb_; a.vin[1] = vb_;
-    a.kout = keys_out ? keys : nullptr; a.vout = vals;
-    a.tile_lo = tlo; a.tile_hi = thi; a.tile_s0 = ts0; a.tile_s1 = ts1; a.seg_start = cst;
-    a.src_sel = (u32)(l & 1); a.kb = d.tkb; a.pad_seg = T; a.direct = 0; a.wlist = wlist;
-    a.wcount = S + LV_WCOUNT; a.fblist = fblist; a.fbcount = S + LV_FB;
-    a.size_lo = 0; a.size_hi = ~0u;
-    if (l == 0 && err && keybits < 8 * sizeof(K)) {
-      // level 0 reads every row (window rows here, large-segment rows in its
-      // histogram), so it checks the declared width of every key
-      a.bad = ~((1ull << keybits) - 1);
-      a.err = err;
-    }
+    // level 0 reads every row (window rows in the tile sorts, large-segment
+    // rows in its histogram), so it checks the declared width of every key
+    const bool check0 = l == 0 && err && keybits < 8 * sizeof(K);
+    const u64 bad = check0 ? ~((1ull << keybits) - 1) : 0ull;
+    u32* errp = check0 ? err : nullptr;
     if (!persist) {
-      const u64 g = (persist_grid(ctx, (const void*)fn_bucket, d.nw_max) + XCDS - 1) / XCDS * XCDS;
-      // rows read (key, oid) and written (oid, and the key when wanted)
-      const double per = keys_out ? 2.0 * (sizeof(K) + 4) : sizeof(K) + 8.0;
-      TRY(klaunch(ctx, "k_tile_bucket", fn_bucket, g, 0.0, S + LV_ROWS_SMALL, per, a, fb, a.tile_lo, a.tile_hi,
-                  a.tile_s0, a.tile_s1, a.wlist));
-    }
-    if (!persist) {
-      // refused windows: the radix tile sort over the bits that vary inside
-      // each; where even those leave no room for the segment index (keys
-      // wider than 53 bits), the comparison rank
-      TileSortArgs<K> b = a;
-      b.wcount = nullptr;
-      b.rklist = rkl; b.rkcount = S + LV_RK;
-      const u64 g = persist_grid(ctx, (const void*)fn_tsort, d.nw_max);
-      TRY(klaunch(ctx, "k_tile_sort", fn_tsort, g, 0.0, nullptr, 0.0, b));
-      if (!d.packable) {
-        b.fblist = rkl; b.fbcount = S + LV_RK;
-        TRY(klaunch(ctx, "k_tile_rank", fn_trank, g, 0.0, nullptr, 0.0, b));
-      }
-    }
-    if (d.nsingle_max && !persist) {
-      // the single-segment windows (in segment order): up to T rows in
-      // T-row tiles, then the larger ones in 2T-row tiles; refused windows of
-      // either size go to the 2T-row radix tile sort
-      TileSortArgs<K> c = a;
-      c.tile_lo = slo; c.tile_hi = shi; c.tile_s0 = sseg; c.tile_s1 = sseg;
-      c.wlist = nullptr; c.wcount = S + LV_NSINGLE; c.fblist = fblist2; c.fbcount = S + LV_FB2;
-      c.size_lo = 0; c.size_hi = T;
-      const u64 g = (persist_grid(ctx, (const void*)fn_bucket, d.nsingle_max) + XCDS - 1) / XCDS * XCDS;
-      TRY(klaunch(ctx, "k_tile_bucket", fn_bucket, g, 0.0, nullptr, 0.0, c, fb, c.tile_lo, c.tile_hi, c.tile_s0,
-                  c.tile_s1, c.wlist));
-      if (d.single_max > T) {
-        c.size_lo = T; c.size_hi = d.single_max;
-        const u64 gb = (persist_grid(ctx, (const void*)fn_bucket2, d.nsingle_max) + XCDS - 1) / XCDS * XCDS;
-        TRY(klaunch(ctx, "k_tile_bucket_2t", fn_bucket2, gb, 0.0, nullptr, 0.0, c, fb, c.tile_lo, c.tile_hi,
-                    c.tile_s0, c.tile_s1, c.wlist));
-      }
-      c.wcount = nullptr;
-      c.rklist = rkl2; c.rkcount = S + LV_RK2;
-      const u64 g2 = persist_grid(ctx, (const void*)fn_tsort2, d.nsingle_max);
-      TRY(klaunch(ctx, "k_tile_sort", fn_tsort2, g2, 0.0, nullptr, 0.0, c));
-      if (!d.packable) {
-        c.fblist = rkl2; c.fbcount = S + LV_RK2;
-        TRY(klaunch(ctx, "k_tile_rank", fn_trank2, g2, 0.0, nullptr, 0.0, c));
-      }
+      const int rc = (narrow && l > 0) ? level_tiles(k32[0], k32[1], l, d, S, cst, bad, errp)
+                                       : level_tiles(keys, kb_, l, d, S, cst, bad, errp);
+      if (rc != FMCS_OK) return rc;
     }
     if (d.nlarge_max == 0) break;
-    const K* skey = (l & 1) ? kb_ : keys;
     const u32* sval = (l & 1) ? vb_ : vals;
-    K* dkey = (l & 1) ? keys : kb_;
     u32* dval = (l & 1) ? vals : vb_;
     if (d.last && !(l & 1)) break;  // every key bit consumed; large segments already in place
     TRY(dd_scan(ctx, ltiles, tbase, S + LV_NLARGE, d.nlarge_max, scan_st2, S + LV_SCAN_B, S + LV_NTILES));
-    if (d.last) {
-      // odd level, every key bit consumed: move the large segments back
-      TRY(klaunch(ctx, "kd_copy_segs", kd_copy_segs<K>, cap_grid(d.ntiles_max, 2048), 0.0, S + LV_ROWS_LARGE,
-                  2.0 * (sizeof(K) + 4), skey, sval, keys, vals, (const u32*)llo, (const u32*)lhi, (const u32*)tbase,
-                  (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES), (u32)WIDE_TILE));
-      break;
-    }
-    TRY(klaunch(ctx, "k_seg_hist", kd_seg_hist<K, WIDE_ITEMS, RADIX_BITS>, cap_grid(d.ntiles_max, 2048), 0.0,
-                S + LV_ROWS_LARGE, (double)sizeof(K), (const K*)skey, (const u32*)llo, (const u32*)lhi,
-                (const u32*)tbase, (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES), d.shift, d.D, hist,
-                tcount, a.bad, a.err, tseg));
-    const int nxt = cur ^ 1;
-    TRY(klaunch(ctx, "k_digit_base", kd_digit_base<RADIX_BITS>, cap_grid(d.nlarge_max, 2048), 0.0, nullptr, 0.0,
-                (const u32*)hist, (const u32*)llo, dbase, persist ? plist + loff[l + 1] : lst[nxt], lsz[nxt],
-                (const u32*)(S + LV_NLARGE),
-                l + 1 < L.size() ? S + DD_LVL_WORDS + LV_NSEG : nullptr));
-    TRY((scatter_pass<K, RADIX_BITS, NoPay, false, false, WIDE_ITEMS>)(ctx, "k_onesweep_msd", skey, sval, dkey, dval, llo, lhi, tbase, S + LV_NLARGE,
-                        S + LV_NTILES, d.ntiles_max, dbase, tcount, d.shift, d.D, 0.0, S + LV_ROWS_LARGE,
-                        2.0 * (sizeof(K) + 4), (const NoPay*)nullptr, (NoPay*)nullptr, 0u, 0ull, (u32*)nullptr,
-                        (u32*)nullptr, (uint8_t*)nullptr, 0u, tseg));
+    // the level's MSD pass over its large segments: keys from the level's
+    // buffer into the next level's (u32 from the first level on when narrowed)
+    int rc;
+    if (!narrow)
+      rc = level_msd((l & 1) ? (const K*)kb_ : (const K*)keys, (l & 1) ? keys : kb_, sval, dval, l, d, S, cur, bad,
+                     errp);
+    else if (l == 0)
+      rc = level_msd((const K*)keys, k32[1], sval, dval, l, d, S, cur, bad, errp);
+    else
+      rc = level_msd((const u32*)k32[l & 1], k32[(l & 1) ^ 1], sval, dval, l, d, S, cur, bad, errp);
+    if (rc != FMCS_OK) return rc;
+    if (d.last) break;
     plo = llo;
     ptb = tbase;
   }

@@ -518,7 +518,7 @@ __global__ void kd_copy_segs(const K* __restrict__ kin, const u32* __restrict__ 
     const u64 r0 = (u64)llo[seg] + (u64)(tile - tile_base[seg]) * tile_rows;
     const u64 r1 = min((u64)lhi[seg], r0 + tile_rows);
     for (u64 r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-      kout[r] = kin[r];
+      if (kout) kout[r] = kin[r];
       vout[r] = vin[r];
     }
     __syncthreads();

@@ -94,8 +94,8 @@ struct NoPay {};
 // (key << fshift) | payload when fused
 template <class P, bool FUSE>
 using PayOut = typename std::conditional<FUSE, u64, P>::type;
-template <class K, int ITEMS, int RB, class P, bool FUSE, bool LSD>
-__global__ void k_onesweep(const K*, const u32*, K*, u32*, const u32*, const u32*, const u32*, u32, const u32*,
+template <class K, int ITEMS, int RB, class P, bool FUSE, bool LSD, class KO = K>
+__global__ void k_onesweep(const K*, const u32*, KO*, u32*, const u32*, const u32*, const u32*, u32, const u32*,
                            const u32*, const u32*, u32, u32, const u32*, const u32*, const P*, PayOut<P, FUSE>*, u32,
                            u64, u32*, u32*, uint8_t*, u32, const u32*);
 template <class K, int E>

@@ -409,10 +409,12 @@ __global__ __launch_bounds__(256) void k_digit_base(const u32* __restrict__ hist
 // the high part of composite keys, for a boundary pass over that part alone.
 // LSD: one segment [seg_lo[0], seg_hi[0]) tiled from tile 0 (no search for
 // a tile's segment).  Else tile_seg (if set) holds every tile's segment;
-// without it the segment is found by search over tile_base.
-template <class K, int ITEMS, int RB, class P, bool FUSE, bool LSD>
+// without it the segment is found by search over tile_base.  KO: the type
+// the keys leave in (narrower than K: the low bits of every key, which an
+// MSD level leaves to sort when they fit KO).
+template <class K, int ITEMS, int RB, class P, bool FUSE, bool LSD, class KO>
 __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, const u32* __restrict__ vin,
-                                                  K* __restrict__ kout, u32* __restrict__ vout,
+                                                  KO* __restrict__ kout, u32* __restrict__ vout,
                                                   const u32* __restrict__ seg_lo, const u32* __restrict__ seg_hi,
                                                   const u32* __restrict__ tile_base, u32 nsegs,
                                                   const u32* __restrict__ digit_base, const u32* __restrict__ tile_pre,
@@ -586,7 +588,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
         const u32 i = it * NT + tid;
         const K k = s_keys[i];
         const u32 g = s_goff[digit_of(k, shift, mask)] + i;
-        if (!FUSE) kout[g] = k;
+        if (!FUSE) kout[g] = (KO)k;
         vout[g] = s_vals[i];
         if (dnext) dnext[g] = (uint8_t)((u64)k >> nshift);
         gpos[it] = g;
@@ -598,7 +600,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
         if (i < cnt) {
           const K k = s_keys[i];
           const u32 g = s_goff[digit_of(k, shift, mask)] + i;
-          if (!FUSE) kout[g] = k;
+          if (!FUSE) kout[g] = (KO)k;
           vout[g] = s_vals[i];
           if (dnext) dnext[g] = (uint8_t)((u64)k >> nshift);
           gpos[it] = g;
@@ -634,7 +636,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
       const u32 i = it * NT + tid;
       const K k = s_keys[i];
       const u32 g = s_goff[digit_of(k, shift, mask)] + i;
-      kout[g] = k;
+      kout[g] = (KO)k;
       vout[g] = s_vals[i];
       if (kside) kside[g] = (u32)((u64)k >> fshift);
       if (dnext) dnext[g] = (uint8_t)((u64)k >> nshift);
@@ -643,7 +645,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
     for (u32 i = tid; i < cnt; i += NT) {
       K k = s_keys[i];
       u32 g = s_goff[digit_of(k, shift, mask)] + i;
-      kout[g] = k;
+      kout[g] = (KO)k;
       vout[g] = s_vals[i];
       if (kside) kside[g] = (u32)((u64)k >> fshift);
       if (dnext) dnext[g] = (uint8_t)((u64)k >> nshift);
@@ -1308,6 +1310,11 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
 
 FMCS_INST_K(u16)
 FMCS_INST_K(u32)
+// the MSD level that narrows u64 keys to the 32 bits still to sort
+template __global__ void k_onesweep<u64, WIDE_ITEMS, 8, NoPay, false, false, u32>(
+    const u64*, const u32*, u32*, u32*, const u32*, const u32*, const u32*, u32, const u32*, const u32*, const u32*,
+    u32, u32, const u32*, const u32*, const NoPay*, PayOut<NoPay, false>*, u32, u64, u32*, u32*, uint8_t*, u32,
+    const u32*);
 FMCS_INST_ONESWEEP_F(u16, u64, true, true)
 FMCS_INST_ONESWEEP_F(u32, u64, true, true)
 template __global__ void k_hash_insert<uint8_t>(const uint8_t*, const u32*, u64, u32*, u32, u32*, u32*);
