@@ -26,7 +26,7 @@ enum Slot {
   WS_ST, WS_SZ, WS_BST, WS_BSZ, WS_FLAG, WS_POS, WS_LLO, WS_LHI, WS_LTILES, WS_TBASE, WS_TLO, WS_THI, WS_TS0,
   WS_TS1, WS_COMP, WS_COMP2, WS_GROUPTMP, WS_MISC, WS_HASH, WS_HASH2, WS_HASH3, WS_HT_TABLE, WS_HT_FIRST,
   WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_DD, WS_SCANST, WS_FBLIST, WS_TCOUNT, WS_CARRY, WS_BCNT, WS_T0P, WS_T1P, WS_PAY, WS_SGL, WS_FBLIST2, WS_RKL, WS_RKL2, WS_DNEXT, WS_TSEG, WS_HG_KEY, WS_HG_POS, WS_HG_SKEY, WS_HG_SPOS, WS_HG_F, WS_HG_AUX, WS_HG_PSTART,
-  WS_HG_TKEY, WS_HG_TFIRST, WS_HG_CHUNK, WS_PLIST, WS_BINSC, WS_FBWIN, WS_FBPOOL, WS_UNB0, WS_UNB1, WS_UNBK, WS_USER0,
+  WS_HG_TKEY, WS_HG_TFIRST, WS_HG_CHUNK, WS_PLIST, WS_BINSC, WS_FBWIN, WS_FBPOOL, WS_UNB0, WS_UNB1, WS_UNBK, WS_DIG0, WS_USER0,
   WS_STAGE0 = WS_USER0 + 24,  // device copies of staged host buffers (one slot per argument)
   WS_COUNT = WS_STAGE0 + 72
 };
@@ -1090,10 +1090,13 @@ int boundary_dd(fmcs_ctx* ctx, const K* v, const u32* in_group, u64 n, u32* out_
 // carried u64).  With more than one pass, the last one also writes the sorted
 // c1 keys to side (u32), which a boundary pass reads at half the bytes of
 // the composites.  Keys above keybits, and c2 values above pbits, set
-// dd[DD_ERR].  Needs n >= 2, keybits >= 1 and keybits + pbits <= 64.
+// dd[DD_ERR].  Needs n >= 2, keybits >= 1 and keybits + pbits <= 64.  With
+// more than one pass and dig set, the last pass also writes every row's
+// composite >> dig_shift as a byte there (the digit the second round's first
+// MSD level sorts on, which its histogram then reads instead of the keys).
 template <class K>
 int radix_sort_dd_fused(fmcs_ctx* ctx, const K* kin, const u64* pin, u32 keybits, u32 pbits, u64* cout, u32* vout,
-                        u32* side, u64 n, u32* dd) {
+                        u32* side, u64 n, u32* dd, uint8_t* dig = nullptr, u32 dig_shift = 0) {
   constexpr u32 RB = RADIX_BITS, R = 1u << RB;
   const u32 npasses = (keybits + RB - 1) / RB;
   // the fused first pass on ONESWEEP_TILE-row tiles, the composite passes
@@ -1157,12 +1160,12 @@ int radix_sort_dd_fused(fmcs_ctx* ctx, const K* kin, const u64* pin, u32 keybits
     } else {
       // 2(w + 4) a row for the composite pass; the next digit (a byte a
       // row) or, on the last pass, the sorted c1 (4 bytes a row) besides
-      ctx->extra_next = last ? 4.0 * n : 1.0 * n;
+      ctx->extra_next = last ? (dig ? 5.0 : 4.0) * n : 1.0 * n;
       TRY((scatter_pass<u64, RB, NoPay, false, true, WIDE_ITEMS>)(ctx, "k_onesweep", sc, sv, dc, dv, csw, csw + 1, csw + 2, one, csw + 3, ntiles_w,
                             dbase + (u64)p * R, tcount, pbits + p * RB, dbits,
                             2.0 * n * (8 + 4), nullptr, 0.0, (const NoPay*)nullptr,
                             (NoPay*)nullptr, pbits, 0ull, (u32*)nullptr, last ? side : (u32*)nullptr,
-                            last ? (uint8_t*)nullptr : dnx, pbits + (p + 1) * RB));
+                            last ? dig : dnx, last ? dig_shift : pbits + (p + 1) * RB));
     }
     sc = dc;
     sv = dv;
@@ -1182,7 +1185,7 @@ struct DdLevel {
 // sorted keys back (keys ends unspecified).
 template <class K>
 int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32* seg_off, const u32* nseg_d,
-                u64 nseg_max, u32* lv, bool keys_out = true, u32* err = nullptr) {
+                u64 nseg_max, u32* lv, bool keys_out = true, u32* err = nullptr, const uint8_t* dig0 = nullptr) {
   if (n <= 1 || nseg_max == 0) return FMCS_OK;
   // The persistent form (tune.med_max set): the levels only split segments
   // above med_max rows; one k_seg_bins launch then sorts every level's
@@ -1404,10 +1407,18 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
                      2.0 * (sizeof(KI) + 4), skey, sval, kfin, vals, (const u32*)llo, (const u32*)lhi,
                      (const u32*)tbase, (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES), (u32)WIDE_TILE);
     }
-    TRY(klaunch(ctx, "k_seg_hist", kd_seg_hist<KI, WIDE_ITEMS, RADIX_BITS>, cap_grid(d.ntiles_max, 2048), 0.0,
-                S + LV_ROWS_LARGE, (double)sizeof(KI), skey, (const u32*)llo, (const u32*)lhi,
-                (const u32*)tbase, (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES), d.shift, d.D, hist,
-                tcount, bad, errp, tseg));
+    if (l == 0 && dig0)
+      // the first level's digits as bytes (written by the previous round's
+      // last radix pass), read instead of the keys
+      TRY(klaunch(ctx, "k_seg_hist", kd_seg_hist<uint8_t, WIDE_ITEMS, RADIX_BITS>, cap_grid(d.ntiles_max, 2048), 0.0,
+                  S + LV_ROWS_LARGE, 1.0, dig0, (const u32*)llo, (const u32*)lhi, (const u32*)tbase,
+                  (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES), 0u, d.D, hist, tcount, 0ull,
+                  (u32*)nullptr, tseg));
+    else
+      TRY(klaunch(ctx, "k_seg_hist", kd_seg_hist<KI, WIDE_ITEMS, RADIX_BITS>, cap_grid(d.ntiles_max, 2048), 0.0,
+                  S + LV_ROWS_LARGE, (double)sizeof(KI), skey, (const u32*)llo, (const u32*)lhi,
+                  (const u32*)tbase, (const u32*)(S + LV_NLARGE), (const u32*)(S + LV_NTILES), d.shift, d.D, hist,
+                  tcount, bad, errp, tseg));
     const int nxt = cur ^ 1;
     TRY(klaunch(ctx, "k_digit_base", kd_digit_base<RADIX_BITS>, cap_grid(d.nlarge_max, 2048), 0.0, nullptr, 0.0,
                 (const u32*)hist, (const u32*)llo, dbase, persist ? plist + loff[l + 1] : lst[nxt], lsz[nxt],
@@ -1565,7 +1576,7 @@ int gather_launch(fmcs_ctx* ctx, const K* column, const u32* order, K* out, u64 
 template <class K, class P = NoPay>
 int dd_round(fmcs_ctx* ctx, u32 r, u32 keybits, u64 n, const K* column, u32* order, const u32* gin,
              const u32* soff, u32* gout, u32* oout, bool want_groups, u32* dd, K* carried = nullptr,
-             const P* pnext = nullptr, P* pout = nullptr, bool check_keys = true) {
+             const P* pnext = nullptr, P* pout = nullptr, bool check_keys = true, const uint8_t* dig0 = nullptr) {
   K* ka = carried;
   if (!ka) TRY(getbuf_t(ctx, WS_KA, n, &ka));
   if (r == 0) {
@@ -1576,7 +1587,7 @@ int dd_round(fmcs_ctx* ctx, u32 r, u32 keybits, u64 n, const K* column, u32* ord
     u32* lv = dd + DD_HDR + (u64)r * DD_LVL_MAX * DD_LVL_WORDS;
     // the sorted keys feed only this round's boundary
     TRY(seg_sort_dd<K>(ctx, ka, order, n, keybits, soff, dd + (r - 1), n, lv, want_groups,
-                       check_keys ? dd + DD_ERR : nullptr));
+                       check_keys ? dd + DD_ERR : nullptr, dig0));
   }
   if (want_groups) TRY(boundary_dd<K>(ctx, ka, gin, n, gout, oout, dd + r));
   return FMCS_OK;
@@ -1621,6 +1632,8 @@ int mrs_dd(fmcs_ctx* ctx, u32 nrounds, const void* const* cols, const u32* banks
   if (ctx->tune.carry_next && nrounds >= 2 && n > 1) TRY(getbuf(ctx, WS_PAY, (size_t)n * banks[1], &carried));
   const u32* gin = nullptr;
   const u32* soff = nullptr;
+  uint8_t* dig0 = nullptr;  // the packed form's round-2 first digits (see radix_sort_dd_fused)
+  u32 dig_shift = 0;
   for (u32 r = 0; r < nrounds; ++r) {
     const bool last = (r + 1 == nrounds);
     const bool want_groups = !last || groups_d;
@@ -1634,10 +1647,17 @@ int mrs_dd(fmcs_ctx* ctx, u32 nrounds, const void* const* cols, const u32* banks
       u64* comp = (u64*)carried;
       u32* side;
       TRY(getbuf_t(ctx, WS_KA, n, &side));
+      // with two or more round-1 passes the last one also writes the second
+      // round's first MSD digit of every row as a byte
+      if (bw[0] > RADIX_BITS) {
+        const u32 d0 = std::min<u32>(std::max<u32>(1, std::min<u32>(ctx->tune.msd_bits, RADIX_BITS)), bw[1]);
+        TRY(getbuf_t(ctx, WS_DIG0, n, &dig0));
+        dig_shift = bw[1] - d0;
+      }
       rc = banks[0] == 2 ? radix_sort_dd_fused<u16>(ctx, (const u16*)cols[0], (const u64*)cols[1], bw[0], bw[1],
-                                                    comp, order, side, n, dd)
+                                                    comp, order, side, n, dd, dig0, dig_shift)
                          : radix_sort_dd_fused<u32>(ctx, (const u32*)cols[0], (const u64*)cols[1], bw[0], bw[1],
-                                                    comp, order, side, n, dd);
+                                                    comp, order, side, n, dd, dig0, dig_shift);
       // round-1 groups: the sorted c1 keys (one pass: the composites' c1 part)
       if (rc == FMCS_OK && want_groups)
         rc = bw[0] > RADIX_BITS ? boundary_dd<u32>(ctx, side, nullptr, n, gout, oout, dd + 0)
@@ -1655,7 +1675,7 @@ int mrs_dd(fmcs_ctx* ctx, u32 nrounds, const void* const* cols, const u32* banks
       switch (banks[r]) {
         case 2: rc = dd_round<u16>(ctx, r, bw[r], n, (const u16*)cols[r], order, gin, soff, gout, oout, want_groups, dd, (u16*)kc, (const NoPay*)nullptr, (NoPay*)nullptr, check); break;
         case 4: rc = dd_round<u32>(ctx, r, bw[r], n, (const u32*)cols[r], order, gin, soff, gout, oout, want_groups, dd, (u32*)kc, (const NoPay*)nullptr, (NoPay*)nullptr, check); break;
-        default: rc = dd_round<u64>(ctx, r, bw[r], n, (const u64*)cols[r], order, gin, soff, gout, oout, want_groups, dd, (u64*)kc, (const NoPay*)nullptr, (NoPay*)nullptr, check); break;
+        default: rc = dd_round<u64>(ctx, r, bw[r], n, (const u64*)cols[r], order, gin, soff, gout, oout, want_groups, dd, (u64*)kc, (const NoPay*)nullptr, (NoPay*)nullptr, check, r == 1 ? dig0 : nullptr); break;
       }
     }
     if (rc != FMCS_OK) return rc;
