@@ -183,6 +183,9 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="sweep the batched sort's bin thresholds and tile shapes first and save the best")
     ap.add_argument("--mode", choices=["graph", "device", "batched"], default="graph")
+    ap.add_argument("--small-max", type=int, default=None,
+                    help="tuning small_max (largest segment packed into a window tile; default: the tuning file)")
+    ap.add_argument("--window", type=int, default=None, help="tuning window (default: the tuning file)")
     ap.add_argument("--med-max", type=int, default=None,
                     help="tuning med_max: 0 = level-synchronous tile sorts, 1024..16384 = the persistent "
                          "size-binned launch with single segments up to this many rows (default: the tuning file)")
@@ -229,10 +232,14 @@ def main():
         tuning = best
     else:
         tuning = autotune.load(ctx)
-    if args.med_max is not None:
+    if args.small_max is not None or args.window is not None or args.med_max is not None:
         t = ctx.tuning()
-        ctx.set_tuning(min(t["small_max"], 512) if args.med_max else t["small_max"], t["window"], t["msd_bits"],
-                       med_max=args.med_max)
+        sm = args.small_max if args.small_max is not None else t["small_max"]
+        mm = args.med_max if args.med_max is not None else t["med_max"]
+        if mm:
+            sm = min(sm, 512)  # the persistent form's window tiles are 1024 rows
+        w = min(args.window if args.window is not None else t["window"], sm + 1)
+        ctx.set_tuning(sm, w, t["msd_bits"], med_max=mm)
         tuning = ctx.tuning()
     log(f"tuning: {tuning or ctx.tuning()}")
 
