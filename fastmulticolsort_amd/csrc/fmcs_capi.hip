@@ -398,7 +398,10 @@ int radix_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, 
   u32 *hist, *dbase, *tcount;
   TRY(getbuf_t(ctx, WS_HIST, (u64)npasses * RADIX, &hist));
   TRY(getbuf_t(ctx, WS_DBASE, (u64)npasses * RADIX, &dbase));
-  u64 ntiles = (n + ONESWEEP_TILE - 1) / ONESWEEP_TILE;
+  // 64-bit keys on the taller tiles (longer digit runs)
+  constexpr int IT = sizeof(K) == 8 ? WIDE_ITEMS : ONESWEEP_ITEMS;
+  constexpr u64 TILE = 256 * IT;
+  u64 ntiles = (n + TILE - 1) / TILE;
   TRY(getbuf_t(ctx, WS_TCOUNT, ntiles * RADIX, &tcount));
   TRY(fill(ctx, hist, (u64)npasses * RADIX, 0));
   {
@@ -464,10 +467,10 @@ int radix_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, 
       dk = (i & 1) ? t1k : t0k;
       dv = (i & 1) ? t1v : t0v;
     }
-    TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<K, ONESWEEP_ITEMS, RADIX_BITS>, cap_grid(ntiles, 2048),
+    TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<K, IT, RADIX_BITS>, cap_grid(ntiles, 2048),
                 (double)n * sizeof(K), nullptr, 0.0, sk, cst, cst + 1, cst + 2, cst + 4, cst + 3, p * RADIX_BITS,
                 (u32)RADIX_BITS, (u32*)nullptr, tcount, (u64)0, (u32*)nullptr, (u32*)nullptr));
-    TRY((scatter_pass<K, RADIX_BITS, NoPay, false, true>)(ctx, "k_onesweep", sk, sv, dk, dv, cst, cst + 1, cst + 2,
+    TRY((scatter_pass<K, RADIX_BITS, NoPay, false, true, IT>)(ctx, "k_onesweep", sk, sv, dk, dv, cst, cst + 1, cst + 2,
                                                          cst + 4, cst + 3, ntiles,
                         dbase + (u64)p * RADIX, tcount, p * RADIX_BITS, (u32)RADIX_BITS, 2.0 * n * (sizeof(K) + 4),
                         nullptr, 0.0));
@@ -997,7 +1000,9 @@ int radix_sort_dd_t(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* v
       return klaunch(ctx, "kd_copy_pairs", kd_copy_pairs<K>, 1, 0.0, nullptr, 0.0, kin, vin, kout, vout, n);
     return FMCS_OK;
   }
-  const u64 ntiles = (n + ONESWEEP_TILE - 1) / ONESWEEP_TILE;
+  // 64-bit keys without a payload on the taller tiles (longer digit runs)
+  constexpr int IT = (sizeof(K) == 8 && !HASP) ? WIDE_ITEMS : ONESWEEP_ITEMS;
+  const u64 ntiles = (n + 256 * IT - 1) / (256 * IT);
   u32 *hist, *dbase, *tcount;
   TRY(getbuf_t(ctx, WS_HIST, (u64)npasses * R, &hist));
   TRY(getbuf_t(ctx, WS_DBASE, (u64)npasses * R, &dbase));
@@ -1045,13 +1050,13 @@ int radix_sort_dd_t(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* v
     u32* hp = hist + (u64)p * R;
     // the first pass also flags keys with bits above keybits
     const u64 bad = (p == 0 && keybits < 8 * sizeof(K)) ? ~((1ull << keybits) - 1) : 0ull;
-    TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<K, ONESWEEP_ITEMS, RB>, cap_grid(ntiles, 2048), (double)n * sizeof(K),
+    TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<K, IT, RB>, cap_grid(ntiles, 2048), (double)n * sizeof(K),
                 nullptr, 0.0, sk, (const u32*)cst, (const u32*)cst + 1, (const u32*)cst + 2, one, (const u32*)cst + 3,
                 p * RB, (u32)RB, hp, tcount, bad, dd + DD_ERR, (u32*)nullptr));
     TRY(klaunch(ctx, "k_digit_base", kd_digit_base<RB>, 1, 0.0, nullptr, 0.0, (const u32*)hp, (const u32*)cst,
                 dbase + (u64)p * R, (u32*)nullptr, (u32*)nullptr, one, (u32*)nullptr));
     const double pb = HASP ? 2.0 * n * sizeof(P) : 0.0;
-    const int rc = scatter_pass<K, RB, P, false, true>(ctx, "k_onesweep", sk, sv, dk, dv, cst, cst + 1, cst + 2, one, cst + 3,
+    const int rc = scatter_pass<K, RB, P, false, true, IT>(ctx, "k_onesweep", sk, sv, dk, dv, cst, cst + 1, cst + 2, one, cst + 3,
                                           ntiles, dbase + (u64)p * R, tcount, p * RB, (u32)RB,
                                           2.0 * n * (sizeof(K) + 4) + pb, nullptr, 0.0, sp, dp);
     if (rc != FMCS_OK) return rc;
