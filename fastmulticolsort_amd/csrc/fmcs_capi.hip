@@ -382,9 +382,12 @@ int scan_excl(fmcs_ctx* ctx, const u32* in, u32* out, u64 n, u32* total_d, bool 
 // ------------------------------------------------------------------------
 // LSD radix sort of (kin, vin) -> (kout, vout) over the low keybits bits.
 // Ping-pongs through two workspace pairs; never writes a buffer a pass reads.
+// strict: a key with a bit at or above keybits is refused (FMCS_E_ARG);
+// otherwise the bits above keybits are ignored (a partition on low bits).
 // ------------------------------------------------------------------------
 template <class K>
-int radix_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, u64 n, u32 keybits) {
+int radix_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, u64 n, u32 keybits,
+               bool strict = true) {
   if (n == 0) return FMCS_OK;
   if (n >= (1ull << 30)) return fail(ctx, FMCS_E_UNSUPPORTED, "radix sort supports n < 2^30");
   keybits = std::min<u32>(keybits, 8 * sizeof(K));
@@ -394,45 +397,37 @@ int radix_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, 
     if (vout != vin) HIPCHK(hipMemcpyAsync(vout, vin, n * sizeof(u32), hipMemcpyDeviceToDevice, ctx->stream));
     return FMCS_OK;
   }
-  u32 last_bits = keybits - (npasses - 1) * RADIX_BITS;
   u32 *hist, *dbase, *tcount;
-  TRY(getbuf_t(ctx, WS_HIST, (u64)npasses * RADIX, &hist));
+  // every pass's digit counts, then the flag of keys above keybits
+  TRY(getbuf_t(ctx, WS_HIST, (u64)npasses * RADIX + 1, &hist));
   TRY(getbuf_t(ctx, WS_DBASE, (u64)npasses * RADIX, &dbase));
   // 64-bit keys on the taller tiles (longer digit runs)
   constexpr int IT = sizeof(K) == 8 ? WIDE_ITEMS : ONESWEEP_ITEMS;
   constexpr u64 TILE = 256 * IT;
   u64 ntiles = (n + TILE - 1) / TILE;
   TRY(getbuf_t(ctx, WS_TCOUNT, ntiles * RADIX, &tcount));
-  TRY(fill(ctx, hist, (u64)npasses * RADIX, 0));
+  TRY(fill(ctx, hist, (u64)npasses * RADIX + 1, 0));
   {
     u32 blocks = (u32)std::min<u64>((n + 255) / 256, 2048);
-    // the last pass's digit is masked to last_bits inside k_hist_all via a
-    // key mask applied here: keys are assumed < 2^keybits, and the same mask
-    // is used by the passes, so histogram and scatter always agree.
     Prof pf(ctx, "k_hist_all", (double)n * sizeof(K));
-    hipLaunchKernelGGL((k_hist_all<K>), dim3(blocks), dim3(256), 0, ctx->stream, kin, n, npasses, (u32)RADIX_BITS, hist);
+    const u64 bad = strict && keybits < 64 ? ~((1ull << keybits) - 1) : 0ull;
+    hipLaunchKernelGGL((k_hist_all<K>), dim3(blocks), dim3(256), 0, ctx->stream, kin, n, npasses, keybits, bad, hist);
     TRY(launch_check(ctx, "k_hist_all"));
   }
   // per-pass trivial detection (one digit holds every row) and digit bases
-  std::vector<u32> h((size_t)npasses * RADIX);
-  TRY(readback(ctx, hist, npasses * RADIX, h.data()));
+  std::vector<u32> h((size_t)npasses * RADIX + 1);
+  TRY(readback(ctx, hist, npasses * RADIX + 1, h.data()));
+  if (h[(size_t)npasses * RADIX]) return fail(ctx, FMCS_E_ARG, "keys exceed keybits");
   hipLaunchKernelGGL(k_digit_base, dim3(npasses), dim3(256), 0, ctx->stream, hist, (const u32*)nullptr, dbase,
                      (u32*)nullptr, (u32*)nullptr);
   TRY(launch_check(ctx, "k_digit_base"));
   std::vector<u32> passes;
   for (u32 p = 0; p < npasses; ++p) {
-    u32 bits = (p == npasses - 1) ? last_bits : RADIX_BITS;
     u64 tot = 0, mx = 0;
     for (u32 d = 0; d < RADIX; ++d) {
       tot += h[p * RADIX + d];
       mx = std::max<u64>(mx, h[p * RADIX + d]);
     }
-    // digits above `bits` in the last pass would break the scatter: the
-    // histogram used 8-bit digits, so the pass must too unless all keys fit.
-    bool fits = true;
-    if (bits < RADIX_BITS)
-      for (u32 d = (1u << bits); d < RADIX; ++d) fits = fits && (h[p * RADIX + d] == 0);
-    if (!fits) return fail(ctx, FMCS_E_ARG, "keys exceed keybits");
     if (mx != tot) passes.push_back(p);
   }
   K *t0k, *t1k;
@@ -467,12 +462,13 @@ int radix_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, 
       dk = (i & 1) ? t1k : t0k;
       dv = (i & 1) ? t1v : t0v;
     }
+    const u32 dsh = digit_shift(keybits, npasses, p), dw = digit_width(keybits, npasses, p);
     TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<K, IT, RADIX_BITS>, cap_grid(ntiles, 2048),
-                (double)n * sizeof(K), nullptr, 0.0, sk, cst, cst + 1, cst + 2, cst + 4, cst + 3, p * RADIX_BITS,
-                (u32)RADIX_BITS, (u32*)nullptr, tcount, (u64)0, (u32*)nullptr, (u32*)nullptr));
+                (double)n * sizeof(K), nullptr, 0.0, sk, cst, cst + 1, cst + 2, cst + 4, cst + 3, dsh, dw,
+                (u32*)nullptr, tcount, (u64)0, (u32*)nullptr, (u32*)nullptr));
     TRY((scatter_pass<K, RADIX_BITS, NoPay, false, true, IT>)(ctx, "k_onesweep", sk, sv, dk, dv, cst, cst + 1, cst + 2,
                                                          cst + 4, cst + 3, ntiles,
-                        dbase + (u64)p * RADIX, tcount, p * RADIX_BITS, (u32)RADIX_BITS, 2.0 * n * (sizeof(K) + 4),
+                        dbase + (u64)p * RADIX, tcount, dsh, dw, 2.0 * n * (sizeof(K) + 4),
                         nullptr, 0.0));
     sk = dk;
     sv = dv;
@@ -1050,14 +1046,15 @@ int radix_sort_dd_t(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* v
     u32* hp = hist + (u64)p * R;
     // the first pass also flags keys with bits above keybits
     const u64 bad = (p == 0 && keybits < 8 * sizeof(K)) ? ~((1ull << keybits) - 1) : 0ull;
+    const u32 dsh = digit_shift(keybits, npasses, p), dw = digit_width(keybits, npasses, p);
     TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<K, IT, RB>, cap_grid(ntiles, 2048), (double)n * sizeof(K),
                 nullptr, 0.0, sk, (const u32*)cst, (const u32*)cst + 1, (const u32*)cst + 2, one, (const u32*)cst + 3,
-                p * RB, (u32)RB, hp, tcount, bad, dd + DD_ERR, (u32*)nullptr));
+                dsh, dw, hp, tcount, bad, dd + DD_ERR, (u32*)nullptr));
     TRY(klaunch(ctx, "k_digit_base", kd_digit_base<RB>, 1, 0.0, nullptr, 0.0, (const u32*)hp, (const u32*)cst,
                 dbase + (u64)p * R, (u32*)nullptr, (u32*)nullptr, one, (u32*)nullptr));
     const double pb = HASP ? 2.0 * n * sizeof(P) : 0.0;
     const int rc = scatter_pass<K, RB, P, false, true, IT>(ctx, "k_onesweep", sk, sv, dk, dv, cst, cst + 1, cst + 2, one, cst + 3,
-                                          ntiles, dbase + (u64)p * R, tcount, p * RB, (u32)RB,
+                                          ntiles, dbase + (u64)p * R, tcount, dsh, dw,
                                           2.0 * n * (sizeof(K) + 4) + pb, nullptr, 0.0, sp, dp);
     if (rc != FMCS_OK) return rc;
     sk = dk;
@@ -1099,11 +1096,15 @@ int boundary_dd(fmcs_ctx* ctx, const K* v, const u32* in_group, u64 n, u32* out_
 // more than one pass and dig set, the last pass also writes every row's
 // composite >> dig_shift as a byte there (the digit the second round's first
 // MSD level sorts on, which its histogram then reads instead of the keys).
+// The key bits are split over the passes by digit_width / digit_shift.
 template <class K>
 int radix_sort_dd_fused(fmcs_ctx* ctx, const K* kin, const u64* pin, u32 keybits, u32 pbits, u64* cout, u32* vout,
                         u32* side, u64 n, u32* dd, uint8_t* dig = nullptr, u32 dig_shift = 0) {
   constexpr u32 RB = RADIX_BITS, R = 1u << RB;
   const u32 npasses = (keybits + RB - 1) / RB;
+  // pass p sorts c1's bits dshift(p) .. dshift(p) + dwidth(p) - 1
+  auto dwidth = [&](u32 p) { return digit_width(keybits, npasses, p); };
+  auto dshift = [&](u32 p) { return digit_shift(keybits, npasses, p); };
   // the fused first pass on ONESWEEP_TILE-row tiles, the composite passes
   // on WIDE_TILE-row tiles (each with its own one-segment constants)
   const u64 ntiles = (n + ONESWEEP_TILE - 1) / ONESWEEP_TILE;
@@ -1138,7 +1139,7 @@ int radix_sort_dd_fused(fmcs_ctx* ctx, const K* kin, const u64* pin, u32 keybits
     const bool last = (p + 1 == npasses);
     u64* dc = last ? cout : ((p & 1) ? t1c : t0c);
     u32* dv = last ? vout : ((p & 1) ? t1v : t0v);
-    const u32 dbits = std::min<u32>(RB, keybits - p * RB);
+    const u32 dbits = dwidth(p);
     u32* hp = hist + (u64)p * R;
     if (p == 0) {
       TRY(klaunch(ctx, "k_tile_hist", kd_seg_hist<K, ONESWEEP_ITEMS, RB>, cap_grid(ntiles, 2048),
@@ -1154,23 +1155,23 @@ int radix_sort_dd_fused(fmcs_ctx* ctx, const K* kin, const u64* pin, u32 keybits
     TRY(klaunch(ctx, "k_digit_base", kd_digit_base<RB>, 1, 0.0, nullptr, 0.0, (const u32*)hp, (const u32*)cst,
                 dbase + (u64)p * R, (u32*)nullptr, (u32*)nullptr, one, (u32*)nullptr));
     if (p == 0) {
-      // the next digit: c1's bits RB .. 2 RB - 1 (a byte a row beside the
+      // the next digit: c1's bits from dshift(1) (a byte a row beside the
       // pass's own bytes: c1 and c2 read, composite and oid written)
       ctx->extra_next = last ? 0.0 : 1.0 * n;
       TRY((scatter_pass<K, RB, u64, true, true>(ctx, "k_onesweep_fuse", kin, (const u32*)nullptr, (K*)nullptr, dv, cst, cst + 1,
                                           cst + 2, one, cst + 3, ntiles, dbase, tcount, 0u, dbits,
                                           (double)n * (sizeof(K) + 8 + 8 + 4), nullptr, 0.0, pin, dc,
                                           pbits, pbad, dd + DD_ERR, (u32*)nullptr, last ? (uint8_t*)nullptr : dnx,
-                                          RB)));
+                                          last ? 0u : dshift(1))));
     } else {
       // 2(w + 4) a row for the composite pass; the next digit (a byte a
       // row) or, on the last pass, the sorted c1 (4 bytes a row) besides
       ctx->extra_next = last ? (dig ? 5.0 : 4.0) * n : 1.0 * n;
       TRY((scatter_pass<u64, RB, NoPay, false, true, WIDE_ITEMS>)(ctx, "k_onesweep", sc, sv, dc, dv, csw, csw + 1, csw + 2, one, csw + 3, ntiles_w,
-                            dbase + (u64)p * R, tcount, pbits + p * RB, dbits,
+                            dbase + (u64)p * R, tcount, pbits + dshift(p), dbits,
                             2.0 * n * (8 + 4), nullptr, 0.0, (const NoPay*)nullptr,
                             (NoPay*)nullptr, pbits, 0ull, (u32*)nullptr, last ? side : (u32*)nullptr,
-                            last ? dig : dnx, last ? dig_shift : pbits + (p + 1) * RB));
+                            last ? dig : dnx, last ? dig_shift : pbits + dshift(p + 1)));
     }
     sc = dc;
     sv = dv;
@@ -1827,7 +1828,7 @@ int hash_first_partitioned(fmcs_ctx* ctx, const K* keys, const u32* seg, u64 n, 
   TRY(klaunch(ctx, "k_hash_prep", k_hash_prep<K>, cap_grid((n + 255) / 256, 8192), rd + 12.0 * n, nullptr, 0.0, keys,
               seg, n, kb, hk, pos));
   // stable partition of (hash, position) on the hash's low PB bits
-  TRY(radix_sort<u64>(ctx, hk, pos, hs, ps, n, PB));
+  TRY(radix_sort<u64>(ctx, hk, pos, hs, ps, n, PB, false));
   u32* nlarge_d = ctx->small_d + 61;
   u32* marker = ctx->small_d + 62;
   TRY(fill(ctx, nlarge_d, 1, 0));

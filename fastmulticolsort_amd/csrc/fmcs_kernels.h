@@ -65,6 +65,18 @@ constexpr int ONESWEEP_TILE = 256 * ONESWEEP_ITEMS;
 // against 1.198); 12 and 24 rows were slower everywhere.
 constexpr int WIDE_ITEMS = 20;
 constexpr int WIDE_TILE = 256 * WIDE_ITEMS;
+// Digit split of an LSD sort over keybits bits in npasses = ceil(keybits / RB)
+// passes: widths as even as possible, the narrower digits first.  A pass's
+// time grows steeply with its number of digit values (each tile's output
+// runs get shorter than a few cache lines), so 20 bits sort as 6 + 7 + 7
+// rather than 8 + 8 + 4.  Pass p sorts bits digit_shift .. + digit_width - 1.
+__host__ __device__ inline u32 digit_width(u32 keybits, u32 npasses, u32 p) {
+  return keybits / npasses + (p >= npasses - keybits % npasses ? 1u : 0u);
+}
+__host__ __device__ inline u32 digit_shift(u32 keybits, u32 npasses, u32 p) {
+  const u32 narrow = npasses - keybits % npasses;
+  return p * (keybits / npasses) + (p > narrow ? p - narrow : 0u);
+}
 constexpr int BOUNDARY_ITEMS = 32;
 constexpr int BOUNDARY_TILE = 256 * BOUNDARY_ITEMS;
 constexpr int TILESORT_E = 4;
@@ -86,7 +98,7 @@ __global__ void k_bcount(const K*, const u32*, u64, u32, u32*);
 template <class K, int ITEMS>
 __global__ void k_bwrite(const K*, const u32*, u64, u32, const u32*, u32*, u32*, u32*);
 template <class K>
-__global__ void k_hist_all(const K*, u64, u32, u32, u32*);
+__global__ void k_hist_all(const K*, u64, u32, u32, u64, u32*);
 __global__ void k_digit_base(const u32*, const u32*, u32*, u32*, u32*);
 // P = NoPay: no payload column
 struct NoPay {};

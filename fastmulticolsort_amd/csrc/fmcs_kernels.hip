@@ -347,21 +347,33 @@ __global__ __launch_bounds__(256) void k_gather_win(const T* __restrict__ vals, 
 // ---------------------------------------------------------------------------
 // Radix histograms.
 // k_hist_all: one pass over all keys computing the digit histogram of every
-// LSD pass at once (npasses <= 8).
+// LSD pass at once (npasses <= 8, digits split by digit_width/digit_shift).
+// A key with a bit in bad sets hist[npasses * RADIX].
 // (kd_seg_hist, fmcs_dd.hip: per-segment and per-tile histograms of one
 // digit over tiles that never cross a segment.)
 // ---------------------------------------------------------------------------
 template <class K>
-__global__ __launch_bounds__(256) void k_hist_all(const K* __restrict__ keys, u64 n, u32 npasses, u32 dbits,
-                                                  u32* __restrict__ hist) {
+__global__ __launch_bounds__(256) void k_hist_all(const K* __restrict__ keys, u64 n, u32 npasses, u32 keybits,
+                                                  u64 bad, u32* __restrict__ hist) {
   __shared__ u32 sh[8][RADIX];
   for (u32 i = threadIdx.x; i < 8 * RADIX; i += blockDim.x) (&sh[0][0])[i] = 0;
   __syncthreads();
-  u32 mask = (1u << dbits) - 1;
+  // digit_width / digit_shift unrolled: the first `narrow` digits are wb
+  // bits wide, the rest wb + 1 (no per-pass arrays, which would live in
+  // scratch memory under a runtime loop)
+  const u32 wb = keybits / npasses, narrow = npasses - keybits % npasses;
+  u64 over = 0;
   for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) {
     u64 k = (u64)keys[i];
-    for (u32 p = 0; p < npasses; ++p) atomicAdd(&sh[p][(u32)(k >> (p * dbits)) & mask], 1u);
+    over |= k & bad;
+    u32 s = 0;
+    for (u32 p = 0; p < npasses; ++p) {
+      const u32 w = wb + (p >= narrow ? 1u : 0u);
+      atomicAdd(&sh[p][(u32)(k >> s) & ((1u << w) - 1)], 1u);
+      s += w;
+    }
   }
+  if (__ballot(over != 0) && __lane_id() == 0) atomicOr(&hist[npasses * RADIX], 1u);
   __syncthreads();
   for (u32 p = 0; p < npasses; ++p) {
     u32 c = sh[p][threadIdx.x];
@@ -1279,7 +1291,7 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
   template __global__ void k_gather_win<K>(const K*, const u32*, K*, u64, u32, u32);                                \
   template __global__ void k_bcount<K, BOUNDARY_ITEMS>(const K*, const u32*, u64, u32, u32*);                       \
   template __global__ void k_bwrite<K, BOUNDARY_ITEMS>(const K*, const u32*, u64, u32, const u32*, u32*, u32*, u32*); \
-  template __global__ void k_hist_all<K>(const K*, u64, u32, u32, u32*);                                            \
+  template __global__ void k_hist_all<K>(const K*, u64, u32, u32, u64, u32*);                                            \
   FMCS_INST_ONESWEEP_F(K, NoPay, false, false)                                                                      \
   FMCS_INST_ONESWEEP_IF(K, WIDE_ITEMS, NoPay, false, false)                                                         \
   FMCS_INST_ONESWEEP_IF(K, WIDE_ITEMS, NoPay, false, true)                                                          \

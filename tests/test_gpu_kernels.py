@@ -8,6 +8,7 @@ import torch
 
 import oracle_bind as ob
 from fastmulticolsort_amd import datagen
+from fastmulticolsort_amd._lib import FmcsError
 from fastmulticolsort_amd.engine import (MODE_BATCHED, MODE_DEVICE, MODE_PLAIN, MODE_UNBINNED, construct_entries)
 
 pytestmark = pytest.mark.gpu
@@ -131,6 +132,23 @@ def test_sort_pairs(ctx, dt, bits, n):
     exp = np.argsort(k, kind="stable").astype(np.uint32)
     assert np.array_equal(host(od), exp)
     assert np.array_equal(host(kd), k[exp])
+
+
+@pytest.mark.parametrize("dt,bits,bad_bit", [(np.uint32, 20, 21), (np.uint32, 20, 31), (np.uint16, 12, 15),
+                                              (np.uint64, 33, 63), (np.uint64, 40, 41)])
+def test_sort_pairs_reports_keys_wider_than_declared(ctx, dt, bits, bad_bit):
+    # any bit at or above the declared width is refused, also one far above
+    # the last digit (the digits are 6-8 bits wide, split evenly)
+    n = 50000
+    log("sort_pairs over-wide key", dt, bits, bad_bit)
+    k = datagen.uniform_bits(4, n, bits).astype(dt)
+    k[n // 2] |= dt(1) << dt(bad_bit)
+    with pytest.raises(FmcsError):
+        ctx.sort_pairs(dev(k), dev(np.arange(n, dtype=np.uint32)), bits)
+    k[n // 2] &= dt((1 << bits) - 1)
+    kd, od = dev(k), dev(np.arange(n, dtype=np.uint32))
+    ctx.sort_pairs(kd, od, bits)
+    assert np.array_equal(host(od), np.argsort(k, kind="stable").astype(np.uint32))
 
 
 @pytest.mark.parametrize("dt,bits,n", SORT_CASES + [(np.uint64, 62, 3 * 2048 + 5), (np.uint32, 31, 2048 * 9)])
