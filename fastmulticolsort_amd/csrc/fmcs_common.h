@@ -120,6 +120,28 @@ __device__ __forceinline__ u32 block_excl_scan(u32 v, u32* s_tmp, u32* total) {
   return r;
 }
 
+// The same scan with one barrier: every thread adds up the wave totals
+// before its own wave (NT / WAVE broadcast reads).  s_tmp (NT / WAVE words)
+// must not be written again until every thread has passed a later barrier:
+// callers with scans in a row alternate two buffers.
+template <int NT>
+__device__ __forceinline__ u32 block_excl_scan1(u32 v, u32* s_tmp, u32* total) {
+  constexpr int NW = NT / WAVE;
+  const u32 lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+  const u32 inc = wave_incl_scan(v);
+  if (lane == WAVE - 1) s_tmp[wv] = inc;
+  __syncthreads();
+  u32 pre = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    const u32 c = s_tmp[w];
+    pre += (u32)w < wv ? c : 0u;
+    tot += c;
+  }
+  *total = tot;
+  return pre + inc - v;
+}
+
 template <class K>
 __device__ __forceinline__ u32 digit_of(K k, u32 shift, u32 mask) {
   return (u32)((u64)k >> shift) & mask;

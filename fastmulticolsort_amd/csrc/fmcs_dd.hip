@@ -290,9 +290,10 @@ __global__ __launch_bounds__(256) void kd_seg_hist(const K* __restrict__ keys, c
                                                    u32* __restrict__ tile_counts, u64 bad_mask,
                                                    u32* __restrict__ err, u32* __restrict__ tile_seg) {
   constexpr int TILE = 256 * ITEMS;
-  constexpr int PV = 16 / sizeof(K);           // keys per 16-byte vector
-  constexpr int NV = TILE / PV / 256;          // vectors per thread in a full tile
-  constexpr bool VEC = TILE % (PV * 256) == 0;  // a full tile is whole vectors per thread
+  constexpr int PV = 16 / sizeof(K);         // keys per 16-byte vector
+  constexpr int NVT = TILE / PV;             // vectors in a full tile
+  constexpr int NV = (NVT + 255) / 256;      // per thread (the last one ragged unless NVT % 256 == 0)
+  constexpr bool VEC = TILE % PV == 0;       // a full tile is whole vectors
   // HCOPY sub-histograms, lane l counting into copy l % HCOPY: lanes of one
   // instruction that share a digit mostly hit different copies, so a skewed
   // key does not serialize the LDS atomics.  Copies are HSTRIDE words apart
@@ -330,16 +331,20 @@ __global__ __launch_bounds__(256) void kd_seg_hist(const K* __restrict__ keys, c
     if (VEC && rowend - row0 == (u64)TILE && ((uintptr_t)(keys + row0) & 15) == 0) {
       const uint4* vp = (const uint4*)(keys + row0);
       uint4 x[VEC ? NV : 1];
+      // every load unconditional (a ragged last vector re-reads the tile's
+      // last one and is not counted)
 #pragma unroll
-      for (int v = 0; v < NV; ++v) x[v] = vp[v * 256 + threadIdx.x];
+      for (int v = 0; v < NV; ++v) x[v] = vp[min(v * 256 + (int)threadIdx.x, NVT - 1)];
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
-        K k[PV];
-        unpack16<K>(x[v], k);
+        if (NVT % 256 == 0 || v * 256 + (int)threadIdx.x < NVT) {
+          K k[PV];
+          unpack16<K>(x[v], k);
 #pragma unroll
-        for (int i = 0; i < PV; ++i) {
-          atomicAdd(&myh[digit_of(k[i], shift, mask)], 1u);
-          bad |= (u64)k[i] & bad_mask;
+          for (int i = 0; i < PV; ++i) {
+            atomicAdd(&myh[digit_of(k[i], shift, mask)], 1u);
+            bad |= (u64)k[i] & bad_mask;
+          }
         }
       }
     } else {

@@ -271,7 +271,7 @@ struct BucketLds {
   static constexpr int BB = 4 * T;
   __align__(16) unsigned char raw[AB + BB];
   u32 cnt[NB];
-  u32 tmp[NT / WAVE + 1];
+  u32 tmp[2 * (NT / WAVE)];  // two buffers of the one-barrier block scans
   u32 over;
 };
 
@@ -375,7 +375,7 @@ __device__ __forceinline__ bool tile_bucket_body(const WinRows<K, E>& r, const u
       sum += f[e];
     }
     u32 nsegs;
-    u32 run = block_excl_scan<NT>(sum, s_tmp, &nsegs);
+    u32 run = block_excl_scan1<NT>(sum, s_tmp, &nsegs);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       run += f[e];
@@ -400,7 +400,7 @@ __device__ __forceinline__ bool tile_bucket_body(const WinRows<K, E>& r, const u
       lbs[e] = lb;
     }
     u32 btot;
-    u32 brun = block_excl_scan<NT>(bsum, s_tmp, &btot);
+    u32 brun = block_excl_scan1<NT>(bsum, s_tmp + NT / WAVE, &btot);
 #pragma unroll
     for (int e = 0; e < SE; ++e) {
       const u32 sg = tid * SE + e;
@@ -441,7 +441,7 @@ __device__ __forceinline__ bool tile_bucket_body(const WinRows<K, E>& r, const u
     }
     if (cmax > BUCKET_LIMIT) s_over = 1;
     u32 tot;
-    u32 crun = block_excl_scan<NT>(csum, s_tmp, &tot);
+    u32 crun = block_excl_scan1<NT>(csum, s_tmp, &tot);
     if (s_over) return true;
 #pragma unroll
     for (int j = 0; j < BPT; ++j) {
