@@ -30,13 +30,13 @@ HBM_PEAK_GBS = 8000.0
 # fastmulticolsort_amd.rocprof_summary from the runs named inside it
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "config3_2p27_traffic.json")
 # the bench's kernel timer names -> the kernel instance this workload runs
-PROFILE_NAME = {"k_gather": "k_gather<unsigned long>", "k_tile_bucket": "k_tile_bucket<unsigned long, 4>",
-                "k_tile_bucket_2t": "k_tile_bucket<unsigned long, 8>",
-                "k_onesweep": "k_onesweep<unsigned long, 20, 8, NoPay, false, true>",
-                "k_onesweep_fuse": "k_onesweep<unsigned int, 16, 8, unsigned long, true, true>",
-                "k_onesweep_msd": "k_onesweep<unsigned long, 20, 8, NoPay, false, false>",
+PROFILE_NAME = {"k_gather": "k_gather<unsigned long>", "k_tile_bucket": "k_tile_bucket<unsigned int, 8>",
+                "k_tile_bucket_2t": "k_tile_bucket<unsigned int, 8>",
+                "k_onesweep": "k_onesweep<unsigned long, 20, 8, NoPay, false, true, unsigned long>",
+                "k_onesweep_fuse": "k_onesweep<unsigned int, 16, 8, unsigned long, true, true, unsigned int>",
+                "k_onesweep_msd": "k_onesweep<unsigned long, 20, 8, NoPay, false, false, unsigned int>",
                 "k_bwrite": "k_bwrite<unsigned int, 32>", "k_bcount": "k_bcount<unsigned int, 32>",
-                "k_seg_hist": "kd_seg_hist<unsigned long, 20, 8>"}
+                "k_seg_hist": "kd_seg_hist<unsigned int, 20, 8>"}
 
 
 def measured_traffic(kernel):
