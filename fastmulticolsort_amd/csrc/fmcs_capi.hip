@@ -26,7 +26,7 @@ enum Slot {
   WS_ST, WS_SZ, WS_BST, WS_BSZ, WS_FLAG, WS_POS, WS_LLO, WS_LHI, WS_LTILES, WS_TBASE, WS_TLO, WS_THI, WS_TS0,
   WS_TS1, WS_COMP, WS_COMP2, WS_GROUPTMP, WS_MISC, WS_HASH, WS_HASH2, WS_HASH3, WS_HT_TABLE, WS_HT_FIRST,
   WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_DD, WS_SCANST, WS_FBLIST, WS_TCOUNT, WS_CARRY, WS_BCNT, WS_T0P, WS_T1P, WS_PAY, WS_SGL, WS_FBLIST2, WS_RKL, WS_RKL2, WS_DNEXT, WS_TSEG, WS_HG_KEY, WS_HG_POS, WS_HG_SKEY, WS_HG_SPOS, WS_HG_F, WS_HG_AUX, WS_HG_PSTART,
-  WS_HG_TKEY, WS_HG_TFIRST, WS_HG_CHUNK, WS_PLIST, WS_BINSC, WS_FBWIN, WS_FBPOOL, WS_UNB0, WS_UNB1, WS_UNBK, WS_DIG0, WS_USER0,
+  WS_HG_TKEY, WS_HG_TFIRST, WS_HG_CHUNK, WS_PLIST, WS_BINSC, WS_FBWIN, WS_FBPOOL, WS_UNB0, WS_UNB1, WS_UNBK, WS_DIG0, WS_BBITS, WS_USER0,
   WS_STAGE0 = WS_USER0 + 24,  // device copies of staged host buffers (one slot per argument)
   WS_COUNT = WS_STAGE0 + 72
 };
@@ -522,18 +522,24 @@ int merge_sort(fmcs_ctx* ctx, const K* kin, const u32* vin, K* kout, u32* vout, 
 }
 
 // Group boundaries: head counts per tile, their exclusive scan, then ids
-// and offsets; the group count goes to *count_d (device).
+// and offsets; the group count goes to *count_d (device).  Without per-row
+// ids the count pass also writes every row's head bit (n / 8 bytes) and the
+// write pass reads those instead of the keys (and group ids) again.
 template <class K>
 int boundary_launch(fmcs_ctx* ctx, const K* v, const u32* in_group, u64 n, u32* out_group, u32* seg_off,
                     u32* count_d, u32 shift = 0) {
   const u64 ntiles = (n + BOUNDARY_TILE - 1) / BOUNDARY_TILE;
   u32* tc;
   TRY(getbuf_t(ctx, WS_BCNT, 2 * ntiles, &tc));
+  u32* hb = nullptr;
+  if (!out_group) TRY(getbuf_t(ctx, WS_BBITS, ntiles * 256, &hb));
   const double rd = (double)n * (sizeof(K) + (in_group ? 4 : 0));
-  TRY(klaunch(ctx, "k_bcount", k_bcount<K, BOUNDARY_ITEMS>, ntiles, rd, nullptr, 0.0, v, in_group, n, shift, tc));
+  TRY(klaunch(ctx, "k_bcount", k_bcount<K, BOUNDARY_ITEMS>, ntiles, rd, nullptr, 0.0, v, in_group, n, shift, tc,
+              hb));
   TRY(scan_excl(ctx, tc, tc + ntiles, ntiles, nullptr));
-  return klaunch(ctx, "k_bwrite", k_bwrite<K, BOUNDARY_ITEMS>, ntiles, rd + (out_group ? 4.0 * n : 0.0), nullptr,
-                 0.0, v, in_group, n, shift, (const u32*)(tc + ntiles), out_group, seg_off, count_d);
+  return klaunch(ctx, "k_bwrite", k_bwrite<K, BOUNDARY_ITEMS>, ntiles,
+                 hb ? n / 8.0 : rd + (out_group ? 4.0 * n : 0.0), nullptr, 0.0, v, in_group, n, shift,
+                 (const u32*)(tc + ntiles), out_group, seg_off, count_d, (const u32*)hb);
 }
 
 // boundary: dense ids + optional segment offsets; returns count on host

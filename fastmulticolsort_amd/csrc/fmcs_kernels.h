@@ -94,9 +94,9 @@ __global__ void k_gather(const T*, const u32*, T*, u64);
 template <class T>
 __global__ void k_gather_win(const T*, const u32*, T*, u64, u32, u32);
 template <class K, int ITEMS>
-__global__ void k_bcount(const K*, const u32*, u64, u32, u32*);
+__global__ void k_bcount(const K*, const u32*, u64, u32, u32*, u32*);
 template <class K, int ITEMS>
-__global__ void k_bwrite(const K*, const u32*, u64, u32, const u32*, u32*, u32*, u32*);
+__global__ void k_bwrite(const K*, const u32*, u64, u32, const u32*, u32*, u32*, u32*, const u32*);
 template <class K>
 __global__ void k_hist_all(const K*, u64, u32, u32, u64, u32*);
 __global__ void k_digit_base(const u32*, const u32*, u32*, u32*, u32*);

@@ -139,7 +139,10 @@ __global__ __launch_bounds__(256) void k_gather(const T* __restrict__ vals, cons
 // r == 0 or its value >> shift (or incoming group id) differs from row r - 1
 // (shift > 0: the column is the high part of a composite key).
 // k_bcount counts the heads of every tile; after an exclusive scan of the
-// counts, k_bwrite recomputes the heads and writes ids / offsets.
+// counts, k_bwrite recomputes the heads and writes ids / offsets.  When
+// hbits is set, k_bcount also writes each thread's 32 head bits there (bit
+// v * PV + i for row base + (v * 256 + tid) * PV + i) and k_bwrite reads
+// them instead of the rows.
 // ---------------------------------------------------------------------------
 // Row r of a tile is read as element i of lane l's 16-byte vector v:
 // r = base + (v * 256 + tid) * PV + i, PV = 16 / sizeof(K) keys per vector.
@@ -239,13 +242,20 @@ __device__ __forceinline__ void tile_heads(const K* __restrict__ v, const u32* _
 
 template <class K, int ITEMS>
 __global__ __launch_bounds__(256) void k_bcount(const K* __restrict__ v, const u32* __restrict__ in_group, u64 n,
-                                                u32 shift, u32* __restrict__ tcnt) {
-  constexpr int NT = 256, NW = NT / WAVE, TILE = NT * ITEMS, NV = ITEMS * sizeof(K) / 16;
+                                                u32 shift, u32* __restrict__ tcnt, u32* __restrict__ hbits) {
+  constexpr int NT = 256, NW = NT / WAVE, TILE = NT * ITEMS, NV = ITEMS * sizeof(K) / 16, PV = 16 / sizeof(K);
+  static_assert(NV * PV == 32, "32 head bits a thread");
   __shared__ u32 s_w[NW];
   const u32 tid = threadIdx.x;
   const bool vec_ok = ((uintptr_t)v & 15) == 0 && ((uintptr_t)in_group & 15) == 0;
   u32 hm[NV];
   tile_heads<K, ITEMS>(v, in_group, n, shift, (u64)blockIdx.x * TILE, vec_ok, hm);
+  if (hbits) {
+    u32 m = 0;
+#pragma unroll
+    for (int vv = 0; vv < NV; ++vv) m |= hm[vv] << (vv * PV);
+    hbits[(u64)blockIdx.x * NT + tid] = m;
+  }
   u32 c = 0;
 #pragma unroll
   for (int vv = 0; vv < NV; ++vv) c += (u32)__popc(hm[vv]);
@@ -262,7 +272,8 @@ __global__ __launch_bounds__(256) void k_bcount(const K* __restrict__ v, const u
 template <class K, int ITEMS>
 __global__ __launch_bounds__(256) void k_bwrite(const K* __restrict__ v, const u32* __restrict__ in_group, u64 n,
                                                 u32 shift, const u32* __restrict__ tpre, u32* __restrict__ out_group,
-                                                u32* __restrict__ seg_off, u32* __restrict__ ngroups_out) {
+                                                u32* __restrict__ seg_off, u32* __restrict__ ngroups_out,
+                                                const u32* __restrict__ hbits) {
   constexpr int NT = 256, NW = NT / WAVE, TILE = NT * ITEMS, PV = 16 / sizeof(K), NV = ITEMS / PV;
   __shared__ u32 s_cnt[NV][NW];
   const u32 tid = threadIdx.x, wv = tid / WAVE, lane = tid & (WAVE - 1);
@@ -270,7 +281,13 @@ __global__ __launch_bounds__(256) void k_bwrite(const K* __restrict__ v, const u
   const u64 base = (u64)tile * TILE;
   const bool vec_ok = ((uintptr_t)v & 15) == 0 && ((uintptr_t)in_group & 15) == 0;
   u32 hm[NV], lex[NV];
-  tile_heads<K, ITEMS>(v, in_group, n, shift, base, vec_ok, hm);
+  if (hbits) {
+    const u32 m = hbits[(u64)tile * NT + tid];
+#pragma unroll
+    for (int vv = 0; vv < NV; ++vv) hm[vv] = (m >> (vv * PV)) & ((1u << PV) - 1u);
+  } else {
+    tile_heads<K, ITEMS>(v, in_group, n, shift, base, vec_ok, hm);
+  }
 #pragma unroll
   for (int vv = 0; vv < NV; ++vv) {
     const u32 c = (u32)__popc(hm[vv]);
@@ -1289,8 +1306,9 @@ __global__ void k_uniform_offsets(u32* __restrict__ off, u32 nseg, u32 size, u64
 #define FMCS_INST_K(K)                                                                                              \
   template __global__ void k_gather<K>(const K*, const u32*, K*, u64);                                              \
   template __global__ void k_gather_win<K>(const K*, const u32*, K*, u64, u32, u32);                                \
-  template __global__ void k_bcount<K, BOUNDARY_ITEMS>(const K*, const u32*, u64, u32, u32*);                       \
-  template __global__ void k_bwrite<K, BOUNDARY_ITEMS>(const K*, const u32*, u64, u32, const u32*, u32*, u32*, u32*); \
+  template __global__ void k_bcount<K, BOUNDARY_ITEMS>(const K*, const u32*, u64, u32, u32*, u32*);                 \
+  template __global__ void k_bwrite<K, BOUNDARY_ITEMS>(const K*, const u32*, u64, u32, const u32*, u32*, u32*, u32*, \
+                                                       const u32*);                                                \
   template __global__ void k_hist_all<K>(const K*, u64, u32, u32, u64, u32*);                                            \
   FMCS_INST_ONESWEEP_F(K, NoPay, false, false)                                                                      \
   FMCS_INST_ONESWEEP_IF(K, WIDE_ITEMS, NoPay, false, false)                                                         \
