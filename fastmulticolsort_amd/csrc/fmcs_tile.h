@@ -257,6 +257,96 @@ __device__ __forceinline__ bool tile_sort_win(const K* __restrict__ kin, const u
 // skew) is flagged in `fallback` and left to the radix tile sort.
 // ---------------------------------------------------------------------------
 
+// LDS vectors of the bucket sort's fixed-layout phases: N (2, 4 or 8)
+// consecutive u8 / u16 / u32 at an address aligned to their total size, in
+// one LDS instruction (or two for 8 u32) instead of N.  The sort is bound
+// by its LDS instructions, so the per-thread runs of flags, segment ids,
+// segment starts and bin counts move as vectors.
+template <int N>
+__device__ __forceinline__ void lds_ld_u8(const uint8_t* p, u32 (&v)[N]) {
+  if constexpr (N == 8) {
+    const uint2 x = *(const uint2*)p;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      v[i] = (x.x >> (8 * i)) & 0xffu;
+      v[4 + i] = (x.y >> (8 * i)) & 0xffu;
+    }
+  } else if constexpr (N == 4) {
+    const u32 x = *(const u32*)p;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = (x >> (8 * i)) & 0xffu;
+  } else {
+    const u32 x = *(const u16*)p;
+    v[0] = x & 0xffu;
+    v[1] = x >> 8;
+  }
+}
+template <int N>
+__device__ __forceinline__ void lds_st_u8(uint8_t* p, const u32 (&v)[N]) {
+  auto pack4 = [&](int i) {
+    return (v[i] & 0xffu) | ((v[i + 1] & 0xffu) << 8) | ((v[i + 2] & 0xffu) << 16) | (v[i + 3] << 24);
+  };
+  if constexpr (N == 8)
+    *(uint2*)p = make_uint2(pack4(0), pack4(4));
+  else if constexpr (N == 4)
+    *(u32*)p = pack4(0);
+  else
+    *(u16*)p = (u16)((v[0] & 0xffu) | (v[1] << 8));
+}
+template <int N>
+__device__ __forceinline__ void lds_ld_u16(const u16* p, u32 (&v)[N]) {
+  u32 w[N / 2];
+  if constexpr (N == 8) {
+    const uint4 x = *(const uint4*)p;
+    w[0] = x.x; w[1] = x.y; w[2] = x.z; w[3] = x.w;
+  } else if constexpr (N == 4) {
+    const uint2 x = *(const uint2*)p;
+    w[0] = x.x; w[1] = x.y;
+  } else {
+    w[0] = *(const u32*)p;
+  }
+#pragma unroll
+  for (int i = 0; i < N / 2; ++i) {
+    v[2 * i] = w[i] & 0xffffu;
+    v[2 * i + 1] = w[i] >> 16;
+  }
+}
+template <int N>
+__device__ __forceinline__ void lds_st_u16(u16* p, const u32 (&v)[N]) {
+  u32 w[N / 2];
+#pragma unroll
+  for (int i = 0; i < N / 2; ++i) w[i] = (v[2 * i] & 0xffffu) | (v[2 * i + 1] << 16);
+  if constexpr (N == 8)
+    *(uint4*)p = make_uint4(w[0], w[1], w[2], w[3]);
+  else if constexpr (N == 4)
+    *(uint2*)p = make_uint2(w[0], w[1]);
+  else
+    *(u32*)p = w[0];
+}
+template <int N>
+__device__ __forceinline__ void lds_ld_u32(const u32* p, u32 (&v)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; i += 4) {
+    if constexpr (N >= 4) {
+      const uint4 x = *(const uint4*)(p + i);
+      v[i] = x.x; v[i + 1] = x.y; v[i + 2] = x.z; v[i + 3] = x.w;
+    } else {
+      const uint2 x = *(const uint2*)p;
+      v[0] = x.x; v[1] = x.y;
+    }
+  }
+}
+template <int N>
+__device__ __forceinline__ void lds_st_u32(u32* p, const u32 (&v)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; i += 4) {
+    if constexpr (N >= 4)
+      *(uint4*)(p + i) = make_uint4(v[i], v[i + 1], v[i + 2], v[i + 3]);
+    else
+      *(uint2*)p = make_uint2(v[0], v[1]);
+  }
+}
+
 // LDS of the bucket sort.  The segment arrays are dead once the rows are
 // scattered into their bins, so they share LDS with the row buffers (fewer
 // bytes a block, more windows in flight a CU): region A holds the rows' keys
@@ -350,8 +440,11 @@ __device__ __forceinline__ bool tile_bucket_body(const WinRows<K, E>& r, const u
   const u64 kmask = kb >= 64 ? ~0ull : ((1ull << kb) - 1);
   const K (&kk)[E] = r.kk;
   const u32 (&oo)[E] = r.oo;
-  for (u32 i = tid; i < (u32)T; i += NT) s_f[i] = 0;
-  for (u32 i = tid; i < (u32)NB; i += NT) s_cnt[i] = 0;
+  static_assert(E == 2 || E == 4 || E == 8, "LDS vectors of 2, 4 or 8");
+  // flags and bin counts cleared with 16-byte stores (s_f at 5T and the
+  // counts are 16-byte aligned)
+  for (u32 i = tid; i < (u32)T / 16; i += NT) ((uint4*)s_f)[i] = make_uint4(0u, 0u, 0u, 0u);
+  for (u32 i = tid; i < (u32)NB / 4; i += NT) ((uint4*)s_cnt)[i] = make_uint4(0u, 0u, 0u, 0u);
   if (tid == 0) s_over = 0;
   __syncthreads();
 #pragma unroll
@@ -368,24 +461,28 @@ __device__ __forceinline__ bool tile_bucket_body(const WinRows<K, E>& r, const u
   // local segment of every position (blocked scan of the flags)
   {
     u32 f[E];
+    lds_ld_u8<E>(s_f + tid * E, f);
     u32 sum = 0;
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      f[e] = s_f[tid * E + e];
-      sum += f[e];
-    }
+    for (int e = 0; e < E; ++e) sum += f[e];
     u32 nsegs;
     u32 run = block_excl_scan1<NT>(sum, s_tmp, &nsegs);
+    u32 sid[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       run += f[e];
       const u32 p = tid * E + e;
-      s_sid[p] = (u16)(run - 1);
+      sid[e] = run - 1;
       if (f[e] && p < cnt) s_segstart[run - 1] = (u16)p;
     }
+    lds_st_u16<E>(s_sid + tid * E, sid);
     if (tid == 0) s_segstart[nsegs] = (u16)cnt;
     __syncthreads();
-    // bins per segment: 2^lb >= m, laid out in segment order
+    // bins per segment: 2^lb >= m, laid out in segment order (entries past
+    // nsegs are computed from stale starts and never read)
+    u32 ss[SE];
+    lds_ld_u16<SE>(s_segstart + tid * SE, ss);
+    const u32 snext = s_segstart[tid * SE + SE];  // T + 1 entries
     u32 bsum = 0;
     u32 lbs[SE];
 #pragma unroll
@@ -393,7 +490,7 @@ __device__ __forceinline__ bool tile_bucket_body(const WinRows<K, E>& r, const u
       const u32 sg = tid * SE + e;
       u32 lb = 0;
       if (sg < nsegs) {
-        const u32 m = (u32)s_segstart[sg + 1] - (u32)s_segstart[sg];
+        const u32 m = (e + 1 < SE ? ss[e + 1] : snext) - ss[e];
         lb = m <= 2 ? 0u : 31u - __clz(m - 1);  // 2^lb >= m / 2: about two rows a bin
         bsum += 1u << lb;
       }
@@ -401,15 +498,14 @@ __device__ __forceinline__ bool tile_bucket_body(const WinRows<K, E>& r, const u
     }
     u32 btot;
     u32 brun = block_excl_scan1<NT>(bsum, s_tmp + NT / WAVE, &btot);
+    u32 segb[SE];
 #pragma unroll
     for (int e = 0; e < SE; ++e) {
-      const u32 sg = tid * SE + e;
-      if (sg < nsegs) {
-        s_lb[sg] = (uint8_t)lbs[e];
-        s_segbase[sg] = (u16)brun;
-        brun += 1u << lbs[e];
-      }
+      segb[e] = brun;
+      if (tid * SE + e < nsegs) brun += 1u << lbs[e];
     }
+    lds_st_u8<SE>(s_lb + tid * SE, lbs);
+    lds_st_u16<SE>(s_segbase + tid * SE, segb);
   }
   __syncthreads();
   // bin of every row, and the bin counts
@@ -432,10 +528,10 @@ __device__ __forceinline__ bool tile_bucket_body(const WinRows<K, E>& r, const u
   // exclusive scan of the counts (BPT consecutive bins per thread) -> cursors
   {
     u32 c[BPT];
+    lds_ld_u32<BPT>(s_cnt + tid * BPT, c);
     u32 csum = 0, cmax = 0;
 #pragma unroll
     for (int j = 0; j < BPT; ++j) {
-      c[j] = s_cnt[tid * BPT + j];
       csum += c[j];
       cmax = max(cmax, c[j]);
     }
@@ -443,11 +539,13 @@ __device__ __forceinline__ bool tile_bucket_body(const WinRows<K, E>& r, const u
     u32 tot;
     u32 crun = block_excl_scan1<NT>(csum, s_tmp, &tot);
     if (s_over) return true;
+    u32 cur[BPT];
 #pragma unroll
     for (int j = 0; j < BPT; ++j) {
-      s_cnt[tid * BPT + j] = crun;
+      cur[j] = crun;
       crun += c[j];
     }
+    lds_st_u32<BPT>(s_cnt + tid * BPT, cur);
   }
   __syncthreads();
   // scatter rows into their bins (order inside a bin is arbitrary); the
