@@ -198,6 +198,13 @@ u64 cap_grid(u64 want, u64 cap) { return std::max<u64>(1, std::min<u64>(want, ca
 // atomic counter): twice the resident blocks, so a block that finishes early
 // is replaced at once and the CUs stay full; extra blocks find no work.
 u64 persist_grid(fmcs_ctx* ctx, const void* fn, u64 want) { return cap_grid(want, 2ull * resident(ctx, fn)); }
+// Grid of the bucket tile sort's window loops: four times the resident
+// blocks (windows differ in cost; the later blocks even out the end of the
+// launch: 1.129 -> 1.112 ms a step against twice)
+u64 bucket_grid(fmcs_ctx* ctx, const void* fn, u64 want) {
+  const u64 g = cap_grid(want, 4ull * resident(ctx, fn));
+  return (g + XCDS - 1) / XCDS * XCDS;
+}
 
 // Grid of k_onesweep: persistent, a multiple of XCDS (its XCD-aware tile walk).
 u64 onesweep_grid(fmcs_ctx* ctx, const void* fn, u64 ntiles) {
@@ -1356,7 +1363,7 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
     a.size_lo = 0; a.size_hi = ~0u;
     a.bad = bad; a.err = errp;
     {
-      const u64 g = (persist_grid(ctx, (const void*)fn_bucket, d.nw_max) + XCDS - 1) / XCDS * XCDS;
+      const u64 g = bucket_grid(ctx, (const void*)fn_bucket, d.nw_max);
       // rows read (key, oid) and written (oid, and the key when wanted)
       const double per = keys_out ? 2.0 * (sizeof(KL) + 4) : sizeof(KL) + 8.0;
       TRY(klaunch(ctx, "k_tile_bucket", fn_bucket, g, 0.0, S + LV_ROWS_SMALL, per, a, fb, a.tile_lo, a.tile_hi,
@@ -1384,12 +1391,12 @@ int seg_sort_dd(fmcs_ctx* ctx, K* keys, u32* vals, u64 n, u32 keybits, const u32
       c.tile_lo = slo; c.tile_hi = shi; c.tile_s0 = sseg; c.tile_s1 = sseg;
       c.wlist = nullptr; c.wcount = S + LV_NSINGLE; c.fblist = fblist2; c.fbcount = S + LV_FB2;
       c.size_lo = 0; c.size_hi = T;
-      const u64 g = (persist_grid(ctx, (const void*)fn_bucket, d.nsingle_max) + XCDS - 1) / XCDS * XCDS;
+      const u64 g = bucket_grid(ctx, (const void*)fn_bucket, d.nsingle_max);
       TRY(klaunch(ctx, "k_tile_bucket", fn_bucket, g, 0.0, nullptr, 0.0, c, fb, c.tile_lo, c.tile_hi, c.tile_s0,
                   c.tile_s1, c.wlist));
       if (d.single_max > T) {
         c.size_lo = T; c.size_hi = d.single_max;
-        const u64 gb = (persist_grid(ctx, (const void*)fn_bucket2, d.nsingle_max) + XCDS - 1) / XCDS * XCDS;
+        const u64 gb = bucket_grid(ctx, (const void*)fn_bucket2, d.nsingle_max);
         TRY(klaunch(ctx, "k_tile_bucket_2t", fn_bucket2, gb, 0.0, nullptr, 0.0, c, fb, c.tile_lo, c.tile_hi,
                     c.tile_s0, c.tile_s1, c.wlist));
       }
