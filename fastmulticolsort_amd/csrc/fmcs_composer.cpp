@@ -428,51 +428,6 @@ int fmcs_calibrate(fmcs_ctx* ctx, uint64_t nrows, fmcs_cost_model* out) {
   if ((rc = fmcs::internal::uniform_offsets(ctx, o0, (uint32_t)n - 1, n, 1))) return rc;  // o0 = 0..n-1
   if (hipMemcpyAsync(perm, o0, n * 4, hipMemcpyDeviceToDevice, stream_of(ctx)) != hipSuccess) return FMCS_E_HIP;
   if ((rc = fmcs_sort_pairs(ctx, 8, 64, n, k0, perm))) return rc;
-  const uint32_t banks[3] = {2, 4, 8}, bits[3] = {16, 32, 64};
-  for (int b = 0; b < 3; ++b) {
-    a.bank = banks[b];
-    a.bits = bits[b];
-    if ((rc = fmcs::internal::fill_random(ctx, k0, a.bank, n, 777 + b, 8))) return rc;
-    // one LSD pass (8-bit keys) including the reset copies, minus the copies
-    double t_pass = time_ms_best(ctx, 3, 3, [](fmcs_ctx* c, void* x) {
-      CalArgs* q = (CalArgs*)x;
-      int r = reset_pairs(c, q);
-      return r ? r : fmcs_sort_pairs(c, q->bank, 8, q->n, q->keys, q->oids);
-    }, &a);
-    double t_copy = time_ms_best(ctx, 3, 3, [](fmcs_ctx* c, void* x) { return reset_pairs(c, (CalArgs*)x); }, &a);
-    out->radix_pass_ns[b] = std::max(1e-6, (t_pass - t_copy)) * 1e6 / n;
-    if ((rc = fmcs::internal::fill_random(ctx, k0, a.bank, n, 999 + b, a.bits))) return rc;
-    double t_g = time_ms_best(ctx, 3, 3, [](fmcs_ctx* c, void* x) {
-      CalArgs* q = (CalArgs*)x;
-      return fmcs_gather(c, q->bank, q->n, q->keys0, q->perm, q->keys);
-    }, &a);
-    out->gather_ns[b] = t_g * 1e6 / n;
-    // boundary over sorted keys
-    if ((rc = reset_pairs(ctx, &a)) || (rc = fmcs_sort_pairs(ctx, a.bank, a.bits, n, a.keys, a.oids))) return rc;
-    double t_b = time_ms_best(ctx, 3, 3, [](fmcs_ctx* c, void* x) {
-      CalArgs* q = (CalArgs*)x;
-      uint32_t ng;
-      return fmcs_extract_group_info(c, q->bank, q->n, q->keys, q->oids, (uint32_t*)q->vals2, nullptr, &ng);
-    }, &a);
-    out->boundary_ns[b] = t_b * 1e6 / n;
-    // segmented sorts over uniform groups of 2^i rows
-    for (int i = 0; i < FMCS_COST_GBUCKETS; ++i) {
-      uint64_t g = 1ull << i;
-      if (g > n) {
-        out->segsort_ns[b][i] = out->segsort_ns[b][i - 1];
-        continue;
-      }
-      a.nseg = (uint32_t)((n + g - 1) / g);
-      if ((rc = fmcs::internal::uniform_offsets(ctx, off, a.nseg, n, (uint32_t)g))) return rc;
-      double t = time_ms_best(ctx, 3, 2, [](fmcs_ctx* c, void* x) {
-        CalArgs* q = (CalArgs*)x;
-        int r = reset_pairs(c, q);
-        return r ? r : fmcs_segmented_sort(c, q->bank, q->bits, q->n, q->keys, q->oids, q->off, q->nseg,
-                                           FMCS_MODE_DEVICE, nullptr);
-      }, &a);
-      out->segsort_ns[b][i] = std::max(1e-6, t - t_copy) * 1e6 / n;
-    }
-  }
   // Fixed costs (launches, list set-up) of the device-driven plans, from
   // sorts of a few rows (means of 60 runs each): per LSD pass (a 40-bit against a 16-bit first
   // round: 5 passes against 2), per level of a later round (a 48-bit against
@@ -512,6 +467,78 @@ int fmcs_calibrate(fmcs_ctx* ctx, uint64_t nrows, fmcs_cost_model* out) {
     out->round_overhead_ms = std::max(0.005, t2a - t20 - 3.0 * out->level_overhead_ms);
     if ((rc = fmcs_ctx_sync(ctx))) return rc;
     a.n = n;
+  }
+  const uint32_t banks[3] = {2, 4, 8}, bits[3] = {16, 32, 64};
+  for (int b = 0; b < 3; ++b) {
+    a.bank = banks[b];
+    a.bits = bits[b];
+    // one LSD pass as the plans run it: device-driven single-round sorts of
+    // three 8-bit digits against one (two against one in a u16 bank), so the
+    // fixed costs of a call cancel (an eager sort would add its host
+    // synchronisation to every pass)
+    auto one_round = [](fmcs_ctx* c, void* x) {
+      CalArgs* q = (CalArgs*)x;
+      const void* cols[1] = {q->keys0};
+      const uint32_t bk[1] = {q->bank}, w[1] = {q->bits};
+      return fmcs_multi_round_sort(c, 1, cols, bk, w, q->n, q->oids, nullptr, FMCS_MODE_DEVICE, nullptr);
+    };
+    const uint32_t wide = banks[b] == 2 ? 16u : 24u;
+    a.bits = 8;
+    if ((rc = fmcs::internal::fill_random(ctx, k0, a.bank, n, 777 + b, 8))) return rc;
+    const double t_one = time_ms_best(ctx, 3, 3, one_round, &a);
+    a.bits = wide;
+    if ((rc = fmcs::internal::fill_random(ctx, k0, a.bank, n, 778 + b, wide))) return rc;
+    const double t_wide = time_ms_best(ctx, 3, 3, one_round, &a);
+    a.bits = bits[b];
+    out->radix_pass_ns[b] = std::max(1e-6, (t_wide - t_one) / (wide / 8 - 1)) * 1e6 / n;
+    double t_copy = time_ms_best(ctx, 3, 3, [](fmcs_ctx* c, void* x) { return reset_pairs(c, (CalArgs*)x); }, &a);
+    if ((rc = fmcs::internal::fill_random(ctx, k0, a.bank, n, 999 + b, a.bits))) return rc;
+    double t_g = time_ms_best(ctx, 3, 3, [](fmcs_ctx* c, void* x) {
+      CalArgs* q = (CalArgs*)x;
+      return fmcs_gather(c, q->bank, q->n, q->keys0, q->perm, q->keys);
+    }, &a);
+    out->gather_ns[b] = t_g * 1e6 / n;
+    // boundary over sorted keys
+    if ((rc = reset_pairs(ctx, &a)) || (rc = fmcs_sort_pairs(ctx, a.bank, a.bits, n, a.keys, a.oids))) return rc;
+    double t_b = time_ms_best(ctx, 3, 3, [](fmcs_ctx* c, void* x) {
+      CalArgs* q = (CalArgs*)x;
+      uint32_t ng;
+      return fmcs_extract_group_info(c, q->bank, q->n, q->keys, q->oids, (uint32_t*)q->vals2, nullptr, &ng);
+    }, &a);
+    out->boundary_ns[b] = t_b * 1e6 / n;
+    // the fixed cost of a device-driven segmented sort of these keys (the
+    // launches of its level schedule), from a 4096-row sort in one-row
+    // segments; the plan estimate adds it per level itself
+    double t_fix;
+    {
+      a.n = 4096;
+      a.nseg = 4096;
+      if ((rc = fmcs::internal::uniform_offsets(ctx, off, a.nseg, a.n, 1u))) return rc;
+      t_fix = time_ms_best(ctx, 3, 3, [](fmcs_ctx* c, void* x) {
+        CalArgs* q = (CalArgs*)x;
+        int r = reset_pairs(c, q);
+        return r ? r : fmcs_segmented_sort(c, q->bank, q->bits, q->n, q->keys, q->oids, q->off, q->nseg,
+                                           FMCS_MODE_DEVICE, nullptr);
+      }, &a);
+      a.n = n;
+    }
+    // segmented sorts over uniform groups of 2^i rows
+    for (int i = 0; i < FMCS_COST_GBUCKETS; ++i) {
+      uint64_t g = 1ull << i;
+      if (g > n) {
+        out->segsort_ns[b][i] = out->segsort_ns[b][i - 1];
+        continue;
+      }
+      a.nseg = (uint32_t)((n + g - 1) / g);
+      if ((rc = fmcs::internal::uniform_offsets(ctx, off, a.nseg, n, (uint32_t)g))) return rc;
+      double t = time_ms_best(ctx, 3, 2, [](fmcs_ctx* c, void* x) {
+        CalArgs* q = (CalArgs*)x;
+        int r = reset_pairs(c, q);
+        return r ? r : fmcs_segmented_sort(c, q->bank, q->bits, q->n, q->keys, q->oids, q->off, q->nseg,
+                                           FMCS_MODE_DEVICE, nullptr);
+      }, &a);
+      out->segsort_ns[b][i] = std::max(1e-6, t - t_copy - t_fix) * 1e6 / n;
+    }
   }
   // stitch: two u32 sources into one u64 bank
   {

@@ -8,7 +8,8 @@ Prints one JSON object per measurement:
   * config 1 (2^24 rows, c1 17-bit u32 over 8192 codes, c2 33-bit u64): every
     split of the two-round sweep (TwoRoundsExhaustive,
     src/stitch_composer.cpp:4197-4258) run end to end through the composer,
-    the analytic model's pick and its regret;
+    the analytic model's pick and its regret, under the default constants
+    and under constants calibrated on the device;
   * config 2 (2^26 rows, 12/20/30-bit columns stitched into one 62-bit u64
     key): the stitch, the LSD radix sort, the bitonic + merge-path sort and
     the payload gathers, with their HBM roofline fractions;
@@ -92,17 +93,24 @@ def config1(ctx, n=1 << 24):
         ok = bool(np.array_equal(order.cpu().numpy(), exp))
         rows.append({"split": split, "ms": round(ms, 3), "ok": ok})
         print(f"config 1 split {split}: {ms:.3f} ms ok={ok}", file=sys.stderr, flush=True)
-    model = planner.default_cost_model()
+    # the analytic pick under the default constants and under constants
+    # measured on this device (fmcs_calibrate)
     card = [8192, datagen.exact_cardinality(c2)]
-    _, pick, _, _ = planner.optimize(model, n, bws, card)
+    _, pick, _, _ = planner.optimize(planner.default_cost_model(), n, bws, card)
+    cal = planner.calibrate(ctx)
+    _, cpick, _, _ = planner.optimize(cal, n, bws, card)
     best = min(rows, key=lambda r: r["ms"])
     picked = next((r for r in rows if r["split"] == pick), None)
+    cpicked = next((r for r in rows if r["split"] == cpick), None)
     _emit({"config": 1, "what": "two-round sweep, every split end to end (composer, device-driven batched)",
            "rows": n, "plans": len(rows), "all_ok": all(r["ok"] for r in rows),
            "best": best, "worst": max(rows, key=lambda r: r["ms"]),
            "best_Mrows_s": round(n / best["ms"] / 1e3, 1),
            "analytic_pick": pick, "analytic_pick_ms": picked["ms"] if picked else None,
-           "analytic_regret": round(picked["ms"] / best["ms"] - 1, 4) if picked else None, "sweep": rows})
+           "analytic_regret": round(picked["ms"] / best["ms"] - 1, 4) if picked else None,
+           "calibrated_pick": cpick, "calibrated_pick_ms": cpicked["ms"] if cpicked else None,
+           "calibrated_regret": round(cpicked["ms"] / best["ms"] - 1, 4) if cpicked else None,
+           "calibrated_model": cal.to_dict(), "sweep": rows})
 
 
 def config2(ctx, n=1 << 26):
