@@ -179,8 +179,9 @@ int fmcs_extract_group_info(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void
 
 /* Mergesort::avxmergesort_anytype<T> (src/mergesort.h:62-64, :2878-3096)
  * as used by Composer::subsort (src/composer.cpp:490-589): sorts the pairs
- * (keys_d[i], oids_d[i]) by the low keybits bits of the key, in place.  Ties
- * keep their incoming order (LSD radix sort, stable). */
+ * (keys_d[i], oids_d[i]) by the keybits-bit key, in place.  Ties keep their
+ * incoming order (LSD radix sort, stable).  A key with a bit at or above
+ * keybits is refused: FMCS_E_ARG and nothing is sorted. */
 int fmcs_sort_pairs(fmcs_ctx* ctx, uint32_t bank, uint32_t keybits, uint64_t n, void* keys_d, uint32_t* oids_d);
 
 /* fmcs_sort_pairs without any device allocation.  No counterpart in the
