@@ -447,26 +447,27 @@ extern "C" {
 int fmcs_cost_model_default(fmcs_cost_model* m) {
   if (!m) return FMCS_E_ARG;
   std::memset(m, 0, sizeof(*m));
-  // Per-row times measured on MI355X (2^27 rows); replaced by
-  // fmcs_calibrate() on the running device.
-  const double radix[3] = {0.0060, 0.0067, 0.0105};
-  const double gather[3] = {0.0150, 0.0170, 0.0213};
-  const double bnd[3] = {0.0045, 0.0055, 0.0075};
+  // Constants measured on MI355X by fmcs_calibrate (2^24 rows, the
+  // tuning file's settings); replaced by fmcs_calibrate() on the running
+  // device.  Per-row ns: an LSD pass, a gather, a boundary pass, by bank
+  // (u16, u32, u64); the segmented sort by bank and log2(average group size).
+  static const double radix[3] = {0.007184, 0.00818, 0.009577};
+  static const double gather[3] = {0.0163, 0.01823, 0.02006};
+  static const double bnd[3] = {0.003929, 0.004761, 0.007405};
+  static const double seg[3][FMCS_COST_GBUCKETS] = {
+      {0.02143, 0.01415, 0.01041, 0.008636, 0.007965, 0.007262, 0.007022, 0.006923, 0.006961, 0.007016, 0.005465, 0.00553, 0.01742, 0.01667, 0.01532, 0.01396, 0.01325, 0.01446, 0.01622, 0.02776, 0.03676, 0.05208, 0.0504, 0.04944, 0.04838, 0.04838, 0.04838, 0.04838, 0.04838, 0.04838, 0.04838, 0.04838},
+      {0.02094, 0.01374, 0.0105, 0.008983, 0.007981, 0.007363, 0.007074, 0.007217, 0.007168, 0.007068, 0.005336, 0.005388, 0.01783, 0.01717, 0.01566, 0.01416, 0.01343, 0.01403, 0.01384, 0.02194, 0.02403, 0.02391, 0.02299, 0.02188, 0.02101, 0.02101, 0.02101, 0.02101, 0.02101, 0.02101, 0.02101, 0.02101},
+      {0.01966, 0.01238, 0.00922, 0.00761, 0.006579, 0.005799, 0.005619, 0.00561, 0.005563, 0.005657, 0.004405, 0.005248, 0.01788, 0.01912, 0.01688, 0.01523, 0.01461, 0.01537, 0.01574, 0.02443, 0.02603, 0.02656, 0.02477, 0.02375, 0.02302, 0.02302, 0.02302, 0.02302, 0.02302, 0.02302, 0.02302, 0.02302}};
   for (int b = 0; b < 3; ++b) {
     m->radix_pass_ns[b] = radix[b];
     m->gather_ns[b] = gather[b];
     m->boundary_ns[b] = bnd[b];
-    for (int i = 0; i < FMCS_COST_GBUCKETS; ++i) {
-      // small groups: one LDS tile pass; large: MSD passes plus the tile
-      double base = 0.020 * (1 + 0.5 * b);
-      double msd_levels = i <= 10 ? 0 : std::ceil((i - 10) / 8.0);
-      m->segsort_ns[b][i] = base + msd_levels * radix[b] * 1.4;
-    }
+    for (int i = 0; i < FMCS_COST_GBUCKETS; ++i) m->segsort_ns[b][i] = seg[b][i];
   }
-  m->stitch_ns_per_byte = 1.0 / 5000.0;  // 5 TB/s
-  m->round_overhead_ms = 0.05;
-  m->pass_overhead_ms = 0.015;
-  m->level_overhead_ms = 0.05;
+  m->stitch_ns_per_byte = 0.0001571;  // about 6.4 TB/s
+  m->round_overhead_ms = 0.005;
+  m->pass_overhead_ms = 0.02042;
+  m->level_overhead_ms = 0.03186;
   m->carry_next = 1;
   return FMCS_OK;
 }
