@@ -95,6 +95,29 @@ def test_gather_windowed(ctx, dt, kb):
 
 
 @pytest.mark.parametrize("dt", [np.uint16, np.uint32, np.uint64])
+@pytest.mark.parametrize("n", [1, 2, 8191, 8192, 8193, 300001])
+@pytest.mark.parametrize("with_in_group", [False, True])
+def test_group_offsets_without_ids(ctx, dt, n, with_in_group):
+    # no per-row ids wanted: the count pass writes every row's head bit and
+    # the write pass reads those bits instead of the keys (8192-row tiles)
+    log("boundary offsets only", dt, n, with_in_group)
+    import ctypes as C
+    v = np.sort(datagen.uniform_below(13, n, max(2, n // 7)).astype(dt))
+    ing = np.sort(datagen.uniform_below(14, n, max(2, n // 40))).astype(np.uint32)
+    exp, ng = ob.extract_group_info(v, ing) if with_in_group else ob.extract_group_info(v)
+    heads = np.flatnonzero(np.r_[1, np.diff(exp.astype(np.int64)) != 0])
+    dv, di = dev(v), dev(ing)
+    off = torch.full((n + 1,), 0xFFFFFFFF, dtype=torch.uint32, device="cuda")
+    cnt = C.c_uint32(0)
+    rc = ctx.lib.fmcs_extract_group_info(ctx.h, v.dtype.itemsize, n, C.c_void_p(dv.data_ptr()),
+                                         C.c_void_p(di.data_ptr()) if with_in_group else None, None,
+                                         C.c_void_p(off.data_ptr()), C.byref(cnt))
+    assert rc == 0
+    assert cnt.value == ng
+    assert np.array_equal(host(off)[: ng + 1], np.r_[heads, n].astype(np.uint32))
+
+
+@pytest.mark.parametrize("dt", [np.uint16, np.uint32, np.uint64])
 @pytest.mark.parametrize("n", [1, 2, 4095, 4096, 4097, 300001])
 def test_extract_group_info(ctx, dt, n):
     log("boundary", dt, n)
