@@ -27,6 +27,7 @@ EXPORTS = [
     "fmcs_ctx_create", "fmcs_ctx_destroy", "fmcs_ctx_set_stream", "fmcs_ctx_sync",
     "fmcs_ctx_set_tuning", "fmcs_ctx_get_tuning", "fmcs_ctx_trim", "fmcs_last_error",
     "fmcs_bank_bytes", "fmcs_construct_entries", "fmcs_stitch", "fmcs_stitch_dir", "fmcs_gather",
+    "fmcs_gather_columns",
     "fmcs_extract_group_info", "fmcs_sort_pairs", "fmcs_sort_pairs_noalloc", "fmcs_sort_pairs_merge", "fmcs_fill_column", "fmcs_segmented_sort",
     "fmcs_sort_by_group",
     "fmcs_multi_round_sort", "fmcs_last_breakdown", "fmcs_hash_round", "fmcs_multi_round_hash",
@@ -99,6 +100,7 @@ def load():
         "fmcs_stitch_dir": (i32, [vp, u64, u32, P(vp), P(u32), P(u32), u32, P(Entry), u32, P(vp), P(u32),
                                   P(C.c_uint8)]),
         "fmcs_gather": (i32, [vp, u32, u64, vp, vp, vp]),
+        "fmcs_gather_columns": (i32, [vp, u64, u32, P(vp), P(u32), vp, P(vp)]),
         "fmcs_extract_group_info": (i32, [vp, u32, u64, vp, vp, vp, vp, P(u32)]),
         "fmcs_sort_pairs": (i32, [vp, u32, u32, u64, vp, vp]),
         "fmcs_sort_pairs_noalloc": (i32, [vp, u32, u32, u64, vp, vp]),

@@ -26,7 +26,7 @@ enum Slot {
   WS_ST, WS_SZ, WS_BST, WS_BSZ, WS_FLAG, WS_POS, WS_LLO, WS_LHI, WS_LTILES, WS_TBASE, WS_TLO, WS_THI, WS_TS0,
   WS_TS1, WS_COMP, WS_COMP2, WS_GROUPTMP, WS_MISC, WS_HASH, WS_HASH2, WS_HASH3, WS_HT_TABLE, WS_HT_FIRST,
   WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_DD, WS_SCANST, WS_FBLIST, WS_TCOUNT, WS_CARRY, WS_BCNT, WS_T0P, WS_T1P, WS_PAY, WS_SGL, WS_FBLIST2, WS_RKL, WS_RKL2, WS_DNEXT, WS_TSEG, WS_HG_KEY, WS_HG_POS, WS_HG_SKEY, WS_HG_SPOS, WS_HG_F, WS_HG_AUX, WS_HG_PSTART,
-  WS_HG_TKEY, WS_HG_TFIRST, WS_HG_CHUNK, WS_PLIST, WS_BINSC, WS_FBWIN, WS_FBPOOL, WS_UNB0, WS_UNB1, WS_UNBK, WS_DIG0, WS_BBITS, WS_USER0,
+  WS_HG_TKEY, WS_HG_TFIRST, WS_HG_CHUNK, WS_PLIST, WS_BINSC, WS_FBWIN, WS_FBPOOL, WS_UNB0, WS_UNB1, WS_UNBK, WS_DIG0, WS_BBITS, WS_REC, WS_USER0,
   WS_STAGE0 = WS_USER0 + 24,  // device copies of staged host buffers (one slot per argument)
   WS_COUNT = WS_STAGE0 + 72
 };
@@ -2373,6 +2373,83 @@ int fmcs_gather(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* values_d, 
     case 2: TRY(gather_launch<u16>(ctx, (const u16*)values_d, order_d, (u16*)out_d, n)); break;
     case 4: TRY(gather_launch<u32>(ctx, (const u32*)values_d, order_d, (u32*)out_d, n)); break;
     default: TRY(gather_launch<u64>(ctx, (const u64*)values_d, order_d, (u64*)out_d, n)); break;
+  }
+  return st.finish();
+}
+
+int fmcs_gather_columns(fmcs_ctx* ctx, uint64_t n, uint32_t ncols, const void* const* cols_d, const uint32_t* banks,
+                        const uint32_t* order_d, void* const* outs_d) {
+  if (!ctx) return FMCS_E_ARG;
+  if (ncols == 0 || ncols > 32 || !cols_d || !banks || !outs_d)
+    return fail(ctx, FMCS_E_ARG, "gather_columns: 1 to 32 columns with their banks and outputs");
+  for (u32 k = 0; k < ncols; ++k)
+    if (!bank_ok(banks[k])) return fail(ctx, FMCS_E_ARG, "gather_columns: banks must be 2, 4 or 8");
+  if (n == 0) return FMCS_OK;
+  if (n > (1ull << 32)) return fail(ctx, FMCS_E_ARG, "gather_columns: more than 2^32 rows");
+  Stager st(ctx);
+  std::vector<const void*> src(cols_d, cols_d + ncols);
+  std::vector<void*> dst(outs_d, outs_d + ncols);
+  for (u32 k = 0; k < ncols; ++k) {
+    TRY(st.arg(src[k], n * banks[k], Stager::IN, "cols_d"));
+    TRY(st.arg(dst[k], n * banks[k], Stager::OUT, "outs_d"));
+  }
+  TRY(st.arg(order_d, n * 4, Stager::IN, "order_d"));
+  if (((uintptr_t)order_d & 15) != 0) return fail(ctx, FMCS_E_ARG, "gather_columns: order must be 16-byte aligned");
+  // columns widest first, first fit into records of at most 16 bytes (each
+  // column then sits at an offset aligned to its width)
+  std::vector<u32> idx(ncols);
+  for (u32 k = 0; k < ncols; ++k) idx[k] = k;
+  std::stable_sort(idx.begin(), idx.end(), [&](u32 a, u32 b) { return banks[a] > banks[b]; });
+  std::vector<std::vector<u32>> groups;
+  std::vector<u32> fill;
+  for (u32 k : idx) {
+    size_t g = 0;
+    while (g < groups.size() && (fill[g] + banks[k] > 16 || groups[g].size() == 8)) ++g;
+    if (g == groups.size()) {
+      groups.emplace_back();
+      fill.push_back(0);
+    }
+    groups[g].push_back(k);
+    fill[g] += banks[k];
+  }
+  const u64 grid = (n + 1023) / 1024;
+  for (size_t g = 0; g < groups.size(); ++g) {
+    bool aligned = true;
+    for (u32 k : groups[g]) aligned = aligned && ((uintptr_t)src[k] & 15) == 0 && ((uintptr_t)dst[k] & 15) == 0;
+    if (groups[g].size() == 1 || !aligned) {
+      // one column (or unaligned columns): the column gather
+      for (u32 k : groups[g]) {
+        switch (banks[k]) {
+          case 2: TRY(gather_launch<u16>(ctx, (const u16*)src[k], order_d, (u16*)dst[k], n)); break;
+          case 4: TRY(gather_launch<u32>(ctx, (const u32*)src[k], order_d, (u32*)dst[k], n)); break;
+          default: TRY(gather_launch<u64>(ctx, (const u64*)src[k], order_d, (u64*)dst[k], n)); break;
+        }
+      }
+      continue;
+    }
+    RecCols c;
+    std::memset(&c, 0, sizeof(c));
+    u32 off = 0;
+    for (u32 k : groups[g]) {
+      c.src[c.ncol] = src[k];
+      c.dst[c.ncol] = dst[k];
+      c.width[c.ncol] = banks[k];
+      c.off[c.ncol] = off;
+      off += banks[k];
+      ++c.ncol;
+    }
+    const u32 R = off <= 8 ? 8 : 16;
+    void* rec;
+    TRY(getbuf(ctx, WS_REC, n * R, &rec));
+    if (R == 8) {
+      TRY(klaunch(ctx, "k_rec_pack", k_rec_pack<8>, grid, (double)n * (off + 8), nullptr, 0.0, c, rec, n));
+      TRY(klaunch(ctx, "k_rec_gather", k_rec_gather<8>, grid, (double)n * (4 + 8 + off), nullptr, 0.0, c,
+                  (const void*)rec, order_d, n));
+    } else {
+      TRY(klaunch(ctx, "k_rec_pack", k_rec_pack<16>, grid, (double)n * (off + 16), nullptr, 0.0, c, rec, n));
+      TRY(klaunch(ctx, "k_rec_gather", k_rec_gather<16>, grid, (double)n * (4 + 16 + off), nullptr, 0.0, c,
+                  (const void*)rec, order_d, n));
+    }
   }
   return st.finish();
 }

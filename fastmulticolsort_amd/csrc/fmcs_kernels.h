@@ -93,6 +93,21 @@ template <class T>
 __global__ void k_gather(const T*, const u32*, T*, u64);
 template <class T>
 __global__ void k_gather_win(const T*, const u32*, T*, u64, u32, u32);
+// Up to eight payload columns of 2, 4 or 8 bytes packed side by side into
+// R-byte row records (R = 8 or 16; each column at an offset aligned to its
+// width), so that a gather by an order touches one memory line a row
+// instead of one per column.
+struct RecCols {
+  const void* src[8];
+  void* dst[8];
+  u32 width[8];
+  u32 off[8];
+  u32 ncol;
+};
+template <int R>
+__global__ void k_rec_pack(RecCols, void*, u64);
+template <int R>
+__global__ void k_rec_gather(RecCols, const void*, const u32*, u64);
 template <class K, int ITEMS>
 __global__ void k_bcount(const K*, const u32*, u64, u32, u32*, u32*);
 template <class K, int ITEMS>

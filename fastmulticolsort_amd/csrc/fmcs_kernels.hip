@@ -115,6 +115,143 @@ __global__ __launch_bounds__(256) void k_stitch(StitchArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Row records of several payload columns for the gather by one order
+// (BATproject of each column, src/composer.cpp:608-628).  Four rows a
+// thread; column pointers are 16-byte aligned (checked by the host), so the
+// four rows of a column move as one 8-, 16- or 32-byte vector.  Column
+// loops run over the uniform descriptor; records are assembled with static
+// word indices only (no register array indexed at run time).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void col_load4(const void* p, u32 wd, u64 i0, u32 m, u64 (&v)[4]) {
+  if (m == 4) {
+    if (wd == 2) {
+      const uint2 t = *(const uint2*)((const u16*)p + i0);
+      v[0] = t.x & 0xffffu; v[1] = t.x >> 16; v[2] = t.y & 0xffffu; v[3] = t.y >> 16;
+    } else if (wd == 4) {
+      const uint4 t = *(const uint4*)((const u32*)p + i0);
+      v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+    } else {
+      const uint4* q = (const uint4*)((const u64*)p + i0);
+      const uint4 a = q[0], b = q[1];
+      v[0] = a.x | (u64)a.y << 32; v[1] = a.z | (u64)a.w << 32;
+      v[2] = b.x | (u64)b.y << 32; v[3] = b.z | (u64)b.w << 32;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      v[r] = 0;
+      if ((u32)r < m)
+        v[r] = wd == 2 ? (u64)((const u16*)p)[i0 + r] : wd == 4 ? (u64)((const u32*)p)[i0 + r]
+                                                             : ((const u64*)p)[i0 + r];
+    }
+  }
+}
+
+__device__ __forceinline__ void col_store4(void* p, u32 wd, u64 i0, u32 m, const u64 (&v)[4]) {
+  if (m == 4) {
+    if (wd == 2) {
+      *(uint2*)((u16*)p + i0) = make_uint2((u32)v[0] | (u32)v[1] << 16, (u32)v[2] | (u32)v[3] << 16);
+    } else if (wd == 4) {
+      *(uint4*)((u32*)p + i0) = make_uint4((u32)v[0], (u32)v[1], (u32)v[2], (u32)v[3]);
+    } else {
+      uint4* q = (uint4*)((u64*)p + i0);
+      q[0] = make_uint4((u32)v[0], (u32)(v[0] >> 32), (u32)v[1], (u32)(v[1] >> 32));
+      q[1] = make_uint4((u32)v[2], (u32)(v[2] >> 32), (u32)v[3], (u32)(v[3] >> 32));
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if ((u32)r >= m) continue;
+      if (wd == 2) ((u16*)p)[i0 + r] = (u16)v[r];
+      else if (wd == 4) ((u32*)p)[i0 + r] = (u32)v[r];
+      else ((u64*)p)[i0 + r] = v[r];
+    }
+  }
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_rec_pack(RecCols c, void* __restrict__ rec, u64 n) {
+  constexpr int NW = R / 4;
+  const u64 i0 = ((u64)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (i0 >= n) return;
+  const u32 m = (u32)(n - i0 < 4 ? n - i0 : 4);
+  u32 w[4][NW];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int j = 0; j < NW; ++j) w[r][j] = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if ((u32)k >= c.ncol) break;
+    const u32 wd = c.width[k], q = c.off[k] >> 2, sh = (c.off[k] & 2) * 8;
+    u64 v[4];
+    col_load4(c.src[k], wd, i0, m, v);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        if ((u32)j == q) w[r][j] |= (u32)v[r] << sh;
+        if (wd == 8 && (u32)j == q + 1) w[r][j] |= (u32)(v[r] >> 32);
+      }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    if ((u32)r >= m) continue;
+    if constexpr (R == 16)
+      ((uint4*)rec)[i0 + r] = make_uint4(w[r][0], w[r][1], w[r][2], w[r][3]);
+    else
+      ((uint2*)rec)[i0 + r] = make_uint2(w[r][0], w[r][1]);
+  }
+}
+
+// outs[k][i] = cols[k][order[i]]: one random record load a row (four in
+// flight a thread), then every column's four values as one vector store.
+template <int R>
+__global__ __launch_bounds__(256) void k_rec_gather(RecCols c, const void* __restrict__ rec,
+                                                    const u32* __restrict__ order, u64 n) {
+  constexpr int NW = R / 4;
+  const u64 i0 = ((u64)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (i0 >= n) return;
+  const u32 m = (u32)(n - i0 < 4 ? n - i0 : 4);
+  u32 o[4];
+  if (m == 4) {
+    const uint4 t = *(const uint4*)(order + i0);
+    o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+  } else {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) o[r] = order[i0 + ((u32)r < m ? r : 0)];
+  }
+  u32 w[4][NW];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    if constexpr (R == 16) {
+      const uint4 t = ((const uint4*)rec)[o[r]];
+      w[r][0] = t.x; w[r][1] = t.y; w[r][2] = t.z; w[r][3] = t.w;
+    } else {
+      const uint2 t = ((const uint2*)rec)[o[r]];
+      w[r][0] = t.x; w[r][1] = t.y;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if ((u32)k >= c.ncol) break;
+    const u32 wd = c.width[k], q = c.off[k] >> 2, sh = (c.off[k] & 2) * 8;
+    const u32 mask = wd == 2 ? 0xffffu : 0xffffffffu;
+    u64 v[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      v[r] = 0;
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        if ((u32)j == q) v[r] |= (w[r][j] >> sh) & mask;
+        if (wd == 8 && (u32)j == q + 1) v[r] |= (u64)w[r][j] << 32;
+      }
+    }
+    col_store4(c.dst[k], wd, i0, m, v);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // a6: oid-permutation gather, out[i] = vals[order[i]] (src/composer.cpp:608-628).
 // Four rows per thread: one 16-byte order load, four random loads, one
 // vector store.
@@ -1350,6 +1487,10 @@ FMCS_INST_ONESWEEP_F(u32, u64, true, true)
 template __global__ void k_hash_insert<uint8_t>(const uint8_t*, const u32*, u64, u32*, u32, u32*, u32*);
 template __global__ void k_gather<uint8_t>(const uint8_t*, const u32*, uint8_t*, u64);
 template __global__ void k_fill_random<uint8_t>(uint8_t*, u64, u64, u32);
+template __global__ void k_rec_pack<8>(RecCols, void*, u64);
+template __global__ void k_rec_pack<16>(RecCols, void*, u64);
+template __global__ void k_rec_gather<8>(RecCols, const void*, const u32*, u64);
+template __global__ void k_rec_gather<16>(RecCols, const void*, const u32*, u64);
 FMCS_INST_K(u64)
 template __global__ void k_scan_excl<8, false>(const u32*, u32*, u64, u32*, u32*, u32*);
 template __global__ void k_scan_excl<8, true>(const u32*, u32*, u64, u32*, u32*, u32*);

@@ -151,13 +151,23 @@ def config2(ctx, n=1 << 26):
         res[name] = {"ms": round(ms, 3), "Mrows_s": round(n / ms / 1e3, 1), "ok": ok, "kernels": kern}
     _emit({"config": 2, "what": "62-bit single-round sort of (key, oid): LSD radix vs bitonic tiles + merge path",
            "rows": n, **res})
-    # oid payload gather of the three source columns in sorted order
+    # oid payload gather of the three source columns in sorted order: one
+    # column gather each, and all three through packed 16-byte row records
     outs = [torch.empty_like(c) for c in src]
     ms, kern = _timed(ctx, lambda: [ctx.gather(c, oo, out=o) for c, o in zip(src, outs)])
     ok = all(np.array_equal(o.cpu().numpy(), c[exp]) for o, c in zip(outs, cols))
     gbytes = n * (3 * 4 + 2 * (2 + 4 + 4))
-    _emit({"config": 2, "what": "payload gather of the 3 source columns by the sorted oids", "rows": n,
-           "ms": round(ms, 3), "ok": ok, "achieved_GBs": round(gbytes / (ms / 1e3) / 1e9, 1), "kernels": kern})
+    _emit({"config": 2, "what": "payload gather of the 3 source columns by the sorted oids, column by column",
+           "rows": n, "ms": round(ms, 3), "ok": ok, "achieved_GBs": round(gbytes / (ms / 1e3) / 1e9, 1),
+           "kernels": kern})
+    outs = [torch.empty_like(c) for c in src]
+    ms, kern = _timed(ctx, lambda: ctx.gather_columns(src, oo, outs=outs))
+    ok = all(np.array_equal(o.cpu().numpy(), c[exp]) for o, c in zip(outs, cols))
+    # pack: 10 B read + 16 B written a row; gather: 4 B order + 16 B record + 10 B written
+    rbytes = n * (10 + 16 + 4 + 16 + 10)
+    _emit({"config": 2, "what": "payload gather of the 3 source columns by the sorted oids, packed row records",
+           "rows": n, "ms": round(ms, 3), "ok": ok, "achieved_GBs": round(rbytes / (ms / 1e3) / 1e9, 1),
+           "kernels": kern})
 
 
 def config4(ctx, n=1 << 25, max_rrs=24, plans=None, hash_types=(1, 0)):

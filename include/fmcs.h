@@ -169,6 +169,21 @@ int fmcs_stitch_dir(fmcs_ctx* ctx, uint64_t n, uint32_t nsrc, const void* const*
 int fmcs_gather(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* values_d, const uint32_t* order_d,
                 void* out_d);
 
+/* Composer::BATproject<T> (src/composer.cpp:608-628) for several payload
+ * columns by one order, as the reference's tuple reconstruction projects
+ * every output column by the same oids, with packed row elements as in its
+ * disabled ChainComposer::BATproject_pack (src/composer.cpp:591-606):
+ * outs_d[k][i] = cols_d[k][order_d[i]]
+ * for k < ncols (1..32; banks[k] 2, 4 or 8 bytes).  Columns are grouped,
+ * widest first, into row records of at most 16 bytes; a group of two or more
+ * 16-byte-aligned columns is packed in one sequential pass and gathered with
+ * one random record read a row, so a row's random reads touch one memory
+ * line per group instead of one per column (the workspace holds n records of
+ * 8 or 16 bytes).  Other columns take the column gather.  Every order_d[i]
+ * must be below n. */
+int fmcs_gather_columns(fmcs_ctx* ctx, uint64_t n, uint32_t ncols, const void* const* cols_d,
+                        const uint32_t* banks, const uint32_t* order_d, void* const* outs_d);
+
 /* Composer::extractGroupInfo<T> (src/composer.cpp:630-678): dense monotone
  * group ids of sorted values (and the previous groups, if in_group_d is not
  * NULL).  seg_offsets_d (optional, ngroups+1 entries, up to n+1) receives the

@@ -76,6 +76,45 @@ def test_gather(ctx, dt, n):
     assert np.array_equal(host(out), v[order])
 
 
+_WDT = {2: np.uint16, 4: np.uint32, 8: np.uint64}
+
+
+@pytest.mark.parametrize("widths", [[2, 4, 4], [8, 8, 8], [4, 4], [2, 2], [8, 2, 2, 4], [2] * 9, [8], [4, 8, 2]])
+@pytest.mark.parametrize("n", [1, 3, 4, 5, 1027, 100003])
+@pytest.mark.parametrize("repeats", [False, True])
+def test_gather_columns(ctx, widths, n, repeats):
+    # several columns by one order through packed row records (groups of
+    # 2..8 columns in 8- or 16-byte records, lone columns through the column
+    # gather), ragged tails of the 4-row vectors, orders with repeated rows
+    log("gather_columns", widths, n, repeats)
+    cols = [datagen.splitmix64(11 + k, n).astype(_WDT[w]) for k, w in enumerate(widths)]
+    if repeats:
+        order = (datagen.splitmix64(5, n) % np.uint64(n)).astype(np.uint32)
+    else:
+        order = datagen.permutation(6, n).astype(np.uint32)
+    outs = ctx.gather_columns([dev(c) for c in cols], dev(order))
+    for c, o in zip(cols, outs):
+        assert np.array_equal(host(o), c[order])
+
+
+def test_gather_columns_unaligned_and_errors(ctx):
+    # columns whose pointers are not 16-byte aligned take the column gather
+    n = 5003
+    base = [datagen.splitmix64(21 + k, n + 1).astype(dt) for k, dt in enumerate([np.uint16, np.uint32, np.uint16])]
+    dbase = [dev(b) for b in base]
+    cols = [d[1:] for d in dbase]
+    order = datagen.permutation(9, n).astype(np.uint32)
+    outs = ctx.gather_columns(cols, dev(order))
+    for b, o in zip(base, outs):
+        assert np.array_equal(host(o), b[1:][order])
+    import ctypes as C
+    lib = ctx.lib
+    nul = (C.c_void_p * 1)(None)
+    bad = (C.c_uint32 * 1)(3)
+    assert lib.fmcs_gather_columns(ctx.h, n, 0, nul, bad, C.c_void_p(dev(order).data_ptr()), nul) != 0
+    assert lib.fmcs_gather_columns(ctx.h, n, 1, nul, bad, C.c_void_p(dev(order).data_ptr()), nul) != 0
+
+
 @pytest.mark.parametrize("dt", [np.uint16, np.uint32, np.uint64])
 @pytest.mark.parametrize("kb", [1, 3, 64])
 def test_gather_windowed(ctx, dt, kb):
