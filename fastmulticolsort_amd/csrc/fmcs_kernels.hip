@@ -116,37 +116,13 @@ __global__ __launch_bounds__(256) void k_stitch(StitchArgs a) {
 
 // ---------------------------------------------------------------------------
 // Row records of several payload columns for the gather by one order
-// (BATproject of each column, src/composer.cpp:608-628).  Four rows a
-// thread; column pointers are 16-byte aligned (checked by the host), so the
-// four rows of a column move as one 8-, 16- or 32-byte vector.  Column
-// loops run over the uniform descriptor; records are assembled with static
-// word indices only (no register array indexed at run time).
+// (BATproject of each column, src/composer.cpp:608-628).  Column loops run
+// over the uniform descriptor; records are assembled with static word
+// indices only (no register array indexed at run time).  The gather writes
+// four consecutive rows a thread: column pointers are 16-byte aligned
+// (checked by the host), so a column's four values leave as one 8-, 16- or
+// 32-byte vector.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void col_load4(const void* p, u32 wd, u64 i0, u32 m, u64 (&v)[4]) {
-  if (m == 4) {
-    if (wd == 2) {
-      const uint2 t = *(const uint2*)((const u16*)p + i0);
-      v[0] = t.x & 0xffffu; v[1] = t.x >> 16; v[2] = t.y & 0xffffu; v[3] = t.y >> 16;
-    } else if (wd == 4) {
-      const uint4 t = *(const uint4*)((const u32*)p + i0);
-      v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
-    } else {
-      const uint4* q = (const uint4*)((const u64*)p + i0);
-      const uint4 a = q[0], b = q[1];
-      v[0] = a.x | (u64)a.y << 32; v[1] = a.z | (u64)a.w << 32;
-      v[2] = b.x | (u64)b.y << 32; v[3] = b.z | (u64)b.w << 32;
-    }
-  } else {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      v[r] = 0;
-      if ((u32)r < m)
-        v[r] = wd == 2 ? (u64)((const u16*)p)[i0 + r] : wd == 4 ? (u64)((const u32*)p)[i0 + r]
-                                                             : ((const u64*)p)[i0 + r];
-    }
-  }
-}
-
 __device__ __forceinline__ void col_store4(void* p, u32 wd, u64 i0, u32 m, const u64 (&v)[4]) {
   if (m == 4) {
     if (wd == 2) {
@@ -169,12 +145,12 @@ __device__ __forceinline__ void col_store4(void* p, u32 wd, u64 i0, u32 m, const
   }
 }
 
+// Rows b*1024 + wave*256 + r*64 + lane (r < 4): every column load and
+// record store of a wave covers 64 consecutive rows.
 template <int R>
 __global__ __launch_bounds__(256) void k_rec_pack(RecCols c, void* __restrict__ rec, u64 n) {
   constexpr int NW = R / 4;
-  const u64 i0 = ((u64)blockIdx.x * blockDim.x + threadIdx.x) * 4;
-  if (i0 >= n) return;
-  const u32 m = (u32)(n - i0 < 4 ? n - i0 : 4);
+  const u64 base = (u64)blockIdx.x * 1024 + (threadIdx.x >> 6) * 256 + (threadIdx.x & 63);
   u32 w[4][NW];
 #pragma unroll
   for (int r = 0; r < 4; ++r)
@@ -184,23 +160,27 @@ __global__ __launch_bounds__(256) void k_rec_pack(RecCols c, void* __restrict__ 
   for (int k = 0; k < 8; ++k) {
     if ((u32)k >= c.ncol) break;
     const u32 wd = c.width[k], q = c.off[k] >> 2, sh = (c.off[k] & 2) * 8;
-    u64 v[4];
-    col_load4(c.src[k], wd, i0, m, v);
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int r = 0; r < 4; ++r) {
+      const u64 row = base + (u64)r * 64;
+      if (row >= n) continue;
+      const u64 v = wd == 2 ? (u64)((const u16*)c.src[k])[row]
+                            : wd == 4 ? (u64)((const u32*)c.src[k])[row] : ((const u64*)c.src[k])[row];
 #pragma unroll
       for (int j = 0; j < NW; ++j) {
-        if ((u32)j == q) w[r][j] |= (u32)v[r] << sh;
-        if (wd == 8 && (u32)j == q + 1) w[r][j] |= (u32)(v[r] >> 32);
+        if ((u32)j == q) w[r][j] |= (u32)v << sh;
+        if (wd == 8 && (u32)j == q + 1) w[r][j] |= (u32)(v >> 32);
       }
+    }
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    if ((u32)r >= m) continue;
+    const u64 row = base + (u64)r * 64;
+    if (row >= n) continue;
     if constexpr (R == 16)
-      ((uint4*)rec)[i0 + r] = make_uint4(w[r][0], w[r][1], w[r][2], w[r][3]);
+      ((uint4*)rec)[row] = make_uint4(w[r][0], w[r][1], w[r][2], w[r][3]);
     else
-      ((uint2*)rec)[i0 + r] = make_uint2(w[r][0], w[r][1]);
+      ((uint2*)rec)[row] = make_uint2(w[r][0], w[r][1]);
   }
 }
 
