@@ -2412,7 +2412,7 @@ int fmcs_gather_columns(fmcs_ctx* ctx, uint64_t n, uint32_t ncols, const void* c
     groups[g].push_back(k);
     fill[g] += banks[k];
   }
-  const u64 grid = (n + 1023) / 1024;
+  const u64 grid = (n + 1023) / 1024, ggrid = (n + 256 * REC_ROWS - 1) / (256 * REC_ROWS);
   for (size_t g = 0; g < groups.size(); ++g) {
     bool aligned = true;
     for (u32 k : groups[g]) aligned = aligned && ((uintptr_t)src[k] & 15) == 0 && ((uintptr_t)dst[k] & 15) == 0;
@@ -2443,11 +2443,11 @@ int fmcs_gather_columns(fmcs_ctx* ctx, uint64_t n, uint32_t ncols, const void* c
     TRY(getbuf(ctx, WS_REC, n * R, &rec));
     if (R == 8) {
       TRY(klaunch(ctx, "k_rec_pack", k_rec_pack<8>, grid, (double)n * (off + 8), nullptr, 0.0, c, rec, n));
-      TRY(klaunch(ctx, "k_rec_gather", k_rec_gather<8>, grid, (double)n * (4 + 8 + off), nullptr, 0.0, c,
+      TRY(klaunch(ctx, "k_rec_gather", k_rec_gather<8>, ggrid, (double)n * (4 + 8 + off), nullptr, 0.0, c,
                   (const void*)rec, order_d, n));
     } else {
       TRY(klaunch(ctx, "k_rec_pack", k_rec_pack<16>, grid, (double)n * (off + 16), nullptr, 0.0, c, rec, n));
-      TRY(klaunch(ctx, "k_rec_gather", k_rec_gather<16>, grid, (double)n * (4 + 16 + off), nullptr, 0.0, c,
+      TRY(klaunch(ctx, "k_rec_gather", k_rec_gather<16>, ggrid, (double)n * (4 + 16 + off), nullptr, 0.0, c,
                   (const void*)rec, order_d, n));
     }
   }

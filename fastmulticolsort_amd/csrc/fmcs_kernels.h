@@ -97,6 +97,7 @@ __global__ void k_gather_win(const T*, const u32*, T*, u64, u32, u32);
 // R-byte row records (R = 8 or 16; each column at an offset aligned to its
 // width), so that a gather by an order touches one memory line a row
 // instead of one per column.
+constexpr int REC_ROWS = 4;  // rows a thread of the record gather (a multiple of 4)
 struct RecCols {
   const void* src[8];
   void* dst[8];

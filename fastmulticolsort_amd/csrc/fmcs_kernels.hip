@@ -184,26 +184,30 @@ __global__ __launch_bounds__(256) void k_rec_pack(RecCols c, void* __restrict__ 
   }
 }
 
-// outs[k][i] = cols[k][order[i]]: one random record load a row (four in
-// flight a thread), then every column's four values as one vector store.
+// outs[k][i] = cols[k][order[i]]: one random record load a row (REC_ROWS
+// consecutive rows a thread, all their loads in flight), then every
+// column's values as vector stores of four rows.
 template <int R>
 __global__ __launch_bounds__(256) void k_rec_gather(RecCols c, const void* __restrict__ rec,
                                                     const u32* __restrict__ order, u64 n) {
-  constexpr int NW = R / 4;
-  const u64 i0 = ((u64)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  constexpr int NW = R / 4, RT = REC_ROWS;
+  const u64 i0 = ((u64)blockIdx.x * blockDim.x + threadIdx.x) * RT;
   if (i0 >= n) return;
-  const u32 m = (u32)(n - i0 < 4 ? n - i0 : 4);
-  u32 o[4];
-  if (m == 4) {
-    const uint4 t = *(const uint4*)(order + i0);
-    o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+  const u32 m = (u32)(n - i0 < RT ? n - i0 : RT);
+  u32 o[RT];
+  if (m == RT) {
+#pragma unroll
+    for (int g = 0; g < RT / 4; ++g) {
+      const uint4 t = *(const uint4*)(order + i0 + 4 * g);
+      o[4 * g] = t.x; o[4 * g + 1] = t.y; o[4 * g + 2] = t.z; o[4 * g + 3] = t.w;
+    }
   } else {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) o[r] = order[i0 + ((u32)r < m ? r : 0)];
+    for (int r = 0; r < RT; ++r) o[r] = order[i0 + ((u32)r < m ? r : 0)];
   }
-  u32 w[4][NW];
+  u32 w[RT][NW];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
+  for (int r = 0; r < RT; ++r) {
     if constexpr (R == 16) {
       const uint4 t = ((const uint4*)rec)[o[r]];
       w[r][0] = t.x; w[r][1] = t.y; w[r][2] = t.z; w[r][3] = t.w;
@@ -217,17 +221,21 @@ __global__ __launch_bounds__(256) void k_rec_gather(RecCols c, const void* __res
     if ((u32)k >= c.ncol) break;
     const u32 wd = c.width[k], q = c.off[k] >> 2, sh = (c.off[k] & 2) * 8;
     const u32 mask = wd == 2 ? 0xffffu : 0xffffffffu;
-    u64 v[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      v[r] = 0;
+    for (int g = 0; g < RT / 4; ++g) {
+      if ((u32)(4 * g) >= m) break;
+      u64 v[4];
 #pragma unroll
-      for (int j = 0; j < NW; ++j) {
-        if ((u32)j == q) v[r] |= (w[r][j] >> sh) & mask;
-        if (wd == 8 && (u32)j == q + 1) v[r] |= (u64)w[r][j] << 32;
+      for (int r = 0; r < 4; ++r) {
+        v[r] = 0;
+#pragma unroll
+        for (int j = 0; j < NW; ++j) {
+          if ((u32)j == q) v[r] |= (w[4 * g + r][j] >> sh) & mask;
+          if (wd == 8 && (u32)j == q + 1) v[r] |= (u64)w[4 * g + r][j] << 32;
+        }
       }
+      col_store4(c.dst[k], wd, i0 + 4 * g, m - 4 * g < 4 ? m - 4 * g : 4, v);
     }
-    col_store4(c.dst[k], wd, i0, m, v);
   }
 }
 
