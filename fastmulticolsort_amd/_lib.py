@@ -100,7 +100,7 @@ def load():
         "fmcs_stitch_dir": (i32, [vp, u64, u32, P(vp), P(u32), P(u32), u32, P(Entry), u32, P(vp), P(u32),
                                   P(C.c_uint8)]),
         "fmcs_gather": (i32, [vp, u32, u64, vp, vp, vp]),
-        "fmcs_gather_columns": (i32, [vp, u64, u32, P(vp), P(u32), vp, P(vp)]),
+        "fmcs_gather_columns": (i32, [vp, u64, u32, P(vp), P(u32), P(u32), vp, P(vp)]),
         "fmcs_extract_group_info": (i32, [vp, u32, u64, vp, vp, vp, vp, P(u32)]),
         "fmcs_sort_pairs": (i32, [vp, u32, u32, u64, vp, vp]),
         "fmcs_sort_pairs_noalloc": (i32, [vp, u32, u32, u64, vp, vp]),

@@ -26,7 +26,7 @@ enum Slot {
   WS_ST, WS_SZ, WS_BST, WS_BSZ, WS_FLAG, WS_POS, WS_LLO, WS_LHI, WS_LTILES, WS_TBASE, WS_TLO, WS_THI, WS_TS0,
   WS_TS1, WS_COMP, WS_COMP2, WS_GROUPTMP, WS_MISC, WS_HASH, WS_HASH2, WS_HASH3, WS_HT_TABLE, WS_HT_FIRST,
   WS_HT_RANK, WS_HT_SLOT, WS_HT_HEAD, WS_HT_PREFIX, WS_FALLBACK, WS_WLIST, WS_DD, WS_SCANST, WS_FBLIST, WS_TCOUNT, WS_CARRY, WS_BCNT, WS_T0P, WS_T1P, WS_PAY, WS_SGL, WS_FBLIST2, WS_RKL, WS_RKL2, WS_DNEXT, WS_TSEG, WS_HG_KEY, WS_HG_POS, WS_HG_SKEY, WS_HG_SPOS, WS_HG_F, WS_HG_AUX, WS_HG_PSTART,
-  WS_HG_TKEY, WS_HG_TFIRST, WS_HG_CHUNK, WS_PLIST, WS_BINSC, WS_FBWIN, WS_FBPOOL, WS_UNB0, WS_UNB1, WS_UNBK, WS_DIG0, WS_BBITS, WS_REC, WS_USER0,
+  WS_HG_TKEY, WS_HG_TFIRST, WS_HG_CHUNK, WS_PLIST, WS_BINSC, WS_FBWIN, WS_FBPOOL, WS_UNB0, WS_UNB1, WS_UNBK, WS_DIG0, WS_BBITS, WS_REC, WS_RECERR, WS_USER0,
   WS_STAGE0 = WS_USER0 + 24,  // device copies of staged host buffers (one slot per argument)
   WS_COUNT = WS_STAGE0 + 72
 };
@@ -2378,12 +2378,15 @@ int fmcs_gather(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* values_d, 
 }
 
 int fmcs_gather_columns(fmcs_ctx* ctx, uint64_t n, uint32_t ncols, const void* const* cols_d, const uint32_t* banks,
-                        const uint32_t* order_d, void* const* outs_d) {
+                        const uint32_t* bits, const uint32_t* order_d, void* const* outs_d) {
   if (!ctx) return FMCS_E_ARG;
   if (ncols == 0 || ncols > 32 || !cols_d || !banks || !outs_d)
     return fail(ctx, FMCS_E_ARG, "gather_columns: 1 to 32 columns with their banks and outputs");
-  for (u32 k = 0; k < ncols; ++k)
+  for (u32 k = 0; k < ncols; ++k) {
     if (!bank_ok(banks[k])) return fail(ctx, FMCS_E_ARG, "gather_columns: banks must be 2, 4 or 8");
+    if (bits && (bits[k] == 0 || bits[k] > 8 * banks[k]))
+      return fail(ctx, FMCS_E_ARG, "gather_columns: bits must be 1 .. 8 x bank");
+  }
   if (n == 0) return FMCS_OK;
   if (n > (1ull << 32)) return fail(ctx, FMCS_E_ARG, "gather_columns: more than 2^32 rows");
   Stager st(ctx);
@@ -2395,49 +2398,111 @@ int fmcs_gather_columns(fmcs_ctx* ctx, uint64_t n, uint32_t ncols, const void* c
   }
   TRY(st.arg(order_d, n * 4, Stager::IN, "order_d"));
   if (((uintptr_t)order_d & 15) != 0) return fail(ctx, FMCS_E_ARG, "gather_columns: order must be 16-byte aligned");
-  // columns widest first, first fit into records of at most 16 bytes (each
-  // column then sits at an offset aligned to its width)
-  std::vector<u32> idx(ncols);
-  for (u32 k = 0; k < ncols; ++k) idx[k] = k;
-  std::stable_sort(idx.begin(), idx.end(), [&](u32 a, u32 b) { return banks[a] > banks[b]; });
-  std::vector<std::vector<u32>> groups;
-  std::vector<u32> fill;
-  for (u32 k : idx) {
-    size_t g = 0;
-    while (g < groups.size() && (fill[g] + banks[k] > 16 || groups[g].size() == 8)) ++g;
-    if (g == groups.size()) {
-      groups.emplace_back();
-      fill.push_back(0);
+  // Columns widest first, first fit into records: of at most 16 bytes
+  // (each column at an offset aligned to its width), or, with declared
+  // widths, of at most 64 bits.  Bit records are taken when they need no
+  // more records a row than byte records (they are half the size or less).
+  auto pack = [&](const uint32_t* w, u32 cap) {
+    std::vector<u32> idx(ncols);
+    for (u32 k = 0; k < ncols; ++k) idx[k] = k;
+    std::stable_sort(idx.begin(), idx.end(), [&](u32 a, u32 b) { return w[a] > w[b]; });
+    std::vector<std::vector<u32>> gs;
+    std::vector<u32> fill;
+    for (u32 k : idx) {
+      size_t g = 0;
+      while (g < gs.size() && (fill[g] + w[k] > cap || gs[g].size() == 8)) ++g;
+      if (g == gs.size()) {
+        gs.emplace_back();
+        fill.push_back(0);
+      }
+      gs[g].push_back(k);
+      fill[g] += w[k];
     }
-    groups[g].push_back(k);
-    fill[g] += banks[k];
+    return gs;
+  };
+  std::vector<std::vector<u32>> groups = pack(banks, 16);
+  bool use_bits = false;
+  if (bits) {
+    std::vector<std::vector<u32>> bg = pack(bits, 64);
+    bool multi = false;
+    for (auto& g : bg) multi = multi || g.size() > 1;
+    if (multi && bg.size() <= groups.size()) {
+      groups.swap(bg);
+      use_bits = true;
+    }
   }
   const u64 grid = (n + 1023) / 1024, ggrid = (n + 256 * REC_ROWS - 1) / (256 * REC_ROWS);
-  for (size_t g = 0; g < groups.size(); ++g) {
-    bool aligned = true;
-    for (u32 k : groups[g]) aligned = aligned && ((uintptr_t)src[k] & 15) == 0 && ((uintptr_t)dst[k] & 15) == 0;
-    if (groups[g].size() == 1 || !aligned) {
-      // one column (or unaligned columns): the column gather
-      for (u32 k : groups[g]) {
-        switch (banks[k]) {
-          case 2: TRY(gather_launch<u16>(ctx, (const u16*)src[k], order_d, (u16*)dst[k], n)); break;
-          case 4: TRY(gather_launch<u32>(ctx, (const u32*)src[k], order_d, (u32*)dst[k], n)); break;
-          default: TRY(gather_launch<u64>(ctx, (const u64*)src[k], order_d, (u64*)dst[k], n)); break;
-        }
-      }
-      continue;
+  auto column = [&](u32 k) -> int {
+    switch (banks[k]) {
+      case 2: return gather_launch<u16>(ctx, (const u16*)src[k], order_d, (u16*)dst[k], n);
+      case 4: return gather_launch<u32>(ctx, (const u32*)src[k], order_d, (u32*)dst[k], n);
+      default: return gather_launch<u64>(ctx, (const u64*)src[k], order_d, (u64*)dst[k], n);
     }
-    RecCols c;
+  };
+  // a record group: two or more columns, all 16-byte aligned (else the
+  // column gather takes them one by one)
+  auto recordable = [&](const std::vector<u32>& gr) {
+    bool a = gr.size() > 1;
+    for (u32 k : gr) a = a && ((uintptr_t)src[k] & 15) == 0 && ((uintptr_t)dst[k] & 15) == 0;
+    return a;
+  };
+  auto describe = [&](const std::vector<u32>& gr, RecCols& c) -> u32 {
     std::memset(&c, 0, sizeof(c));
     u32 off = 0;
-    for (u32 k : groups[g]) {
+    for (u32 k : gr) {
       c.src[c.ncol] = src[k];
       c.dst[c.ncol] = dst[k];
       c.width[c.ncol] = banks[k];
+      c.bits[c.ncol] = use_bits ? bits[k] : 8 * banks[k];
       c.off[c.ncol] = off;
-      off += banks[k];
+      off += use_bits ? bits[k] : banks[k];
       ++c.ncol;
     }
+    return off;
+  };
+  if (use_bits) {
+    // every bit group is packed, and the declared widths checked, before
+    // any output is written
+    std::vector<size_t> bg;
+    for (size_t g = 0; g < groups.size(); ++g)
+      if (recordable(groups[g])) bg.push_back(g);
+    if (!bg.empty()) {
+      u64* rec;
+      u32* err;
+      TRY(getbuf_t(ctx, WS_REC, n * bg.size(), &rec));
+      TRY(getbuf_t(ctx, WS_RECERR, 1, &err));
+      TRY(fill(ctx, err, 1, 0u));
+      std::vector<RecCols> cs(bg.size());
+      for (size_t i = 0; i < bg.size(); ++i) {
+        describe(groups[bg[i]], cs[i]);
+        double w = 0;
+        for (u32 k : groups[bg[i]]) w += banks[k];
+        TRY(klaunch(ctx, "k_bitrec_pack", k_bitrec_pack, grid, (double)n * (w + 8), nullptr, 0.0, cs[i], rec + i * n,
+                    n, err));
+      }
+      u32 h = 0;
+      HIPCHK(hipMemcpyAsync(&h, err, sizeof(u32), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      if (h) return fail(ctx, FMCS_E_ARG, "gather_columns: a value is wider than its declared bits");
+      for (size_t i = 0; i < bg.size(); ++i) {
+        double w = 0;
+        for (u32 k : groups[bg[i]]) w += banks[k];
+        TRY(klaunch(ctx, "k_bitrec_gather", k_bitrec_gather, grid, (double)n * (4 + 8 + w), nullptr, 0.0, cs[i],
+                    (const u64*)(rec + i * n), order_d, n));
+      }
+    }
+    for (size_t g = 0; g < groups.size(); ++g)
+      if (!recordable(groups[g]))
+        for (u32 k : groups[g]) TRY(column(k));
+    return st.finish();
+  }
+  for (size_t g = 0; g < groups.size(); ++g) {
+    if (!recordable(groups[g])) {
+      for (u32 k : groups[g]) TRY(column(k));
+      continue;
+    }
+    RecCols c;
+    const u32 off = describe(groups[g], c);
     const u32 R = off <= 8 ? 8 : 16;
     void* rec;
     TRY(getbuf(ctx, WS_REC, n * R, &rec));

@@ -102,13 +102,18 @@ struct RecCols {
   const void* src[8];
   void* dst[8];
   u32 width[8];
-  u32 off[8];
+  u32 off[8];   // byte offset (byte records) or bit offset (bit records)
+  u32 bits[8];  // bit records: the column's declared width
   u32 ncol;
 };
 template <int R>
 __global__ void k_rec_pack(RecCols, void*, u64);
 template <int R>
 __global__ void k_rec_gather(RecCols, const void*, const u32*, u64);
+// Bit records: the columns' values at bit offsets of one u64 (their
+// declared widths sum to at most 64); a value wider than declared sets *err.
+__global__ void k_bitrec_pack(RecCols, u64*, u64, u32*);
+__global__ void k_bitrec_gather(RecCols, const u64*, const u32*, u64);
 template <class K, int ITEMS>
 __global__ void k_bcount(const K*, const u32*, u64, u32, u32*, u32*);
 template <class K, int ITEMS>

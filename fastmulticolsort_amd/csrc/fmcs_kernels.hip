@@ -239,6 +239,62 @@ __global__ __launch_bounds__(256) void k_rec_gather(RecCols c, const void* __res
   }
 }
 
+// Bit records (declared widths summing to at most 64 bits): the same row
+// mappings as the byte records, fields at bit offsets of one u64.
+__global__ __launch_bounds__(256) void k_bitrec_pack(RecCols c, u64* __restrict__ rec, u64 n, u32* __restrict__ err) {
+  const u64 base = (u64)blockIdx.x * 1024 + (threadIdx.x >> 6) * 256 + (threadIdx.x & 63);
+  u64 w[4] = {0, 0, 0, 0};
+  bool bad = false;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if ((u32)k >= c.ncol) break;
+    const u32 wd = c.width[k], b = c.bits[k], off = c.off[k];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const u64 row = base + (u64)r * 64;
+      if (row >= n) continue;
+      const u64 v = wd == 2 ? (u64)((const u16*)c.src[k])[row]
+                            : wd == 4 ? (u64)((const u32*)c.src[k])[row] : ((const u64*)c.src[k])[row];
+      bad = bad || (b < 64 && (v >> b) != 0);
+      w[r] |= v << off;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const u64 row = base + (u64)r * 64;
+    if (row < n) rec[row] = w[r];
+  }
+  if (bad) atomicOr(err, 1u);
+}
+
+__global__ __launch_bounds__(256) void k_bitrec_gather(RecCols c, const u64* __restrict__ rec,
+                                                       const u32* __restrict__ order, u64 n) {
+  const u64 i0 = ((u64)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (i0 >= n) return;
+  const u32 m = (u32)(n - i0 < 4 ? n - i0 : 4);
+  u32 o[4];
+  if (m == 4) {
+    const uint4 t = *(const uint4*)(order + i0);
+    o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+  } else {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) o[r] = order[i0 + ((u32)r < m ? r : 0)];
+  }
+  u64 w[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) w[r] = rec[o[r]];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if ((u32)k >= c.ncol) break;
+    const u32 b = c.bits[k], off = c.off[k];
+    const u64 mask = b >= 64 ? ~0ull : ((1ull << b) - 1);
+    u64 v[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[r] = (w[r] >> off) & mask;
+    col_store4(c.dst[k], c.width[k], i0, m, v);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // a6: oid-permutation gather, out[i] = vals[order[i]] (src/composer.cpp:608-628).
 // Four rows per thread: one 16-byte order load, four random loads, one

@@ -137,8 +137,9 @@ class Context:
         self._chk(self.lib.fmcs_gather(self.h, bank_of(values), values.numel(), _p(values), _p(order), _p(out)))
         return out
 
-    def gather_columns(self, columns, order, outs=None):
-        """outs[k][i] = columns[k][order[i]] for every column, through row records
+    def gather_columns(self, columns, order, outs=None, bits=None):
+        """outs[k][i] = columns[k][order[i]] for every column, through row records;
+        bits: the columns' declared widths (bit-packed records), or None
         (see fmcs_gather_columns)."""
         if outs is None:
             outs = [torch.empty_like(c) for c in columns]
@@ -146,7 +147,8 @@ class Context:
         srcs = (C.c_void_p * k)(*[c.data_ptr() for c in columns])
         dsts = (C.c_void_p * k)(*[o.data_ptr() for o in outs])
         banks = (C.c_uint32 * k)(*[bank_of(c) for c in columns])
-        self._chk(self.lib.fmcs_gather_columns(self.h, order.numel(), k, srcs, banks, _p(order), dsts))
+        bw = None if bits is None else (C.c_uint32 * k)(*bits)
+        self._chk(self.lib.fmcs_gather_columns(self.h, order.numel(), k, srcs, banks, bw, _p(order), dsts))
         return outs
 
     def extract_group_info(self, sorted_values, in_group=None, out_group=None, seg_offsets=None):

@@ -168,6 +168,15 @@ def config2(ctx, n=1 << 26):
     _emit({"config": 2, "what": "payload gather of the 3 source columns by the sorted oids, packed row records",
            "rows": n, "ms": round(ms, 3), "ok": ok, "achieved_GBs": round(rbytes / (ms / 1e3) / 1e9, 1),
            "kernels": kern})
+    # the columns' declared widths (12 + 20 + 30 bits) in one 8-byte bit record
+    outs = [torch.empty_like(c) for c in src]
+    ms, kern = _timed(ctx, lambda: ctx.gather_columns(src, oo, outs=outs, bits=list(bws)))
+    ok = all(np.array_equal(o.cpu().numpy(), c[exp]) for o, c in zip(outs, cols))
+    # pack: 10 B read + 8 B written a row; gather: 4 B order + 8 B record + 10 B written
+    rbytes = n * (10 + 8 + 4 + 8 + 10)
+    _emit({"config": 2, "what": "payload gather of the 3 source columns by the sorted oids, 8-byte bit records",
+           "rows": n, "ms": round(ms, 3), "ok": ok, "achieved_GBs": round(rbytes / (ms / 1e3) / 1e9, 1),
+           "kernels": kern})
 
 
 def config4(ctx, n=1 << 25, max_rrs=24, plans=None, hash_types=(1, 0)):

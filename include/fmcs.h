@@ -179,10 +179,15 @@ int fmcs_gather(fmcs_ctx* ctx, uint32_t bank, uint64_t n, const void* values_d, 
  * 16-byte-aligned columns is packed in one sequential pass and gathered with
  * one random record read a row, so a row's random reads touch one memory
  * line per group instead of one per column (the workspace holds n records of
- * 8 or 16 bytes).  Other columns take the column gather.  Every order_d[i]
- * must be below n. */
+ * 8 or 16 bytes).  Other columns take the column gather.  bits (NULL, or
+ * each column's declared width, 1 .. 8 x bank) lets columns whose widths sum
+ * to at most 64 share an 8-byte record at bit offsets, taken when that needs
+ * no more records a row; a value wider than its declared bits then returns
+ * FMCS_E_ARG before any output is written (one host synchronisation).
+ * Every order_d[i] must be below n. */
 int fmcs_gather_columns(fmcs_ctx* ctx, uint64_t n, uint32_t ncols, const void* const* cols_d,
-                        const uint32_t* banks, const uint32_t* order_d, void* const* outs_d);
+                        const uint32_t* banks, const uint32_t* bits, const uint32_t* order_d,
+                        void* const* outs_d);
 
 /* Composer::extractGroupInfo<T> (src/composer.cpp:630-678): dense monotone
  * group ids of sorted values (and the previous groups, if in_group_d is not

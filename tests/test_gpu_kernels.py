@@ -111,8 +111,40 @@ def test_gather_columns_unaligned_and_errors(ctx):
     lib = ctx.lib
     nul = (C.c_void_p * 1)(None)
     bad = (C.c_uint32 * 1)(3)
-    assert lib.fmcs_gather_columns(ctx.h, n, 0, nul, bad, C.c_void_p(dev(order).data_ptr()), nul) != 0
-    assert lib.fmcs_gather_columns(ctx.h, n, 1, nul, bad, C.c_void_p(dev(order).data_ptr()), nul) != 0
+    assert lib.fmcs_gather_columns(ctx.h, n, 0, nul, bad, None, C.c_void_p(dev(order).data_ptr()), nul) != 0
+    assert lib.fmcs_gather_columns(ctx.h, n, 1, nul, bad, None, C.c_void_p(dev(order).data_ptr()), nul) != 0
+
+
+@pytest.mark.parametrize("widths,bits", [([2, 4, 4], [12, 20, 30]), ([8, 4], [40, 24]), ([2, 2, 4, 8], [16, 9, 7, 32]),
+                                         ([2, 4, 4], [16, 32, 32]), ([8, 8], [33, 31]), ([4] * 9, [7] * 9),
+                                         ([8, 2], [64, 1])])
+@pytest.mark.parametrize("n", [1, 5, 1027, 100003])
+def test_gather_columns_bits(ctx, widths, bits, n):
+    # declared widths: columns summing to at most 64 bits share an 8-byte
+    # record at bit offsets (taken when it needs no more records a row),
+    # values of exactly their declared width, ragged tails, repeated rows
+    log("gather_columns bits", widths, bits, n)
+    cols = [(datagen.splitmix64(31 + k, n) & np.uint64((1 << b) - 1)).astype(_WDT[w])
+            for k, (w, b) in enumerate(zip(widths, bits))]
+    order = (datagen.splitmix64(7, n) % np.uint64(n)).astype(np.uint32)
+    outs = ctx.gather_columns([dev(c) for c in cols], dev(order), bits=bits)
+    for c, o in zip(cols, outs):
+        assert np.array_equal(host(o), c[order])
+
+
+def test_gather_columns_bits_too_wide(ctx):
+    # a value above its declared width: FMCS_E_ARG and no output written
+    n = 4099
+    cols = [(datagen.splitmix64(41 + k, n) & np.uint64((1 << b) - 1)).astype(dt)
+            for k, (dt, b) in enumerate([(np.uint16, 12), (np.uint32, 20), (np.uint32, 30)])]
+    cols[1][777] = 1 << 20
+    outs = [torch.zeros(n, dtype=torch.from_numpy(c).dtype, device="cuda") for c in cols]
+    with pytest.raises(FmcsError):
+        ctx.gather_columns([dev(c) for c in cols], dev(datagen.permutation(2, n).astype(np.uint32)), outs=outs,
+                           bits=[12, 20, 30])
+    assert not any(host(o).any() for o in outs)
+    with pytest.raises(FmcsError):
+        ctx.gather_columns([dev(c) for c in cols], dev(np.arange(n, dtype=np.uint32)), bits=[12, 33, 30])
 
 
 @pytest.mark.parametrize("dt", [np.uint16, np.uint32, np.uint64])
