@@ -2477,7 +2477,7 @@ int fmcs_gather_columns(fmcs_ctx* ctx, uint64_t n, uint32_t ncols, const void* c
         describe(groups[bg[i]], cs[i]);
         double w = 0;
         for (u32 k : groups[bg[i]]) w += banks[k];
-        TRY(klaunch(ctx, "k_bitrec_pack", k_bitrec_pack, grid, (double)n * (w + 8), nullptr, 0.0, cs[i], rec + i * n,
+        TRY(klaunch(ctx, "k_bitrec_pack", k_bitrec_pack, (n + 2047) / 2048, (double)n * (w + 8), nullptr, 0.0, cs[i], rec + i * n,
                     n, err));
       }
       u32 h = 0;

@@ -239,18 +239,20 @@ __global__ __launch_bounds__(256) void k_rec_gather(RecCols c, const void* __res
   }
 }
 
-// Bit records (declared widths summing to at most 64 bits): the same row
-// mappings as the byte records, fields at bit offsets of one u64.
+// Bit records (declared widths summing to at most 64 bits): fields at bit
+// offsets of one u64.  The pack takes rows b*2048 + wave*512 + r*64 + lane
+// (r < 8: eight loads of a column in flight a thread); the gather maps rows
+// as the byte-record gather does.
 __global__ __launch_bounds__(256) void k_bitrec_pack(RecCols c, u64* __restrict__ rec, u64 n, u32* __restrict__ err) {
-  const u64 base = (u64)blockIdx.x * 1024 + (threadIdx.x >> 6) * 256 + (threadIdx.x & 63);
-  u64 w[4] = {0, 0, 0, 0};
+  const u64 base = (u64)blockIdx.x * 2048 + (threadIdx.x >> 6) * 512 + (threadIdx.x & 63);
+  u64 w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   bool bad = false;
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     if ((u32)k >= c.ncol) break;
     const u32 wd = c.width[k], b = c.bits[k], off = c.off[k];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < 8; ++r) {
       const u64 row = base + (u64)r * 64;
       if (row >= n) continue;
       const u64 v = wd == 2 ? (u64)((const u16*)c.src[k])[row]
@@ -260,7 +262,7 @@ __global__ __launch_bounds__(256) void k_bitrec_pack(RecCols c, u64* __restrict_
     }
   }
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
+  for (int r = 0; r < 8; ++r) {
     const u64 row = base + (u64)r * 64;
     if (row < n) rec[row] = w[r];
   }
